@@ -1,0 +1,294 @@
+"""Deterministic synthetic workloads written as Luz scene files (``.luz`` + ``.obj`` + ``.hdr``).
+
+The files use the reference's own scene-file format (docs/scene-files.md), so the same
+bytes feed the reference renderer and this package's loader.  Every generator is a pure
+function of its arguments: numbers are written with ``repr``-exact formatting and all
+randomness comes from ``random.Random(seed)`` (Mersenne Twister), so two machines produce
+byte-identical files.
+
+Workloads
+---------
+* ``cornell``      Cornell box: 5 Lambertian walls (10 triangles), 1 rectangle area light of
+                   radiance (17, 12, 4), 2 Lambertian boxes of 12 triangles; 300x300, 50 spp,
+                   depth 8, fp32 path state.
+* ``spheres``      512 seeded displaced spheres of 3840 triangles each (1,966,080 triangles),
+                   radius U[0.5, 3], centres uniform in a 200^3 box; 40 % Lambertian, 30 %
+                   GGX metal (alpha U[0.05, 0.5]), 30 % smooth dielectric IOR 1.5; procedural
+                   HDR sky (gradient + sun disk of 5e4 cd/m^2); 1920x1080, 64 spp, depth 12.
+"""
+
+from __future__ import annotations
+
+import math
+import os
+import random
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+LUMINANCE_WEIGHTS = (0.2722287168, 0.6740817658, 0.0536895174)  # ColorManagement.cpp:313-320
+LUMENS_PER_RADIANT_WATT = 683.0
+
+
+def _f(x: float) -> str:
+    """Shortest round-trip decimal for a double (strtod on the reader side is exact)."""
+    r = repr(float(x))
+    return r[:-2] if r.endswith(".0") else r
+
+
+def _v(v) -> str:
+    return "(" + ",".join(_f(c) for c in v) + ")"
+
+
+@dataclass
+class Workload:
+    name: str
+    directory: str
+    scene_path: str
+    width: int
+    height: int
+    spp: int
+    depth: int
+    seed: int
+    triangles: int
+    extra: dict = field(default_factory=dict)
+
+
+# ---------------------------------------------------------------------------------------
+# Cornell box
+# ---------------------------------------------------------------------------------------
+
+_CORNELL_WALLS = [
+    # (name, material, quad corners) -- each quad becomes two triangles (v0,v1,v2),(v0,v2,v3)
+    ("floor", "white", [(0, 0, 0), (555, 0, 0), (555, 0, 555), (0, 0, 555)]),
+    ("ceiling", "white", [(0, 555, 0), (0, 555, 555), (555, 555, 555), (555, 555, 0)]),
+    ("back", "white", [(0, 0, 555), (555, 0, 555), (555, 555, 555), (0, 555, 555)]),
+    ("left", "red", [(555, 0, 0), (555, 555, 0), (555, 555, 555), (555, 0, 555)]),
+    ("right", "green", [(0, 0, 0), (0, 0, 555), (0, 555, 555), (0, 555, 0)]),
+]
+
+_SHORT_BOX_TOP = [(130, 165, 65), (82, 165, 225), (240, 165, 272), (290, 165, 114)]
+_TALL_BOX_TOP = [(423, 330, 247), (265, 330, 296), (314, 330, 456), (472, 330, 406)]
+
+
+def _box_quads(top):
+    bottom = [(x, 0.0, z) for (x, _, z) in top]
+    quads = [list(top), list(reversed(bottom))]
+    for i in range(4):
+        j = (i + 1) % 4
+        quads.append([bottom[i], bottom[j], top[j], top[i]])
+    return quads
+
+
+def cornell_scene_text(width=300, height=300, spp=50, depth=8) -> str:
+    light_color = (17.0, 12.0, 4.0)
+    lum = sum(c * w for c, w in zip(light_color, LUMINANCE_WEIGHTS))
+    lines = [
+        "[settings]",
+        f"resolution={width},{height}",
+        f"samples={spp}",
+        f"maxlightbounces={depth}",
+        "adaptive=0",
+        "denoise=0",
+        "view_transform=raw",
+        "bloom=0",
+        "outputfilename=render.tiff",
+        "sky=none",
+        "background=(0,0,0)",
+        "",
+        "[materials]",
+        "material white {", "type=lambertian", "color=(0.73,0.73,0.73)", "}",
+        "material red {", "type=lambertian", "color=(0.65,0.05,0.05)", "}",
+        "material green {", "type=lambertian", "color=(0.12,0.45,0.15)", "}",
+        "material light {", "type=emissive", f"color={_v(light_color)}", f"radiance={_f(lum)}", "}",
+        "",
+        "[scene]",
+        "camera main {",
+        "position=(278,273,-800)",
+        "direction=(0,0,1)",
+        "up=(0,1,0)",
+        "focal_length_mm=35",
+        "sensor_width_mm=25",
+        "sensor_height_mm=25",
+        "pinhole=1",
+        "focus_distance=10",
+        "}",
+        "objects{",
+    ]
+    for _, mat, q in _CORNELL_WALLS:
+        lines.append(f"triangle={_v(q[0])},{_v(q[1])},{_v(q[2])},material={mat}")
+        lines.append(f"triangle={_v(q[0])},{_v(q[2])},{_v(q[3])},material={mat}")
+    lines.append("rectangle=(278,554,279.5),(0,-1,0),130,105,material=light")
+    for top in (_SHORT_BOX_TOP, _TALL_BOX_TOP):
+        for q in _box_quads(top):
+            lines.append(f"triangle={_v(q[0])},{_v(q[1])},{_v(q[2])},material=white")
+            lines.append(f"triangle={_v(q[0])},{_v(q[2])},{_v(q[3])},material=white")
+    lines.append("}")
+    return "\n".join(lines) + "\n"
+
+
+def write_cornell(directory: str, width=300, height=300, spp=50, depth=8, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "cornell.luz")
+    _write_if_changed(path, cornell_scene_text(width, height, spp, depth).encode())
+    return Workload("cornell", directory, path, width, height, spp, depth, seed, 34)
+
+
+# ---------------------------------------------------------------------------------------
+# 2M-triangle displaced spheres under a procedural HDR sky
+# ---------------------------------------------------------------------------------------
+
+SPHERE_SLICES = 48
+SPHERE_STACKS = 41  # 2 * 48 * (41 - 1) = 3840 triangles per sphere
+
+
+def _sphere_obj(rng: random.Random, radius: float) -> str:
+    # Smooth radial displacement: a sum of 4 random planar waves over the unit direction.
+    waves = []
+    for _ in range(4):
+        a = rng.uniform(0.0, 2.0 * math.pi)
+        z = rng.uniform(-1.0, 1.0)
+        r = math.sqrt(1.0 - z * z)
+        waves.append((r * math.cos(a), z, r * math.sin(a), rng.uniform(2.0, 6.0), rng.uniform(0.0, 2 * math.pi)))
+    amp = 0.12
+    verts = []
+
+    def vertex(x, y, z):
+        d = sum(math.sin(f * (x * wx + y * wy + z * wz) + ph) for (wx, wy, wz, f, ph) in waves) * 0.25
+        s = radius * (1.0 + amp * d)
+        verts.append((x * s, y * s, z * s))
+
+    vertex(0.0, 1.0, 0.0)
+    for k in range(1, SPHERE_STACKS):
+        theta = math.pi * k / SPHERE_STACKS
+        st, ct = math.sin(theta), math.cos(theta)
+        for j in range(SPHERE_SLICES):
+            phi = 2.0 * math.pi * j / SPHERE_SLICES
+            vertex(st * math.cos(phi), ct, st * math.sin(phi))
+    vertex(0.0, -1.0, 0.0)
+    south = len(verts)  # 1-based index of the last vertex
+    out = ["v %.9g %.9g %.9g" % v for v in verts]
+
+    def ring(k, j):
+        return 2 + (k - 1) * SPHERE_SLICES + (j % SPHERE_SLICES)
+
+    for j in range(SPHERE_SLICES):
+        out.append(f"f 1 {ring(1, j + 1)} {ring(1, j)}")
+    for k in range(1, SPHERE_STACKS - 1):
+        for j in range(SPHERE_SLICES):
+            a, b = ring(k, j), ring(k, j + 1)
+            c, d = ring(k + 1, j), ring(k + 1, j + 1)
+            out.append(f"f {a} {b} {d}")
+            out.append(f"f {a} {d} {c}")
+    for j in range(SPHERE_SLICES):
+        out.append(f"f {south} {ring(SPHERE_STACKS - 1, j)} {ring(SPHERE_STACKS - 1, j + 1)}")
+    return "\n".join(out) + "\n"
+
+
+def _rgbe_bytes(rgb: np.ndarray) -> bytes:
+    """Flat (non-RLE) Radiance RGBE pixels for an (H, W, 3) float array."""
+    m = rgb.max(axis=2)
+    mant, exp = np.frexp(m)
+    scale = np.where(m > 1e-32, mant * 256.0 / np.where(m > 1e-32, m, 1.0), 0.0)
+    out = np.zeros(rgb.shape[:2] + (4,), dtype=np.uint8)
+    out[..., :3] = np.clip(np.floor(rgb * scale[..., None]), 0, 255).astype(np.uint8)
+    out[..., 3] = np.where(m > 1e-32, exp + 128, 0).astype(np.uint8)
+    return out.tobytes()
+
+
+def sky_hdr(width=1024, height=512, sun_luminance=5.0e4) -> bytes:
+    """Gradient sky + sun disk, linear sRGB radiance, equirectangular (Luz convention)."""
+    v = 1.0 - (np.arange(height) + 0.5) / height  # EnvironmentMap: row y <-> v = 1 - (y+0.5)/H
+    u = (np.arange(width) + 0.5) / width
+    uu, vv = np.meshgrid(u, v)
+    elev = (vv - 0.5) * math.pi  # asin(y)
+    phi = (uu - 0.5) * 2.0 * math.pi
+    dy = np.sin(elev)
+    rad = np.cos(elev)
+    dx, dz = rad * np.cos(phi), rad * np.sin(phi)
+    t = np.clip(dy, 0.0, 1.0)[..., None]
+    zenith = np.array([0.25, 0.45, 1.0]) * 0.8
+    horizon = np.array([1.0, 0.95, 0.9]) * 1.2
+    ground = np.array([0.35, 0.3, 0.25]) * 0.25
+    sky = horizon * (1.0 - t) + zenith * t
+    img = np.where((dy >= 0.0)[..., None], sky, ground)
+    sun_dir = np.array([math.cos(math.radians(35)) * math.cos(1.0), math.sin(math.radians(35)),
+                        math.cos(math.radians(35)) * math.sin(1.0)])
+    cosang = dx * sun_dir[0] + dy * sun_dir[1] + dz * sun_dir[2]
+    sun_radius = math.radians(1.5)
+    sun = (cosang >= math.cos(sun_radius))[..., None]
+    sun_rgb = np.array([1.0, 0.95, 0.85])
+    sun_rgb = sun_rgb * ((sun_luminance / LUMENS_PER_RADIANT_WATT) / float(np.dot(sun_rgb, [0.2126, 0.7152, 0.0722])))
+    img = np.where(sun, sun_rgb, img)
+    header = (b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {height} +X {width}\n".encode())
+    return header + _rgbe_bytes(img.astype(np.float64))
+
+
+def spheres_scene(directory: str, width=1920, height=1080, spp=64, depth=12, count=512,
+                  seed=1002, render_seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    rng = random.Random(seed)
+    mats, objs = [], []
+    for i in range(count):
+        radius = rng.uniform(0.5, 3.0)
+        centre = (rng.uniform(-100.0, 100.0), rng.uniform(-100.0, 100.0), rng.uniform(-100.0, 100.0))
+        pick = rng.random()
+        if pick < 0.4:
+            albedo = tuple(rng.uniform(0.1, 0.9) for _ in range(3))
+            mats += [f"material m{i} {{", "type=lambertian", f"color={_v(albedo)}", "}"]
+        elif pick < 0.7:
+            alpha = rng.uniform(0.05, 0.5)
+            tint = tuple(rng.uniform(0.5, 0.95) for _ in range(3))
+            mats += [f"material m{i} {{", "type=metal", f"color={_v(tint)}", f"roughness={_f(math.sqrt(alpha))}", "}"]
+        else:
+            mats += [f"material m{i} {{", "type=dielectric", "color=(1,1,1)", "ior=1.5", "roughness=0", "}"]
+        obj_name = f"sphere_{i:03d}.obj"
+        _write_if_changed(os.path.join(directory, obj_name), _sphere_obj(rng, radius).encode())
+        objs += [f"object s{i} {{", f"file={obj_name}", f"position={_v(centre)}", f"material=m{i}", "}"]
+    _write_if_changed(os.path.join(directory, "sky.hdr"), sky_hdr())
+    lines = [
+        "[settings]",
+        f"resolution={width},{height}",
+        f"samples={spp}",
+        f"maxlightbounces={depth}",
+        "adaptive=0",
+        "denoise=0",
+        "view_transform=raw",
+        "bloom=0",
+        "outputfilename=render.tiff",
+        "environment=sky.hdr",
+        "",
+        "[materials]",
+        *mats,
+        "",
+        "[scene]",
+        "camera main {",
+        "position=(0,0,-135)",
+        "direction=(0,0,1)",
+        "up=(0,1,0)",
+        "focal_length_mm=24",
+        "sensor_width_mm=36",
+        "sensor_height_mm=20.25",
+        "pinhole=1",
+        "focus_distance=10",
+        "}",
+        *objs,
+    ]
+    path = os.path.join(directory, "spheres.luz")
+    _write_if_changed(path, ("\n".join(lines) + "\n").encode())
+    return Workload("spheres", directory, path, width, height, spp, depth, render_seed,
+                    count * 2 * SPHERE_SLICES * (SPHERE_STACKS - 1), {"instances": count})
+
+
+def _write_if_changed(path: str, data: bytes) -> None:
+    try:
+        with open(path, "rb") as fh:
+            if fh.read() == data:
+                return
+    except OSError:
+        pass
+    tmp = path + ".tmp"
+    with open(tmp, "wb") as fh:
+        fh.write(data)
+    os.replace(tmp, path)
