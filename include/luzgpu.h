@@ -1,0 +1,122 @@
+/*
+ * luzgpu — C ABI of the MI355X radiance path for the Luz renderer.
+ *
+ * Drop-in boundary. The reference has no FFI: its seam is the C++ call
+ * `bool Renderer::render(Scene&)` (include/luz/Renderer/Renderer.hpp:7), whose worker
+ * loop `Renderer::internal::_manageThreads` (src/renderer/Threads.cpp:375-548) runs
+ * `_threadRender` (Threads.cpp:551-641) over 16-pixel blocks. These entry points
+ * replace that worker loop: the scene (as loaded by SceneFile::read,
+ * include/luz/SceneFile/SceneFile.hpp:8) is uploaded once, then each render call
+ * produces the raw scene-linear image that `_manageThreads` leaves in
+ * Scene::getImage() before post-processing.
+ *
+ * Conventions: every function returns an int status (0 = OK, negative = error) and
+ * never throws; the message of the last failure on the calling thread is available
+ * from luzgpu_last_error(). Output buffers are caller-owned. A context owns one HIP
+ * device and stream; calls on one context must be serialised by the caller (the
+ * reference's render is not re-entrant either: global render seed, Sampler.cpp:28).
+ */
+#ifndef LUZGPU_H
+#define LUZGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUZGPU_OK 0
+#define LUZGPU_E_INVALID (-1)      /* bad argument (reference: std::invalid_argument)   */
+#define LUZGPU_E_UNSUPPORTED (-2)  /* scene feature outside the device path             */
+#define LUZGPU_E_OOM (-3)          /* device or host allocation failed                  */
+#define LUZGPU_E_DEVICE (-4)       /* HIP runtime error                                 */
+#define LUZGPU_E_IO (-5)           /* file missing or malformed (reference: runtime_error) */
+
+#define LUZGPU_INTEGRATOR_WAVEFRONT 0u
+#define LUZGPU_INTEGRATOR_MEGAKERNEL 1u
+
+typedef struct luzgpu_ctx luzgpu_ctx;
+typedef struct luzgpu_scene luzgpu_scene;
+
+/* Render settings. Fields left at their "scene" value (< 0 / 0) use what the scene
+ * file set, as the reference's FlagsParser overrides only the flags it is given
+ * (src/cli/FlagsParser.cpp:87-117). */
+typedef struct luzgpu_settings
+{
+	uint32_t	seed;            /* --seed: Sampler render seed = (uint32_t)seed, Threads.cpp:383-388 */
+	int32_t		samples;         /* spp, < 0: scene value (Scene::getSampleCount)                     */
+	int32_t		max_bounces;     /* < 0: scene value (Scene::getMaxLightBounces)                      */
+	uint32_t	sample_begin;    /* first sample index of this call (sample-range sharding)           */
+	int32_t		sample_count;    /* samples in this call, < 0: samples - sample_begin                 */
+	uint32_t	integrator;      /* LUZGPU_INTEGRATOR_*                                               */
+	uint32_t	output_on_device;/* 1: rgb_out is a device pointer on the context's device            */
+	int32_t		width;           /* <= 0: scene value (--resolution, FlagsParser.cpp)                 */
+	int32_t		height;          /* <= 0: scene value                                                 */
+} luzgpu_settings;
+
+typedef struct luzgpu_stats
+{
+	uint64_t	rendered_samples;    /* SceneRenderStats::renderedSamples (Scene.hpp:16-24) */
+	uint64_t	closest_hit_rays;
+	uint64_t	shadow_rays;
+	double		render_ms;           /* SceneRenderStats::renderMS                         */
+} luzgpu_stats;
+
+typedef struct luzgpu_scene_info
+{
+	uint32_t	width, height, samples;
+	int32_t		max_bounces;
+	uint32_t	hittables, triangles, meshes, instances, lights;
+	uint32_t	tlas_nodes, blas_nodes;
+	uint32_t	env_width, env_height;
+	uint32_t	tlas_depth, blas_depth;
+} luzgpu_scene_info;
+
+/* Context: one HIP device + stream. (Threads.cpp:375-548 worker pool) */
+int		luzgpu_create(int device, luzgpu_ctx** out);
+void	luzgpu_destroy(luzgpu_ctx* ctx);
+const char*	luzgpu_last_error(void);
+
+/* Scene upload from a .luz file: SceneFile::read (SceneFile.cpp:256-321) followed by the
+ * pre-render sync of Renderer::render (Renderer.cpp:18-20: updateLights,
+ * updateAccelerationStructure). With ctx == NULL the scene is parsed and its acceleration
+ * structures built on the host only (for inspection; it cannot be rendered). */
+int		luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out);
+void	luzgpu_scene_release(luzgpu_scene* scene);
+int		luzgpu_scene_get_info(const luzgpu_scene* scene, luzgpu_scene_info* out);
+
+/* Renders samples [sample_begin, sample_begin + sample_count) and writes
+ * rgb_out[W*H*3] (fp32, row-major, y down) = mean over those samples of the cleaned
+ * per-sample radiance — Scene::getImage() after _threadRender with adaptive sampling and
+ * denoising off and a raw view transform (Threads.cpp:551-583, 77-88). */
+int		luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings,
+			float* rgb_out, luzgpu_stats* stats);
+
+/* --- introspection used by the parity tests (no counterpart in the reference API) --- */
+
+/* Primary-ray closest hits of sample 0, stream 0 (RayHelper.cpp:85-106 + HitHelper.cpp:5-32):
+ * ids_out[W*H*2] = (top-level hittable index in Scene::getHittables() order, triangle index
+ * within the mesh or -1), t_out[W*H] = hit distance (or -1 on a miss). */
+int		luzgpu_primary_hits(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings,
+			int64_t* ids_out, float* t_out);
+
+/* Evaluates the device sampler (Sampler.cpp:161-237) for n keys; keys[7*i..] =
+ * (seed, x, y, sample, stream, bounce, dimension). s1[i] = sample1D, s2[2*i..] = sample2D. */
+int		luzgpu_sampler_eval(luzgpu_ctx* ctx, size_t n, const uint32_t* keys, double* s1, float* s2);
+
+/* Top-level BVH topology in nested-parenthesis form over hittable indices
+ * (BVHNode.cpp:142-185). needed receives the length including the terminator. */
+int		luzgpu_scene_tlas_topology(const luzgpu_scene* scene, char* buf, size_t cap, size_t* needed);
+
+/* Mesh BVH of mesh `mesh` (Mesh::_packedBVHNodes / _triangleIndices, Mesh.hpp:37-67):
+ * boxes[6*k] (min xyz, max xyz, FP64), links[5*k] (left, right, start, count, leaf),
+ * tri_index[t]. Pass NULL buffers to query node_count / tri_count. */
+int		luzgpu_scene_blas(const luzgpu_scene* scene, uint32_t mesh, double* boxes, int64_t* links,
+			int64_t* tri_index, uint64_t* node_count, uint64_t* tri_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,104 @@
+"""ctypes binding of the luzgpu C ABI (include/luzgpu.h).
+
+The library is the in-tree ``luz_amd/libluzgpu.so``. Loading fails loudly when it is
+missing: there is no CPU fallback for the device path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libluzgpu.so")
+
+OK = 0
+E_INVALID = -1
+E_UNSUPPORTED = -2
+E_OOM = -3
+E_DEVICE = -4
+E_IO = -5
+
+INTEGRATOR_WAVEFRONT = 0
+INTEGRATOR_MEGAKERNEL = 1
+
+EXPORTED = [
+    "luzgpu_create", "luzgpu_destroy", "luzgpu_last_error", "luzgpu_scene_load",
+    "luzgpu_scene_release", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_primary_hits",
+    "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas",
+]
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint32),
+        ("samples", ctypes.c_int32),
+        ("max_bounces", ctypes.c_int32),
+        ("sample_begin", ctypes.c_uint32),
+        ("sample_count", ctypes.c_int32),
+        ("integrator", ctypes.c_uint32),
+        ("output_on_device", ctypes.c_uint32),
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("rendered_samples", ctypes.c_uint64),
+        ("closest_hit_rays", ctypes.c_uint64),
+        ("shadow_rays", ctypes.c_uint64),
+        ("render_ms", ctypes.c_double),
+    ]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("width", "height", "samples")] + [
+        ("max_bounces", ctypes.c_int32)] + [
+        (n, ctypes.c_uint32) for n in ("hittables", "triangles", "meshes", "instances", "lights",
+                                        "tlas_nodes", "blas_nodes", "env_width", "env_height",
+                                        "tlas_depth", "blas_depth")]
+
+
+class LuzGpuError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"luzgpu error {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run luz_amd/build.py (or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.luzgpu_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.luzgpu_destroy.argtypes = [vp]
+    lib.luzgpu_destroy.restype = None
+    lib.luzgpu_last_error.restype = ctypes.c_char_p
+    lib.luzgpu_scene_load.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
+    lib.luzgpu_scene_release.argtypes = [vp]
+    lib.luzgpu_scene_release.restype = None
+    lib.luzgpu_scene_get_info.argtypes = [vp, ctypes.POINTER(SceneInfo)]
+    lib.luzgpu_render.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, ctypes.POINTER(Stats)]
+    lib.luzgpu_primary_hits.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, vp]
+    lib.luzgpu_sampler_eval.argtypes = [vp, ctypes.c_size_t, vp, vp, vp]
+    lib.luzgpu_scene_tlas_topology.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    lib.luzgpu_scene_blas.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint64)]
+    for name in ("luzgpu_create", "luzgpu_scene_load", "luzgpu_scene_get_info", "luzgpu_render",
+                 "luzgpu_primary_hits", "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas"):
+        getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != OK:
+        msg = load().luzgpu_last_error().decode(errors="replace")
+        raise LuzGpuError(code, msg)

@@ -1,0 +1,164 @@
+// Device-resident scene layout (HBM), shared by the host packer and the HIP kernels.
+// Plain PODs only: every array is a flat, 16-byte aligned buffer.
+//
+//   BvhNode   64 B  two child boxes (fp32, rounded outward) + two child references.
+//                   One cache-line fetch tests both children of a binary node; leaves
+//                   are folded into their parent's reference (count > 0).
+//   GpuTri    48 B  v0 | e1 | e2 in fp32 (e1, e2 formed in FP64, then rounded), with
+//                   the triangle's index in its mesh (after degenerate removal) in v0.w.
+//   Hittable  8 B   kind:2 | index:30, material id — the reference's top-level list.
+#pragma once
+
+#include <cstdint>
+
+namespace luzgpu
+{
+	struct alignas(16) BvhNode
+	{
+		float	lo0[3];
+		float	hi0[3];
+		float	lo1[3];
+		float	hi1[3];
+		std::int32_t	child[2];	// count == 0: node index; count > 0: first primitive
+		std::int32_t	count[2];	// 0 = internal child, > 0 = leaf primitive count, -1 = empty
+	};
+	static_assert(sizeof(BvhNode) == 64, "BvhNode must be 64 bytes");
+
+	struct alignas(16) GpuTri
+	{
+		float	v0[3];
+		std::int32_t	prim;
+		float	e1[3];
+		float	pad1;
+		float	e2[3];
+		float	pad2;
+	};
+	static_assert(sizeof(GpuTri) == 48, "GpuTri must be 48 bytes");
+
+	enum : std::uint32_t
+	{
+		GPU_HIT_TRIANGLE = 0,
+		GPU_HIT_RECTANGLE = 1,
+		GPU_HIT_INSTANCE = 2,
+	};
+
+	struct GpuHittable
+	{
+		std::uint32_t	kindIndex;	// kind << 30 | index
+		std::uint32_t	material;
+	};
+
+	struct alignas(16) GpuRect
+	{
+		float	position[3];
+		float	halfWidth;
+		float	normal[3];
+		float	halfHeight;
+		float	widthAxis[3];
+		float	area;
+		float	heightAxis[3];
+		float	width;
+		float	height;
+		float	pad[3];
+	};
+
+	struct alignas(16) GpuInstance
+	{
+		float	worldToLocal[12];	// rows of a 3x4 affine map (inverse rotate, inverse scale)
+		float	normalToWorld[9];	// rotate(n / scale), renormalised after use
+		std::int32_t	blasRoot;	// global BvhNode index of the mesh's root
+		std::int32_t	triBase;	// first GpuTri of the mesh
+		std::int32_t	normalBase;	// first vertex-normal triple, -1 if the mesh has none
+		float	pad[2];
+	};
+
+	enum : std::uint32_t
+	{
+		GPU_MAT_LAMBERTIAN = 1,
+		GPU_MAT_METAL = 2,
+		GPU_MAT_DIELECTRIC = 3,
+		GPU_MAT_EMISSIVE = 4,
+	};
+
+	struct alignas(16) GpuMaterial
+	{
+		std::uint32_t	kind;
+		float	color[3];		// albedo / f0 / tint / emitted radiance
+		float	alpha;			// GGX alpha = max(1e-4, roughness^2)
+		float	roughness;
+		float	ior;
+		std::uint32_t	flags;	// bit0 conductor Fresnel, bit1 medium absorption
+		float	eta[3];
+		float	pad0;
+		float	k[3];
+		float	pad1;
+		float	absorption[3];
+		float	pad2;
+	};
+
+	enum : std::uint32_t
+	{
+		GPU_MATF_CONDUCTOR = 1u,
+		GPU_MATF_ABSORPTION = 2u,
+	};
+
+	struct alignas(16) GpuLight
+	{
+		std::uint32_t	kind;		// GPU_HIT_TRIANGLE / GPU_HIT_RECTANGLE
+		std::uint32_t	index;		// into top-level triangles / rectangles
+		std::uint32_t	hittable;
+		std::uint32_t	material;
+		float	emitted[3];
+		float	area;
+		float	selectionProbability;	// weight / total (or 1/n when unweighted)
+		float	pad[3];
+	};
+
+	// Everything the per-pixel camera needs, in FP64 exactly as the reference's
+	// RenderCamera (src/renderer/RendererInternal.hpp:13-26).
+	struct GpuCamera
+	{
+		double	position[3];
+		double	u[3];
+		double	v[3];
+		double	horizontal[3];
+		double	vertical[3];
+		double	lowerLeftCorner[3];
+		double	lensRadius;
+		double	inverseWidthMinusOne;
+		double	inverseHeightMinusOne;
+	};
+
+	enum : std::uint32_t
+	{
+		GPU_SKY_NONE = 0,
+		GPU_SKY_ENVIRONMENT = 1,
+	};
+
+	// Scalars of one render: all device pointers live in GpuSceneView.
+	struct GpuParams
+	{
+		GpuCamera	camera;
+		std::uint32_t	width;
+		std::uint32_t	height;
+		std::uint32_t	renderSeed;
+		std::int32_t	maxBounces;
+		std::uint32_t	sampleBegin;
+		std::uint32_t	sampleCount;	// samples in this launch
+		std::uint32_t	totalSamples;	// divisor for the final mean
+		std::uint32_t	sky;
+		float	background[3];
+		float	backgroundLuminance;
+		std::uint32_t	lightCount;
+		std::int32_t	tlasRoot;		// -1: no bounded hittables
+		std::uint32_t	envWidth;
+		std::uint32_t	envHeight;
+		float	envStrength;
+		float	envRotation;
+		std::uint32_t	envLighting;
+		std::uint32_t	hasEnvironment;
+		double	envTotalWeight;
+		float	metersPerUnit;
+		std::uint32_t	unboundedCount;	// rectangles without a box: tested linearly
+	};
+}

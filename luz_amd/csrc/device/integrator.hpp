@@ -1,0 +1,263 @@
+// Per-path integrator pieces shared by the megakernel and the wavefront stages:
+// camera rays (FP64, RayHelper.cpp:85-106), direct lighting (ColorHelper.cpp:973-1033),
+// environment lighting (ColorHelper.cpp:1035-1098) and the bounce loop
+// (calculateLightRaysColor, ColorHelper.cpp:1136-1326).
+#pragma once
+
+#include "shade.hpp"
+
+namespace luzgpu
+{
+	// Thin-lens disk sample (Sampler.cpp:267-292), fp32 as in the reference.
+	__device__ __forceinline__ void	unitDisk(const PixelSampler& ps, float& dx, float& dy)
+	{
+		float sx, sy;
+		ps.sample2D(DIM_LENS, sx, sy);
+		sx = 2.0f * sx - 1.0f;
+		sy = 2.0f * sy - 1.0f;
+		if (sx == 0.0f && sy == 0.0f) { dx = 0.0f; dy = 0.0f; return; }
+		float r, th;
+		if (fabsf(sx) > fabsf(sy)) { r = sx; th = (kPiF / 4.0f) * (sy / sx); }
+		else { r = sy; th = (kPiF / 2.0f) - ((kPiF / 4.0f) * (sx / sy)); }
+		dx = r * cosf(th);
+		dy = r * sinf(th);
+	}
+
+	// Camera ray in FP64 with the reference's operation order, then rounded to fp32.
+	__device__ __forceinline__ Ray	cameraRay(const GpuParams& p, std::uint32_t x, std::uint32_t y, const PixelSampler& ps)
+	{
+#pragma clang fp contract(off)
+		const GpuCamera& c = p.camera;
+		float cx, cy;
+		ps.sample2D(DIM_CAMERA, cx, cy);
+		const double xU = (static_cast<double>(x) + static_cast<double>(cx)) * c.inverseWidthMinusOne;
+		const double yV = (static_cast<double>(y) + static_cast<double>(cy)) * c.inverseHeightMinusOne;
+		double off[3] = {0.0, 0.0, 0.0};
+		if (c.lensRadius > 0.0)
+		{
+			float dx, dy;
+			unitDisk(ps, dx, dy);
+			const double rx = c.lensRadius * static_cast<double>(dx);
+			const double ry = c.lensRadius * static_cast<double>(dy);
+			for (int k = 0; k < 3; k++) off[k] = (rx * c.u[k]) + (ry * c.v[k]);
+		}
+		double d[3], o[3];
+		for (int k = 0; k < 3; k++)
+		{
+			const double hx = xU * c.horizontal[k];
+			const double vy = yV * c.vertical[k];
+			d[k] = (((c.lowerLeftCorner[k] - hx) - vy) - c.position[k]) - off[k];
+			o[k] = c.position[k] + off[k];
+		}
+		const double l = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+		f3 dir = mk(0, 0, 0);
+		if (isfinite(l) && l > 0.0)
+		{
+			const double inv = 1.0 / l;
+			dir = mk(static_cast<float>(inv * d[0]), static_cast<float>(inv * d[1]), static_cast<float>(inv * d[2]));
+		}
+		return makeRay(mk(static_cast<float>(o[0]), static_cast<float>(o[1]), static_cast<float>(o[2])), dir);
+	}
+
+	__device__ __forceinline__ float	directProbability(int bounce)
+	{
+		return bounce < 2 ? 1.0f : 0.5f;
+	}
+
+	__device__ __forceinline__ bool	samplesVisibleInfinite(const GpuParams& p, const Scatter& sc)
+	{
+		return (p.sky == GPU_SKY_ENVIRONMENT && hasEnvLight(p)) || (hasConstantBackgroundLight(p) && sc.pdfType != PDF_BSDF);
+	}
+
+	// Unoccluded contribution of one light sample and the shadow ray that gates it.
+	struct	Connection
+	{
+		f3	contribution;
+		f3	direction;
+		float	tMax;
+		bool	active;
+	};
+
+	__device__ __forceinline__ Connection	connectLight(const SceneView& s, const GpuParams& p, const Surface& sf, const GpuMaterial& m,
+		const Scatter& sc, int bounce, const PixelSampler& ps)
+	{
+		Connection c;
+		c.active = false;
+		const float prob = directProbability(bounce);
+		if (prob < 1.0f && ps.sample1D(DIM_LIGHT_STRATEGY) >= static_cast<double>(prob)) return c;
+		const LightSample ls = sampleLight(s, p, sf.position, ps);
+		if (!ls.valid || !leavesOpaque(sf, m.kind, ls.direction)) return c;
+		const float spdf = scatterPdf(m, sf, sc, ls.direction);
+		const f3 f = scatterBsdfCos(m, sf, sc, ls.direction);
+		if (spdf <= 0.0f || !isfinite(spdf) || maxc(f) <= 0.0f) return c;
+		const float w = powerHeuristic(ls.pdf, spdf);
+		c.contribution = f * ls.emitted * (w / (ls.pdf * prob));
+		c.direction = ls.direction;
+		c.tMax = ls.tMax - kTMinF;
+		c.active = c.tMax > kTMinF;	// isShadowOccluded: tMax <= T_MIN is never occluded
+		if (!c.active)
+		{
+			c.active = true;
+			c.tMax = -1.0f;	// no shadow test needed
+		}
+		return c;
+	}
+
+	__device__ __forceinline__ Connection	connectInfinite(const SceneView& s, const GpuParams& p, const Surface& sf, const GpuMaterial& m,
+		const Scatter& sc, int bounce, const PixelSampler& ps)
+	{
+		Connection c;
+		c.active = false;
+		const bool constBg = hasConstantBackgroundLight(p);
+		if (constBg && sc.pdfType == PDF_BSDF) return c;
+		const float prob = directProbability(bounce);
+		if (prob < 1.0f && ps.sample1D(DIM_ENVIRONMENT_STRATEGY) >= static_cast<double>(prob)) return c;
+		const LightSample ls = sampleInfinite(s, p, ps);
+		if (!ls.valid || !leavesOpaque(sf, m.kind, ls.direction)) return c;
+		const float spdf = scatterPdf(m, sf, sc, ls.direction);
+		const f3 f = scatterBsdfCos(m, sf, sc, ls.direction);
+		if (spdf <= 0.0f || !isfinite(spdf) || maxc(f) <= 0.0f) return c;
+		const float w = powerHeuristic(ls.pdf, spdf);
+		f3 contrib = f * ls.emitted * (w / (ls.pdf * prob));
+		if (constBg) contrib = clampConstantBackground(p, contrib);
+		c.contribution = contrib;
+		c.direction = ls.direction;
+		c.tMax = INFINITY;
+		c.active = true;
+		return c;
+	}
+
+	__device__ __forceinline__ f3	missRadiance(const SceneView& s, const GpuParams& p, f3 dir, float envWeight)
+	{
+		f3 sky;
+		if (p.sky == GPU_SKY_ENVIRONMENT && p.hasEnvironment && p.envStrength > 0.0f)
+		{
+			sky = envLookup(s, p, dir) * p.envStrength;
+		}
+		else
+		{
+			sky = mk(p.background[0], p.background[1], p.background[2]);
+		}
+		return sky * envWeight;
+	}
+
+	__device__ __forceinline__ f3	cosineDirection(const Surface& sf, const PixelSampler& ps)
+	{
+		// Sampler::cosineHemisphere (Sampler.cpp:239-246) in the ONB of the shading normal
+		float sx, sy;
+		ps.sample2D(DIM_BSDF_DIRECTION, sx, sy);
+		const float phi = (2.0f * kPiF) * sx;
+		const float r = sqrtf(sy);
+		const float z = sqrtf(fmaxf(0.0f, 1.0f - sy));
+		float sp, cp;
+		sincosf(phi, &sp, &cp);
+		return Onb(sf.normal).local(r * cp, r * sp, z);
+	}
+
+	__device__ __forceinline__ f3	cleanColor(f3 c)
+	{
+		return mk(isfinite(c.x) ? c.x : 0.0f, isfinite(c.y) ? c.y : 0.0f, isfinite(c.z) ? c.z : 0.0f);
+	}
+
+	// Whole path in one thread: the plain (megakernel) integrator.
+	template <typename Stack>
+	__device__ f3	tracePath(const SceneView& s, const GpuParams& p, std::uint32_t x, std::uint32_t y, std::uint32_t sample,
+		Stack& stack, std::uint32_t& closestRays, std::uint32_t& shadowRays)
+	{
+		PixelSampler ps;
+		ps.begin(p.renderSeed, x, y, sample, 0);
+		Ray ray = cameraRay(p, x, y, ps);
+		f3 acc = mk(0, 0, 0), thr = mk(1, 1, 1), prevOrigin = mk(0, 0, 0);
+		bool prevSpecular = true, prevInfinite = false;
+		float prevPdf = 0.0f;
+		const bool sampledInfinite = hasEnvLight(p) || hasConstantBackgroundLight(p);
+		for (int b = 0; b <= p.maxBounces; b++)
+		{
+			ps.bounce = static_cast<std::uint32_t>(b);
+			HitInfo h;
+			closestRays++;
+			if (!traverse<false>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack))
+			{
+				float w = 1.0f;
+				if (b > 0 && !prevSpecular && prevInfinite)
+				{
+					w = powerHeuristic(prevPdf, infiniteLightPdf(s, p, ray.d));
+				}
+				f3 c = thr * missRadiance(s, p, ray.d, w);
+				if (b > 0 && hasConstantBackgroundLight(p)) c = clampConstantBackground(p, c);
+				return acc + clampRay(c);
+			}
+			const Surface sf = surfaceAt(s, ray, h);
+			const GpuMaterial& m = s.materials[sf.material];
+			const f3 emitted = m.kind == GPU_MAT_EMISSIVE ? ld3(m.color) : mk(0, 0, 0);
+			Scatter sc;
+			if (!scatter(m, sf, ray.d, ps, sc))
+			{
+				const float lp = (!prevSpecular && p.lightCount > 0) ? lightsPdf(s, p, prevOrigin, ray.d) : 0.0f;
+				const float w = prevSpecular ? 1.0f : powerHeuristic(prevPdf, lp);
+				return acc + clampRay((thr * emitted) * w);
+			}
+			f3 att = sc.attenuation;
+			if (!sf.frontFace && sc.absorbing)
+			{
+				const float dm = h.t * p.metersPerUnit;
+				if (isfinite(dm) && dm > 0.0f)
+				{
+					att = att * mk(expf(-m.absorption[0] * dm), expf(-m.absorption[1] * dm), expf(-m.absorption[2] * dm));
+				}
+			}
+			if (sc.specular)
+			{
+				if (!leavesOpaque(sf, m.kind, sc.direction)) return acc;
+				thr = clampRay(thr * att);
+				if (terminated(thr) || !russianRoulette(thr, b, ps)) return acc;
+				ray = makeRay(sf.position, sc.direction);
+				prevSpecular = true;
+				prevPdf = 0.0f;
+				continue;
+			}
+			acc = acc + thr * emitted;
+			if (terminated(thr * att)) return acc;
+			if (p.lightCount > 0)
+			{
+				const Connection c = connectLight(s, p, sf, m, sc, b, ps);
+				if (c.active)
+				{
+					bool occluded = false;
+					if (c.tMax > 0.0f)
+					{
+						HitInfo sh;
+						shadowRays++;
+						occluded = traverse<true>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, c.tMax, sh, stack);
+					}
+					if (!occluded) acc = acc + clampRay(thr * c.contribution);
+				}
+			}
+			if (sampledInfinite)
+			{
+				const Connection c = connectInfinite(s, p, sf, m, sc, b, ps);
+				if (c.active)
+				{
+					HitInfo sh;
+					shadowRays++;
+					if (!traverse<true>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack))
+					{
+						acc = acc + clampRay(thr * c.contribution);
+					}
+				}
+			}
+			const f3 dir = sc.pdfType == PDF_COSINE ? cosineDirection(sf, ps) : sc.direction;
+			if (!leavesOpaque(sf, m.kind, dir)) return acc;
+			const float pdf = scatterPdf(m, sf, sc, dir);
+			if (pdf <= 0.0f || !isfinite(pdf)) return acc;
+			thr = clampRay(thr * att);
+			if (terminated(thr) || !russianRoulette(thr, b, ps)) return acc;
+			ray = makeRay(sf.position, dir);
+			prevSpecular = false;
+			prevOrigin = sf.position;
+			prevPdf = pdf;
+			prevInfinite = samplesVisibleInfinite(p, sc);
+		}
+		return acc;
+	}
+}

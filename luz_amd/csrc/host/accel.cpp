@@ -1,0 +1,989 @@
+// Acceleration structures and device packing.
+//
+// Topology parity: both builders reproduce the reference's build step by step in FP64
+// and call the same libstdc++ algorithms with the same comparison outcomes
+// (std::partition / std::nth_element for mesh BVHs, std::sort for the top level), so
+// the node order, leaf ranges and triangle order are identical to the reference's.
+//   mesh BVH (binned SAH, 12 bins, leaf <= 8)   Mesh.cpp:217-306,407-521
+//   top-level BVH (median, n-ary leaves)         BVHNode.cpp:86-185
+//   bounded/unbounded split                      Scene.cpp:655-688
+//   light list + selection CDF                   Scene.cpp:615-638
+//   camera                                       RayHelper.cpp:50-82
+//   environment distribution                     EnvironmentMap.cpp:627-671
+// The device then traverses a repacked form: binary 64-byte nodes holding both child
+// boxes in fp32 (rounded outward) with leaves folded into their parent's reference.
+#include "accel.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <functional>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+namespace luzgpu
+{
+	namespace
+	{
+		constexpr std::size_t	kMeshLeaf = 8;
+		constexpr std::size_t	kBins = 12;
+		constexpr std::size_t	kTopLeaf = 8;
+		constexpr std::size_t	kTopRootLeaf = 16;
+
+		AABBd	merge(const AABBd& a, const AABBd& b)
+		{
+			// Mesh.cpp:57-71 (std::min/std::max)
+			AABBd r;
+			for (int i = 0; i < 3; i++)
+			{
+				r.lo[i] = std::min(a.lo[i], b.lo[i]);
+				r.hi[i] = std::max(a.hi[i], b.hi[i]);
+			}
+			return (r);
+		}
+
+		AABBd	mergeF(const AABBd& a, const AABBd& b)
+		{
+			// Utilities.cpp:58-70 (fmin/fmax)
+			AABBd r;
+			for (int i = 0; i < 3; i++)
+			{
+				r.lo[i] = std::fmin(a.lo[i], b.lo[i]);
+				r.hi[i] = std::fmax(a.hi[i], b.hi[i]);
+			}
+			return (r);
+		}
+
+		double	surfaceArea(const AABBd& b)
+		{
+			const dvec3 e = b.hi - b.lo;
+			const double x = std::max(0.0, e[0]), y = std::max(0.0, e[1]), z = std::max(0.0, e[2]);
+			return (2.0 * ((x * y) + (x * z) + (y * z)));
+		}
+
+		int	largestAxis(const AABBd& b)
+		{
+			const dvec3 e = b.hi - b.lo;
+			if (e[0] >= e[1] && e[0] >= e[2]) return (0);
+			if (e[1] >= e[2]) return (1);
+			return (2);
+		}
+
+		AABBd	triangleBox(const Triangle& t)
+		{
+			// Triangle.cpp:114-125: vertex bounds padded by T_MIN
+			AABBd b;
+			for (int i = 0; i < 3; i++)
+			{
+				b.lo[i] = std::min(t.v0[i], std::min(t.v1[i], t.v2[i])) - kTMin;
+				b.hi[i] = std::max(t.v0[i], std::max(t.v1[i], t.v2[i])) + kTMin;
+			}
+			return (b);
+		}
+
+		struct	BinnedSplit
+		{
+			bool	found = false;
+			int	axis = 0;
+			std::size_t	splitBin = 0;
+			double	centroidMin = 0.0;
+			double	binScale = 0.0;
+		};
+
+		struct	Bin
+		{
+			AABBd	box;
+			std::size_t	count = 0;
+			bool	has = false;
+		};
+
+		std::size_t	binFor(double c, double mn, double scale)
+		{
+			std::size_t b = static_cast<std::size_t>((c - mn) * scale);
+			return (b >= kBins ? kBins - 1 : b);
+		}
+
+		struct	BlasBuilder
+		{
+			std::vector<AABBd>	boxes;
+			std::vector<dvec3>	centroids;
+			Blas	out;
+
+			BinnedSplit	findSplit(std::size_t start, std::size_t end)
+			{
+				const std::size_t n = end - start;
+				const std::size_t minSide = std::max<std::size_t>(1, n / 16);
+				BinnedSplit best;
+				double bestCost = std::numeric_limits<double>::infinity();
+				for (int axis = 0; axis < 3; axis++)
+				{
+					double cmin = std::numeric_limits<double>::infinity();
+					double cmax = -std::numeric_limits<double>::infinity();
+					for (std::size_t i = start; i < end; i++)
+					{
+						const double c = centroids[out.triangleIndices[i]][axis];
+						cmin = std::min(cmin, c);
+						cmax = std::max(cmax, c);
+					}
+					const double extent = cmax - cmin;
+					if (extent <= 1e-12)
+					{
+						continue;
+					}
+					const double scale = static_cast<double>(kBins) / extent;
+					std::array<Bin, kBins> bins;
+					for (std::size_t i = start; i < end; i++)
+					{
+						const std::size_t t = out.triangleIndices[i];
+						Bin& b = bins[binFor(centroids[t][axis], cmin, scale)];
+						b.box = b.has ? merge(b.box, boxes[t]) : boxes[t];
+						b.count++;
+						b.has = true;
+					}
+					std::array<Bin, kBins> left, right;
+					Bin run;
+					for (std::size_t i = 0; i < kBins; i++)
+					{
+						if (bins[i].has)
+						{
+							run.box = run.has ? merge(run.box, bins[i].box) : bins[i].box;
+							run.count += bins[i].count;
+							run.has = true;
+						}
+						left[i] = run;
+					}
+					run = Bin();
+					for (std::size_t i = kBins; i > 0; i--)
+					{
+						if (bins[i - 1].has)
+						{
+							run.box = run.has ? merge(run.box, bins[i - 1].box) : bins[i - 1].box;
+							run.count += bins[i - 1].count;
+							run.has = true;
+						}
+						right[i - 1] = run;
+					}
+					for (std::size_t s = 0; s + 1 < kBins; s++)
+					{
+						const Bin& l = left[s];
+						const Bin& r = right[s + 1];
+						if (!l.has || !r.has || l.count < minSide || r.count < minSide)
+						{
+							continue;
+						}
+						const double cost = (surfaceArea(l.box) * static_cast<double>(l.count))
+							+ (surfaceArea(r.box) * static_cast<double>(r.count));
+						if (cost < bestCost)
+						{
+							bestCost = cost;
+							best = BinnedSplit{true, axis, s, cmin, scale};
+						}
+					}
+				}
+				return (best);
+			}
+
+			std::size_t	build(std::size_t start, std::size_t end)
+			{
+				const std::size_t index = out.nodes.size();
+				const std::size_t n = end - start;
+				AABBd box;
+				for (std::size_t i = start; i < end; i++)
+				{
+					box = (i == start) ? boxes[out.triangleIndices[i]] : merge(box, boxes[out.triangleIndices[i]]);
+				}
+				out.nodes.push_back(BlasNode());
+				out.nodes[index].box = box;
+				if (n <= kMeshLeaf)
+				{
+					out.nodes[index].start = start;
+					out.nodes[index].count = n;
+					out.nodes[index].leaf = true;
+					return (index);
+				}
+				const BinnedSplit split = findSplit(start, end);
+				std::size_t mid = start;
+				if (split.found)
+				{
+					auto it = std::partition(out.triangleIndices.begin() + start, out.triangleIndices.begin() + end,
+						[&](std::size_t t) {
+							return (binFor(centroids[t][split.axis], split.centroidMin, split.binScale) <= split.splitBin);
+						});
+					mid = static_cast<std::size_t>(it - out.triangleIndices.begin());
+				}
+				if (mid == start || mid == end)
+				{
+					const int axis = largestAxis(box);
+					mid = start + n / 2;
+					std::nth_element(out.triangleIndices.begin() + start, out.triangleIndices.begin() + mid,
+						out.triangleIndices.begin() + end,
+						[&](std::size_t a, std::size_t b) {
+							const double ca = centroids[a][axis], cb = centroids[b][axis];
+							if (ca == cb)
+							{
+								return (boxes[a].lo[axis] < boxes[b].lo[axis]);
+							}
+							return (ca < cb);
+						});
+				}
+				const std::size_t l = build(start, mid);
+				const std::size_t r = build(mid, end);
+				out.nodes[index].left = l;
+				out.nodes[index].right = r;
+				return (index);
+			}
+		};
+
+		// ---- top-level hittables ------------------------------------------------------
+
+		struct	InstanceXf
+		{
+			dvec3	position, scale;
+			double	cx = 1, sx = 0, cy = 1, sy = 0, cz = 1, sz = 0;
+
+			explicit InstanceXf(const Instance& inst) : position(inst.position), scale(inst.scale)
+			{
+				// MeshInstance.cpp:48-75
+				const double d2r = 3.14159265358979323846 / 180.0;
+				cx = std::cos(inst.rotationDegrees[0] * d2r); sx = std::sin(inst.rotationDegrees[0] * d2r);
+				cy = std::cos(inst.rotationDegrees[1] * d2r); sy = std::sin(inst.rotationDegrees[1] * d2r);
+				cz = std::cos(inst.rotationDegrees[2] * d2r); sz = std::sin(inst.rotationDegrees[2] * d2r);
+			}
+
+			dvec3	rotate(dvec3 v) const
+			{
+				// MeshInstance.cpp:78-98
+				double x = v[0], y = v[1], z = v[2];
+				double ny = (y * cx) - (z * sx);
+				double nz = (y * sx) + (z * cx);
+				y = ny; z = nz;
+				double nx = (x * cy) + (z * sy);
+				nz = (-x * sy) + (z * cy);
+				x = nx; z = nz;
+				nx = (x * cz) - (y * sz);
+				ny = (x * sz) + (y * cz);
+				return {nx, ny, z};
+			}
+
+			dvec3	transformPoint(const dvec3& p) const
+			{
+				const dvec3 s = {p[0] * scale[0], p[1] * scale[1], p[2] * scale[2]};
+				return (rotate(s) + position);
+			}
+		};
+
+		bool	rectangleBasis(const Rectangle& r, dvec3& n, dvec3& wa, dvec3& ha)
+		{
+			// Rectangle.cpp:14-50 (no explicit axes)
+			const dvec3& o = r.orientation;
+			if (dot(o, o) <= 0.0)
+			{
+				return (false);
+			}
+			n = o * (1.0 / std::sqrt(dot(o, o)));
+			if (std::fabs(n[1]) > 0.999) { wa = {1, 0, 0}; ha = {0, 0, 1}; return (true); }
+			if (std::fabs(n[2]) > 0.999) { wa = {1, 0, 0}; ha = {0, 1, 0}; return (true); }
+			if (std::fabs(n[0]) > 0.999) { wa = {0, 0, 1}; ha = {0, 1, 0}; return (true); }
+			dvec3 helper = {0, 1, 0};
+			if (std::fabs(dot(helper, n)) > 0.9)
+			{
+				helper = {1, 0, 0};
+			}
+			const dvec3 c1 = cross(helper, n);
+			wa = c1 * (1.0 / std::sqrt(dot(c1, c1)));
+			const dvec3 c2 = cross(n, wa);
+			ha = c2 * (1.0 / std::sqrt(dot(c2, c2)));
+			return (true);
+		}
+
+		bool	rectangleBox(const Rectangle& r, AABBd& out)
+		{
+			// Rectangle.cpp:295-343
+			dvec3 n, wa, ha;
+			if (r.width <= 0.0 || r.height <= 0.0 || !rectangleBasis(r, n, wa, ha))
+			{
+				return (false);
+			}
+			const double hw = r.width / 2.0, hh = r.height / 2.0;
+			const dvec3 c0 = (r.position + (wa * hw)) + (ha * hh);
+			const dvec3 c1 = (r.position + (wa * hw)) - (ha * hh);
+			const dvec3 c2 = (r.position - (wa * hw)) + (ha * hh);
+			const dvec3 c3 = (r.position - (wa * hw)) - (ha * hh);
+			for (int i = 0; i < 3; i++)
+			{
+				out.lo[i] = std::min(std::min(c0[i], c1[i]), std::min(c2[i], c3[i]));
+				out.hi[i] = std::max(std::max(c0[i], c1[i]), std::max(c2[i], c3[i]));
+				if (std::fabs(out.hi[i] - out.lo[i]) <= kTMin)
+				{
+					out.lo[i] -= kTMin;
+					out.hi[i] += kTMin;
+				}
+			}
+			return (true);
+		}
+
+		bool	instanceBox(const Instance& inst, const Blas& blas, AABBd& out)
+		{
+			// MeshInstance.cpp:212-262
+			if (blas.nodes.empty())
+			{
+				return (false);
+			}
+			const InstanceXf xf(inst);
+			const AABBd& lb = blas.nodes[0].box;
+			double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+			bool any = false;
+			for (int c = 0; c < 8; c++)
+			{
+				const dvec3 corner = {(c & 1) ? lb.hi[0] : lb.lo[0], (c & 2) ? lb.hi[1] : lb.lo[1], (c & 4) ? lb.hi[2] : lb.lo[2]};
+				const dvec3 w = xf.transformPoint(corner);
+				if (!std::isfinite(w[0]) || !std::isfinite(w[1]) || !std::isfinite(w[2]))
+				{
+					continue;
+				}
+				for (int i = 0; i < 3; i++)
+				{
+					mn[i] = std::min(mn[i], w[i]);
+					mx[i] = std::max(mx[i], w[i]);
+				}
+				any = true;
+			}
+			if (!any)
+			{
+				return (false);
+			}
+			out.lo = {mn[0], mn[1], mn[2]};
+			out.hi = {mx[0], mx[1], mx[2]};
+			return (true);
+		}
+
+		struct	TopNode
+		{
+			std::vector<std::int64_t>	children;	// >= 0 hittable, < 0: ~node index
+			std::vector<AABBd>	childBoxes;
+			AABBd	box;
+		};
+
+		struct	TlasBuilder
+		{
+			const std::vector<AABBd>&	boxes;	// per scene hittable
+			std::vector<TopNode>	nodes;
+
+			std::int64_t	build(std::vector<std::uint32_t>& items, std::size_t start, std::size_t end, bool root)
+			{
+				// BVHNode.cpp:142-185
+				AABBd range;
+				for (std::size_t i = start; i < end; i++)
+				{
+					range = (i == start) ? boxes[items[i]] : mergeF(range, boxes[items[i]]);
+				}
+				const int axis = largestAxis(range);
+				const std::size_t n = end - start;
+				const std::size_t self = nodes.size();
+				nodes.push_back(TopNode());
+				TopNode node;
+				if (n <= kTopLeaf || (root && n <= kTopRootLeaf))
+				{
+					for (std::size_t i = start; i < end; i++)
+					{
+						node.children.push_back(items[i]);
+						node.childBoxes.push_back(boxes[items[i]]);
+					}
+				}
+				else
+				{
+					std::sort(items.begin() + start, items.begin() + end, [&](std::uint32_t a, std::uint32_t b) {
+						return (boxes[a].lo[axis] < boxes[b].lo[axis]);
+					});
+					const std::size_t mid = start + n / 2;
+					const std::int64_t l = build(items, start, mid, false);
+					const std::int64_t r = build(items, mid, end, false);
+					node.children = {~l, ~r};
+					node.childBoxes = {nodes[static_cast<std::size_t>(l)].box, nodes[static_cast<std::size_t>(r)].box};
+				}
+				for (std::size_t i = 0; i < node.childBoxes.size(); i++)
+				{
+					node.box = (i == 0) ? node.childBoxes[0] : mergeF(node.box, node.childBoxes[i]);
+				}
+				nodes[self] = node;
+				return (static_cast<std::int64_t>(self));
+			}
+
+			void	dump(std::ostringstream& os, std::int64_t node) const
+			{
+				const TopNode& n = nodes[static_cast<std::size_t>(node)];
+				os << "(";
+				for (std::size_t i = 0; i < n.children.size(); i++)
+				{
+					if (i) os << ",";
+					if (n.children[i] >= 0) os << n.children[i];
+					else dump(os, ~n.children[i]);
+				}
+				os << ")";
+			}
+		};
+
+		float	roundDown(double v)
+		{
+			float f = static_cast<float>(v);
+			if (static_cast<double>(f) > v) f = std::nextafter(f, -INFINITY);
+			return (f);
+		}
+
+		float	roundUp(double v)
+		{
+			float f = static_cast<float>(v);
+			if (static_cast<double>(f) < v) f = std::nextafter(f, INFINITY);
+			return (f);
+		}
+
+		void	setChild(BvhNode& node, int slot, const AABBd* box, std::int32_t child, std::int32_t count)
+		{
+			float* lo = slot == 0 ? node.lo0 : node.lo1;
+			float* hi = slot == 0 ? node.hi0 : node.hi1;
+			for (int i = 0; i < 3; i++)
+			{
+				if (box)
+				{
+					lo[i] = roundDown(box->lo[i]);
+					hi[i] = roundUp(box->hi[i]);
+				}
+				else
+				{
+					lo[i] = INFINITY;	// empty slot: no ray enters it
+					hi[i] = -INFINITY;
+				}
+			}
+			node.child[slot] = child;
+			node.count[slot] = count;
+		}
+
+		GpuTri	packTriangle(const Triangle& t, std::int32_t prim)
+		{
+			GpuTri g{};
+			const dvec3 e1 = t.v1 - t.v0, e2 = t.v2 - t.v0;
+			for (int i = 0; i < 3; i++)
+			{
+				g.v0[i] = static_cast<float>(t.v0[i]);
+				g.e1[i] = static_cast<float>(e1[i]);
+				g.e2[i] = static_cast<float>(e2[i]);
+			}
+			g.prim = prim;
+			return (g);
+		}
+
+		// Mesh BVH -> binary device nodes appended to `nodes`; returns the root index.
+		std::int32_t	packBlas(const Blas& blas, std::int32_t triBase, std::vector<BvhNode>& nodes)
+		{
+			std::vector<std::int32_t> map(blas.nodes.size(), -1);
+			const std::int32_t base = static_cast<std::int32_t>(nodes.size());
+			// device nodes for internal reference nodes, in reference (preorder) order
+			std::int32_t next = base;
+			for (std::size_t i = 0; i < blas.nodes.size(); i++)
+			{
+				if (!blas.nodes[i].leaf)
+				{
+					map[i] = next++;
+				}
+			}
+			const bool rootLeaf = blas.nodes.empty() || blas.nodes[0].leaf;
+			nodes.resize(static_cast<std::size_t>(next) + (rootLeaf ? 1 : 0));
+			if (rootLeaf)
+			{
+				BvhNode& r = nodes[static_cast<std::size_t>(next)];
+				if (blas.nodes.empty())
+				{
+					setChild(r, 0, nullptr, 0, -1);
+				}
+				else
+				{
+					setChild(r, 0, &blas.nodes[0].box, triBase + static_cast<std::int32_t>(blas.nodes[0].start),
+						static_cast<std::int32_t>(blas.nodes[0].count));
+				}
+				setChild(r, 1, nullptr, 0, -1);
+				return (next);
+			}
+			for (std::size_t i = 0; i < blas.nodes.size(); i++)
+			{
+				const BlasNode& n = blas.nodes[i];
+				if (n.leaf)
+				{
+					continue;
+				}
+				BvhNode& d = nodes[static_cast<std::size_t>(map[i])];
+				const std::size_t kids[2] = {n.left, n.right};
+				for (int s = 0; s < 2; s++)
+				{
+					const BlasNode& c = blas.nodes[kids[s]];
+					if (c.leaf)
+					{
+						setChild(d, s, &c.box, triBase + static_cast<std::int32_t>(c.start), static_cast<std::int32_t>(c.count));
+					}
+					else
+					{
+						setChild(d, s, &c.box, map[kids[s]], 0);
+					}
+				}
+			}
+			return (map[0]);
+		}
+
+		dvec3	normalizeD(const dvec3& v)
+		{
+			return (v * (1.0 / std::sqrt(dot(v, v))));
+		}
+	}
+
+	Blas	buildBlas(const Mesh& mesh)
+	{
+		BlasBuilder b;
+		const std::size_t n = mesh.triangles.size();
+		if (n == 0)
+		{
+			return (Blas());
+		}
+		b.boxes.reserve(n);
+		b.centroids.reserve(n);
+		b.out.triangleIndices.resize(n);
+		std::iota(b.out.triangleIndices.begin(), b.out.triangleIndices.end(), 0);
+		for (const Triangle& t : mesh.triangles)
+		{
+			const AABBd box = triangleBox(t);
+			b.boxes.push_back(box);
+			// boxCentroid: (min + max) / 2.0 == 0.5 * (min + max)
+			b.centroids.push_back((box.lo + box.hi) * 0.5);
+		}
+		b.out.nodes.reserve(2 * ((n + kMeshLeaf - 1) / kMeshLeaf));
+		b.build(0, n);
+		return (std::move(b.out));
+	}
+
+	GpuCamera	makeCamera(const Camera& cam, std::uint32_t width, std::uint32_t height, double metersPerUnit)
+	{
+		// RayHelper.cpp:50-82
+		GpuCamera gc{};
+		const double W = width, H = height;
+		gc.inverseWidthMinusOne = W > 1.0 ? 1.0 / (W - 1.0) : 0.0;
+		gc.inverseHeightMinusOne = H > 1.0 ? 1.0 / (H - 1.0) : 0.0;
+		const double focus = cam.focusDistance / metersPerUnit;
+		double vw = focus * cam.sensorWidth / cam.focalLength;
+		double vh = focus * cam.sensorHeight / cam.focalLength;
+		const double imageAspect = W / H;
+		const double viewportAspect = vw / vh;
+		if (imageAspect > viewportAspect) vh = vw / imageAspect;
+		else if (imageAspect < viewportAspect) vw = vh * imageAspect;
+		const dvec3 w = normalizeD(cam.direction);
+		dvec3 up = cam.up;
+		if (!std::isfinite(up[0]) || !std::isfinite(up[1]) || !std::isfinite(up[2]) || dot(up, up) <= 1e-12)
+		{
+			up = {0.0, 1.0, 0.0};
+		}
+		up = normalizeD(up);
+		if (dot(cross(up, w), cross(up, w)) <= 1e-12) up = {0.0, 0.0, 1.0};
+		if (dot(cross(up, w), cross(up, w)) <= 1e-12) up = {1.0, 0.0, 0.0};
+		const dvec3 u = normalizeD(cross(up, w));
+		const dvec3 v = cross(w, u);
+		const double aperture = cam.pinhole ? 0.0 : cam.focalLength / cam.fNumber;
+		gc.lensRadius = aperture / (2.0 * metersPerUnit);
+		const dvec3 horizontal = u * vw;
+		const dvec3 vertical = v * vh;
+		const dvec3 llc = ((cam.position + (horizontal * (1.0 / 2.0))) + (vertical * (1.0 / 2.0))) + (w * focus);
+		for (int k = 0; k < 3; k++)
+		{
+			gc.position[k] = cam.position[k];
+			gc.u[k] = u[k];
+			gc.v[k] = v[k];
+			gc.horizontal[k] = horizontal[k];
+			gc.vertical[k] = vertical[k];
+			gc.lowerLeftCorner[k] = llc[k];
+		}
+		return (gc);
+	}
+
+	PackedScene	packScene(const Scene& scene)
+	{
+		PackedScene p;
+		const Settings& s = scene.settings;
+		if (s.adaptive)
+		{
+			throw std::runtime_error("Unsupported for the device path: adaptive sampling (set adaptive=0)");
+		}
+		if (s.denoise)
+		{
+			throw std::runtime_error("Unsupported for the device path: denoise (set denoise=0)");
+		}
+
+		// ---- materials -------------------------------------------------------------
+		for (const Material& m : scene.materials)
+		{
+			GpuMaterial g{};
+			g.kind = m.kind;
+			for (int i = 0; i < 3; i++)
+			{
+				g.color[i] = static_cast<float>(m.color[i]);
+				g.eta[i] = static_cast<float>(m.eta[i]);
+				g.k[i] = static_cast<float>(m.k[i]);
+				g.absorption[i] = static_cast<float>(m.absorption[i]);
+			}
+			// BSDF.cpp:54-58: alpha = max(1e-4, clamp01(r)^2)
+			const double r = std::isfinite(m.roughness) ? std::max(0.0, std::min(1.0, m.roughness)) : 0.0;
+			g.alpha = static_cast<float>(std::max(1e-4, r * r));
+			g.roughness = static_cast<float>(m.roughness);
+			g.ior = static_cast<float>(m.ior);
+			g.flags = (m.conductor ? GPU_MATF_CONDUCTOR : 0u)
+				| ((m.absorption[0] > 0.0 || m.absorption[1] > 0.0 || m.absorption[2] > 0.0) ? GPU_MATF_ABSORPTION : 0u);
+			p.materials.push_back(g);
+		}
+
+		// ---- meshes ------------------------------------------------------------------
+		std::vector<std::int32_t> meshRoot(scene.meshes.size()), meshTriBase(scene.meshes.size()), meshNormalBase(scene.meshes.size(), -1);
+		p.blas.reserve(scene.meshes.size());
+		for (std::size_t m = 0; m < scene.meshes.size(); m++)
+		{
+			const Mesh& mesh = scene.meshes[m];
+			p.blas.push_back(buildBlas(mesh));
+			const Blas& blas = p.blas.back();
+			const std::int32_t triBase = static_cast<std::int32_t>(p.meshTris.size());
+			meshTriBase[m] = triBase;
+			bool anyNormals = false;
+			for (const Triangle& t : mesh.triangles)
+			{
+				anyNormals = anyNormals || t.hasNormals;
+			}
+			if (anyNormals)
+			{
+				meshNormalBase[m] = static_cast<std::int32_t>(p.meshNormals.size() / 9);
+			}
+			for (std::size_t k = 0; k < blas.triangleIndices.size(); k++)
+			{
+				const Triangle& t = mesh.triangles[blas.triangleIndices[k]];
+				p.meshTris.push_back(packTriangle(t, static_cast<std::int32_t>(blas.triangleIndices[k])));
+				if (anyNormals)
+				{
+					const dvec3* ns[3] = {&t.n0, &t.n1, &t.n2};
+					for (int v = 0; v < 3; v++)
+						for (int i = 0; i < 3; i++)
+							p.meshNormals.push_back(t.hasNormals ? static_cast<float>((*ns[v])[i]) : 0.0f);
+				}
+			}
+			meshRoot[m] = packBlas(blas, triBase, p.blasNodes);
+			p.triangleCount += mesh.triangles.size();
+		}
+
+		// ---- top-level hittables -----------------------------------------------------
+		const std::size_t nh = scene.hittables.size();
+		std::vector<AABBd> boxes(nh);
+		std::vector<bool> bounded(nh, false);
+		for (std::size_t i = 0; i < nh; i++)
+		{
+			const Hittable& h = scene.hittables[i];
+			if (h.material >= scene.materials.size())
+			{
+				throw std::runtime_error("Hittable references an unknown material");
+			}
+			GpuHittable gh{};
+			gh.material = h.material;
+			switch (h.kind)
+			{
+				case HIT_TRIANGLE:
+				{
+					const Triangle& t = scene.triangles[h.index];
+					boxes[i] = triangleBox(t);
+					bounded[i] = true;
+					gh.kindIndex = (GPU_HIT_TRIANGLE << 30) | static_cast<std::uint32_t>(p.topTris.size());
+					p.topTris.push_back(packTriangle(t, static_cast<std::int32_t>(i)));
+					p.triangleCount++;
+					break;
+				}
+				case HIT_RECTANGLE:
+				{
+					const Rectangle& r = scene.rectangles[h.index];
+					bounded[i] = rectangleBox(r, boxes[i]);
+					dvec3 n{}, wa{}, ha{};
+					rectangleBasis(r, n, wa, ha);
+					GpuRect g{};
+					for (int k = 0; k < 3; k++)
+					{
+						g.position[k] = static_cast<float>(r.position[k]);
+						g.normal[k] = static_cast<float>(n[k]);
+						g.widthAxis[k] = static_cast<float>(wa[k]);
+						g.heightAxis[k] = static_cast<float>(ha[k]);
+					}
+					g.halfWidth = static_cast<float>(r.width / 2.0);
+					g.halfHeight = static_cast<float>(r.height / 2.0);
+					g.width = static_cast<float>(r.width);
+					g.height = static_cast<float>(r.height);
+					g.area = static_cast<float>(r.height * r.width);
+					gh.kindIndex = (GPU_HIT_RECTANGLE << 30) | static_cast<std::uint32_t>(p.rects.size());
+					p.rects.push_back(g);
+					break;
+				}
+				case HIT_INSTANCE:
+				{
+					const Instance& inst = scene.instances[h.index];
+					bounded[i] = instanceBox(inst, p.blas[inst.mesh], boxes[i]);
+					const InstanceXf xf(inst);
+					// world -> local: inverseRotate(p - position) / scale as a 3x4 matrix
+					auto inverseRotate = [&](dvec3 v) {
+						// MeshInstance.cpp:100-121
+						double x = v[0], y = v[1], z = v[2];
+						double nx = (x * xf.cz) + (y * xf.sz);
+						double ny = (-x * xf.sz) + (y * xf.cz);
+						x = nx; y = ny;
+						nx = (x * xf.cy) - (z * xf.sy);
+						double nz = (x * xf.sy) + (z * xf.cy);
+						x = nx; z = nz;
+						ny = (y * xf.cx) + (z * xf.sx);
+						nz = (-y * xf.sx) + (z * xf.cx);
+						return dvec3{x, ny, nz};
+					};
+					GpuInstance g{};
+					double cols[3][3];
+					for (int c = 0; c < 3; c++)
+					{
+						dvec3 e{}; e[c] = 1.0;
+						const dvec3 r = inverseRotate(e);
+						for (int k = 0; k < 3; k++) cols[c][k] = r[k] / inst.scale[k];
+					}
+					const dvec3 pr = inverseRotate(inst.position);
+					for (int row = 0; row < 3; row++)
+					{
+						for (int c = 0; c < 3; c++) g.worldToLocal[row * 4 + c] = static_cast<float>(cols[c][row]);
+						g.worldToLocal[row * 4 + 3] = static_cast<float>(-pr[row] / inst.scale[row]);
+					}
+					// normal to world: rotate(n / scale)
+					for (int c = 0; c < 3; c++)
+					{
+						dvec3 e{}; e[c] = 1.0 / inst.scale[c];
+						const dvec3 r = xf.rotate(e);
+						for (int row = 0; row < 3; row++) g.normalToWorld[row * 3 + c] = static_cast<float>(r[row]);
+					}
+					g.blasRoot = meshRoot[inst.mesh];
+					g.triBase = meshTriBase[inst.mesh];
+					g.normalBase = meshNormalBase[inst.mesh];
+					gh.kindIndex = (GPU_HIT_INSTANCE << 30) | static_cast<std::uint32_t>(p.instances.size());
+					p.instances.push_back(g);
+					break;
+				}
+			}
+			p.hittables.push_back(gh);
+		}
+
+		// ---- top-level BVH (Scene.cpp:655-688) ------------------------------------------
+		std::vector<std::uint32_t> items;
+		for (std::size_t i = 0; i < nh; i++)
+		{
+			if (bounded[i]) items.push_back(static_cast<std::uint32_t>(i));
+			else p.unbounded.push_back(static_cast<std::uint32_t>(i));
+		}
+		for (std::uint32_t u : p.unbounded)
+		{
+			if ((p.hittables[u].kindIndex >> 30) != GPU_HIT_RECTANGLE)
+			{
+				throw std::runtime_error("Unsupported for the device path: unbounded hittable");
+			}
+		}
+		p.params.tlasRoot = -1;
+		if (items.size() == 1)
+		{
+			BvhNode root{};
+			setChild(root, 0, &boxes[items[0]], 0, 1);
+			setChild(root, 1, nullptr, 0, -1);
+			p.tlasPrims.push_back(items[0]);
+			p.tlasNodes.push_back(root);
+			p.params.tlasRoot = 0;
+			p.tlasTopology = std::to_string(items[0]);
+		}
+		else if (items.size() > 1)
+		{
+			TlasBuilder tb{boxes, {}};
+			tb.build(items, 0, items.size(), true);
+			std::ostringstream os;
+			tb.dump(os, 0);
+			p.tlasTopology = os.str();
+			// internal reference nodes (two node children) -> device nodes; leaf nodes
+			// (hittable children only) -> primitive ranges
+			std::vector<std::int32_t> map(tb.nodes.size(), -1);
+			std::vector<std::int32_t> leafStart(tb.nodes.size(), -1);
+			std::int32_t next = 0;
+			for (std::size_t i = 0; i < tb.nodes.size(); i++)
+			{
+				const bool leaf = tb.nodes[i].children.empty() || tb.nodes[i].children[0] >= 0;
+				if (!leaf)
+				{
+					map[i] = next++;
+				}
+				else
+				{
+					leafStart[i] = static_cast<std::int32_t>(p.tlasPrims.size());
+					for (std::int64_t c : tb.nodes[i].children) p.tlasPrims.push_back(static_cast<std::uint32_t>(c));
+				}
+			}
+			if (map[0] < 0)
+			{
+				BvhNode root{};
+				setChild(root, 0, &tb.nodes[0].box, leafStart[0], static_cast<std::int32_t>(tb.nodes[0].children.size()));
+				setChild(root, 1, nullptr, 0, -1);
+				p.tlasNodes.push_back(root);
+			}
+			else
+			{
+				p.tlasNodes.resize(static_cast<std::size_t>(next));
+				for (std::size_t i = 0; i < tb.nodes.size(); i++)
+				{
+					if (map[i] < 0) continue;
+					BvhNode& d = p.tlasNodes[static_cast<std::size_t>(map[i])];
+					for (int s = 0; s < 2; s++)
+					{
+						const std::size_t c = static_cast<std::size_t>(~tb.nodes[i].children[s]);
+						if (map[c] >= 0) setChild(d, s, &tb.nodes[c].box, map[c], 0);
+						else setChild(d, s, &tb.nodes[c].box, leafStart[c], static_cast<std::int32_t>(tb.nodes[c].children.size()));
+					}
+				}
+			}
+			p.params.tlasRoot = 0;
+		}
+
+		// ---- lights (Scene.cpp:615-638) -----------------------------------------------------
+		double total = 0.0;
+		std::vector<double> weights;
+		for (std::size_t i = 0; i < nh; i++)
+		{
+			const Hittable& h = scene.hittables[i];
+			const Material& m = scene.materials[h.material];
+			if (m.kind != MAT_EMISSIVE)
+			{
+				continue;
+			}
+			GpuLight l{};
+			l.hittable = static_cast<std::uint32_t>(i);
+			l.material = h.material;
+			for (int k = 0; k < 3; k++) l.emitted[k] = static_cast<float>(m.color[k]);
+			const double lum = luminance(m.color);
+			double area = 0.0;
+			if (h.kind == HIT_TRIANGLE)
+			{
+				const Triangle& t = scene.triangles[h.index];
+				const dvec3 c = cross(t.v1 - t.v0, t.v2 - t.v0);
+				const double ls = dot(c, c);
+				area = ls <= 1e-24 ? 0.0 : std::sqrt(ls) / 2.0;
+				l.kind = GPU_HIT_TRIANGLE;
+				l.index = p.hittables[i].kindIndex & 0x3fffffffu;
+			}
+			else if (h.kind == HIT_RECTANGLE)
+			{
+				const Rectangle& r = scene.rectangles[h.index];
+				area = r.width * r.height;
+				l.kind = GPU_HIT_RECTANGLE;
+				l.index = p.hittables[i].kindIndex & 0x3fffffffu;
+			}
+			else
+			{
+				throw std::runtime_error("Unsupported for the device path: emissive mesh instance");
+			}
+			l.area = static_cast<float>(area);
+			double w = (area <= 0.0 || !std::isfinite(lum) || lum <= 0.0) ? 0.0 : area * lum;
+			if (!std::isfinite(w) || w <= 0.0) w = 0.0;
+			total += w;
+			weights.push_back(w);
+			p.lightCdf.push_back(total);
+			p.lights.push_back(l);
+		}
+		if (total <= 0.0)
+		{
+			p.lightCdf.clear();
+		}
+		for (std::size_t i = 0; i < p.lights.size(); i++)
+		{
+			// lightSelectionProbability (ColorHelper.cpp:176-199)
+			double prob;
+			if (p.lightCdf.empty())
+			{
+				prob = 1.0 / static_cast<double>(p.lights.size());
+			}
+			else
+			{
+				const double prev = i == 0 ? 0.0 : p.lightCdf[i - 1];
+				const double w = p.lightCdf[i] - prev;
+				prob = (w <= 0.0 || total <= 0.0) ? 0.0 : w / total;
+			}
+			p.lights[i].selectionProbability = static_cast<float>(prob);
+		}
+
+		// ---- environment (EnvironmentMap.cpp:627-671) ----------------------------------------
+		GpuParams& gp = p.params;
+		gp.hasEnvironment = scene.hasEnvironment ? 1u : 0u;
+		gp.envTotalWeight = 0.0;
+		if (scene.hasEnvironment)
+		{
+			const Environment& env = scene.environment;
+			const std::size_t W = env.width, H = env.height;
+			p.envPixels.resize(W * H * 4);
+			p.envPdf.assign(W * H, 0.0f);
+			p.envCdf.reserve(W * H);
+			std::vector<double> weightsE(W * H), solid(W * H);
+			double tw = 0.0;
+			const double deltaPhi = 2.0 * kPi / static_cast<double>(W);
+			for (std::size_t y = 0; y < H; y++)
+			{
+				const double t0 = kPi * static_cast<double>(y) / static_cast<double>(H);
+				const double t1 = kPi * static_cast<double>(y + 1) / static_cast<double>(H);
+				const double sa = deltaPhi * std::max(0.0, std::cos(t0) - std::cos(t1));
+				for (std::size_t x = 0; x < W; x++)
+				{
+					const std::size_t i = y * W + x;
+					const dvec3& c = env.pixels[i];
+					dvec3 sc;
+					for (int k = 0; k < 3; k++) sc[k] = std::isfinite(c[k]) ? std::max(0.0, c[k]) : 0.0;
+					const double lum = luminance(sc);
+					const double w = (std::isfinite(lum) && lum > 0.0) ? lum * sa : 0.0;
+					solid[i] = sa;
+					weightsE[i] = w;
+					tw += w;
+					p.envCdf.push_back(tw);
+					for (int k = 0; k < 3; k++) p.envPixels[i * 4 + k] = static_cast<float>(c[k]);
+					p.envPixels[i * 4 + 3] = 0.0f;
+				}
+			}
+			if (!std::isfinite(tw) || tw <= 0.0)
+			{
+				tw = 0.0;
+				p.envCdf.clear();
+			}
+			else
+			{
+				for (std::size_t i = 0; i < W * H; i++)
+				{
+					if (weightsE[i] > 0.0 && solid[i] > 0.0)
+					{
+						p.envPdf[i] = static_cast<float>((weightsE[i] / tw) / solid[i]);
+					}
+				}
+			}
+			gp.envTotalWeight = tw;
+			gp.envWidth = static_cast<std::uint32_t>(W);
+			gp.envHeight = static_cast<std::uint32_t>(H);
+			gp.envStrength = static_cast<float>(env.strength);
+			gp.envRotation = static_cast<float>(env.rotation);
+			gp.envLighting = env.lighting ? 1u : 0u;
+		}
+
+		gp.camera = makeCamera(scene.camera, s.width, s.height, s.metersPerUnit);
+		p.camera = scene.camera;
+		p.metersPerUnit = s.metersPerUnit;
+		gp.width = s.width;
+		gp.height = s.height;
+		gp.maxBounces = s.maxBounces;
+		gp.totalSamples = s.samples;
+		gp.sky = s.sky == SKY_ENVIRONMENT ? GPU_SKY_ENVIRONMENT : GPU_SKY_NONE;
+		for (int k = 0; k < 3; k++) gp.background[k] = static_cast<float>(s.background[k]);
+		gp.backgroundLuminance = static_cast<float>(luminance(s.background));
+		gp.lightCount = static_cast<std::uint32_t>(p.lights.size());
+		gp.metersPerUnit = static_cast<float>(s.metersPerUnit);
+		gp.unboundedCount = static_cast<std::uint32_t>(p.unbounded.size());
+		return (p);
+	}
+}

@@ -1,0 +1,67 @@
+// Host-side acceleration-structure build and device packing.
+#pragma once
+
+#include "scene.hpp"
+#include "../common/layout.h"
+
+#include <string>
+#include <vector>
+
+namespace luzgpu
+{
+	struct	AABBd
+	{
+		dvec3	lo{};
+		dvec3	hi{};
+	};
+
+	// One node of a mesh BVH exactly as the reference's Mesh::PackedBVHNode (Mesh.hpp:37-45).
+	struct	BlasNode
+	{
+		AABBd	box;
+		std::size_t	left = 0, right = 0, start = 0, count = 0;
+		bool	leaf = false;
+	};
+
+	struct	Blas
+	{
+		std::vector<BlasNode>	nodes;
+		std::vector<std::size_t>	triangleIndices;
+	};
+
+	Blas	buildBlas(const Mesh& mesh);
+
+	struct	PackedScene
+	{
+		std::vector<BvhNode>	tlasNodes;
+		std::vector<BvhNode>	blasNodes;
+		std::vector<GpuTri>	topTris;
+		std::vector<GpuTri>	meshTris;
+		std::vector<float>	meshNormals;	// 9 floats per mesh triangle slot (leaf order)
+		std::vector<std::uint32_t>	tlasPrims;
+		std::vector<GpuHittable>	hittables;
+		std::vector<GpuRect>	rects;
+		std::vector<GpuInstance>	instances;
+		std::vector<GpuMaterial>	materials;
+		std::vector<GpuLight>	lights;
+		std::vector<double>	lightCdf;
+		std::vector<float>	envPixels;	// float4 per texel
+		std::vector<float>	envPdf;		// (weight / total) / solid angle per texel
+		std::vector<double>	envCdf;
+		std::vector<std::uint32_t>	unbounded;
+		GpuParams	params{};
+		std::string	tlasTopology;	// nested "(...)" form over top-level hittable indices
+		std::vector<Blas>	blas;		// reference-shaped mesh BVHs, for topology checks
+		std::size_t	triangleCount = 0;
+		Camera	camera;
+		double	metersPerUnit = 1.0;
+	};
+
+	// RenderCamera for an image of width x height (RayHelper.cpp:50-82).
+	GpuCamera	makeCamera(const Camera& cam, std::uint32_t width, std::uint32_t height, double metersPerUnit);
+
+	// Builds the TLAS/BLAS exactly as the reference does (BVHNode.cpp:142-185,
+	// Mesh.cpp:217-521), repacks them into the 64-byte device layout and derives
+	// light and environment sampling tables. Throws on unsupported content.
+	PackedScene	packScene(const Scene& scene);
+}

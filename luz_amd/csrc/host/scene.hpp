@@ -1,0 +1,158 @@
+// Host-side scene model: the flattened form of the reference's polymorphic Scene
+// (include/luz/Scene/Scene.hpp:26-182) that crosses into the device library.
+// Everything here is FP64, as in the reference, so host-side derived data (BVH
+// topology, light CDFs, environment CDF) can be built bit-identically to it.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace luzgpu
+{
+	using dvec3 = std::array<double, 3>;
+
+	inline dvec3	operator+(const dvec3& a, const dvec3& b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+	inline dvec3	operator-(const dvec3& a, const dvec3& b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+	// Vector3 * double is computed as f * v (Vector3.hpp:82-89).
+	inline dvec3	operator*(const dvec3& a, double f) { return {f * a[0], f * a[1], f * a[2]}; }
+	inline double	dot(const dvec3& a, const dvec3& b) { return (a[0] * b[0]) + (a[1] * b[1]) + (a[2] * b[2]); }
+	inline dvec3	cross(const dvec3& a, const dvec3& b)
+	{
+		return {(a[1] * b[2]) - (a[2] * b[1]), (a[2] * b[0]) - (a[0] * b[2]), (a[0] * b[1]) - (a[1] * b[0])};
+	}
+
+	constexpr double	kTMin = 0.001;	// Defaults.hpp T_MIN
+	constexpr double	kPi = 3.14159265358979323846;
+
+	enum MaterialKind : std::uint32_t
+	{
+		MAT_LAMBERTIAN = 1,
+		MAT_METAL = 2,
+		MAT_DIELECTRIC = 3,
+		MAT_EMISSIVE = 4,
+	};
+
+	struct	Material
+	{
+		MaterialKind	kind = MAT_LAMBERTIAN;
+		dvec3	color = {0.6, 0.6, 0.6};	// albedo / Schlick f0 / dielectric tint / radiance
+		double	roughness = 0.0;
+		double	ior = 1.5;
+		bool	conductor = false;
+		dvec3	eta = {1.0, 1.0, 1.0};
+		dvec3	k = {0.0, 0.0, 0.0};
+		dvec3	absorption = {0.0, 0.0, 0.0};	// dielectric interior absorption, 1/m
+		std::string	name;
+	};
+
+	struct	Triangle
+	{
+		dvec3	v0, v1, v2;
+		dvec3	n0{}, n1{}, n2{};
+		bool	hasNormals = false;
+	};
+
+	struct	Mesh
+	{
+		std::vector<Triangle>	triangles;	// after the reference's zero-area removal
+		std::string	source;
+	};
+
+	enum HittableKind : std::uint32_t
+	{
+		HIT_TRIANGLE = 0,
+		HIT_RECTANGLE = 1,
+		HIT_INSTANCE = 2,
+	};
+
+	struct	Rectangle
+	{
+		dvec3	position{};
+		dvec3	orientation = {0.0, 0.0, 1.0};
+		double	width = 1.0;
+		double	height = 1.0;
+	};
+
+	struct	Instance
+	{
+		std::uint32_t	mesh = 0;
+		dvec3	position{};
+		dvec3	rotationDegrees{};
+		dvec3	scale = {1.0, 1.0, 1.0};
+	};
+
+	// One entry per reference top-level hittable, in Scene::getHittables() order.
+	struct	Hittable
+	{
+		HittableKind	kind = HIT_TRIANGLE;
+		std::uint32_t	index = 0;		// into triangles / rectangles / instances
+		std::uint32_t	material = 0;
+	};
+
+	enum SkyKind : std::uint32_t
+	{
+		SKY_NONE = 0,
+		SKY_ENVIRONMENT = 1,
+	};
+
+	struct	Camera
+	{
+		dvec3	position{};
+		dvec3	direction = {0.0, 0.0, -1.0};
+		dvec3	up = {0.0, 1.0, 0.0};
+		double	focalLength = 0.050;
+		double	sensorWidth = 0.036;
+		double	sensorHeight = 0.02025;
+		double	fNumber = 8.0;
+		bool	pinhole = false;
+		double	focusDistance = 10.0;
+	};
+
+	struct	Environment
+	{
+		std::uint32_t	width = 0;
+		std::uint32_t	height = 0;
+		std::vector<dvec3>	pixels;		// scene-linear ACEScg, row 0 = top (v = 1)
+		double	strength = 1.0;
+		double	rotation = 0.0;
+		bool	lighting = true;
+	};
+
+	struct	Settings
+	{
+		std::uint32_t	width = 1920;
+		std::uint32_t	height = 1080;
+		std::uint32_t	samples = 48;
+		std::int32_t	maxBounces = 6;
+		bool	adaptive = true;
+		bool	denoise = true;
+		SkyKind	sky = SKY_NONE;
+		bool	skySet = false;
+		dvec3	background{};
+		double	metersPerUnit = 1.0;
+	};
+
+	struct	Scene
+	{
+		Settings	settings;
+		Camera	camera;
+		bool	hasCamera = false;
+		std::vector<Material>	materials;
+		std::vector<Triangle>	triangles;
+		std::vector<Rectangle>	rectangles;
+		std::vector<Mesh>	meshes;
+		std::vector<Instance>	instances;
+		std::vector<Hittable>	hittables;
+		Environment	environment;
+		bool	hasEnvironment = false;
+	};
+
+	// Reads the subset of the reference's .luz format listed in scene_file.cpp.
+	// Throws std::runtime_error with the reference's style of message on bad input.
+	Scene	readSceneFile(const std::string& path);
+	Mesh	readObjMesh(const std::string& path);
+	Environment	readHDREnvironment(const std::string& path);
+	double	luminance(const dvec3& c);
+}

@@ -1,0 +1,977 @@
+// Scene-file front end: the subset of the reference's .luz / OBJ / Radiance-HDR readers
+// that the device path consumes. Semantics follow the reference loader so the same file
+// produces the same FP64 scene on both sides:
+//   sections and blank-line termination  src/io/SceneFile/SceneFile.cpp:256-321
+//   [settings] keys                       src/io/SceneFile/Settings.cpp:162-640
+//   material blocks                       src/io/SceneFile/Materials.cpp:26-70,760-870
+//   primitives / object blocks            src/io/SceneFile/Objects.cpp:357-406,800-880,1502-1620
+//   OBJ faces (fan triangulation)         src/io/OBJReader.cpp:576-641
+//   degenerate-triangle removal           src/io/OBJReader.cpp:418-445, Mesh.cpp:360-385
+//   RGBE decode + sRGB->ACEScg            src/core/EnvironmentMap.cpp:140-150,376-454
+// Unsupported keys raise an error naming the key (the reference is strict the same way).
+#include "scene.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+
+namespace luzgpu
+{
+	double	luminance(const dvec3& c)
+	{
+		// ColorManagement.cpp:313-320
+		return (c[0] * 0.2722287168 + c[1] * 0.6740817658 + c[2] * 0.0536895174);
+	}
+
+	namespace
+	{
+		std::string	trim(const std::string& s)
+		{
+			std::size_t a = 0, b = s.size();
+			while (a < b && std::isspace(static_cast<unsigned char>(s[a]))) a++;
+			while (b > a && std::isspace(static_cast<unsigned char>(s[b - 1]))) b--;
+			return (s.substr(a, b - a));
+		}
+
+		std::string	lower(std::string s)
+		{
+			for (char& c : s) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+			return (s);
+		}
+
+		double	parseDouble(const std::string& text, const std::string& what)
+		{
+			const std::string t = trim(text);
+			char* end = nullptr;
+			const double v = std::strtod(t.c_str(), &end);
+			if (t.empty() || end != t.c_str() + t.size() || !std::isfinite(v))
+			{
+				throw std::runtime_error("Invalid " + what + ": " + text);
+			}
+			return (v);
+		}
+
+		// "(x,y,z)" with strtod semantics (the reference reads these with sscanf %lf).
+		dvec3	parseTriple(const std::string& text, const std::string& what)
+		{
+			std::string t = trim(text);
+			std::string lt = lower(t);
+			if (lt.rfind("acescg(", 0) == 0)
+			{
+				t = t.substr(6);
+			}
+			if (t.size() < 2 || t.front() != '(' || t.back() != ')')
+			{
+				throw std::runtime_error("Unsupported " + what + " value (only (x,y,z) / acescg(...) are supported): " + text);
+			}
+			const std::string body = t.substr(1, t.size() - 2);
+			dvec3 out{};
+			std::size_t start = 0;
+			for (int i = 0; i < 3; i++)
+			{
+				const std::size_t comma = body.find(',', start);
+				if ((i < 2) != (comma != std::string::npos))
+				{
+					throw std::runtime_error("Invalid " + what + ": " + text);
+				}
+				out[i] = parseDouble(body.substr(start, comma == std::string::npos ? std::string::npos : comma - start), what);
+				start = comma + 1;
+			}
+			return (out);
+		}
+
+		bool	splitAssignment(const std::string& line, std::string& key, std::string& value)
+		{
+			const std::size_t eq = line.find('=');
+			if (eq == std::string::npos)
+			{
+				return (false);
+			}
+			key = lower(trim(line.substr(0, eq)));
+			value = trim(line.substr(eq + 1));
+			return (true);
+		}
+
+		bool	namedBlock(const std::string& line, const std::string& keyword, std::string& name)
+		{
+			const std::string t = trim(line);
+			if (t.rfind(keyword + " ", 0) != 0 || t.back() != '{')
+			{
+				return (false);
+			}
+			name = trim(t.substr(keyword.size(), t.size() - keyword.size() - 1));
+			return (!name.empty());
+		}
+
+		std::string	resolvePath(const std::filesystem::path& base, const std::string& file)
+		{
+			std::filesystem::path p(file);
+			if (p.is_absolute())
+			{
+				return (p.string());
+			}
+			return ((base / p).string());
+		}
+
+		// Triangle area as in Triangle::_updateCache (Triangle.cpp:94-112).
+		double	triangleArea(const Triangle& t)
+		{
+			const dvec3 n = cross(t.v1 - t.v0, t.v2 - t.v0);
+			const double ls = dot(n, n);
+			if (ls <= 1e-24)
+			{
+				return (0.0);
+			}
+			return (std::sqrt(ls) / 2.0);
+		}
+
+		dvec3	normalizedOrZero(const dvec3& n)
+		{
+			// OBJReader.cpp:57-70 (identity transform branch)
+			const double ls = dot(n, n);
+			if (ls <= 1e-24)
+			{
+				return {0.0, 0.0, 0.0};
+			}
+			if (std::fabs(ls - 1.0) <= 1e-12)
+			{
+				return (n);
+			}
+			const double len = std::sqrt(ls);
+			return (n * (1.0 / len));
+		}
+
+		struct	Reader
+		{
+			std::ifstream	in;
+			std::filesystem::path	base;
+			Scene	scene;
+			std::map<std::string, std::uint32_t>	materialIndex;
+			std::map<std::string, std::uint32_t>	meshByPath;
+
+			bool	next(std::string& line)
+			{
+				if (!std::getline(in, line))
+				{
+					return (false);
+				}
+				if (!line.empty() && line.back() == '\r')
+				{
+					line.pop_back();
+				}
+				return (true);
+			}
+
+			void	settings()
+			{
+				std::string line;
+				Settings& s = scene.settings;
+				while (next(line))
+				{
+					if (line.empty())
+					{
+						break;
+					}
+					if (line[0] == '#')
+					{
+						continue;
+					}
+					std::string key, value;
+					if (!splitAssignment(line, key, value))
+					{
+						throw std::runtime_error("Invalid setting: " + line);
+					}
+					if (key == "resolution")
+					{
+						long long w, h;
+						if (std::sscanf(value.c_str(), "%lld,%lld", &w, &h) != 2 || w <= 0 || h <= 0)
+						{
+							throw std::runtime_error("Invalid resolution setting. Use resolution=WIDTH,HEIGHT.");
+						}
+						s.width = static_cast<std::uint32_t>(w);
+						s.height = static_cast<std::uint32_t>(h);
+					}
+					else if (key == "samples")
+					{
+						s.samples = static_cast<std::uint32_t>(std::max(0.0, parseDouble(value, "samples setting")));
+					}
+					else if (key == "maxlightbounces")
+					{
+						s.maxBounces = static_cast<std::int32_t>(parseDouble(value, "maxlightbounces setting"));
+					}
+					else if (key == "adaptive" || key == "adaptivesampling" || key == "adaptive_sampling")
+					{
+						s.adaptive = parseDouble(value, "adaptive setting") != 0.0;
+					}
+					else if (key == "denoise")
+					{
+						s.denoise = parseDouble(value, "denoise setting") != 0.0;
+					}
+					else if (key == "view_transform" || key == "outputfilename")
+					{
+						// display-only: the device path produces the raw scene-linear image
+					}
+					else if (key == "bloom")
+					{
+						if (parseDouble(value, "bloom setting") != 0.0)
+						{
+							throw std::runtime_error("Unsupported setting for the device path: bloom=1");
+						}
+					}
+					else if (key == "exposure")
+					{
+						if (parseDouble(value, "exposure setting") != 0.0)
+						{
+							throw std::runtime_error("Unsupported setting for the device path: exposure != 0");
+						}
+					}
+					else if (key == "sky")
+					{
+						const std::string v = lower(value);
+						if (v == "none")
+						{
+							s.sky = SKY_NONE;
+						}
+						else if (v == "environment")
+						{
+							s.sky = SKY_ENVIRONMENT;
+						}
+						else
+						{
+							throw std::runtime_error("Unsupported sky for the device path: " + value);
+						}
+						s.skySet = true;
+					}
+					else if (key == "background" || key == "backgroundcolor" || key == "background_color")
+					{
+						s.background = parseTriple(value, "background color");
+					}
+					else if (key == "environment" || key == "environmentmap" || key == "environment_map")
+					{
+						std::string path = value;
+						std::string rotation;
+						const std::size_t comma = value.find(',');
+						if (comma != std::string::npos)
+						{
+							path = trim(value.substr(0, comma));
+							rotation = trim(value.substr(comma + 1));
+						}
+						const double strength = scene.environment.strength;
+						const bool lighting = scene.environment.lighting;
+						scene.environment = readHDREnvironment(resolvePath(base, path));
+						scene.environment.strength = strength;
+						scene.environment.lighting = lighting;
+						if (!rotation.empty())
+						{
+							scene.environment.rotation = parseDouble(rotation, "environment rotation");
+						}
+						scene.hasEnvironment = true;
+						if (!s.skySet)
+						{
+							s.sky = SKY_ENVIRONMENT;
+						}
+					}
+					else if (key == "environment_scale" || key == "environmentscale")
+					{
+						scene.environment.strength = parseDouble(value, "environment scale");
+					}
+					else if (key == "environment_lighting" || key == "environmentlighting")
+					{
+						scene.environment.lighting = parseDouble(value, "environment_lighting") != 0.0;
+					}
+					else if (key == "environmentrotation" || key == "environment_rotation")
+					{
+						scene.environment.rotation = parseDouble(value, "environment rotation");
+					}
+					else if (key == "meters_per_unit" || key == "metersperunit")
+					{
+						s.metersPerUnit = parseDouble(value, "meters_per_unit");
+					}
+					else
+					{
+						throw std::runtime_error("Unsupported setting for the device path: " + line);
+					}
+				}
+			}
+
+			void	materialBlock(const std::string& name)
+			{
+				std::string line;
+				std::string type = "lambertian";
+				dvec3 color = {0.6, 0.6, 0.6};
+				double roughness = 0.0;
+				double fuzz = -1.0;
+				double ior = 1.5;
+				double radiance = -1.0;
+				double lum = -1.0;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "}")
+					{
+						Material m;
+						m.name = name;
+						if (type == "lambertian")
+						{
+							m.kind = MAT_LAMBERTIAN;
+							m.color = color;
+						}
+						else if (type == "metal")
+						{
+							m.kind = MAT_METAL;
+							m.color = color;
+							m.roughness = fuzz >= 0.0 ? fuzz : roughness;
+							if (!(m.roughness >= 0.0 && m.roughness <= 1.0))
+							{
+								throw std::runtime_error("Metal roughness must be finite and in [0,1].");
+							}
+						}
+						else if (type == "dielectric")
+						{
+							m.kind = MAT_DIELECTRIC;
+							m.color = color;
+							m.ior = ior;
+							m.roughness = roughness;
+							if (!(ior > 0.0))
+							{
+								throw std::runtime_error("Dielectric refractive index must be finite and positive.");
+							}
+						}
+						else if (type == "emissive")
+						{
+							// Emissive::fromRadiance / fromLuminance (LightUnits.cpp:36-73)
+							double target;
+							if (radiance >= 0.0)
+							{
+								target = radiance;
+							}
+							else if (lum >= 0.0)
+							{
+								target = lum / 683.0;
+							}
+							else
+							{
+								throw std::runtime_error("Emissive material requires radiance or luminance.");
+							}
+							m.kind = MAT_EMISSIVE;
+							if (target == 0.0)
+							{
+								m.color = {0.0, 0.0, 0.0};
+							}
+							else
+							{
+								const double cl = luminance(color);
+								if (!(cl > 0.0))
+								{
+									throw std::runtime_error("Light color must have positive luminance for physical units.");
+								}
+								const double f = target / cl;
+								m.color = {color[0] * f, color[1] * f, color[2] * f};
+							}
+						}
+						else
+						{
+							throw std::runtime_error("Unsupported material type for the device path: " + type);
+						}
+						if (materialIndex.count(name))
+						{
+							throw std::runtime_error("Duplicate material name: " + name);
+						}
+						materialIndex[name] = static_cast<std::uint32_t>(scene.materials.size());
+						scene.materials.push_back(m);
+						return;
+					}
+					std::string key, value;
+					if (!splitAssignment(t, key, value))
+					{
+						throw std::runtime_error("Unknown material property: " + line);
+					}
+					if (key == "type") type = lower(value);
+					else if (key == "color" || key == "basecolor" || key == "base_color") color = parseTriple(value, "material color");
+					else if (key == "roughness") roughness = parseDouble(value, "roughness");
+					else if (key == "fuzz") fuzz = parseDouble(value, "fuzz");
+					else if (key == "ior" || key == "refractiveindex" || key == "refractive_index") ior = parseDouble(value, "ior");
+					else if (key == "radiance" || key == "surfaceradiance" || key == "surface_radiance") radiance = parseDouble(value, "radiance");
+					else if (key == "luminance" || key == "nits" || key == "cd_m2") lum = parseDouble(value, "luminance");
+					else throw std::runtime_error("Unsupported material property for the device path: " + line);
+				}
+				throw std::runtime_error("Material block '" + name + "' is missing a closing }.");
+			}
+
+			void	materials()
+			{
+				std::string line;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty())
+					{
+						break;
+					}
+					if (t[0] == '#')
+					{
+						continue;
+					}
+					std::string name;
+					if (!namedBlock(t, "material", name))
+					{
+						throw std::runtime_error("Invalid material block header: " + line);
+					}
+					materialBlock(name);
+				}
+			}
+
+			std::uint32_t	material(const std::string& name)
+			{
+				const auto it = materialIndex.find(name);
+				if (it == materialIndex.end())
+				{
+					throw std::runtime_error("Unknown material: " + name);
+				}
+				return (it->second);
+			}
+
+			void	cameraBlock()
+			{
+				std::string line;
+				Camera& c = scene.camera;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "}")
+					{
+						scene.hasCamera = true;
+						return;
+					}
+					std::string key, value;
+					if (!splitAssignment(t, key, value))
+					{
+						throw std::runtime_error("Invalid camera property: " + line);
+					}
+					if (key == "position") c.position = parseTriple(value, "camera position");
+					else if (key == "direction") c.direction = parseTriple(value, "camera direction");
+					else if (key == "up") c.up = parseTriple(value, "camera up");
+					else if (key == "focal_length") c.focalLength = parseDouble(value, key);
+					else if (key == "focal_length_mm") c.focalLength = parseDouble(value, key) / 1000.0;
+					else if (key == "sensor_width") c.sensorWidth = parseDouble(value, key);
+					else if (key == "sensor_width_mm") c.sensorWidth = parseDouble(value, key) / 1000.0;
+					else if (key == "sensor_height") c.sensorHeight = parseDouble(value, key);
+					else if (key == "sensor_height_mm") c.sensorHeight = parseDouble(value, key) / 1000.0;
+					else if (key == "f_stop" || key == "fstop" || key == "f_number" || key == "fnumber")
+					{
+						c.fNumber = parseDouble(value, key);
+						c.pinhole = false;
+					}
+					else if (key == "pinhole") c.pinhole = parseDouble(value, key) != 0.0;
+					else if (key == "focus_distance") c.focusDistance = parseDouble(value, key);
+					else throw std::runtime_error("Unsupported camera property for the device path: " + line);
+				}
+				throw std::runtime_error("Camera block is missing a closing }.");
+			}
+
+			std::uint32_t	meshFor(const std::string& path)
+			{
+				const auto it = meshByPath.find(path);
+				if (it != meshByPath.end())
+				{
+					return (it->second);
+				}
+				const std::uint32_t id = static_cast<std::uint32_t>(scene.meshes.size());
+				scene.meshes.push_back(readObjMesh(path));
+				meshByPath[path] = id;
+				return (id);
+			}
+
+			void	addInstance(const std::string& path, const dvec3& position, const dvec3& rotation, const dvec3& scale, std::uint32_t mat)
+			{
+				// Objects.cpp:357-406: an invertible scale makes a MeshInstance over the
+				// per-file Mesh; a degenerate scale bakes a Mesh (not handled here).
+				if (!(std::fabs(scale[0]) > 1e-12 && std::fabs(scale[1]) > 1e-12 && std::fabs(scale[2]) > 1e-12))
+				{
+					throw std::runtime_error("Unsupported for the device path: object with a non-invertible scale");
+				}
+				Instance inst;
+				inst.mesh = meshFor(path);
+				inst.position = position;
+				inst.rotationDegrees = rotation;
+				inst.scale = scale;
+				scene.instances.push_back(inst);
+				scene.hittables.push_back(Hittable{HIT_INSTANCE, static_cast<std::uint32_t>(scene.instances.size() - 1), mat});
+			}
+
+			void	objectBlock(const std::string& name)
+			{
+				std::string line;
+				std::string file;
+				dvec3 position{}, rotation{}, scale = {1.0, 1.0, 1.0};
+				std::int64_t mat = -1;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "}")
+					{
+						if (file.empty())
+						{
+							throw std::runtime_error("Object block '" + name + "' ended before mesh or file was defined.");
+						}
+						if (mat < 0)
+						{
+							Material m;	// ObjectBlock default: Lambertian(0.6)
+							mat = static_cast<std::int64_t>(scene.materials.size());
+							scene.materials.push_back(m);
+						}
+						addInstance(resolvePath(base, file), position, rotation, scale, static_cast<std::uint32_t>(mat));
+						return;
+					}
+					std::string key, value;
+					if (!splitAssignment(t, key, value))
+					{
+						throw std::runtime_error("Invalid object property: " + t);
+					}
+					if (key == "file" || key == "path") file = value;
+					else if (key == "position") position = parseTriple(value, "object position");
+					else if (key == "rotation") rotation = parseTriple(value, "object rotation");
+					else if (key == "scale") scale = parseTriple(value, "object scale");
+					else if (key == "material") mat = material(value);
+					else throw std::runtime_error("Unsupported object property for the device path: " + t);
+				}
+				throw std::runtime_error("Object block '" + name + "' is missing a closing }.");
+			}
+
+			// "(a),(b),...,rest" -> fixed number of triples then the remainder
+			std::vector<std::string>	splitTopLevel(const std::string& s)
+			{
+				std::vector<std::string> parts;
+				int depth = 0;
+				std::string cur;
+				for (char ch : s)
+				{
+					if (ch == '(') depth++;
+					if (ch == ')') depth--;
+					if (ch == ',' && depth == 0)
+					{
+						parts.push_back(cur);
+						cur.clear();
+						continue;
+					}
+					cur.push_back(ch);
+				}
+				parts.push_back(cur);
+				return (parts);
+			}
+
+			std::uint32_t	primitiveMaterial(const std::string& part, const std::string& line)
+			{
+				std::string key, value;
+				if (!splitAssignment(part, key, value) || key != "material")
+				{
+					throw std::runtime_error("Unsupported primitive material syntax for the device path (use material=NAME): " + line);
+				}
+				return (material(value));
+			}
+
+			void	objects()
+			{
+				std::string line;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "}")
+					{
+						return;
+					}
+					const std::string lt = lower(t);
+					std::string name;
+					if (namedBlock(t, "object", name))
+					{
+						objectBlock(name);
+						continue;
+					}
+					if (lt.rfind("triangle=", 0) == 0)
+					{
+						const auto p = splitTopLevel(t.substr(9));
+						if (p.size() != 4)
+						{
+							throw std::runtime_error("Invalid triangle object: " + line);
+						}
+						Triangle tri;
+						tri.v0 = parseTriple(p[0], "triangle vertex");
+						tri.v1 = parseTriple(p[1], "triangle vertex");
+						tri.v2 = parseTriple(p[2], "triangle vertex");
+						scene.triangles.push_back(tri);
+						scene.hittables.push_back(Hittable{HIT_TRIANGLE, static_cast<std::uint32_t>(scene.triangles.size() - 1), primitiveMaterial(p[3], line)});
+						continue;
+					}
+					if (lt.rfind("rectangle=", 0) == 0)
+					{
+						const auto p = splitTopLevel(t.substr(10));
+						if (p.size() != 5)
+						{
+							throw std::runtime_error("Unsupported rectangle form for the device path: " + line);
+						}
+						Rectangle r;
+						r.position = parseTriple(p[0], "rectangle position");
+						r.orientation = parseTriple(p[1], "rectangle orientation");
+						r.width = parseDouble(p[2], "rectangle width");
+						r.height = parseDouble(p[3], "rectangle height");
+						scene.rectangles.push_back(r);
+						scene.hittables.push_back(Hittable{HIT_RECTANGLE, static_cast<std::uint32_t>(scene.rectangles.size() - 1), primitiveMaterial(p[4], line)});
+						continue;
+					}
+					if (lt.rfind("obj=", 0) == 0)
+					{
+						const auto p = splitTopLevel(t.substr(4));
+						if (p.size() == 1)
+						{
+							Material m;
+							scene.materials.push_back(m);
+							addInstance(resolvePath(base, trim(p[0])), dvec3{}, dvec3{}, dvec3{1.0, 1.0, 1.0},
+								static_cast<std::uint32_t>(scene.materials.size() - 1));
+							continue;
+						}
+						if (p.size() == 3)
+						{
+							addInstance(resolvePath(base, trim(p[0])), parseTriple(p[1], "obj position"), dvec3{},
+								dvec3{1.0, 1.0, 1.0}, primitiveMaterial(p[2], line));
+							continue;
+						}
+						throw std::runtime_error("Invalid OBJ object: " + line);
+					}
+					throw std::runtime_error("Unsupported object line for the device path: " + t);
+				}
+				throw std::runtime_error("Objects section is missing a closing }.");
+			}
+
+			void	sceneSection()
+			{
+				std::string line;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty())
+					{
+						break;
+					}
+					if (t[0] == '#')
+					{
+						continue;
+					}
+					std::string name;
+					if (namedBlock(t, "camera", name))
+					{
+						cameraBlock();
+					}
+					else if (namedBlock(t, "object", name))
+					{
+						objectBlock(name);
+					}
+					else if (lower(t) == "objects{" || lower(t) == "objects {")
+					{
+						objects();
+					}
+					else
+					{
+						throw std::runtime_error("Unsupported scene entry for the device path: " + t);
+					}
+				}
+			}
+
+			void	run()
+			{
+				std::string line;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "[settings]") settings();
+					else if (t == "[materials]") materials();
+					else if (t == "[scene]") sceneSection();
+					else throw std::runtime_error("Unsupported section for the device path: " + t);
+				}
+			}
+		};
+	}
+
+	Scene	readSceneFile(const std::string& path)
+	{
+		Reader r;
+		r.in.open(path);
+		if (!r.in)
+		{
+			throw std::runtime_error("Scene file could not be opened: " + path);
+		}
+		r.base = std::filesystem::path(path).parent_path();
+		r.run();
+		if (!r.scene.hasCamera)
+		{
+			throw std::runtime_error("No camera on the Scene.");
+		}
+		return (std::move(r.scene));
+	}
+
+	Mesh	readObjMesh(const std::string& path)
+	{
+		FILE* f = std::fopen(path.c_str(), "rb");
+		if (!f)
+		{
+			throw std::runtime_error("OBJ file could not be opened: " + path);
+		}
+		std::vector<dvec3> vertices, normals;
+		struct FV { int v, n; };
+		Mesh mesh;
+		mesh.source = path;
+		char buf[4096];
+		std::vector<FV> face;
+		auto resolve = [&](long idx, std::size_t count) -> int {
+			long r = idx > 0 ? idx - 1 : static_cast<long>(count) + idx;
+			if (idx == 0 || r < 0 || static_cast<std::size_t>(r) >= count)
+			{
+				throw std::runtime_error("OBJ index out of range in " + path);
+			}
+			return (static_cast<int>(r));
+		};
+		while (std::fgets(buf, sizeof(buf), f))
+		{
+			if (buf[0] == 'v' && buf[1] == ' ')
+			{
+				double x, y, z;
+				if (std::sscanf(buf + 2, "%lf %lf %lf", &x, &y, &z) != 3)
+				{
+					std::fclose(f);
+					throw std::runtime_error("Invalid OBJ vector line in " + path);
+				}
+				vertices.push_back({x, y, z});
+			}
+			else if (buf[0] == 'v' && buf[1] == 'n' && buf[2] == ' ')
+			{
+				double x, y, z;
+				if (std::sscanf(buf + 3, "%lf %lf %lf", &x, &y, &z) != 3)
+				{
+					std::fclose(f);
+					throw std::runtime_error("Invalid OBJ vector line in " + path);
+				}
+				normals.push_back({x, y, z});
+			}
+			else if (buf[0] == 'f' && buf[1] == ' ')
+			{
+				face.clear();
+				const char* c = buf + 2;
+				while (true)
+				{
+					while (std::isspace(static_cast<unsigned char>(*c))) c++;
+					if (*c == '\0') break;
+					char* end;
+					const long vi = std::strtol(c, &end, 10);
+					if (end == c)
+					{
+						std::fclose(f);
+						throw std::runtime_error("Invalid OBJ face vertex index in " + path);
+					}
+					c = end;
+					FV fv{resolve(vi, vertices.size()), -1};
+					if (*c == '/')
+					{
+						c++;
+						if (*c != '/' && !std::isspace(static_cast<unsigned char>(*c)) && *c)
+						{
+							std::strtol(c, &end, 10);	// texture index: unused on the device path
+							c = end;
+						}
+						if (*c == '/')
+						{
+							c++;
+							if (*c && !std::isspace(static_cast<unsigned char>(*c)))
+							{
+								fv.n = resolve(std::strtol(c, &end, 10), normals.size());
+								c = end;
+							}
+						}
+					}
+					face.push_back(fv);
+				}
+				if (face.size() < 3)
+				{
+					std::fclose(f);
+					throw std::runtime_error("OBJ face has fewer than three vertices in " + path);
+				}
+				for (std::size_t i = 1; i + 1 < face.size(); i++)
+				{
+					Triangle t;
+					t.v0 = vertices[face[0].v];
+					t.v1 = vertices[face[i].v];
+					t.v2 = vertices[face[i + 1].v];
+					if (face[0].n >= 0 && face[i].n >= 0 && face[i + 1].n >= 0)
+					{
+						t.n0 = normalizedOrZero(normals[face[0].n]);
+						t.n1 = normalizedOrZero(normals[face[i].n]);
+						t.n2 = normalizedOrZero(normals[face[i + 1].n]);
+						t.hasNormals = dot(t.n0, t.n0) > 1e-24 && dot(t.n1, t.n1) > 1e-24 && dot(t.n2, t.n2) > 1e-24;
+					}
+					// OBJReader.cpp:436 skips area <= 1e-24; Mesh.cpp:369-379 drops area <= 0.
+					if (triangleArea(t) > 1e-24)
+					{
+						mesh.triangles.push_back(t);
+					}
+				}
+			}
+		}
+		std::fclose(f);
+		return (mesh);
+	}
+
+	Environment	readHDREnvironment(const std::string& path)
+	{
+		std::ifstream in(path, std::ios::binary);
+		if (!in)
+		{
+			throw std::runtime_error("HDR environment map could not be opened: " + path);
+		}
+		std::string line;
+		std::getline(in, line);
+		if (line.rfind("#?RADIANCE", 0) != 0 && line.rfind("#?RGBE", 0) != 0)
+		{
+			throw std::runtime_error("Unsupported HDR environment header: " + path);
+		}
+		bool rgbe = false;
+		while (std::getline(in, line))
+		{
+			if (line.empty() || line == "\r")
+			{
+				break;
+			}
+			if (line.rfind("FORMAT=", 0) == 0 && line.find("32-bit_rle_rgbe") != std::string::npos)
+			{
+				rgbe = true;
+			}
+		}
+		if (!rgbe)
+		{
+			throw std::runtime_error("HDR environment map must use FORMAT=32-bit_rle_rgbe: " + path);
+		}
+		std::getline(in, line);
+		char ya[8], xa[8];
+		int h = 0, w = 0;
+		if (std::sscanf(line.c_str(), "%7s %d %7s %d", ya, &h, xa, &w) != 4 || std::string(ya) != "-Y" || std::string(xa) != "+X" || w <= 0 || h <= 0)
+		{
+			throw std::runtime_error("Unsupported HDR environment resolution line (only -Y H +X W): " + line);
+		}
+		static const double M[3][3] = {	// ColorManagement.cpp:121-125, linear sRGB -> ACEScg
+			{0.6131324224, 0.3395380158, 0.0474166960},
+			{0.0701243808, 0.9163940113, 0.0134515239},
+			{0.0205876575, 0.1095745716, 0.8697854040}
+		};
+		auto clampNonNegative = [](double v) { return (std::isfinite(v) && v > 0.0) ? v : 0.0; };
+		auto decode = [&](const unsigned char* p) -> dvec3 {
+			if (p[3] == 0)
+			{
+				return {0.0, 0.0, 0.0};
+			}
+			const double s = std::ldexp(1.0, static_cast<int>(p[3]) - (128 + 8));
+			const double r = clampNonNegative(p[0] * s), g = clampNonNegative(p[1] * s), b = clampNonNegative(p[2] * s);
+			dvec3 o;
+			for (int i = 0; i < 3; i++)
+			{
+				o[i] = clampNonNegative(M[i][0] * r + M[i][1] * g + M[i][2] * b);
+			}
+			return (o);
+		};
+		Environment env;
+		env.width = static_cast<std::uint32_t>(w);
+		env.height = static_cast<std::uint32_t>(h);
+		env.pixels.resize(static_cast<std::size_t>(w) * h);
+		std::vector<unsigned char> scan(static_cast<std::size_t>(w) * 4);
+		for (int y = 0; y < h; y++)
+		{
+			unsigned char head[4];
+			if (!in.read(reinterpret_cast<char*>(head), 4))
+			{
+				throw std::runtime_error("Truncated HDR environment scanline header.");
+			}
+			const int encoded = (head[2] << 8) | head[3];
+			if (w >= 8 && w <= 32767 && head[0] == 2 && head[1] == 2 && (head[2] & 0x80) == 0 && encoded == w)
+			{
+				for (int ch = 0; ch < 4; ch++)
+				{
+					int pos = 0;
+					while (pos < w)
+					{
+						unsigned char code;
+						if (!in.read(reinterpret_cast<char*>(&code), 1))
+						{
+							throw std::runtime_error("Truncated HDR environment RLE packet.");
+						}
+						if (code > 128)
+						{
+							const int count = code - 128;
+							unsigned char value;
+							if (pos + count > w || !in.read(reinterpret_cast<char*>(&value), 1))
+							{
+								throw std::runtime_error("Invalid HDR environment RLE run.");
+							}
+							for (int i = 0; i < count; i++) scan[(pos + i) * 4 + ch] = value;
+							pos += count;
+						}
+						else
+						{
+							const int count = code;
+							if (count == 0 || pos + count > w)
+							{
+								throw std::runtime_error("Invalid HDR environment RLE literal.");
+							}
+							for (int i = 0; i < count; i++)
+							{
+								unsigned char value;
+								if (!in.read(reinterpret_cast<char*>(&value), 1))
+								{
+									throw std::runtime_error("Truncated HDR environment RLE literal.");
+								}
+								scan[(pos + i) * 4 + ch] = value;
+							}
+							pos += count;
+						}
+					}
+				}
+			}
+			else
+			{
+				std::memcpy(scan.data(), head, 4);
+				if (!in.read(reinterpret_cast<char*>(scan.data() + 4), static_cast<std::streamsize>(w - 1) * 4))
+				{
+					throw std::runtime_error("Truncated HDR environment scanline.");
+				}
+			}
+			for (int x = 0; x < w; x++)
+			{
+				env.pixels[static_cast<std::size_t>(y) * w + x] = decode(&scan[x * 4]);
+			}
+		}
+		return (env);
+	}
+}

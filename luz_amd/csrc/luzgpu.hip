@@ -1,0 +1,659 @@
+// luzgpu: device memory management, kernels and the C ABI (include/luzgpu.h).
+//
+// Kernels
+//   render_mega      one thread per (pixel, sample): the whole path in registers, the
+//                    traversal stack in LDS. The plain single-scene kernel.
+//   render_wavefront stages (raygen / extend / shade / shadow) over SoA path state in
+//                    HBM with wave-level ballot compaction; see wavefront.hpp.
+//   resolve          per pixel: sum of the per-sample colours in sample order, divided
+//                    by the sample count, cleaned (Threads.cpp:563-583).
+#include <hip/hip_runtime.h>
+
+#include "../../include/luzgpu.h"
+#include "common/layout.h"
+#include "device/integrator.hpp"
+#include "device/wavefront.hpp"
+#include "host/accel.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace luzgpu
+{
+	namespace
+	{
+		thread_local std::string	g_lastError;
+
+		struct	Error
+		{
+			int	code;
+			std::string	message;
+		};
+
+		void	check(hipError_t e, const char* what)
+		{
+			if (e != hipSuccess)
+			{
+				throw Error{e == hipErrorOutOfMemory ? LUZGPU_E_OOM : LUZGPU_E_DEVICE,
+					std::string(what) + ": " + hipGetErrorString(e)};
+			}
+		}
+
+		struct	DeviceBuffer
+		{
+			void*	ptr = nullptr;
+			size_t	bytes = 0;
+
+			DeviceBuffer() = default;
+			DeviceBuffer(const DeviceBuffer&) = delete;
+			DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+			~DeviceBuffer() { release(); }
+
+			void	release()
+			{
+				if (ptr) (void)hipFree(ptr);
+				ptr = nullptr;
+				bytes = 0;
+			}
+
+			void	reserve(size_t n)
+			{
+				if (n <= bytes) return;
+				release();
+				check(hipMalloc(&ptr, n), "hipMalloc");
+				bytes = n;
+			}
+
+			template <typename T>
+			void	upload(const std::vector<T>& v, hipStream_t stream)
+			{
+				if (v.empty()) return;
+				reserve(v.size() * sizeof(T));
+				check(hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+			}
+
+			template <typename T> T*	as() const { return static_cast<T*>(ptr); }
+		};
+	}
+
+	// Scalars the kernels need beyond GpuParams.
+	struct	LaunchCounters
+	{
+		unsigned long long	closestRays;
+		unsigned long long	shadowRays;
+	};
+
+	constexpr int	kMegaBlock = 128;
+	constexpr int	kStackDepth = 32;
+
+	__global__ __launch_bounds__(kMegaBlock) void	render_mega(SceneView s, GpuParams p, float* __restrict__ sampleColors,
+		unsigned long long* __restrict__ counters)
+	{
+		__shared__ std::uint32_t stackMem[kStackDepth * kMegaBlock];
+		const std::uint32_t pixels = p.width * p.height;
+		const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * kMegaBlock + threadIdx.x;
+		std::uint32_t closest = 0, shadow = 0;
+		if (gid < static_cast<std::uint64_t>(pixels) * p.sampleCount)
+		{
+			const std::uint32_t slot = static_cast<std::uint32_t>(gid / pixels);
+			const std::uint32_t pixel = static_cast<std::uint32_t>(gid % pixels);
+			LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
+			const f3 c = cleanColor(tracePath(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest, shadow));
+			float* out = sampleColors + (static_cast<size_t>(slot) * pixels + pixel) * 3;
+			out[0] = c.x;
+			out[1] = c.y;
+			out[2] = c.z;
+		}
+		// one atomic per wave for the ray counters
+		for (int off = 32; off > 0; off >>= 1)
+		{
+			closest += __shfl_down(closest, off);
+			shadow += __shfl_down(shadow, off);
+		}
+		if ((threadIdx.x & 63) == 0)
+		{
+			atomicAdd(counters + 0, static_cast<unsigned long long>(closest));
+			atomicAdd(counters + 1, static_cast<unsigned long long>(shadow));
+		}
+	}
+
+	// Adds this launch's samples into the FP64 pixel sums, in sample order.
+	__global__ void	accumulate(const float* __restrict__ sampleColors, double* __restrict__ sums, std::uint32_t pixels, std::uint32_t count)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= pixels) return;
+		double r = sums[3 * i], g = sums[3 * i + 1], b = sums[3 * i + 2];
+		for (std::uint32_t k = 0; k < count; k++)
+		{
+			const float* c = sampleColors + (static_cast<size_t>(k) * pixels + i) * 3;
+			r += static_cast<double>(c[0]);
+			g += static_cast<double>(c[1]);
+			b += static_cast<double>(c[2]);
+		}
+		sums[3 * i] = r;
+		sums[3 * i + 1] = g;
+		sums[3 * i + 2] = b;
+	}
+
+	__global__ void	resolve(const double* __restrict__ sums, float* __restrict__ out, std::uint32_t pixels, double count)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= pixels) return;
+		for (int k = 0; k < 3; k++)
+		{
+			const double v = sums[3 * i + k] / count;
+			out[3 * i + k] = isfinite(v) ? static_cast<float>(v) : 0.0f;
+		}
+	}
+
+	__global__ void	zero_doubles(double* p, size_t n)
+	{
+		const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+		if (i < n) p[i] = 0.0;
+	}
+
+	__global__ void	zero_u64(unsigned long long* p, size_t n)
+	{
+		const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+		if (i < n) p[i] = 0ull;
+	}
+
+	__global__ void	primary_hits(SceneView s, GpuParams p, long long* ids, float* ts)
+	{
+		__shared__ std::uint32_t stackMem[kStackDepth * kMegaBlock];
+		const std::uint32_t pixels = p.width * p.height;
+		const std::uint32_t i = blockIdx.x * kMegaBlock + threadIdx.x;
+		if (i >= pixels) return;
+		PixelSampler ps;
+		ps.begin(p.renderSeed, i % p.width, i / p.width, 0, 0);
+		const Ray ray = cameraRay(p, i % p.width, i / p.width, ps);
+		LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
+		HitInfo h;
+		if (!traverse<false>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack))
+		{
+			ids[2 * i] = -1;
+			ids[2 * i + 1] = -1;
+			ts[i] = -1.0f;
+			return;
+		}
+		ids[2 * i] = h.hittable;
+		ids[2 * i + 1] = h.tri >= 0 ? s.meshTris[h.tri].prim : -1;
+		ts[i] = h.t;
+	}
+
+	__global__ void	sampler_eval(const std::uint32_t* keys, double* s1, float* s2, size_t n)
+	{
+		const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+		if (i >= n) return;
+		const std::uint32_t* k = keys + 7 * i;
+		PixelSampler ps;
+		ps.begin(k[0], k[1], k[2], k[3], k[4]);
+		ps.bounce = k[5];
+		s1[i] = ps.sample1D(k[6]);
+		ps.sample2D(k[6], s2[2 * i], s2[2 * i + 1]);
+	}
+
+	struct	GpuScene
+	{
+		PackedScene	host;
+		DeviceBuffer	tlas, blas, topTris, meshTris, meshNormals, tlasPrims, hittables, rects, instances, materials,
+			lights, lightCdf, envPixels, envPdf, envCdf, unbounded;
+		SceneView	view{};
+		luzgpu_scene_info	info{};
+	};
+}
+
+struct luzgpu_ctx
+{
+	int	device = 0;
+	hipStream_t	stream = nullptr;
+	luzgpu::DeviceBuffer	sampleColors;
+	luzgpu::DeviceBuffer	sums;
+	luzgpu::DeviceBuffer	counters;
+	luzgpu::DeviceBuffer	output;
+	luzgpu::DeviceBuffer	scratch;
+	luzgpu::WavefrontState	wave;
+};
+
+struct luzgpu_scene
+{
+	luzgpu_ctx*	ctx = nullptr;
+	luzgpu::GpuScene	gpu;
+};
+
+namespace luzgpu
+{
+	namespace
+	{
+		template <typename F>
+		int	guarded(F&& f)
+		{
+			try
+			{
+				f();
+				return LUZGPU_OK;
+			}
+			catch (const Error& e)
+			{
+				g_lastError = e.message;
+				return e.code;
+			}
+			catch (const std::bad_alloc&)
+			{
+				g_lastError = "host allocation failed";
+				return LUZGPU_E_OOM;
+			}
+			catch (const std::runtime_error& e)
+			{
+				g_lastError = e.what();
+				const std::string m = e.what();
+				if (m.find("Unsupported") != std::string::npos) return LUZGPU_E_UNSUPPORTED;
+				return LUZGPU_E_IO;
+			}
+			catch (const std::exception& e)
+			{
+				g_lastError = e.what();
+				return LUZGPU_E_INVALID;
+			}
+		}
+
+		int	depthOf(const std::vector<BlasNode>& n, std::size_t i)
+		{
+			if (n.empty() || n[i].leaf) return 0;
+			return 1 + std::max(depthOf(n, n[i].left), depthOf(n, n[i].right));
+		}
+
+		int	stringDepth(const std::string& s)
+		{
+			int d = 0, m = 0;
+			for (char c : s)
+			{
+				if (c == '(') m = std::max(m, ++d);
+				if (c == ')') d--;
+			}
+			return m;
+		}
+
+		void	uploadScene(luzgpu_ctx* ctx, GpuScene& g)
+		{
+			PackedScene& h = g.host;
+			hipStream_t st = ctx->stream;
+			g.tlas.upload(h.tlasNodes, st);
+			g.blas.upload(h.blasNodes, st);
+			g.topTris.upload(h.topTris, st);
+			g.meshTris.upload(h.meshTris, st);
+			g.meshNormals.upload(h.meshNormals, st);
+			g.tlasPrims.upload(h.tlasPrims, st);
+			g.hittables.upload(h.hittables, st);
+			g.rects.upload(h.rects, st);
+			g.instances.upload(h.instances, st);
+			g.materials.upload(h.materials, st);
+			g.lights.upload(h.lights, st);
+			g.lightCdf.upload(h.lightCdf, st);
+			g.envPixels.upload(h.envPixels, st);
+			g.envPdf.upload(h.envPdf, st);
+			g.envCdf.upload(h.envCdf, st);
+			g.unbounded.upload(h.unbounded, st);
+			check(hipStreamSynchronize(st), "scene upload");
+			SceneView& v = g.view;
+			v.tlas = g.tlas.as<BvhNode>();
+			v.blas = g.blas.as<BvhNode>();
+			v.topTris = g.topTris.as<GpuTri>();
+			v.meshTris = g.meshTris.as<GpuTri>();
+			v.meshNormals = g.meshNormals.as<float>();
+			v.tlasPrims = g.tlasPrims.as<std::uint32_t>();
+			v.hittables = g.hittables.as<GpuHittable>();
+			v.rects = g.rects.as<GpuRect>();
+			v.instances = g.instances.as<GpuInstance>();
+			v.materials = g.materials.as<GpuMaterial>();
+			v.lights = g.lights.as<GpuLight>();
+			v.lightCdf = h.lightCdf.empty() ? nullptr : g.lightCdf.as<double>();
+			v.envPixels = g.envPixels.as<float4>();
+			v.envPdf = g.envPdf.as<float>();
+			v.envCdf = g.envCdf.as<double>();
+			v.unbounded = g.unbounded.as<std::uint32_t>();
+		}
+
+		GpuParams	resolveParams(const GpuScene& g, const luzgpu_settings* s)
+		{
+			GpuParams p = g.host.params;
+			if (s)
+			{
+				p.renderSeed = s->seed;
+				if (s->samples >= 0) p.totalSamples = static_cast<std::uint32_t>(s->samples);
+				if (s->max_bounces >= 0) p.maxBounces = s->max_bounces;
+				p.sampleBegin = s->sample_begin;
+				p.sampleCount = s->sample_count >= 0 ? static_cast<std::uint32_t>(s->sample_count)
+					: (p.totalSamples > s->sample_begin ? p.totalSamples - s->sample_begin : 0u);
+				if (s->width > 0 || s->height > 0)
+				{
+					p.width = s->width > 0 ? static_cast<std::uint32_t>(s->width) : p.width;
+					p.height = s->height > 0 ? static_cast<std::uint32_t>(s->height) : p.height;
+					p.camera = makeCamera(g.host.camera, p.width, p.height, g.host.metersPerUnit);
+				}
+			}
+			else
+			{
+				p.renderSeed = 0x9e3779b9u;	// Sampler.cpp:28 default
+				p.sampleBegin = 0;
+				p.sampleCount = p.totalSamples;
+			}
+			return p;
+		}
+
+		unsigned	blocks(size_t n, unsigned b) { return static_cast<unsigned>((n + b - 1) / b); }
+	}
+}
+
+using namespace luzgpu;
+
+extern "C" {
+
+const char*	luzgpu_last_error(void)
+{
+	return g_lastError.c_str();
+}
+
+int	luzgpu_create(int device, luzgpu_ctx** out)
+{
+	if (!out)
+	{
+		g_lastError = "luzgpu_create: out is NULL";
+		return LUZGPU_E_INVALID;
+	}
+	*out = nullptr;
+	return guarded([&] {
+		int count = 0;
+		check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+		if (device < 0 || device >= count)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_create: device " + std::to_string(device) + " not present"};
+		}
+		check(hipSetDevice(device), "hipSetDevice");
+		auto ctx = std::make_unique<luzgpu_ctx>();
+		ctx->device = device;
+		check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+		*out = ctx.release();
+	});
+}
+
+void	luzgpu_destroy(luzgpu_ctx* ctx)
+{
+	if (!ctx) return;
+	(void)hipSetDevice(ctx->device);
+	(void)hipStreamSynchronize(ctx->stream);
+	ctx->wave.release();
+	ctx->sampleColors.release();
+	ctx->sums.release();
+	ctx->counters.release();
+	ctx->output.release();
+	ctx->scratch.release();
+	(void)hipStreamDestroy(ctx->stream);
+	delete ctx;
+}
+
+int	luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out)
+{
+	if (!path || !out)
+	{
+		g_lastError = "luzgpu_scene_load: NULL argument";
+		return LUZGPU_E_INVALID;
+	}
+	*out = nullptr;
+	return guarded([&] {
+		if (ctx)
+		{
+			check(hipSetDevice(ctx->device), "hipSetDevice");
+		}
+		auto sc = std::make_unique<luzgpu_scene>();
+		sc->ctx = ctx;
+		const Scene scene = readSceneFile(path);
+		sc->gpu.host = packScene(scene);
+		const PackedScene& h = sc->gpu.host;
+		int blasDepth = 0;
+		for (const Blas& b : h.blas) blasDepth = std::max(blasDepth, depthOf(b.nodes, 0));
+		const int tlasDepth = stringDepth(h.tlasTopology);
+		// a near-first DFS holds at most one deferred entry per level, plus the instance
+		// resume / exit markers
+		if (tlasDepth + blasDepth + 3 > kStackDepth)
+		{
+			throw std::runtime_error("Unsupported for the device path: BVH depth " + std::to_string(tlasDepth) + "+"
+				+ std::to_string(blasDepth) + " exceeds the traversal stack");
+		}
+		if (ctx)
+		{
+			uploadScene(ctx, sc->gpu);
+		}
+		luzgpu_scene_info& info = sc->gpu.info;
+		info.width = h.params.width;
+		info.height = h.params.height;
+		info.samples = h.params.totalSamples;
+		info.max_bounces = h.params.maxBounces;
+		info.hittables = static_cast<std::uint32_t>(h.hittables.size());
+		info.triangles = static_cast<std::uint32_t>(h.triangleCount);
+		info.meshes = static_cast<std::uint32_t>(h.blas.size());
+		info.instances = static_cast<std::uint32_t>(h.instances.size());
+		info.lights = static_cast<std::uint32_t>(h.lights.size());
+		info.tlas_nodes = static_cast<std::uint32_t>(h.tlasNodes.size());
+		info.blas_nodes = static_cast<std::uint32_t>(h.blasNodes.size());
+		info.env_width = h.params.envWidth;
+		info.env_height = h.params.envHeight;
+		info.tlas_depth = static_cast<std::uint32_t>(tlasDepth);
+		info.blas_depth = static_cast<std::uint32_t>(blasDepth);
+		*out = sc.release();
+	});
+}
+
+void	luzgpu_scene_release(luzgpu_scene* scene)
+{
+	if (!scene) return;
+	if (scene->ctx)
+	{
+		(void)hipSetDevice(scene->ctx->device);
+		(void)hipStreamSynchronize(scene->ctx->stream);
+	}
+	delete scene;
+}
+
+int	luzgpu_scene_get_info(const luzgpu_scene* scene, luzgpu_scene_info* out)
+{
+	if (!scene || !out)
+	{
+		g_lastError = "luzgpu_scene_get_info: NULL argument";
+		return LUZGPU_E_INVALID;
+	}
+	*out = scene->gpu.info;
+	return LUZGPU_OK;
+}
+
+int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings, float* rgb_out, luzgpu_stats* stats)
+{
+	if (!ctx || !scene || !rgb_out || scene->ctx != ctx)
+	{
+		g_lastError = "luzgpu_render: NULL argument or scene from another context";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		const GpuParams p = resolveParams(scene->gpu, settings);
+		if (p.sampleBegin + p.sampleCount > p.totalSamples || p.sampleCount == 0)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render: sample range outside [0, samples) or empty"};
+		}
+		if (p.maxBounces < 0)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render: negative max_bounces"};
+		}
+		const std::uint32_t integrator = settings ? settings->integrator : LUZGPU_INTEGRATOR_WAVEFRONT;
+		if (integrator > LUZGPU_INTEGRATOR_MEGAKERNEL)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render: unknown integrator"};
+		}
+		const std::uint32_t pixels = p.width * p.height;
+		hipStream_t st = ctx->stream;
+		ctx->sums.reserve(static_cast<size_t>(pixels) * 3 * sizeof(double));
+		ctx->counters.reserve(4 * sizeof(unsigned long long));
+		zero_doubles<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(ctx->sums.as<double>(), static_cast<size_t>(pixels) * 3);
+		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 4);
+		// samples per launch: bounded so the per-sample colour buffer stays ~1 GiB
+		const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount,
+			static_cast<std::uint32_t>((1ull << 30) / (static_cast<unsigned long long>(pixels) * 12ull))));
+		ctx->sampleColors.reserve(static_cast<size_t>(pixels) * perLaunch * 3 * sizeof(float));
+		hipEvent_t e0, e1;
+		check(hipEventCreate(&e0), "hipEventCreate");
+		check(hipEventCreate(&e1), "hipEventCreate");
+		check(hipEventRecord(e0, st), "hipEventRecord");
+		for (std::uint32_t done = 0; done < p.sampleCount; done += perLaunch)
+		{
+			GpuParams q = p;
+			q.sampleBegin = p.sampleBegin + done;
+			q.sampleCount = std::min(perLaunch, p.sampleCount - done);
+			if (integrator == LUZGPU_INTEGRATOR_MEGAKERNEL)
+			{
+				const size_t threads = static_cast<size_t>(pixels) * q.sampleCount;
+				render_mega<<<blocks(threads, kMegaBlock), kMegaBlock, 0, st>>>(scene->gpu.view, q, ctx->sampleColors.as<float>(),
+					ctx->counters.as<unsigned long long>());
+			}
+			else
+			{
+				renderWavefront(ctx->wave, scene->gpu.view, q, ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>(), st);
+			}
+			check(hipGetLastError(), "render launch");
+			accumulate<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sums.as<double>(), pixels, q.sampleCount);
+		}
+		check(hipEventRecord(e1, st), "hipEventRecord");
+		float* dst = rgb_out;
+		if (!(settings && settings->output_on_device))
+		{
+			ctx->output.reserve(static_cast<size_t>(pixels) * 3 * sizeof(float));
+			dst = ctx->output.as<float>();
+		}
+		resolve<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), dst, pixels, static_cast<double>(p.sampleCount));
+		check(hipGetLastError(), "resolve launch");
+		unsigned long long counters[2] = {0, 0};
+		check(hipMemcpyAsync(counters, ctx->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, st), "counter readback");
+		if (dst != rgb_out)
+		{
+			check(hipMemcpyAsync(rgb_out, dst, static_cast<size_t>(pixels) * 3 * sizeof(float), hipMemcpyDeviceToHost, st), "image readback");
+		}
+		check(hipStreamSynchronize(st), "render");
+		float ms = 0.0f;
+		check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		if (stats)
+		{
+			stats->rendered_samples = static_cast<std::uint64_t>(pixels) * p.sampleCount;
+			stats->closest_hit_rays = counters[0];
+			stats->shadow_rays = counters[1];
+			stats->render_ms = ms;
+		}
+	});
+}
+
+int	luzgpu_primary_hits(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings, int64_t* ids_out, float* t_out)
+{
+	if (!ctx || !scene || !ids_out || !t_out || scene->ctx != ctx)
+	{
+		g_lastError = "luzgpu_primary_hits: NULL argument";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		const GpuParams p = resolveParams(scene->gpu, settings);
+		const std::uint32_t pixels = p.width * p.height;
+		hipStream_t st = ctx->stream;
+		ctx->scratch.reserve(static_cast<size_t>(pixels) * (2 * sizeof(long long) + sizeof(float)));
+		long long* ids = ctx->scratch.as<long long>();
+		float* ts = reinterpret_cast<float*>(ids + 2 * static_cast<size_t>(pixels));
+		primary_hits<<<blocks(pixels, kMegaBlock), kMegaBlock, 0, st>>>(scene->gpu.view, p, ids, ts);
+		check(hipGetLastError(), "primary_hits launch");
+		check(hipMemcpyAsync(ids_out, ids, static_cast<size_t>(pixels) * 2 * sizeof(long long), hipMemcpyDeviceToHost, st), "readback");
+		check(hipMemcpyAsync(t_out, ts, static_cast<size_t>(pixels) * sizeof(float), hipMemcpyDeviceToHost, st), "readback");
+		check(hipStreamSynchronize(st), "primary_hits");
+	});
+}
+
+int	luzgpu_sampler_eval(luzgpu_ctx* ctx, size_t n, const uint32_t* keys, double* s1, float* s2)
+{
+	if (!ctx || (n && (!keys || !s1 || !s2)))
+	{
+		g_lastError = "luzgpu_sampler_eval: NULL argument";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		if (n == 0) return;
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		hipStream_t st = ctx->stream;
+		const size_t keyBytes = (n * 7 * sizeof(std::uint32_t) + 15) & ~size_t(15);
+		ctx->scratch.reserve(keyBytes + n * (sizeof(double) + 2 * sizeof(float)));
+		std::uint32_t* dk = ctx->scratch.as<std::uint32_t>();
+		double* d1 = reinterpret_cast<double*>(ctx->scratch.as<char>() + keyBytes);
+		float* d2 = reinterpret_cast<float*>(d1 + n);
+		check(hipMemcpyAsync(dk, keys, n * 7 * sizeof(std::uint32_t), hipMemcpyHostToDevice, st), "upload");
+		sampler_eval<<<blocks(n, 256), 256, 0, st>>>(dk, d1, d2, n);
+		check(hipGetLastError(), "sampler_eval launch");
+		check(hipMemcpyAsync(s1, d1, n * sizeof(double), hipMemcpyDeviceToHost, st), "readback");
+		check(hipMemcpyAsync(s2, d2, n * 2 * sizeof(float), hipMemcpyDeviceToHost, st), "readback");
+		check(hipStreamSynchronize(st), "sampler_eval");
+	});
+}
+
+int	luzgpu_scene_tlas_topology(const luzgpu_scene* scene, char* buf, size_t cap, size_t* needed)
+{
+	if (!scene)
+	{
+		g_lastError = "luzgpu_scene_tlas_topology: NULL scene";
+		return LUZGPU_E_INVALID;
+	}
+	const std::string& t = scene->gpu.host.tlasTopology;
+	if (needed) *needed = t.size() + 1;
+	if (buf && cap > 0)
+	{
+		const size_t n = std::min(cap - 1, t.size());
+		std::memcpy(buf, t.data(), n);
+		buf[n] = '\0';
+	}
+	return LUZGPU_OK;
+}
+
+int	luzgpu_scene_blas(const luzgpu_scene* scene, uint32_t mesh, double* boxes, int64_t* links, int64_t* tri_index,
+	uint64_t* node_count, uint64_t* tri_count)
+{
+	if (!scene || mesh >= scene->gpu.host.blas.size())
+	{
+		g_lastError = "luzgpu_scene_blas: bad scene or mesh index";
+		return LUZGPU_E_INVALID;
+	}
+	const Blas& b = scene->gpu.host.blas[mesh];
+	if (node_count) *node_count = b.nodes.size();
+	if (tri_count) *tri_count = b.triangleIndices.size();
+	for (size_t i = 0; i < b.nodes.size(); i++)
+	{
+		const BlasNode& n = b.nodes[i];
+		if (boxes)
+		{
+			for (int k = 0; k < 3; k++) { boxes[6 * i + k] = n.box.lo[k]; boxes[6 * i + 3 + k] = n.box.hi[k]; }
+		}
+		if (links)
+		{
+			links[5 * i] = static_cast<int64_t>(n.left);
+			links[5 * i + 1] = static_cast<int64_t>(n.right);
+			links[5 * i + 2] = static_cast<int64_t>(n.start);
+			links[5 * i + 3] = static_cast<int64_t>(n.count);
+			links[5 * i + 4] = n.leaf ? 1 : 0;
+		}
+	}
+	if (tri_index)
+	{
+		for (size_t i = 0; i < b.triangleIndices.size(); i++) tri_index[i] = static_cast<int64_t>(b.triangleIndices[i]);
+	}
+	return LUZGPU_OK;
+}
+
+}

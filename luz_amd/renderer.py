@@ -1,0 +1,169 @@
+"""Python mirror of the reference's entry points for the radiance path.
+
+``SceneFile.read(path)`` and ``Renderer.render(scene, ...)`` keep the reference's names
+(include/luz/SceneFile/SceneFile.hpp:8, include/luz/Renderer/Renderer.hpp:7) and the same
+error behaviour: bad files raise (the reference throws ``std::runtime_error``), a render
+returns the raw scene-linear image that ``_manageThreads`` leaves in ``Scene::getImage()``.
+Everything runs on the GPU through the C ABI; there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+class Context:
+    """One HIP device and stream (the reference's worker pool, Threads.cpp:375-548)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _lib.load()
+        self._h = ctypes.c_void_p()
+        _lib.check(self._lib.luzgpu_create(device, ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.luzgpu_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sampler_eval(self, keys: np.ndarray):
+        """keys: (n, 7) uint32 rows (seed, x, y, sample, stream, bounce, dim)."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        n = keys.shape[0]
+        s1 = np.zeros(n, dtype=np.float64)
+        s2 = np.zeros((n, 2), dtype=np.float32)
+        _lib.check(self._lib.luzgpu_sampler_eval(self._h, n, keys.ctypes.data, s1.ctypes.data, s2.ctypes.data))
+        return s1, s2
+
+
+@dataclass
+class RenderStats:
+    rendered_samples: int
+    closest_hit_rays: int
+    shadow_rays: int
+    render_ms: float
+
+
+class Scene:
+    """A scene uploaded to the device (SceneFile::read + Renderer.cpp:18-20 pre-render sync)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self._lib = _lib.load()
+        self._h = handle
+        info = _lib.SceneInfo()
+        _lib.check(self._lib.luzgpu_scene_get_info(self._h, ctypes.byref(info)))
+        self.info = {name: getattr(info, name) for name, _ in _lib.SceneInfo._fields_}
+
+    @property
+    def width(self) -> int:
+        return self.info["width"]
+
+    @property
+    def height(self) -> int:
+        return self.info["height"]
+
+    def release(self) -> None:
+        if self._h:
+            self._lib.luzgpu_scene_release(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def tlas_topology(self) -> str:
+        need = ctypes.c_size_t()
+        _lib.check(self._lib.luzgpu_scene_tlas_topology(self._h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _lib.check(self._lib.luzgpu_scene_tlas_topology(self._h, buf, need.value, None))
+        return buf.value.decode()
+
+    def blas(self, mesh: int):
+        nodes, tris = ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.check(self._lib.luzgpu_scene_blas(self._h, mesh, None, None, None, ctypes.byref(nodes), ctypes.byref(tris)))
+        boxes = np.zeros((nodes.value, 6), dtype=np.float64)
+        links = np.zeros((nodes.value, 5), dtype=np.int64)
+        idx = np.zeros(tris.value, dtype=np.int64)
+        _lib.check(self._lib.luzgpu_scene_blas(self._h, mesh, boxes.ctypes.data, links.ctypes.data, idx.ctypes.data,
+                                                   None, None))
+        return boxes, links, idx
+
+
+class SceneFile:
+    @staticmethod
+    def read(ctx, path: str) -> Scene:
+        """Loads ``path``; with ``ctx=None`` the scene is built on the host only (inspection)."""
+        h = ctypes.c_void_p()
+        lib = _lib.load()
+        _lib.check(lib.luzgpu_scene_load(ctx.handle if ctx is not None else None, path.encode(), ctypes.byref(h)))
+        return Scene(ctx, h)
+
+
+def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=False, resolution=None):
+    s = _lib.Settings()
+    if resolution is not None:
+        s.width, s.height = int(resolution[0]), int(resolution[1])
+    s.seed = int(seed) & 0xFFFFFFFF
+    s.samples = -1 if samples is None else int(samples)
+    s.max_bounces = -1 if max_bounces is None else int(max_bounces)
+    s.sample_begin = int(sample_begin)
+    s.sample_count = -1 if sample_count is None else int(sample_count)
+    s.integrator = {"wavefront": _lib.INTEGRATOR_WAVEFRONT, "megakernel": _lib.INTEGRATOR_MEGAKERNEL}[integrator]
+    s.output_on_device = 1 if on_device else 0
+    return s
+
+
+class Renderer:
+    """``Renderer::render`` for the device path."""
+
+    @staticmethod
+    def render(scene: Scene, seed: int, samples=None, max_bounces=None, sample_begin=0, sample_count=None,
+               integrator: str = "megakernel", out=None, resolution=None):
+        """Returns (image[H, W, 3] float32, RenderStats). ``out`` may be a device tensor
+        (torch, float32, contiguous, H*W*3) that receives the image without a host copy."""
+        s = _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=out is not None,
+                      resolution=resolution)
+        st = _lib.Stats()
+        w, h = resolution if resolution is not None else (scene.width, scene.height)
+        if out is not None:
+            ptr = out.data_ptr()
+            img = out
+        else:
+            img = np.zeros((h, w, 3), dtype=np.float32)
+            ptr = img.ctypes.data
+        _lib.check(scene.ctx._lib.luzgpu_render(scene.ctx.handle, scene._h, ctypes.byref(s), ctypes.c_void_p(ptr),
+                                                ctypes.byref(st)))
+        return img, RenderStats(st.rendered_samples, st.closest_hit_rays, st.shadow_rays, st.render_ms)
+
+    @staticmethod
+    def primary_hits(scene: Scene, seed: int, resolution=None):
+        s = _settings(seed, None, None, 0, None, "megakernel", resolution=resolution)
+        w, h = resolution if resolution is not None else (scene.width, scene.height)
+        ids = np.zeros((h, w, 2), dtype=np.int64)
+        t = np.zeros((h, w), dtype=np.float32)
+        _lib.check(scene.ctx._lib.luzgpu_primary_hits(scene.ctx.handle, scene._h, ctypes.byref(s), ids.ctypes.data,
+                                                      t.ctypes.data))
+        return ids, t

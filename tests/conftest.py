@@ -1,0 +1,45 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+@pytest.fixture(scope="session")
+def workload_dir(tmp_path_factory):
+    return str(tmp_path_factory.mktemp("workloads"))
+
+
+@pytest.fixture(scope="session")
+def cornell(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_cornell(os.path.join(workload_dir, "cornell"))
+
+
+@pytest.fixture(scope="session")
+def spheres(workload_dir):
+    from luz_amd import workloads
+    return workloads.spheres_scene(os.path.join(workload_dir, "spheres"))
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    from luz_amd import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def rel_mse(x, r):
+    """Relative MSE of an image against a reference: sum((x-r)^2) / sum(r^2)."""
+    import numpy as np
+    x = x.astype(np.float64)
+    r = r.astype(np.float64)
+    return float(((x - r) ** 2).sum() / max((r ** 2).sum(), 1e-300))

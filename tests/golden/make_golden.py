@@ -1,0 +1,132 @@
+"""Writes the committed golden fixtures from the reference renderer.
+
+Runs only in the container that holds the reference sources: it needs
+``oracle/_ref/luzref`` (built from /root/reference by ``make -C oracle ref``), which wraps
+the reference's own Sampler, BVH builders, intersection code and renderer
+(oracle/ref_harness.cpp). Output: small ``.npz`` / ``.txt`` files next to this script.
+
+    python tests/golden/make_golden.py
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from luz_amd import workloads  # noqa: E402
+
+LUZREF = os.path.join(ROOT, "oracle", "_ref", "luzref")
+
+
+def run(*args):
+    proc = subprocess.run([LUZREF, *map(str, args)], stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    if proc.returncode != 0:
+        raise RuntimeError(f"luzref {args[0]} failed: {proc.stderr.decode()[-2000:]}")
+    return proc.stdout.decode(errors="replace")
+
+
+def read_image(path):
+    data = open(path, "rb").read()
+    w, h = struct.unpack("<II", data[:8])
+    ms = struct.unpack("<d", data[8:16])[0]
+    return np.frombuffer(data[16:], dtype="<f8").reshape(h, w, 3), ms
+
+
+def sampler_fixture(tmp):
+    out = os.path.join(tmp, "sampler.bin")
+    run("sampler", out)
+    rec = np.dtype([("k", "<u4", 7), ("s1", "<f8"), ("s2", "<f4", 2)])
+    arr = np.fromfile(out, dtype=rec)
+    # deterministic subset: every 5th record plus the reference test's known case
+    keep = np.zeros(len(arr), dtype=bool)
+    keep[::5] = True
+    k = arr["k"]
+    known = (k[:, 0] == 1234) & (k[:, 1] == 13) & (k[:, 2] == 17) & (k[:, 3] == 5) & (k[:, 4] == 0) & (k[:, 5] == 0)
+    keep |= known
+    arr = arr[keep]
+    np.savez_compressed(os.path.join(HERE, "sampler.npz"), keys=arr["k"], s1=arr["s1"], s2=arr["s2"])
+    print("sampler records", len(arr))
+
+
+def read_primary(path):
+    data = open(path, "rb").read()
+    w, h = struct.unpack("<II", data[:8])
+    rec = np.dtype([("top", "<i8"), ("tri", "<i8"), ("t", "<f8"), ("margin", "<f8"), ("top2", "<i8"), ("tri2", "<i8"),
+                    ("t2", "<f8"), ("tref", "<f8"), ("refhit", "u1")])
+    return np.frombuffer(data[8:], dtype=rec).reshape(h, w)
+
+
+def primary_fixture(name, scene, seed, tmp, w=None, h=None):
+    out = os.path.join(tmp, f"{name}_primary.bin")
+    args = ["primary", scene, seed, out] + ([w, h] if w else [])
+    run(*args)
+    p = read_primary(out)
+    # the brute-force closest hit must agree with the reference's own traversal
+    assert np.all((p["top"] >= 0) == (p["refhit"] == 1))
+    hit = p["top"] >= 0
+    assert np.allclose(p["t"][hit], p["tref"][hit], rtol=1e-12, atol=0), "brute force disagrees with reference traversal"
+    # fp32 resolution: barycentric margin or a second candidate within 1e-5 relative
+    gap = np.where(np.isfinite(p["t2"]), (p["t2"] - p["t"]) / np.maximum(p["t"], 1e-300), np.inf)
+    near_tie = hit & ((p["margin"] < 1e-5) | (gap < 1e-5))
+    np.savez_compressed(os.path.join(HERE, f"{name}_primary.npz"), top=p["top"].astype(np.int32),
+                        tri=p["tri"].astype(np.int32), t=p["t"].astype(np.float32), near_tie=near_tie)
+    print(name, "primary near ties", int(near_tie.sum()), "of", near_tie.size)
+
+
+def main():
+    if not os.path.exists(LUZREF):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    tmp = tempfile.mkdtemp(prefix="luz_golden_")
+    sampler_fixture(tmp)
+
+    # Cornell box, 300x300, 50 spp, depth 8, seed 1 (the plain CPU-runnable config)
+    c = workloads.write_cornell(os.path.join(tmp, "cornell"))
+    out = os.path.join(tmp, "cornell.bin")
+    run("render", c.scene_path, 1, os.cpu_count() or 8, out)
+    img, ms = read_image(out)
+    np.savez_compressed(os.path.join(HERE, "cornell_300_50spp_seed1.npz"), image=img.astype(np.float32),
+                        render_ms=ms, threads=os.cpu_count() or 8)
+    with open(os.path.join(HERE, "cornell_tlas.txt"), "w") as fh:
+        tl = os.path.join(tmp, "c_tlas.txt")
+        run("tlas", c.scene_path, tl)
+        fh.write(open(tl).read())
+    primary_fixture("cornell", c.scene_path, 1, tmp)
+
+    # 512 displaced spheres: topology hashes, a reduced-resolution render, primary hits
+    s = workloads.spheres_scene(os.path.join(tmp, "spheres"))
+    tl = os.path.join(tmp, "s_tlas.txt")
+    run("tlas", s.scene_path, tl)
+    with open(os.path.join(HERE, "spheres_tlas.txt"), "w") as fh:
+        fh.write(open(tl).read())
+    digests = []
+    for i in range(s.extra["instances"]):
+        b = os.path.join(tmp, f"blas_{i}.bin")
+        run("blas", os.path.join(s.directory, f"sphere_{i:03d}.obj"), b)
+        raw = open(b, "rb").read()
+        n_nodes, n_tris = struct.unpack("<QQ", raw[:16])
+        # topology part only: node boxes + links + triangle order (vertices excluded)
+        topo = raw[16:16 + n_nodes * 88 + n_tris * 8]
+        digests.append(hashlib.sha256(topo).hexdigest())
+    with open(os.path.join(HERE, "spheres_blas_sha256.txt"), "w") as fh:
+        fh.write("\n".join(digests) + "\n")
+    out = os.path.join(tmp, "spheres_small.bin")
+    run("render", s.scene_path, 1, os.cpu_count() or 8, out, 16, 192, 108)
+    img, ms = read_image(out)
+    np.savez_compressed(os.path.join(HERE, "spheres_192x108_16spp_seed1.npz"), image=img.astype(np.float32),
+                        render_ms=ms, threads=os.cpu_count() or 8)
+    primary_fixture("spheres", s.scene_path, 1, tmp, 240, 135)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,64 @@
+"""Acceleration-structure topology against the reference (CPU: host-side builders only).
+
+Golden data: tests/golden/*_tlas.txt and spheres_blas_sha256.txt were written by the
+reference's own BVHNode / Mesh builders (tests/golden/make_golden.py)."""
+
+import hashlib
+import os
+
+import numpy as np
+
+from conftest import GOLDEN
+from luz_amd import SceneFile
+
+
+def test_cornell_tlas_matches_reference(cornell):
+    scene = SceneFile.read(None, cornell.scene_path)
+    assert scene.tlas_topology() == open(os.path.join(GOLDEN, "cornell_tlas.txt")).read().strip()
+    assert scene.info["hittables"] == 35 and scene.info["lights"] == 1
+
+
+def test_spheres_tlas_and_all_blas_match_reference(spheres):
+    scene = SceneFile.read(None, spheres.scene_path)
+    assert scene.tlas_topology() == open(os.path.join(GOLDEN, "spheres_tlas.txt")).read().strip()
+    digests = open(os.path.join(GOLDEN, "spheres_blas_sha256.txt")).read().split()
+    assert scene.info["meshes"] == len(digests) == 512
+    assert scene.info["triangles"] == 512 * 3840
+    for mesh in range(scene.info["meshes"]):
+        boxes, links, order = scene.blas(mesh)
+        blob = np.concatenate([boxes.view(np.uint8), links.view(np.uint8)], axis=1).tobytes() + order.tobytes()
+        assert hashlib.sha256(blob).hexdigest() == digests[mesh], f"mesh {mesh}"
+
+
+def test_blas_invariants(spheres):
+    """Each triangle exactly once; child boxes inside the parent; leaves hold <= 8."""
+    scene = SceneFile.read(None, spheres.scene_path)
+    boxes, links, order = scene.blas(0)
+    assert sorted(order.tolist()) == list(range(len(order)))
+    covered = np.zeros(len(order), dtype=int)
+    for k, (left, right, start, count, leaf) in enumerate(links):
+        if leaf:
+            assert 1 <= count <= 8
+            covered[start:start + count] += 1
+        else:
+            for c in (left, right):
+                assert np.all(boxes[c, :3] >= boxes[k, :3]) and np.all(boxes[c, 3:] <= boxes[k, 3:])
+    assert np.all(covered == 1)
+
+
+def test_unsupported_feature_is_reported(tmp_path):
+    import pytest
+    from luz_amd import _lib
+    p = tmp_path / "bad.luz"
+    p.write_text("[settings]\nsky=atmosphere\n\n[scene]\ncamera c {\nposition=(0,0,0)\n}\n")
+    with pytest.raises(_lib.LuzGpuError) as e:
+        SceneFile.read(None, str(p))
+    assert e.value.code == _lib.E_UNSUPPORTED
+
+
+def test_missing_file_is_io_error():
+    import pytest
+    from luz_amd import _lib
+    with pytest.raises(_lib.LuzGpuError) as e:
+        SceneFile.read(None, "/nonexistent/scene.luz")
+    assert e.value.code == _lib.E_IO
