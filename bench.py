@@ -1,0 +1,198 @@
+"""Throughput benchmark of the radiance path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload spheres|cornell]
+                    [--integrator megakernel|wavefront] [--no-cpu-baseline]
+
+A step is one full render of the workload at its configured resolution and spp
+(Renderer::render worker loop, Threads.cpp:375-548), inputs resident in HBM and the image
+left in HBM. Default workload: 512 displaced spheres of 3840 triangles (1,966,080
+triangles), 40/30/30 Lambertian / GGX metal / dielectric, procedural HDR sky, 1920x1080,
+64 spp, depth 12 (synthetic, luz_amd/workloads.py). Metric: Msamples/s =
+W*H*spp*steps / seconds (docker/benchmark/benchmark.sh:150-151), with Mrays/s beside it.
+
+The CPU baseline is the reference renderer itself (oracle/_ref, built from the reference
+sources by oracle/Makefile) on this host's cores for a bounded sample of the same
+workload (16 spp at full resolution), timed by its own render clock.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene_path: str, seed: int, spp: int, threads: int):
+    exe = os.path.join(ROOT, "oracle", "_ref", "luzref")
+    if not os.path.exists(exe):
+        return None
+    out = os.path.join(tempfile.mkdtemp(prefix="luz_cpu_"), "img.bin")
+    t0 = time.time()
+    proc = subprocess.run([exe, "render", scene_path, str(seed), str(threads), out, str(spp)],
+                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=900)
+    if proc.returncode != 0:
+        log("cpu baseline failed:", proc.stdout.decode(errors="replace")[-500:])
+        return None
+    data = open(out, "rb").read(16)
+    w, h = struct.unpack("<II", data[:8])
+    render_ms = struct.unpack("<d", data[8:16])[0]
+    log(f"cpu baseline: {w}x{h}x{spp} in {render_ms:.0f} ms render ({time.time() - t0:.1f} s wall incl. load)")
+    return {
+        "value": round(w * h * spp / (render_ms / 1000.0) / 1e6, 4),
+        "unit": "Msamples/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": f"reference Renderer::render of the same scene, {w}x{h} at {spp} spp (of the workload's full spp), "
+                  f"--threads {threads}, seed {seed}",
+    }
+
+
+def load_traffic(workload: str, kernel: str):
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path))
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="spheres", choices=["spheres", "cornell"])
+    ap.add_argument("--integrator", default="megakernel", choices=["megakernel", "wavefront"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-spp", type=int, default=16)
+    ap.add_argument("--probe-copy", action="store_true", help="run one plain 1 GiB copy kernel (PMC scale probe)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from luz_amd import Context, Renderer, SceneFile, workloads
+
+    wdir = os.path.join(tempfile.gettempdir(), "luz_bench_workloads")
+    t0 = time.time()
+    if args.workload == "spheres":
+        wl = workloads.spheres_scene(os.path.join(wdir, "spheres"))
+    else:
+        wl = workloads.write_cornell(os.path.join(wdir, "cornell"))
+    log(f"workload {wl.name}: {wl.width}x{wl.height} {wl.spp} spp depth {wl.depth}, {wl.triangles} triangles "
+        f"(generated in {time.time() - t0:.1f} s)")
+
+    if args.probe_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 1 GiB device-to-device copy done")
+        return
+
+    ctx = Context(0)
+    t0 = time.time()
+    scene = SceneFile.read(ctx, wl.scene_path)
+    log(f"scene upload {time.time() - t0:.1f} s: {scene.info}")
+    pixels = scene.width * scene.height
+    spp = scene.info["samples"]
+    out = torch.empty(pixels * 3, dtype=torch.float32, device="cuda")
+
+    # visit counts for the roofline (untimed, a 4-spp slice of the same workload)
+    _, cst = Renderer.render(scene, seed=args.seed, sample_count=min(4, spp), integrator=args.integrator, out=out,
+                             count_visits=args.integrator == "megakernel")
+    rays = cst.closest_hit_rays + cst.shadow_rays
+    nodes_per_sample = cst.nodes_visited / max(cst.rendered_samples, 1)
+    prims_per_sample = cst.prims_tested / max(cst.rendered_samples, 1)
+    log(f"visit counts: {nodes_per_sample:.1f} nodes, {prims_per_sample:.1f} primitives, "
+        f"{rays / max(cst.rendered_samples, 1):.2f} rays per sample")
+
+    for i in range(args.warmup):
+        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out)
+        log(f"warmup {i}: {st.render_ms:.1f} ms")
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out)
+        stats.append(st)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    for i, st in enumerate(stats):
+        log(f"step {i}: {st.render_ms:.1f} ms render, {st.kernel_ms:.1f} ms integrator kernel")
+    samples = pixels * spp * args.steps
+    value = samples / elapsed / 1e6
+    rays_total = sum(s.closest_hit_rays + s.shadow_rays for s in stats)
+    launches = sum(s.kernel_launches for s in stats)
+    kernel_s = sum(s.kernel_ms for s in stats) / 1000.0
+    samples_per_launch = samples / max(launches, 1)
+    # algorithmic bytes per sample: BVH nodes (64 B) and primitives (48 B) fetched on the
+    # reference-order traversal, plus the 12 B fp32 colour written per sample
+    bytes_per_sample = nodes_per_sample * 64.0 + prims_per_sample * 48.0 + 12.0
+    bytes_per_launch = bytes_per_sample * samples_per_launch
+    achieved = bytes_per_launch / (kernel_s / max(launches, 1)) / 1e9
+    kernel = "render_mega" if args.integrator == "megakernel" else "wavefront_extend"
+    traffic = load_traffic(wl.name, kernel)
+    line = {
+        "metric": "Msamples/s",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1000.0 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{wl.name}: {wl.triangles} triangles in {wl.extra.get('instances', 0)} instances, "
+                        f"{wl.width}x{wl.height}, {wl.spp} spp, depth {wl.depth}, seed {args.seed}",
+            "width": wl.width, "height": wl.height, "spp": wl.spp, "max_depth": wl.depth,
+            "triangles": wl.triangles, "integrator": args.integrator,
+        },
+        "mrays_per_s": round(rays_total / elapsed / 1e6, 3),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "kernel": kernel,
+            "kernel_ms_per_launch": round(kernel_s * 1000.0 / max(launches, 1), 3),
+            "algorithmic_bytes_per_launch": round(bytes_per_launch),
+        },
+        "cpu_baseline": None,
+    }
+    scene.release()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(wl.scene_path, args.seed, min(args.cpu_spp, wl.spp), threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

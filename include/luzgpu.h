@@ -53,6 +53,8 @@ typedef struct luzgpu_settings
 	uint32_t	output_on_device;/* 1: rgb_out is a device pointer on the context's device            */
 	int32_t		width;           /* <= 0: scene value (--resolution, FlagsParser.cpp)                 */
 	int32_t		height;          /* <= 0: scene value                                                 */
+	uint32_t	count_visits;    /* 1: also count BVH nodes / primitives visited (slower)             */
+	uint32_t	reserved;
 } luzgpu_settings;
 
 typedef struct luzgpu_stats
@@ -60,7 +62,12 @@ typedef struct luzgpu_stats
 	uint64_t	rendered_samples;    /* SceneRenderStats::renderedSamples (Scene.hpp:16-24) */
 	uint64_t	closest_hit_rays;
 	uint64_t	shadow_rays;
-	double		render_ms;           /* SceneRenderStats::renderMS                         */
+	double		render_ms;           /* SceneRenderStats::renderMS (all device work)       */
+	double		kernel_ms;           /* integrator kernels only (HIP events)               */
+	uint64_t	nodes_visited;       /* with count_visits                                  */
+	uint64_t	prims_tested;        /* with count_visits                                  */
+	uint32_t	kernel_launches;
+	uint32_t	reserved;
 } luzgpu_stats;
 
 typedef struct luzgpu_scene_info

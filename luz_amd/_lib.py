@@ -40,6 +40,8 @@ class Settings(ctypes.Structure):
         ("output_on_device", ctypes.c_uint32),
         ("width", ctypes.c_int32),
         ("height", ctypes.c_int32),
+        ("count_visits", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -49,6 +51,11 @@ class Stats(ctypes.Structure):
         ("closest_hit_rays", ctypes.c_uint64),
         ("shadow_rays", ctypes.c_uint64),
         ("render_ms", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("nodes_visited", ctypes.c_uint64),
+        ("prims_tested", ctypes.c_uint64),
+        ("kernel_launches", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

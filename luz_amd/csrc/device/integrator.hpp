@@ -160,9 +160,9 @@ namespace luzgpu
 	}
 
 	// Whole path in one thread: the plain (megakernel) integrator.
-	template <typename Stack>
+	template <bool Count, typename Stack>
 	__device__ f3	tracePath(const SceneView& s, const GpuParams& p, std::uint32_t x, std::uint32_t y, std::uint32_t sample,
-		Stack& stack, std::uint32_t& closestRays, std::uint32_t& shadowRays)
+		Stack& stack, std::uint32_t& closestRays, std::uint32_t& shadowRays, VisitCounter& visits)
 	{
 		PixelSampler ps;
 		ps.begin(p.renderSeed, x, y, sample, 0);
@@ -176,7 +176,7 @@ namespace luzgpu
 			ps.bounce = static_cast<std::uint32_t>(b);
 			HitInfo h;
 			closestRays++;
-			if (!traverse<false>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack))
+			if (!traverse<false, Count>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack, &visits))
 			{
 				float w = 1.0f;
 				if (b > 0 && !prevSpecular && prevInfinite)
@@ -228,7 +228,7 @@ namespace luzgpu
 					{
 						HitInfo sh;
 						shadowRays++;
-						occluded = traverse<true>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, c.tMax, sh, stack);
+						occluded = traverse<true, Count>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, c.tMax, sh, stack, &visits);
 					}
 					if (!occluded) acc = acc + clampRay(thr * c.contribution);
 				}
@@ -240,7 +240,7 @@ namespace luzgpu
 				{
 					HitInfo sh;
 					shadowRays++;
-					if (!traverse<true>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack))
+					if (!traverse<true, Count>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack, &visits))
 					{
 						acc = acc + clampRay(thr * c.contribution);
 					}

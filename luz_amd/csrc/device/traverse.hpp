@@ -183,9 +183,16 @@ namespace luzgpu
 		__device__ __forceinline__ void	reset() { sp = 0; }
 	};
 
-	template <bool AnyHit, typename Stack>
+	// Node / primitive visit counts (only maintained when Count is true).
+	struct	VisitCounter
+	{
+		std::uint32_t	nodes;
+		std::uint32_t	prims;
+	};
+
+	template <bool AnyHit, bool Count = false, typename Stack>
 	__device__ __forceinline__ bool	traverse(const SceneView& s, std::int32_t tlasRoot, const Ray& worldRay, float tMin, float tMax,
-		HitInfo& hit, Stack& stack)
+		HitInfo& hit, Stack& stack, VisitCounter* counter = nullptr)
 	{
 		hit.t = tMax;
 		hit.hittable = 0xffffffffu;
@@ -215,6 +222,7 @@ namespace luzgpu
 			else if ((entry & kLeafBit) == 0)
 			{
 				const BvhNode& n = nodes[entry];
+				if (Count) counter->nodes++;
 				float t0, t1;
 				const bool h0 = n.count[0] >= 0 && boxHit(ray, n.lo0, n.hi0, closest, t0);
 				const bool h1 = n.count[1] >= 0 && boxHit(ray, n.lo1, n.hi1, closest, t1);
@@ -240,6 +248,7 @@ namespace luzgpu
 				bool descended = false;
 				for (std::int32_t i = 0; i < count; i++)
 				{
+					if (Count) counter->prims++;
 					if (inInstance)
 					{
 						const GpuTri& tri = s.meshTris[start + i];

@@ -91,6 +91,8 @@ namespace luzgpu
 	constexpr int	kMegaBlock = 128;
 	constexpr int	kStackDepth = 32;
 
+	// counters: [0] closest-hit rays, [1] shadow rays, [2] nodes visited, [3] primitives tested
+	template <bool Count>
 	__global__ __launch_bounds__(kMegaBlock) void	render_mega(SceneView s, GpuParams p, float* __restrict__ sampleColors,
 		unsigned long long* __restrict__ counters)
 	{
@@ -98,12 +100,13 @@ namespace luzgpu
 		const std::uint32_t pixels = p.width * p.height;
 		const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * kMegaBlock + threadIdx.x;
 		std::uint32_t closest = 0, shadow = 0;
+		VisitCounter visits{0u, 0u};
 		if (gid < static_cast<std::uint64_t>(pixels) * p.sampleCount)
 		{
 			const std::uint32_t slot = static_cast<std::uint32_t>(gid / pixels);
 			const std::uint32_t pixel = static_cast<std::uint32_t>(gid % pixels);
 			LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
-			const f3 c = cleanColor(tracePath(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest, shadow));
+			const f3 c = cleanColor(tracePath<Count>(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest, shadow, visits));
 			float* out = sampleColors + (static_cast<size_t>(slot) * pixels + pixel) * 3;
 			out[0] = c.x;
 			out[1] = c.y;
@@ -114,11 +117,21 @@ namespace luzgpu
 		{
 			closest += __shfl_down(closest, off);
 			shadow += __shfl_down(shadow, off);
+			if (Count)
+			{
+				visits.nodes += __shfl_down(visits.nodes, off);
+				visits.prims += __shfl_down(visits.prims, off);
+			}
 		}
 		if ((threadIdx.x & 63) == 0)
 		{
 			atomicAdd(counters + 0, static_cast<unsigned long long>(closest));
 			atomicAdd(counters + 1, static_cast<unsigned long long>(shadow));
+			if (Count)
+			{
+				atomicAdd(counters + 2, static_cast<unsigned long long>(visits.nodes));
+				atomicAdd(counters + 3, static_cast<unsigned long long>(visits.prims));
+			}
 		}
 	}
 
@@ -500,30 +513,45 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		ctx->counters.reserve(4 * sizeof(unsigned long long));
 		zero_doubles<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(ctx->sums.as<double>(), static_cast<size_t>(pixels) * 3);
 		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 4);
-		// samples per launch: bounded so the per-sample colour buffer stays ~1 GiB
+		// samples per launch: bounded so the per-sample colour buffer stays within 4 GiB
 		const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount,
-			static_cast<std::uint32_t>((1ull << 30) / (static_cast<unsigned long long>(pixels) * 12ull))));
+			static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(pixels) * 12ull))));
 		ctx->sampleColors.reserve(static_cast<size_t>(pixels) * perLaunch * 3 * sizeof(float));
 		hipEvent_t e0, e1;
 		check(hipEventCreate(&e0), "hipEventCreate");
 		check(hipEventCreate(&e1), "hipEventCreate");
+		std::vector<hipEvent_t> ks, ke;
 		check(hipEventRecord(e0, st), "hipEventRecord");
 		for (std::uint32_t done = 0; done < p.sampleCount; done += perLaunch)
 		{
+			ks.emplace_back();
+			ke.emplace_back();
+			check(hipEventCreate(&ks.back()), "hipEventCreate");
+			check(hipEventCreate(&ke.back()), "hipEventCreate");
+			check(hipEventRecord(ks.back(), st), "hipEventRecord");
 			GpuParams q = p;
 			q.sampleBegin = p.sampleBegin + done;
 			q.sampleCount = std::min(perLaunch, p.sampleCount - done);
 			if (integrator == LUZGPU_INTEGRATOR_MEGAKERNEL)
 			{
 				const size_t threads = static_cast<size_t>(pixels) * q.sampleCount;
-				render_mega<<<blocks(threads, kMegaBlock), kMegaBlock, 0, st>>>(scene->gpu.view, q, ctx->sampleColors.as<float>(),
-					ctx->counters.as<unsigned long long>());
+				if (settings && settings->count_visits)
+				{
+					render_mega<true><<<blocks(threads, kMegaBlock), kMegaBlock, 0, st>>>(scene->gpu.view, q, ctx->sampleColors.as<float>(),
+						ctx->counters.as<unsigned long long>());
+				}
+				else
+				{
+					render_mega<false><<<blocks(threads, kMegaBlock), kMegaBlock, 0, st>>>(scene->gpu.view, q, ctx->sampleColors.as<float>(),
+						ctx->counters.as<unsigned long long>());
+				}
 			}
 			else
 			{
 				renderWavefront(ctx->wave, scene->gpu.view, q, ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>(), st);
 			}
 			check(hipGetLastError(), "render launch");
+			check(hipEventRecord(ke.back(), st), "hipEventRecord");
 			accumulate<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sums.as<double>(), pixels, q.sampleCount);
 		}
 		check(hipEventRecord(e1, st), "hipEventRecord");
@@ -535,7 +563,7 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		}
 		resolve<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), dst, pixels, static_cast<double>(p.sampleCount));
 		check(hipGetLastError(), "resolve launch");
-		unsigned long long counters[2] = {0, 0};
+		unsigned long long counters[4] = {0, 0, 0, 0};
 		check(hipMemcpyAsync(counters, ctx->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, st), "counter readback");
 		if (dst != rgb_out)
 		{
@@ -544,6 +572,15 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		check(hipStreamSynchronize(st), "render");
 		float ms = 0.0f;
 		check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+		double kernelMs = 0.0;
+		for (size_t i = 0; i < ks.size(); i++)
+		{
+			float k = 0.0f;
+			check(hipEventElapsedTime(&k, ks[i], ke[i]), "hipEventElapsedTime");
+			kernelMs += k;
+			(void)hipEventDestroy(ks[i]);
+			(void)hipEventDestroy(ke[i]);
+		}
 		(void)hipEventDestroy(e0);
 		(void)hipEventDestroy(e1);
 		if (stats)
@@ -552,6 +589,10 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			stats->closest_hit_rays = counters[0];
 			stats->shadow_rays = counters[1];
 			stats->render_ms = ms;
+			stats->kernel_ms = kernelMs;
+			stats->kernel_launches = static_cast<std::uint32_t>(ks.size());
+			stats->nodes_visited = counters[2];
+			stats->prims_tested = counters[3];
 		}
 	});
 }

@@ -62,6 +62,10 @@ class RenderStats:
     closest_hit_rays: int
     shadow_rays: int
     render_ms: float
+    kernel_ms: float
+    kernel_launches: int
+    nodes_visited: int
+    prims_tested: int
 
 
 class Scene:
@@ -122,8 +126,10 @@ class SceneFile:
         return Scene(ctx, h)
 
 
-def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=False, resolution=None):
+def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=False, resolution=None,
+              count_visits=False):
     s = _lib.Settings()
+    s.count_visits = 1 if count_visits else 0
     if resolution is not None:
         s.width, s.height = int(resolution[0]), int(resolution[1])
     s.seed = int(seed) & 0xFFFFFFFF
@@ -141,11 +147,11 @@ class Renderer:
 
     @staticmethod
     def render(scene: Scene, seed: int, samples=None, max_bounces=None, sample_begin=0, sample_count=None,
-               integrator: str = "megakernel", out=None, resolution=None):
+               integrator: str = "megakernel", out=None, resolution=None, count_visits=False):
         """Returns (image[H, W, 3] float32, RenderStats). ``out`` may be a device tensor
         (torch, float32, contiguous, H*W*3) that receives the image without a host copy."""
         s = _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=out is not None,
-                      resolution=resolution)
+                      resolution=resolution, count_visits=count_visits)
         st = _lib.Stats()
         w, h = resolution if resolution is not None else (scene.width, scene.height)
         if out is not None:
@@ -156,7 +162,8 @@ class Renderer:
             ptr = img.ctypes.data
         _lib.check(scene.ctx._lib.luzgpu_render(scene.ctx.handle, scene._h, ctypes.byref(s), ctypes.c_void_p(ptr),
                                                 ctypes.byref(st)))
-        return img, RenderStats(st.rendered_samples, st.closest_hit_rays, st.shadow_rays, st.render_ms)
+        return img, RenderStats(st.rendered_samples, st.closest_hit_rays, st.shadow_rays, st.render_ms, st.kernel_ms,
+                                st.kernel_launches, st.nodes_visited, st.prims_tested)
 
     @staticmethod
     def primary_hits(scene: Scene, seed: int, resolution=None):
