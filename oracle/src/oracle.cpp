@@ -1,0 +1,1595 @@
+// ORACLE — test infrastructure only. FP64 CPU restatement of the reference's radiance
+// path, used by tests/ (as the checker), by __graft_entry__.smoke() and as an alternative
+// CPU baseline. Nothing in the product links or calls it.
+//
+// Each function follows the reference file:line it cites, in the reference's own
+// operation order, so that on the same libm it reproduces the reference's FP64 values.
+// Scenes are read with the product's .luz reader (luz_amd/csrc/host/scene_file.cpp), whose
+// output is pinned to the reference separately (topology fixtures, golden images).
+//
+// Pinning (tests/test_oracle.py): sampler against the reference's golden vectors
+// (bit-exact); primary hits against the reference's brute-force IDs; Cornell and sphere
+// images against the reference's FP64 renders (golden .npz); closed forms (Beer–Lambert
+// exp(-2) style attenuation is not on this path; GGX/cosine pdf normalisation and the
+// conductor Fresnel normal-incidence value are checked by Monte Carlo / closed form).
+#include "../../luz_amd/csrc/host/scene.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace oracle
+{
+	using luzgpu::dvec3;
+	const double	T_MIN = 0.001;					// Defaults.hpp
+	const double	T_MAX = std::numeric_limits<double>::max();
+	const double	PI = 3.14159265358979323846;	// Defaults.hpp D_PI
+
+	// ---- value types (Vector3.hpp, Color.hpp) -------------------------------------------
+	struct	V
+	{
+		double	x = 0, y = 0, z = 0;
+	};
+	inline V	operator+(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+	inline V	operator-(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+	inline V	operator*(double f, V v) { return {f * v.x, f * v.y, f * v.z}; }		// Vector3.hpp:77-80
+	inline V	operator*(V v, double f) { return f * v; }							// Vector3.hpp:82-85
+	inline V	operator*(V a, V b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+	inline V	operator/(V v, double f) { return (1.0 / f) * v; }					// Vector3.hpp:92-95
+	inline double	dot(V a, V b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }	// Utilities.hpp:13-19
+	inline V	cross(V a, V b)
+	{
+		return {(a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x)};
+	}
+	inline double	len2(V a) { return (a.x * a.x) + (a.y * a.y) + (a.z * a.z); }
+	inline V	normalize(V a) { return a / std::sqrt(len2(a)); }					// Utilities.hpp:38-41
+	inline V	reflect(V v, V n) { return v - 2.0 * dot(v, n) * n; }					// Utilities.hpp:43-46
+	inline V	fromD(const dvec3& d) { return {d[0], d[1], d[2]}; }
+	inline double	at(V v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+	struct	C	// Color: division divides directly (Color.hpp:68-71)
+	{
+		double	r = 0, g = 0, b = 0;
+	};
+	inline C	operator+(C a, C b) { return {a.r + b.r, a.g + b.g, a.b + b.b}; }
+	inline C	operator*(C a, double f) { return {a.r * f, a.g * f, a.b * f}; }
+	inline C	operator*(C a, C b) { return {a.r * b.r, a.g * b.g, a.b * b.b}; }
+	inline C	operator/(C a, double f) { return {a.r / f, a.g / f, a.b / f}; }
+	inline double	maxChannel(C c) { return std::max(c.r, std::max(c.g, c.b)); }
+	inline double	luminance(C c) { return c.r * 0.2722287168 + c.g * 0.6740817658 + c.b * 0.0536895174; }
+	inline C	fromC(const dvec3& d) { return {d[0], d[1], d[2]}; }
+
+	// ---- sampler (Sampler.cpp:28-237) --------------------------------------------------------
+	inline std::uint64_t	splitMix64(std::uint64_t v)
+	{
+		v += 0x9e3779b97f4a7c15ull;
+		v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ull;
+		v = (v ^ (v >> 27)) * 0x94d049bb133111ebull;
+		return v ^ (v >> 31);
+	}
+	inline double	unit32(std::uint32_t h) { return static_cast<double>(h >> 8) * (1.0 / 16777216.0); }
+	inline double	unit64(std::uint64_t h) { return static_cast<double>(h >> 11) * (1.0 / 9007199254740992.0); }
+	inline double	fract(double v) { return v - std::floor(v); }
+
+	double	dimensionStep(std::uint32_t dim, std::uint32_t comp)	// Sampler.cpp:59-98
+	{
+		const std::uint64_t key = (static_cast<std::uint64_t>(dim) + 1ull) * 0x9e3779b97f4a7c15ull
+			^ (static_cast<std::uint64_t>(comp) + 1ull) * 0xda942042e4dd58b5ull;
+		double step = unit32(static_cast<std::uint32_t>(splitMix64(key) >> 32));
+		if (step < 0.05 || step > 0.95) step = fract(step + 0.6180339887498948);
+		return step;
+	}
+
+	struct	Sampler
+	{
+		std::uint64_t	baseKey = 0;
+		std::uint32_t	sample = 0, bounce = 0;
+
+		void	begin(std::uint32_t seed, std::size_t x, std::size_t y, std::uint32_t s, std::uint32_t stream)	// :161-173
+		{
+			baseKey = seed;
+			baseKey ^= (static_cast<std::uint64_t>(x) + 1ull) * 0xbf58476d1ce4e5b9ull;
+			baseKey ^= (static_cast<std::uint64_t>(y) + 1ull) * 0x94d049bb133111ebull;
+			baseKey ^= (static_cast<std::uint64_t>(stream) + 1ull) * 0xda942042e4dd58b5ull;
+			sample = s;
+			bounce = 0;
+		}
+		std::uint64_t	ctxKey(std::uint32_t dk) const { return baseKey ^ ((static_cast<std::uint64_t>(dk) + 1ull) * 0x9e3779b97f4a7c15ull); }
+		static bool	progressive(std::uint32_t d)	// :112-126
+		{
+			const std::uint32_t b = d % 32;
+			return b == 0 || b == 1 || b == 4 || b == 5 || b == 12 || b == 13 || b == 16;
+		}
+		double	hashed(std::uint32_t dk, std::uint32_t comp) const	// :128-135
+		{
+			std::uint64_t key = ctxKey(dk);
+			key ^= (static_cast<std::uint64_t>(sample) + 1ull) * 0x94d049bb133111ebull;
+			key ^= (static_cast<std::uint64_t>(comp) + 1ull) * 0xbf58476d1ce4e5b9ull;
+			return unit64(splitMix64(key));
+		}
+		double	s1(std::uint32_t dim) const	// :193-209
+		{
+			const std::uint32_t dk = dim + bounce * 32;
+			if (!progressive(dim)) return hashed(dk, 0);
+			const double shift = unit32(static_cast<std::uint32_t>(splitMix64(ctxKey(dk)) >> 32));
+			const double step = dimensionStep(dk, 0);
+			return fract(0.5 + (static_cast<double>(sample) + 1.0) * step + shift);
+		}
+		void	s2(std::uint32_t dim, float& x, float& y) const	// :211-237
+		{
+			const std::uint32_t dk = dim + bounce * 32;
+			if (!progressive(dim))
+			{
+				x = static_cast<float>(hashed(dk, 0));
+				y = static_cast<float>(hashed(dk, 1));
+				return;
+			}
+			const std::uint64_t key = ctxKey(dk);
+			const double sx = unit32(static_cast<std::uint32_t>(splitMix64(key) >> 32));
+			const double sy = unit32(static_cast<std::uint32_t>(splitMix64(key ^ 0xda942042e4dd58b5ull) >> 32));
+			const double stx = dimensionStep(dk, 0), sty = dimensionStep(dk, 1);
+			const double s = static_cast<double>(sample) + 1.0;
+			x = static_cast<float>(fract(0.5 + s * stx + sx));
+			y = static_cast<float>(fract(0.5 + s * sty + sy));
+		}
+	};
+
+	enum Dim : std::uint32_t
+	{
+		D_CAMERA = 0, D_LENS = 1, D_LIGHT_STRATEGY = 2, D_LIGHT_SELECTION = 3, D_LIGHT_SURFACE_POINT = 5,
+		D_BSDF_DIRECTION = 6, D_MATERIAL_DECISION = 7, D_RR = 9, D_ENV_SELECTION = 12, D_ENV_POINT = 13, D_ENV_STRATEGY = 14
+	};
+
+	// ---- rays (Ray.hpp) -----------------------------------------------------------------------
+	inline double	invComp(double c)
+	{
+		if (c == 0.0 || !std::isfinite(c)) return 0.0;
+		const double i = 1.0 / c;
+		return std::isfinite(i) ? i : 0.0;
+	}
+	struct	Ray
+	{
+		V	o, d, inv;
+		static Ray	normalized(V o, V d) { return Ray{o, d, {invComp(d.x), invComp(d.y), invComp(d.z)}}; }	// :23-26
+		static Ray	make(V o, V d)	// :18-21, _normalize :53-65
+		{
+			const double l = std::sqrt((d.x * d.x) + (d.y * d.y) + (d.z * d.z));
+			const V n = (!std::isfinite(l) || l <= 0.0) ? V{} : d / l;
+			return normalized(o, n);
+		}
+		V	at(double t) const { return o + (d * t); }
+	};
+
+	// ---- hit record (Hittable.hpp:12-36) ----------------------------------------------------
+	struct	Hit
+	{
+		double	t = 0;
+		V	p, n, gn;
+		bool	front = true;
+		std::uint32_t	material = 0;
+		std::int64_t	top = -1, tri = -1;
+	};
+	inline void	setFace(Hit&, const Ray& r, V outward, V& n, bool& front)
+	{
+		front = ((r.d.x * outward.x) + (r.d.y * outward.y) + (r.d.z * outward.z)) < 0.0;
+		n = front ? outward : outward * -1.0;
+	}
+
+	// ---- geometry ---------------------------------------------------------------------------------
+	struct	Tri	// Triangle.cpp:94-126
+	{
+		V	v0, v1, v2, e1, e2, face;
+		V	n0, n1, n2;
+		bool	hasN = false, degenerate = false;
+		double	area = 0;
+		V	lo, hi;
+		std::uint32_t	material = 0;
+
+		void	init()
+		{
+			e1 = v1 - v0;
+			e2 = v2 - v0;
+			const V g = cross(e1, e2);
+			const double ls = len2(g);
+			degenerate = ls <= 1e-24;
+			if (degenerate) { face = V{}; area = 0.0; }
+			else { const double l = std::sqrt(ls); face = g / l; area = l / 2.0; }
+			lo = {std::min(v0.x, std::min(v1.x, v2.x)) - T_MIN, std::min(v0.y, std::min(v1.y, v2.y)) - T_MIN,
+				std::min(v0.z, std::min(v1.z, v2.z)) - T_MIN};
+			hi = {std::max(v0.x, std::max(v1.x, v2.x)) + T_MIN, std::max(v0.y, std::max(v1.y, v2.y)) + T_MIN,
+				std::max(v0.z, std::max(v1.z, v2.z)) + T_MIN};
+		}
+
+		// Triangle::hit (Triangle.cpp:196-270): returns t, u, v; record built by caller
+		bool	intersect(const Ray& r, double tMin, double tMax, double& t, double& u, double& v) const
+		{
+			if (degenerate) return false;
+			const V p = cross(r.d, e2);
+			const double det = dot(e1, p);
+			if (std::fabs(det) < 1e-12) return false;
+			const double invDet = 1.0 / det;
+			const V tv = r.o - v0;
+			u = dot(tv, p) * invDet;
+			if (u < 0.0 || u > 1.0) return false;
+			const V q = cross(tv, e1);
+			v = dot(r.d, q) * invDet;
+			if (v < 0.0 || u + v > 1.0) return false;
+			t = dot(e2, q) * invDet;
+			return !(t > tMax || t < tMin);
+		}
+
+		void	record(const Ray& r, double t, double u, double v, Hit& h) const	// :241-270
+		{
+			h.t = t;
+			V n = face;
+			if (hasN)
+			{
+				const V in = (n0 * (1.0 - u - v)) + (n1 * u) + (n2 * v);
+				n = len2(in) <= 1e-24 ? face : normalize(in);
+			}
+			setFace(h, r, n, h.n, h.front);
+			bool f2;
+			setFace(h, r, face, h.gn, f2);
+			h.p = r.at(t);
+		}
+	};
+
+	struct	Rect	// Rectangle.cpp:14-50,212-259
+	{
+		V	pos, n, wa, ha;
+		double	w = 0, h = 0;
+		bool	ok = false;
+		std::uint32_t	material = 0;
+
+		bool	intersect(const Ray& r, double tMin, double tMax, double& t, double& wd, double& hd) const
+		{
+			if (!ok || w <= 0.0 || h <= 0.0) return false;
+			const double a = dot(r.o - pos, n);
+			const double b = dot(r.d, n);
+			if (b == 0.0 || (a < 0.0 && b < 0.0) || (a > 0.0 && b > 0.0)) return false;
+			t = -a / b;
+			if (t > tMax || t < tMin) return false;
+			const V d = r.at(t) - pos;
+			wd = dot(d, wa);
+			hd = dot(d, ha);
+			return !(std::fabs(wd) > (w / 2.0) || std::fabs(hd) > (h / 2.0));
+		}
+	};
+
+	// AABB::hit / boxHitDistance (AABB.cpp:37-80, BVHNode.cpp:20-61)
+	inline bool	boxHit(V lo, V hi, const Ray& r, double tMax, double& tNear)
+	{
+		double tMin = T_MIN;
+		if (r.d.x == 0.0 && r.d.y == 0.0 && r.d.z == 0.0) return false;
+		for (int a = 0; a < 3; a++)
+		{
+			const double inv = at(r.inv, a), o = at(r.o, a);
+			if (inv == 0.0)
+			{
+				if (o < at(lo, a) || o > at(hi, a)) return false;
+				continue;
+			}
+			double t0 = (at(lo, a) - o) * inv, t1 = (at(hi, a) - o) * inv;
+			if (inv < 0.0) std::swap(t0, t1);
+			tMin = t0 > tMin ? t0 : tMin;
+			tMax = t1 < tMax ? t1 : tMax;
+			if (tMax <= tMin) return false;
+		}
+		tNear = tMin;
+		return true;
+	}
+
+	// A plain median-split BVH over a mesh's triangles. Closest-hit results do not depend
+	// on the tree shape (only exact-distance ties could), so this oracle uses its own.
+	struct	MeshBvh
+	{
+		struct	Node
+		{
+			V	lo, hi;
+			std::uint32_t	left = 0, right = 0, start = 0, count = 0;
+		};
+		std::vector<Node>	nodes;
+		std::vector<std::uint32_t>	order;
+
+		void	build(const std::vector<Tri>& tris)
+		{
+			order.resize(tris.size());
+			std::iota(order.begin(), order.end(), 0u);
+			nodes.clear();
+			if (!tris.empty()) make(tris, 0, static_cast<std::uint32_t>(tris.size()));
+		}
+
+		std::uint32_t	make(const std::vector<Tri>& tris, std::uint32_t s, std::uint32_t e)
+		{
+			Node n;
+			n.lo = {INFINITY, INFINITY, INFINITY};
+			n.hi = {-INFINITY, -INFINITY, -INFINITY};
+			for (std::uint32_t i = s; i < e; i++)
+			{
+				const Tri& t = tris[order[i]];
+				n.lo = {std::min(n.lo.x, t.lo.x), std::min(n.lo.y, t.lo.y), std::min(n.lo.z, t.lo.z)};
+				n.hi = {std::max(n.hi.x, t.hi.x), std::max(n.hi.y, t.hi.y), std::max(n.hi.z, t.hi.z)};
+			}
+			const std::uint32_t idx = static_cast<std::uint32_t>(nodes.size());
+			nodes.push_back(n);
+			if (e - s <= 4)
+			{
+				nodes[idx].start = s;
+				nodes[idx].count = e - s;
+				return idx;
+			}
+			const V ext = n.hi - n.lo;
+			const int axis = (ext.x >= ext.y && ext.x >= ext.z) ? 0 : (ext.y >= ext.z ? 1 : 2);
+			const std::uint32_t mid = s + (e - s) / 2;
+			std::nth_element(order.begin() + s, order.begin() + mid, order.begin() + e, [&](std::uint32_t a, std::uint32_t b) {
+				return at(tris[a].lo, axis) + at(tris[a].hi, axis) < at(tris[b].lo, axis) + at(tris[b].hi, axis);
+			});
+			const std::uint32_t l = make(tris, s, mid);
+			const std::uint32_t r = make(tris, mid, e);
+			nodes[idx].left = l;
+			nodes[idx].right = r;
+			return idx;
+		}
+
+		// closest (any = false) or any hit in [tMin, tMax]
+		bool	trace(const std::vector<Tri>& tris, const Ray& r, double tMin, double tMax, bool any,
+			std::uint32_t& triOut, double& tOut, double& uOut, double& vOut) const
+		{
+			if (nodes.empty()) return false;
+			std::uint32_t stack[128];
+			int sp = 0;
+			stack[sp++] = 0;
+			bool found = false;
+			double closest = tMax;
+			while (sp > 0)
+			{
+				const Node& n = nodes[stack[--sp]];
+				double tn;
+				if (!boxHit(n.lo, n.hi, r, closest, tn)) continue;
+				if (n.count > 0)
+				{
+					for (std::uint32_t i = n.start; i < n.start + n.count; i++)
+					{
+						double t, u, v;
+						if (tris[order[i]].intersect(r, tMin, closest, t, u, v))
+						{
+							found = true;
+							closest = t;
+							triOut = order[i];
+							tOut = t; uOut = u; vOut = v;
+							if (any) return true;
+						}
+					}
+					continue;
+				}
+				stack[sp++] = n.right;
+				stack[sp++] = n.left;
+			}
+			return found;
+		}
+	};
+
+	struct	Mesh
+	{
+		std::vector<Tri>	tris;
+		MeshBvh	bvh;
+		V	lo, hi;	// root box of the reference's mesh BVH (union of padded triangle boxes)
+	};
+
+	struct	Instance	// MeshInstance.cpp:48-167
+	{
+		std::uint32_t	mesh = 0, material = 0;
+		V	position, scale;
+		double	cx = 1, sx = 0, cy = 1, sy = 0, cz = 1, sz = 0;
+		V	lo, hi;
+
+		V	rotate(V v) const
+		{
+			double x = v.x, y = v.y, z = v.z;
+			double ny = (y * cx) - (z * sx), nz = (y * sx) + (z * cx);
+			y = ny; z = nz;
+			double nx = (x * cy) + (z * sy);
+			nz = (-x * sy) + (z * cy);
+			x = nx; z = nz;
+			nx = (x * cz) - (y * sz);
+			ny = (x * sz) + (y * cz);
+			return {nx, ny, z};
+		}
+		V	inverseRotate(V v) const
+		{
+			double x = v.x, y = v.y, z = v.z;
+			double nx = (x * cz) + (y * sz), ny = (-x * sz) + (y * cz);
+			x = nx; y = ny;
+			nx = (x * cy) - (z * sy);
+			double nz = (x * sy) + (z * cy);
+			x = nx; z = nz;
+			ny = (y * cx) + (z * sx);
+			nz = (-y * sx) + (z * cx);
+			return {x, ny, nz};
+		}
+		V	transformNormal(V n) const	// :151-164
+		{
+			const V s = rotate({n.x / scale.x, n.y / scale.y, n.z / scale.z});
+			const double ls = len2(s);
+			if (!std::isfinite(ls) || ls <= 1e-24) return V{};
+			return s / std::sqrt(ls);
+		}
+		bool	localRay(const Ray& r, Ray& local, double& scaleOut) const	// :251-272
+		{
+			const V ur = inverseRotate(r.d);
+			const V ld = {ur.x / scale.x, ur.y / scale.y, ur.z / scale.z};
+			scaleOut = std::sqrt(len2(ld));
+			if (!std::isfinite(scaleOut) || scaleOut <= 0.0) return false;
+			const V uo = inverseRotate(r.o - position);
+			const V lo2 = {uo.x / scale.x, uo.y / scale.y, uo.z / scale.z};
+			local = Ray::normalized(lo2, ld / scaleOut);
+			return true;
+		}
+	};
+
+	enum Kind { K_TRI, K_RECT, K_INST };
+
+	struct	Top
+	{
+		Kind	kind;
+		std::uint32_t	index;
+		std::uint32_t	material;
+	};
+
+	struct	Mat
+	{
+		luzgpu::MaterialKind	kind;
+		C	color;
+		double	roughness = 0, alpha = 1e-4, ior = 1.5;
+	};
+
+	struct	Light
+	{
+		Kind	kind;
+		std::uint32_t	index;
+		C	emitted;
+		double	area;
+	};
+
+	struct	Env
+	{
+		std::size_t	W = 0, H = 0;
+		std::vector<C>	px;
+		std::vector<double>	weights, solid, cdf;
+		double	total = 0, strength = 1, rotation = 0;
+		bool	lighting = true;
+	};
+
+	struct	Scene
+	{
+		std::vector<Top>	top;
+		std::vector<Tri>	topTris;	// indexed by Top.index for K_TRI
+		std::vector<Rect>	rects;
+		std::vector<Mesh>	meshes;
+		std::vector<Instance>	instances;
+		std::vector<Mat>	mats;
+		std::vector<Light>	lights;
+		std::vector<double>	lightCdf;
+		double	lightTotal = 0;
+		bool	hasEnv = false;
+		Env	env;
+		luzgpu::SkyKind	sky = luzgpu::SKY_NONE;
+		C	background;
+		int	maxBounces = 8;
+		std::uint32_t	width = 0, height = 0, samples = 1;
+		double	mpu = 1.0;
+		// RenderCamera (RendererInternal.hpp:13-26)
+		V	camPos, camU, camV, horizontal, vertical, llc;
+		double	lensRadius = 0, invW = 0, invH = 0;
+		luzgpu::Camera	camera;
+	};
+
+	void	setupCamera(Scene& s)	// RayHelper.cpp:50-82
+	{
+		const luzgpu::Camera& c = s.camera;
+		const double W = s.width, H = s.height;
+		s.invW = W > 1.0 ? 1.0 / (W - 1.0) : 0.0;
+		s.invH = H > 1.0 ? 1.0 / (H - 1.0) : 0.0;
+		s.camPos = fromD(c.position);
+		const double focus = c.focusDistance / s.mpu;
+		double vw = focus * c.sensorWidth / c.focalLength;
+		double vh = focus * c.sensorHeight / c.focalLength;
+		const double ia = W / H, va = vw / vh;
+		if (ia > va) vh = vw / ia;
+		else if (ia < va) vw = vh * ia;
+		const V w = normalize(fromD(c.direction));
+		V up = fromD(c.up);
+		if (!std::isfinite(up.x) || !std::isfinite(up.y) || !std::isfinite(up.z) || len2(up) <= 1e-12) up = {0, 1, 0};
+		up = normalize(up);
+		if (len2(cross(up, w)) <= 1e-12) up = {0, 0, 1};
+		if (len2(cross(up, w)) <= 1e-12) up = {1, 0, 0};
+		s.camU = normalize(cross(up, w));
+		s.camV = cross(w, s.camU);
+		s.lensRadius = (c.pinhole ? 0.0 : c.focalLength / c.fNumber) / (2.0 * s.mpu);
+		s.horizontal = s.camU * vw;
+		s.vertical = s.camV * vh;
+		s.llc = s.camPos + (s.horizontal / 2.0) + (s.vertical / 2.0) + (w * focus);
+	}
+
+	Scene	build(const luzgpu::Scene& in, std::uint32_t width, std::uint32_t height)
+	{
+		Scene s;
+		for (const auto& m : in.materials)
+		{
+			Mat o;
+			o.kind = m.kind;
+			o.color = fromC(m.color);
+			o.roughness = m.roughness;
+			const double r = std::isfinite(m.roughness) ? std::max(0.0, std::min(1.0, m.roughness)) : 0.0;
+			o.alpha = std::max(1e-4, r * r);	// BSDF.cpp:54-58
+			o.ior = m.ior;
+			s.mats.push_back(o);
+		}
+		for (const auto& mesh : in.meshes)
+		{
+			Mesh m;
+			for (const auto& t : mesh.triangles)
+			{
+				Tri o;
+				o.v0 = fromD(t.v0); o.v1 = fromD(t.v1); o.v2 = fromD(t.v2);
+				o.n0 = fromD(t.n0); o.n1 = fromD(t.n1); o.n2 = fromD(t.n2);
+				o.hasN = t.hasNormals;
+				o.init();
+				m.tris.push_back(o);
+			}
+			m.bvh.build(m.tris);
+			m.lo = {INFINITY, INFINITY, INFINITY};
+			m.hi = {-INFINITY, -INFINITY, -INFINITY};
+			for (const Tri& t : m.tris)
+			{
+				m.lo = {std::min(m.lo.x, t.lo.x), std::min(m.lo.y, t.lo.y), std::min(m.lo.z, t.lo.z)};
+				m.hi = {std::max(m.hi.x, t.hi.x), std::max(m.hi.y, t.hi.y), std::max(m.hi.z, t.hi.z)};
+			}
+			s.meshes.push_back(std::move(m));
+		}
+		for (const auto& h : in.hittables)
+		{
+			Top t{K_TRI, 0, h.material};
+			if (h.kind == luzgpu::HIT_TRIANGLE)
+			{
+				const auto& tr = in.triangles[h.index];
+				Tri o;
+				o.v0 = fromD(tr.v0); o.v1 = fromD(tr.v1); o.v2 = fromD(tr.v2);
+				o.init();
+				o.material = h.material;
+				t.kind = K_TRI;
+				t.index = static_cast<std::uint32_t>(s.topTris.size());
+				s.topTris.push_back(o);
+			}
+			else if (h.kind == luzgpu::HIT_RECTANGLE)
+			{
+				const auto& r = in.rectangles[h.index];
+				Rect o;
+				o.pos = fromD(r.position);
+				o.w = r.width;
+				o.h = r.height;
+				const V ori = fromD(r.orientation);
+				o.ok = len2(ori) > 0.0;	// Rectangle.cpp:14-50
+				if (o.ok)
+				{
+					o.n = normalize(ori);
+					if (std::fabs(o.n.y) > 0.999) { o.wa = {1, 0, 0}; o.ha = {0, 0, 1}; }
+					else if (std::fabs(o.n.z) > 0.999) { o.wa = {1, 0, 0}; o.ha = {0, 1, 0}; }
+					else if (std::fabs(o.n.x) > 0.999) { o.wa = {0, 0, 1}; o.ha = {0, 1, 0}; }
+					else
+					{
+						V helper = {0, 1, 0};
+						if (std::fabs(dot(helper, o.n)) > 0.9) helper = {1, 0, 0};
+						o.wa = normalize(cross(helper, o.n));
+						o.ha = normalize(cross(o.n, o.wa));
+					}
+				}
+				o.material = h.material;
+				t.kind = K_RECT;
+				t.index = static_cast<std::uint32_t>(s.rects.size());
+				s.rects.push_back(o);
+			}
+			else
+			{
+				const auto& inst = in.instances[h.index];
+				Instance o;
+				o.mesh = inst.mesh;
+				o.material = h.material;
+				o.position = fromD(inst.position);
+				o.scale = fromD(inst.scale);
+				const double d2r = 3.14159265358979323846 / 180.0;
+				o.cx = std::cos(inst.rotationDegrees[0] * d2r); o.sx = std::sin(inst.rotationDegrees[0] * d2r);
+				o.cy = std::cos(inst.rotationDegrees[1] * d2r); o.sy = std::sin(inst.rotationDegrees[1] * d2r);
+				o.cz = std::cos(inst.rotationDegrees[2] * d2r); o.sz = std::sin(inst.rotationDegrees[2] * d2r);
+				t.kind = K_INST;
+				t.index = static_cast<std::uint32_t>(s.instances.size());
+				s.instances.push_back(o);
+			}
+			s.top.push_back(t);
+		}
+		// lights (Scene.cpp:615-638)
+		for (std::size_t i = 0; i < s.top.size(); i++)
+		{
+			const Top& t = s.top[i];
+			if (s.mats[t.material].kind != luzgpu::MAT_EMISSIVE) continue;
+			Light l{t.kind, t.index, s.mats[t.material].color, 0.0};
+			if (t.kind == K_TRI) l.area = s.topTris[t.index].area;
+			else if (t.kind == K_RECT) l.area = s.rects[t.index].h * s.rects[t.index].w;
+			else throw std::runtime_error("oracle: emissive mesh instances are not restated");
+			const double lum = luminance(l.emitted);
+			double w = (l.area <= 0.0 || !std::isfinite(lum) || lum <= 0.0) ? 0.0 : l.area * lum;
+			if (!std::isfinite(w) || w <= 0.0) w = 0.0;
+			s.lightTotal += w;
+			s.lightCdf.push_back(s.lightTotal);
+			s.lights.push_back(l);
+		}
+		if (s.lightTotal <= 0.0) s.lightCdf.clear();
+		// environment (EnvironmentMap.cpp:627-671)
+		s.hasEnv = in.hasEnvironment;
+		if (s.hasEnv)
+		{
+			Env& e = s.env;
+			e.W = in.environment.width;
+			e.H = in.environment.height;
+			e.strength = in.environment.strength;
+			e.rotation = in.environment.rotation;
+			e.lighting = in.environment.lighting;
+			for (const auto& p : in.environment.pixels) e.px.push_back(fromC(p));
+			e.weights.assign(e.W * e.H, 0.0);
+			e.solid.assign(e.W * e.H, 0.0);
+			const double dphi = 2.0 * PI / static_cast<double>(e.W);
+			for (std::size_t y = 0; y < e.H; y++)
+			{
+				const double t0 = PI * static_cast<double>(y) / static_cast<double>(e.H);
+				const double t1 = PI * static_cast<double>(y + 1) / static_cast<double>(e.H);
+				const double sa = dphi * std::max(0.0, std::cos(t0) - std::cos(t1));
+				for (std::size_t x = 0; x < e.W; x++)
+				{
+					const std::size_t i = y * e.W + x;
+					const C c = e.px[i];
+					const C sc = {std::isfinite(c.r) ? std::max(0.0, c.r) : 0.0, std::isfinite(c.g) ? std::max(0.0, c.g) : 0.0,
+						std::isfinite(c.b) ? std::max(0.0, c.b) : 0.0};
+					const double lum = luminance(sc);
+					const double w = (std::isfinite(lum) && lum > 0.0) ? lum * sa : 0.0;
+					e.solid[i] = sa;
+					e.weights[i] = w;
+					e.total += w;
+					e.cdf.push_back(e.total);
+				}
+			}
+			if (!std::isfinite(e.total) || e.total <= 0.0)
+			{
+				e.total = 0.0;
+				e.cdf.clear();
+				e.weights.assign(e.W * e.H, 0.0);
+			}
+		}
+		s.sky = in.settings.sky;
+		s.background = fromC(in.settings.background);
+		s.maxBounces = in.settings.maxBounces;
+		s.width = width ? width : in.settings.width;
+		s.height = height ? height : in.settings.height;
+		s.samples = in.settings.samples;
+		s.mpu = in.settings.metersPerUnit;
+		s.camera = in.camera;
+		setupCamera(s);
+		for (Instance& inst : s.instances)	// MeshInstance.cpp:212-249
+		{
+			const Mesh& m = s.meshes[inst.mesh];
+			inst.lo = {INFINITY, INFINITY, INFINITY};
+			inst.hi = {-INFINITY, -INFINITY, -INFINITY};
+			for (int c = 0; c < 8; c++)
+			{
+				const V corner = {(c & 1) ? m.hi.x : m.lo.x, (c & 2) ? m.hi.y : m.lo.y, (c & 4) ? m.hi.z : m.lo.z};
+				const V w = inst.rotate(corner * inst.scale) + inst.position;
+				inst.lo = {std::min(inst.lo.x, w.x), std::min(inst.lo.y, w.y), std::min(inst.lo.z, w.z)};
+				inst.hi = {std::max(inst.hi.x, w.x), std::max(inst.hi.y, w.y), std::max(inst.hi.z, w.z)};
+			}
+		}
+		return s;
+	}
+
+	// ---- scene queries (HitHelper.cpp:5-32, ColorHelper.cpp:801-829) ----------------------------
+	bool	closestHit(const Scene& s, const Ray& r, Hit& h, bool any = false, double tMax = T_MAX)
+	{
+		bool found = false;
+		double closest = tMax;
+		for (std::size_t i = 0; i < s.top.size(); i++)
+		{
+			const Top& t = s.top[i];
+			if (t.kind == K_TRI)
+			{
+				double tt, u, v;
+				if (s.topTris[t.index].intersect(r, T_MIN, closest, tt, u, v))
+				{
+					if (any) return true;
+					s.topTris[t.index].record(r, tt, u, v, h);
+					h.material = t.material;
+					h.top = static_cast<std::int64_t>(i);
+					h.tri = -1;
+					closest = tt;
+					found = true;
+				}
+			}
+			else if (t.kind == K_RECT)
+			{
+				double tt, wd, hd;
+				const Rect& rc = s.rects[t.index];
+				if (rc.intersect(r, T_MIN, closest, tt, wd, hd))
+				{
+					if (any) return true;
+					h.t = tt;
+					setFace(h, r, rc.n, h.n, h.front);
+					h.gn = h.n;
+					h.p = r.at(tt);
+					h.material = t.material;
+					h.top = static_cast<std::int64_t>(i);
+					h.tri = -1;
+					closest = tt;
+					found = true;
+				}
+			}
+			else
+			{
+				const Instance& inst = s.instances[t.index];
+				double tn;
+				if (!boxHit(inst.lo, inst.hi, r, closest, tn)) continue;
+				Ray local;
+				double scale;
+				if (!inst.localRay(r, local, scale)) continue;
+				const Mesh& m = s.meshes[inst.mesh];
+				std::uint32_t tri;
+				double lt, u, v;
+				const double lMax = closest == T_MAX ? T_MAX : closest * scale;
+				if (!m.bvh.trace(m.tris, local, T_MIN * scale, lMax, any, tri, lt, u, v)) continue;
+				const double wt = lt / scale;
+				if (!std::isfinite(wt) || wt < T_MIN || wt > closest) continue;
+				if (any) return true;
+				// local record, then MeshInstance::_transformHitRecord (:274-301)
+				Hit lh;
+				m.tris[tri].record(local, lt, u, v, lh);
+				V wn = inst.transformNormal(lh.n);
+				V wg = inst.transformNormal(lh.gn);
+				if (dot(r.d, wn) > 0.0) wn = wn * -1.0;
+				if (dot(r.d, wg) > 0.0) wg = wg * -1.0;
+				h.t = wt;
+				h.p = r.at(wt);
+				h.n = wn;
+				h.gn = wg;
+				h.front = dot(r.d, wn) < 0.0;
+				h.material = t.material;
+				h.top = static_cast<std::int64_t>(i);
+				h.tri = tri;
+				closest = wt;
+				found = true;
+			}
+		}
+		return found;
+	}
+
+	bool	occluded(const Scene& s, const Ray& r, double tMax)	// ColorHelper.cpp:801-829
+	{
+		if (tMax <= T_MIN) return false;
+		Hit h;
+		return closestHit(s, r, h, true, tMax);
+	}
+
+	// ---- BSDF helpers (BSDF.cpp) -----------------------------------------------------------------
+	double	clamp01(double v) { return std::isfinite(v) ? std::max(0.0, std::min(1.0, v)) : 0.0; }
+	C	clamp01(C c) { return {clamp01(c.r), clamp01(c.g), clamp01(c.b)}; }
+	V	safeNormalize(V v, V fb)
+	{
+		const double l2 = len2(v);
+		if (!std::isfinite(l2) || l2 <= 0.0) return fb;
+		return v / std::sqrt(l2);
+	}
+	C	schlick(C f0, double c)	// :81-92
+	{
+		c = clamp01(c);
+		const double f = std::pow(1.0 - c, 5.0);
+		f0 = clamp01(f0);
+		return {f0.r + (1.0 - f0.r) * f, f0.g + (1.0 - f0.g) * f, f0.b + (1.0 - f0.b) * f};
+	}
+	double	dielectricFresnel(double c, double ei, double et)	// :93-125
+	{
+		c = clamp01(std::fabs(c));
+		if (!std::isfinite(ei) || !std::isfinite(et) || ei <= 0.0 || et <= 0.0) return 1.0;
+		const double si = std::sqrt(std::max(0.0, 1.0 - c * c));
+		const double st = ei / et * si;
+		if (st >= 1.0) return 1.0;
+		const double ct = std::sqrt(std::max(0.0, 1.0 - st * st));
+		const double rp = ((et * c) - (ei * ct)) / ((et * c) + (ei * ct));
+		const double rs = ((ei * c) - (et * ct)) / ((ei * c) + (et * ct));
+		return clamp01(0.5 * (rp * rp + rs * rs));
+	}
+	double	ggxD(double noH, double a)	// :209-218
+	{
+		noH = clamp01(noH);
+		a = std::max(1e-4, a);
+		const double a2 = a * a, den = noH * noH * (a2 - 1.0) + 1.0, v = a2 / (PI * den * den);
+		return std::isfinite(v) ? v : 0.0;
+	}
+	double	g1(double noW, double a)	// :220-236
+	{
+		noW = clamp01(noW);
+		a = std::max(1e-4, a);
+		if (noW <= 0.0) return 0.0;
+		const double a2 = a * a, den = noW + std::sqrt(a2 + (1.0 - a2) * noW * noW);
+		if (den <= 1e-10 || !std::isfinite(den)) return 0.0;
+		return 2.0 * noW / den;
+	}
+	struct	Onb	// ONB.cpp:5-13
+	{
+		V	u, v, w;
+		explicit Onb(V n)
+		{
+			w = normalize(n);
+			const V a = (std::fabs(w.x) > 0.9) ? V{0, 1, 0} : V{1, 0, 0};
+			v = normalize(cross(w, a));
+			u = cross(w, v);
+		}
+		V	local(double a, double b, double c) const { return a * u + b * v + c * w; }
+	};
+	V	ggxHalf(V n, double a, float sx, float sy)	// :192-207
+	{
+		a = std::max(1e-4, a);
+		const double a2 = a * a, phi = 2.0 * PI * clamp01(sx), u = std::min(1.0 - 1e-7, clamp01(sy));
+		const double ct = std::sqrt((1.0 - u) / (1.0 + (a2 - 1.0) * u));
+		const double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+		return Onb(n).local(st * std::cos(phi), st * std::sin(phi), ct);
+	}
+	double	ggxReflPdf(V n, V view, V l, double a)	// :243-267
+	{
+		const double noV = dot(n, view), noL = dot(n, l);
+		if (noV <= 0.0 || noL <= 0.0) return 0.0;
+		const V h = safeNormalize(view + l, n);
+		const double voH = dot(view, h), noH = dot(n, h);
+		if (voH <= 1e-10 || noH <= 0.0) return 0.0;
+		const double pdf = ggxD(noH, a) * noH / (4.0 * voH);
+		return (std::isfinite(pdf) && pdf > 0.0) ? pdf : 0.0;
+	}
+	C	ggxReflBsdf(C F, V n, V view, V l, double a)	// :269-287
+	{
+		const double noV = dot(n, view), noL = dot(n, l);
+		if (noV <= 0.0 || noL <= 0.0) return {};
+		const V h = safeNormalize(view + l, n);
+		const double sc = ggxD(dot(n, h), a) * (g1(noV, a) * g1(noL, a)) / (4.0 * noV);
+		if (!std::isfinite(sc) || sc <= 0.0) return {};
+		return clamp01(F) * sc;
+	}
+	C	metalBsdf(const Mat& m, V n, V view, V l)	// :306-323
+	{
+		const V h = safeNormalize(view + l, n);
+		return ggxReflBsdf(schlick(clamp01(m.color), std::max(0.0, dot(view, h))), n, view, l, m.alpha);
+	}
+	bool	refractDir(V i, V mn, double eta, V& out)	// :325-350
+	{
+		const double c = std::min(1.0, dot(i * -1.0, mn));
+		const V perp = eta * (i + c * mn);
+		const double p2 = len2(perp);
+		if (p2 > 1.0) return false;
+		const V par = -std::sqrt(std::max(0.0, 1.0 - p2)) * mn;
+		out = safeNormalize(perp + par, reflect(i, mn));
+		return len2(out) > 0.0;
+	}
+	double	dielPdf(V n, V view, V l, double a, double ei, double et)	// :352-392
+	{
+		const double noV = dot(n, view), noL = dot(n, l);
+		if (noV <= 0.0 || noL == 0.0 || ei <= 0.0 || et <= 0.0) return 0.0;
+		if (noL > 0.0)
+		{
+			const V h = safeNormalize(view + l, n);
+			return dielectricFresnel(dot(view, h), ei, et) * ggxReflPdf(n, view, l, a);
+		}
+		const double eta = ei / et;
+		V h = safeNormalize(view + l * eta, n);
+		if (dot(n, h) < 0.0) h = h * -1.0;
+		const double noH = dot(n, h), voH = dot(view, h), loH = dot(l, h);
+		if (noH <= 0.0 || voH <= 0.0 || loH >= 0.0) return 0.0;
+		const double den = voH + eta * loH;
+		if (std::fabs(den) <= 1e-10) return 0.0;
+		const double F = dielectricFresnel(voH, ei, et);
+		const double pdf = (1.0 - F) * ggxD(noH, a) * noH * std::fabs((eta * eta * loH) / (den * den));
+		return (std::isfinite(pdf) && pdf > 0.0) ? pdf : 0.0;
+	}
+	C	dielBsdf(C color, V n, V view, V l, double a, double ei, double et)	// :394-458
+	{
+		const double noV = dot(n, view), noL = dot(n, l);
+		if (noV <= 0.0 || noL == 0.0 || ei <= 0.0 || et <= 0.0) return {};
+		color = clamp01(color);
+		if (noL > 0.0)
+		{
+			const V h = safeNormalize(view + l, n);
+			const double F = dielectricFresnel(dot(view, h), ei, et);
+			return ggxReflBsdf({F, F, F}, n, view, l, a);
+		}
+		const double eta = ei / et;
+		V h = safeNormalize(view + l * eta, n);
+		if (dot(n, h) < 0.0) h = h * -1.0;
+		const double noH = dot(n, h), voH = dot(view, h), loH = dot(l, h);
+		if (noH <= 0.0 || voH <= 0.0 || loH >= 0.0) return {};
+		const double den = voH + eta * loH;
+		if (std::fabs(den) <= 1e-10) return {};
+		const double F = dielectricFresnel(voH, ei, et);
+		const double sc = (1.0 - F) * ggxD(noH, a) * (g1(noV, a) * g1(std::fabs(noL), a)) * eta * eta * std::fabs(voH * loH)
+			/ (noV * den * den);
+		if (!std::isfinite(sc) || sc <= 0.0) return {};
+		return color * sc;
+	}
+
+	// ---- scatter (Material.hpp:23-41 + each material's scatter) -----------------------------------
+	enum PdfType { PDF_NONE, PDF_COSINE, PDF_BSDF };
+	struct	Scatter
+	{
+		PdfType	type = PDF_NONE;
+		bool	specular = false;
+		C	att;
+		V	dir;		// specular ray direction (normalized) or sampled BSDF direction
+		V	incident;
+		Onb	basis = Onb(V{0, 0, 1});
+	};
+
+	double	bsdfPdf(const Mat& m, const Hit& h, const Scatter& sc, V l)
+	{
+		const V view = safeNormalize(sc.incident * -1.0, h.n);
+		if (m.kind == luzgpu::MAT_METAL) return ggxReflPdf(h.n, view, l, m.alpha);	// Metal.cpp:202-216
+		const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0;		// Dielectric.cpp:230-245
+		return dielPdf(h.n, view, l, m.alpha, ei, et);
+	}
+	C	bsdfEval(const Mat& m, const Hit& h, const Scatter& sc, V l)
+	{
+		const V view = safeNormalize(sc.incident * -1.0, h.n);
+		if (m.kind == luzgpu::MAT_METAL) return metalBsdf(m, h.n, view, l);	// Metal.cpp:172-200
+		const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0;
+		return dielBsdf(m.color, h.n, view, l, m.alpha, ei, et);				// Dielectric.cpp:210-228
+	}
+	double	scatterPdf(const Mat& m, const Hit& h, const Scatter& sc, V l)	// ColorHelper.cpp:39-93
+	{
+		if (sc.type == PDF_COSINE)
+		{
+			const double c = dot(l, sc.basis.w);
+			return c <= 0.0 ? 0.0 : c / PI;
+		}
+		if (sc.type == PDF_BSDF) return bsdfPdf(m, h, sc, l);
+		return 0.0;
+	}
+	C	scatterBsdfCos(const Mat& m, const Hit& h, const Scatter& sc, V l)	// ColorHelper.cpp:95-113
+	{
+		if (sc.type == PDF_BSDF) return bsdfEval(m, h, sc, l);
+		return sc.att * scatterPdf(m, h, sc, l);
+	}
+
+	bool	scatter(const Mat& m, const Hit& h, const Ray& r, const Sampler& ps, Scatter& sc)
+	{
+		sc.incident = r.d;
+		if (m.kind == luzgpu::MAT_LAMBERTIAN)	// Lambertian.cpp:15-24
+		{
+			sc.att = m.color;
+			sc.type = PDF_COSINE;
+			sc.basis = Onb(h.n);
+			return true;
+		}
+		if (m.kind == luzgpu::MAT_METAL)	// Metal.cpp:126-170
+		{
+			const V view = safeNormalize(r.d * -1.0, h.n);
+			const double ct = std::min(1.0, std::max(0.0, dot(view, h.n)));
+			if (m.roughness <= 1e-4)
+			{
+				sc.dir = Ray::make(h.p, reflect(r.d, h.n)).d;
+				sc.att = schlick(clamp01(m.color), ct);
+				sc.specular = true;
+				return true;
+			}
+			float sx, sy;
+			ps.s2(D_BSDF_DIRECTION, sx, sy);
+			V half = ggxHalf(h.n, m.alpha, sx, sy);
+			if (dot(view, half) <= 0.0) half = h.n;
+			V dir = reflect(r.d, half);
+			if (dot(dir, h.n) <= 0.0) dir = reflect(r.d, h.n);
+			const double pdf = ggxReflPdf(h.n, view, dir, m.alpha);
+			const C f = metalBsdf(m, h.n, view, dir);
+			if (pdf <= 0.0 || !std::isfinite(pdf) || maxChannel(f) <= 0.0) return false;
+			sc.dir = dir;
+			sc.att = f / pdf;
+			sc.type = PDF_BSDF;
+			return true;
+		}
+		if (m.kind == luzgpu::MAT_DIELECTRIC)	// Dielectric.cpp:127-208
+		{
+			const V view = safeNormalize(r.d * -1.0, h.n);
+			const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0, ratio = ei / et;
+			if (m.roughness <= 1e-4)
+			{
+				const double ct = std::fmin(dot(view, h.n), 1.0);
+				const double st = std::sqrt(1.0 - ct * ct);
+				V dir;
+				if (ratio * st > 1.0 || dielectricFresnel(ct, ei, et) > ps.s1(D_MATERIAL_DECISION))
+				{
+					dir = reflect(r.d, h.n);
+					sc.att = {1, 1, 1};
+				}
+				else
+				{
+					const double c = std::fmin(dot(r.d * -1.0, h.n), 1.0);	// Utilities.hpp:59-66
+					const V perp = ratio * (r.d + c * h.n);
+					const V par = -std::sqrt(std::fabs(1.0 - len2(perp))) * h.n;
+					dir = perp + par;
+					sc.att = m.color;
+				}
+				sc.dir = Ray::make(h.p, dir).d;
+				sc.specular = true;
+				return true;
+			}
+			float sx, sy;
+			ps.s2(D_BSDF_DIRECTION, sx, sy);
+			V half = ggxHalf(h.n, m.alpha, sx, sy);
+			if (dot(view, half) <= 0.0) half = h.n;
+			const double F = dielectricFresnel(dot(view, half), ei, et);
+			V refr;
+			const bool can = refractDir(r.d, half, ratio, refr);
+			const V dir = (!can || ps.s1(D_MATERIAL_DECISION) < F) ? reflect(r.d, half) : refr;
+			const double pdf = dielPdf(h.n, view, dir, m.alpha, ei, et);
+			const C f = dielBsdf(m.color, h.n, view, dir, m.alpha, ei, et);
+			if (pdf <= 0.0 || !std::isfinite(pdf) || maxChannel(f) <= 0.0) return false;
+			sc.dir = dir;
+			sc.att = f / pdf;
+			sc.type = PDF_BSDF;
+			return true;
+		}
+		return false;	// Emissive / Material::scatter (Material.cpp:40-46)
+	}
+
+	// ---- lights and environment -----------------------------------------------------------------------
+	bool	leavesOpaque(const Mat& m, const Hit& h, V d)	// ColorHelper.cpp:357-374
+	{
+		if (m.kind == luzgpu::MAT_DIELECTRIC) return true;
+		V g = h.gn;
+		if (len2(g) <= 1e-12) g = h.n;
+		if (len2(g) <= 1e-12) return true;
+		return dot(g, d) > 1e-7;
+	}
+
+	double	selectionProbability(const Scene& s, std::size_t i)	// ColorHelper.cpp:176-199
+	{
+		if (s.lights.empty()) return 0.0;
+		if (s.lightCdf.empty() || !(s.lightTotal > 0.0)) return 1.0 / static_cast<double>(s.lights.size());
+		const double prev = i == 0 ? 0.0 : s.lightCdf[i - 1];
+		const double w = s.lightCdf[i] - prev;
+		if (w <= 0.0 || s.lightTotal <= 0.0) return 0.0;
+		return w / s.lightTotal;
+	}
+
+	struct	LS
+	{
+		V	dir;
+		C	emitted;
+		double	pdf = 0, tMax = 0;
+		bool	valid = false;
+	};
+
+	LS	sampleLight(const Scene& s, V origin, const Sampler& ps)	// ColorHelper.cpp:914-971
+	{
+		LS out;
+		if (s.lights.empty()) return out;
+		std::size_t idx = 0;
+		if (s.lights.size() > 1)	// :201-231
+		{
+			const double u = ps.s1(D_LIGHT_SELECTION);
+			if (!s.lightCdf.empty() && s.lightTotal > 0.0)
+			{
+				const auto it = std::upper_bound(s.lightCdf.begin(), s.lightCdf.end(), u * s.lightTotal);
+				idx = std::min<std::size_t>(static_cast<std::size_t>(it - s.lightCdf.begin()), s.lights.size() - 1);
+			}
+			else
+			{
+				idx = std::min<std::size_t>(static_cast<std::size_t>(u * s.lights.size()), s.lights.size() - 1);
+			}
+		}
+		const double sel = selectionProbability(s, idx);
+		if (sel <= 0.0 || !std::isfinite(sel)) return out;
+		const Light& L = s.lights[idx];
+		float sx, sy;
+		ps.s2(D_LIGHT_SURFACE_POINT, sx, sy);
+		V point, normal;
+		if (L.kind == K_RECT)	// Rectangle.cpp:381-419
+		{
+			const Rect& r = s.rects[L.index];
+			point = r.pos + (r.wa * ((sx - 0.5) * r.w)) + (r.ha * ((sy - 0.5) * r.h));
+			normal = r.n;
+		}
+		else	// Triangle.cpp:391-432
+		{
+			const Tri& t = s.topTris[L.index];
+			double u = sx, v = sy;
+			if (u + v > 1.0) { u = 1.0 - u; v = 1.0 - v; }
+			point = t.v0 + ((t.v1 - t.v0) * u) + ((t.v2 - t.v0) * v);
+			normal = t.face;
+		}
+		const V d = point - origin;
+		const double d2 = len2(d);
+		if (!std::isfinite(d2) || d2 <= 0.0) return out;
+		const double dist = std::sqrt(d2);
+		out.dir = d / dist;
+		const double c = std::fabs(dot(out.dir, normal));
+		if (L.area <= 0.0 || c <= 0.0) return out;
+		const double pdf = d2 / (c * L.area);
+		if (!(std::isfinite(pdf) && pdf > 0.0)) return out;
+		out.pdf = pdf * sel;
+		out.tMax = dist;
+		if (out.pdf <= 0.0 || out.tMax <= T_MIN || !std::isfinite(out.pdf) || !std::isfinite(out.tMax)) return out;
+		out.emitted = L.emitted;
+		if (maxChannel(out.emitted) <= 0.0) return out;
+		out.valid = true;
+		return out;
+	}
+
+	double	lightPdfValue(const Scene& s, const Light& L, V origin, V vec)	// Rectangle.cpp:345-359, Triangle.cpp:343-370
+	{
+		const Ray r = Ray::make(origin, vec);
+		if (L.kind == K_RECT)
+		{
+			const Rect& rc = s.rects[L.index];
+			double t, wd, hd;
+			if (!rc.intersect(r, T_MIN, T_MAX, t, wd, hd)) return 0.0;
+			Hit h;
+			setFace(h, r, rc.n, h.n, h.front);
+			const double d2 = t * t * len2(vec);
+			const double c = std::fabs(dot(vec, h.n) / std::sqrt(len2(vec)));
+			return d2 / (c * (rc.h * rc.w));
+		}
+		const Tri& tr = s.topTris[L.index];
+		double t, u, v;
+		if (!tr.intersect(r, T_MIN, T_MAX, t, u, v) || tr.area <= 0.0) return 0.0;
+		Hit g;
+		setFace(g, r, tr.face, g.n, g.front);
+		const double d2 = t * t * len2(vec);
+		const double c = std::fabs(dot(vec, g.n) / std::sqrt(len2(vec)));
+		if (c <= 0.0) return 0.0;
+		return d2 / (c * tr.area);
+	}
+
+	double	lightsPdf(const Scene& s, V origin, V dir)	// ColorHelper.cpp:233-257
+	{
+		if (s.lights.empty()) return 0.0;
+		if (s.lights.size() == 1) return lightPdfValue(s, s.lights[0], origin, dir);
+		double sum = 0.0;
+		for (std::size_t i = 0; i < s.lights.size(); i++) sum += selectionProbability(s, i) * lightPdfValue(s, s.lights[i], origin, dir);
+		return sum;
+	}
+
+	double	wrap01(double v) { v = v - std::floor(v); if (v < 0.0) v += 1.0; return v; }
+	void	dirToUV(const Env& e, V d, double& u, double& v)	// EnvironmentMap.cpp:600-616
+	{
+		const double l2 = len2(d);
+		if (!std::isfinite(l2) || l2 <= 0.0) { u = 0.0; v = 0.5; return; }
+		const V n = d / std::sqrt(l2);
+		const double phi = std::atan2(n.z, n.x);
+		const double y = std::clamp(n.y, -1.0, 1.0);
+		u = wrap01(0.5 + (phi / (2.0 * PI)) + (e.rotation / 360.0));
+		v = clamp01(0.5 + (std::asin(y) / PI));
+	}
+	C	envLookup(const Env& e, V d)	// :456-466, sampleUV :574-598
+	{
+		double u, v;
+		dirToUV(e, d, u, v);
+		u = wrap01(u);
+		v = clamp01(v);
+		const double x = (u * static_cast<double>(e.W)) - 0.5, y = ((1.0 - v) * static_cast<double>(e.H)) - 0.5;
+		const long long xf = static_cast<long long>(std::floor(x)), yf = static_cast<long long>(std::floor(y));
+		auto wrapI = [](long long i, std::size_t n) { long long m = i % static_cast<long long>(n); if (m < 0) m += static_cast<long long>(n); return static_cast<std::size_t>(m); };
+		const std::size_t x0 = wrapI(xf, e.W), x1 = wrapI(xf + 1, e.W);
+		const std::size_t y0 = static_cast<std::size_t>(std::clamp<long long>(yf, 0, static_cast<long long>(e.H - 1)));
+		const std::size_t y1 = static_cast<std::size_t>(std::clamp<long long>(yf + 1, 0, static_cast<long long>(e.H - 1)));
+		const double tx = x - std::floor(x), ty = y - std::floor(y);
+		const C top = e.px[y0 * e.W + x0] * (1.0 - tx) + e.px[y0 * e.W + x1] * tx;
+		const C bot = e.px[y1 * e.W + x0] * (1.0 - tx) + e.px[y1 * e.W + x1] * tx;
+		const C c = top * (1.0 - ty) + bot * ty;
+		return {std::isfinite(c.r) ? std::max(0.0, c.r) : 0.0, std::isfinite(c.g) ? std::max(0.0, c.g) : 0.0,
+			std::isfinite(c.b) ? std::max(0.0, c.b) : 0.0};
+	}
+	double	envPdf(const Env& e, V d)	// :513-539
+	{
+		if (e.total <= 0.0 || e.weights.empty()) return 1.0 / (4.0 * PI);
+		double u, v;
+		dirToUV(e, d, u, v);
+		const std::size_t x = std::min(static_cast<std::size_t>(std::floor(wrap01(u) * static_cast<double>(e.W))), e.W - 1);
+		const std::size_t y = std::min(static_cast<std::size_t>(std::floor((1.0 - clamp01(v)) * static_cast<double>(e.H))), e.H - 1);
+		const std::size_t i = y * e.W + x;
+		if (e.weights[i] <= 0.0 || e.solid[i] <= 0.0) return 0.0;
+		return (e.weights[i] / e.total) / e.solid[i];
+	}
+	double	clampSample(double v)
+	{
+		if (!std::isfinite(v) || v <= 0.0) return 0.0;
+		if (v >= 1.0) return std::nextafter(1.0, 0.0);
+		return v;
+	}
+
+	bool	hasEnvLight(const Scene& s) { return s.hasEnv && s.env.lighting && s.env.strength > 0.0; }
+	bool	hasConstBg(const Scene& s) { return !hasEnvLight(s) && s.sky == luzgpu::SKY_NONE && maxChannel(s.background) > 0.0; }
+
+	LS	sampleInfinite(const Scene& s, V origin, const Sampler& ps)	// ColorHelper.cpp:831-903
+	{
+		LS out;
+		if (hasEnvLight(s))
+		{
+			const Env& e = s.env;
+			const double sel = ps.s1(D_ENV_SELECTION);
+			float jx, jy;
+			ps.s2(D_ENV_POINT, jx, jy);
+			jx = static_cast<float>(clampSample(jx));
+			jy = static_cast<float>(clampSample(jy));
+			V dir;
+			double pdf;
+			if (e.total <= 0.0 || e.cdf.empty())
+			{
+				const double z = 1.0 - (2.0 * jy), rad = std::sqrt(std::max(0.0, 1.0 - z * z)), phi = 2.0 * PI * jx;
+				dir = {rad * std::cos(phi), z, rad * std::sin(phi)};
+				pdf = 1.0 / (4.0 * PI);
+			}
+			else
+			{
+				const double target = clampSample(sel) * e.total;
+				auto it = std::lower_bound(e.cdf.begin(), e.cdf.end(), target);
+				if (it == e.cdf.end()) it = e.cdf.end() - 1;
+				const std::size_t index = static_cast<std::size_t>(it - e.cdf.begin());
+				const std::size_t y = index / e.W, x = index % e.W;
+				const double u = (static_cast<double>(x) + jx) / static_cast<double>(e.W);
+				const double v = 1.0 - ((static_cast<double>(y) + jy) / static_cast<double>(e.H));
+				const double phi = (wrap01(u - (e.rotation / 360.0)) - 0.5) * 2.0 * PI;	// :618-625
+				const double yy = std::sin((clamp01(v) - 0.5) * PI);
+				const double rad = std::sqrt(std::max(0.0, 1.0 - yy * yy));
+				dir = {rad * std::cos(phi), yy, rad * std::sin(phi)};
+				pdf = envPdf(e, dir);
+			}
+			if (!(pdf > 0.0 && std::isfinite(pdf))) return out;
+			out.dir = dir;
+			out.emitted = envLookup(e, dir) * e.strength;
+			out.pdf = pdf;
+			out.tMax = T_MAX;
+			if (maxChannel(out.emitted) <= 0.0) return out;
+			if (occluded(s, Ray::normalized(origin, out.dir), T_MAX)) return out;
+			out.valid = true;
+			return out;
+		}
+		if (hasConstBg(s))
+		{
+			float sx, sy;
+			ps.s2(D_ENV_POINT, sx, sy);
+			const float z = 1.0f - (2.0f * sy);	// Sampler::sphereDirection (Sampler.cpp:248-256)
+			const float rr = std::sqrt(std::max(0.0f, 1.0f - z * z));
+			const float phi = static_cast<float>(2.0 * PI) * sx;
+			out.dir = {rr * std::cos(phi), rr * std::sin(phi), z};
+			out.emitted = s.background;
+			out.pdf = 1.0 / (4.0 * PI);
+			out.tMax = T_MAX;
+			if (maxChannel(out.emitted) <= 0.0) return out;
+			if (occluded(s, Ray::normalized(origin, out.dir), T_MAX)) return out;
+			out.valid = true;
+		}
+		return out;
+	}
+
+	C	clampRay(C c)	// ColorHelper.cpp:259-273
+	{
+		const double l = luminance(c);
+		if (!std::isfinite(l)) return {};
+		if (l > 1.0e12) c = c * (1.0e12 / l);
+		return c;
+	}
+	C	clampBg(const Scene& s, C c)	// :290-300
+	{
+		const double bl = luminance(s.background);
+		if (!std::isfinite(bl) || bl <= 0.0) return c;
+		const double l = luminance(c), cap = bl * 4.0;
+		if (!std::isfinite(l)) return {};
+		if (cap > 0.0 && l > cap) c = c * (cap / l);
+		return c;
+	}
+	double	power(double a, double b)	// :307-318
+	{
+		const double f = a * a, g = b * b, sum = f + g;
+		if (sum <= 0.0 || !std::isfinite(sum)) return 0.0;
+		return f / sum;
+	}
+	bool	terminated(C t) { const double m = maxChannel(t); return !std::isfinite(m) || m <= 1e-6; }	// :320-325
+	bool	roulette(C& t, int b, const Sampler& ps)	// :674-693
+	{
+		if (b < 3) return true;
+		double sp = std::max(0.05, std::min(0.95, maxChannel(t)));
+		if (ps.s1(D_RR) > sp) return false;
+		t = t / sp;
+		return true;
+	}
+
+	C	direct(const Scene& s, const Mat& m, const Hit& h, const Scatter& sc, int b, const Sampler& ps)	// :973-1033
+	{
+		if (s.lights.empty()) return {};
+		const double prob = b < 2 ? 1.0 : 0.5;
+		if (prob < 1.0 && ps.s1(D_LIGHT_STRATEGY) >= prob) return {};
+		const LS ls = sampleLight(s, h.p, ps);
+		if (!ls.valid || !leavesOpaque(m, h, ls.dir)) return {};
+		const double sp = scatterPdf(m, h, sc, ls.dir);
+		const C f = scatterBsdfCos(m, h, sc, ls.dir);
+		if (sp <= 0.0 || !std::isfinite(sp) || maxChannel(f) <= 0.0) return {};
+		if (occluded(s, Ray::normalized(h.p, ls.dir), ls.tMax - T_MIN)) return {};
+		const double w = power(ls.pdf, sp);
+		return f * ls.emitted * (w / (ls.pdf * prob));
+	}
+
+	C	environment(const Scene& s, const Mat& m, const Hit& h, const Scatter& sc, int b, const Sampler& ps)	// :1035-1098
+	{
+		const bool cb = hasConstBg(s);
+		if (cb && sc.type == PDF_BSDF) return {};
+		const double prob = b < 2 ? 1.0 : 0.5;
+		if (prob < 1.0 && ps.s1(D_ENV_STRATEGY) >= prob) return {};
+		const LS ls = sampleInfinite(s, h.p, ps);
+		if (!ls.valid || !leavesOpaque(m, h, ls.dir)) return {};
+		const double sp = scatterPdf(m, h, sc, ls.dir);
+		const C f = scatterBsdfCos(m, h, sc, ls.dir);
+		if (sp <= 0.0 || !std::isfinite(sp) || maxChannel(f) <= 0.0) return {};
+		const double w = power(ls.pdf, sp);
+		C c = f * ls.emitted * (w / (ls.pdf * prob));
+		if (cb) c = clampBg(s, c);
+		return c;
+	}
+
+	C	trace(const Scene& s, Ray ray, Sampler& ps)	// calculateLightRaysColor, ColorHelper.cpp:1136-1326
+	{
+		C acc, thr = {1, 1, 1};
+		bool prevSpec = true, prevInf = false;
+		V prevOrigin;
+		double prevPdf = 0.0;
+		const bool sampledInf = hasEnvLight(s) || hasConstBg(s);
+		for (int b = 0; b <= s.maxBounces; b++)
+		{
+			ps.bounce = static_cast<std::uint32_t>(b);
+			Hit h;
+			if (!closestHit(s, ray, h))
+			{
+				double w = 1.0;
+				if (b > 0 && !prevSpec && prevInf)
+				{
+					w = power(prevPdf, hasEnvLight(s) ? envPdf(s.env, ray.d) : (hasConstBg(s) ? 1.0 / (4.0 * PI) : 0.0));
+				}
+				C sky;
+				if (s.sky == luzgpu::SKY_ENVIRONMENT && s.hasEnv && s.env.strength > 0.0) sky = envLookup(s.env, ray.d) * s.env.strength * w;
+				else sky = s.background * w;
+				C c = thr * sky;
+				if (b > 0 && hasConstBg(s)) c = clampBg(s, c);
+				return acc + clampRay(c);
+			}
+			const Mat& m = s.mats[h.material];
+			const C emitted = m.kind == luzgpu::MAT_EMISSIVE ? m.color : C{};
+			Scatter sc;
+			if (!scatter(m, h, ray, ps, sc))
+			{
+				const double lp = (!prevSpec && !s.lights.empty()) ? lightsPdf(s, prevOrigin, ray.d) : 0.0;
+				const double w = prevSpec ? 1.0 : power(prevPdf, lp);
+				return acc + clampRay((thr * emitted) * w);
+			}
+			const C att = sc.att;
+			if (sc.specular)
+			{
+				if (!leavesOpaque(m, h, sc.dir)) return acc;
+				thr = clampRay(thr * att);
+				if (terminated(thr) || !roulette(thr, b, ps)) return acc;
+				ray = Ray::normalized(h.p, sc.dir);
+				prevSpec = true;
+				prevPdf = 0.0;
+				continue;
+			}
+			acc = acc + thr * emitted;
+			if (terminated(thr * att)) return acc;
+			if (!s.lights.empty()) acc = acc + clampRay(thr * direct(s, m, h, sc, b, ps));
+			if (sampledInf) acc = acc + clampRay(thr * environment(s, m, h, sc, b, ps));
+			V dir;
+			if (sc.type == PDF_COSINE)	// Sampler::cosineHemisphere (Sampler.cpp:239-246)
+			{
+				float sx, sy;
+				ps.s2(D_BSDF_DIRECTION, sx, sy);
+				const float phi = static_cast<float>(2.0 * PI) * sx;
+				const float r = std::sqrt(sy);
+				const float z = std::sqrt(std::max(0.0f, 1.0f - sy));
+				const V local = {static_cast<double>(r * std::cos(phi)), static_cast<double>(r * std::sin(phi)), static_cast<double>(z)};
+				dir = local.x * sc.basis.u + local.y * sc.basis.v + local.z * sc.basis.w;
+			}
+			else
+			{
+				dir = sc.dir;
+			}
+			const Ray scattered = Ray::normalized(h.p, dir);
+			if (!leavesOpaque(m, h, scattered.d)) return acc;
+			const double pdf = scatterPdf(m, h, sc, scattered.d);
+			if (pdf <= 0.0 || !std::isfinite(pdf)) return acc;
+			thr = clampRay(thr * att);
+			if (terminated(thr) || !roulette(thr, b, ps)) return acc;
+			ray = scattered;
+			prevSpec = false;
+			prevOrigin = h.p;
+			prevPdf = pdf;
+			prevInf = (s.sky == luzgpu::SKY_ENVIRONMENT && hasEnvLight(s)) || (hasConstBg(s) && sc.type != PDF_BSDF);
+		}
+		return acc;
+	}
+
+	Ray	cameraRay(const Scene& s, std::size_t x, std::size_t y, const Sampler& ps)	// RayHelper.cpp:85-106
+	{
+		float cx, cy;
+		ps.s2(D_CAMERA, cx, cy);
+		const double xU = double(static_cast<double>(x) + cx) * s.invW;
+		const double yV = double(static_cast<double>(y) + cy) * s.invH;
+		V off;
+		if (s.lensRadius > 0.0)
+		{
+			float sx, sy;	// Sampler::unitDisk (Sampler.cpp:267-292)
+			ps.s2(D_LENS, sx, sy);
+			sx = (2.0f * sx) - 1.0f;
+			sy = (2.0f * sy) - 1.0f;
+			V rd;
+			if (!(sx == 0.0f && sy == 0.0f))
+			{
+				float rad, th;
+				const float pi = static_cast<float>(PI);
+				if (std::fabs(sx) > std::fabs(sy)) { rad = sx; th = (pi / 4.0f) * (sy / sx); }
+				else { rad = sy; th = (pi / 2.0f) - ((pi / 4.0f) * (sx / sy)); }
+				rd = {rad * std::cos(th), rad * std::sin(th), 0.0};
+			}
+			rd = rd * s.lensRadius;
+			off = s.camU * rd.x + s.camV * rd.y;
+		}
+		return Ray::make(s.camPos + off, s.llc - (s.horizontal * xU) - (s.vertical * yV) - s.camPos - off);
+	}
+
+	C	clean(C c) { return {std::isfinite(c.r) ? c.r : 0.0, std::isfinite(c.g) ? c.g : 0.0, std::isfinite(c.b) ? c.b : 0.0}; }	// Threads.cpp:122-137
+}
+
+extern "C" {
+
+// Sampler values for n keys (seed, x, y, sample, stream, bounce, dim).
+void	oracle_sampler(std::size_t n, const std::uint32_t* keys, double* s1, float* s2)
+{
+	for (std::size_t i = 0; i < n; i++)
+	{
+		const std::uint32_t* k = keys + 7 * i;
+		oracle::Sampler ps;
+		ps.begin(k[0], k[1], k[2], k[3], k[4]);
+		ps.bounce = k[5];
+		s1[i] = ps.s1(k[6]);
+		ps.s2(k[6], s2[2 * i], s2[2 * i + 1]);
+	}
+}
+
+static thread_local std::string	g_oracleError;
+
+const char*	oracle_last_error(void)
+{
+	return g_oracleError.c_str();
+}
+
+// FP64 render of samples [sample_begin, sample_begin + spp) — _manageThreads/_threadRender
+// (Threads.cpp:375-583) with 16-pixel blocks over `threads` workers. out: H*W*3 doubles.
+int	oracle_render(const char* path, std::uint32_t seed, std::uint32_t spp, std::uint32_t sample_begin, std::uint32_t width,
+	std::uint32_t height, std::uint32_t max_bounces_plus1, std::uint32_t threads, double* out, double* render_ms)
+{
+	try
+	{
+		const luzgpu::Scene in = luzgpu::readSceneFile(path);
+		oracle::Scene s = oracle::build(in, width, height);
+		if (max_bounces_plus1 > 0) s.maxBounces = static_cast<int>(max_bounces_plus1) - 1;
+		const std::size_t W = s.width, H = s.height, total = W * H;
+		std::atomic<std::size_t> next(0);
+		const auto t0 = std::chrono::steady_clock::now();
+		std::vector<std::thread> pool;
+		for (std::uint32_t t = 0; t < std::max(1u, threads); t++)
+		{
+			pool.emplace_back([&]() {
+				while (true)
+				{
+					const std::size_t start = next.fetch_add(16);
+					if (start >= total) break;
+					for (std::size_t i = start; i < std::min(start + 16, total); i++)
+					{
+						const std::size_t x = i % W, y = i / W;
+						oracle::C sum;
+						for (std::uint32_t k = 0; k < spp; k++)
+						{
+							oracle::Sampler ps;
+							ps.begin(seed, x, y, sample_begin + k, 0);
+							const oracle::Ray r = oracle::cameraRay(s, x, y, ps);
+							sum = sum + oracle::clean(oracle::trace(s, r, ps));
+						}
+						const oracle::C mean = oracle::clean(sum / static_cast<double>(spp));
+						out[3 * i] = mean.r;
+						out[3 * i + 1] = mean.g;
+						out[3 * i + 2] = mean.b;
+					}
+				}
+			});
+		}
+		for (auto& t : pool) t.join();
+		if (render_ms) *render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		g_oracleError = e.what();
+		return -1;
+	}
+}
+
+// Primary-ray closest hit of sample 0 / stream 0 for every pixel: ids[2*i] = top-level
+// hittable, ids[2*i+1] = mesh triangle (or -1); t[i] (or -1).
+int	oracle_primary(const char* path, std::uint32_t seed, std::uint32_t width, std::uint32_t height, std::int64_t* ids, double* t)
+{
+	try
+	{
+		const luzgpu::Scene in = luzgpu::readSceneFile(path);
+		const oracle::Scene s = oracle::build(in, width, height);
+		for (std::size_t i = 0; i < static_cast<std::size_t>(s.width) * s.height; i++)
+		{
+			oracle::Sampler ps;
+			ps.begin(seed, i % s.width, i / s.width, 0, 0);
+			const oracle::Ray r = oracle::cameraRay(s, i % s.width, i / s.width, ps);
+			oracle::Hit h;
+			if (oracle::closestHit(s, r, h)) { ids[2 * i] = h.top; ids[2 * i + 1] = h.tri; t[i] = h.t; }
+			else { ids[2 * i] = -1; ids[2 * i + 1] = -1; t[i] = -1.0; }
+		}
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		g_oracleError = e.what();
+		return -1;
+	}
+}
+
+// Monte-Carlo normalisation check helpers (closed forms): integral of the GGX reflection
+// pdf over the sphere for a view direction at cos(theta) = mu, and the Schlick / dielectric
+// Fresnel at normal incidence.
+double	oracle_ggx_pdf_integral(double alpha, double mu, std::uint32_t n)
+{
+	const oracle::V nrm = {0, 0, 1};
+	const oracle::V view = {std::sqrt(1.0 - mu * mu), 0.0, mu};
+	double sum = 0.0;
+	for (std::uint32_t i = 0; i < n; i++)
+	{
+		for (std::uint32_t j = 0; j < n; j++)
+		{
+			const double z = (i + 0.5) / n;				// upper hemisphere, uniform in z
+			const double phi = 2.0 * oracle::PI * (j + 0.5) / n;
+			const double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+			sum += oracle::ggxReflPdf(nrm, view, {r * std::cos(phi), r * std::sin(phi), z}, alpha);
+		}
+	}
+	return sum * (2.0 * oracle::PI) / (static_cast<double>(n) * n);
+}
+
+// Integral of D(h) (n.h) over the hemisphere of microfacet normals (= 1 for any alpha).
+double	oracle_ggx_ndf_integral(double alpha, std::uint32_t n)
+{
+	double sum = 0.0;
+	for (std::uint32_t i = 0; i < n; i++)
+	{
+		const double z = (i + 0.5) / n;	// n.h, uniform in z; the integrand is symmetric in phi
+		sum += oracle::ggxD(z, alpha) * z;
+	}
+	return sum * (2.0 * oracle::PI) / static_cast<double>(n);
+}
+
+double	oracle_dielectric_fresnel(double cos_theta, double eta_i, double eta_t)
+{
+	return oracle::dielectricFresnel(cos_theta, eta_i, eta_t);
+}
+
+}

@@ -1,0 +1,66 @@
+"""ctypes access to the FP64 oracle (oracle/liboracle.so) — the checker, never the product."""
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            import subprocess
+            subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True)
+        lib = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        lib.oracle_sampler.argtypes = [ctypes.c_size_t, vp, vp, vp]
+        lib.oracle_sampler.restype = None
+        lib.oracle_render.argtypes = [ctypes.c_char_p] + [ctypes.c_uint32] * 7 + [vp, vp]
+        lib.oracle_render.restype = ctypes.c_int
+        lib.oracle_primary.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp]
+        lib.oracle_primary.restype = ctypes.c_int
+        lib.oracle_last_error.restype = ctypes.c_char_p
+        lib.oracle_ggx_pdf_integral.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint32]
+        lib.oracle_ggx_pdf_integral.restype = ctypes.c_double
+        lib.oracle_ggx_ndf_integral.argtypes = [ctypes.c_double, ctypes.c_uint32]
+        lib.oracle_ggx_ndf_integral.restype = ctypes.c_double
+        lib.oracle_dielectric_fresnel.argtypes = [ctypes.c_double] * 3
+        lib.oracle_dielectric_fresnel.restype = ctypes.c_double
+        _lib = lib
+    return _lib
+
+
+def sampler(keys):
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    n = keys.shape[0]
+    s1 = np.zeros(n, dtype=np.float64)
+    s2 = np.zeros((n, 2), dtype=np.float32)
+    load().oracle_sampler(n, keys.ctypes.data, s1.ctypes.data, s2.ctypes.data)
+    return s1, s2
+
+
+def render(path, seed, spp, width, height, sample_begin=0, max_bounces=None, threads=None):
+    lib = load()
+    out = np.zeros((height, width, 3), dtype=np.float64)
+    ms = ctypes.c_double()
+    threads = threads or min(16, os.cpu_count() or 1)
+    rc = lib.oracle_render(path.encode(), seed, spp, sample_begin, width, height,
+                           0 if max_bounces is None else max_bounces + 1, threads, out.ctypes.data, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return out, ms.value
+
+
+def primary(path, seed, width, height):
+    lib = load()
+    ids = np.zeros((height, width, 2), dtype=np.int64)
+    t = np.zeros((height, width), dtype=np.float64)
+    rc = lib.oracle_primary(path.encode(), seed, width, height, ids.ctypes.data, t.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return ids, t
