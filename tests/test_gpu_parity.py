@@ -60,7 +60,7 @@ def test_primary_hits_spheres(ctx, spheres):
     assert _primary_check(scene, "spheres_primary.npz", resolution=(240, 135)) <= 3
 
 
-@pytest.mark.parametrize("integrator", ["megakernel"])
+@pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
 def test_cornell_image_parity(ctx, cornell, integrator):
     from luz_amd import Renderer, SceneFile
     scene = SceneFile.read(ctx, cornell.scene_path)
@@ -72,7 +72,7 @@ def test_cornell_image_parity(ctx, cornell, integrator):
     assert e <= REL_MSE_BAR
 
 
-@pytest.mark.parametrize("integrator", ["megakernel"])
+@pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
 def test_spheres_image_parity(ctx, spheres, integrator):
     from luz_amd import Renderer, SceneFile
     scene = SceneFile.read(ctx, spheres.scene_path)
@@ -81,3 +81,22 @@ def test_spheres_image_parity(ctx, spheres, integrator):
     e = rel_mse(img, ref)
     print(f"spheres {integrator}: rel-MSE {e:.3e}, {stats.render_ms:.1f} ms")
     assert e <= REL_MSE_BAR
+
+
+def test_megakernel_and_wavefront_agree(ctx, cornell):
+    """Both integrators draw the same random numbers: images agree to fp32 noise."""
+    from luz_amd import Renderer, SceneFile
+    scene = SceneFile.read(ctx, cornell.scene_path)
+    a, _ = Renderer.render(scene, seed=7, samples=8, integrator="megakernel")
+    b, _ = Renderer.render(scene, seed=7, samples=8, integrator="wavefront")
+    assert rel_mse(b, a) < 1e-6
+
+
+def test_sample_range_sharding_composes(ctx, cornell):
+    """Rendering [0,4) and [4,8) separately and averaging equals rendering [0,8)."""
+    from luz_amd import Renderer, SceneFile
+    scene = SceneFile.read(ctx, cornell.scene_path)
+    full, _ = Renderer.render(scene, seed=3, samples=8, integrator="wavefront")
+    lo, _ = Renderer.render(scene, seed=3, samples=8, sample_begin=0, sample_count=4, integrator="wavefront")
+    hi, _ = Renderer.render(scene, seed=3, samples=8, sample_begin=4, sample_count=4, integrator="wavefront")
+    assert np.allclose((lo.astype(np.float64) + hi) / 2.0, full, rtol=1e-5, atol=1e-6)
