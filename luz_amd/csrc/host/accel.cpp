@@ -461,6 +461,91 @@ namespace luzgpu
 			node.count[slot] = count;
 		}
 
+		int	deviceDepth(const std::vector<BvhNode>& nodes, std::int32_t root)
+		{
+			if (root < 0) return 0;
+			const BvhNode& n = nodes[static_cast<std::size_t>(root)];
+			int d = 0;
+			for (int s = 0; s < 2; s++)
+			{
+				if (n.count[s] == 0) d = std::max(d, deviceDepth(nodes, n.child[s]));
+			}
+			return 1 + d;
+		}
+
+		struct	TlasEmitter
+		{
+			const std::vector<AABBd>&	boxes;
+			const std::vector<GpuHittable>&	hittables;
+			std::vector<BvhNode>&	nodes;
+			std::vector<std::uint32_t>&	prims;
+
+			struct	Ref
+			{
+				std::int32_t	child;
+				std::int32_t	count;	// 0: node, > 0: primitive range
+				AABBd	box;
+			};
+
+			Ref	group(std::vector<std::uint32_t> items)
+			{
+				AABBd box = boxes[items[0]];
+				bool instances = false;
+				for (std::uint32_t h : items)
+				{
+					box = merge(box, boxes[h]);
+					instances = instances || (hittables[h].kindIndex >> 30) == GPU_HIT_INSTANCE;
+				}
+				if (items.size() == 1 || (!instances && items.size() <= 4))
+				{
+					const std::int32_t start = static_cast<std::int32_t>(prims.size());
+					prims.insert(prims.end(), items.begin(), items.end());
+					return Ref{start, static_cast<std::int32_t>(items.size()), box};
+				}
+				AABBd cb;
+				for (std::size_t i = 0; i < items.size(); i++)
+				{
+					const dvec3 c = (boxes[items[i]].lo + boxes[items[i]].hi) * 0.5;
+					const AABBd cc{c, c};
+					cb = i == 0 ? cc : merge(cb, cc);
+				}
+				const int axis = largestAxis(cb);
+				const std::size_t mid = items.size() / 2;
+				std::nth_element(items.begin(), items.begin() + static_cast<std::ptrdiff_t>(mid), items.end(),
+					[&](std::uint32_t a, std::uint32_t b) {
+						const double ca = boxes[a].lo[axis] + boxes[a].hi[axis], cbv = boxes[b].lo[axis] + boxes[b].hi[axis];
+						return ca < cbv || (ca == cbv && a < b);
+					});
+				const std::size_t idx = nodes.size();
+				nodes.push_back(BvhNode{});
+				const Ref l = group(std::vector<std::uint32_t>(items.begin(), items.begin() + static_cast<std::ptrdiff_t>(mid)));
+				const Ref r = group(std::vector<std::uint32_t>(items.begin() + static_cast<std::ptrdiff_t>(mid), items.end()));
+				setChild(nodes[idx], 0, &l.box, l.child, l.count);
+				setChild(nodes[idx], 1, &r.box, r.child, r.count);
+				return Ref{static_cast<std::int32_t>(idx), 0, box};
+			}
+
+			Ref	fromReference(const TlasBuilder& tb, std::size_t node)
+			{
+				const TopNode& n = tb.nodes[node];
+				if (n.children.empty() || n.children[0] >= 0)
+				{
+					std::vector<std::uint32_t> items;
+					for (std::int64_t c : n.children) items.push_back(static_cast<std::uint32_t>(c));
+					Ref r = group(items);
+					r.box = n.box;
+					return r;
+				}
+				const std::size_t idx = nodes.size();
+				nodes.push_back(BvhNode{});
+				const Ref l = fromReference(tb, static_cast<std::size_t>(~n.children[0]));
+				const Ref r = fromReference(tb, static_cast<std::size_t>(~n.children[1]));
+				setChild(nodes[idx], 0, &l.box, l.child, l.count);
+				setChild(nodes[idx], 1, &r.box, r.child, r.count);
+				return Ref{static_cast<std::int32_t>(idx), 0, n.box};
+			}
+		};
+
 		GpuTri	packTriangle(const Triangle& t, std::int32_t prim)
 		{
 			GpuTri g{};
@@ -670,6 +755,7 @@ namespace luzgpu
 				}
 			}
 			meshRoot[m] = packBlas(blas, triBase, p.blasNodes);
+			p.blasDeviceDepth = std::max(p.blasDeviceDepth, deviceDepth(p.blasNodes, meshRoot[m]));
 			p.triangleCount += mesh.triangles.size();
 		}
 
@@ -787,64 +873,45 @@ namespace luzgpu
 			}
 		}
 		p.params.tlasRoot = -1;
-		if (items.size() == 1)
-		{
-			BvhNode root{};
-			setChild(root, 0, &boxes[items[0]], 0, 1);
-			setChild(root, 1, nullptr, 0, -1);
-			p.tlasPrims.push_back(items[0]);
-			p.tlasNodes.push_back(root);
-			p.params.tlasRoot = 0;
-			p.tlasTopology = std::to_string(items[0]);
-		}
-		else if (items.size() > 1)
+		if (!items.empty())
 		{
 			TlasBuilder tb{boxes, {}};
-			tb.build(items, 0, items.size(), true);
-			std::ostringstream os;
-			tb.dump(os, 0);
-			p.tlasTopology = os.str();
-			// internal reference nodes (two node children) -> device nodes; leaf nodes
-			// (hittable children only) -> primitive ranges
-			std::vector<std::int32_t> map(tb.nodes.size(), -1);
-			std::vector<std::int32_t> leafStart(tb.nodes.size(), -1);
-			std::int32_t next = 0;
-			for (std::size_t i = 0; i < tb.nodes.size(); i++)
+			if (items.size() == 1)
 			{
-				const bool leaf = tb.nodes[i].children.empty() || tb.nodes[i].children[0] >= 0;
-				if (!leaf)
-				{
-					map[i] = next++;
-				}
-				else
-				{
-					leafStart[i] = static_cast<std::int32_t>(p.tlasPrims.size());
-					for (std::int64_t c : tb.nodes[i].children) p.tlasPrims.push_back(static_cast<std::uint32_t>(c));
-				}
-			}
-			if (map[0] < 0)
-			{
-				BvhNode root{};
-				setChild(root, 0, &tb.nodes[0].box, leafStart[0], static_cast<std::int32_t>(tb.nodes[0].children.size()));
-				setChild(root, 1, nullptr, 0, -1);
-				p.tlasNodes.push_back(root);
+				// Scene.cpp:681-684: a single bounded hittable is the acceleration structure
+				TopNode leaf;
+				leaf.children = {items[0]};
+				leaf.childBoxes = {boxes[items[0]]};
+				leaf.box = boxes[items[0]];
+				tb.nodes.push_back(leaf);
+				p.tlasTopology = std::to_string(items[0]);
 			}
 			else
 			{
-				p.tlasNodes.resize(static_cast<std::size_t>(next));
-				for (std::size_t i = 0; i < tb.nodes.size(); i++)
-				{
-					if (map[i] < 0) continue;
-					BvhNode& d = p.tlasNodes[static_cast<std::size_t>(map[i])];
-					for (int s = 0; s < 2; s++)
-					{
-						const std::size_t c = static_cast<std::size_t>(~tb.nodes[i].children[s]);
-						if (map[c] >= 0) setChild(d, s, &tb.nodes[c].box, map[c], 0);
-						else setChild(d, s, &tb.nodes[c].box, leafStart[c], static_cast<std::int32_t>(tb.nodes[c].children.size()));
-					}
-				}
+				tb.build(items, 0, items.size(), true);
+				std::ostringstream os;
+				tb.dump(os, 0);
+				p.tlasTopology = os.str();
 			}
-			p.params.tlasRoot = 0;
+			// Device form: the reference's binary upper levels are kept node for node; each
+			// n-ary leaf (up to 16 hittables) is refined into a small binary tree so every
+			// mesh instance has its own box test before its mesh BVH is entered, and plain
+			// primitives sit in leaves of at most 4.
+			TlasEmitter em{boxes, p.hittables, p.tlasNodes, p.tlasPrims};
+			const TlasEmitter::Ref root = em.fromReference(tb, 0);
+			if (root.count > 0)
+			{
+				BvhNode r{};
+				setChild(r, 0, &root.box, root.child, root.count);
+				setChild(r, 1, nullptr, 0, -1);
+				p.tlasNodes.push_back(r);
+				p.params.tlasRoot = static_cast<std::int32_t>(p.tlasNodes.size() - 1);
+			}
+			else
+			{
+				p.params.tlasRoot = root.child;
+			}
+			p.tlasDeviceDepth = deviceDepth(p.tlasNodes, p.params.tlasRoot);
 		}
 
 		// ---- lights (Scene.cpp:615-638) -----------------------------------------------------

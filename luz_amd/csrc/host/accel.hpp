@@ -53,6 +53,8 @@ namespace luzgpu
 		std::string	tlasTopology;	// nested "(...)" form over top-level hittable indices
 		std::vector<Blas>	blas;		// reference-shaped mesh BVHs, for topology checks
 		std::size_t	triangleCount = 0;
+		int	tlasDeviceDepth = 0;	// device-layout node levels (leaves folded into parents)
+		int	blasDeviceDepth = 0;
 		Camera	camera;
 		double	metersPerUnit = 1.0;
 	};

@@ -275,23 +275,6 @@ namespace luzgpu
 			}
 		}
 
-		int	depthOf(const std::vector<BlasNode>& n, std::size_t i)
-		{
-			if (n.empty() || n[i].leaf) return 0;
-			return 1 + std::max(depthOf(n, n[i].left), depthOf(n, n[i].right));
-		}
-
-		int	stringDepth(const std::string& s)
-		{
-			int d = 0, m = 0;
-			for (char c : s)
-			{
-				if (c == '(') m = std::max(m, ++d);
-				if (c == ')') d--;
-			}
-			return m;
-		}
-
 		void	uploadScene(luzgpu_ctx* ctx, GpuScene& g)
 		{
 			PackedScene& h = g.host;
@@ -428,12 +411,12 @@ int	luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out)
 		const Scene scene = readSceneFile(path);
 		sc->gpu.host = packScene(scene);
 		const PackedScene& h = sc->gpu.host;
-		int blasDepth = 0;
-		for (const Blas& b : h.blas) blasDepth = std::max(blasDepth, depthOf(b.nodes, 0));
-		const int tlasDepth = stringDepth(h.tlasTopology);
-		// a near-first DFS holds at most one deferred entry per level, plus the instance
-		// resume / exit markers
-		if (tlasDepth + blasDepth + 3 > kStackDepth)
+		// The device traversal holds at most one deferred entry per node level of each
+		// BVH plus the instance exit marker; scenes that could exceed the LDS stack are
+		// refused rather than traversed incompletely.
+		const int tlasDepth = h.tlasDeviceDepth;
+		const int blasDepth = h.blasDeviceDepth;
+		if (tlasDepth + blasDepth + 2 > kStackDepth)
 		{
 			throw std::runtime_error("Unsupported for the device path: BVH depth " + std::to_string(tlasDepth) + "+"
 				+ std::to_string(blasDepth) + " exceeds the traversal stack");
