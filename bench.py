@@ -77,7 +77,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="spheres", choices=["spheres", "cornell"])
-    ap.add_argument("--integrator", default="megakernel", choices=["megakernel", "wavefront"])
+    ap.add_argument("--integrator", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=16)
@@ -118,40 +118,52 @@ def main():
     spp = scene.info["samples"]
     out = torch.empty(pixels * 3, dtype=torch.float32, device="cuda")
 
+    wavefront = args.integrator == "wavefront"
     # visit counts for the roofline (untimed, a 4-spp slice of the same workload)
     _, cst = Renderer.render(scene, seed=args.seed, sample_count=min(4, spp), integrator=args.integrator, out=out,
-                             count_visits=args.integrator == "megakernel")
-    rays = cst.closest_hit_rays + cst.shadow_rays
-    nodes_per_sample = cst.nodes_visited / max(cst.rendered_samples, 1)
-    prims_per_sample = cst.prims_tested / max(cst.rendered_samples, 1)
-    log(f"visit counts: {nodes_per_sample:.1f} nodes, {prims_per_sample:.1f} primitives, "
-        f"{rays / max(cst.rendered_samples, 1):.2f} rays per sample")
+                             count_visits=True)
+    log(f"visit counts: {cst.nodes_visited / max(cst.rendered_samples, 1):.1f} nodes, "
+        f"{cst.prims_tested / max(cst.rendered_samples, 1):.1f} primitives, "
+        f"{(cst.closest_hit_rays + cst.shadow_rays) / max(cst.rendered_samples, 1):.2f} rays per sample")
 
     for i in range(args.warmup):
-        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out)
+        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out, stage_times=wavefront)
         log(f"warmup {i}: {st.render_ms:.1f} ms")
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out)
+        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out, stage_times=wavefront)
         stats.append(st)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     for i, st in enumerate(stats):
-        log(f"step {i}: {st.render_ms:.1f} ms render, {st.kernel_ms:.1f} ms integrator kernel")
+        extra = (f", extend {st.extend_ms:.1f} / shade {st.shade_ms:.1f} / shadow {st.shadow_ms:.1f} ms"
+                 if wavefront else "")
+        log(f"step {i}: {st.render_ms:.1f} ms render, {st.kernel_ms:.1f} ms integrator kernels{extra}")
     samples = pixels * spp * args.steps
     value = samples / elapsed / 1e6
     rays_total = sum(s.closest_hit_rays + s.shadow_rays for s in stats)
-    launches = sum(s.kernel_launches for s in stats)
-    kernel_s = sum(s.kernel_ms for s in stats) / 1000.0
-    samples_per_launch = samples / max(launches, 1)
-    # algorithmic bytes per sample: BVH nodes (64 B) and primitives (48 B) fetched on the
-    # reference-order traversal, plus the 12 B fp32 colour written per sample
-    bytes_per_sample = nodes_per_sample * 64.0 + prims_per_sample * 48.0 + 12.0
-    bytes_per_launch = bytes_per_sample * samples_per_launch
+    if wavefront:
+        # dominant kernel: wf_extend (closest hit). Algorithmic bytes per closest-hit ray:
+        # ray in (2 x 16 B), hit out (16 B + 4 B), BVH nodes (64 B) and primitives (48 B)
+        # visited along the reference-order traversal.
+        kernel = "wf_extend"
+        rays_c = max(cst.closest_hit_rays, 1)
+        per_ray = 32.0 + 20.0 + cst.closest_nodes / rays_c * 64.0 + cst.closest_prims / rays_c * 48.0
+        closest = sum(s.closest_hit_rays for s in stats)
+        launches = sum(s.extend_launches for s in stats)
+        kernel_s = sum(s.extend_ms for s in stats) / 1000.0
+        bytes_per_launch = per_ray * closest / max(launches, 1)
+    else:
+        # dominant kernel: render_mega. Algorithmic bytes per sample: BVH nodes (64 B) and
+        # primitives (48 B) of every ray of the path, plus the 12 B colour written.
+        kernel = "render_mega"
+        per_sample = (cst.nodes_visited * 64.0 + cst.prims_tested * 48.0) / max(cst.rendered_samples, 1) + 12.0
+        launches = sum(s.kernel_launches for s in stats)
+        kernel_s = sum(s.kernel_ms for s in stats) / 1000.0
+        bytes_per_launch = per_sample * samples / max(launches, 1)
     achieved = bytes_per_launch / (kernel_s / max(launches, 1)) / 1e9
-    kernel = "render_mega" if args.integrator == "megakernel" else "wavefront_extend"
     traffic = load_traffic(wl.name, kernel)
     line = {
         "metric": "Msamples/s",

@@ -54,7 +54,7 @@ typedef struct luzgpu_settings
 	int32_t		width;           /* <= 0: scene value (--resolution, FlagsParser.cpp)                 */
 	int32_t		height;          /* <= 0: scene value                                                 */
 	uint32_t	count_visits;    /* 1: also count BVH nodes / primitives visited (slower)             */
-	uint32_t	reserved;
+	uint32_t	stage_times;     /* 1: time each wavefront stage with HIP events                      */
 } luzgpu_settings;
 
 typedef struct luzgpu_stats
@@ -67,7 +67,12 @@ typedef struct luzgpu_stats
 	uint64_t	nodes_visited;       /* with count_visits                                  */
 	uint64_t	prims_tested;        /* with count_visits                                  */
 	uint32_t	kernel_launches;
-	uint32_t	reserved;
+	uint32_t	extend_launches;     /* wavefront: closest-hit (extend) launches          */
+	uint64_t	closest_nodes;       /* with count_visits: nodes / primitives of closest-hit rays */
+	uint64_t	closest_prims;
+	double		extend_ms;           /* wavefront, with stage_times                        */
+	double		shade_ms;
+	double		shadow_ms;
 } luzgpu_stats;
 
 typedef struct luzgpu_scene_info

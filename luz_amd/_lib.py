@@ -41,7 +41,7 @@ class Settings(ctypes.Structure):
         ("width", ctypes.c_int32),
         ("height", ctypes.c_int32),
         ("count_visits", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("stage_times", ctypes.c_uint32),
     ]
 
 
@@ -55,7 +55,12 @@ class Stats(ctypes.Structure):
         ("nodes_visited", ctypes.c_uint64),
         ("prims_tested", ctypes.c_uint64),
         ("kernel_launches", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("extend_launches", ctypes.c_uint32),
+        ("closest_nodes", ctypes.c_uint64),
+        ("closest_prims", ctypes.c_uint64),
+        ("extend_ms", ctypes.c_double),
+        ("shade_ms", ctypes.c_double),
+        ("shadow_ms", ctypes.c_double),
     ]
 
 

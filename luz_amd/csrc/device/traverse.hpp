@@ -190,6 +190,34 @@ namespace luzgpu
 		std::uint32_t	prims;
 	};
 
+	// LDS stack of LdsDepth entries per thread, continued in a per-thread column of global
+	// memory (`spill`, stride = threads in the grid) for deeper entries. Keeping the LDS
+	// part short lets 8 waves per SIMD share a CU's LDS.
+	template <int LdsDepth, int Block>
+	struct	SpillStack
+	{
+		std::uint32_t*	base;
+		std::uint32_t*	spill;
+		std::uint32_t	stride;
+		int	sp;
+		int	capacity;
+
+		__device__ __forceinline__ void	push(std::uint32_t v)
+		{
+			if (sp < LdsDepth) base[sp * Block] = v;
+			else spill[static_cast<size_t>(min(sp, capacity - 1) - LdsDepth) * stride] = v;
+			sp++;
+		}
+		__device__ __forceinline__ std::uint32_t	pop()
+		{
+			sp--;
+			if (sp < LdsDepth) return base[sp * Block];
+			return spill[static_cast<size_t>(min(sp, capacity - 1) - LdsDepth) * stride];
+		}
+		__device__ __forceinline__ bool	empty() const { return sp == 0; }
+		__device__ __forceinline__ void	reset() { sp = 0; }
+	};
+
 	template <bool AnyHit, bool Count = false, typename Stack>
 	__device__ __forceinline__ bool	traverse(const SceneView& s, std::int32_t tlasRoot, const Ray& worldRay, float tMin, float tMax,
 		HitInfo& hit, Stack& stack, VisitCounter* counter = nullptr)

@@ -20,6 +20,7 @@
 #include "integrator.hpp"
 
 #include <stdexcept>
+#include <vector>
 
 namespace luzgpu
 {
@@ -51,11 +52,14 @@ namespace luzgpu
 		// counters: [b] rays in queue b, [kMaxBounce + 1 + b] shadow rays of bounce b
 		std::uint32_t*	counts;
 		std::uint32_t	capacity;
+		std::uint32_t*	spill;		// traversal stack spill, [entry][grid thread]
 	};
 
 	constexpr int	kWaveBlock = 256;
-	constexpr int	kWaveStack = 32;
+	constexpr int	kWaveLdsStack = 16;
+	constexpr int	kWaveStackCapacity = 64;
 	constexpr int	kMaxBounceSlots = 64;
+	constexpr unsigned	kWaveGrid = 2048;	// 256 CUs x 8 waves per SIMD x 4 SIMDs / 4 waves per block
 
 	__device__ __forceinline__ std::uint32_t	laneId() { return threadIdx.x & 63u; }
 
@@ -99,26 +103,48 @@ namespace luzgpu
 		}
 	}
 
+	// counters: [0] closest rays, [1] shadow rays, [2]/[3] closest nodes/primitives,
+	// [4]/[5] shadow nodes/primitives (the last four only when Count)
+	template <bool Count>
 	__global__ __launch_bounds__(kWaveBlock) void	wf_extend(SceneView s, GpuParams p, WaveBuffers w, int b,
 		unsigned long long* counters)
 	{
-		__shared__ std::uint32_t stackMem[kWaveStack * kWaveBlock];
-		LdsStack<kWaveStack, kWaveBlock> stack{stackMem + threadIdx.x, 0};
+		__shared__ std::uint32_t stackMem[kWaveLdsStack * kWaveBlock];
+		const std::uint32_t gtid = blockIdx.x * kWaveBlock + threadIdx.x;
+		SpillStack<kWaveLdsStack, kWaveBlock> stack{stackMem + threadIdx.x, w.spill + gtid, gridDim.x * kWaveBlock, 0,
+			kWaveStackCapacity};
 		const std::uint32_t n = w.counts[b];
 		const int q = b & 1;
 		std::uint32_t rays = 0;
-		for (std::uint32_t i = blockIdx.x * kWaveBlock + threadIdx.x; i < n; i += gridDim.x * kWaveBlock)
+		VisitCounter visits{0u, 0u};
+		for (std::uint32_t i = gtid; i < n; i += gridDim.x * kWaveBlock)
 		{
 			const float4 o = w.qOrigin[q][i], d = w.qDir[q][i];
 			const Ray r = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
 			HitInfo h;
-			const bool found = traverse<false>(s, p.tlasRoot, r, kTMinF, INFINITY, h, stack);
+			const bool found = traverse<false, Count>(s, p.tlasRoot, r, kTMinF, INFINITY, h, stack, &visits);
 			w.qHit[i] = make_float4(found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.hittable));
 			w.qTri[i] = h.tri;
 			rays++;
 		}
-		for (int off = 32; off > 0; off >>= 1) rays += __shfl_down(rays, off);
-		if (laneId() == 0 && rays) atomicAdd(counters + 0, static_cast<unsigned long long>(rays));
+		for (int off = 32; off > 0; off >>= 1)
+		{
+			rays += __shfl_down(rays, off);
+			if (Count)
+			{
+				visits.nodes += __shfl_down(visits.nodes, off);
+				visits.prims += __shfl_down(visits.prims, off);
+			}
+		}
+		if (laneId() == 0 && rays)
+		{
+			atomicAdd(counters + 0, static_cast<unsigned long long>(rays));
+			if (Count)
+			{
+				atomicAdd(counters + 2, static_cast<unsigned long long>(visits.nodes));
+				atomicAdd(counters + 3, static_cast<unsigned long long>(visits.prims));
+			}
+		}
 	}
 
 	__global__ __launch_bounds__(kWaveBlock) void	wf_shade(SceneView s, GpuParams p, WaveBuffers w, int b)
@@ -304,23 +330,44 @@ namespace luzgpu
 		}
 	}
 
+	template <bool Count>
 	__global__ __launch_bounds__(kWaveBlock) void	wf_shadow(SceneView s, GpuParams p, WaveBuffers w, int b,
 		unsigned long long* counters)
 	{
-		__shared__ std::uint32_t stackMem[kWaveStack * kWaveBlock];
-		LdsStack<kWaveStack, kWaveBlock> stack{stackMem + threadIdx.x, 0};
+		__shared__ std::uint32_t stackMem[kWaveLdsStack * kWaveBlock];
+		const std::uint32_t gtid = blockIdx.x * kWaveBlock + threadIdx.x;
+		SpillStack<kWaveLdsStack, kWaveBlock> stack{stackMem + threadIdx.x, w.spill + gtid, gridDim.x * kWaveBlock, 0,
+			kWaveStackCapacity};
 		const std::uint32_t n = w.counts[kMaxBounceSlots + b];
 		std::uint32_t rays = 0;
-		for (std::uint32_t i = blockIdx.x * kWaveBlock + threadIdx.x; i < n; i += gridDim.x * kWaveBlock)
+		VisitCounter visits{0u, 0u};
+		for (std::uint32_t i = gtid; i < n; i += gridDim.x * kWaveBlock)
 		{
 			const float4 o = w.sOrigin[i], d = w.sDir[i];
 			HitInfo h;
-			const bool occluded = traverse<true>(s, p.tlasRoot, makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z)), kTMinF, o.w, h, stack);
+			const bool occluded = traverse<true, Count>(s, p.tlasRoot, makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z)), kTMinF, o.w,
+				h, stack, &visits);
 			w.visible[__float_as_uint(d.w)] = occluded ? 0 : 1;
 			rays++;
 		}
-		for (int off = 32; off > 0; off >>= 1) rays += __shfl_down(rays, off);
-		if (laneId() == 0 && rays) atomicAdd(counters + 1, static_cast<unsigned long long>(rays));
+		for (int off = 32; off > 0; off >>= 1)
+		{
+			rays += __shfl_down(rays, off);
+			if (Count)
+			{
+				visits.nodes += __shfl_down(visits.nodes, off);
+				visits.prims += __shfl_down(visits.prims, off);
+			}
+		}
+		if (laneId() == 0 && rays)
+		{
+			atomicAdd(counters + 1, static_cast<unsigned long long>(rays));
+			if (Count)
+			{
+				atomicAdd(counters + 4, static_cast<unsigned long long>(visits.nodes));
+				atomicAdd(counters + 5, static_cast<unsigned long long>(visits.prims));
+			}
+		}
 	}
 
 	__global__ __launch_bounds__(kWaveBlock) void	wf_resolve(GpuParams p, WaveBuffers w, int b, float* __restrict__ sampleColors)
@@ -382,7 +429,9 @@ namespace luzgpu
 		void	ensure(std::uint32_t capacity)
 		{
 			const size_t n = capacity;
-			const size_t need = n * (16 * 2 * 2 + 4 * 2 + 16 + 4 + 16 * 3 + 32 + 2 + 32 * 2) + kMaxBounceSlots * 2 * 4 + 64 * 16;
+			const size_t spillBytes = static_cast<size_t>(kWaveStackCapacity - kWaveLdsStack) * kWaveGrid * kWaveBlock * 4;
+			const size_t need = n * (16 * 2 * 2 + 4 * 2 + 16 + 4 + 16 * 3 + 32 + 2 + 32 * 2) + kMaxBounceSlots * 2 * 4 + 64 * 16
+				+ spillBytes + 256;
 			if (need > bytes)
 			{
 				release();
@@ -411,12 +460,21 @@ namespace luzgpu
 			buf.sOrigin = reinterpret_cast<float4*>(take(32 * n));
 			buf.sDir = reinterpret_cast<float4*>(take(32 * n));
 			buf.counts = reinterpret_cast<std::uint32_t*>(take(kMaxBounceSlots * 2 * 4));
+			buf.spill = reinterpret_cast<std::uint32_t*>(take(spillBytes));
 			buf.capacity = capacity;
 		}
 	};
 
+	struct	StageTimes
+	{
+		double	extendMs = 0.0, shadeMs = 0.0, shadowMs = 0.0;
+		std::uint32_t	extendLaunches = 0;
+	};
+
+	// Enqueues one launch of the wavefront integrator. With `events`, each stage is
+	// bracketed by HIP events (on this stream) and the per-stage times are returned.
 	inline void	renderWavefront(WavefrontState& st, const SceneView& s, const GpuParams& p, float* sampleColors,
-		unsigned long long* counters, hipStream_t stream)
+		unsigned long long* counters, hipStream_t stream, bool count, StageTimes* times)
 	{
 		if (p.maxBounces + 2 > kMaxBounceSlots)
 		{
@@ -427,7 +485,14 @@ namespace luzgpu
 		const std::uint32_t perPass = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount, (16u << 20) / pixels));
 		st.ensure(pixels * perPass);
 		const WaveBuffers& w = st.buf;
-		const unsigned grid = 2048;
+		std::vector<hipEvent_t> ev;
+		auto mark = [&]() {
+			if (!times) return;
+			hipEvent_t e;
+			if (hipEventCreate(&e) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
+			if (hipEventRecord(e, stream) != hipSuccess) throw std::runtime_error("hipEventRecord failed");
+			ev.push_back(e);
+		};
 		for (std::uint32_t s0 = 0; s0 < p.sampleCount; s0 += perPass)
 		{
 			GpuParams q = p;
@@ -435,14 +500,36 @@ namespace luzgpu
 			q.sampleCount = std::min(perPass, p.sampleCount - s0);
 			float* colors = sampleColors + static_cast<size_t>(s0) * pixels * 3;
 			wf_zero_counts<<<1, 2 * kMaxBounceSlots, 0, stream>>>(w.counts, 2 * kMaxBounceSlots, pixels * q.sampleCount);
-			wf_raygen<<<grid, kWaveBlock, 0, stream>>>(q, w);
+			wf_raygen<<<kWaveGrid, kWaveBlock, 0, stream>>>(q, w);
 			for (int b = 0; b <= q.maxBounces; b++)
 			{
-				wf_extend<<<grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
-				wf_shade<<<grid, kWaveBlock, 0, stream>>>(s, q, w, b);
-				wf_shadow<<<grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
-				wf_resolve<<<grid, kWaveBlock, 0, stream>>>(q, w, b, colors);
+				mark();
+				if (count) wf_extend<true><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
+				else wf_extend<false><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
+				mark();
+				wf_shade<<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b);
+				mark();
+				if (count) wf_shadow<true><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
+				else wf_shadow<false><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
+				mark();
+				wf_resolve<<<kWaveGrid, kWaveBlock, 0, stream>>>(q, w, b, colors);
 			}
+		}
+		if (times)
+		{
+			if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("wavefront stage timing failed");
+			for (size_t k = 0; k + 3 < ev.size(); k += 4)
+			{
+				float a = 0, b2 = 0, c = 0;
+				(void)hipEventElapsedTime(&a, ev[k], ev[k + 1]);
+				(void)hipEventElapsedTime(&b2, ev[k + 1], ev[k + 2]);
+				(void)hipEventElapsedTime(&c, ev[k + 2], ev[k + 3]);
+				times->extendMs += a;
+				times->shadeMs += b2;
+				times->shadowMs += c;
+				times->extendLaunches++;
+			}
+			for (hipEvent_t e : ev) (void)hipEventDestroy(e);
 		}
 	}
 }

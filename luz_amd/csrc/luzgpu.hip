@@ -493,9 +493,9 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		const std::uint32_t pixels = p.width * p.height;
 		hipStream_t st = ctx->stream;
 		ctx->sums.reserve(static_cast<size_t>(pixels) * 3 * sizeof(double));
-		ctx->counters.reserve(4 * sizeof(unsigned long long));
+		ctx->counters.reserve(8 * sizeof(unsigned long long));
 		zero_doubles<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(ctx->sums.as<double>(), static_cast<size_t>(pixels) * 3);
-		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 4);
+		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
 		// samples per launch: bounded so the per-sample colour buffer stays within 4 GiB
 		const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount,
 			static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(pixels) * 12ull))));
@@ -504,6 +504,7 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		check(hipEventCreate(&e0), "hipEventCreate");
 		check(hipEventCreate(&e1), "hipEventCreate");
 		std::vector<hipEvent_t> ks, ke;
+		StageTimes stages;
 		check(hipEventRecord(e0, st), "hipEventRecord");
 		for (std::uint32_t done = 0; done < p.sampleCount; done += perLaunch)
 		{
@@ -531,7 +532,8 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			}
 			else
 			{
-				renderWavefront(ctx->wave, scene->gpu.view, q, ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>(), st);
+				renderWavefront(ctx->wave, scene->gpu.view, q, ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>(), st,
+					settings && settings->count_visits, settings && settings->stage_times ? &stages : nullptr);
 			}
 			check(hipGetLastError(), "render launch");
 			check(hipEventRecord(ke.back(), st), "hipEventRecord");
@@ -546,7 +548,7 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		}
 		resolve<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), dst, pixels, static_cast<double>(p.sampleCount));
 		check(hipGetLastError(), "resolve launch");
-		unsigned long long counters[4] = {0, 0, 0, 0};
+		unsigned long long counters[6] = {0, 0, 0, 0, 0, 0};
 		check(hipMemcpyAsync(counters, ctx->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, st), "counter readback");
 		if (dst != rgb_out)
 		{
@@ -574,8 +576,14 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			stats->render_ms = ms;
 			stats->kernel_ms = kernelMs;
 			stats->kernel_launches = static_cast<std::uint32_t>(ks.size());
-			stats->nodes_visited = counters[2];
-			stats->prims_tested = counters[3];
+			stats->nodes_visited = counters[2] + counters[4];
+			stats->prims_tested = counters[3] + counters[5];
+			stats->closest_nodes = counters[2];
+			stats->closest_prims = counters[3];
+			stats->extend_ms = stages.extendMs;
+			stats->shade_ms = stages.shadeMs;
+			stats->shadow_ms = stages.shadowMs;
+			stats->extend_launches = stages.extendLaunches;
 		}
 	});
 }

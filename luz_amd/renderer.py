@@ -66,6 +66,12 @@ class RenderStats:
     kernel_launches: int
     nodes_visited: int
     prims_tested: int
+    extend_launches: int = 0
+    closest_nodes: int = 0
+    closest_prims: int = 0
+    extend_ms: float = 0.0
+    shade_ms: float = 0.0
+    shadow_ms: float = 0.0
 
 
 class Scene:
@@ -127,9 +133,10 @@ class SceneFile:
 
 
 def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=False, resolution=None,
-              count_visits=False):
+              count_visits=False, stage_times=False):
     s = _lib.Settings()
     s.count_visits = 1 if count_visits else 0
+    s.stage_times = 1 if stage_times else 0
     if resolution is not None:
         s.width, s.height = int(resolution[0]), int(resolution[1])
     s.seed = int(seed) & 0xFFFFFFFF
@@ -147,11 +154,11 @@ class Renderer:
 
     @staticmethod
     def render(scene: Scene, seed: int, samples=None, max_bounces=None, sample_begin=0, sample_count=None,
-               integrator: str = "megakernel", out=None, resolution=None, count_visits=False):
+               integrator: str = "wavefront", out=None, resolution=None, count_visits=False, stage_times=False):
         """Returns (image[H, W, 3] float32, RenderStats). ``out`` may be a device tensor
         (torch, float32, contiguous, H*W*3) that receives the image without a host copy."""
         s = _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=out is not None,
-                      resolution=resolution, count_visits=count_visits)
+                      resolution=resolution, count_visits=count_visits, stage_times=stage_times)
         st = _lib.Stats()
         w, h = resolution if resolution is not None else (scene.width, scene.height)
         if out is not None:
@@ -163,7 +170,8 @@ class Renderer:
         _lib.check(scene.ctx._lib.luzgpu_render(scene.ctx.handle, scene._h, ctypes.byref(s), ctypes.c_void_p(ptr),
                                                 ctypes.byref(st)))
         return img, RenderStats(st.rendered_samples, st.closest_hit_rays, st.shadow_rays, st.render_ms, st.kernel_ms,
-                                st.kernel_launches, st.nodes_visited, st.prims_tested)
+                                st.kernel_launches, st.nodes_visited, st.prims_tested, st.extend_launches,
+                                st.closest_nodes, st.closest_prims, st.extend_ms, st.shade_ms, st.shadow_ms)
 
     @staticmethod
     def primary_hits(scene: Scene, seed: int, resolution=None):
