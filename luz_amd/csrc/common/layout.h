@@ -24,6 +24,22 @@ namespace luzgpu
 	};
 	static_assert(sizeof(BvhNode) == 64, "BvhNode must be 64 bytes");
 
+	// 4-wide node in one 64-byte line. Child boxes are quantised to 8 bits per plane
+	// against the node's own box: plane = fma(q, 2^exp, origin), with q chosen on the host so
+	// that the decoded fp32 box contains the child's box (conservative by construction).
+	struct alignas(64) Bvh4Node
+	{
+		float	origin[3];
+		std::int8_t	exp[3];
+		std::uint8_t	pad0;
+		std::uint8_t	qlo[3][4];	// [axis][child]
+		std::uint8_t	qhi[3][4];
+		std::int32_t	child[4];	// count == 0: node index; count 1..16: first primitive
+		std::uint8_t	count[4];	// 0 internal, 1..16 leaf, 255 empty
+		std::uint32_t	pad1;
+	};
+	static_assert(sizeof(Bvh4Node) == 64, "Bvh4Node must be 64 bytes");
+
 	struct alignas(16) GpuTri
 	{
 		float	v0[3];

@@ -24,8 +24,8 @@ namespace luzgpu
 
 	struct	SceneView
 	{
-		const BvhNode*	tlas;
-		const BvhNode*	blas;
+		const Bvh4Node*	tlas;
+		const Bvh4Node*	blas;
 		const GpuTri*	topTris;
 		const GpuTri*	meshTris;
 		const float*	meshNormals;
@@ -235,7 +235,7 @@ namespace luzgpu
 		Ray ray = worldRay;
 		bool inInstance = false;
 		std::uint32_t instHittable = 0;
-		const BvhNode* nodes = s.tlas;
+		const Bvh4Node* nodes = s.tlas;
 		float closest = tMax;
 		bool found = false;
 		std::uint32_t entry = static_cast<std::uint32_t>(tlasRoot);
@@ -249,23 +249,50 @@ namespace luzgpu
 			}
 			else if ((entry & kLeafBit) == 0)
 			{
-				const BvhNode& n = nodes[entry];
+				const Bvh4Node& n = nodes[entry];
 				if (Count) counter->nodes++;
-				float t0, t1;
-				const bool h0 = n.count[0] >= 0 && boxHit(ray, n.lo0, n.hi0, closest, t0);
-				const bool h1 = n.count[1] >= 0 && boxHit(ray, n.lo1, n.hi1, closest, t1);
-				const std::uint32_t e0 = n.count[0] > 0 ? leafEntry(n.child[0], n.count[0]) : static_cast<std::uint32_t>(n.child[0]);
-				const std::uint32_t e1 = n.count[1] > 0 ? leafEntry(n.child[1], n.count[1]) : static_cast<std::uint32_t>(n.child[1]);
-				if (h0 && h1)
+				// decode the four child boxes (plane = fma(q, 2^exp, origin)) and test them
+				float sc[3];
+				for (int a = 0; a < 3; a++) sc[a] = __int_as_float((static_cast<int>(n.exp[a]) + 127) << 23);
+				std::uint32_t e[4];
+				float tn[4];
+				int hits = 0;
+				for (int k = 0; k < 4; k++)
 				{
-					const bool firstNear = t0 <= t1;
-					stack.push(firstNear ? e1 : e0);
-					entry = firstNear ? e0 : e1;
-					continue;
+					if (n.count[k] == 255) continue;
+					float lo[3], hi[3];
+					for (int a = 0; a < 3; a++)
+					{
+						lo[a] = fmaf(static_cast<float>(n.qlo[a][k]), sc[a], n.origin[a]);
+						hi[a] = fmaf(static_cast<float>(n.qhi[a][k]), sc[a], n.origin[a]);
+					}
+					float t;
+					if (boxHit(ray, lo, hi, closest, t))
+					{
+						tn[hits] = t;
+						e[hits] = n.count[k] > 0 ? leafEntry(n.child[k], n.count[k]) : static_cast<std::uint32_t>(n.child[k]);
+						hits++;
+					}
 				}
-				if (h0 || h1)
+				if (hits > 0)
 				{
-					entry = h0 ? e0 : e1;
+					// order by entry distance (insertion sort of <= 4), push far to near
+					for (int i = 1; i < hits; i++)
+					{
+						const float kt = tn[i];
+						const std::uint32_t ke = e[i];
+						int j = i - 1;
+						while (j >= 0 && tn[j] > kt)
+						{
+							tn[j + 1] = tn[j];
+							e[j + 1] = e[j];
+							j--;
+						}
+						tn[j + 1] = kt;
+						e[j + 1] = ke;
+					}
+					for (int i = hits - 1; i > 0; i--) stack.push(e[i]);
+					entry = e[0];
 					continue;
 				}
 			}

@@ -546,6 +546,152 @@ namespace luzgpu
 			}
 		};
 
+		// ---- 4-wide device nodes --------------------------------------------------------------
+		struct	WideSlot
+		{
+			float	lo[3], hi[3];
+			std::int32_t	child;
+			std::int32_t	count;	// 0: binary node index (internal), > 0 leaf range
+		};
+
+		float	slotArea(const WideSlot& s)
+		{
+			const float x = s.hi[0] - s.lo[0], y = s.hi[1] - s.lo[1], z = s.hi[2] - s.lo[2];
+			return x * y + x * z + y * z;
+		}
+
+		void	encodeWide(Bvh4Node& n, const std::vector<WideSlot>& slots)
+		{
+			float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+			for (const WideSlot& s : slots)
+				for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], s.lo[a]); hi[a] = std::max(hi[a], s.hi[a]); }
+			for (int a = 0; a < 3; a++)
+			{
+				n.origin[a] = lo[a];
+				const double extent = static_cast<double>(hi[a]) - static_cast<double>(lo[a]);
+				int e = extent > 0.0 ? static_cast<int>(std::ceil(std::log2(extent / 255.0))) : -100;
+				e = std::max(e, -100);
+				for (;; e++)
+				{
+					if (e > 127) throw std::runtime_error("BVH box too large to quantise");
+					const float scale = std::ldexp(1.0f, e);
+					bool ok = true;
+					for (std::size_t k = 0; k < slots.size() && ok; k++)
+					{
+						long ql = static_cast<long>(std::floor((static_cast<double>(slots[k].lo[a]) - lo[a]) / scale));
+						long qh = static_cast<long>(std::ceil((static_cast<double>(slots[k].hi[a]) - lo[a]) / scale));
+						ql = std::max(0L, std::min(255L, ql));
+						qh = std::max(0L, std::min(255L, qh));
+						while (ql > 0 && std::fma(static_cast<float>(ql), scale, lo[a]) > slots[k].lo[a]) ql--;
+						while (qh < 255 && std::fma(static_cast<float>(qh), scale, lo[a]) < slots[k].hi[a]) qh++;
+						if (std::fma(static_cast<float>(ql), scale, lo[a]) > slots[k].lo[a]
+							|| std::fma(static_cast<float>(qh), scale, lo[a]) < slots[k].hi[a])
+						{
+							ok = false;
+							break;
+						}
+						n.qlo[a][k] = static_cast<std::uint8_t>(ql);
+						n.qhi[a][k] = static_cast<std::uint8_t>(qh);
+					}
+					if (ok)
+					{
+						n.exp[a] = static_cast<std::int8_t>(e);
+						break;
+					}
+				}
+			}
+			for (std::size_t k = 0; k < 4; k++)
+			{
+				if (k < slots.size())
+				{
+					n.child[k] = slots[k].child;
+					n.count[k] = static_cast<std::uint8_t>(slots[k].count);
+				}
+				else
+				{
+					n.child[k] = 0;
+					n.count[k] = 255;
+					for (int a = 0; a < 3; a++) { n.qlo[a][k] = 255; n.qhi[a][k] = 0; }
+				}
+			}
+		}
+
+		// Collapses the binary device tree below `node` into 4-wide nodes (largest-area
+		// internal child opened first). Returns the wide index of the new node.
+		std::int32_t	widen(const std::vector<BvhNode>& bin, std::int32_t node, std::vector<Bvh4Node>& out)
+		{
+			std::vector<WideSlot> slots;
+			auto addChildren = [&](std::int32_t b) {
+				const BvhNode& n = bin[static_cast<std::size_t>(b)];
+				for (int s = 0; s < 2; s++)
+				{
+					if (n.count[s] < 0) continue;
+					WideSlot w;
+					const float* lo = s == 0 ? n.lo0 : n.lo1;
+					const float* hi = s == 0 ? n.hi0 : n.hi1;
+					for (int a = 0; a < 3; a++) { w.lo[a] = lo[a]; w.hi[a] = hi[a]; }
+					w.child = n.child[s];
+					w.count = n.count[s];
+					slots.push_back(w);
+				}
+			};
+			addChildren(node);
+			while (slots.size() < 4)
+			{
+				int best = -1;
+				float bestArea = -1.0f;
+				for (std::size_t k = 0; k < slots.size(); k++)
+				{
+					if (slots[k].count == 0 && slotArea(slots[k]) > bestArea)
+					{
+						best = static_cast<int>(k);
+						bestArea = slotArea(slots[k]);
+					}
+				}
+				if (best < 0) break;
+				const BvhNode& open = bin[static_cast<std::size_t>(slots[static_cast<std::size_t>(best)].child)];
+				const int kids = (open.count[0] >= 0 ? 1 : 0) + (open.count[1] >= 0 ? 1 : 0);
+				if (slots.size() - 1 + static_cast<std::size_t>(kids) > 4) break;
+				const std::int32_t b = slots[static_cast<std::size_t>(best)].child;
+				slots.erase(slots.begin() + best);
+				addChildren(b);
+			}
+			const std::size_t idx = out.size();
+			out.push_back(Bvh4Node{});
+			for (WideSlot& w : slots)
+			{
+				if (w.count == 0) w.child = widen(bin, w.child, out);
+			}
+			encodeWide(out[idx], slots);
+			return static_cast<std::int32_t>(idx);
+		}
+
+		// Worst-case traversal stack entries below `root`: a node defers every hit child but
+		// the nearest, so the bound is the largest sum of (children - 1) along a root-to-leaf path.
+		int	wideStackBound(const std::vector<Bvh4Node>& nodes, std::int32_t root)
+		{
+			const Bvh4Node& n = nodes[static_cast<std::size_t>(root)];
+			int valid = 0, below = 0;
+			for (int k = 0; k < 4; k++)
+			{
+				if (n.count[k] == 255) continue;
+				valid++;
+				if (n.count[k] == 0) below = std::max(below, wideStackBound(nodes, n.child[k]));
+			}
+			return std::max(0, valid - 1) + below;
+		}
+
+		int	wideDepth(const std::vector<Bvh4Node>& nodes, std::int32_t root)
+		{
+			const Bvh4Node& n = nodes[static_cast<std::size_t>(root)];
+			int d = 0;
+			for (int k = 0; k < 4; k++)
+			{
+				if (n.count[k] == 0) d = std::max(d, wideDepth(nodes, n.child[k]));
+			}
+			return 1 + d;
+		}
+
 		GpuTri	packTriangle(const Triangle& t, std::int32_t prim)
 		{
 			GpuTri g{};
@@ -754,8 +900,10 @@ namespace luzgpu
 							p.meshNormals.push_back(t.hasNormals ? static_cast<float>((*ns[v])[i]) : 0.0f);
 				}
 			}
-			meshRoot[m] = packBlas(blas, triBase, p.blasNodes);
-			p.blasDeviceDepth = std::max(p.blasDeviceDepth, deviceDepth(p.blasNodes, meshRoot[m]));
+			const std::int32_t binaryRoot = packBlas(blas, triBase, p.blasNodes);
+			meshRoot[m] = widen(p.blasNodes, binaryRoot, p.blasWide);
+			p.blasDeviceDepth = std::max(p.blasDeviceDepth, wideDepth(p.blasWide, meshRoot[m]));
+			p.blasStackBound = std::max(p.blasStackBound, wideStackBound(p.blasWide, meshRoot[m]));
 			p.triangleCount += mesh.triangles.size();
 		}
 
@@ -911,7 +1059,9 @@ namespace luzgpu
 			{
 				p.params.tlasRoot = root.child;
 			}
-			p.tlasDeviceDepth = deviceDepth(p.tlasNodes, p.params.tlasRoot);
+			p.params.tlasRoot = widen(p.tlasNodes, p.params.tlasRoot, p.tlasWide);
+			p.tlasDeviceDepth = wideDepth(p.tlasWide, p.params.tlasRoot);
+			p.tlasStackBound = wideStackBound(p.tlasWide, p.params.tlasRoot);
 		}
 
 		// ---- lights (Scene.cpp:615-638) -----------------------------------------------------

@@ -33,8 +33,10 @@ namespace luzgpu
 
 	struct	PackedScene
 	{
-		std::vector<BvhNode>	tlasNodes;
+		std::vector<BvhNode>	tlasNodes;	// binary device form (intermediate)
 		std::vector<BvhNode>	blasNodes;
+		std::vector<Bvh4Node>	tlasWide;	// what the kernels traverse
+		std::vector<Bvh4Node>	blasWide;
 		std::vector<GpuTri>	topTris;
 		std::vector<GpuTri>	meshTris;
 		std::vector<float>	meshNormals;	// 9 floats per mesh triangle slot (leaf order)
@@ -53,8 +55,10 @@ namespace luzgpu
 		std::string	tlasTopology;	// nested "(...)" form over top-level hittable indices
 		std::vector<Blas>	blas;		// reference-shaped mesh BVHs, for topology checks
 		std::size_t	triangleCount = 0;
-		int	tlasDeviceDepth = 0;	// device-layout node levels (leaves folded into parents)
+		int	tlasDeviceDepth = 0;	// device-layout (4-wide) node levels
 		int	blasDeviceDepth = 0;
+		int	tlasStackBound = 0;	// worst-case deferred traversal entries
+		int	blasStackBound = 0;
 		Camera	camera;
 		double	metersPerUnit = 1.0;
 	};

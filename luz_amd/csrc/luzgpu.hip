@@ -89,7 +89,7 @@ namespace luzgpu
 	};
 
 	constexpr int	kMegaBlock = 128;
-	constexpr int	kStackDepth = 32;
+	constexpr int	kStackDepth = 48;
 
 	// counters: [0] closest-hit rays, [1] shadow rays, [2] nodes visited, [3] primitives tested
 	template <bool Count>
@@ -279,8 +279,8 @@ namespace luzgpu
 		{
 			PackedScene& h = g.host;
 			hipStream_t st = ctx->stream;
-			g.tlas.upload(h.tlasNodes, st);
-			g.blas.upload(h.blasNodes, st);
+			g.tlas.upload(h.tlasWide, st);
+			g.blas.upload(h.blasWide, st);
 			g.topTris.upload(h.topTris, st);
 			g.meshTris.upload(h.meshTris, st);
 			g.meshNormals.upload(h.meshNormals, st);
@@ -297,8 +297,8 @@ namespace luzgpu
 			g.unbounded.upload(h.unbounded, st);
 			check(hipStreamSynchronize(st), "scene upload");
 			SceneView& v = g.view;
-			v.tlas = g.tlas.as<BvhNode>();
-			v.blas = g.blas.as<BvhNode>();
+			v.tlas = g.tlas.as<Bvh4Node>();
+			v.blas = g.blas.as<Bvh4Node>();
 			v.topTris = g.topTris.as<GpuTri>();
 			v.meshTris = g.meshTris.as<GpuTri>();
 			v.meshNormals = g.meshNormals.as<float>();
@@ -411,15 +411,16 @@ int	luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out)
 		const Scene scene = readSceneFile(path);
 		sc->gpu.host = packScene(scene);
 		const PackedScene& h = sc->gpu.host;
-		// The device traversal holds at most one deferred entry per node level of each
-		// BVH plus the instance exit marker; scenes that could exceed the LDS stack are
-		// refused rather than traversed incompletely.
+		// Worst-case stack use: deferred TLAS entries, a leaf-resume entry and the instance
+		// exit marker, then deferred mesh-BVH entries. Scenes that could exceed the smaller
+		// of the two kernels' stacks are refused rather than traversed incompletely.
 		const int tlasDepth = h.tlasDeviceDepth;
 		const int blasDepth = h.blasDeviceDepth;
-		if (tlasDepth + blasDepth + 2 > kStackDepth)
+		const int bound = h.tlasStackBound + 2 + h.blasStackBound;
+		if (bound > kStackDepth)
 		{
-			throw std::runtime_error("Unsupported for the device path: BVH depth " + std::to_string(tlasDepth) + "+"
-				+ std::to_string(blasDepth) + " exceeds the traversal stack");
+			throw std::runtime_error("Unsupported for the device path: traversal stack bound " + std::to_string(bound)
+				+ " exceeds " + std::to_string(kStackDepth));
 		}
 		if (ctx)
 		{
@@ -435,8 +436,8 @@ int	luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out)
 		info.meshes = static_cast<std::uint32_t>(h.blas.size());
 		info.instances = static_cast<std::uint32_t>(h.instances.size());
 		info.lights = static_cast<std::uint32_t>(h.lights.size());
-		info.tlas_nodes = static_cast<std::uint32_t>(h.tlasNodes.size());
-		info.blas_nodes = static_cast<std::uint32_t>(h.blasNodes.size());
+		info.tlas_nodes = static_cast<std::uint32_t>(h.tlasWide.size());
+		info.blas_nodes = static_cast<std::uint32_t>(h.blasWide.size());
 		info.env_width = h.params.envWidth;
 		info.env_height = h.params.envHeight;
 		info.tlas_depth = static_cast<std::uint32_t>(tlasDepth);
