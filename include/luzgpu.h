@@ -55,6 +55,10 @@ typedef struct luzgpu_settings
 	int32_t		height;          /* <= 0: scene value                                                 */
 	uint32_t	count_visits;    /* 1: also count BVH nodes / primitives visited (slower)             */
 	uint32_t	stage_times;     /* 1: time each wavefront stage with HIP events                      */
+	/* wavefront launch shape (0 = built-in default); no effect on the image */
+	uint32_t	traversal_grid;  /* persistent blocks per traversal launch, 1..2048                   */
+	int32_t		extend_refill;   /* idle lanes (1..64) at which a wave takes new closest-hit rays     */
+	int32_t		shadow_refill;   /* idle lanes (1..64) at which a wave takes new shadow rays          */
 } luzgpu_settings;
 
 typedef struct luzgpu_stats

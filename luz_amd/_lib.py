@@ -42,6 +42,9 @@ class Settings(ctypes.Structure):
         ("height", ctypes.c_int32),
         ("count_visits", ctypes.c_uint32),
         ("stage_times", ctypes.c_uint32),
+        ("traversal_grid", ctypes.c_uint32),
+        ("extend_refill", ctypes.c_int32),
+        ("shadow_refill", ctypes.c_int32),
     ]
 
 

@@ -6,7 +6,7 @@
 //                   are folded into their parent's reference (count > 0).
 //   GpuTri    48 B  v0 | e1 | e2 in fp32 (e1, e2 formed in FP64, then rounded), with
 //                   the triangle's index in its mesh (after degenerate removal) in v0.w.
-//   Hittable  8 B   kind:2 | index:30, material id — the reference's top-level list.
+//   Hittable  8 B   kind:4 | index:28, material id — the reference's top-level list.
 #pragma once
 
 #include <cstdint>
@@ -56,12 +56,45 @@ namespace luzgpu
 		GPU_HIT_TRIANGLE = 0,
 		GPU_HIT_RECTANGLE = 1,
 		GPU_HIT_INSTANCE = 2,
+		GPU_HIT_SPHERE = 3,
+		GPU_HIT_CUBE = 4,		// six GpuRect faces at cubeFaces[6 * index]
+		GPU_HIT_VOLUME = 5,
+		GPU_HIT_PLANE = 6,		// unbounded: tested after the BVH
+		GPU_HIT_DIRECTIONAL = 7,	// unbounded light, never hit
 	};
+	constexpr std::uint32_t	kKindShift = 28;
+	constexpr std::uint32_t	kIndexMask = (1u << kKindShift) - 1u;
 
 	struct GpuHittable
 	{
-		std::uint32_t	kindIndex;	// kind << 30 | index
+		std::uint32_t	kindIndex;	// kind << 28 | index
 		std::uint32_t	material;
+	};
+
+	struct alignas(16) GpuSphere
+	{
+		float	center[3];
+		float	radius;
+	};
+
+	// Plane.cpp: plane through (0, y, 0) with the (unnormalised) orientation as normal.
+	struct alignas(16) GpuPlane
+	{
+		float	point[3];
+		float	pad0;
+		float	orientation[3];
+		float	pad1;
+	};
+
+	// ConstantVolume.cpp: boundary (sphere: center/radius; box: lo/hi) and -1/density.
+	struct alignas(16) GpuVolume
+	{
+		float	center[3];
+		float	radius;
+		float	lo[3];
+		float	negInvDensity;
+		float	hi[3];
+		std::uint32_t	sphere;
 	};
 
 	struct alignas(16) GpuRect
@@ -78,15 +111,19 @@ namespace luzgpu
 		float	pad[3];
 	};
 
+	// Traversal reads the first 64 bytes (the 3x4 map and the mesh links) in one group of
+	// 16-byte loads; shading reads the normal map.
 	struct alignas(16) GpuInstance
 	{
 		float	worldToLocal[12];	// rows of a 3x4 affine map (inverse rotate, inverse scale)
-		float	normalToWorld[9];	// rotate(n / scale), renormalised after use
-		std::int32_t	blasRoot;	// global BvhNode index of the mesh's root
+		std::int32_t	blasRoot;	// global Bvh4Node index of the mesh's root
 		std::int32_t	triBase;	// first GpuTri of the mesh
 		std::int32_t	normalBase;	// first vertex-normal triple, -1 if the mesh has none
-		float	pad[2];
+		std::int32_t	pad0;
+		float	normalToWorld[9];	// rotate(n / scale), renormalised after use
+		float	pad[3];
 	};
+	static_assert(sizeof(GpuInstance) == 112, "GpuInstance must be 112 bytes");
 
 	enum : std::uint32_t
 	{
@@ -94,6 +131,8 @@ namespace luzgpu
 		GPU_MAT_METAL = 2,
 		GPU_MAT_DIELECTRIC = 3,
 		GPU_MAT_EMISSIVE = 4,
+		GPU_MAT_ISOTROPIC = 5,
+		GPU_MAT_HG = 6,
 	};
 
 	struct alignas(16) GpuMaterial
@@ -105,7 +144,7 @@ namespace luzgpu
 		float	ior;
 		std::uint32_t	flags;	// bit0 conductor Fresnel, bit1 medium absorption
 		float	eta[3];
-		float	pad0;
+		float	anisotropy;		// Henyey-Greenstein g
 		float	k[3];
 		float	pad1;
 		float	absorption[3];
@@ -120,14 +159,14 @@ namespace luzgpu
 
 	struct alignas(16) GpuLight
 	{
-		std::uint32_t	kind;		// GPU_HIT_TRIANGLE / GPU_HIT_RECTANGLE
-		std::uint32_t	index;		// into top-level triangles / rectangles
+		std::uint32_t	kind;		// GPU_HIT_*: triangle, rectangle, sphere, directional; others never sample
+		std::uint32_t	index;		// into top-level triangles / rectangles / spheres
 		std::uint32_t	hittable;
 		std::uint32_t	material;
 		float	emitted[3];
 		float	area;
 		float	selectionProbability;	// weight / total (or 1/n when unweighted)
-		float	pad[3];
+		float	direction[3];		// directional: the sampled direction (minus the light's)
 	};
 
 	// Everything the per-pixel camera needs, in FP64 exactly as the reference's
@@ -175,6 +214,7 @@ namespace luzgpu
 		std::uint32_t	hasEnvironment;
 		double	envTotalWeight;
 		float	metersPerUnit;
-		std::uint32_t	unboundedCount;	// rectangles without a box: tested linearly
+		std::uint32_t	unboundedCount;	// hittables without a box (planes), tested after the BVH
+		std::uint32_t	hasVolumes;		// rays carry a volume free-flight sample
 	};
 }

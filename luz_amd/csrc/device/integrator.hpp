@@ -154,6 +154,59 @@ namespace luzgpu
 		return Onb(sf.normal).local(r * cp, r * sp, z);
 	}
 
+	// Sampler::sphereDirection (Sampler.cpp:249-257)
+	__device__ __forceinline__ f3	sphereDirection(const PixelSampler& ps)
+	{
+		float sx, sy;
+		ps.sample2D(DIM_BSDF_DIRECTION, sx, sy);
+		const float z = 1.0f - (2.0f * sy);
+		const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+		const float phi = (2.0f * kPiF) * sx;
+		return mk(r * cosf(phi), r * sinf(phi), z);
+	}
+
+	// henyeyGreensteinDirection (ColorHelper.cpp:115-148)
+	__device__ __forceinline__ f3	hgDirection(const Scatter& sc, const PixelSampler& ps)
+	{
+		const float pl2 = len2(sc.cosineW);
+		if (pl2 <= 0.0f || !isfinite(pl2)) return sphereDirection(ps);
+		const f3 pd = sc.cosineW * (1.0f / sqrtf(pl2));
+		const float g = fmaxf(-0.99f, fminf(0.99f, sc.g));
+		float sx, sy;
+		ps.sample2D(DIM_BSDF_DIRECTION, sx, sy);
+		const float phi = 2.0f * kPiF * sx;
+		float c;
+		if (fabsf(g) < 1e-3f)
+		{
+			c = 1.0f - (2.0f * sy);
+		}
+		else
+		{
+			const float rm = (1.0f - (g * g)) / (1.0f - g + (2.0f * g * sy));
+			c = (1.0f + (g * g) - (rm * rm)) / (2.0f * g);
+			c = fmaxf(-1.0f, fminf(1.0f, c));
+		}
+		const float st = sqrtf(fmaxf(0.0f, 1.0f - (c * c)));
+		return Onb(pd).local(st * cosf(phi), st * sinf(phi), c);
+	}
+
+	// scatterPDFGenerate (ColorHelper.cpp:150-166)
+	__device__ __forceinline__ f3	scatterDirection(const Surface& sf, const Scatter& sc, const PixelSampler& ps)
+	{
+		if (sc.pdfType == PDF_COSINE) return cosineDirection(sf, ps);
+		if (sc.pdfType == PDF_SPHERE) return sphereDirection(ps);
+		if (sc.pdfType == PDF_HG) return hgDirection(sc, ps);
+		return sc.direction;
+	}
+
+	// log(max(u, 1e-12)) of the bounce's DIM_VOLUME_DISTANCE sample: every ConstantVolume
+	// test of this bounce's rays (closest hit and shadow) draws the same value
+	// (ConstantVolume.cpp:96).
+	__device__ __forceinline__ float	volumeLogU(const GpuParams& p, const PixelSampler& ps)
+	{
+		return p.hasVolumes ? static_cast<float>(log(fmax(ps.sample1D(DIM_VOLUME_DISTANCE), 1e-12))) : 0.0f;
+	}
+
 	__device__ __forceinline__ f3	cleanColor(f3 c)
 	{
 		return mk(isfinite(c.x) ? c.x : 0.0f, isfinite(c.y) ? c.y : 0.0f, isfinite(c.z) ? c.z : 0.0f);
@@ -174,9 +227,10 @@ namespace luzgpu
 		for (int b = 0; b <= p.maxBounces; b++)
 		{
 			ps.bounce = static_cast<std::uint32_t>(b);
+			const float logU = volumeLogU(p, ps);
 			HitInfo h;
 			closestRays++;
-			if (!traverse<false, Count>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack, &visits))
+			if (!traverse<false, Count>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack, &visits, logU))
 			{
 				float w = 1.0f;
 				if (b > 0 && !prevSpecular && prevInfinite)
@@ -228,7 +282,8 @@ namespace luzgpu
 					{
 						HitInfo sh;
 						shadowRays++;
-						occluded = traverse<true, Count>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, c.tMax, sh, stack, &visits);
+						occluded = traverse<true, Count>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, c.tMax, sh, stack, &visits,
+							logU);
 					}
 					if (!occluded) acc = acc + clampRay(thr * c.contribution);
 				}
@@ -240,13 +295,13 @@ namespace luzgpu
 				{
 					HitInfo sh;
 					shadowRays++;
-					if (!traverse<true, Count>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack, &visits))
+					if (!traverse<true, Count>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack, &visits, logU))
 					{
 						acc = acc + clampRay(thr * c.contribution);
 					}
 				}
 			}
-			const f3 dir = sc.pdfType == PDF_COSINE ? cosineDirection(sf, ps) : sc.direction;
+			const f3 dir = scatterDirection(sf, sc, ps);
 			if (!leavesOpaque(sf, m.kind, dir)) return acc;
 			const float pdf = scatterPdf(m, sf, sc, dir);
 			if (pdf <= 0.0f || !isfinite(pdf)) return acc;

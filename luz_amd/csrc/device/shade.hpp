@@ -7,6 +7,10 @@
 //   Lambertian                        Lambertian.cpp:15-36
 //   Metal (Schlick / conductor, GGX)  Metal.cpp:126-221, BSDF.cpp:49-315
 //   Dielectric (smooth / rough GGX)   Dielectric.cpp:127-250, BSDF.cpp:93-125,317-458
+//   Isotropic / Henyey-Greenstein     Isotropic.cpp:15-24, HenyeyGreenstein.cpp:54-63,
+//                                     ColorHelper.cpp:39-93,115-148
+//   sphere / cube / plane / volume    Sphere.cpp:226-466, Cube.cpp:97-118, Plane.cpp:51-70,
+//   records and light sampling        ConstantVolume.cpp:103-115, DirectionalLight.cpp:74-103
 //   light sampling + MIS              ColorHelper.cpp:176-257,307-325,914-1033
 //   environment map                   EnvironmentMap.cpp:468-616, ColorHelper.cpp:1035-1098
 #pragma once
@@ -38,10 +42,35 @@ namespace luzgpu
 	{
 		Surface sf;
 		const GpuHittable gh = s.hittables[h.hittable];
-		const std::uint32_t kind = gh.kindIndex >> 30;
-		const std::uint32_t idx = gh.kindIndex & 0x3fffffffu;
+		const std::uint32_t kind = gh.kindIndex >> kKindShift;
+		const std::uint32_t idx = gh.kindIndex & kIndexMask;
 		sf.material = gh.material;
 		sf.position = ray.o + ray.d * h.t;
+		if (kind == GPU_HIT_SPHERE)
+		{
+			const GpuSphere sp = s.spheres[idx];
+			const f3 outward = (sf.position - ld3(sp.center)) * (1.0f / sp.radius);
+			sf.frontFace = dot(ray.d, outward) < 0.0f;
+			sf.normal = sf.frontFace ? outward : -outward;
+			sf.geometricNormal = sf.normal;
+			return sf;
+		}
+		if (kind == GPU_HIT_CUBE || kind == GPU_HIT_PLANE)
+		{
+			const f3 n = kind == GPU_HIT_CUBE ? ld3(s.cubeFaces[6 * idx + h.tri].normal) : ld3(s.planes[idx].orientation);
+			sf.frontFace = dot(ray.d, n) < 0.0f;
+			sf.normal = sf.frontFace ? n : -n;
+			sf.geometricNormal = sf.normal;
+			return sf;
+		}
+		if (kind == GPU_HIT_VOLUME)
+		{
+			// ConstantVolume::hit: arbitrary normal; the phase functions ignore it
+			sf.normal = mk(1.0f, 0.0f, 0.0f);
+			sf.geometricNormal = sf.normal;
+			sf.frontFace = true;
+			return sf;
+		}
 		if (kind == GPU_HIT_TRIANGLE)
 		{
 			const f3 fn = faceNormalOf(s.topTris[idx]);
@@ -271,6 +300,8 @@ namespace luzgpu
 	{
 		PDF_NONE = 0,
 		PDF_COSINE = 1,
+		PDF_SPHERE = 2,
+		PDF_HG = 3,
 		PDF_BSDF = 4,
 	};
 
@@ -281,13 +312,29 @@ namespace luzgpu
 		f3	attenuation;
 		f3	direction;		// specular ray direction or sampled BSDF direction
 		f3	incident;		// incoming ray direction (BSDF evaluations)
-		f3	cosineW;		// cosine basis axis
+		f3	cosineW;		// cosine basis axis; Henyey-Greenstein: the phase direction
+		float	g;			// Henyey-Greenstein anisotropy
 		bool	absorbing;	// dielectric medium absorption
 	};
 
+	// materialCanTransmitThroughGeometry (ColorHelper.cpp:327-355)
 	__device__ __forceinline__ bool	canTransmit(std::uint32_t kind)
 	{
-		return kind == GPU_MAT_DIELECTRIC;
+		return kind == GPU_MAT_DIELECTRIC || kind == GPU_MAT_ISOTROPIC || kind == GPU_MAT_HG;
+	}
+
+	// Henyey-Greenstein pdf of scatterPDFValue (ColorHelper.cpp:56-77)
+	__device__ __forceinline__ float	hgPdf(f3 phaseDir, float gIn, f3 dir)
+	{
+		const float dl2 = len2(dir), pl2 = len2(phaseDir);
+		if (dl2 <= 0.0f || pl2 <= 0.0f || !isfinite(dl2) || !isfinite(pl2)) return 0.0f;
+		const f3 nd = dir * (1.0f / sqrtf(dl2));
+		const f3 pd = phaseDir * (1.0f / sqrtf(pl2));
+		const float c = fmaxf(-1.0f, fminf(1.0f, dot(nd, pd)));
+		const float g = fmaxf(-0.99f, fminf(0.99f, gIn));
+		const float g2 = g * g;
+		const float den = fmaxf(1e-12f, 1.0f + g2 - (2.0f * g * c));
+		return (1.0f - g2) / (4.0f * kPiF * den * sqrtf(den));
 	}
 
 	// leavesOpaqueGeometricSurface (ColorHelper.cpp:357-374)
@@ -330,6 +377,8 @@ namespace luzgpu
 			const float c = dot(dir, sc.cosineW);
 			return c <= 0.0f ? 0.0f : c / kPiF;
 		}
+		if (sc.pdfType == PDF_SPHERE) return 1.0f / (4.0f * kPiF);
+		if (sc.pdfType == PDF_HG) return hgPdf(sc.cosineW, sc.g, dir);
 		if (sc.pdfType == PDF_BSDF) return bsdfPdf(m, sf, sc, dir);
 		return 0.0f;
 	}
@@ -362,6 +411,20 @@ namespace luzgpu
 			sc.attenuation = ld3(m.color);
 			sc.pdfType = PDF_COSINE;
 			sc.cosineW = normalize(sf.normal);
+			return true;
+		}
+		if (m.kind == GPU_MAT_ISOTROPIC)
+		{
+			sc.attenuation = ld3(m.color);
+			sc.pdfType = PDF_SPHERE;
+			return true;
+		}
+		if (m.kind == GPU_MAT_HG)
+		{
+			sc.attenuation = ld3(m.color);
+			sc.pdfType = PDF_HG;
+			sc.cosineW = len2(inDir) <= 0.0f ? mk(0.0f, 0.0f, 1.0f) : normalize(inDir);
+			sc.g = m.anisotropy;
 			return true;
 		}
 		if (m.kind == GPU_MAT_METAL)
@@ -481,6 +544,51 @@ namespace luzgpu
 		bool	valid;
 	};
 
+	// Sphere::sampleLight (Sphere.cpp:382-442): cone sampling toward the sphere, FP64 like
+	// the reference except the fp32 cone direction of Sphere::randomToSphere.
+	__device__ __forceinline__ LightSample	sampleSphereLight(const SceneView& s, const GpuLight& L, f3 origin, const PixelSampler& ps)
+	{
+		LightSample ls;
+		ls.valid = false;
+		const GpuSphere sp = s.spheres[L.index];
+		const double cx = static_cast<double>(sp.center[0]) - origin.x, cy = static_cast<double>(sp.center[1]) - origin.y,
+			cz = static_cast<double>(sp.center[2]) - origin.z;
+		const double r = sp.radius;
+		const double d2 = cx * cx + cy * cy + cz * cz;
+		if (r <= 0.0 || !isfinite(d2) || d2 <= r * r) return ls;
+		float sx, sy;
+		ps.sample2D(DIM_LIGHT_SURFACE_POINT, sx, sy);
+		const float rr = static_cast<float>(r * r / d2);
+		const float z = 1.0f + sy * (sqrtf(fmaxf(0.0f, 1.0f - rr)) - 1.0f);
+		const float phi = (2.0f * kPiF) * sx;
+		const float rad = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+		const float lx = cosf(phi) * rad, ly = sinf(phi) * rad;
+		f3 dir = Onb(mk(static_cast<float>(cx), static_cast<float>(cy), static_cast<float>(cz))).local(lx, ly, z);
+		const float dl2 = len2(dir);
+		if (!isfinite(dl2) || dl2 <= 0.0f) return ls;
+		dir = dir * (1.0f / sqrtf(dl2));
+		const double ox = -cx, oy = -cy, oz = -cz;	// origin - center
+		const double a = static_cast<double>(dir.x) * dir.x + static_cast<double>(dir.y) * dir.y + static_cast<double>(dir.z) * dir.z;
+		const double b = ox * dir.x + oy * dir.y + oz * dir.z;
+		const double c = d2 - r * r;
+		const double disc = b * b - a * c;
+		if (disc < 0.0) return ls;
+		const double sq = sqrt(disc);
+		double root = (-b - sq) / a;
+		if (root < 0.001 || root > 1.7976931348623157e308) root = (-b + sq) / a;
+		if (root < 0.001 || root > 1.7976931348623157e308) return ls;
+		const double cosMax = sqrt(1.0 - r * r / d2);
+		const double solid = 2.0 * 3.14159265358979323846 * (1.0 - cosMax);
+		if (solid <= 0.0 || !isfinite(solid)) return ls;
+		ls.direction = dir;
+		ls.pdf = static_cast<float>(1.0 / solid) * L.selectionProbability;
+		ls.tMax = static_cast<float>(root);
+		if (ls.pdf <= 0.0f || ls.tMax <= kTMinF || !isfinite(ls.pdf) || !isfinite(ls.tMax)) return ls;
+		ls.emitted = ld3(s.materials[L.material].color);
+		ls.valid = maxc(ls.emitted) > 0.0f;
+		return ls;
+	}
+
 	__device__ __forceinline__ LightSample	sampleLight(const SceneView& s, const GpuParams& p, f3 origin, const PixelSampler& ps)
 	{
 		LightSample ls;
@@ -511,6 +619,24 @@ namespace luzgpu
 		}
 		const GpuLight& L = s.lights[idx];
 		if (!(L.selectionProbability > 0.0f)) return ls;
+		if (L.kind == GPU_HIT_DIRECTIONAL)
+		{
+			// DirectionalLight::sampleLight: delta direction, pdf 1, tMax = T_MAX
+			ls.direction = ld3(L.direction);
+			ls.pdf = L.selectionProbability;
+			ls.tMax = INFINITY;
+			ls.emitted = ld3(s.materials[L.material].color);
+			ls.valid = len2(ls.direction) > 0.0f && ls.pdf > 0.0f && maxc(ls.emitted) > 0.0f;
+			return ls;
+		}
+		if (L.kind == GPU_HIT_SPHERE)
+		{
+			return sampleSphereLight(s, L, origin, ps);
+		}
+		if (L.kind != GPU_HIT_RECTANGLE && L.kind != GPU_HIT_TRIANGLE)
+		{
+			return ls;	// Hittable::sampleLight default: never valid (pdfValue 0)
+		}
 		float sx, sy;
 		ps.sample2D(DIM_LIGHT_SURFACE_POINT, sx, sy);
 		f3 point, normal;
@@ -552,6 +678,21 @@ namespace luzgpu
 		const Ray r = makeRay(origin, normalizeOrZero(vec));
 		const float vl2 = len2(vec);
 		float t;
+		if (L.kind == GPU_HIT_SPHERE)
+		{
+			// Sphere::pdfValue (Sphere.cpp:362-375)
+			const GpuSphere sp = s.spheres[L.index];
+			if (!sphereHit(sp, r, kTMinF, INFINITY, t)) return 0.0f;
+			const double cx = static_cast<double>(sp.center[0]) - origin.x, cy = static_cast<double>(sp.center[1]) - origin.y,
+				cz = static_cast<double>(sp.center[2]) - origin.z;
+			const double rr = static_cast<double>(sp.radius) * sp.radius;
+			const double cosMax = sqrt(1.0 - rr / (cx * cx + cy * cy + cz * cz));
+			return static_cast<float>(1.0 / (2.0 * 3.14159265358979323846 * (1.0 - cosMax)));
+		}
+		if (L.kind != GPU_HIT_RECTANGLE && L.kind != GPU_HIT_TRIANGLE)
+		{
+			return 0.0f;	// directional lights and the Hittable::pdfValue default
+		}
 		if (L.kind == GPU_HIT_RECTANGLE)
 		{
 			const GpuRect& rc = s.rects[L.index];

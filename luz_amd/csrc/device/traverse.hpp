@@ -29,7 +29,7 @@ namespace luzgpu
 		const GpuTri*	topTris;
 		const GpuTri*	meshTris;
 		const float*	meshNormals;
-		const std::uint32_t*	tlasPrims;
+		const uint2*	tlasLeaf;	// top-level leaf items: (hittable index, kind << 30 | index)
 		const GpuHittable*	hittables;
 		const GpuRect*	rects;
 		const GpuInstance*	instances;
@@ -39,7 +39,12 @@ namespace luzgpu
 		const float4*	envPixels;
 		const float*	envPdf;
 		const double*	envCdf;
-		const std::uint32_t*	unbounded;
+		const std::uint32_t*	unbounded;	// hittable ids of the planes, tested after the BVH
+		std::uint32_t	unboundedCount;
+		const GpuSphere*	spheres;
+		const GpuRect*	cubeFaces;
+		const GpuVolume*	volumes;
+		const GpuPlane*	planes;
 	};
 
 	struct	Ray
@@ -92,17 +97,25 @@ namespace luzgpu
 		return tf > tn;
 	}
 
+	// The 48-byte triangle is fetched as three 16-byte loads issued together (one memory
+	// round trip), not field by field.
 	__device__ __forceinline__ bool	triHit(const GpuTri& tri, const Ray& r, float tMin, float tMax, float& t, float& u, float& v)
 	{
-		const f3 e1 = ld3(tri.e1), e2 = ld3(tri.e2);
+		const float4* tp = reinterpret_cast<const float4*>(&tri);
+		const float4 A = tp[0], B = tp[1], C = tp[2];
+		f3 v0 = mk(A.x, A.y, A.z);
+		const f3 e1 = mk(B.x, B.y, B.z), e2 = mk(C.x, C.y, C.z);
+		// v0 is needed only after the first exit: pin it here so its load is issued with
+		// the edges' instead of after the determinant test
+		asm volatile("" : "+v"(v0.x), "+v"(v0.y), "+v"(v0.z));
 		const f3 p = cross(r.d, e2);
 		const float det = dot(e1, p);
+		const f3 tv = r.o - v0;
 		if (fabsf(det) < 1e-12f)
 		{
 			return false;
 		}
 		const float inv = 1.0f / det;
-		const f3 tv = r.o - ld3(tri.v0);
 		u = dot(tv, p) * inv;
 		if (u < 0.0f || u > 1.0f)
 		{
@@ -137,9 +150,104 @@ namespace luzgpu
 		return !(fabsf(dot(d, ld3(rc.widthAxis))) > rc.halfWidth || fabsf(dot(d, ld3(rc.heightAxis))) > rc.halfHeight);
 	}
 
-	__device__ __forceinline__ Ray	toLocal(const GpuInstance& in, const Ray& w)
+	__device__ __forceinline__ bool	zeroDir(const Ray& r)
 	{
-		const float* m = in.worldToLocal;
+		return r.d.x == 0.0f && r.d.y == 0.0f && r.d.z == 0.0f;
+	}
+
+	// Sphere::hit / hitAny (Sphere.cpp:226-301), in FP64 from the fp32 ray: the reference
+	// solves the quadratic for a unit direction (a = 1).
+	__device__ __forceinline__ bool	sphereHit(const GpuSphere& sp, const Ray& r, float tMin, float tMax, float& t)
+	{
+		if (zeroDir(r)) return false;
+		const double ox = static_cast<double>(r.o.x) - sp.center[0], oy = static_cast<double>(r.o.y) - sp.center[1],
+			oz = static_cast<double>(r.o.z) - sp.center[2];
+		const double dx = r.d.x, dy = r.d.y, dz = r.d.z;
+		const double b = ox * dx + oy * dy + oz * dz;
+		const double rr = static_cast<double>(sp.radius) * sp.radius;
+		const double c = (ox * ox + oy * oy + oz * oz) - rr;
+		const double disc = b * b - c;
+		if (disc < 0.0) return false;
+		const double sq = sqrt(disc);
+		double root = -b - sq;
+		if (root < tMin || root > tMax)
+		{
+			root = -b + sq;
+			if (root < tMin || root > tMax) return false;
+		}
+		t = static_cast<float>(root);
+		return true;
+	}
+
+	// ConstantVolume::sampleScatteringDistance (ConstantVolume.cpp:62-101) with the boundary's
+	// hitInterval over (-T_MAX, T_MAX) (Sphere.cpp:303-339, Cube.cpp:120-164). `logU` is
+	// log(max(u, 1e-12)) of the path's DIM_VOLUME_DISTANCE sample at this bounce.
+	__device__ __forceinline__ bool	volumeHit(const GpuVolume& v, const Ray& r, float tMin, float tMax, float logU, float& t)
+	{
+		if (zeroDir(r)) return false;
+		double entry, exit;
+		const double o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z};
+		if (v.sphere)
+		{
+			const double ox = o[0] - v.center[0], oy = o[1] - v.center[1], oz = o[2] - v.center[2];
+			const double b = ox * d[0] + oy * d[1] + oz * d[2];
+			const double c = (ox * ox + oy * oy + oz * oz) - static_cast<double>(v.radius) * v.radius;
+			const double disc = b * b - c;
+			if (disc < 0.0) return false;
+			const double sq = sqrt(disc);
+			entry = fmin(-b - sq, -b + sq);
+			exit = fmax(-b - sq, -b + sq);
+			if (entry >= exit) return false;
+		}
+		else
+		{
+			entry = -1.7976931348623157e308;
+			exit = 1.7976931348623157e308;
+			const float lo[3] = {v.lo[0], v.lo[1], v.lo[2]}, hi[3] = {v.hi[0], v.hi[1], v.hi[2]};
+			for (int a = 0; a < 3; a++)
+			{
+				if (d[a] == 0.0)
+				{
+					if (o[a] < lo[a] || o[a] > hi[a]) return false;
+					continue;
+				}
+				const double inv = 1.0 / d[a];
+				double t0 = (lo[a] - o[a]) * inv, t1 = (hi[a] - o[a]) * inv;
+				if (t0 > t1)
+				{
+					const double x = t0; t0 = t1; t1 = x;
+				}
+				entry = fmax(entry, t0);
+				exit = fmin(exit, t1);
+				if (entry >= exit) return false;
+			}
+		}
+		if (entry < tMin) entry = tMin;
+		if (exit > tMax) exit = tMax;
+		if (entry >= exit) return false;
+		if (entry < 0.0) entry = 0.0;
+		const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+		if (!(len > 0.0) || !isfinite(len)) return false;
+		const double inside = (exit - entry) * len;
+		const double dist = static_cast<double>(v.negInvDensity) * logU;
+		if (dist > inside) return false;
+		t = static_cast<float>(entry + dist / len);
+		return true;
+	}
+
+	// Plane::hit / hitAny (Plane.cpp:51-87)
+	__device__ __forceinline__ bool	planeHit(const GpuPlane& pl, const Ray& r, float tMin, float tMax, float& t)
+	{
+		const f3 o = ld3(pl.orientation);
+		const float d = dot(r.d, o);
+		if (d == 0.0f) return false;
+		t = dot(ld3(pl.point) - r.o, o) / d;
+		return !(t > tMax || t < tMin);
+	}
+
+	__device__ __forceinline__ Ray	toLocal(const float4 r0, const float4 r1, const float4 r2, const Ray& w)
+	{
+		const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
 		const f3 o = mk(m[0] * w.o.x + m[1] * w.o.y + m[2] * w.o.z + m[3],
 			m[4] * w.o.x + m[5] * w.o.y + m[6] * w.o.z + m[7],
 			m[8] * w.o.x + m[9] * w.o.y + m[10] * w.o.z + m[11]);
@@ -147,6 +255,12 @@ namespace luzgpu
 			m[4] * w.d.x + m[5] * w.d.y + m[6] * w.d.z,
 			m[8] * w.d.x + m[9] * w.d.y + m[10] * w.d.z);
 		return makeRay(o, d);
+	}
+
+	__device__ __forceinline__ Ray	toLocal(const GpuInstance& in, const Ray& w)
+	{
+		const float4* mp = reinterpret_cast<const float4*>(in.worldToLocal);
+		return toLocal(mp[0], mp[1], mp[2], w);
 	}
 
 	// Stack entries: bit31 set = leaf (bits 27..30 = count-1, bits 0..26 = first prim);
@@ -218,165 +332,266 @@ namespace luzgpu
 		__device__ __forceinline__ void	reset() { sp = 0; }
 	};
 
-	template <bool AnyHit, bool Count = false, typename Stack>
-	__device__ __forceinline__ bool	traverse(const SceneView& s, std::int32_t tlasRoot, const Ray& worldRay, float tMin, float tMax,
-		HitInfo& hit, Stack& stack, VisitCounter* counter = nullptr)
+	// Resumable per-lane traversal state: one call of traceStep processes one stack entry
+	// (a node, a leaf or an instance exit), so a persistent kernel can hand a lane a new ray
+	// as soon as its traversal ends instead of waiting for the whole wave.
+	struct	TraceState
 	{
-		hit.t = tMax;
-		hit.hittable = 0xffffffffu;
-		hit.tri = -1;
-		// an any-hit query returns with entries still on the stack: every traversal
-		// starts from an empty stack
+		Ray	world;		// world-space ray, restored when an instance is left
+		Ray	ray;		// ray of the current level
+		HitInfo	hit;	// hit.t is the current closest distance
+		const Bvh4Node*	nodes;	// node array of the current level
+		float	logU;	// log(max(u, 1e-12)) of the volume free-flight sample (scenes with volumes)
+		std::uint32_t	entry;
+		std::uint32_t	instHittable;
+		bool	inInstance;
+		bool	found;
+	};
+
+	// Starts a traversal; returns false when there is nothing to traverse.
+	template <typename Stack>
+	__device__ __forceinline__ bool	traceBegin(TraceState& st, const SceneView& s, std::int32_t tlasRoot, const Ray& worldRay, float tMax,
+		float logU, Stack& stack)
+	{
+		const Bvh4Node* tlas = s.tlas;
+		st.logU = logU;
+		st.world = worldRay;
+		st.ray = worldRay;
+		st.hit.t = tMax;
+		st.hit.hittable = 0xffffffffu;
+		st.hit.tri = -1;
+		// without bounded hittables only the unbounded ones are tested: start from the
+		// instance-exit entry, which leads straight to the end of the traversal
+		st.entry = tlasRoot >= 0 ? static_cast<std::uint32_t>(tlasRoot) : kExitInstance;
+		st.nodes = tlas;
+		st.instHittable = 0;
+		st.inInstance = false;
+		st.found = false;
+		// an any-hit query ends with entries still on the stack: every traversal starts
+		// from an empty stack
 		stack.reset();
-		if (tlasRoot < 0)
+		return tlasRoot >= 0 || s.unboundedCount > 0;
+	}
+
+	// Processes the current entry; returns true when the traversal has ended (st.found and
+	// st.hit hold the result).
+	template <bool AnyHit, bool Count, typename Stack>
+	__device__ __forceinline__ bool	traceStep(const SceneView& s, TraceState& st, float tMin, Stack& stack,
+		VisitCounter* counter)
+	{
+		const std::uint32_t entry = st.entry;
+		if (entry == kExitInstance)
 		{
-			return false;
+			st.ray = st.world;
+			st.inInstance = false;
+			st.nodes = s.tlas;
 		}
-		Ray ray = worldRay;
-		bool inInstance = false;
-		std::uint32_t instHittable = 0;
-		const Bvh4Node* nodes = s.tlas;
-		float closest = tMax;
-		bool found = false;
-		std::uint32_t entry = static_cast<std::uint32_t>(tlasRoot);
-		while (true)
+		else if ((entry & kLeafBit) == 0)
 		{
-			if (entry == kExitInstance)
+			// the whole 64-byte node in four 16-byte loads issued together:
+			//   n0 = origin xyz | exp[3]      n1 = qlo[axis][child] x3 | qhi[0][child]
+			//   n2 = qhi[1], qhi[2] | child 0, 1      n3 = child 2, 3 | count[4] | pad
+			const uint4* np = reinterpret_cast<const uint4*>(st.nodes + entry);
+			const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+			if (Count) counter->nodes++;
+			// decode the four child boxes (plane = fma(q, 2^exp, origin)) and test them
+			const float org[3] = {__uint_as_float(n0.x), __uint_as_float(n0.y), __uint_as_float(n0.z)};
+			float sc[3];
+			for (int a = 0; a < 3; a++)
 			{
-				ray = worldRay;
-				inInstance = false;
-				nodes = s.tlas;
+				const int ex = static_cast<int>(static_cast<std::int8_t>((n0.w >> (8 * a)) & 0xffu));
+				sc[a] = __int_as_float((ex + 127) << 23);
 			}
-			else if ((entry & kLeafBit) == 0)
+			const std::uint32_t qlo[3] = {n1.x, n1.y, n1.z};
+			const std::uint32_t qhi[3] = {n1.w, n2.x, n2.y};
+			const std::uint32_t child[4] = {n2.z, n2.w, n3.x, n3.y};
+			std::uint32_t e[4];
+			float tn[4];
+			int hits = 0;
+			for (int k = 0; k < 4; k++)
 			{
-				const Bvh4Node& n = nodes[entry];
-				if (Count) counter->nodes++;
-				// decode the four child boxes (plane = fma(q, 2^exp, origin)) and test them
-				float sc[3];
-				for (int a = 0; a < 3; a++) sc[a] = __int_as_float((static_cast<int>(n.exp[a]) + 127) << 23);
-				std::uint32_t e[4];
-				float tn[4];
-				int hits = 0;
-				for (int k = 0; k < 4; k++)
+				const std::uint32_t cnt = (n3.z >> (8 * k)) & 0xffu;
+				if (cnt == 255u) continue;
+				float lo[3], hi[3];
+				for (int a = 0; a < 3; a++)
 				{
-					if (n.count[k] == 255) continue;
-					float lo[3], hi[3];
-					for (int a = 0; a < 3; a++)
-					{
-						lo[a] = fmaf(static_cast<float>(n.qlo[a][k]), sc[a], n.origin[a]);
-						hi[a] = fmaf(static_cast<float>(n.qhi[a][k]), sc[a], n.origin[a]);
-					}
-					float t;
-					if (boxHit(ray, lo, hi, closest, t))
-					{
-						tn[hits] = t;
-						e[hits] = n.count[k] > 0 ? leafEntry(n.child[k], n.count[k]) : static_cast<std::uint32_t>(n.child[k]);
-						hits++;
-					}
+					lo[a] = fmaf(static_cast<float>((qlo[a] >> (8 * k)) & 0xffu), sc[a], org[a]);
+					hi[a] = fmaf(static_cast<float>((qhi[a] >> (8 * k)) & 0xffu), sc[a], org[a]);
 				}
-				if (hits > 0)
+				float t;
+				if (boxHit(st.ray, lo, hi, st.hit.t, t))
 				{
-					// order by entry distance (insertion sort of <= 4), push far to near
-					for (int i = 1; i < hits; i++)
+					tn[hits] = t;
+					e[hits] = cnt > 0 ? leafEntry(static_cast<std::int32_t>(child[k]), static_cast<std::int32_t>(cnt)) : child[k];
+					hits++;
+				}
+			}
+			if (hits > 0)
+			{
+				// order by entry distance (insertion sort of <= 4), push far to near
+				for (int i = 1; i < hits; i++)
+				{
+					const float kt = tn[i];
+					const std::uint32_t ke = e[i];
+					int j = i - 1;
+					while (j >= 0 && tn[j] > kt)
 					{
-						const float kt = tn[i];
-						const std::uint32_t ke = e[i];
-						int j = i - 1;
-						while (j >= 0 && tn[j] > kt)
-						{
-							tn[j + 1] = tn[j];
-							e[j + 1] = e[j];
-							j--;
-						}
-						tn[j + 1] = kt;
-						e[j + 1] = ke;
+						tn[j + 1] = tn[j];
+						e[j + 1] = e[j];
+						j--;
 					}
-					for (int i = hits - 1; i > 0; i--) stack.push(e[i]);
-					entry = e[0];
+					tn[j + 1] = kt;
+					e[j + 1] = ke;
+				}
+				for (int i = hits - 1; i > 0; i--) stack.push(e[i]);
+				st.entry = e[0];
+				return false;
+			}
+		}
+		else
+		{
+			const std::int32_t start = static_cast<std::int32_t>(entry & 0x07ffffffu);
+			const std::int32_t count = static_cast<std::int32_t>((entry >> 27) & 0xfu) + 1;
+			for (std::int32_t i = 0; i < count; i++)
+			{
+				if (Count) counter->prims++;
+				if (st.inInstance)
+				{
+					const GpuTri& tri = s.meshTris[start + i];
+					float t, u, v;
+					if (triHit(tri, st.ray, tMin, st.hit.t, t, u, v))
+					{
+						st.found = true;
+						st.hit.t = t; st.hit.u = u; st.hit.v = v;
+						st.hit.hittable = st.instHittable;
+						st.hit.tri = start + i;
+						if (AnyHit) return true;
+					}
 					continue;
 				}
-			}
-			else
-			{
-				const std::int32_t start = static_cast<std::int32_t>(entry & 0x07ffffffu);
-				const std::int32_t count = static_cast<std::int32_t>((entry >> 27) & 0xfu) + 1;
-				bool descended = false;
-				for (std::int32_t i = 0; i < count; i++)
+				const uint2 item = s.tlasLeaf[start + i];
+				const std::uint32_t h = item.x;
+				const std::uint32_t kind = item.y >> kKindShift;
+				const std::uint32_t idx = item.y & kIndexMask;
+				if (kind == GPU_HIT_TRIANGLE)
 				{
-					if (Count) counter->prims++;
-					if (inInstance)
+					float t, u, v;
+					if (triHit(s.topTris[idx], st.ray, tMin, st.hit.t, t, u, v))
 					{
-						const GpuTri& tri = s.meshTris[start + i];
-						float t, u, v;
-						if (triHit(tri, ray, tMin, closest, t, u, v))
-						{
-							found = true;
-							closest = t;
-							hit.t = t; hit.u = u; hit.v = v;
-							hit.hittable = instHittable;
-							hit.tri = start + i;
-							if (AnyHit) return true;
-						}
-						continue;
+						st.found = true;
+						st.hit.t = t; st.hit.u = u; st.hit.v = v;
+						st.hit.hittable = h;
+						st.hit.tri = -1;
+						if (AnyHit) return true;
 					}
-					const std::uint32_t h = s.tlasPrims[start + i];
-					const GpuHittable gh = s.hittables[h];
-					const std::uint32_t kind = gh.kindIndex >> 30;
-					const std::uint32_t idx = gh.kindIndex & 0x3fffffffu;
-					if (kind == GPU_HIT_TRIANGLE)
+				}
+				else if (kind == GPU_HIT_RECTANGLE)
+				{
+					float t;
+					if (rectHit(s.rects[idx], st.ray, tMin, st.hit.t, t))
 					{
-						float t, u, v;
-						if (triHit(s.topTris[idx], ray, tMin, closest, t, u, v))
-						{
-							found = true;
-							closest = t;
-							hit.t = t; hit.u = u; hit.v = v;
-							hit.hittable = h;
-							hit.tri = -1;
-							if (AnyHit) return true;
-						}
+						st.found = true;
+						st.hit.t = t;
+						st.hit.hittable = h;
+						st.hit.tri = -1;
+						if (AnyHit) return true;
 					}
-					else if (kind == GPU_HIT_RECTANGLE)
+				}
+				else if (kind == GPU_HIT_SPHERE)
+				{
+					float t;
+					if (sphereHit(s.spheres[idx], st.ray, tMin, st.hit.t, t))
+					{
+						st.found = true;
+						st.hit.t = t;
+						st.hit.hittable = h;
+						st.hit.tri = -1;
+						if (AnyHit) return true;
+					}
+				}
+				else if (kind == GPU_HIT_CUBE)
+				{
+					// Cube::hit (Cube.cpp:97-111): its six faces, closest first-found
+					for (int f = 0; f < 6; f++)
 					{
 						float t;
-						if (rectHit(s.rects[idx], ray, tMin, closest, t))
+						if (rectHit(s.cubeFaces[6 * idx + f], st.ray, tMin, st.hit.t, t))
 						{
-							found = true;
-							closest = t;
-							hit.t = t;
-							hit.hittable = h;
-							hit.tri = -1;
+							st.found = true;
+							st.hit.t = t;
+							st.hit.hittable = h;
+							st.hit.tri = f;
 							if (AnyHit) return true;
 						}
 					}
-					else
+				}
+				else if (kind == GPU_HIT_VOLUME)
+				{
+					float t;
+					if (volumeHit(s.volumes[idx], st.ray, tMin, st.hit.t, st.logU, t))
 					{
-						const GpuInstance& in = s.instances[idx];
-						if (i + 1 < count)
-						{
-							stack.push(leafEntry(start + i + 1, count - i - 1));
-						}
-						stack.push(kExitInstance);
-						ray = toLocal(in, worldRay);
-						inInstance = true;
-						instHittable = h;
-						nodes = s.blas;
-						entry = static_cast<std::uint32_t>(in.blasRoot);
-						descended = true;
-						break;
+						st.found = true;
+						st.hit.t = t;
+						st.hit.hittable = h;
+						st.hit.tri = -1;
+						if (AnyHit) return true;
 					}
 				}
-				if (descended)
+				else
 				{
-					continue;
+					// 3x4 map and mesh links: one group of four 16-byte loads
+					const float4* ip = reinterpret_cast<const float4*>(s.instances + idx);
+					const float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
+					const int4 links = reinterpret_cast<const int4*>(ip)[3];
+					if (i + 1 < count)
+					{
+						stack.push(leafEntry(start + i + 1, count - i - 1));
+					}
+					stack.push(kExitInstance);
+					st.ray = toLocal(r0, r1, r2, st.world);
+					st.inInstance = true;
+					st.nodes = s.blas;
+					st.instHittable = h;
+					st.entry = static_cast<std::uint32_t>(links.x);
+					return false;
 				}
 			}
-			if (stack.empty())
-			{
-				break;
-			}
-			entry = stack.pop();
 		}
-		return found;
+		if (stack.empty())
+		{
+			// the unbounded hittables after the acceleration structure (HitHelper.cpp:5-32)
+			for (std::uint32_t i = 0; i < s.unboundedCount; i++)
+			{
+				const std::uint32_t h = s.unbounded[i];
+				const GpuPlane& pl = s.planes[s.hittables[h].kindIndex & kIndexMask];
+				float t;
+				if (planeHit(pl, st.world, tMin, st.hit.t, t))
+				{
+					st.found = true;
+					st.hit.t = t;
+					st.hit.hittable = h;
+					st.hit.tri = -1;
+					if (AnyHit) return true;
+				}
+			}
+			return true;
+		}
+		st.entry = stack.pop();
+		return false;
+	}
+
+	template <bool AnyHit, bool Count = false, typename Stack>
+	__device__ __forceinline__ bool	traverse(const SceneView& s, std::int32_t tlasRoot, const Ray& worldRay, float tMin, float tMax,
+		HitInfo& hit, Stack& stack, VisitCounter* counter = nullptr, float logU = 0.0f)
+	{
+		TraceState st;
+		if (traceBegin(st, s, tlasRoot, worldRay, tMax, logU, stack))
+		{
+			while (!traceStep<AnyHit, Count>(s, st, tMin, stack, counter))
+			{
+			}
+		}
+		hit = st.hit;
+		return st.found;
 	}
 }

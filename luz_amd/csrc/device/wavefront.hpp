@@ -20,6 +20,7 @@
 #include "integrator.hpp"
 
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace luzgpu
@@ -60,6 +61,7 @@ namespace luzgpu
 	constexpr int	kWaveStackCapacity = 64;
 	constexpr int	kMaxBounceSlots = 64;
 	constexpr unsigned	kWaveGrid = 2048;	// 256 CUs x 8 waves per SIMD x 4 SIMDs / 4 waves per block
+	constexpr int	kDefaultRefill = 16;
 
 	__device__ __forceinline__ std::uint32_t	laneId() { return threadIdx.x & 63u; }
 
@@ -103,34 +105,66 @@ namespace luzgpu
 		}
 	}
 
+	// Queue slots handed to the lanes of one wave, claimed from the queue's fetch counter in
+	// chunks (one atomic per chunk). All fields are wave-uniform.
+	constexpr std::uint32_t	kFetchChunk = 256;
+
+	struct	WaveFeed
+	{
+		std::uint32_t	next;
+		std::uint32_t	end;
+		bool	drained;
+
+		// Gives every lane in `idle` (ballot mask) the next queue slot, in lane order; lanes
+		// left without one (queue exhausted) get 0xffffffff.
+		__device__ __forceinline__ std::uint32_t	hand(unsigned long long idle, std::uint32_t n, std::uint32_t* fetch)
+		{
+			const std::uint32_t want = static_cast<std::uint32_t>(__popcll(idle));
+			const std::uint32_t rank = static_cast<std::uint32_t>(__popcll(idle & ((1ull << laneId()) - 1ull)));
+			const std::uint32_t avail = end - next;
+			std::uint32_t slot;
+			if (want <= avail)
+			{
+				slot = next + rank;
+				next += want;
+				return slot;
+			}
+			std::uint32_t c = 0;
+			if (laneId() == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(idle)) - 1))
+			{
+				c = atomicAdd(fetch, kFetchChunk);
+			}
+			c = __builtin_amdgcn_readfirstlane(__shfl(c, __ffsll(static_cast<long long>(idle)) - 1));
+			const std::uint32_t cEnd = c >= n ? c : min(c + kFetchChunk, n);
+			if (rank < avail) slot = next + rank;
+			else
+			{
+				const std::uint32_t k = c + (rank - avail);
+				slot = k < cEnd ? k : 0xffffffffu;
+			}
+			if (c >= n)
+			{
+				next = end;
+				drained = true;
+			}
+			else
+			{
+				next = min(c + (want - avail), cEnd);
+				end = cEnd;
+			}
+			return slot;
+		}
+	};
+
 	// counters: [0] closest rays, [1] shadow rays, [2]/[3] closest nodes/primitives,
 	// [4]/[5] shadow nodes/primitives (the last four only when Count)
-	template <bool Count>
-	__global__ __launch_bounds__(kWaveBlock) void	wf_extend(SceneView s, GpuParams p, WaveBuffers w, int b,
-		unsigned long long* counters)
+	__device__ __forceinline__ void	flushCounters(unsigned long long* counters, int slot, std::uint32_t rays,
+		VisitCounter visits, bool count)
 	{
-		__shared__ std::uint32_t stackMem[kWaveLdsStack * kWaveBlock];
-		const std::uint32_t gtid = blockIdx.x * kWaveBlock + threadIdx.x;
-		SpillStack<kWaveLdsStack, kWaveBlock> stack{stackMem + threadIdx.x, w.spill + gtid, gridDim.x * kWaveBlock, 0,
-			kWaveStackCapacity};
-		const std::uint32_t n = w.counts[b];
-		const int q = b & 1;
-		std::uint32_t rays = 0;
-		VisitCounter visits{0u, 0u};
-		for (std::uint32_t i = gtid; i < n; i += gridDim.x * kWaveBlock)
-		{
-			const float4 o = w.qOrigin[q][i], d = w.qDir[q][i];
-			const Ray r = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
-			HitInfo h;
-			const bool found = traverse<false, Count>(s, p.tlasRoot, r, kTMinF, INFINITY, h, stack, &visits);
-			w.qHit[i] = make_float4(found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.hittable));
-			w.qTri[i] = h.tri;
-			rays++;
-		}
 		for (int off = 32; off > 0; off >>= 1)
 		{
 			rays += __shfl_down(rays, off);
-			if (Count)
+			if (count)
 			{
 				visits.nodes += __shfl_down(visits.nodes, off);
 				visits.prims += __shfl_down(visits.prims, off);
@@ -138,13 +172,130 @@ namespace luzgpu
 		}
 		if (laneId() == 0 && rays)
 		{
-			atomicAdd(counters + 0, static_cast<unsigned long long>(rays));
-			if (Count)
+			atomicAdd(counters + slot, static_cast<unsigned long long>(rays));
+			if (count)
 			{
-				atomicAdd(counters + 2, static_cast<unsigned long long>(visits.nodes));
-				atomicAdd(counters + 3, static_cast<unsigned long long>(visits.prims));
+				atomicAdd(counters + 2 + 2 * slot, static_cast<unsigned long long>(visits.nodes));
+				atomicAdd(counters + 3 + 2 * slot, static_cast<unsigned long long>(visits.prims));
 			}
 		}
+	}
+
+	// Persistent traversal over one ray queue with lane refill: a lane whose traversal has
+	// ended takes the next ray of the queue once at least `refillAt` lanes of its wave are
+	// idle (refillAt = 64: the wave refills only when all its lanes are done).
+	// Source supplies the queue: count(), load(slot, ray, tMax) and store(slot, state).
+	template <bool AnyHit, bool Count, typename Source>
+	__device__ __forceinline__ void	traceQueue(const SceneView& s, std::int32_t tlasRoot, Source& src, std::uint32_t* fetch,
+		int refillAt, unsigned long long* counters, int counterSlot)
+	{
+		__shared__ std::uint32_t stackMem[kWaveLdsStack * kWaveBlock];
+		const std::uint32_t gtid = blockIdx.x * kWaveBlock + threadIdx.x;
+		SpillStack<kWaveLdsStack, kWaveBlock> stack{stackMem + threadIdx.x, src.spill + gtid, gridDim.x * kWaveBlock, 0,
+			kWaveStackCapacity};
+		const std::uint32_t n = src.count();
+		WaveFeed feed{0u, 0u, false};
+		std::uint32_t rays = 0;
+		VisitCounter visits{0u, 0u};
+		TraceState st;
+		std::uint32_t slot = 0xffffffffu;
+		bool active = false;
+		while (true)
+		{
+			const unsigned long long idle = __ballot(!active);
+			if (!feed.drained && (__popcll(idle) >= refillAt || idle == ~0ull))
+			{
+				const std::uint32_t k = feed.hand(idle, n, fetch);
+				if (!active && k != 0xffffffffu)
+				{
+					slot = k;
+					Ray r;
+					float tMax, logU;
+					src.load(slot, r, tMax, logU);
+					active = traceBegin(st, s, tlasRoot, r, tMax, logU, stack);
+					if (!active)
+					{
+						src.store(slot, st);
+						rays++;
+					}
+				}
+			}
+			if (__ballot(active) == 0ull && feed.drained)
+			{
+				break;
+			}
+			if (active && traceStep<AnyHit, Count>(s, st, kTMinF, stack, &visits))
+			{
+				src.store(slot, st);
+				rays++;
+				active = false;
+			}
+		}
+		flushCounters(counters, counterSlot, rays, visits, Count);
+	}
+
+	// The volume free-flight sample of path `path` at bounce b (scenes with volumes only).
+	__device__ __forceinline__ float	pathVolumeLogU(const GpuParams& p, std::uint32_t path, int b)
+	{
+		if (!p.hasVolumes) return 0.0f;
+		const std::uint32_t pixels = p.width * p.height;
+		const std::uint32_t pixel = path % pixels, slot = path / pixels;
+		PixelSampler ps;
+		ps.begin(p.renderSeed, pixel % p.width, pixel / p.width, p.sampleBegin + slot, 0);
+		ps.bounce = static_cast<std::uint32_t>(b);
+		return volumeLogU(p, ps);
+	}
+
+	struct	ExtendQueue
+	{
+		const WaveBuffers&	w;
+		const GpuParams&	p;
+		int	b;
+		std::uint32_t*	spill;
+		__device__ __forceinline__ std::uint32_t	count() const { return w.counts[b]; }
+		__device__ __forceinline__ void	load(std::uint32_t i, Ray& r, float& tMax, float& logU) const
+		{
+			const float4 o = w.qOrigin[b & 1][i], d = w.qDir[b & 1][i];
+			r = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
+			tMax = INFINITY;
+			logU = p.hasVolumes ? pathVolumeLogU(p, w.qPath[b & 1][i], b) : 0.0f;
+		}
+		__device__ __forceinline__ void	store(std::uint32_t i, const TraceState& st) const
+		{
+			w.qHit[i] = make_float4(st.found ? st.hit.t : -1.0f, st.hit.u, st.hit.v, __uint_as_float(st.hit.hittable));
+			w.qTri[i] = st.hit.tri;
+		}
+	};
+
+	struct	ShadowQueue
+	{
+		const WaveBuffers&	w;
+		const GpuParams&	p;
+		int	b;
+		std::uint32_t*	spill;
+		std::uint32_t	target;	// connection slot of the ray being traced
+		__device__ __forceinline__ std::uint32_t	count() const { return w.counts[kMaxBounceSlots + b]; }
+		__device__ __forceinline__ void	load(std::uint32_t i, Ray& r, float& tMax, float& logU)
+		{
+			const float4 o = w.sOrigin[i], d = w.sDir[i];
+			r = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
+			tMax = o.w;
+			target = __float_as_uint(d.w);
+			logU = p.hasVolumes ? pathVolumeLogU(p, target >> 1, b) : 0.0f;
+		}
+		__device__ __forceinline__ void	store(std::uint32_t, const TraceState& st) const
+		{
+			w.visible[target] = st.found ? 0 : 1;
+		}
+	};
+
+	// closest hit of every ray of queue b
+	template <bool Count>
+	__global__ __launch_bounds__(kWaveBlock) void	wf_extend(SceneView s, GpuParams p, WaveBuffers w, int b,
+		unsigned long long* counters, int refillAt)
+	{
+		ExtendQueue src{w, p, b, w.spill};
+		traceQueue<false, Count>(s, p.tlasRoot, src, w.counts + 2 * kMaxBounceSlots + b, refillAt, counters, 0);
 	}
 
 	__global__ __launch_bounds__(kWaveBlock) void	wf_shade(SceneView s, GpuParams p, WaveBuffers w, int b)
@@ -266,7 +417,7 @@ namespace luzgpu
 										flags |= PF_CONN_ENV;
 									}
 								}
-								const f3 dir = sc.pdfType == PDF_COSINE ? cosineDirection(sf, ps) : sc.direction;
+								const f3 dir = scatterDirection(sf, sc, ps);
 								if (leavesOpaque(sf, m.kind, dir))
 								{
 									const float pdf = scatterPdf(m, sf, sc, dir);
@@ -330,44 +481,13 @@ namespace luzgpu
 		}
 	}
 
+	// any-hit test of the shadow rays appended in bounce b
 	template <bool Count>
 	__global__ __launch_bounds__(kWaveBlock) void	wf_shadow(SceneView s, GpuParams p, WaveBuffers w, int b,
-		unsigned long long* counters)
+		unsigned long long* counters, int refillAt)
 	{
-		__shared__ std::uint32_t stackMem[kWaveLdsStack * kWaveBlock];
-		const std::uint32_t gtid = blockIdx.x * kWaveBlock + threadIdx.x;
-		SpillStack<kWaveLdsStack, kWaveBlock> stack{stackMem + threadIdx.x, w.spill + gtid, gridDim.x * kWaveBlock, 0,
-			kWaveStackCapacity};
-		const std::uint32_t n = w.counts[kMaxBounceSlots + b];
-		std::uint32_t rays = 0;
-		VisitCounter visits{0u, 0u};
-		for (std::uint32_t i = gtid; i < n; i += gridDim.x * kWaveBlock)
-		{
-			const float4 o = w.sOrigin[i], d = w.sDir[i];
-			HitInfo h;
-			const bool occluded = traverse<true, Count>(s, p.tlasRoot, makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z)), kTMinF, o.w,
-				h, stack, &visits);
-			w.visible[__float_as_uint(d.w)] = occluded ? 0 : 1;
-			rays++;
-		}
-		for (int off = 32; off > 0; off >>= 1)
-		{
-			rays += __shfl_down(rays, off);
-			if (Count)
-			{
-				visits.nodes += __shfl_down(visits.nodes, off);
-				visits.prims += __shfl_down(visits.prims, off);
-			}
-		}
-		if (laneId() == 0 && rays)
-		{
-			atomicAdd(counters + 1, static_cast<unsigned long long>(rays));
-			if (Count)
-			{
-				atomicAdd(counters + 4, static_cast<unsigned long long>(visits.nodes));
-				atomicAdd(counters + 5, static_cast<unsigned long long>(visits.prims));
-			}
-		}
+		ShadowQueue src{w, p, b, w.spill, 0u};
+		traceQueue<true, Count>(s, p.tlasRoot, src, w.counts + 3 * kMaxBounceSlots + b, refillAt, counters, 1);
 	}
 
 	__global__ __launch_bounds__(kWaveBlock) void	wf_resolve(GpuParams p, WaveBuffers w, int b, float* __restrict__ sampleColors)
@@ -430,7 +550,7 @@ namespace luzgpu
 		{
 			const size_t n = capacity;
 			const size_t spillBytes = static_cast<size_t>(kWaveStackCapacity - kWaveLdsStack) * kWaveGrid * kWaveBlock * 4;
-			const size_t need = n * (16 * 2 * 2 + 4 * 2 + 16 + 4 + 16 * 3 + 32 + 2 + 32 * 2) + kMaxBounceSlots * 2 * 4 + 64 * 16
+			const size_t need = n * (16 * 2 * 2 + 4 * 2 + 16 + 4 + 16 * 3 + 32 + 2 + 32 * 2) + kMaxBounceSlots * 4 * 4 + 64 * 16
 				+ spillBytes + 256;
 			if (need > bytes)
 			{
@@ -459,9 +579,32 @@ namespace luzgpu
 			buf.visible = reinterpret_cast<std::uint8_t*>(take(2 * n));
 			buf.sOrigin = reinterpret_cast<float4*>(take(32 * n));
 			buf.sDir = reinterpret_cast<float4*>(take(32 * n));
-			buf.counts = reinterpret_cast<std::uint32_t*>(take(kMaxBounceSlots * 2 * 4));
+			buf.counts = reinterpret_cast<std::uint32_t*>(take(kMaxBounceSlots * 4 * 4));
 			buf.spill = reinterpret_cast<std::uint32_t*>(take(spillBytes));
 			buf.capacity = capacity;
+		}
+	};
+
+	// Launch shape of the traversal stages: persistent blocks per launch and the number of
+	// idle lanes at which a wave takes new rays (1..64).
+	struct	WaveTuning
+	{
+		unsigned	grid = 0;
+		int	extendRefill = 0;
+		int	shadowRefill = 0;
+
+		WaveTuning	resolved() const
+		{
+			WaveTuning t = *this;
+			if (t.grid == 0) t.grid = kWaveGrid;
+			if (t.extendRefill == 0) t.extendRefill = kDefaultRefill;
+			if (t.shadowRefill == 0) t.shadowRefill = kDefaultRefill;
+			if (t.grid > kWaveGrid || t.extendRefill < 1 || t.extendRefill > 64 || t.shadowRefill < 1 || t.shadowRefill > 64)
+			{
+				throw std::invalid_argument("wavefront tuning out of range: grid 1.." + std::to_string(kWaveGrid)
+					+ ", refill 1..64");
+			}
+			return t;
 		}
 	};
 
@@ -474,8 +617,9 @@ namespace luzgpu
 	// Enqueues one launch of the wavefront integrator. With `events`, each stage is
 	// bracketed by HIP events (on this stream) and the per-stage times are returned.
 	inline void	renderWavefront(WavefrontState& st, const SceneView& s, const GpuParams& p, float* sampleColors,
-		unsigned long long* counters, hipStream_t stream, bool count, StageTimes* times)
+		unsigned long long* counters, hipStream_t stream, bool count, StageTimes* times, WaveTuning tune)
 	{
+		tune = tune.resolved();
 		if (p.maxBounces + 2 > kMaxBounceSlots)
 		{
 			throw std::runtime_error("Unsupported: more than 62 bounces in the wavefront integrator");
@@ -499,18 +643,18 @@ namespace luzgpu
 			q.sampleBegin = p.sampleBegin + s0;
 			q.sampleCount = std::min(perPass, p.sampleCount - s0);
 			float* colors = sampleColors + static_cast<size_t>(s0) * pixels * 3;
-			wf_zero_counts<<<1, 2 * kMaxBounceSlots, 0, stream>>>(w.counts, 2 * kMaxBounceSlots, pixels * q.sampleCount);
+			wf_zero_counts<<<1, 4 * kMaxBounceSlots, 0, stream>>>(w.counts, 4 * kMaxBounceSlots, pixels * q.sampleCount);
 			wf_raygen<<<kWaveGrid, kWaveBlock, 0, stream>>>(q, w);
 			for (int b = 0; b <= q.maxBounces; b++)
 			{
 				mark();
-				if (count) wf_extend<true><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
-				else wf_extend<false><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
+				if (count) wf_extend<true><<<tune.grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters, tune.extendRefill);
+				else wf_extend<false><<<tune.grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters, tune.extendRefill);
 				mark();
 				wf_shade<<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b);
 				mark();
-				if (count) wf_shadow<true><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
-				else wf_shadow<false><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b, counters);
+				if (count) wf_shadow<true><<<tune.grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters, tune.shadowRefill);
+				else wf_shadow<false><<<tune.grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters, tune.shadowRefill);
 				mark();
 				wf_resolve<<<kWaveGrid, kWaveBlock, 0, stream>>>(q, w, b, colors);
 			}

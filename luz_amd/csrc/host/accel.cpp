@@ -325,6 +325,66 @@ namespace luzgpu
 			return (true);
 		}
 
+		GpuRect	packRect(const Rectangle& r)
+		{
+			dvec3 n{}, wa{}, ha{};
+			rectangleBasis(r, n, wa, ha);
+			GpuRect g{};
+			for (int k = 0; k < 3; k++)
+			{
+				g.position[k] = static_cast<float>(r.position[k]);
+				g.normal[k] = static_cast<float>(n[k]);
+				g.widthAxis[k] = static_cast<float>(wa[k]);
+				g.heightAxis[k] = static_cast<float>(ha[k]);
+			}
+			g.halfWidth = static_cast<float>(r.width / 2.0);
+			g.halfHeight = static_cast<float>(r.height / 2.0);
+			g.width = static_cast<float>(r.width);
+			g.height = static_cast<float>(r.height);
+			g.area = static_cast<float>(r.height * r.width);
+			return (g);
+		}
+
+		// Cube::_generateFaces (Cube.cpp:72-95): front, back, top, bottom, right, left.
+		std::array<Rectangle, 6>	cubeFaces(const Cube& c)
+		{
+			auto face = [](dvec3 pos, dvec3 o, double w, double h) {
+				Rectangle r;
+				r.position = pos;
+				r.orientation = o;
+				r.width = w;
+				r.height = h;
+				return r;
+			};
+			const dvec3 p = c.position;
+			return {
+				face(p - dvec3{0.0, 0.0, c.depth / 2.0}, {0.0, 0.0, -1.0}, c.width, c.height),
+				face(p + dvec3{0.0, 0.0, c.depth / 2.0}, {0.0, 0.0, 1.0}, c.width, c.height),
+				face(p + dvec3{0.0, c.height / 2.0, 0.0}, {0.0, 1.0, 0.0}, c.width, c.depth),
+				face(p - dvec3{0.0, c.height / 2.0, 0.0}, {0.0, -1.0, 0.0}, c.width, c.depth),
+				face(p + dvec3{c.width / 2.0, 0.0, 0.0}, {1.0, 0.0, 0.0}, c.depth, c.height),
+				face(p - dvec3{c.width / 2.0, 0.0, 0.0}, {-1.0, 0.0, 0.0}, c.depth, c.height),
+			};
+		}
+
+		// Cube::createBoundingBox (Cube.cpp:166-183)
+		AABBd	boxAround(const dvec3& p, double w, double h, double d)
+		{
+			AABBd b;
+			b.lo = {p[0] - w / 2.0, p[1] - h / 2.0, p[2] - d / 2.0};
+			b.hi = {p[0] + w / 2.0, p[1] + h / 2.0, p[2] + d / 2.0};
+			return (b);
+		}
+
+		// Sphere::createBoundingBox (Sphere.cpp:352-360)
+		AABBd	sphereBox(const dvec3& p, double r)
+		{
+			AABBd b;
+			b.lo = p - dvec3{r, r, r};
+			b.hi = p + dvec3{r, r, r};
+			return (b);
+		}
+
 		bool	instanceBox(const Instance& inst, const Blas& blas, AABBd& out)
 		{
 			// MeshInstance.cpp:212-262
@@ -494,7 +554,7 @@ namespace luzgpu
 				for (std::uint32_t h : items)
 				{
 					box = merge(box, boxes[h]);
-					instances = instances || (hittables[h].kindIndex >> 30) == GPU_HIT_INSTANCE;
+					instances = instances || (hittables[h].kindIndex >> kKindShift) == GPU_HIT_INSTANCE;
 				}
 				if (items.size() == 1 || (!instances && items.size() <= 4))
 				{
@@ -864,6 +924,7 @@ namespace luzgpu
 			g.alpha = static_cast<float>(std::max(1e-4, r * r));
 			g.roughness = static_cast<float>(m.roughness);
 			g.ior = static_cast<float>(m.ior);
+			g.anisotropy = static_cast<float>(m.anisotropy);
 			g.flags = (m.conductor ? GPU_MATF_CONDUCTOR : 0u)
 				| ((m.absorption[0] > 0.0 || m.absorption[1] > 0.0 || m.absorption[2] > 0.0) ? GPU_MATF_ABSORPTION : 0u);
 			p.materials.push_back(g);
@@ -927,7 +988,7 @@ namespace luzgpu
 					const Triangle& t = scene.triangles[h.index];
 					boxes[i] = triangleBox(t);
 					bounded[i] = true;
-					gh.kindIndex = (GPU_HIT_TRIANGLE << 30) | static_cast<std::uint32_t>(p.topTris.size());
+					gh.kindIndex = (GPU_HIT_TRIANGLE << kKindShift) | static_cast<std::uint32_t>(p.topTris.size());
 					p.topTris.push_back(packTriangle(t, static_cast<std::int32_t>(i)));
 					p.triangleCount++;
 					break;
@@ -936,23 +997,65 @@ namespace luzgpu
 				{
 					const Rectangle& r = scene.rectangles[h.index];
 					bounded[i] = rectangleBox(r, boxes[i]);
-					dvec3 n{}, wa{}, ha{};
-					rectangleBasis(r, n, wa, ha);
-					GpuRect g{};
+					gh.kindIndex = (GPU_HIT_RECTANGLE << kKindShift) | static_cast<std::uint32_t>(p.rects.size());
+					p.rects.push_back(packRect(r));
+					break;
+				}
+				case HIT_SPHERE:
+				{
+					const Sphere& sp = scene.spheres[h.index];
+					boxes[i] = sphereBox(sp.position, sp.radius);
+					bounded[i] = true;
+					GpuSphere g{};
+					for (int k = 0; k < 3; k++) g.center[k] = static_cast<float>(sp.position[k]);
+					g.radius = static_cast<float>(sp.radius);
+					gh.kindIndex = (GPU_HIT_SPHERE << kKindShift) | static_cast<std::uint32_t>(p.spheres.size());
+					p.spheres.push_back(g);
+					break;
+				}
+				case HIT_CUBE:
+				{
+					const Cube& c = scene.cubes[h.index];
+					boxes[i] = boxAround(c.position, c.width, c.height, c.depth);
+					bounded[i] = true;
+					gh.kindIndex = (GPU_HIT_CUBE << kKindShift) | static_cast<std::uint32_t>(p.cubeFaces.size() / 6);
+					for (const Rectangle& f : cubeFaces(c)) p.cubeFaces.push_back(packRect(f));
+					break;
+				}
+				case HIT_VOLUME:
+				{
+					const Volume& v = scene.volumes[h.index];
+					boxes[i] = v.sphere ? sphereBox(v.position, v.radius) : boxAround(v.position, v.size[0], v.size[1], v.size[2]);
+					bounded[i] = true;
+					GpuVolume g{};
 					for (int k = 0; k < 3; k++)
 					{
-						g.position[k] = static_cast<float>(r.position[k]);
-						g.normal[k] = static_cast<float>(n[k]);
-						g.widthAxis[k] = static_cast<float>(wa[k]);
-						g.heightAxis[k] = static_cast<float>(ha[k]);
+						g.center[k] = static_cast<float>(v.position[k]);
+						g.lo[k] = static_cast<float>(boxes[i].lo[k]);
+						g.hi[k] = static_cast<float>(boxes[i].hi[k]);
 					}
-					g.halfWidth = static_cast<float>(r.width / 2.0);
-					g.halfHeight = static_cast<float>(r.height / 2.0);
-					g.width = static_cast<float>(r.width);
-					g.height = static_cast<float>(r.height);
-					g.area = static_cast<float>(r.height * r.width);
-					gh.kindIndex = (GPU_HIT_RECTANGLE << 30) | static_cast<std::uint32_t>(p.rects.size());
-					p.rects.push_back(g);
+					g.radius = static_cast<float>(v.radius);
+					g.sphere = v.sphere ? 1u : 0u;
+					g.negInvDensity = static_cast<float>(-1.0 / v.density);
+					gh.kindIndex = (GPU_HIT_VOLUME << kKindShift) | static_cast<std::uint32_t>(p.volumes.size());
+					p.volumes.push_back(g);
+					break;
+				}
+				case HIT_PLANE:
+				{
+					const Plane& pl = scene.planes[h.index];
+					GpuPlane g{};
+					g.point[0] = 0.0f;
+					g.point[1] = static_cast<float>(pl.y);
+					g.point[2] = 0.0f;
+					for (int k = 0; k < 3; k++) g.orientation[k] = static_cast<float>(pl.orientation[k]);
+					gh.kindIndex = (GPU_HIT_PLANE << kKindShift) | static_cast<std::uint32_t>(p.planes.size());
+					p.planes.push_back(g);
+					break;
+				}
+				case HIT_DIRECTIONAL:
+				{
+					gh.kindIndex = (GPU_HIT_DIRECTIONAL << kKindShift) | h.index;
 					break;
 				}
 				case HIT_INSTANCE:
@@ -998,7 +1101,7 @@ namespace luzgpu
 					g.blasRoot = meshRoot[inst.mesh];
 					g.triBase = meshTriBase[inst.mesh];
 					g.normalBase = meshNormalBase[inst.mesh];
-					gh.kindIndex = (GPU_HIT_INSTANCE << 30) | static_cast<std::uint32_t>(p.instances.size());
+					gh.kindIndex = (GPU_HIT_INSTANCE << kKindShift) | static_cast<std::uint32_t>(p.instances.size());
 					p.instances.push_back(g);
 					break;
 				}
@@ -1013,12 +1116,16 @@ namespace luzgpu
 			if (bounded[i]) items.push_back(static_cast<std::uint32_t>(i));
 			else p.unbounded.push_back(static_cast<std::uint32_t>(i));
 		}
-		for (std::uint32_t u : p.unbounded)
+		// the device tests planes after the BVH (HitHelper.cpp:5-32); a directional light is
+		// never hit, and a rectangle without a box is degenerate and is never hit either
+		// (Rectangle::hit rejects it through the same basis test)
 		{
-			if ((p.hittables[u].kindIndex >> 30) != GPU_HIT_RECTANGLE)
+			std::vector<std::uint32_t> tested;
+			for (std::uint32_t u : p.unbounded)
 			{
-				throw std::runtime_error("Unsupported for the device path: unbounded hittable");
+				if ((p.hittables[u].kindIndex >> kKindShift) == GPU_HIT_PLANE) tested.push_back(u);
 			}
+			p.unbounded.swap(tested);
 		}
 		p.params.tlasRoot = -1;
 		if (!items.empty())
@@ -1063,6 +1170,12 @@ namespace luzgpu
 			p.tlasDeviceDepth = wideDepth(p.tlasWide, p.params.tlasRoot);
 			p.tlasStackBound = wideStackBound(p.tlasWide, p.params.tlasRoot);
 		}
+		p.tlasLeaf.reserve(2 * p.tlasPrims.size());
+		for (const std::uint32_t h : p.tlasPrims)
+		{
+			p.tlasLeaf.push_back(h);
+			p.tlasLeaf.push_back(p.hittables[h].kindIndex);
+		}
 
 		// ---- lights (Scene.cpp:615-638) -----------------------------------------------------
 		double total = 0.0;
@@ -1088,20 +1201,43 @@ namespace luzgpu
 				const double ls = dot(c, c);
 				area = ls <= 1e-24 ? 0.0 : std::sqrt(ls) / 2.0;
 				l.kind = GPU_HIT_TRIANGLE;
-				l.index = p.hittables[i].kindIndex & 0x3fffffffu;
+				l.index = p.hittables[i].kindIndex & kIndexMask;
 			}
 			else if (h.kind == HIT_RECTANGLE)
 			{
 				const Rectangle& r = scene.rectangles[h.index];
 				area = r.width * r.height;
 				l.kind = GPU_HIT_RECTANGLE;
-				l.index = p.hittables[i].kindIndex & 0x3fffffffu;
+				l.index = p.hittables[i].kindIndex & kIndexMask;
+			}
+			else if (h.kind == HIT_SPHERE)
+			{
+				// Sphere::lightSelectionWeight (Sphere.cpp:501-515)
+				const Sphere& sp = scene.spheres[h.index];
+				area = 4.0 * kPi * sp.radius * sp.radius;
+				l.kind = GPU_HIT_SPHERE;
+				l.index = p.hittables[i].kindIndex & kIndexMask;
+			}
+			else if (h.kind == HIT_DIRECTIONAL)
+			{
+				// DirectionalLight::sampleLight / lightSelectionWeight (DirectionalLight.cpp:74-103)
+				const DirectionalLight& d = scene.directionals[h.index];
+				for (int k = 0; k < 3; k++) l.direction[k] = static_cast<float>(d.direction[k] * -1.0);
+				l.kind = GPU_HIT_DIRECTIONAL;
+				area = 1.0;
+			}
+			else if (h.kind == HIT_CUBE || h.kind == HIT_PLANE)
+			{
+				// Hittable::sampleLight / pdfValue defaults (Hittable.cpp): never a valid
+				// sample, weight = luminance
+				l.kind = h.kind == HIT_CUBE ? GPU_HIT_CUBE : GPU_HIT_PLANE;
+				area = 1.0;
 			}
 			else
 			{
 				throw std::runtime_error("Unsupported for the device path: emissive mesh instance");
 			}
-			l.area = static_cast<float>(area);
+			l.area = static_cast<float>(h.kind == HIT_DIRECTIONAL || h.kind == HIT_CUBE || h.kind == HIT_PLANE ? 0.0 : area);
 			double w = (area <= 0.0 || !std::isfinite(lum) || lum <= 0.0) ? 0.0 : area * lum;
 			if (!std::isfinite(w) || w <= 0.0) w = 0.0;
 			total += w;
@@ -1201,6 +1337,7 @@ namespace luzgpu
 		gp.lightCount = static_cast<std::uint32_t>(p.lights.size());
 		gp.metersPerUnit = static_cast<float>(s.metersPerUnit);
 		gp.unboundedCount = static_cast<std::uint32_t>(p.unbounded.size());
+		gp.hasVolumes = p.volumes.empty() ? 0u : 1u;
 		return (p);
 	}
 }

@@ -41,6 +41,11 @@ namespace luzgpu
 		std::vector<GpuTri>	meshTris;
 		std::vector<float>	meshNormals;	// 9 floats per mesh triangle slot (leaf order)
 		std::vector<std::uint32_t>	tlasPrims;
+		std::vector<std::uint32_t>	tlasLeaf;	// per tlasPrims item: hittable index, kind << 28 | index
+		std::vector<GpuSphere>	spheres;
+		std::vector<GpuRect>	cubeFaces;	// six per cube
+		std::vector<GpuVolume>	volumes;
+		std::vector<GpuPlane>	planes;
 		std::vector<GpuHittable>	hittables;
 		std::vector<GpuRect>	rects;
 		std::vector<GpuInstance>	instances;

@@ -32,6 +32,8 @@ namespace luzgpu
 		MAT_METAL = 2,
 		MAT_DIELECTRIC = 3,
 		MAT_EMISSIVE = 4,
+		MAT_ISOTROPIC = 5,	// Isotropic.cpp:15-24
+		MAT_HG = 6,			// HenyeyGreenstein.cpp:54-63
 	};
 
 	struct	Material
@@ -44,6 +46,7 @@ namespace luzgpu
 		dvec3	eta = {1.0, 1.0, 1.0};
 		dvec3	k = {0.0, 0.0, 0.0};
 		dvec3	absorption = {0.0, 0.0, 0.0};	// dielectric interior absorption, 1/m
+		double	anisotropy = 0.0;			// Henyey-Greenstein g
 		std::string	name;
 	};
 
@@ -65,6 +68,48 @@ namespace luzgpu
 		HIT_TRIANGLE = 0,
 		HIT_RECTANGLE = 1,
 		HIT_INSTANCE = 2,
+		HIT_SPHERE = 3,
+		HIT_CUBE = 4,
+		HIT_VOLUME = 5,
+		HIT_PLANE = 6,			// unbounded
+		HIT_DIRECTIONAL = 7,	// unbounded, never hit
+	};
+
+	struct	Sphere
+	{
+		dvec3	position{};
+		double	radius = 1.0;
+	};
+
+	// Cube.cpp:72-95: six axis-aligned rectangles around the position (the orientation
+	// of the cube's transform is not used by its faces).
+	struct	Cube
+	{
+		dvec3	position{};
+		double	width = 1.0, height = 1.0, depth = 1.0;
+	};
+
+	// Plane.cpp: the plane through (0, y, 0) with normal `orientation` (not normalised).
+	struct	Plane
+	{
+		double	y = 0.0;
+		dvec3	orientation = {0.0, -1.0, 0.0};
+	};
+
+	// ConstantVolume (ConstantVolume.cpp) over a sphere or box boundary; density in scene
+	// units (Objects.cpp:733-739), phase function = the hittable's material.
+	struct	Volume
+	{
+		bool	sphere = false;
+		dvec3	position{};
+		double	radius = 1.0;
+		dvec3	size = {1.0, 1.0, 1.0};
+		double	density = 0.1;
+	};
+
+	struct	DirectionalLight
+	{
+		dvec3	direction = {0.0, -1.0, 0.0};	// normalised (DirectionalLight.cpp:9-17)
 	};
 
 	struct	Rectangle
@@ -144,6 +189,11 @@ namespace luzgpu
 		std::vector<Rectangle>	rectangles;
 		std::vector<Mesh>	meshes;
 		std::vector<Instance>	instances;
+		std::vector<Sphere>	spheres;
+		std::vector<Cube>	cubes;
+		std::vector<Plane>	planes;
+		std::vector<Volume>	volumes;
+		std::vector<DirectionalLight>	directionals;
 		std::vector<Hittable>	hittables;
 		Environment	environment;
 		bool	hasEnvironment = false;

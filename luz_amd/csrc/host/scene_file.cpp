@@ -312,6 +312,7 @@ namespace luzgpu
 				double ior = 1.5;
 				double radiance = -1.0;
 				double lum = -1.0;
+				double anisotropy = 0.0;
 				while (next(line))
 				{
 					const std::string t = trim(line);
@@ -381,6 +382,23 @@ namespace luzgpu
 								m.color = {color[0] * f, color[1] * f, color[2] * f};
 							}
 						}
+						else if (type == "isotropic")
+						{
+							m.kind = MAT_ISOTROPIC;
+							m.color = color;
+						}
+						else if (type == "volume" || type == "phase" || type == "henyey_greenstein" || type == "henyeygreenstein"
+							|| type == "hg")
+						{
+							// HenyeyGreenstein.cpp:10-18
+							if (!std::isfinite(anisotropy) || anisotropy < -0.99 || anisotropy > 0.99)
+							{
+								throw std::invalid_argument("Volume anisotropy must be finite and between -0.99 and 0.99.");
+							}
+							m.kind = MAT_HG;
+							m.color = color;
+							m.anisotropy = anisotropy;
+						}
 						else
 						{
 							throw std::runtime_error("Unsupported material type for the device path: " + type);
@@ -405,6 +423,7 @@ namespace luzgpu
 					else if (key == "ior" || key == "refractiveindex" || key == "refractive_index") ior = parseDouble(value, "ior");
 					else if (key == "radiance" || key == "surfaceradiance" || key == "surface_radiance") radiance = parseDouble(value, "radiance");
 					else if (key == "luminance" || key == "nits" || key == "cd_m2") lum = parseDouble(value, "luminance");
+					else if (key == "anisotropy" || key == "g") anisotropy = parseDouble(value, "anisotropy");
 					else throw std::runtime_error("Unsupported material property for the device path: " + line);
 				}
 				throw std::runtime_error("Material block '" + name + "' is missing a closing }.");
@@ -590,6 +609,198 @@ namespace luzgpu
 				return (material(value));
 			}
 
+			std::uint32_t	addHittable(HittableKind kind, std::size_t index, std::uint32_t mat)
+			{
+				scene.hittables.push_back(Hittable{kind, static_cast<std::uint32_t>(index), mat});
+				return (static_cast<std::uint32_t>(scene.hittables.size() - 1));
+			}
+
+			// volume NAME { ... } (Objects.cpp:1224-1330, 690-760): a ConstantVolume over a
+			// sphere or box boundary with an isotropic or Henyey-Greenstein phase function.
+			void	volumeBlock(const std::string& name)
+			{
+				std::string line;
+				std::string shape = "box";
+				Volume v;
+				v.size = {1.0, 1.0, 1.0};
+				double density = 0.1;
+				dvec3 color = {0.72, 0.78, 0.86};
+				double g = 0.0;
+				std::int64_t mat = -1;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "}")
+					{
+						auto positive = [&](double x, const std::string& what) {
+							if (!std::isfinite(x) || x <= 0.0)
+							{
+								throw std::runtime_error("Volume '" + name + "' " + what + " must be finite and positive.");
+							}
+						};
+						if (mat < 0)
+						{
+							Material m;
+							m.color = color;
+							if (std::fabs(g) <= 1e-12)
+							{
+								m.kind = MAT_ISOTROPIC;
+							}
+							else
+							{
+								if (!std::isfinite(g) || g < -0.99 || g > 0.99)
+								{
+									throw std::invalid_argument("Volume anisotropy must be finite and between -0.99 and 0.99.");
+								}
+								m.kind = MAT_HG;
+								m.anisotropy = g;
+							}
+							mat = static_cast<std::int64_t>(scene.materials.size());
+							scene.materials.push_back(m);
+						}
+						if (shape == "sphere")
+						{
+							positive(v.radius, "radius");
+							v.sphere = true;
+						}
+						else if (shape == "box" || shape == "cube")
+						{
+							positive(v.size[0], "width");
+							positive(v.size[1], "height");
+							positive(v.size[2], "depth");
+							v.sphere = false;
+						}
+						else
+						{
+							throw std::runtime_error("Unknown volume shape: " + shape);
+						}
+						positive(density, "density");
+						v.density = density * scene.settings.metersPerUnit;
+						positive(v.density, "scene-unit density");
+						scene.volumes.push_back(v);
+						addHittable(HIT_VOLUME, scene.volumes.size() - 1, static_cast<std::uint32_t>(mat));
+						return;
+					}
+					std::string key, value;
+					if (!splitAssignment(t, key, value))
+					{
+						throw std::runtime_error("Invalid volume property: " + t);
+					}
+					if (key == "shape" || key == "type") shape = lower(value);
+					else if (key == "position" || key == "center") v.position = parseTriple(value, "volume position");
+					else if (key == "size" || key == "dimensions") v.size = parseTriple(value, "volume size");
+					else if (key == "width") v.size[0] = parseDouble(value, key);
+					else if (key == "height") v.size[1] = parseDouble(value, key);
+					else if (key == "depth") v.size[2] = parseDouble(value, key);
+					else if (key == "radius") v.radius = parseDouble(value, key);
+					else if (key == "density" || key == "extinction" || key == "sigma_t") density = parseDouble(value, key);
+					else if (key == "color" || key == "albedo" || key == "scatteringcolor" || key == "scattering_color")
+						color = parseTriple(value, "volume color");
+					else if (key == "anisotropy" || key == "g") g = parseDouble(value, key);
+					else if (key == "material")
+					{
+						mat = material(value);
+						const MaterialKind k = scene.materials[static_cast<std::size_t>(mat)].kind;
+						if (k != MAT_ISOTROPIC && k != MAT_HG)
+						{
+							throw std::runtime_error("Unsupported for the device path: volume with a surface material");
+						}
+					}
+					else throw std::runtime_error("Unsupported volume property for the device path: " + t);
+				}
+				throw std::runtime_error("Volume '" + name + "' is missing a closing }.");
+			}
+
+			// directional_light NAME { ... } (Objects.cpp:1163-1222): a delta light with the
+			// emission color scaled to the given irradiance / illuminance (LightUnits.cpp:101-111).
+			void	directionalBlock(const std::string& name)
+			{
+				std::string line;
+				DirectionalLight d;
+				dvec3 direction = {0.0, -1.0, 0.0};
+				dvec3 color = {1.0, 1.0, 1.0};
+				double irradiance = -1.0;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "}")
+					{
+						if (dot(direction, direction) <= 0.0)
+						{
+							throw std::runtime_error("Directional light '" + name + "' direction must be non-zero.");
+						}
+						if (irradiance < 0.0)
+						{
+							throw std::runtime_error("Directional light must define irradiance, illuminance, or solar.");
+						}
+						Material m;
+						m.kind = MAT_EMISSIVE;
+						if (irradiance == 0.0)
+						{
+							m.color = {0.0, 0.0, 0.0};
+						}
+						else
+						{
+							const double cl = luminance(color);
+							if (!(cl > 0.0))
+							{
+								throw std::invalid_argument("Light color must have positive luminance for physical units.");
+							}
+							const double f = irradiance / cl;
+							m.color = {color[0] * f, color[1] * f, color[2] * f};
+						}
+						scene.materials.push_back(m);
+						const double len = std::sqrt(dot(direction, direction));
+						d.direction = direction * (1.0 / len);
+						scene.directionals.push_back(d);
+						addHittable(HIT_DIRECTIONAL, scene.directionals.size() - 1, static_cast<std::uint32_t>(scene.materials.size() - 1));
+						return;
+					}
+					std::string key, value;
+					if (!splitAssignment(t, key, value))
+					{
+						throw std::runtime_error("Invalid directional light property: " + t);
+					}
+					if (key == "direction") direction = parseTriple(value, "directional light direction");
+					else if (key == "color") color = parseTriple(value, "directional light color");
+					else if (key == "irradiance" || key == "w_m2" || key == "wm2" || key == "illuminance" || key == "lux" || key == "lx")
+					{
+						const double v = parseDouble(value, key);
+						if (!(v >= 0.0))
+						{
+							throw std::invalid_argument("Directional irradiance must be finite and non-negative.");
+						}
+						irradiance = (key == "irradiance" || key == "w_m2" || key == "wm2") ? v : v / 683.0;
+					}
+					else throw std::runtime_error("Unsupported directional light property for the device path: " + t);
+				}
+				throw std::runtime_error("Directional light '" + name + "' is missing a closing }.");
+			}
+
+			bool	lightOrVolumeBlock(const std::string& t)
+			{
+				std::string name;
+				if (namedBlock(t, "volume", name))
+				{
+					volumeBlock(name);
+					return (true);
+				}
+				if (namedBlock(t, "directional_light", name))
+				{
+					directionalBlock(name);
+					return (true);
+				}
+				return (false);
+			}
+
 			void	objects()
 			{
 				std::string line;
@@ -642,6 +853,58 @@ namespace luzgpu
 						scene.hittables.push_back(Hittable{HIT_RECTANGLE, static_cast<std::uint32_t>(scene.rectangles.size() - 1), primitiveMaterial(p[4], line)});
 						continue;
 					}
+					if (lightOrVolumeBlock(t))
+					{
+						continue;
+					}
+					if (lt.rfind("sphere=", 0) == 0)
+					{
+						// sphere=(x,y,z),r,material=NAME (Objects.cpp:1425-1448)
+						const auto p = splitTopLevel(t.substr(7));
+						if (p.size() != 3)
+						{
+							throw std::runtime_error("Invalid sphere object: " + line);
+						}
+						Sphere sp;
+						sp.position = parseTriple(p[0], "sphere position");
+						sp.radius = parseDouble(p[1], "sphere radius");
+						scene.spheres.push_back(sp);
+						addHittable(HIT_SPHERE, scene.spheres.size() - 1, primitiveMaterial(p[2], line));
+						continue;
+					}
+					if (lt.rfind("cube=", 0) == 0)
+					{
+						// cube=(p),(o),w,h,d,material=NAME (Objects.cpp:1449-1474)
+						const auto p = splitTopLevel(t.substr(5));
+						if (p.size() != 6)
+						{
+							throw std::runtime_error("Invalid cube object: " + line);
+						}
+						Cube c;
+						c.position = parseTriple(p[0], "cube position");
+						(void)parseTriple(p[1], "cube orientation");
+						c.width = parseDouble(p[2], "cube width");
+						c.height = parseDouble(p[3], "cube height");
+						c.depth = parseDouble(p[4], "cube depth");
+						scene.cubes.push_back(c);
+						addHittable(HIT_CUBE, scene.cubes.size() - 1, primitiveMaterial(p[5], line));
+						continue;
+					}
+					if (lt.rfind("plane=", 0) == 0)
+					{
+						// plane=y,(o),material=NAME (Objects.cpp:1475-1496)
+						const auto p = splitTopLevel(t.substr(6));
+						if (p.size() != 3)
+						{
+							throw std::runtime_error("Invalid plane object: " + line);
+						}
+						Plane pl;
+						pl.y = parseDouble(p[0], "plane y");
+						pl.orientation = parseTriple(p[1], "plane orientation");
+						scene.planes.push_back(pl);
+						addHittable(HIT_PLANE, scene.planes.size() - 1, primitiveMaterial(p[2], line));
+						continue;
+					}
 					if (lt.rfind("obj=", 0) == 0)
 					{
 						const auto p = splitTopLevel(t.substr(4));
@@ -692,6 +955,9 @@ namespace luzgpu
 					else if (lower(t) == "objects{" || lower(t) == "objects {")
 					{
 						objects();
+					}
+					else if (lightOrVolumeBlock(t))
+					{
 					}
 					else
 					{

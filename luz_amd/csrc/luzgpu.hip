@@ -187,15 +187,18 @@ namespace luzgpu
 		const Ray ray = cameraRay(p, i % p.width, i / p.width, ps);
 		LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
 		HitInfo h;
-		if (!traverse<false>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack))
+		if (!traverse<false>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack, nullptr, volumeLogU(p, ps)))
 		{
 			ids[2 * i] = -1;
 			ids[2 * i + 1] = -1;
 			ts[i] = -1.0f;
 			return;
 		}
+		// the second id is the triangle within the mesh for instances (cubes keep their
+		// face index in h.tri, which is not reported)
+		const bool instance = (s.hittables[h.hittable].kindIndex >> kKindShift) == GPU_HIT_INSTANCE;
 		ids[2 * i] = h.hittable;
-		ids[2 * i + 1] = h.tri >= 0 ? s.meshTris[h.tri].prim : -1;
+		ids[2 * i + 1] = instance ? s.meshTris[h.tri].prim : -1;
 		ts[i] = h.t;
 	}
 
@@ -215,7 +218,7 @@ namespace luzgpu
 	{
 		PackedScene	host;
 		DeviceBuffer	tlas, blas, topTris, meshTris, meshNormals, tlasPrims, hittables, rects, instances, materials,
-			lights, lightCdf, envPixels, envPdf, envCdf, unbounded;
+			lights, lightCdf, envPixels, envPdf, envCdf, unbounded, spheres, cubeFaces, volumes, planes;
 		SceneView	view{};
 		luzgpu_scene_info	info{};
 	};
@@ -284,7 +287,7 @@ namespace luzgpu
 			g.topTris.upload(h.topTris, st);
 			g.meshTris.upload(h.meshTris, st);
 			g.meshNormals.upload(h.meshNormals, st);
-			g.tlasPrims.upload(h.tlasPrims, st);
+			g.tlasPrims.upload(h.tlasLeaf, st);
 			g.hittables.upload(h.hittables, st);
 			g.rects.upload(h.rects, st);
 			g.instances.upload(h.instances, st);
@@ -295,6 +298,10 @@ namespace luzgpu
 			g.envPdf.upload(h.envPdf, st);
 			g.envCdf.upload(h.envCdf, st);
 			g.unbounded.upload(h.unbounded, st);
+			g.spheres.upload(h.spheres, st);
+			g.cubeFaces.upload(h.cubeFaces, st);
+			g.volumes.upload(h.volumes, st);
+			g.planes.upload(h.planes, st);
 			check(hipStreamSynchronize(st), "scene upload");
 			SceneView& v = g.view;
 			v.tlas = g.tlas.as<Bvh4Node>();
@@ -302,7 +309,7 @@ namespace luzgpu
 			v.topTris = g.topTris.as<GpuTri>();
 			v.meshTris = g.meshTris.as<GpuTri>();
 			v.meshNormals = g.meshNormals.as<float>();
-			v.tlasPrims = g.tlasPrims.as<std::uint32_t>();
+			v.tlasLeaf = g.tlasPrims.as<uint2>();
 			v.hittables = g.hittables.as<GpuHittable>();
 			v.rects = g.rects.as<GpuRect>();
 			v.instances = g.instances.as<GpuInstance>();
@@ -313,6 +320,11 @@ namespace luzgpu
 			v.envPdf = g.envPdf.as<float>();
 			v.envCdf = g.envCdf.as<double>();
 			v.unbounded = g.unbounded.as<std::uint32_t>();
+			v.unboundedCount = static_cast<std::uint32_t>(h.unbounded.size());
+			v.spheres = g.spheres.as<GpuSphere>();
+			v.cubeFaces = g.cubeFaces.as<GpuRect>();
+			v.volumes = g.volumes.as<GpuVolume>();
+			v.planes = g.planes.as<GpuPlane>();
 		}
 
 		GpuParams	resolveParams(const GpuScene& g, const luzgpu_settings* s)
@@ -491,6 +503,14 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		{
 			throw Error{LUZGPU_E_INVALID, "luzgpu_render: unknown integrator"};
 		}
+		WaveTuning tune;
+		if (settings)
+		{
+			tune.grid = settings->traversal_grid;
+			tune.extendRefill = settings->extend_refill;
+			tune.shadowRefill = settings->shadow_refill;
+		}
+		tune = tune.resolved();
 		const std::uint32_t pixels = p.width * p.height;
 		hipStream_t st = ctx->stream;
 		ctx->sums.reserve(static_cast<size_t>(pixels) * 3 * sizeof(double));
@@ -534,7 +554,7 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			else
 			{
 				renderWavefront(ctx->wave, scene->gpu.view, q, ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>(), st,
-					settings && settings->count_visits, settings && settings->stage_times ? &stages : nullptr);
+					settings && settings->count_visits, settings && settings->stage_times ? &stages : nullptr, tune);
 			}
 			check(hipGetLastError(), "render launch");
 			check(hipEventRecord(ke.back(), st), "hipEventRecord");

@@ -133,8 +133,12 @@ class SceneFile:
 
 
 def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=False, resolution=None,
-              count_visits=False, stage_times=False):
+              count_visits=False, stage_times=False, tuning=None):
     s = _lib.Settings()
+    if tuning:
+        s.traversal_grid = int(tuning.get("grid", 0))
+        s.extend_refill = int(tuning.get("extend_refill", 0))
+        s.shadow_refill = int(tuning.get("shadow_refill", 0))
     s.count_visits = 1 if count_visits else 0
     s.stage_times = 1 if stage_times else 0
     if resolution is not None:
@@ -154,11 +158,14 @@ class Renderer:
 
     @staticmethod
     def render(scene: Scene, seed: int, samples=None, max_bounces=None, sample_begin=0, sample_count=None,
-               integrator: str = "wavefront", out=None, resolution=None, count_visits=False, stage_times=False):
+               integrator: str = "wavefront", out=None, resolution=None, count_visits=False, stage_times=False,
+               tuning=None):
         """Returns (image[H, W, 3] float32, RenderStats). ``out`` may be a device tensor
-        (torch, float32, contiguous, H*W*3) that receives the image without a host copy."""
+        (torch, float32, contiguous, H*W*3) that receives the image without a host copy.
+        ``tuning`` ({"grid", "extend_refill", "shadow_refill"}) sets the wavefront launch
+        shape; it never changes the image."""
         s = _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=out is not None,
-                      resolution=resolution, count_visits=count_visits, stage_times=stage_times)
+                      resolution=resolution, count_visits=count_visits, stage_times=stage_times, tuning=tuning)
         st = _lib.Stats()
         w, h = resolution if resolution is not None else (scene.width, scene.height)
         if out is not None:

@@ -15,6 +15,15 @@ Workloads
                    radius U[0.5, 3], centres uniform in a 200^3 box; 40 % Lambertian, 30 %
                    GGX metal (alpha U[0.05, 0.5]), 30 % smooth dielectric IOR 1.5; procedural
                    HDR sky (gradient + sun disk of 5e4 cd/m^2); 1920x1080, 64 spp, depth 12.
+* ``primitives``   analytic primitives: a ground plane, spheres (glass, Lambertian, an
+                   emissive sphere light), cubes (rough metal, an emissive cube), a rectangle
+                   light, under a constant background; 160x120, 32 spp, depth 8.
+* ``volume``       participating media: a Henyey-Greenstein box of fog (g 0.6) and an
+                   isotropic sphere of smoke over a ground plane, lit by a directional sun and
+                   a sphere light; 160x120, 32 spp, depth 16.
+* ``terrain``      a 1,002,528-triangle displaced grid terrain inside a box of
+                   Henyey-Greenstein fog (sigma_t 0.05, albedo 0.9, g 0.6) under a directional
+                   light; 1280x720, 32 spp, depth 16.
 """
 
 from __future__ import annotations
@@ -279,6 +288,135 @@ def spheres_scene(directory: str, width=1920, height=1080, spp=64, depth=12, cou
     _write_if_changed(path, ("\n".join(lines) + "\n").encode())
     return Workload("spheres", directory, path, width, height, spp, depth, render_seed,
                     count * 2 * SPHERE_SLICES * (SPHERE_STACKS - 1), {"instances": count})
+
+
+_HEADER = ["adaptive=0", "denoise=0", "view_transform=raw", "bloom=0", "outputfilename=render.tiff"]
+
+
+def primitives_scene_text(width=160, height=120, spp=32, depth=8) -> str:
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "sky=none", "background=(0.05,0.06,0.08)", "",
+        "[materials]",
+        "material floor {", "type=lambertian", "color=(0.6,0.6,0.6)", "}",
+        "material red {", "type=lambertian", "color=(0.7,0.15,0.1)", "}",
+        "material glass {", "type=dielectric", "color=(1,1,1)", "ior=1.5", "roughness=0", "}",
+        "material brushed {", "type=metal", "color=(0.85,0.8,0.7)", "roughness=0.35", "}",
+        "material mirror {", "type=metal", "color=(0.9,0.9,0.9)", "roughness=0", "}",
+        "material bulb {", "type=emissive", "color=(1,0.85,0.6)", "radiance=12", "}",
+        "material panel {", "type=emissive", "color=(0.6,0.7,1)", "radiance=4", "}",
+        "material block {", "type=emissive", "color=(1,0.3,0.2)", "radiance=2", "}",
+        "",
+        "[scene]",
+        "camera main {", "position=(0,3,-12)", "direction=(0,-0.15,1)", "up=(0,1,0)", "focal_length_mm=35",
+        "sensor_width_mm=36", "sensor_height_mm=27", "pinhole=1", "focus_distance=10", "}",
+        "objects{",
+        "plane=0,(0,1,0),material=floor",
+        "sphere=(-2.5,1.5,2),1.5,material=glass",
+        "sphere=(2.5,1,1),1,material=red",
+        "sphere=(0.5,0.6,-1.5),0.6,material=mirror",
+        "cube=(0,1,4),(0,0,-1),2,2,2,material=brushed",
+        "sphere=(0,6,2),0.75,material=bulb",
+        "rectangle=(-4,5,6),(0.5,-1,-0.3),2,2,material=panel",
+        "cube=(4,3,6),(0,0,-1),1,1,1,material=block",
+        "}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def write_primitives(directory: str, width=160, height=120, spp=32, depth=8, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "primitives.luz")
+    _write_if_changed(path, primitives_scene_text(width, height, spp, depth).encode())
+    return Workload("primitives", directory, path, width, height, spp, depth, seed, 0)
+
+
+def volume_scene_text(width=160, height=120, spp=32, depth=16) -> str:
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "sky=none", "background=(0.02,0.02,0.03)", "",
+        "[materials]",
+        "material ground {", "type=lambertian", "color=(0.5,0.45,0.4)", "}",
+        "material steel {", "type=metal", "color=(0.8,0.8,0.85)", "roughness=0.2", "}",
+        "material glass {", "type=dielectric", "color=(1,1,1)", "ior=1.5", "roughness=0", "}",
+        "material lamp {", "type=emissive", "color=(1,0.8,0.5)", "radiance=20", "}",
+        "",
+        "[scene]",
+        "camera main {", "position=(0,4,-20)", "direction=(0,-0.1,1)", "up=(0,1,0)", "focal_length_mm=35",
+        "sensor_width_mm=36", "sensor_height_mm=27", "pinhole=1", "focus_distance=10", "}",
+        "objects{",
+        "plane=0,(0,1,0),material=ground",
+        "sphere=(0,2,0),2,material=steel",
+        "sphere=(-5,1.5,-3),1.5,material=glass",
+        "cube=(-4,1,4),(0,0,-1),2,2,2,material=ground",
+        "sphere=(4,5,-2),0.5,material=lamp",
+        "volume fog {", "shape=box", "position=(0,5,0)", "size=(30,10,30)", "density=0.05",
+        "color=(0.9,0.9,0.9)", "anisotropy=0.6", "}",
+        "volume smoke {", "shape=sphere", "position=(5,3,3)", "radius=2", "density=0.5", "color=(0.8,0.7,0.6)", "}",
+        "directional_light sun {", "direction=(-1,-2,-1)", "color=(1,0.95,0.9)", "irradiance=3", "}",
+        "}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def write_volume(directory: str, width=160, height=120, spp=32, depth=16, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "volume.luz")
+    _write_if_changed(path, volume_scene_text(width, height, spp, depth).encode())
+    return Workload("volume", directory, path, width, height, spp, depth, seed, 0)
+
+
+TERRAIN_QUADS = 708  # 708 x 708 quads x 2 = 1,002,528 triangles
+
+
+def _terrain_obj(rng: random.Random, n: int, extent: float) -> bytes:
+    """Height field over an n x n grid: a sum of seeded sinusoidal ridges."""
+    waves = [(rng.uniform(0.0, 2 * math.pi), rng.uniform(0.02, 0.12), rng.uniform(0.0, 2 * math.pi),
+              rng.uniform(0.5, 3.0)) for _ in range(8)]
+    xs = np.linspace(-extent, extent, n + 1)
+    gx, gz = np.meshgrid(xs, xs)
+    hgt = np.zeros_like(gx)
+    for ang, freq, phase, amp in waves:
+        hgt += amp * np.sin(freq * (gx * math.cos(ang) + gz * math.sin(ang)) + phase)
+    verts = np.stack([gx.ravel(), hgt.ravel(), gz.ravel()], axis=1)
+    vtxt = "\n".join("v %.6f %.6f %.6f" % tuple(v) for v in verts)
+    i = np.arange(n)
+    a = (i[None, :] + (n + 1) * i[:, None]).ravel() + 1  # 1-based corner (row r, column c)
+    b, c, d = a + 1, a + (n + 1), a + (n + 2)
+    faces = np.empty((2 * n * n, 3), dtype=np.int64)
+    faces[0::2] = np.stack([a, c, b], axis=1)
+    faces[1::2] = np.stack([b, c, d], axis=1)
+    ftxt = "\n".join("f %d %d %d" % tuple(f) for f in faces)
+    return (vtxt + "\n" + ftxt + "\n").encode()
+
+
+def terrain_scene(directory: str, width=1280, height=720, spp=32, depth=16, quads=TERRAIN_QUADS, seed=1005,
+                  render_seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    rng = random.Random(seed)
+    obj = os.path.join(directory, f"terrain_{quads}.obj")
+    if not os.path.exists(obj):
+        _write_if_changed(obj, _terrain_obj(rng, quads, 100.0))
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "sky=none", "background=(0.03,0.04,0.06)", "",
+        "[materials]",
+        "material rock {", "type=lambertian", "color=(0.45,0.4,0.35)", "}",
+        "",
+        "[scene]",
+        "camera main {", "position=(0,25,-120)", "direction=(0,-0.25,1)", "up=(0,1,0)", "focal_length_mm=28",
+        "sensor_width_mm=36", "sensor_height_mm=20.25", "pinhole=1", "focus_distance=10", "}",
+        "objects{",
+        f"object land {{", f"file={os.path.basename(obj)}", "material=rock", "}",
+        "volume haze {", "shape=box", "position=(0,15,0)", "size=(210,50,210)", "density=0.05",
+        "color=(0.9,0.9,0.9)", "anisotropy=0.6", "}",
+        "directional_light sun {", "direction=(-0.4,-1,0.6)", "color=(1,0.95,0.88)", "irradiance=4", "}",
+        "}",
+    ]
+    path = os.path.join(directory, f"terrain_{quads}.luz")
+    _write_if_changed(path, ("\n".join(lines) + "\n").encode())
+    return Workload("terrain", directory, path, width, height, spp, depth, render_seed, 2 * quads * quads,
+                    {"instances": 1})
 
 
 def _write_if_changed(path: str, data: bytes) -> None:

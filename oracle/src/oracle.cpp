@@ -147,7 +147,8 @@ namespace oracle
 	enum Dim : std::uint32_t
 	{
 		D_CAMERA = 0, D_LENS = 1, D_LIGHT_STRATEGY = 2, D_LIGHT_SELECTION = 3, D_LIGHT_SURFACE_POINT = 5,
-		D_BSDF_DIRECTION = 6, D_MATERIAL_DECISION = 7, D_RR = 9, D_ENV_SELECTION = 12, D_ENV_POINT = 13, D_ENV_STRATEGY = 14
+		D_BSDF_DIRECTION = 6, D_MATERIAL_DECISION = 7, D_RR = 9, D_VOLUME_DISTANCE = 10, D_ENV_SELECTION = 12,
+		D_ENV_POINT = 13, D_ENV_STRATEGY = 14
 	};
 
 	// ---- rays (Ray.hpp) -----------------------------------------------------------------------
@@ -263,6 +264,152 @@ namespace oracle
 			wd = dot(d, wa);
 			hd = dot(d, ha);
 			return !(std::fabs(wd) > (w / 2.0) || std::fabs(hd) > (h / 2.0));
+		}
+	};
+
+	// Rectangle basis from the orientation (Rectangle.cpp:14-50, 127-152)
+	Rect	makeRect(V pos, V ori, double w, double h, std::uint32_t material)
+	{
+		Rect o;
+		o.pos = pos;
+		o.w = w;
+		o.h = h;
+		o.ok = len2(ori) > 0.0;
+		if (o.ok)
+		{
+			o.n = normalize(ori);
+			if (std::fabs(o.n.y) > 0.999) { o.wa = {1, 0, 0}; o.ha = {0, 0, 1}; }
+			else if (std::fabs(o.n.z) > 0.999) { o.wa = {1, 0, 0}; o.ha = {0, 1, 0}; }
+			else if (std::fabs(o.n.x) > 0.999) { o.wa = {0, 0, 1}; o.ha = {0, 1, 0}; }
+			else
+			{
+				V helper = {0, 1, 0};
+				if (std::fabs(dot(helper, o.n)) > 0.9) helper = {1, 0, 0};
+				o.wa = normalize(cross(helper, o.n));
+				o.ha = normalize(cross(o.n, o.wa));
+			}
+		}
+		o.material = material;
+		return o;
+	}
+
+	inline bool	zeroDir(const Ray& r) { return r.d.x == 0.0 && r.d.y == 0.0 && r.d.z == 0.0; }
+
+	struct	Sphere	// Sphere.cpp
+	{
+		V	c;
+		double	r = 1.0;
+
+		// Sphere::hit / hitAny (:226-301): unit direction, a = 1
+		bool	intersect(const Ray& ray, double tMin, double tMax, double& t) const
+		{
+			if (zeroDir(ray)) return false;
+			const V oc = ray.o - c;
+			const double b = dot(oc, ray.d);
+			const double cc = dot(oc, oc) - (r * r);
+			const double disc = (b * b) - cc;
+			if (disc < 0.0) return false;
+			const double sq = std::sqrt(disc);
+			double root = -b - sq;
+			if (root < tMin || root > tMax)
+			{
+				root = -b + sq;
+				if (root < tMin || root > tMax) return false;
+			}
+			t = root;
+			return true;
+		}
+		// Sphere::hitInterval (:303-339)
+		bool	interval(const Ray& ray, double tMin, double tMax, double& t0, double& t1) const
+		{
+			if (zeroDir(ray)) return false;
+			const V oc = ray.o - c;
+			const double b = dot(oc, ray.d);
+			const double cc = dot(oc, oc) - (r * r);
+			const double disc = (b * b) - cc;
+			if (disc < 0.0) return false;
+			const double sq = std::sqrt(disc);
+			double r0 = -b - sq, r1 = -b + sq;
+			if (r0 > r1) std::swap(r0, r1);
+			if (r0 < tMin) r0 = tMin;
+			if (r1 > tMax) r1 = tMax;
+			if (r0 >= r1) return false;
+			t0 = r0;
+			t1 = r1;
+			return true;
+		}
+	};
+
+	// Cube::hitInterval over its bounding box (Cube.cpp:120-164)
+	inline bool	boxInterval(V lo, V hi, const Ray& r, double tMin, double tMax, double& t0, double& t1)
+	{
+		double a0 = tMin, a1 = tMax;
+		for (int a = 0; a < 3; a++)
+		{
+			const double d = at(r.d, a), o = at(r.o, a);
+			if (d == 0.0)
+			{
+				if (o < at(lo, a) || o > at(hi, a)) return false;
+				continue;
+			}
+			const double inv = 1.0 / d;
+			double x0 = (at(lo, a) - o) * inv, x1 = (at(hi, a) - o) * inv;
+			if (x0 > x1) std::swap(x0, x1);
+			a0 = std::max(a0, x0);
+			a1 = std::min(a1, x1);
+			if (a0 >= a1) return false;
+		}
+		t0 = a0;
+		t1 = a1;
+		return true;
+	}
+
+	struct	Cube	// Cube.cpp:72-118
+	{
+		Rect	faces[6];
+		V	lo, hi;
+	};
+
+	struct	Plane	// Plane.cpp:51-87
+	{
+		double	y = 0.0;
+		V	o;
+		bool	intersect(const Ray& r, double tMin, double tMax, double& t) const
+		{
+			const V pos = {0.0, y, 0.0};
+			const double d = dot(r.d, o);
+			if (d == 0.0) return false;
+			t = dot(pos - r.o, o) / d;
+			return !(t > tMax || t < tMin);
+		}
+	};
+
+	struct	Volume	// ConstantVolume.cpp
+	{
+		bool	isSphere = false;
+		Sphere	sphere;
+		V	lo, hi;
+		double	negInv = -1.0;
+
+		// sampleScatteringDistance (:62-101) with u = DIM_VOLUME_DISTANCE of this bounce
+		bool	sample(const Ray& r, double tMin, double tMax, double u, double& t) const
+		{
+			double e0, e1;
+			if (isSphere ? !sphere.interval(r, -T_MAX, T_MAX, e0, e1) : (zeroDir(r) || !boxInterval(lo, hi, r, -T_MAX, T_MAX, e0, e1)))
+			{
+				return false;
+			}
+			if (e0 < tMin) e0 = tMin;
+			if (e1 > tMax) e1 = tMax;
+			if (e0 >= e1) return false;
+			if (e0 < 0) e0 = 0;
+			const double len = std::sqrt(len2(r.d));
+			if (len <= 0.0 || !std::isfinite(len)) return false;
+			const double inside = (e1 - e0) * len;
+			const double dist = negInv * std::log(std::max(u, 1e-12));
+			if (dist > inside) return false;
+			t = e0 + dist / len;
+			return true;
 		}
 	};
 
@@ -437,7 +584,7 @@ namespace oracle
 		}
 	};
 
-	enum Kind { K_TRI, K_RECT, K_INST };
+	enum Kind { K_TRI, K_RECT, K_INST, K_SPHERE, K_CUBE, K_VOLUME, K_PLANE, K_DIR };
 
 	struct	Top
 	{
@@ -450,7 +597,7 @@ namespace oracle
 	{
 		luzgpu::MaterialKind	kind;
 		C	color;
-		double	roughness = 0, alpha = 1e-4, ior = 1.5;
+		double	roughness = 0, alpha = 1e-4, ior = 1.5, g = 0.0;
 	};
 
 	struct	Light
@@ -459,6 +606,7 @@ namespace oracle
 		std::uint32_t	index;
 		C	emitted;
 		double	area;
+		V	dir;	// directional: -direction
 	};
 
 	struct	Env
@@ -475,6 +623,11 @@ namespace oracle
 		std::vector<Top>	top;
 		std::vector<Tri>	topTris;	// indexed by Top.index for K_TRI
 		std::vector<Rect>	rects;
+		std::vector<Sphere>	spheres;
+		std::vector<Cube>	cubes;
+		std::vector<Plane>	planes;
+		std::vector<Volume>	volumes;
+		std::vector<V>	directionals;
 		std::vector<Mesh>	meshes;
 		std::vector<Instance>	instances;
 		std::vector<Mat>	mats;
@@ -533,6 +686,7 @@ namespace oracle
 			const double r = std::isfinite(m.roughness) ? std::max(0.0, std::min(1.0, m.roughness)) : 0.0;
 			o.alpha = std::max(1e-4, r * r);	// BSDF.cpp:54-58
 			o.ior = m.ior;
+			o.g = m.anisotropy;
 			s.mats.push_back(o);
 		}
 		for (const auto& mesh : in.meshes)
@@ -574,30 +728,61 @@ namespace oracle
 			else if (h.kind == luzgpu::HIT_RECTANGLE)
 			{
 				const auto& r = in.rectangles[h.index];
-				Rect o;
-				o.pos = fromD(r.position);
-				o.w = r.width;
-				o.h = r.height;
-				const V ori = fromD(r.orientation);
-				o.ok = len2(ori) > 0.0;	// Rectangle.cpp:14-50
-				if (o.ok)
-				{
-					o.n = normalize(ori);
-					if (std::fabs(o.n.y) > 0.999) { o.wa = {1, 0, 0}; o.ha = {0, 0, 1}; }
-					else if (std::fabs(o.n.z) > 0.999) { o.wa = {1, 0, 0}; o.ha = {0, 1, 0}; }
-					else if (std::fabs(o.n.x) > 0.999) { o.wa = {0, 0, 1}; o.ha = {0, 1, 0}; }
-					else
-					{
-						V helper = {0, 1, 0};
-						if (std::fabs(dot(helper, o.n)) > 0.9) helper = {1, 0, 0};
-						o.wa = normalize(cross(helper, o.n));
-						o.ha = normalize(cross(o.n, o.wa));
-					}
-				}
-				o.material = h.material;
 				t.kind = K_RECT;
 				t.index = static_cast<std::uint32_t>(s.rects.size());
-				s.rects.push_back(o);
+				s.rects.push_back(makeRect(fromD(r.position), fromD(r.orientation), r.width, r.height, h.material));
+			}
+			else if (h.kind == luzgpu::HIT_SPHERE)
+			{
+				const auto& sp = in.spheres[h.index];
+				t.kind = K_SPHERE;
+				t.index = static_cast<std::uint32_t>(s.spheres.size());
+				s.spheres.push_back(Sphere{fromD(sp.position), sp.radius});
+			}
+			else if (h.kind == luzgpu::HIT_CUBE)
+			{
+				// Cube::_generateFaces (Cube.cpp:72-95)
+				const auto& c = in.cubes[h.index];
+				const V p = fromD(c.position);
+				Cube o;
+				o.faces[0] = makeRect(p - V{0.0, 0.0, c.depth / 2.0}, {0, 0, -1}, c.width, c.height, h.material);
+				o.faces[1] = makeRect(p + V{0.0, 0.0, c.depth / 2.0}, {0, 0, 1}, c.width, c.height, h.material);
+				o.faces[2] = makeRect(p + V{0.0, c.height / 2.0, 0.0}, {0, 1, 0}, c.width, c.depth, h.material);
+				o.faces[3] = makeRect(p - V{0.0, c.height / 2.0, 0.0}, {0, -1, 0}, c.width, c.depth, h.material);
+				o.faces[4] = makeRect(p + V{c.width / 2.0, 0.0, 0.0}, {1, 0, 0}, c.depth, c.height, h.material);
+				o.faces[5] = makeRect(p - V{c.width / 2.0, 0.0, 0.0}, {-1, 0, 0}, c.depth, c.height, h.material);
+				o.lo = {p.x - c.width / 2.0, p.y - c.height / 2.0, p.z - c.depth / 2.0};
+				o.hi = {p.x + c.width / 2.0, p.y + c.height / 2.0, p.z + c.depth / 2.0};
+				t.kind = K_CUBE;
+				t.index = static_cast<std::uint32_t>(s.cubes.size());
+				s.cubes.push_back(o);
+			}
+			else if (h.kind == luzgpu::HIT_VOLUME)
+			{
+				const auto& v = in.volumes[h.index];
+				Volume o;
+				o.isSphere = v.sphere;
+				o.sphere = Sphere{fromD(v.position), v.radius};
+				const V p = fromD(v.position);
+				o.lo = {p.x - v.size[0] / 2.0, p.y - v.size[1] / 2.0, p.z - v.size[2] / 2.0};
+				o.hi = {p.x + v.size[0] / 2.0, p.y + v.size[1] / 2.0, p.z + v.size[2] / 2.0};
+				o.negInv = -1.0 / v.density;
+				t.kind = K_VOLUME;
+				t.index = static_cast<std::uint32_t>(s.volumes.size());
+				s.volumes.push_back(o);
+			}
+			else if (h.kind == luzgpu::HIT_PLANE)
+			{
+				const auto& pl = in.planes[h.index];
+				t.kind = K_PLANE;
+				t.index = static_cast<std::uint32_t>(s.planes.size());
+				s.planes.push_back(Plane{pl.y, fromD(pl.orientation)});
+			}
+			else if (h.kind == luzgpu::HIT_DIRECTIONAL)
+			{
+				t.kind = K_DIR;
+				t.index = static_cast<std::uint32_t>(s.directionals.size());
+				s.directionals.push_back(fromD(in.directionals[h.index].direction));
 			}
 			else
 			{
@@ -622,12 +807,23 @@ namespace oracle
 		{
 			const Top& t = s.top[i];
 			if (s.mats[t.material].kind != luzgpu::MAT_EMISSIVE) continue;
-			Light l{t.kind, t.index, s.mats[t.material].color, 0.0};
-			if (t.kind == K_TRI) l.area = s.topTris[t.index].area;
-			else if (t.kind == K_RECT) l.area = s.rects[t.index].h * s.rects[t.index].w;
-			else throw std::runtime_error("oracle: emissive mesh instances are not restated");
+			Light l{t.kind, t.index, s.mats[t.material].color, 0.0, V{}};
 			const double lum = luminance(l.emitted);
-			double w = (l.area <= 0.0 || !std::isfinite(lum) || lum <= 0.0) ? 0.0 : l.area * lum;
+			double w;
+			if (t.kind == K_TRI || t.kind == K_RECT || t.kind == K_SPHERE)
+			{
+				if (t.kind == K_TRI) l.area = s.topTris[t.index].area;
+				else if (t.kind == K_RECT) l.area = s.rects[t.index].h * s.rects[t.index].w;
+				else l.area = 4.0 * PI * s.spheres[t.index].r * s.spheres[t.index].r;	// Sphere.cpp:501-515
+				w = (l.area <= 0.0 || !std::isfinite(lum) || lum <= 0.0) ? 0.0 : l.area * lum;
+			}
+			else if (t.kind == K_DIR || t.kind == K_CUBE || t.kind == K_PLANE)
+			{
+				// Hittable / DirectionalLight::lightSelectionWeight: the luminance
+				if (t.kind == K_DIR) l.dir = s.directionals[t.index] * -1.0;
+				w = (!std::isfinite(lum) || lum <= 0.0) ? 0.0 : lum;
+			}
+			else throw std::runtime_error("oracle: emissive mesh instances are not restated");
 			if (!std::isfinite(w) || w <= 0.0) w = 0.0;
 			s.lightTotal += w;
 			s.lightCdf.push_back(s.lightTotal);
@@ -700,13 +896,69 @@ namespace oracle
 	}
 
 	// ---- scene queries (HitHelper.cpp:5-32, ColorHelper.cpp:801-829) ----------------------------
-	bool	closestHit(const Scene& s, const Ray& r, Hit& h, bool any = false, double tMax = T_MAX)
+	bool	closestHit(const Scene& s, const Ray& r, Hit& h, const Sampler* ps, bool any = false, double tMax = T_MAX)
 	{
 		bool found = false;
 		double closest = tMax;
+		// bounded hittables (the acceleration structure), then the unbounded ones
+		for (int pass = 0; pass < 2; pass++)
 		for (std::size_t i = 0; i < s.top.size(); i++)
 		{
 			const Top& t = s.top[i];
+			const bool unbounded = t.kind == K_PLANE || t.kind == K_DIR;
+			if (unbounded != (pass == 1)) continue;
+			if (t.kind == K_DIR) continue;	// DirectionalLight::hit is always false
+			if (t.kind == K_SPHERE || t.kind == K_PLANE || t.kind == K_VOLUME)
+			{
+				double tt;
+				bool ok;
+				if (t.kind == K_SPHERE) ok = s.spheres[t.index].intersect(r, T_MIN, closest, tt);
+				else if (t.kind == K_PLANE) ok = s.planes[t.index].intersect(r, T_MIN, closest, tt);
+				else ok = s.volumes[t.index].sample(r, T_MIN, closest, ps->s1(D_VOLUME_DISTANCE), tt);
+				if (!ok) continue;
+				if (any) return true;
+				h.t = tt;
+				h.p = r.at(tt);
+				if (t.kind == K_SPHERE)
+				{
+					const Sphere& sp = s.spheres[t.index];
+					setFace(h, r, (h.p - sp.c) / sp.r, h.n, h.front);
+				}
+				else if (t.kind == K_PLANE)
+				{
+					setFace(h, r, s.planes[t.index].o, h.n, h.front);
+				}
+				else
+				{
+					h.n = {1.0, 0.0, 0.0};	// ConstantVolume::hit: arbitrary normal
+				}
+				h.gn = h.n;
+				h.material = t.material;
+				h.top = static_cast<std::int64_t>(i);
+				h.tri = -1;
+				closest = tt;
+				found = true;
+				continue;
+			}
+			if (t.kind == K_CUBE)
+			{
+				for (const Rect& rc : s.cubes[t.index].faces)
+				{
+					double tt, wd, hd;
+					if (!rc.intersect(r, T_MIN, closest, tt, wd, hd)) continue;
+					if (any) return true;
+					h.t = tt;
+					setFace(h, r, rc.n, h.n, h.front);
+					h.gn = h.n;
+					h.p = r.at(tt);
+					h.material = t.material;
+					h.top = static_cast<std::int64_t>(i);
+					h.tri = -1;
+					closest = tt;
+					found = true;
+				}
+				continue;
+			}
 			if (t.kind == K_TRI)
 			{
 				double tt, u, v;
@@ -777,11 +1029,11 @@ namespace oracle
 		return found;
 	}
 
-	bool	occluded(const Scene& s, const Ray& r, double tMax)	// ColorHelper.cpp:801-829
+	bool	occluded(const Scene& s, const Ray& r, double tMax, const Sampler* ps)	// ColorHelper.cpp:801-829
 	{
 		if (tMax <= T_MIN) return false;
 		Hit h;
-		return closestHit(s, r, h, true, tMax);
+		return closestHit(s, r, h, ps, true, tMax);
 	}
 
 	// ---- BSDF helpers (BSDF.cpp) -----------------------------------------------------------------
@@ -928,7 +1180,7 @@ namespace oracle
 	}
 
 	// ---- scatter (Material.hpp:23-41 + each material's scatter) -----------------------------------
-	enum PdfType { PDF_NONE, PDF_COSINE, PDF_BSDF };
+	enum PdfType { PDF_NONE, PDF_COSINE, PDF_SPHERE, PDF_HG, PDF_BSDF };
 	struct	Scatter
 	{
 		PdfType	type = PDF_NONE;
@@ -937,6 +1189,8 @@ namespace oracle
 		V	dir;		// specular ray direction (normalized) or sampled BSDF direction
 		V	incident;
 		Onb	basis = Onb(V{0, 0, 1});
+		V	phaseDir;	// Henyey-Greenstein
+		double	g = 0.0;
 	};
 
 	double	bsdfPdf(const Mat& m, const Hit& h, const Scatter& sc, V l)
@@ -960,6 +1214,18 @@ namespace oracle
 			const double c = dot(l, sc.basis.w);
 			return c <= 0.0 ? 0.0 : c / PI;
 		}
+		if (sc.type == PDF_SPHERE) return 1.0 / (4.0 * PI);
+		if (sc.type == PDF_HG)	// :56-77
+		{
+			const double dl2 = len2(l), pl2 = len2(sc.phaseDir);
+			if (dl2 <= 0.0 || pl2 <= 0.0 || !std::isfinite(dl2) || !std::isfinite(pl2)) return 0.0;
+			const V nd = l / std::sqrt(dl2), pd = sc.phaseDir / std::sqrt(pl2);
+			const double c = std::max(-1.0, std::min(1.0, dot(nd, pd)));
+			const double g = std::max(-0.99, std::min(0.99, sc.g));
+			const double g2 = g * g;
+			const double den = std::max(1e-12, 1.0 + g2 - (2.0 * g * c));
+			return (1.0 - g2) / (4.0 * PI * den * std::sqrt(den));
+		}
 		if (sc.type == PDF_BSDF) return bsdfPdf(m, h, sc, l);
 		return 0.0;
 	}
@@ -977,6 +1243,20 @@ namespace oracle
 			sc.att = m.color;
 			sc.type = PDF_COSINE;
 			sc.basis = Onb(h.n);
+			return true;
+		}
+		if (m.kind == luzgpu::MAT_ISOTROPIC)	// Isotropic.cpp:15-24
+		{
+			sc.att = m.color;
+			sc.type = PDF_SPHERE;
+			return true;
+		}
+		if (m.kind == luzgpu::MAT_HG)	// HenyeyGreenstein.cpp:54-63
+		{
+			sc.att = m.color;
+			sc.type = PDF_HG;
+			sc.phaseDir = len2(r.d) <= 0.0 ? V{0, 0, 1} : normalize(r.d);
+			sc.g = m.g;
 			return true;
 		}
 		if (m.kind == luzgpu::MAT_METAL)	// Metal.cpp:126-170
@@ -1052,7 +1332,8 @@ namespace oracle
 	// ---- lights and environment -----------------------------------------------------------------------
 	bool	leavesOpaque(const Mat& m, const Hit& h, V d)	// ColorHelper.cpp:357-374
 	{
-		if (m.kind == luzgpu::MAT_DIELECTRIC) return true;
+		// materialCanTransmitThroughGeometry (:327-355)
+		if (m.kind == luzgpu::MAT_DIELECTRIC || m.kind == luzgpu::MAT_ISOTROPIC || m.kind == luzgpu::MAT_HG) return true;
 		V g = h.gn;
 		if (len2(g) <= 1e-12) g = h.n;
 		if (len2(g) <= 1e-12) return true;
@@ -1098,6 +1379,53 @@ namespace oracle
 		const double sel = selectionProbability(s, idx);
 		if (sel <= 0.0 || !std::isfinite(sel)) return out;
 		const Light& L = s.lights[idx];
+		if (L.kind == K_DIR)	// DirectionalLight::sampleLight (DirectionalLight.cpp:74-88)
+		{
+			out.dir = L.dir;
+			out.pdf = 1.0 * sel;
+			out.tMax = T_MAX;
+			out.emitted = L.emitted;
+			out.valid = len2(out.dir) > 0.0 && maxChannel(out.emitted) > 0.0;
+			return out;
+		}
+		if (L.kind == K_SPHERE)	// Sphere::sampleLight (Sphere.cpp:382-442)
+		{
+			const Sphere& sp = s.spheres[L.index];
+			const V toC = sp.c - origin;
+			const double d2 = len2(toC);
+			if (sp.r <= 0.0 || !std::isfinite(d2) || d2 <= sp.r * sp.r) return out;
+			float sx, sy;
+			ps.s2(D_LIGHT_SURFACE_POINT, sx, sy);
+			const float rr = static_cast<float>(sp.r * sp.r / d2);	// :365-380
+			const float z = 1.0f + sy * (std::sqrt(std::max(0.0f, 1.0f - rr)) - 1.0f);
+			const float phi = static_cast<float>(2.0 * PI) * sx;
+			const float rad = std::sqrt(std::max(0.0f, 1.0f - z * z));
+			const V local = {static_cast<double>(std::cos(phi) * rad), static_cast<double>(std::sin(phi) * rad), static_cast<double>(z)};
+			const Onb uvw(toC);
+			V dir = local.x * uvw.u + local.y * uvw.v + local.z * uvw.w;
+			const double dl2 = len2(dir);
+			if (!std::isfinite(dl2) || dl2 <= 0.0) return out;
+			dir = dir / std::sqrt(dl2);
+			const V oc = origin - sp.c;
+			const double a = len2(dir), b = dot(oc, dir), c = len2(oc) - (sp.r * sp.r);
+			const double disc = (b * b) - (a * c);
+			if (disc < 0.0) return out;
+			const double sq = std::sqrt(disc);
+			double root = (-b - sq) / a;
+			if (root < T_MIN || root > T_MAX) root = (-b + sq) / a;
+			if (root < T_MIN || root > T_MAX) return out;
+			const double cosMax = std::sqrt(1.0 - sp.r * sp.r / d2);
+			const double solid = 2.0 * PI * (1.0 - cosMax);
+			if (solid <= 0.0 || !std::isfinite(solid)) return out;
+			out.dir = dir;
+			out.pdf = (1.0 / solid) * sel;
+			out.tMax = root;
+			if (out.pdf <= 0.0 || out.tMax <= T_MIN || !std::isfinite(out.pdf) || !std::isfinite(out.tMax)) return out;
+			out.emitted = L.emitted;
+			out.valid = maxChannel(out.emitted) > 0.0;
+			return out;
+		}
+		if (L.kind != K_RECT && L.kind != K_TRI) return out;	// Hittable::sampleLight default
 		float sx, sy;
 		ps.s2(D_LIGHT_SURFACE_POINT, sx, sy);
 		V point, normal;
@@ -1136,6 +1464,15 @@ namespace oracle
 	double	lightPdfValue(const Scene& s, const Light& L, V origin, V vec)	// Rectangle.cpp:345-359, Triangle.cpp:343-370
 	{
 		const Ray r = Ray::make(origin, vec);
+		if (L.kind == K_SPHERE)	// Sphere::pdfValue (Sphere.cpp:362-375)
+		{
+			const Sphere& sp = s.spheres[L.index];
+			double t;
+			if (!sp.intersect(r, T_MIN, T_MAX, t)) return 0.0;
+			const double cosMax = std::sqrt(1.0 - sp.r * sp.r / len2(sp.c - origin));
+			return 1.0 / (2.0 * PI * (1.0 - cosMax));
+		}
+		if (L.kind != K_RECT && L.kind != K_TRI) return 0.0;
 		if (L.kind == K_RECT)
 		{
 			const Rect& rc = s.rects[L.index];
@@ -1258,7 +1595,7 @@ namespace oracle
 			out.pdf = pdf;
 			out.tMax = T_MAX;
 			if (maxChannel(out.emitted) <= 0.0) return out;
-			if (occluded(s, Ray::normalized(origin, out.dir), T_MAX)) return out;
+			if (occluded(s, Ray::normalized(origin, out.dir), T_MAX, &ps)) return out;
 			out.valid = true;
 			return out;
 		}
@@ -1274,7 +1611,7 @@ namespace oracle
 			out.pdf = 1.0 / (4.0 * PI);
 			out.tMax = T_MAX;
 			if (maxChannel(out.emitted) <= 0.0) return out;
-			if (occluded(s, Ray::normalized(origin, out.dir), T_MAX)) return out;
+			if (occluded(s, Ray::normalized(origin, out.dir), T_MAX, &ps)) return out;
 			out.valid = true;
 		}
 		return out;
@@ -1322,7 +1659,7 @@ namespace oracle
 		const double sp = scatterPdf(m, h, sc, ls.dir);
 		const C f = scatterBsdfCos(m, h, sc, ls.dir);
 		if (sp <= 0.0 || !std::isfinite(sp) || maxChannel(f) <= 0.0) return {};
-		if (occluded(s, Ray::normalized(h.p, ls.dir), ls.tMax - T_MIN)) return {};
+		if (occluded(s, Ray::normalized(h.p, ls.dir), ls.tMax - T_MIN, &ps)) return {};
 		const double w = power(ls.pdf, sp);
 		return f * ls.emitted * (w / (ls.pdf * prob));
 	}
@@ -1344,6 +1681,24 @@ namespace oracle
 		return c;
 	}
 
+	// henyeyGreensteinDirection (ColorHelper.cpp:115-148) for a non-zero phase direction
+	V	hgDirection(V phaseDir, double gIn, float sx, float sy)
+	{
+		const V pd = phaseDir / std::sqrt(len2(phaseDir));
+		const double g = std::max(-0.99, std::min(0.99, gIn));
+		const double phi = 2.0 * PI * sx;
+		double c;
+		if (std::fabs(g) < 1e-3) c = 1.0 - (2.0 * sy);
+		else
+		{
+			const double rm = (1.0 - (g * g)) / (1.0 - g + (2.0 * g * sy));
+			c = (1.0 + (g * g) - (rm * rm)) / (2.0 * g);
+			c = std::max(-1.0, std::min(1.0, c));
+		}
+		const double st = std::sqrt(std::max(0.0, 1.0 - (c * c)));
+		return Onb(pd).local(st * std::cos(phi), st * std::sin(phi), c);
+	}
+
 	C	trace(const Scene& s, Ray ray, Sampler& ps)	// calculateLightRaysColor, ColorHelper.cpp:1136-1326
 	{
 		C acc, thr = {1, 1, 1};
@@ -1355,7 +1710,7 @@ namespace oracle
 		{
 			ps.bounce = static_cast<std::uint32_t>(b);
 			Hit h;
-			if (!closestHit(s, ray, h))
+			if (!closestHit(s, ray, h, &ps))
 			{
 				double w = 1.0;
 				if (b > 0 && !prevSpec && prevInf)
@@ -1403,6 +1758,21 @@ namespace oracle
 				const float z = std::sqrt(std::max(0.0f, 1.0f - sy));
 				const V local = {static_cast<double>(r * std::cos(phi)), static_cast<double>(r * std::sin(phi)), static_cast<double>(z)};
 				dir = local.x * sc.basis.u + local.y * sc.basis.v + local.z * sc.basis.w;
+			}
+			else if (sc.type == PDF_SPHERE || (sc.type == PDF_HG && !(len2(sc.phaseDir) > 0.0 && std::isfinite(len2(sc.phaseDir)))))
+			{
+				float sx, sy;	// Sampler::sphereDirection (Sampler.cpp:249-257)
+				ps.s2(D_BSDF_DIRECTION, sx, sy);
+				const float z = 1.0f - (2.0f * sy);
+				const float r = std::sqrt(std::max(0.0f, 1.0f - z * z));
+				const float phi = static_cast<float>(2.0 * PI) * sx;
+				dir = {static_cast<double>(r * std::cos(phi)), static_cast<double>(r * std::sin(phi)), static_cast<double>(z)};
+			}
+			else if (sc.type == PDF_HG)
+			{
+				float sx, sy;
+				ps.s2(D_BSDF_DIRECTION, sx, sy);
+				dir = hgDirection(sc.phaseDir, sc.g, sx, sy);
 			}
 			else
 			{
@@ -1455,6 +1825,26 @@ namespace oracle
 }
 
 extern "C" {
+
+// Henyey-Greenstein phase pdf for the angle between the phase direction and `dir`
+// (scatterPDFValue, ColorHelper.cpp:56-77), and the cosine of a sampled direction
+// (henyeyGreensteinDirection, :115-148) for the 2-D sample (sx, sy).
+double	oracle_hg_pdf(double g, double cosTheta)
+{
+	oracle::Scatter sc;
+	sc.type = oracle::PDF_HG;
+	sc.phaseDir = {0.0, 0.0, 1.0};
+	sc.g = g;
+	const double st = std::sqrt(std::max(0.0, 1.0 - cosTheta * cosTheta));
+	oracle::Mat m;
+	m.kind = luzgpu::MAT_HG;
+	return oracle::scatterPdf(m, oracle::Hit{}, sc, oracle::V{st, 0.0, cosTheta});
+}
+
+double	oracle_hg_sample_cos(double g, float sx, float sy)
+{
+	return oracle::hgDirection(oracle::V{0.0, 0.0, 1.0}, g, sx, sy).z;
+}
 
 // Sampler values for n keys (seed, x, y, sample, stream, bounce, dim).
 void	oracle_sampler(std::size_t n, const std::uint32_t* keys, double* s1, float* s2)
@@ -1542,7 +1932,7 @@ int	oracle_primary(const char* path, std::uint32_t seed, std::uint32_t width, st
 			ps.begin(seed, i % s.width, i / s.width, 0, 0);
 			const oracle::Ray r = oracle::cameraRay(s, i % s.width, i / s.width, ps);
 			oracle::Hit h;
-			if (oracle::closestHit(s, r, h)) { ids[2 * i] = h.top; ids[2 * i + 1] = h.tri; t[i] = h.t; }
+			if (oracle::closestHit(s, r, h, &ps)) { ids[2 * i] = h.top; ids[2 * i + 1] = h.tri; t[i] = h.t; }
 			else { ids[2 * i] = -1; ids[2 * i + 1] = -1; t[i] = -1.0; }
 		}
 		return 0;

@@ -30,6 +30,18 @@ def spheres(workload_dir):
 
 
 @pytest.fixture(scope="session")
+def primitives(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_primitives(os.path.join(workload_dir, "primitives"))
+
+
+@pytest.fixture(scope="session")
+def volume(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_volume(os.path.join(workload_dir, "volume"))
+
+
+@pytest.fixture(scope="session")
 def ctx():
     from luz_amd import Context
     c = Context(0)

@@ -83,11 +83,35 @@ def primary_fixture(name, scene, seed, tmp, w=None, h=None):
     print(name, "primary near ties", int(near_tie.sum()), "of", near_tie.size)
 
 
+def small_scene_fixtures(name, wl, tmp):
+    """Render, TLAS topology and primary hits of a small CPU-runnable workload."""
+    out = os.path.join(tmp, f"{name}.bin")
+    run("render", wl.scene_path, wl.seed, os.cpu_count() or 8, out)
+    img, ms = read_image(out)
+    np.savez_compressed(os.path.join(HERE, f"{name}_{wl.width}x{wl.height}_{wl.spp}spp_seed{wl.seed}.npz"),
+                        image=img.astype(np.float32), render_ms=ms, threads=os.cpu_count() or 8)
+    tl = os.path.join(tmp, f"{name}_tlas.txt")
+    run("tlas", wl.scene_path, tl)
+    with open(os.path.join(HERE, f"{name}_tlas.txt"), "w") as fh:
+        fh.write(open(tl).read())
+    primary_fixture(name, wl.scene_path, wl.seed, tmp)
+
+
 def main():
     if not os.path.exists(LUZREF):
         sys.exit("build the reference harness first: make -C oracle ref")
     tmp = tempfile.mkdtemp(prefix="luz_golden_")
+    only = set(sys.argv[1:])
+    if only:
+        # e.g. `make_golden.py primitives volume`: regenerate just those fixtures
+        if "primitives" in only:
+            small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
+        if "volume" in only:
+            small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
+        return
     sampler_fixture(tmp)
+    small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
+    small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
 
     # Cornell box, 300x300, 50 spp, depth 8, seed 1 (the plain CPU-runnable config)
     c = workloads.write_cornell(os.path.join(tmp, "cornell"))

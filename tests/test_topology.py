@@ -62,3 +62,38 @@ def test_missing_file_is_io_error():
     with pytest.raises(_lib.LuzGpuError) as e:
         SceneFile.read(None, "/nonexistent/scene.luz")
     assert e.value.code == _lib.E_IO
+
+
+def test_primitive_and_volume_scenes_match_reference(primitives, volume):
+    """Spheres, cubes and volumes enter the TLAS, planes and directional lights stay outside."""
+    p = SceneFile.read(None, primitives.scene_path)
+    assert p.tlas_topology() == open(os.path.join(GOLDEN, "primitives_tlas.txt")).read().strip()
+    # emitters: the sphere, the rectangle and the cube (the cube never yields a light sample)
+    assert p.info["hittables"] == 8 and p.info["lights"] == 3
+    v = SceneFile.read(None, volume.scene_path)
+    assert v.tlas_topology() == open(os.path.join(GOLDEN, "volume_tlas.txt")).read().strip()
+    # emitters: the sphere lamp and the directional sun
+    assert v.info["hittables"] == 8 and v.info["lights"] == 2
+
+
+def test_volume_and_phase_validation(tmp_path):
+    """HenyeyGreenstein.cpp:10-18 and Objects.cpp:491-497 reject these inputs."""
+    import pytest
+    from luz_amd import _lib
+    head = ("[settings]\nresolution=8,8\nsamples=1\n\n[materials]\nmaterial m {{\ntype=hg\ncolor=(1,1,1)\nanisotropy={g}\n}}\n\n"
+            "[scene]\ncamera c {{\nposition=(0,0,0)\n}}\nobjects{{\n{obj}\n}}\n")
+    bad_g = tmp_path / "g.luz"
+    bad_g.write_text(head.format(g="1.5", obj="sphere=(0,0,5),1,material=m"))
+    with pytest.raises(_lib.LuzGpuError) as e:
+        SceneFile.read(None, str(bad_g))
+    assert e.value.code == _lib.E_INVALID
+    bad_v = tmp_path / "v.luz"
+    bad_v.write_text(head.format(g="0.5", obj="volume v {\nshape=box\ndensity=-1\n}"))
+    with pytest.raises(_lib.LuzGpuError) as e:
+        SceneFile.read(None, str(bad_v))
+    assert "density" in str(e.value)
+    preset = tmp_path / "p.luz"
+    preset.write_text(head.format(g="0.5", obj="volume v {\nshape=box\npreset=fog\n}"))
+    with pytest.raises(_lib.LuzGpuError) as e:
+        SceneFile.read(None, str(preset))
+    assert e.value.code == _lib.E_UNSUPPORTED
