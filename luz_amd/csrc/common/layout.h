@@ -133,6 +133,9 @@ namespace luzgpu
 		GPU_MAT_EMISSIVE = 4,
 		GPU_MAT_ISOTROPIC = 5,
 		GPU_MAT_HG = 6,
+		GPU_MAT_GLOSSY = 7,
+		GPU_MAT_DIFFUSE_GLOSSY = 8,
+		GPU_MAT_PRINCIPLED = 9,
 	};
 
 	struct alignas(16) GpuMaterial
@@ -149,12 +152,26 @@ namespace luzgpu
 		float	pad1;
 		float	absorption[3];
 		float	pad2;
+		// glossy / diffuse-glossy / principled (Glossy.cpp, DiffuseGlossy.cpp, Principled.cpp)
+		float	glossyColor[3];
+		float	glossyWeight;	// diffuse-glossy mix
+		float	metallic;
+		float	transmission;	// principled transmission (1 - alpha folded in)
+		float	clearcoat;
+		float	clearcoatAlpha;
+		float	sheen;
+		float	diffuseScale;	// principled diffuseScale(), Principled.cpp:141-156
+		float	pad3[2];
+		float	lobe[8];		// principled component weights / total (diffuse, subsurface, sheen,
+								// dielectric specular, metal, transmission, clearcoat), Principled.cpp:158-202
 	};
+	static_assert(sizeof(GpuMaterial) == 160, "GpuMaterial must be 160 bytes");
 
 	enum : std::uint32_t
 	{
 		GPU_MATF_CONDUCTOR = 1u,
 		GPU_MATF_ABSORPTION = 2u,
+		GPU_MATF_SMOOTH_TRANSMISSION = 4u,	// principled isSmoothFullTransmission (Principled.cpp:107-124)
 	};
 
 	struct alignas(16) GpuLight

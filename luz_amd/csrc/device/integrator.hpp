@@ -86,7 +86,7 @@ namespace luzgpu
 		const float prob = directProbability(bounce);
 		if (prob < 1.0f && ps.sample1D(DIM_LIGHT_STRATEGY) >= static_cast<double>(prob)) return c;
 		const LightSample ls = sampleLight(s, p, sf.position, ps);
-		if (!ls.valid || !leavesOpaque(sf, m.kind, ls.direction)) return c;
+		if (!ls.valid || !leavesOpaque(sf, m, ls.direction)) return c;
 		const float spdf = scatterPdf(m, sf, sc, ls.direction);
 		const f3 f = scatterBsdfCos(m, sf, sc, ls.direction);
 		if (spdf <= 0.0f || !isfinite(spdf) || maxc(f) <= 0.0f) return c;
@@ -113,7 +113,7 @@ namespace luzgpu
 		const float prob = directProbability(bounce);
 		if (prob < 1.0f && ps.sample1D(DIM_ENVIRONMENT_STRATEGY) >= static_cast<double>(prob)) return c;
 		const LightSample ls = sampleInfinite(s, p, ps);
-		if (!ls.valid || !leavesOpaque(sf, m.kind, ls.direction)) return c;
+		if (!ls.valid || !leavesOpaque(sf, m, ls.direction)) return c;
 		const float spdf = scatterPdf(m, sf, sc, ls.direction);
 		const f3 f = scatterBsdfCos(m, sf, sc, ls.direction);
 		if (spdf <= 0.0f || !isfinite(spdf) || maxc(f) <= 0.0f) return c;
@@ -262,7 +262,7 @@ namespace luzgpu
 			}
 			if (sc.specular)
 			{
-				if (!leavesOpaque(sf, m.kind, sc.direction)) return acc;
+				if (!leavesOpaque(sf, m, sc.direction)) return acc;
 				thr = clampRay(thr * att);
 				if (terminated(thr) || !russianRoulette(thr, b, ps)) return acc;
 				ray = makeRay(sf.position, sc.direction);
@@ -302,7 +302,7 @@ namespace luzgpu
 				}
 			}
 			const f3 dir = scatterDirection(sf, sc, ps);
-			if (!leavesOpaque(sf, m.kind, dir)) return acc;
+			if (!leavesOpaque(sf, m, dir)) return acc;
 			const float pdf = scatterPdf(m, sf, sc, dir);
 			if (pdf <= 0.0f || !isfinite(pdf)) return acc;
 			thr = clampRay(thr * att);

@@ -318,9 +318,10 @@ namespace luzgpu
 	};
 
 	// materialCanTransmitThroughGeometry (ColorHelper.cpp:327-355)
-	__device__ __forceinline__ bool	canTransmit(std::uint32_t kind)
+	__device__ __forceinline__ bool	canTransmit(const GpuMaterial& m)
 	{
-		return kind == GPU_MAT_DIELECTRIC || kind == GPU_MAT_ISOTROPIC || kind == GPU_MAT_HG;
+		return m.kind == GPU_MAT_DIELECTRIC || m.kind == GPU_MAT_ISOTROPIC || m.kind == GPU_MAT_HG
+			|| (m.kind == GPU_MAT_PRINCIPLED && m.transmission > 0.0f);
 	}
 
 	// Henyey-Greenstein pdf of scatterPDFValue (ColorHelper.cpp:56-77)
@@ -338,22 +339,113 @@ namespace luzgpu
 	}
 
 	// leavesOpaqueGeometricSurface (ColorHelper.cpp:357-374)
-	__device__ __forceinline__ bool	leavesOpaque(const Surface& sf, std::uint32_t kind, f3 dir)
+	__device__ __forceinline__ bool	leavesOpaque(const Surface& sf, const GpuMaterial& m, f3 dir)
 	{
-		if (canTransmit(kind)) return true;
+		if (canTransmit(m)) return true;
 		f3 g = sf.geometricNormal;
 		if (len2(g) <= 1e-12f) g = sf.normal;
 		if (len2(g) <= 1e-12f) return true;
 		return dot(g, dir) > 1e-7f;
 	}
 
+	// GGX-Schlick reflection lobe (BSDF.cpp:312-330)
+	__device__ __forceinline__ f3	schlickBsdfCos(f3 f0, f3 n, f3 view, f3 l, float alpha)
+	{
+		const f3 h = safeNormalize(view + l, n);
+		const float voH = fmaxf(0.0f, dot(view, h));
+		return ggxReflectionBsdfCos(schlick(f0, voH), n, view, l, alpha);
+	}
+
+	__device__ __forceinline__ float	cosinePdf(f3 n, f3 l)
+	{
+		const float c = dot(n, l);
+		return c <= 0.0f ? 0.0f : c / kPiF;
+	}
+
+	// DiffuseGlossy::evaluateBSDFCos / scatteringPDF (DiffuseGlossy.cpp:151-186)
+	__device__ __forceinline__ f3	diffuseGlossyEval(const GpuMaterial& m, f3 n, f3 view, f3 l)
+	{
+		const float noL = dot(n, l);
+		const f3 diffuse = noL <= 0.0f ? mk(0, 0, 0) : clampColor01(ld3(m.color)) * (noL / kPiF);
+		const f3 glossy = ggxReflectionBsdfCos(clampColor01(ld3(m.glossyColor)), n, view, l, m.alpha);
+		return diffuse * (1.0f - m.glossyWeight) + glossy * m.glossyWeight;
+	}
+
+	__device__ __forceinline__ float	diffuseGlossyPdf(const GpuMaterial& m, f3 n, f3 view, f3 l)
+	{
+		return cosinePdf(n, l) * (1.0f - m.glossyWeight) + ggxReflectionPdf(n, view, l, m.alpha) * m.glossyWeight;
+	}
+
+	__device__ __forceinline__ f3	scaledColor(f3 c, float f)
+	{
+		return (!isfinite(f) || f <= 0.0f) ? mk(0, 0, 0) : c * f;
+	}
+
+	// Principled::evaluateBSDFCos without subsurface (Principled.cpp:763-853)
+	__device__ __forceinline__ f3	principledEval(const GpuMaterial& m, const Surface& sf, f3 view, f3 l)
+	{
+		const f3 base = clampColor01(ld3(m.color));
+		const f3 n = sf.normal;
+		const float noL = dot(n, l);
+		const float diffuse = m.diffuseScale;
+		f3 r = (noL <= 0.0f || diffuse <= 0.0f) ? mk(0, 0, 0) : base * (diffuse * noL / kPiF);
+		const float sheenScale = diffuse * m.sheen;
+		if (noL > 0.0f && sheenScale > 0.0f)
+		{
+			const f3 h = safeNormalize(view + l, n);
+			const float x = 1.0f - clamp01(dot(l, h));
+			const float fall = x * x * x * x * x;
+			const f3 sheenColor = mk(1, 1, 1) * 0.5f + base * 0.5f;
+			r = r + sheenColor * (sheenScale * fall * noL / kPiF);
+		}
+		if (noL > 0.0f)
+		{
+			const f3 h = safeNormalize(view + l, n);
+			const float F = dielectricFresnel(dot(view, h), 1.0f, m.ior);
+			r = r + scaledColor(ggxReflectionBsdfCos(mk(F, F, F), n, view, l, m.alpha), 1.0f - m.metallic);
+		}
+		r = r + scaledColor(schlickBsdfCos(base, n, view, l, m.alpha), m.metallic);
+		const float transmission = m.transmission * (1.0f - m.metallic);
+		if (noL < 0.0f && transmission > 0.0f)
+		{
+			const float etaI = sf.frontFace ? 1.0f : m.ior, etaT = sf.frontFace ? m.ior : 1.0f;
+			r = r + scaledColor(dielectricBsdfCos(base, n, view, l, m.alpha, etaI, etaT), transmission);
+		}
+		r = r + scaledColor(schlickBsdfCos(mk(0.04f, 0.04f, 0.04f), n, view, l, m.clearcoatAlpha), m.clearcoat * 0.25f);
+		return r;
+	}
+
+	// Principled::scatteringPDF (Principled.cpp:855-921): lobe-weighted mixture
+	__device__ __forceinline__ float	principledPdf(const GpuMaterial& m, const Surface& sf, f3 view, f3 l)
+	{
+		const f3 n = sf.normal;
+		const float cosP = cosinePdf(n, l);
+		const float spec = ggxReflectionPdf(n, view, l, m.alpha);
+		float trans = 0.0f;
+		if (dot(n, l) < 0.0f)
+		{
+			const float etaI = sf.frontFace ? 1.0f : m.ior, etaT = sf.frontFace ? m.ior : 1.0f;
+			trans = dielectricPdf(n, view, l, m.alpha, etaI, etaT);
+		}
+		const float coat = ggxReflectionPdf(n, view, l, m.clearcoatAlpha);
+		const float pdfs[7] = {cosP, 0.0f, cosP, spec, spec, trans, coat};
+		float pdf = 0.0f;
+		for (int i = 0; i < 7; i++)
+		{
+			if (m.lobe[i] > 0.0f && pdfs[i] > 0.0f) pdf += m.lobe[i] * pdfs[i];
+		}
+		return (isfinite(pdf) && pdf > 0.0f) ? pdf : 0.0f;
+	}
+
 	__device__ __forceinline__ float	bsdfPdf(const GpuMaterial& m, const Surface& sf, const Scatter& sc, f3 dir)
 	{
 		const f3 view = safeNormalize(-sc.incident, sf.normal);
-		if (m.kind == GPU_MAT_METAL)
+		if (m.kind == GPU_MAT_METAL || m.kind == GPU_MAT_GLOSSY)
 		{
 			return ggxReflectionPdf(sf.normal, view, dir, m.alpha);
 		}
+		if (m.kind == GPU_MAT_DIFFUSE_GLOSSY) return diffuseGlossyPdf(m, sf.normal, view, dir);
+		if (m.kind == GPU_MAT_PRINCIPLED) return principledPdf(m, sf, view, dir);
 		const float etaI = sf.frontFace ? 1.0f : m.ior, etaT = sf.frontFace ? m.ior : 1.0f;
 		return dielectricPdf(sf.normal, view, dir, m.alpha, etaI, etaT);
 	}
@@ -365,6 +457,9 @@ namespace luzgpu
 		{
 			return metalBsdfCos(m, sf.normal, view, dir);
 		}
+		if (m.kind == GPU_MAT_GLOSSY) return ggxReflectionBsdfCos(clampColor01(ld3(m.color)), sf.normal, view, dir, m.alpha);
+		if (m.kind == GPU_MAT_DIFFUSE_GLOSSY) return diffuseGlossyEval(m, sf.normal, view, dir);
+		if (m.kind == GPU_MAT_PRINCIPLED) return principledEval(m, sf, view, dir);
 		const float etaI = sf.frontFace ? 1.0f : m.ior, etaT = sf.frontFace ? m.ior : 1.0f;
 		return dielectricBsdfCos(ld3(m.color), sf.normal, view, dir, m.alpha, etaI, etaT);
 	}
@@ -398,6 +493,44 @@ namespace luzgpu
 		return d * (1.0f / l);
 	}
 
+	// Sampler::cosineHemisphere (Sampler.cpp:239-246) in the ONB of `n`
+	__device__ __forceinline__ f3	cosineHemisphereDir(f3 n, const PixelSampler& ps)
+	{
+		float sx, sy;
+		ps.sample2D(DIM_BSDF_DIRECTION, sx, sy);
+		const float phi = (2.0f * kPiF) * sx;
+		const float r = sqrtf(sy);
+		const float z = sqrtf(fmaxf(0.0f, 1.0f - sy));
+		float sp, cp;
+		sincosf(phi, &sp, &cp);
+		return Onb(n).local(r * cp, r * sp, z);
+	}
+
+	// GGX half-vector reflection sample (Glossy.cpp:22-40, Principled.cpp:334-357)
+	__device__ __forceinline__ f3	sampleGGXReflection(f3 inDir, f3 n, float alpha, const PixelSampler& ps)
+	{
+		const f3 view = safeNormalize(-inDir, n);
+		float sx, sy;
+		ps.sample2D(DIM_BSDF_DIRECTION, sx, sy);
+		f3 h = sampleGGXHalf(n, alpha, sx, sy);
+		if (dot(view, h) <= 0.0f) h = n;
+		f3 dir = reflect(inDir, h);
+		if (dot(dir, n) <= 0.0f) dir = reflect(inDir, n);
+		return dir;
+	}
+
+	// Finishes a BSDF-sampled scatter: attenuation = bsdfCos / pdf, rejected when either is 0.
+	__device__ __forceinline__ bool	acceptBsdfSample(const GpuMaterial& m, const Surface& sf, f3 dir, Scatter& sc)
+	{
+		const float pdf = bsdfPdf(m, sf, sc, dir);
+		const f3 f = bsdfEval(m, sf, sc, dir);
+		if (pdf <= 0.0f || !isfinite(pdf) || maxc(f) <= 0.0f) return false;
+		sc.direction = dir;
+		sc.attenuation = f * (1.0f / pdf);
+		sc.pdfType = PDF_BSDF;
+		return true;
+	}
+
 	// Material::scatter for the supported materials. Returns false when the material
 	// does not scatter (emissive) or the BSDF sample is rejected.
 	__device__ __forceinline__ bool	scatter(const GpuMaterial& m, const Surface& sf, f3 inDir, const PixelSampler& ps, Scatter& sc)
@@ -418,6 +551,78 @@ namespace luzgpu
 			sc.attenuation = ld3(m.color);
 			sc.pdfType = PDF_SPHERE;
 			return true;
+		}
+		if (m.kind == GPU_MAT_GLOSSY)
+		{
+			return acceptBsdfSample(m, sf, sampleGGXReflection(inDir, sf.normal, m.alpha, ps), sc);
+		}
+		if (m.kind == GPU_MAT_DIFFUSE_GLOSSY)
+		{
+			const f3 dir = ps.sample1D(DIM_MATERIAL_DECISION) < static_cast<double>(m.glossyWeight)
+				? sampleGGXReflection(inDir, sf.normal, m.alpha, ps) : cosineHemisphereDir(sf.normal, ps);
+			return acceptBsdfSample(m, sf, dir, sc);
+		}
+		if (m.kind == GPU_MAT_PRINCIPLED)
+		{
+			const f3 base = clampColor01(ld3(m.color));
+			if (m.flags & GPU_MATF_SMOOTH_TRANSMISSION)
+			{
+				// scatterSmoothTransmission (Principled.cpp:401-440)
+				const f3 view = safeNormalize(-inDir, sf.normal);
+				const float etaI = sf.frontFace ? 1.0f : m.ior, etaT = sf.frontFace ? m.ior : 1.0f;
+				const float ratio = etaI / etaT;
+				const float cosT = fminf(1.0f, dot(view, sf.normal));
+				const float sinT = sqrtf(fmaxf(0.0f, 1.0f - cosT * cosT));
+				f3 dir;
+				if (ratio * sinT > 1.0f || static_cast<double>(dielectricFresnel(cosT, etaI, etaT)) > ps.sample1D(DIM_MATERIAL_DECISION))
+				{
+					dir = reflect(inDir, sf.normal);
+					sc.attenuation = mk(1, 1, 1);
+				}
+				else
+				{
+					const float c = fminf(dot(-inDir, sf.normal), 1.0f);
+					const f3 perp = (inDir + sf.normal * c) * ratio;
+					dir = perp + sf.normal * -sqrtf(fabsf(1.0f - len2(perp)));
+					sc.attenuation = base;
+				}
+				sc.direction = normalizeOrZero(dir);
+				sc.specular = true;
+				return true;
+			}
+			// selectComponent (Principled.cpp:204-216)
+			double target = ps.sample1D(DIM_MATERIAL_DECISION);
+			int comp = 0;
+			for (int i = 0; i < 7; i++)
+			{
+				target -= static_cast<double>(m.lobe[i]);
+				if (target <= 0.0)
+				{
+					comp = i;
+					break;
+				}
+			}
+			f3 dir;
+			if (comp <= 2)
+			{
+				dir = cosineHemisphereDir(sf.normal, ps);
+			}
+			else if (comp == 5)
+			{
+				// sampleGGXTransmission (Principled.cpp:359-399)
+				const f3 view = safeNormalize(-inDir, sf.normal);
+				const float etaI = sf.frontFace ? 1.0f : m.ior, etaT = sf.frontFace ? m.ior : 1.0f;
+				float sx, sy;
+				ps.sample2D(DIM_BSDF_DIRECTION, sx, sy);
+				f3 h = sampleGGXHalf(sf.normal, m.alpha, sx, sy);
+				if (dot(view, h) <= 0.0f) h = sf.normal;
+				if (!refractDir(inDir, h, etaI / etaT, dir)) dir = reflect(inDir, h);
+			}
+			else
+			{
+				dir = sampleGGXReflection(inDir, sf.normal, comp == 6 ? m.clearcoatAlpha : m.alpha, ps);
+			}
+			return acceptBsdfSample(m, sf, dir, sc);
 		}
 		if (m.kind == GPU_MAT_HG)
 		{

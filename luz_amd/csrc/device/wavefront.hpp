@@ -380,7 +380,7 @@ namespace luzgpu
 						}
 						if (sc.specular)
 						{
-							if (leavesOpaque(sf, m.kind, sc.direction))
+							if (leavesOpaque(sf, m, sc.direction))
 							{
 								thr = clampRay(thr * att);
 								if (!terminated(thr) && russianRoulette(thr, b, ps))
@@ -418,7 +418,7 @@ namespace luzgpu
 									}
 								}
 								const f3 dir = scatterDirection(sf, sc, ps);
-								if (leavesOpaque(sf, m.kind, dir))
+								if (leavesOpaque(sf, m, dir))
 								{
 									const float pdf = scatterPdf(m, sf, sc, dir);
 									if (pdf > 0.0f && isfinite(pdf))

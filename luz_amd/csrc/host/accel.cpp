@@ -925,7 +925,45 @@ namespace luzgpu
 			g.roughness = static_cast<float>(m.roughness);
 			g.ior = static_cast<float>(m.ior);
 			g.anisotropy = static_cast<float>(m.anisotropy);
-			g.flags = (m.conductor ? GPU_MATF_CONDUCTOR : 0u)
+			auto alphaOf = [](double rough) {
+				const double rr = std::isfinite(rough) ? std::max(0.0, std::min(1.0, rough)) : 0.0;
+				return std::max(1e-4, rr * rr);
+			};
+			for (int i = 0; i < 3; i++) g.glossyColor[i] = static_cast<float>(m.glossyColor[i]);
+			g.glossyWeight = static_cast<float>(m.glossyWeight);
+			g.metallic = static_cast<float>(m.metallic);
+			g.transmission = static_cast<float>(m.transmission);
+			g.clearcoat = static_cast<float>(m.clearcoat);
+			g.clearcoatAlpha = static_cast<float>(alphaOf(m.clearcoatRoughness));
+			g.sheen = static_cast<float>(m.sheen);
+			bool smoothTransmission = false;
+			if (m.kind == MAT_PRINCIPLED)
+			{
+				// componentWeights (Principled.cpp:158-202) for the material's constant base color
+				auto c01 = [](double v) { return std::isfinite(v) ? std::max(0.0, std::min(1.0, v)) : 0.0; };
+				const dvec3 base = {c01(m.color[0]), c01(m.color[1]), c01(m.color[2])};
+				auto lumW = [](const dvec3& c) { const double l = luminance(c); return (!std::isfinite(l) || l <= 0.0) ? 0.0 : l; };
+				const double r0 = (m.ior - 1.0) / (m.ior + 1.0);
+				const double f0 = c01(r0 * r0);
+				const double baseLum = std::max(0.05, lumW(base));
+				const double diffuse = (1.0 - m.metallic) * (1.0 - m.transmission) * (1.0 - f0) * (1.0 - 0.25 * m.clearcoat);
+				const double w[7] = {
+					diffuse * 1.0 * baseLum,
+					0.0,
+					diffuse * m.sheen * 0.35,
+					(1.0 - m.metallic) * std::max(0.03, f0) * (1.0 - 0.5 * m.transmission),
+					m.metallic * std::max(0.05, baseLum),
+					(1.0 - m.metallic) * m.transmission,
+					m.clearcoat * 0.25 * (1.0 - 0.5 * m.roughness),
+				};
+				double total = 0.0;
+				for (double x : w) if (std::isfinite(x) && x > 0.0) total += x;
+				for (int i = 0; i < 7; i++) g.lobe[i] = total > 0.0 ? static_cast<float>(w[i] / total) : (i == 0 ? 1.0f : 0.0f);
+				g.diffuseScale = static_cast<float>(diffuse);
+				smoothTransmission = m.metallic <= 1e-6 && m.roughness <= 1e-4 && m.transmission >= 1.0 - 1e-6
+					&& m.clearcoat <= 1e-6 && m.sheen <= 1e-6;
+			}
+			g.flags = (smoothTransmission ? GPU_MATF_SMOOTH_TRANSMISSION : 0u) | (m.conductor ? GPU_MATF_CONDUCTOR : 0u)
 				| ((m.absorption[0] > 0.0 || m.absorption[1] > 0.0 || m.absorption[2] > 0.0) ? GPU_MATF_ABSORPTION : 0u);
 			p.materials.push_back(g);
 		}

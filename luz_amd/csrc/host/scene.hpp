@@ -34,6 +34,9 @@ namespace luzgpu
 		MAT_EMISSIVE = 4,
 		MAT_ISOTROPIC = 5,	// Isotropic.cpp:15-24
 		MAT_HG = 6,			// HenyeyGreenstein.cpp:54-63
+		MAT_GLOSSY = 7,		// Glossy.cpp
+		MAT_DIFFUSE_GLOSSY = 8,	// DiffuseGlossy.cpp
+		MAT_PRINCIPLED = 9,	// Principled.cpp (no subsurface)
 	};
 
 	struct	Material
@@ -47,6 +50,13 @@ namespace luzgpu
 		dvec3	k = {0.0, 0.0, 0.0};
 		dvec3	absorption = {0.0, 0.0, 0.0};	// dielectric interior absorption, 1/m
 		double	anisotropy = 0.0;			// Henyey-Greenstein g
+		dvec3	glossyColor = {1.0, 1.0, 1.0};
+		double	glossyWeight = 0.5;
+		double	metallic = 0.0;
+		double	transmission = 0.0;
+		double	clearcoat = 0.0;
+		double	clearcoatRoughness = 0.03;
+		double	sheen = 0.0;
 		std::string	name;
 	};
 

@@ -309,7 +309,11 @@ namespace luzgpu
 				dvec3 color = {0.6, 0.6, 0.6};
 				double roughness = 0.0;
 				double fuzz = -1.0;
-				double ior = 1.5;
+				double ior = 1.52;	// RI_GLASS (RefractiveIndexes.hpp:3)
+				double metallic = 0.0, transmission = 0.0, clearcoat = 0.0, clearcoatRoughness = 0.03, sheen = 0.0;
+				double subsurface = 0.0, glossyWeight = 0.5, alphaKey = 1.0;
+				dvec3 glossyColor = {1.0, 1.0, 1.0};
+				bool hasGlossyColor = false;
 				double radiance = -1.0;
 				double lum = -1.0;
 				double anisotropy = 0.0;
@@ -382,6 +386,50 @@ namespace luzgpu
 								m.color = {color[0] * f, color[1] * f, color[2] * f};
 							}
 						}
+						else if (type == "glossy" || type == "diffuse_glossy" || type == "diffuseglossy" || type == "principled")
+						{
+							auto unit = [](double v, const std::string& label) {
+								if (!std::isfinite(v) || v < 0.0 || v > 1.0)
+								{
+									throw std::invalid_argument(label + " must be finite and in [0,1].");
+								}
+								return (v);
+							};
+							m.color = color;
+							if (type == "glossy")
+							{
+								m.kind = MAT_GLOSSY;
+								m.roughness = unit(roughness, "Glossy roughness");
+							}
+							else if (type != "principled")
+							{
+								// DiffuseGlossy(color, glossyColor or color, weight, roughness), Materials.cpp:831-840
+								m.kind = MAT_DIFFUSE_GLOSSY;
+								m.glossyColor = hasGlossyColor ? glossyColor : color;
+								m.glossyWeight = unit(glossyWeight, "Diffuse-glossy weight");
+								m.roughness = unit(roughness, "Diffuse-glossy roughness");
+							}
+							else
+							{
+								// buildPrincipledMaterial (Materials.cpp:693-760)
+								if (subsurface != 0.0)
+								{
+									throw std::runtime_error("Unsupported for the device path: principled subsurface");
+								}
+								m.kind = MAT_PRINCIPLED;
+								m.metallic = unit(metallic, "Principled metallic");
+								m.roughness = unit(roughness, "Principled roughness");
+								m.transmission = unit(std::max(transmission, 1.0 - alphaKey), "Principled transmission");
+								if (!std::isfinite(ior) || ior <= 0.0)
+								{
+									throw std::invalid_argument("Principled refractive index must be finite and positive.");
+								}
+								m.ior = ior;
+								m.clearcoat = unit(clearcoat, "Principled clearcoat");
+								m.clearcoatRoughness = unit(clearcoatRoughness, "Principled clearcoat roughness");
+								m.sheen = unit(sheen, "Principled sheen");
+							}
+						}
 						else if (type == "isotropic")
 						{
 							m.kind = MAT_ISOTROPIC;
@@ -424,6 +472,22 @@ namespace luzgpu
 					else if (key == "radiance" || key == "surfaceradiance" || key == "surface_radiance") radiance = parseDouble(value, "radiance");
 					else if (key == "luminance" || key == "nits" || key == "cd_m2") lum = parseDouble(value, "luminance");
 					else if (key == "anisotropy" || key == "g") anisotropy = parseDouble(value, "anisotropy");
+					else if (key == "metallic") metallic = parseDouble(value, key);
+					else if (key == "transmission") transmission = parseDouble(value, key);
+					else if (key == "clearcoat" || key == "clear_coat" || key == "coat") clearcoat = parseDouble(value, key);
+					else if (key == "clearcoatroughness" || key == "clearcoat_roughness" || key == "clear_coat_roughness"
+						|| key == "coatroughness" || key == "coat_roughness") clearcoatRoughness = parseDouble(value, key);
+					else if (key == "sheen") sheen = parseDouble(value, key);
+					else if (key == "subsurface" || key == "subsurface_weight" || key == "subsurfaceweight" || key == "sss")
+						subsurface = parseDouble(value, key);
+					else if (key == "alpha") alphaKey = parseDouble(value, key);
+					else if (key == "glossy_color" || key == "glossycolor" || key == "specular_color" || key == "specularcolor")
+					{
+						glossyColor = parseTriple(value, "glossy color");
+						hasGlossyColor = true;
+					}
+					else if (key == "glossy_weight" || key == "glossyweight" || key == "glossy_mix" || key == "glossymix")
+						glossyWeight = parseDouble(value, key);
 					else throw std::runtime_error("Unsupported material property for the device path: " + line);
 				}
 				throw std::runtime_error("Material block '" + name + "' is missing a closing }.");

@@ -18,6 +18,9 @@ Workloads
 * ``primitives``   analytic primitives: a ground plane, spheres (glass, Lambertian, an
                    emissive sphere light), cubes (rough metal, an emissive cube), a rectangle
                    light, under a constant background; 160x120, 32 spp, depth 8.
+* ``materials``    one sphere per glossy / diffuse-glossy / principled variant (plastic,
+                   metallic, clear-coated sheen, rough transmission, smooth glass) on a plane,
+                   lit by a rectangle light and a constant background; 160x120, 32 spp, depth 8.
 * ``volume``       participating media: a Henyey-Greenstein box of fog (g 0.6) and an
                    isotropic sphere of smoke over a ground plane, lit by a directional sun and
                    a sphere light; 160x120, 32 spp, depth 16.
@@ -329,6 +332,51 @@ def write_primitives(directory: str, width=160, height=120, spp=32, depth=8, see
     path = os.path.join(directory, "primitives.luz")
     _write_if_changed(path, primitives_scene_text(width, height, spp, depth).encode())
     return Workload("primitives", directory, path, width, height, spp, depth, seed, 0)
+
+
+def materials_scene_text(width=160, height=120, spp=32, depth=8) -> str:
+    mats = [
+        ("glossy", ["type=glossy", "color=(0.9,0.7,0.3)", "roughness=0.3"]),
+        ("dglossy", ["type=diffuse_glossy", "color=(0.2,0.5,0.8)", "glossy_color=(1,1,1)", "glossy_weight=0.35",
+                     "roughness=0.2"]),
+        ("plastic", ["type=principled", "color=(0.8,0.1,0.1)", "roughness=0.4"]),
+        ("brass", ["type=principled", "color=(0.9,0.75,0.4)", "metallic=1", "roughness=0.25"]),
+        ("velvet", ["type=principled", "color=(0.3,0.6,0.3)", "roughness=0.6", "sheen=0.8", "clearcoat=0.7",
+                    "clearcoat_roughness=0.1"]),
+        ("frosted", ["type=principled", "color=(0.95,0.95,1)", "transmission=0.9", "roughness=0.3", "ior=1.45"]),
+        ("crystal", ["type=principled", "color=(1,1,1)", "transmission=1", "roughness=0", "ior=1.5"]),
+    ]
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "sky=none", "background=(0.08,0.09,0.1)", "",
+        "[materials]",
+        "material floor {", "type=lambertian", "color=(0.55,0.55,0.55)", "}",
+        "material panel {", "type=emissive", "color=(1,0.95,0.9)", "radiance=8", "}",
+    ]
+    for name, props in mats:
+        lines += [f"material {name} {{", *props, "}"]
+    lines += [
+        "",
+        "[scene]",
+        "camera main {", "position=(0,3.5,-11)", "direction=(0,-0.25,1)", "up=(0,1,0)", "focal_length_mm=30",
+        "sensor_width_mm=36", "sensor_height_mm=27", "pinhole=1", "focus_distance=10", "}",
+        "objects{",
+        "plane=0,(0,1,0),material=floor",
+        "rectangle=(0,7,2),(0,-1,0),6,3,material=panel",
+    ]
+    for i, (name, _) in enumerate(mats):
+        x = -4.5 + 1.5 * i
+        z = 1.0 + (i % 2) * 1.5
+        lines.append(f"sphere=({_f(x)},0.7,{_f(z)}),0.7,material={name}")
+    lines.append("}")
+    return "\n".join(lines) + "\n"
+
+
+def write_materials(directory: str, width=160, height=120, spp=32, depth=8, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "materials.luz")
+    _write_if_changed(path, materials_scene_text(width, height, spp, depth).encode())
+    return Workload("materials", directory, path, width, height, spp, depth, seed, 0)
 
 
 def volume_scene_text(width=160, height=120, spp=32, depth=16) -> str:

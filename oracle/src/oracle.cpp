@@ -598,6 +598,8 @@ namespace oracle
 		luzgpu::MaterialKind	kind;
 		C	color;
 		double	roughness = 0, alpha = 1e-4, ior = 1.5, g = 0.0;
+		C	glossyColor = {1, 1, 1};
+		double	glossyWeight = 0.5, metallic = 0, transmission = 0, clearcoat = 0, clearcoatRoughness = 0.03, sheen = 0;
 	};
 
 	struct	Light
@@ -687,6 +689,13 @@ namespace oracle
 			o.alpha = std::max(1e-4, r * r);	// BSDF.cpp:54-58
 			o.ior = m.ior;
 			o.g = m.anisotropy;
+			o.glossyColor = fromC(m.glossyColor);
+			o.glossyWeight = m.glossyWeight;
+			o.metallic = m.metallic;
+			o.transmission = m.transmission;
+			o.clearcoat = m.clearcoat;
+			o.clearcoatRoughness = m.clearcoatRoughness;
+			o.sheen = m.sheen;
 			s.mats.push_back(o);
 		}
 		for (const auto& mesh : in.meshes)
@@ -1193,10 +1202,115 @@ namespace oracle
 		double	g = 0.0;
 	};
 
+	// ---- glossy / diffuse-glossy / principled (Glossy.cpp, DiffuseGlossy.cpp, Principled.cpp) -----
+	double	roughnessToAlpha(double r) { const double c = clamp01(r); return std::max(1e-4, c * c); }	// BSDF.cpp:54-58
+	double	cosPdf(V n, V l) { const double c = dot(n, l); return c <= 0.0 ? 0.0 : c / PI; }
+	C	schlickBsdf(C f0, V n, V view, V l, double a)	// BSDF.cpp:312-330
+	{
+		const V h = safeNormalize(view + l, n);
+		return ggxReflBsdf(schlick(f0, std::max(0.0, dot(view, h))), n, view, l, a);
+	}
+	C	scaledC(C c, double f) { return (!std::isfinite(f) || f <= 0.0) ? C{} : c * f; }	// Principled.cpp:293-300
+
+	struct	Lobes { double w[7] = {}; double total = 0.0; };
+	double	lumWeight(C c) { const double l = luminance(c); return (!std::isfinite(l) || l <= 0.0) ? 0.0 : l; }
+	double	dielF0(double ior) { const double r0 = (ior - 1.0) / (ior + 1.0); return clamp01(r0 * r0); }
+	double	principledDiffuseScale(const Mat& m)	// :141-156
+	{
+		const double ce = 0.25 * m.clearcoat;
+		return (1.0 - m.metallic) * (1.0 - m.transmission) * (1.0 - dielF0(m.ior)) * (1.0 - ce);
+	}
+	Lobes	principledLobes(const Mat& m, C base)	// componentWeights, :158-202 (subsurface 0)
+	{
+		Lobes r;
+		const double baseLum = std::max(0.05, lumWeight(base));
+		const double diffuse = principledDiffuseScale(m);
+		const double spec = (1.0 - m.metallic) * std::max(0.03, dielF0(m.ior));
+		const double subsurfaceWeight = diffuse * 0.0;
+		const double surfaceDiffuse = diffuse * (1.0 - 0.0);
+		r.w[0] = surfaceDiffuse * baseLum;
+		r.w[1] = subsurfaceWeight * std::max(0.05, lumWeight(base * C{1, 1, 1}));
+		r.w[2] = diffuse * m.sheen * 0.35;
+		r.w[3] = spec * (1.0 - 0.5 * m.transmission);
+		r.w[4] = m.metallic * std::max(0.05, baseLum);
+		r.w[5] = (1.0 - m.metallic) * m.transmission;
+		r.w[6] = m.clearcoat * 0.25 * (1.0 - 0.5 * m.roughness);
+		for (double x : r.w) if (std::isfinite(x) && x > 0.0) r.total += x;
+		if (r.total <= 0.0) { r.w[0] = 1.0; r.total = 1.0; }
+		return r;
+	}
+	C	principledEval(const Mat& m, const Hit& h, V view, V l)	// evaluateBSDFCos, :763-853
+	{
+		const C base = clamp01(m.color);
+		const double a = roughnessToAlpha(m.roughness), ca = roughnessToAlpha(m.clearcoatRoughness);
+		const double transmission = m.transmission * (1.0 - m.metallic);
+		const double diffuse = principledDiffuseScale(m);
+		const V n = h.n;
+		C r;
+		{	// diffuseBSDFCos, :222-234
+			const double noL = dot(n, l), sc = diffuse * (1.0 - 0.0);
+			if (!(noL <= 0.0 || sc <= 0.0)) r = base * (sc * noL / PI);
+		}
+		{	// sheenBSDFCos, :272-291
+			const double noL = dot(n, l), sc = diffuse * m.sheen;
+			if (!(noL <= 0.0 || sc <= 0.0))
+			{
+				const V hv = safeNormalize(view + l, n);
+				const double fall = std::pow(1.0 - clamp01(dot(l, hv)), 5.0);
+				const C sheenColor = C{1, 1, 1} * (1.0 - 0.5) + base * 0.5;
+				r = r + sheenColor * (sc * fall * noL / PI);
+			}
+		}
+		if (dot(n, l) > 0.0)	// dielectricReflectionBSDFCos, :302-327
+		{
+			const V hv = safeNormalize(view + l, n);
+			const double F = dielectricFresnel(dot(view, hv), 1.0, m.ior);
+			r = r + scaledC(ggxReflBsdf(C{F, F, F}, n, view, l, a), 1.0 - m.metallic);
+		}
+		r = r + scaledC(schlickBsdf(base, n, view, l, a), m.metallic);
+		if (dot(n, l) < 0.0 && transmission > 0.0)
+		{
+			const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0;
+			r = r + scaledC(dielBsdf(base, n, view, l, a, ei, et), transmission);
+		}
+		r = r + scaledC(schlickBsdf(C{0.04, 0.04, 0.04}, n, view, l, ca), m.clearcoat * 0.25);
+		return r;
+	}
+	double	principledPdf(const Mat& m, const Hit& h, V view, V l)	// scatteringPDF, :855-921
+	{
+		const Lobes w = principledLobes(m, clamp01(m.color));
+		const double a = roughnessToAlpha(m.roughness), ca = roughnessToAlpha(m.clearcoatRoughness);
+		const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0;
+		double pdfs[7] = {};
+		pdfs[0] = cosPdf(h.n, l);
+		pdfs[1] = cosPdf(h.n, l);	// subsurfaceCosinePDF (Burley)
+		pdfs[2] = pdfs[0];
+		pdfs[3] = ggxReflPdf(h.n, view, l, a);
+		pdfs[4] = pdfs[3];
+		if (dot(h.n, l) < 0.0) pdfs[5] = dielPdf(h.n, view, l, a, ei, et);
+		pdfs[6] = ggxReflPdf(h.n, view, l, ca);
+		double pdf = 0.0;
+		for (int i = 0; i < 7; i++) if (w.w[i] > 0.0 && pdfs[i] > 0.0) pdf += (w.w[i] / w.total) * pdfs[i];
+		return (std::isfinite(pdf) && pdf > 0.0) ? pdf : 0.0;
+	}
+	C	diffuseGlossyEval(const Mat& m, const Hit& h, V view, V l)	// DiffuseGlossy.cpp:151-168
+	{
+		const double noL = dot(h.n, l);
+		const C diffuse = noL <= 0.0 ? C{} : clamp01(m.color) * (noL / PI);
+		const C glossy = ggxReflBsdf(clamp01(m.glossyColor), h.n, view, l, roughnessToAlpha(m.roughness));
+		return diffuse * (1.0 - m.glossyWeight) + glossy * m.glossyWeight;
+	}
+	double	diffuseGlossyPdf(const Mat& m, const Hit& h, V view, V l)	// :170-186
+	{
+		return cosPdf(h.n, l) * (1.0 - m.glossyWeight) + ggxReflPdf(h.n, view, l, roughnessToAlpha(m.roughness)) * m.glossyWeight;
+	}
+
 	double	bsdfPdf(const Mat& m, const Hit& h, const Scatter& sc, V l)
 	{
 		const V view = safeNormalize(sc.incident * -1.0, h.n);
-		if (m.kind == luzgpu::MAT_METAL) return ggxReflPdf(h.n, view, l, m.alpha);	// Metal.cpp:202-216
+		if (m.kind == luzgpu::MAT_METAL || m.kind == luzgpu::MAT_GLOSSY) return ggxReflPdf(h.n, view, l, m.alpha);	// Metal.cpp:202-216
+		if (m.kind == luzgpu::MAT_DIFFUSE_GLOSSY) return diffuseGlossyPdf(m, h, view, l);
+		if (m.kind == luzgpu::MAT_PRINCIPLED) return principledPdf(m, h, view, l);
 		const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0;		// Dielectric.cpp:230-245
 		return dielPdf(h.n, view, l, m.alpha, ei, et);
 	}
@@ -1204,6 +1318,9 @@ namespace oracle
 	{
 		const V view = safeNormalize(sc.incident * -1.0, h.n);
 		if (m.kind == luzgpu::MAT_METAL) return metalBsdf(m, h.n, view, l);	// Metal.cpp:172-200
+		if (m.kind == luzgpu::MAT_GLOSSY) return ggxReflBsdf(clamp01(m.color), h.n, view, l, m.alpha);	// Glossy.cpp:85-100
+		if (m.kind == luzgpu::MAT_DIFFUSE_GLOSSY) return diffuseGlossyEval(m, h, view, l);
+		if (m.kind == luzgpu::MAT_PRINCIPLED) return principledEval(m, h, view, l);
 		const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0;
 		return dielBsdf(m.color, h.n, view, l, m.alpha, ei, et);				// Dielectric.cpp:210-228
 	}
@@ -1244,6 +1361,93 @@ namespace oracle
 			sc.type = PDF_COSINE;
 			sc.basis = Onb(h.n);
 			return true;
+		}
+		auto ggxReflect = [&](double a) {	// Glossy.cpp:22-40
+			const V view = safeNormalize(r.d * -1.0, h.n);
+			float sx, sy;
+			ps.s2(D_BSDF_DIRECTION, sx, sy);
+			V half = ggxHalf(h.n, a, sx, sy);
+			if (dot(view, half) <= 0.0) half = h.n;
+			V dir = reflect(r.d, half);
+			if (dot(dir, h.n) <= 0.0) dir = reflect(r.d, h.n);
+			return dir;
+		};
+		auto cosineDir = [&]() {	// Sampler::cosineHemisphere in ONB(normal)
+			float sx, sy;
+			ps.s2(D_BSDF_DIRECTION, sx, sy);
+			const float phi = static_cast<float>(2.0 * PI) * sx;
+			const float rr = std::sqrt(sy);
+			const float z = std::sqrt(std::max(0.0f, 1.0f - sy));
+			return Onb(h.n).local(static_cast<double>(rr * std::cos(phi)), static_cast<double>(rr * std::sin(phi)), static_cast<double>(z));
+		};
+		auto accept = [&](V dir) {
+			sc.type = PDF_BSDF;
+			const double pdf = bsdfPdf(m, h, sc, dir);
+			const C f = bsdfEval(m, h, sc, dir);
+			if (pdf <= 0.0 || !std::isfinite(pdf) || maxChannel(f) <= 0.0) { sc.type = PDF_NONE; return false; }
+			sc.dir = dir;
+			sc.att = f / pdf;
+			return true;
+		};
+		if (m.kind == luzgpu::MAT_GLOSSY) return accept(ggxReflect(m.alpha));	// Glossy.cpp:64-83
+		if (m.kind == luzgpu::MAT_DIFFUSE_GLOSSY)	// DiffuseGlossy.cpp:122-149
+		{
+			V dir;
+			if (ps.s1(D_MATERIAL_DECISION) < m.glossyWeight) dir = ggxReflect(roughnessToAlpha(m.roughness));
+			else dir = cosineDir();
+			return accept(dir);
+		}
+		if (m.kind == luzgpu::MAT_PRINCIPLED)	// Principled.cpp:668-761
+		{
+			const C base = clamp01(m.color);
+			if (m.metallic <= 1e-6 && m.roughness <= 1e-4 && m.transmission >= 1.0 - 1e-6 && m.clearcoat <= 1e-6 && m.sheen <= 1e-6)
+			{
+				// scatterSmoothTransmission (:401-440)
+				const V view = safeNormalize(r.d * -1.0, h.n);
+				const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0, ratio = ei / et;
+				const double ct = std::min(1.0, dot(view, h.n));
+				const double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+				V dir;
+				if (ratio * st > 1.0 || dielectricFresnel(ct, ei, et) > ps.s1(D_MATERIAL_DECISION))
+				{
+					dir = reflect(r.d, h.n);
+					sc.att = {1, 1, 1};
+				}
+				else
+				{
+					const double c = std::fmin(dot(r.d * -1.0, h.n), 1.0);
+					const V perp = ratio * (r.d + c * h.n);
+					const V par = -std::sqrt(std::fabs(1.0 - len2(perp))) * h.n;
+					dir = perp + par;
+					sc.att = base;
+				}
+				sc.dir = Ray::make(h.p, dir).d;
+				sc.specular = true;
+				return true;
+			}
+			const Lobes w = principledLobes(m, base);
+			double target = ps.s1(D_MATERIAL_DECISION) * w.total;	// selectComponent (:204-216)
+			int comp = 0;
+			for (int i = 0; i < 7; i++)
+			{
+				target -= w.w[i];
+				if (target <= 0.0) { comp = i; break; }
+			}
+			V dir;
+			if (comp <= 2) dir = cosineDir();
+			else if (comp == 5)	// sampleGGXTransmission (:359-399)
+			{
+				const V view = safeNormalize(r.d * -1.0, h.n);
+				const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0;
+				float sx, sy;
+				ps.s2(D_BSDF_DIRECTION, sx, sy);
+				V half = ggxHalf(h.n, roughnessToAlpha(m.roughness), sx, sy);
+				if (dot(view, half) <= 0.0) half = h.n;
+				if (!refractDir(r.d, half, ei / et, dir)) dir = reflect(r.d, half);
+			}
+			else if (comp == 6) dir = ggxReflect(roughnessToAlpha(m.clearcoatRoughness));
+			else dir = ggxReflect(roughnessToAlpha(m.roughness));
+			return accept(dir);
 		}
 		if (m.kind == luzgpu::MAT_ISOTROPIC)	// Isotropic.cpp:15-24
 		{
@@ -1333,7 +1537,8 @@ namespace oracle
 	bool	leavesOpaque(const Mat& m, const Hit& h, V d)	// ColorHelper.cpp:357-374
 	{
 		// materialCanTransmitThroughGeometry (:327-355)
-		if (m.kind == luzgpu::MAT_DIELECTRIC || m.kind == luzgpu::MAT_ISOTROPIC || m.kind == luzgpu::MAT_HG) return true;
+		if (m.kind == luzgpu::MAT_DIELECTRIC || m.kind == luzgpu::MAT_ISOTROPIC || m.kind == luzgpu::MAT_HG
+			|| (m.kind == luzgpu::MAT_PRINCIPLED && m.transmission > 0.0)) return true;
 		V g = h.gn;
 		if (len2(g) <= 1e-12) g = h.n;
 		if (len2(g) <= 1e-12) return true;

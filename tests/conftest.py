@@ -36,6 +36,12 @@ def primitives(workload_dir):
 
 
 @pytest.fixture(scope="session")
+def materials(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_materials(os.path.join(workload_dir, "materials"))
+
+
+@pytest.fixture(scope="session")
 def volume(workload_dir):
     from luz_amd import workloads
     return workloads.write_volume(os.path.join(workload_dir, "volume"))

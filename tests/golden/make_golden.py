@@ -108,10 +108,13 @@ def main():
             small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
         if "volume" in only:
             small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
+        if "materials" in only:
+            small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
         return
     sampler_fixture(tmp)
     small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
     small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
+    small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
 
     # Cornell box, 300x300, 50 spp, depth 8, seed 1 (the plain CPU-runnable config)
     c = workloads.write_cornell(os.path.join(tmp, "cornell"))

@@ -97,3 +97,19 @@ def test_volume_and_phase_validation(tmp_path):
     with pytest.raises(_lib.LuzGpuError) as e:
         SceneFile.read(None, str(preset))
     assert e.value.code == _lib.E_UNSUPPORTED
+
+
+def test_material_validation(tmp_path):
+    """Principled / glossy parameters outside [0,1] are rejected (Principled.cpp:35-42),
+    and subsurface scattering is outside the device path."""
+    import pytest
+    from luz_amd import _lib
+    text = ("[settings]\nresolution=8,8\nsamples=1\n\n[materials]\nmaterial m {{\n{props}\n}}\n\n"
+            "[scene]\ncamera c {{\nposition=(0,0,0)\n}}\nobjects{{\nsphere=(0,0,5),1,material=m\n}}\n")
+    for props, code in (("type=principled\nmetallic=1.5", _lib.E_INVALID), ("type=glossy\nroughness=-0.1", _lib.E_INVALID),
+                        ("type=principled\nsubsurface=0.5", _lib.E_UNSUPPORTED)):
+        f = tmp_path / "m.luz"
+        f.write_text(text.format(props=props))
+        with pytest.raises(_lib.LuzGpuError) as e:
+            SceneFile.read(None, str(f))
+        assert e.value.code == code, props
