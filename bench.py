@@ -71,12 +71,62 @@ def load_traffic(workload: str, kernel: str):
         return None
 
 
+def bench_batch(args, wdir):
+    """32 uneven scenes in one multi-scene launch sequence against the same scenes rendered
+    one at a time by the plain single-scene megakernel (and by the single-scene wavefront)."""
+    import torch
+    from luz_amd import Context, Renderer, SceneFile, workloads
+    wls = workloads.batch_scenes(os.path.join(wdir, "batch"))
+    ctx = Context(0)
+    scenes = [SceneFile.read(ctx, w.scene_path) for w in wls]
+    seeds = [args.seed + i for i in range(len(wls))]
+    outs = [torch.empty(w.width * w.height * 3, dtype=torch.float32, device="cuda") for w in wls]
+    total = sum(w.width * w.height * w.spp for w in wls)
+    log(f"batch: {len(wls)} scenes, {sum(w.triangles for w in wls)} triangles, {total / 1e6:.1f} M samples per step")
+    for i in range(args.warmup):
+        _, st = Renderer.render_batch(scenes, seeds, outs=outs)
+        log(f"warmup {i}: {st.render_ms:.1f} ms")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, st = Renderer.render_batch(scenes, seeds, outs=outs)
+        log(f"step {i}: {st.render_ms:.1f} ms, {st.kernel_launches} passes")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = total * args.steps / elapsed / 1e6
+    # the same scenes one by one (untimed for the headline; each timed by its own events)
+    plain_ms = wave_ms = 0.0
+    for w, sc, sd, o in zip(wls, scenes, seeds, outs):
+        _, a = Renderer.render(sc, seed=sd, integrator="megakernel", out=o)
+        _, b = Renderer.render(sc, seed=sd, integrator="wavefront", out=o)
+        plain_ms += a.render_ms
+        wave_ms += b.render_ms
+    batch_ms = elapsed * 1000.0 / args.steps
+    log(f"one at a time: megakernel {plain_ms:.0f} ms, wavefront {wave_ms:.0f} ms; batched {batch_ms:.0f} ms "
+        f"({plain_ms / batch_ms:.2f}x / {wave_ms / batch_ms:.2f}x)")
+    line = {
+        "metric": "Msamples/s", "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(batch_ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch: {len(wls)} scenes of displaced sphere meshes, triangles log-uniform [1e4, 5e5], "
+                               f"resolutions from 256^2 to 1024^2, spp 16..128, depth 10",
+                   "scenes": len(wls), "samples_per_step": total, "integrator": "wavefront, multi-scene, hipGraph passes"},
+        "batched_vs_plain_megakernel": round(plain_ms / batch_ms, 3),
+        "batched_vs_single_scene_wavefront": round(wave_ms / batch_ms, 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    for sc in scenes:
+        sc.release()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="spheres", choices=["spheres", "cornell"])
+    ap.add_argument("--workload", default="spheres", choices=["spheres", "cornell", "batch", "terrain"])
     ap.add_argument("--integrator", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -94,7 +144,11 @@ def main():
 
     wdir = os.path.join(tempfile.gettempdir(), "luz_bench_workloads")
     t0 = time.time()
-    if args.workload == "spheres":
+    if args.workload == "batch":
+        return bench_batch(args, wdir)
+    if args.workload == "terrain":
+        wl = workloads.terrain_scene(os.path.join(wdir, "terrain"))
+    elif args.workload == "spheres":
         wl = workloads.spheres_scene(os.path.join(wdir, "spheres"))
     else:
         wl = workloads.write_cornell(os.path.join(wdir, "cornell"))

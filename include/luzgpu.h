@@ -109,6 +109,17 @@ int		luzgpu_scene_get_info(const luzgpu_scene* scene, luzgpu_scene_info* out);
 int		luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings,
 			float* rgb_out, luzgpu_stats* stats);
 
+/* Renders n scenes (1..127) in one multi-scene wavefront launch sequence: the scenes'
+ * device arrays are merged (and cached while the same scenes are passed again), every pass
+ * covers a slice of samples of every scene, and the pass's launches are replayed from a
+ * captured hipGraph. Replaces n concurrent Renderer::render calls, which the reference
+ * cannot run (its render seed is global, Sampler.cpp:28). settings[i] (or NULL: scene
+ * defaults) and rgb_out[i] are per scene, as for luzgpu_render; the integrator must be the
+ * wavefront one, and launch tuning, visit counting and stage timing are taken from
+ * settings[0]. stats, if given, receives one aggregate record for the batch. */
+int		luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes, const luzgpu_settings* settings,
+			float* const* rgb_out, luzgpu_stats* stats);
+
 /* --- introspection used by the parity tests (no counterpart in the reference API) --- */
 
 /* Primary-ray closest hits of sample 0, stream 0 (RayHelper.cpp:85-106 + HitHelper.cpp:5-32):

@@ -233,5 +233,24 @@ namespace luzgpu
 		float	metersPerUnit;
 		std::uint32_t	unboundedCount;	// hittables without a box (planes), tested after the BVH
 		std::uint32_t	hasVolumes;		// rays carry a volume free-flight sample
+		// where this scene's tables start in the (possibly multi-scene) device arrays
+		std::uint32_t	lightBase;		// lights / lightCdf
+		std::uint32_t	lightWeighted;	// lightCdf holds weights (else uniform selection)
+		std::uint32_t	envBase;		// envPixels / envPdf / envCdf texel
+		std::uint32_t	unboundedBase;
+	};
+
+	// Scene content classes the kernels are specialised on (a scene without them runs
+	// kernels that do not carry their code or registers).
+	constexpr int	kFeatAnalytic = 1;	// spheres, cubes, volumes, planes, directional lights
+	constexpr int	kFeatRich = 2;		// glossy, diffuse-glossy, principled, isotropic, Henyey-Greenstein
+	constexpr int	kFeatAll = 3;
+
+	// Per-scene traversal entry of a multi-scene launch: TLAS root and the unbounded
+	// (plane) list as base << 16 | count.
+	struct	GpuSceneSlot
+	{
+		std::int32_t	tlasRoot;
+		std::uint32_t	unbounded;
 	};
 }

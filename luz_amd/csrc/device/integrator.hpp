@@ -78,6 +78,7 @@ namespace luzgpu
 		bool	active;
 	};
 
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ Connection	connectLight(const SceneView& s, const GpuParams& p, const Surface& sf, const GpuMaterial& m,
 		const Scatter& sc, int bounce, const PixelSampler& ps)
 	{
@@ -85,10 +86,10 @@ namespace luzgpu
 		c.active = false;
 		const float prob = directProbability(bounce);
 		if (prob < 1.0f && ps.sample1D(DIM_LIGHT_STRATEGY) >= static_cast<double>(prob)) return c;
-		const LightSample ls = sampleLight(s, p, sf.position, ps);
+		const LightSample ls = sampleLight<Feat>(s, p, sf.position, ps);
 		if (!ls.valid || !leavesOpaque(sf, m, ls.direction)) return c;
-		const float spdf = scatterPdf(m, sf, sc, ls.direction);
-		const f3 f = scatterBsdfCos(m, sf, sc, ls.direction);
+		const float spdf = scatterPdf<Feat>(m, sf, sc, ls.direction);
+		const f3 f = scatterBsdfCos<Feat>(m, sf, sc, ls.direction);
 		if (spdf <= 0.0f || !isfinite(spdf) || maxc(f) <= 0.0f) return c;
 		const float w = powerHeuristic(ls.pdf, spdf);
 		c.contribution = f * ls.emitted * (w / (ls.pdf * prob));
@@ -103,6 +104,7 @@ namespace luzgpu
 		return c;
 	}
 
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ Connection	connectInfinite(const SceneView& s, const GpuParams& p, const Surface& sf, const GpuMaterial& m,
 		const Scatter& sc, int bounce, const PixelSampler& ps)
 	{
@@ -114,8 +116,8 @@ namespace luzgpu
 		if (prob < 1.0f && ps.sample1D(DIM_ENVIRONMENT_STRATEGY) >= static_cast<double>(prob)) return c;
 		const LightSample ls = sampleInfinite(s, p, ps);
 		if (!ls.valid || !leavesOpaque(sf, m, ls.direction)) return c;
-		const float spdf = scatterPdf(m, sf, sc, ls.direction);
-		const f3 f = scatterBsdfCos(m, sf, sc, ls.direction);
+		const float spdf = scatterPdf<Feat>(m, sf, sc, ls.direction);
+		const f3 f = scatterBsdfCos<Feat>(m, sf, sc, ls.direction);
 		if (spdf <= 0.0f || !isfinite(spdf) || maxc(f) <= 0.0f) return c;
 		const float w = powerHeuristic(ls.pdf, spdf);
 		f3 contrib = f * ls.emitted * (w / (ls.pdf * prob));
@@ -191,11 +193,12 @@ namespace luzgpu
 	}
 
 	// scatterPDFGenerate (ColorHelper.cpp:150-166)
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ f3	scatterDirection(const Surface& sf, const Scatter& sc, const PixelSampler& ps)
 	{
 		if (sc.pdfType == PDF_COSINE) return cosineDirection(sf, ps);
-		if (sc.pdfType == PDF_SPHERE) return sphereDirection(ps);
-		if (sc.pdfType == PDF_HG) return hgDirection(sc, ps);
+		if ((Feat & kFeatRich) && sc.pdfType == PDF_SPHERE) return sphereDirection(ps);
+		if ((Feat & kFeatRich) && sc.pdfType == PDF_HG) return hgDirection(sc, ps);
 		return sc.direction;
 	}
 
@@ -213,7 +216,7 @@ namespace luzgpu
 	}
 
 	// Whole path in one thread: the plain (megakernel) integrator.
-	template <bool Count, typename Stack>
+	template <bool Count, int Feat, typename Stack>
 	__device__ f3	tracePath(const SceneView& s, const GpuParams& p, std::uint32_t x, std::uint32_t y, std::uint32_t sample,
 		Stack& stack, std::uint32_t& closestRays, std::uint32_t& shadowRays, VisitCounter& visits)
 	{
@@ -230,7 +233,7 @@ namespace luzgpu
 			const float logU = volumeLogU(p, ps);
 			HitInfo h;
 			closestRays++;
-			if (!traverse<false, Count>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack, &visits, logU))
+			if (!traverse<false, Count, Feat>(s, p, ray, kTMinF, INFINITY, h, stack, &visits, logU))
 			{
 				float w = 1.0f;
 				if (b > 0 && !prevSpecular && prevInfinite)
@@ -241,13 +244,13 @@ namespace luzgpu
 				if (b > 0 && hasConstantBackgroundLight(p)) c = clampConstantBackground(p, c);
 				return acc + clampRay(c);
 			}
-			const Surface sf = surfaceAt(s, ray, h);
+			const Surface sf = surfaceAt<Feat>(s, ray, h);
 			const GpuMaterial& m = s.materials[sf.material];
 			const f3 emitted = m.kind == GPU_MAT_EMISSIVE ? ld3(m.color) : mk(0, 0, 0);
 			Scatter sc;
-			if (!scatter(m, sf, ray.d, ps, sc))
+			if (!scatter<Feat>(m, sf, ray.d, ps, sc))
 			{
-				const float lp = (!prevSpecular && p.lightCount > 0) ? lightsPdf(s, p, prevOrigin, ray.d) : 0.0f;
+				const float lp = (!prevSpecular && p.lightCount > 0) ? lightsPdf<Feat>(s, p, prevOrigin, ray.d) : 0.0f;
 				const float w = prevSpecular ? 1.0f : powerHeuristic(prevPdf, lp);
 				return acc + clampRay((thr * emitted) * w);
 			}
@@ -274,7 +277,7 @@ namespace luzgpu
 			if (terminated(thr * att)) return acc;
 			if (p.lightCount > 0)
 			{
-				const Connection c = connectLight(s, p, sf, m, sc, b, ps);
+				const Connection c = connectLight<Feat>(s, p, sf, m, sc, b, ps);
 				if (c.active)
 				{
 					bool occluded = false;
@@ -282,7 +285,7 @@ namespace luzgpu
 					{
 						HitInfo sh;
 						shadowRays++;
-						occluded = traverse<true, Count>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, c.tMax, sh, stack, &visits,
+						occluded = traverse<true, Count, Feat>(s, p, makeRay(sf.position, c.direction), kTMinF, c.tMax, sh, stack, &visits,
 							logU);
 					}
 					if (!occluded) acc = acc + clampRay(thr * c.contribution);
@@ -290,20 +293,20 @@ namespace luzgpu
 			}
 			if (sampledInfinite)
 			{
-				const Connection c = connectInfinite(s, p, sf, m, sc, b, ps);
+				const Connection c = connectInfinite<Feat>(s, p, sf, m, sc, b, ps);
 				if (c.active)
 				{
 					HitInfo sh;
 					shadowRays++;
-					if (!traverse<true, Count>(s, p.tlasRoot, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack, &visits, logU))
+					if (!traverse<true, Count, Feat>(s, p, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack, &visits, logU))
 					{
 						acc = acc + clampRay(thr * c.contribution);
 					}
 				}
 			}
-			const f3 dir = scatterDirection(sf, sc, ps);
+			const f3 dir = scatterDirection<Feat>(sf, sc, ps);
 			if (!leavesOpaque(sf, m, dir)) return acc;
-			const float pdf = scatterPdf(m, sf, sc, dir);
+			const float pdf = scatterPdf<Feat>(m, sf, sc, dir);
 			if (pdf <= 0.0f || !isfinite(pdf)) return acc;
 			thr = clampRay(thr * att);
 			if (terminated(thr) || !russianRoulette(thr, b, ps)) return acc;

@@ -38,6 +38,7 @@ namespace luzgpu
 	}
 
 	// Rebuilds the reference's HitRecord from a traversal result.
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ Surface	surfaceAt(const SceneView& s, const Ray& ray, const HitInfo& h)
 	{
 		Surface sf;
@@ -46,7 +47,7 @@ namespace luzgpu
 		const std::uint32_t idx = gh.kindIndex & kIndexMask;
 		sf.material = gh.material;
 		sf.position = ray.o + ray.d * h.t;
-		if (kind == GPU_HIT_SPHERE)
+		if ((Feat & kFeatAnalytic) && kind == GPU_HIT_SPHERE)
 		{
 			const GpuSphere sp = s.spheres[idx];
 			const f3 outward = (sf.position - ld3(sp.center)) * (1.0f / sp.radius);
@@ -55,7 +56,7 @@ namespace luzgpu
 			sf.geometricNormal = sf.normal;
 			return sf;
 		}
-		if (kind == GPU_HIT_CUBE || kind == GPU_HIT_PLANE)
+		if ((Feat & kFeatAnalytic) && (kind == GPU_HIT_CUBE || kind == GPU_HIT_PLANE))
 		{
 			const f3 n = kind == GPU_HIT_CUBE ? ld3(s.cubeFaces[6 * idx + h.tri].normal) : ld3(s.planes[idx].orientation);
 			sf.frontFace = dot(ray.d, n) < 0.0f;
@@ -63,7 +64,7 @@ namespace luzgpu
 			sf.geometricNormal = sf.normal;
 			return sf;
 		}
-		if (kind == GPU_HIT_VOLUME)
+		if ((Feat & kFeatAnalytic) && kind == GPU_HIT_VOLUME)
 		{
 			// ConstantVolume::hit: arbitrary normal; the phase functions ignore it
 			sf.normal = mk(1.0f, 0.0f, 0.0f);
@@ -437,19 +438,21 @@ namespace luzgpu
 		return (isfinite(pdf) && pdf > 0.0f) ? pdf : 0.0f;
 	}
 
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ float	bsdfPdf(const GpuMaterial& m, const Surface& sf, const Scatter& sc, f3 dir)
 	{
 		const f3 view = safeNormalize(-sc.incident, sf.normal);
-		if (m.kind == GPU_MAT_METAL || m.kind == GPU_MAT_GLOSSY)
+		if (m.kind == GPU_MAT_METAL || ((Feat & kFeatRich) && m.kind == GPU_MAT_GLOSSY))
 		{
 			return ggxReflectionPdf(sf.normal, view, dir, m.alpha);
 		}
-		if (m.kind == GPU_MAT_DIFFUSE_GLOSSY) return diffuseGlossyPdf(m, sf.normal, view, dir);
-		if (m.kind == GPU_MAT_PRINCIPLED) return principledPdf(m, sf, view, dir);
+		if ((Feat & kFeatRich) && m.kind == GPU_MAT_DIFFUSE_GLOSSY) return diffuseGlossyPdf(m, sf.normal, view, dir);
+		if ((Feat & kFeatRich) && m.kind == GPU_MAT_PRINCIPLED) return principledPdf(m, sf, view, dir);
 		const float etaI = sf.frontFace ? 1.0f : m.ior, etaT = sf.frontFace ? m.ior : 1.0f;
 		return dielectricPdf(sf.normal, view, dir, m.alpha, etaI, etaT);
 	}
 
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ f3	bsdfEval(const GpuMaterial& m, const Surface& sf, const Scatter& sc, f3 dir)
 	{
 		const f3 view = safeNormalize(-sc.incident, sf.normal);
@@ -457,14 +460,15 @@ namespace luzgpu
 		{
 			return metalBsdfCos(m, sf.normal, view, dir);
 		}
-		if (m.kind == GPU_MAT_GLOSSY) return ggxReflectionBsdfCos(clampColor01(ld3(m.color)), sf.normal, view, dir, m.alpha);
-		if (m.kind == GPU_MAT_DIFFUSE_GLOSSY) return diffuseGlossyEval(m, sf.normal, view, dir);
-		if (m.kind == GPU_MAT_PRINCIPLED) return principledEval(m, sf, view, dir);
+		if ((Feat & kFeatRich) && m.kind == GPU_MAT_GLOSSY) return ggxReflectionBsdfCos(clampColor01(ld3(m.color)), sf.normal, view, dir, m.alpha);
+		if ((Feat & kFeatRich) && m.kind == GPU_MAT_DIFFUSE_GLOSSY) return diffuseGlossyEval(m, sf.normal, view, dir);
+		if ((Feat & kFeatRich) && m.kind == GPU_MAT_PRINCIPLED) return principledEval(m, sf, view, dir);
 		const float etaI = sf.frontFace ? 1.0f : m.ior, etaT = sf.frontFace ? m.ior : 1.0f;
 		return dielectricBsdfCos(ld3(m.color), sf.normal, view, dir, m.alpha, etaI, etaT);
 	}
 
 	// scatterPDFValue (ColorHelper.cpp:39-93)
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ float	scatterPdf(const GpuMaterial& m, const Surface& sf, const Scatter& sc, f3 dir)
 	{
 		if (sc.pdfType == PDF_COSINE)
@@ -472,17 +476,18 @@ namespace luzgpu
 			const float c = dot(dir, sc.cosineW);
 			return c <= 0.0f ? 0.0f : c / kPiF;
 		}
-		if (sc.pdfType == PDF_SPHERE) return 1.0f / (4.0f * kPiF);
-		if (sc.pdfType == PDF_HG) return hgPdf(sc.cosineW, sc.g, dir);
-		if (sc.pdfType == PDF_BSDF) return bsdfPdf(m, sf, sc, dir);
+		if ((Feat & kFeatRich) && sc.pdfType == PDF_SPHERE) return 1.0f / (4.0f * kPiF);
+		if ((Feat & kFeatRich) && sc.pdfType == PDF_HG) return hgPdf(sc.cosineW, sc.g, dir);
+		if (sc.pdfType == PDF_BSDF) return bsdfPdf<Feat>(m, sf, sc, dir);
 		return 0.0f;
 	}
 
 	// scatterBSDFCos (ColorHelper.cpp:95-113)
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ f3	scatterBsdfCos(const GpuMaterial& m, const Surface& sf, const Scatter& sc, f3 dir)
 	{
-		if (sc.pdfType == PDF_BSDF) return bsdfEval(m, sf, sc, dir);
-		return sc.attenuation * scatterPdf(m, sf, sc, dir);
+		if (sc.pdfType == PDF_BSDF) return bsdfEval<Feat>(m, sf, sc, dir);
+		return sc.attenuation * scatterPdf<Feat>(m, sf, sc, dir);
 	}
 
 	__device__ __forceinline__ f3	normalizeOrZero(f3 d)
@@ -533,6 +538,7 @@ namespace luzgpu
 
 	// Material::scatter for the supported materials. Returns false when the material
 	// does not scatter (emissive) or the BSDF sample is rejected.
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ bool	scatter(const GpuMaterial& m, const Surface& sf, f3 inDir, const PixelSampler& ps, Scatter& sc)
 	{
 		sc.incident = inDir;
@@ -546,6 +552,8 @@ namespace luzgpu
 			sc.cosineW = normalize(sf.normal);
 			return true;
 		}
+		if constexpr ((Feat & kFeatRich) != 0)
+		{
 		if (m.kind == GPU_MAT_ISOTROPIC)
 		{
 			sc.attenuation = ld3(m.color);
@@ -631,6 +639,7 @@ namespace luzgpu
 			sc.cosineW = len2(inDir) <= 0.0f ? mk(0.0f, 0.0f, 1.0f) : normalize(inDir);
 			sc.g = m.anisotropy;
 			return true;
+		}
 		}
 		if (m.kind == GPU_MAT_METAL)
 		{
@@ -794,6 +803,7 @@ namespace luzgpu
 		return ls;
 	}
 
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ LightSample	sampleLight(const SceneView& s, const GpuParams& p, f3 origin, const PixelSampler& ps)
 	{
 		LightSample ls;
@@ -803,15 +813,16 @@ namespace luzgpu
 		if (n > 1)
 		{
 			const double u = ps.sample1D(DIM_LIGHT_SELECTION);
-			if (s.lightCdf != nullptr)
+			if (p.lightWeighted)
 			{
-				const double target = u * s.lightCdf[n - 1];
+				const double* cdf = s.lightCdf + p.lightBase;
+				const double target = u * cdf[n - 1];
 				// std::upper_bound: first element > target
 				std::uint32_t lo = 0, hi = n;
 				while (lo < hi)
 				{
 					const std::uint32_t mid = (lo + hi) >> 1;
-					if (s.lightCdf[mid] > target) hi = mid;
+					if (cdf[mid] > target) hi = mid;
 					else lo = mid + 1;
 				}
 				idx = lo < n - 1 ? lo : n - 1;
@@ -822,9 +833,9 @@ namespace luzgpu
 				idx = r < n - 1 ? r : n - 1;
 			}
 		}
-		const GpuLight& L = s.lights[idx];
+		const GpuLight& L = s.lights[p.lightBase + idx];
 		if (!(L.selectionProbability > 0.0f)) return ls;
-		if (L.kind == GPU_HIT_DIRECTIONAL)
+		if ((Feat & kFeatAnalytic) && L.kind == GPU_HIT_DIRECTIONAL)
 		{
 			// DirectionalLight::sampleLight: delta direction, pdf 1, tMax = T_MAX
 			ls.direction = ld3(L.direction);
@@ -834,7 +845,7 @@ namespace luzgpu
 			ls.valid = len2(ls.direction) > 0.0f && ls.pdf > 0.0f && maxc(ls.emitted) > 0.0f;
 			return ls;
 		}
-		if (L.kind == GPU_HIT_SPHERE)
+		if ((Feat & kFeatAnalytic) && L.kind == GPU_HIT_SPHERE)
 		{
 			return sampleSphereLight(s, L, origin, ps);
 		}
@@ -878,12 +889,13 @@ namespace luzgpu
 	}
 
 	// Hittable::pdfValue of one light for a direction from `origin` (re-intersects it).
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ float	lightPdfValue(const SceneView& s, const GpuLight& L, f3 origin, f3 vec)
 	{
 		const Ray r = makeRay(origin, normalizeOrZero(vec));
 		const float vl2 = len2(vec);
 		float t;
-		if (L.kind == GPU_HIT_SPHERE)
+		if ((Feat & kFeatAnalytic) && L.kind == GPU_HIT_SPHERE)
 		{
 			// Sphere::pdfValue (Sphere.cpp:362-375)
 			const GpuSphere sp = s.spheres[L.index];
@@ -914,14 +926,16 @@ namespace luzgpu
 	}
 
 	// lightPDFValue (ColorHelper.cpp:233-257)
+	template <int Feat = kFeatAll>
 	__device__ __forceinline__ float	lightsPdf(const SceneView& s, const GpuParams& p, f3 origin, f3 dir)
 	{
 		if (p.lightCount == 0) return 0.0f;
-		if (p.lightCount == 1) return lightPdfValue(s, s.lights[0], origin, dir);
+		const GpuLight* lights = s.lights + p.lightBase;
+		if (p.lightCount == 1) return lightPdfValue<Feat>(s, lights[0], origin, dir);
 		float sum = 0.0f;
 		for (std::uint32_t i = 0; i < p.lightCount; i++)
 		{
-			sum += s.lights[i].selectionProbability * lightPdfValue(s, s.lights[i], origin, dir);
+			sum += lights[i].selectionProbability * lightPdfValue<Feat>(s, lights[i], origin, dir);
 		}
 		return sum;
 	}
@@ -959,8 +973,9 @@ namespace luzgpu
 		const int y0 = min(max(static_cast<int>(fy), 0), H - 1);
 		const int y1 = min(max(static_cast<int>(fy) + 1, 0), H - 1);
 		const float tx = x - fx, ty = y - fy;
-		const float4 c00 = s.envPixels[y0 * W + x0], c10 = s.envPixels[y0 * W + x1];
-		const float4 c01 = s.envPixels[y1 * W + x0], c11 = s.envPixels[y1 * W + x1];
+		const float4* px = s.envPixels + p.envBase;
+		const float4 c00 = px[y0 * W + x0], c10 = px[y0 * W + x1];
+		const float4 c01 = px[y1 * W + x0], c11 = px[y1 * W + x1];
 		const f3 top = mk(c00.x, c00.y, c00.z) * (1.0f - tx) + mk(c10.x, c10.y, c10.z) * tx;
 		const f3 bot = mk(c01.x, c01.y, c01.z) * (1.0f - tx) + mk(c11.x, c11.y, c11.z) * tx;
 		const f3 c = top * (1.0f - ty) + bot * ty;
@@ -974,7 +989,7 @@ namespace luzgpu
 		dirToUV(p, d, u, v);
 		const int x = min(static_cast<int>(floorf(wrap01(u) * static_cast<float>(p.envWidth))), static_cast<int>(p.envWidth) - 1);
 		const int y = min(static_cast<int>(floorf((1.0f - clamp01(v)) * static_cast<float>(p.envHeight))), static_cast<int>(p.envHeight) - 1);
-		return s.envPdf[y * static_cast<int>(p.envWidth) + x];
+		return s.envPdf[p.envBase + y * static_cast<int>(p.envWidth) + x];
 	}
 
 	__device__ __forceinline__ double	clampSample(double v)
@@ -1025,7 +1040,7 @@ namespace luzgpu
 				while (lo < hi)	// std::lower_bound: first element >= target
 				{
 					const std::uint32_t mid = (lo + hi) >> 1;
-					if (s.envCdf[mid] < target) lo = mid + 1;
+					if (s.envCdf[p.envBase + mid] < target) lo = mid + 1;
 					else hi = mid;
 				}
 				const std::uint32_t index = lo < n ? lo : n - 1;

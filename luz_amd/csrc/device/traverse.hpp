@@ -40,7 +40,6 @@ namespace luzgpu
 		const float*	envPdf;
 		const double*	envCdf;
 		const std::uint32_t*	unbounded;	// hittable ids of the planes, tested after the BVH
-		std::uint32_t	unboundedCount;
 		const GpuSphere*	spheres;
 		const GpuRect*	cubeFaces;
 		const GpuVolume*	volumes;
@@ -342,6 +341,7 @@ namespace luzgpu
 		HitInfo	hit;	// hit.t is the current closest distance
 		const Bvh4Node*	nodes;	// node array of the current level
 		float	logU;	// log(max(u, 1e-12)) of the volume free-flight sample (scenes with volumes)
+		std::uint32_t	unbounded;	// this ray's scene: unbounded list base << 16 | count
 		std::uint32_t	entry;
 		std::uint32_t	instHittable;
 		bool	inInstance;
@@ -350,11 +350,12 @@ namespace luzgpu
 
 	// Starts a traversal; returns false when there is nothing to traverse.
 	template <typename Stack>
-	__device__ __forceinline__ bool	traceBegin(TraceState& st, const SceneView& s, std::int32_t tlasRoot, const Ray& worldRay, float tMax,
-		float logU, Stack& stack)
+	__device__ __forceinline__ bool	traceBegin(TraceState& st, const SceneView& s, std::int32_t tlasRoot, std::uint32_t unbounded,
+		const Ray& worldRay, float tMax, float logU, Stack& stack)
 	{
 		const Bvh4Node* tlas = s.tlas;
 		st.logU = logU;
+		st.unbounded = unbounded;
 		st.world = worldRay;
 		st.ray = worldRay;
 		st.hit.t = tMax;
@@ -370,12 +371,12 @@ namespace luzgpu
 		// an any-hit query ends with entries still on the stack: every traversal starts
 		// from an empty stack
 		stack.reset();
-		return tlasRoot >= 0 || s.unboundedCount > 0;
+		return tlasRoot >= 0 || (unbounded & 0xffffu) != 0u;
 	}
 
 	// Processes the current entry; returns true when the traversal has ended (st.found and
 	// st.hit hold the result).
-	template <bool AnyHit, bool Count, typename Stack>
+	template <bool AnyHit, bool Count, int Feat, typename Stack>
 	__device__ __forceinline__ bool	traceStep(const SceneView& s, TraceState& st, float tMin, Stack& stack,
 		VisitCounter* counter)
 	{
@@ -497,7 +498,7 @@ namespace luzgpu
 						if (AnyHit) return true;
 					}
 				}
-				else if (kind == GPU_HIT_SPHERE)
+				else if ((Feat & kFeatAnalytic) && kind == GPU_HIT_SPHERE)
 				{
 					float t;
 					if (sphereHit(s.spheres[idx], st.ray, tMin, st.hit.t, t))
@@ -509,7 +510,7 @@ namespace luzgpu
 						if (AnyHit) return true;
 					}
 				}
-				else if (kind == GPU_HIT_CUBE)
+				else if ((Feat & kFeatAnalytic) && kind == GPU_HIT_CUBE)
 				{
 					// Cube::hit (Cube.cpp:97-111): its six faces, closest first-found
 					for (int f = 0; f < 6; f++)
@@ -525,7 +526,7 @@ namespace luzgpu
 						}
 					}
 				}
-				else if (kind == GPU_HIT_VOLUME)
+				else if ((Feat & kFeatAnalytic) && kind == GPU_HIT_VOLUME)
 				{
 					float t;
 					if (volumeHit(s.volumes[idx], st.ray, tMin, st.hit.t, st.logU, t))
@@ -560,9 +561,10 @@ namespace luzgpu
 		if (stack.empty())
 		{
 			// the unbounded hittables after the acceleration structure (HitHelper.cpp:5-32)
-			for (std::uint32_t i = 0; i < s.unboundedCount; i++)
+			const std::uint32_t ubBase = st.unbounded >> 16, ubCount = (Feat & kFeatAnalytic) ? (st.unbounded & 0xffffu) : 0u;
+			for (std::uint32_t i = 0; i < ubCount; i++)
 			{
-				const std::uint32_t h = s.unbounded[i];
+				const std::uint32_t h = s.unbounded[ubBase + i];
 				const GpuPlane& pl = s.planes[s.hittables[h].kindIndex & kIndexMask];
 				float t;
 				if (planeHit(pl, st.world, tMin, st.hit.t, t))
@@ -580,14 +582,19 @@ namespace luzgpu
 		return false;
 	}
 
-	template <bool AnyHit, bool Count = false, typename Stack>
-	__device__ __forceinline__ bool	traverse(const SceneView& s, std::int32_t tlasRoot, const Ray& worldRay, float tMin, float tMax,
+	__device__ __forceinline__ std::uint32_t	unboundedOf(const GpuParams& p)
+	{
+		return (p.unboundedBase << 16) | p.unboundedCount;
+	}
+
+	template <bool AnyHit, bool Count = false, int Feat = kFeatAll, typename Stack>
+	__device__ __forceinline__ bool	traverse(const SceneView& s, const GpuParams& p, const Ray& worldRay, float tMin, float tMax,
 		HitInfo& hit, Stack& stack, VisitCounter* counter = nullptr, float logU = 0.0f)
 	{
 		TraceState st;
-		if (traceBegin(st, s, tlasRoot, worldRay, tMax, logU, stack))
+		if (traceBegin(st, s, p.tlasRoot, unboundedOf(p), worldRay, tMax, logU, stack))
 		{
-			while (!traceStep<AnyHit, Count>(s, st, tMin, stack, counter))
+			while (!traceStep<AnyHit, Count, Feat>(s, st, tMin, stack, counter))
 			{
 			}
 		}

@@ -3,22 +3,35 @@
 // a stack and a hit record (low VGPR count, high occupancy) and the shading kernel carries
 // no traversal state.
 //
-// Per pass (S samples of every pixel, path id = sample_slot * pixels + pixel):
+// It is multi-scene by construction: a launch renders a batch of scenes whose device
+// arrays were merged into one SceneView (per-scene bases in GpuParams), and one render of
+// one scene is a batch of one. A pass renders a slice of samples of every scene of the
+// batch; its paths are numbered globally (scene job j owns [pathBase_j, pathBase_j +
+// pixels_j * samples_j), sample slot major), and every queued ray carries its scene id.
+//
+// Per pass:
 //   raygen            camera rays (FP64, bit-exact sampler) -> queue 0, path state init
 //   for b = 0..maxBounces:
-//     extend          closest hit for every ray of queue b (queue order, coalesced)
-//     shade           miss / emission / scatter / NEE and environment connections; the
+//     extend          closest hit for every ray of queue b (persistent, lane refill)
+//     shade           miss / emission / scatter / NEE and environment connections, one
+//                     scene per wave iteration (waterfall over the wave's scene ids); the
 //                     next ray and up to two shadow rays are appended with one atomic per
 //                     wave (ballot + popcount prefix)
 //     shadow          any-hit test of the appended shadow rays -> visibility bytes
 //     resolve         adds visible connection terms in the reference's order (light, then
 //                     environment) and writes the colour of every path that ended
-// Queue lengths live in device memory and every stage is a grid-stride loop over them, so
-// the host enqueues the whole pass without a synchronisation.
+//   accumulate        adds the pass's sample colours into each scene's FP64 pixel sums
+// Queue lengths and the pass's job table live in device memory and every stage is a
+// grid-stride or work-claiming loop over them, so a pass is a fixed launch sequence that
+// the host captures once as a hipGraph and replays for every pass.
 #pragma once
 
 #include "integrator.hpp"
 
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -49,11 +62,32 @@ namespace luzgpu
 		std::uint8_t*	visible;	// [2 * path + slot]
 		// shadow queue (rebuilt every bounce)
 		float4*	sOrigin;	// xyz, w = tMax
-		float4*	sDir;		// xyz, w = 2 * path + slot (bits)
-		// counters: [b] rays in queue b, [kMaxBounce + 1 + b] shadow rays of bounce b
+		float4*	sDir;		// xyz, w = scene << 25 | (2 * path + slot) (bits)
+		float*	sLogU;		// volume free-flight sample of the shadow ray (batches with volumes)
+		// counters: [b] rays in queue b, [kMaxBounceSlots + b] shadow rays of bounce b,
+		// [2 * kMaxBounceSlots + b] / [3 * ...] extend / shadow work claims
 		std::uint32_t*	counts;
 		std::uint32_t	capacity;
 		std::uint32_t*	spill;		// traversal stack spill, [entry][grid thread]
+	};
+
+	// Queued extension rays carry their scene in qPath (scene << 24 | global path id), and
+	// their volume sample in qOrigin.w.
+	constexpr std::uint32_t	kPathBits = 24;
+	constexpr std::uint32_t	kPathMask = (1u << kPathBits) - 1u;
+	constexpr std::uint32_t	kTargetBits = 25;
+	constexpr std::uint32_t	kTargetMask = (1u << kTargetBits) - 1u;
+	constexpr std::uint32_t	kMaxBatchScenes = 127;
+	constexpr std::uint32_t	kMaxPassPaths = 1u << 24;
+
+	// One scene of a pass. The table lives in device memory: a pass is replayed with a new
+	// table and the same launches.
+	struct	PassJob
+	{
+		GpuParams	params;		// sampleBegin / sampleCount: this pass's slice
+		std::uint32_t	pathBase;	// first global path id of the slice
+		std::uint32_t	pixels;
+		double*	sums;			// the scene's FP64 pixel sums
 	};
 
 	constexpr int	kWaveBlock = 256;
@@ -81,27 +115,50 @@ namespace luzgpu
 		return base + static_cast<std::uint32_t>(__popcll(below));
 	}
 
-	__global__ void	wf_zero_counts(std::uint32_t* counts, std::uint32_t n, std::uint32_t first)
+	// The pass's total path count: written into counts[0] (queue 0 holds every path).
+	__global__ void	wf_zero_counts(std::uint32_t* counts, std::uint32_t n, const PassJob* jobs, std::uint32_t jobCount)
 	{
 		const std::uint32_t i = threadIdx.x;
-		if (i < n) counts[i] = (i == 0) ? first : 0u;
+		if (i < n)
+		{
+			std::uint32_t first = 0;
+			if (i == 0)
+			{
+				for (std::uint32_t j = 0; j < jobCount; j++) first += jobs[j].pixels * jobs[j].params.sampleCount;
+			}
+			counts[i] = first;
+		}
 	}
 
-	__global__ __launch_bounds__(kWaveBlock) void	wf_raygen(GpuParams p, WaveBuffers w)
+	// log(max(u, 1e-12)) of DIM_VOLUME_DISTANCE for (pixel, sample) at bounce b.
+	__device__ __forceinline__ float	pathVolumeLogU(const GpuParams& p, std::uint32_t x, std::uint32_t y, std::uint32_t sample, int b)
 	{
-		const std::uint32_t pixels = p.width * p.height;
+		PixelSampler ps;
+		ps.begin(p.renderSeed, x, y, sample, 0);
+		ps.bounce = static_cast<std::uint32_t>(b);
+		return volumeLogU(p, ps);
+	}
+
+	__global__ __launch_bounds__(kWaveBlock) void	wf_raygen(const PassJob* jobs, std::uint32_t job, WaveBuffers w)
+	{
+		const GpuParams& p = jobs[job].params;
+		const std::uint32_t pixels = jobs[job].pixels;
+		const std::uint32_t base = jobs[job].pathBase;
 		const std::uint32_t n = pixels * p.sampleCount;
 		for (std::uint32_t id = blockIdx.x * kWaveBlock + threadIdx.x; id < n; id += gridDim.x * kWaveBlock)
 		{
 			const std::uint32_t pixel = id % pixels, slot = id / pixels;
+			const std::uint32_t x = pixel % p.width, y = pixel / p.width;
 			PixelSampler ps;
-			ps.begin(p.renderSeed, pixel % p.width, pixel / p.width, p.sampleBegin + slot, 0);
-			const Ray r = cameraRay(p, pixel % p.width, pixel / p.width, ps);
-			w.qOrigin[0][id] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
-			w.qDir[0][id] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
-			w.qPath[0][id] = id;
-			w.throughput[id] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-			w.radiance[id] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(PF_PREV_SPECULAR));
+			ps.begin(p.renderSeed, x, y, p.sampleBegin + slot, 0);
+			const Ray r = cameraRay(p, x, y, ps);
+			const std::uint32_t g = base + id;
+			const float logU = p.hasVolumes ? volumeLogU(p, ps) : 0.0f;
+			w.qOrigin[0][g] = make_float4(r.o.x, r.o.y, r.o.z, logU);
+			w.qDir[0][g] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+			w.qPath[0][g] = (job << kPathBits) | g;
+			w.throughput[g] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+			w.radiance[g] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(PF_PREV_SPECULAR));
 		}
 	}
 
@@ -185,8 +242,8 @@ namespace luzgpu
 	// ended takes the next ray of the queue once at least `refillAt` lanes of its wave are
 	// idle (refillAt = 64: the wave refills only when all its lanes are done).
 	// Source supplies the queue: count(), load(slot, ray, tMax) and store(slot, state).
-	template <bool AnyHit, bool Count, typename Source>
-	__device__ __forceinline__ void	traceQueue(const SceneView& s, std::int32_t tlasRoot, Source& src, std::uint32_t* fetch,
+	template <bool AnyHit, bool Count, int Feat, typename Source>
+	__device__ __forceinline__ void	traceQueue(const SceneView& s, const PassJob* jobs, Source& src, std::uint32_t* fetch,
 		int refillAt, unsigned long long* counters, int counterSlot)
 	{
 		__shared__ std::uint32_t stackMem[kWaveLdsStack * kWaveBlock];
@@ -211,8 +268,9 @@ namespace luzgpu
 					slot = k;
 					Ray r;
 					float tMax, logU;
-					src.load(slot, r, tMax, logU);
-					active = traceBegin(st, s, tlasRoot, r, tMax, logU, stack);
+					const std::uint32_t job = src.load(slot, r, tMax, logU);
+					const GpuParams& jp = jobs[job].params;
+					active = traceBegin(st, s, jp.tlasRoot, unboundedOf(jp), r, tMax, logU, stack);
 					if (!active)
 					{
 						src.store(slot, st);
@@ -224,7 +282,7 @@ namespace luzgpu
 			{
 				break;
 			}
-			if (active && traceStep<AnyHit, Count>(s, st, kTMinF, stack, &visits))
+			if (active && traceStep<AnyHit, Count, Feat>(s, st, kTMinF, stack, &visits))
 			{
 				src.store(slot, st);
 				rays++;
@@ -234,31 +292,20 @@ namespace luzgpu
 		flushCounters(counters, counterSlot, rays, visits, Count);
 	}
 
-	// The volume free-flight sample of path `path` at bounce b (scenes with volumes only).
-	__device__ __forceinline__ float	pathVolumeLogU(const GpuParams& p, std::uint32_t path, int b)
-	{
-		if (!p.hasVolumes) return 0.0f;
-		const std::uint32_t pixels = p.width * p.height;
-		const std::uint32_t pixel = path % pixels, slot = path / pixels;
-		PixelSampler ps;
-		ps.begin(p.renderSeed, pixel % p.width, pixel / p.width, p.sampleBegin + slot, 0);
-		ps.bounce = static_cast<std::uint32_t>(b);
-		return volumeLogU(p, ps);
-	}
-
 	struct	ExtendQueue
 	{
 		const WaveBuffers&	w;
-		const GpuParams&	p;
 		int	b;
 		std::uint32_t*	spill;
 		__device__ __forceinline__ std::uint32_t	count() const { return w.counts[b]; }
-		__device__ __forceinline__ void	load(std::uint32_t i, Ray& r, float& tMax, float& logU) const
+		// returns the ray's scene job
+		__device__ __forceinline__ std::uint32_t	load(std::uint32_t i, Ray& r, float& tMax, float& logU) const
 		{
 			const float4 o = w.qOrigin[b & 1][i], d = w.qDir[b & 1][i];
 			r = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
 			tMax = INFINITY;
-			logU = p.hasVolumes ? pathVolumeLogU(p, w.qPath[b & 1][i], b) : 0.0f;
+			logU = o.w;
+			return w.qPath[b & 1][i] >> kPathBits;
 		}
 		__device__ __forceinline__ void	store(std::uint32_t i, const TraceState& st) const
 		{
@@ -270,18 +317,20 @@ namespace luzgpu
 	struct	ShadowQueue
 	{
 		const WaveBuffers&	w;
-		const GpuParams&	p;
 		int	b;
 		std::uint32_t*	spill;
+		std::uint32_t	volumes;	// the batch has volumes: shadow rays carry a sample
 		std::uint32_t	target;	// connection slot of the ray being traced
 		__device__ __forceinline__ std::uint32_t	count() const { return w.counts[kMaxBounceSlots + b]; }
-		__device__ __forceinline__ void	load(std::uint32_t i, Ray& r, float& tMax, float& logU)
+		__device__ __forceinline__ std::uint32_t	load(std::uint32_t i, Ray& r, float& tMax, float& logU)
 		{
 			const float4 o = w.sOrigin[i], d = w.sDir[i];
 			r = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
 			tMax = o.w;
-			target = __float_as_uint(d.w);
-			logU = p.hasVolumes ? pathVolumeLogU(p, target >> 1, b) : 0.0f;
+			const std::uint32_t tag = __float_as_uint(d.w);
+			target = tag & kTargetMask;
+			logU = volumes ? w.sLogU[i] : 0.0f;
+			return tag >> kTargetBits;
 		}
 		__device__ __forceinline__ void	store(std::uint32_t, const TraceState& st) const
 		{
@@ -290,22 +339,22 @@ namespace luzgpu
 	};
 
 	// closest hit of every ray of queue b
-	template <bool Count>
-	__global__ __launch_bounds__(kWaveBlock) void	wf_extend(SceneView s, GpuParams p, WaveBuffers w, int b,
+	template <bool Count, int Feat>
+	__global__ __launch_bounds__(kWaveBlock) void	wf_extend(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
 		unsigned long long* counters, int refillAt)
 	{
-		ExtendQueue src{w, p, b, w.spill};
-		traceQueue<false, Count>(s, p.tlasRoot, src, w.counts + 2 * kMaxBounceSlots + b, refillAt, counters, 0);
+		ExtendQueue src{w, b, w.spill};
+		traceQueue<false, Count, Feat>(s, jobs, src, w.counts + 2 * kMaxBounceSlots + b, refillAt, counters, 0);
 	}
 
-	__global__ __launch_bounds__(kWaveBlock) void	wf_shade(SceneView s, GpuParams p, WaveBuffers w, int b)
+	template <int Feat>
+	__global__ __launch_bounds__(kWaveBlock) void	wf_shade(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
+		std::uint32_t volumes)
 	{
 		const std::uint32_t n = w.counts[b];
 		const int q = b & 1;
-		const std::uint32_t pixels = p.width * p.height;
 		std::uint32_t* nextCount = w.counts + b + 1;
 		std::uint32_t* shadowCount = w.counts + kMaxBounceSlots + b;
-		const bool sampledInfinite = hasEnvLight(p) || hasConstantBackgroundLight(p);
 		// grid-stride loop with a wave-uniform trip count so every lane reaches the ballots
 		const std::uint32_t stride = gridDim.x * kWaveBlock;
 		const std::uint32_t first = blockIdx.x * kWaveBlock + (threadIdx.x & ~63u);
@@ -317,17 +366,32 @@ namespace luzgpu
 			Ray next{};
 			Connection cl{}, ce{};
 			f3 shadePoint = mk(0, 0, 0);
-			std::uint32_t path = 0;
-			if (valid)
+			const std::uint32_t qp = valid ? w.qPath[q][i] : 0u;
+			const std::uint32_t job = qp >> kPathBits;
+			const std::uint32_t path = qp & kPathMask;
+			float nextLogU = 0.0f, shadowLogU = 0.0f;
+			// one scene per iteration, so the scene's parameters are wave-uniform (scalar loads)
+			unsigned long long pending = __ballot(valid);
+			while (pending)
 			{
-				path = w.qPath[q][i];
-				const std::uint32_t pixel = path % pixels, slot = path / pixels;
+				const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+				const std::uint32_t cur = __builtin_amdgcn_readfirstlane(__shfl(job, leader));
+				const bool mine = valid && job == cur;
+				if (mine)
+				{
+				const PassJob& J = jobs[cur];
+				const GpuParams& p = J.params;
+				const bool sampledInfinite = hasEnvLight(p) || hasConstantBackgroundLight(p);
+				const std::uint32_t local = path - J.pathBase;
+				const std::uint32_t pixel = local % J.pixels, slot = local / J.pixels;
+				const std::uint32_t px = pixel % p.width, py = pixel / p.width;
 				PixelSampler ps;
-				ps.begin(p.renderSeed, pixel % p.width, pixel / p.width, p.sampleBegin + slot, 0);
+				ps.begin(p.renderSeed, px, py, p.sampleBegin + slot, 0);
 				ps.bounce = static_cast<std::uint32_t>(b);
 				const float4 hv = w.qHit[i];
 				const float4 o = w.qOrigin[q][i], d = w.qDir[q][i];
 				const Ray ray = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
+				shadowLogU = o.w;
 				float4 t4 = w.throughput[path];
 				float4 r4 = w.radiance[path];
 				f3 thr = mk(t4.x, t4.y, t4.z), acc = mk(r4.x, r4.y, r4.z);
@@ -355,15 +419,15 @@ namespace luzgpu
 					h.v = hv.z;
 					h.hittable = __float_as_uint(hv.w);
 					h.tri = w.qTri[i];
-					const Surface sf = surfaceAt(s, ray, h);
+					const Surface sf = surfaceAt<Feat>(s, ray, h);
 					shadePoint = sf.position;
 					const GpuMaterial& m = s.materials[sf.material];
 					const f3 emitted = m.kind == GPU_MAT_EMISSIVE ? ld3(m.color) : mk(0, 0, 0);
 					Scatter sc;
-					if (!scatter(m, sf, ray.d, ps, sc))
+					if (!scatter<Feat>(m, sf, ray.d, ps, sc))
 					{
 						const float4 po = w.prevOrigin[path];
-						const float lp = (!prevSpecular && p.lightCount > 0) ? lightsPdf(s, p, mk(po.x, po.y, po.z), ray.d) : 0.0f;
+						const float lp = (!prevSpecular && p.lightCount > 0) ? lightsPdf<Feat>(s, p, mk(po.x, po.y, po.z), ray.d) : 0.0f;
 						const float wgt = prevSpecular ? 1.0f : powerHeuristic(prevPdf, lp);
 						acc = acc + clampRay((thr * emitted) * wgt);
 					}
@@ -399,7 +463,7 @@ namespace luzgpu
 							{
 								if (p.lightCount > 0)
 								{
-									cl = connectLight(s, p, sf, m, sc, b, ps);
+									cl = connectLight<Feat>(s, p, sf, m, sc, b, ps);
 									if (cl.active)
 									{
 										cl.contribution = thr * cl.contribution;	// clampRay applied at resolve
@@ -409,7 +473,7 @@ namespace luzgpu
 								}
 								if (sampledInfinite)
 								{
-									ce = connectInfinite(s, p, sf, m, sc, b, ps);
+									ce = connectInfinite<Feat>(s, p, sf, m, sc, b, ps);
 									if (ce.active)
 									{
 										ce.contribution = thr * ce.contribution;
@@ -417,10 +481,10 @@ namespace luzgpu
 										flags |= PF_CONN_ENV;
 									}
 								}
-								const f3 dir = scatterDirection(sf, sc, ps);
+								const f3 dir = scatterDirection<Feat>(sf, sc, ps);
 								if (leavesOpaque(sf, m, dir))
 								{
-									const float pdf = scatterPdf(m, sf, sc, dir);
+									const float pdf = scatterPdf<Feat>(m, sf, sc, dir);
 									if (pdf > 0.0f && isfinite(pdf))
 									{
 										thr = clampRay(thr * att);
@@ -457,46 +521,56 @@ namespace luzgpu
 					w.connection[2 * path + 1] = make_float4(ce.contribution.x, ce.contribution.y, ce.contribution.z, 0.0f);
 					w.visible[2 * path + 1] = 0;
 				}
+				if (cont && p.hasVolumes)
+				{
+					nextLogU = pathVolumeLogU(p, px, py, p.sampleBegin + slot, b + 1);
+				}
+				}
+				pending &= ~__ballot(mine);
 			}
 			const std::uint32_t pos = waveAppend(cont, nextCount);
 			if (cont)
 			{
 				const int nq = (b + 1) & 1;
-				w.qOrigin[nq][pos] = make_float4(next.o.x, next.o.y, next.o.z, 0.0f);
+				w.qOrigin[nq][pos] = make_float4(next.o.x, next.o.y, next.o.z, nextLogU);
 				w.qDir[nq][pos] = make_float4(next.d.x, next.d.y, next.d.z, 0.0f);
-				w.qPath[nq][pos] = path;
+				w.qPath[nq][pos] = qp;
 			}
+			const std::uint32_t tagBase = job << kTargetBits;
 			const std::uint32_t sl = waveAppend(wantLight, shadowCount);
 			if (wantLight)
 			{
 				w.sOrigin[sl] = make_float4(shadePoint.x, shadePoint.y, shadePoint.z, cl.tMax);
-				w.sDir[sl] = make_float4(cl.direction.x, cl.direction.y, cl.direction.z, __uint_as_float(2u * path));
+				w.sDir[sl] = make_float4(cl.direction.x, cl.direction.y, cl.direction.z, __uint_as_float(tagBase | (2u * path)));
+				if (volumes) w.sLogU[sl] = shadowLogU;
 			}
 			const std::uint32_t se = waveAppend(wantEnv, shadowCount);
 			if (wantEnv)
 			{
 				w.sOrigin[se] = make_float4(shadePoint.x, shadePoint.y, shadePoint.z, INFINITY);
-				w.sDir[se] = make_float4(ce.direction.x, ce.direction.y, ce.direction.z, __uint_as_float(2u * path + 1u));
+				w.sDir[se] = make_float4(ce.direction.x, ce.direction.y, ce.direction.z, __uint_as_float(tagBase | (2u * path + 1u)));
+				if (volumes) w.sLogU[se] = shadowLogU;
 			}
 		}
 	}
 
 	// any-hit test of the shadow rays appended in bounce b
-	template <bool Count>
-	__global__ __launch_bounds__(kWaveBlock) void	wf_shadow(SceneView s, GpuParams p, WaveBuffers w, int b,
-		unsigned long long* counters, int refillAt)
+	template <bool Count, int Feat>
+	__global__ __launch_bounds__(kWaveBlock) void	wf_shadow(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
+		unsigned long long* counters, int refillAt, std::uint32_t volumes)
 	{
-		ShadowQueue src{w, p, b, w.spill, 0u};
-		traceQueue<true, Count>(s, p.tlasRoot, src, w.counts + 3 * kMaxBounceSlots + b, refillAt, counters, 1);
+		ShadowQueue src{w, b, w.spill, volumes, 0u};
+		traceQueue<true, Count, Feat>(s, jobs, src, w.counts + 3 * kMaxBounceSlots + b, refillAt, counters, 1);
 	}
 
-	__global__ __launch_bounds__(kWaveBlock) void	wf_resolve(GpuParams p, WaveBuffers w, int b, float* __restrict__ sampleColors)
+	// sampleColors is indexed by global path id
+	__global__ __launch_bounds__(kWaveBlock) void	wf_resolve(WaveBuffers w, int b, float* __restrict__ sampleColors)
 	{
 		const std::uint32_t n = w.counts[b];
 		const int q = b & 1;
 		for (std::uint32_t i = blockIdx.x * kWaveBlock + threadIdx.x; i < n; i += gridDim.x * kWaveBlock)
 		{
-			const std::uint32_t path = w.qPath[q][i];
+			const std::uint32_t path = w.qPath[q][i] & kPathMask;
 			float4 r4 = w.radiance[path];
 			std::uint32_t flags = __float_as_uint(r4.w);
 			f3 acc = mk(r4.x, r4.y, r4.z);
@@ -532,29 +606,121 @@ namespace luzgpu
 		}
 	}
 
-	// Device buffers for one context (grown on demand, kept between renders).
+	// Adds a pass's sample colours of one scene into its FP64 pixel sums, in sample order
+	// (the per-pixel sum of _threadRender, Threads.cpp:551-583).
+	__global__ void	wf_accumulate(const float* __restrict__ colors, const PassJob* jobs, std::uint32_t job)
+	{
+		const PassJob& J = jobs[job];
+		const std::uint32_t count = J.params.sampleCount;
+		for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < J.pixels; i += gridDim.x * blockDim.x)
+		{
+			double r = J.sums[3 * static_cast<size_t>(i)], g = J.sums[3 * static_cast<size_t>(i) + 1], b = J.sums[3 * static_cast<size_t>(i) + 2];
+			for (std::uint32_t k = 0; k < count; k++)
+			{
+				const float* c = colors + (static_cast<size_t>(J.pathBase) + static_cast<size_t>(k) * J.pixels + i) * 3;
+				r += static_cast<double>(c[0]);
+				g += static_cast<double>(c[1]);
+				b += static_cast<double>(c[2]);
+			}
+			J.sums[3 * static_cast<size_t>(i)] = r;
+			J.sums[3 * static_cast<size_t>(i) + 1] = g;
+			J.sums[3 * static_cast<size_t>(i) + 2] = b;
+		}
+	}
+
+	// Launch shape of the traversal stages: persistent blocks per launch and the number of
+	// idle lanes at which a wave takes new rays (1..64).
+	struct	WaveTuning
+	{
+		unsigned	grid = 0;
+		int	extendRefill = 0;
+		int	shadowRefill = 0;
+
+		WaveTuning	resolved() const
+		{
+			WaveTuning t = *this;
+			if (t.grid == 0) t.grid = kWaveGrid;
+			if (t.extendRefill == 0) t.extendRefill = kDefaultRefill;
+			if (t.shadowRefill == 0) t.shadowRefill = kDefaultRefill;
+			if (t.grid > kWaveGrid || t.extendRefill < 1 || t.extendRefill > 64 || t.shadowRefill < 1 || t.shadowRefill > 64)
+			{
+				throw std::invalid_argument("wavefront tuning out of range: grid 1.." + std::to_string(kWaveGrid)
+					+ ", refill 1..64");
+			}
+			return t;
+		}
+		bool	operator==(const WaveTuning& o) const
+		{
+			return grid == o.grid && extendRefill == o.extendRefill && shadowRefill == o.shadowRefill;
+		}
+	};
+
+	struct	StageTimes
+	{
+		double	extendMs = 0.0, shadeMs = 0.0, shadowMs = 0.0;
+		std::uint32_t	extendLaunches = 0;
+		std::uint32_t	passes = 0;
+		std::uint32_t	graphReplays = 0;
+	};
+
+	// One scene of a render call: its parameters (sampleBegin / sampleCount: the whole
+	// range of this call) and its FP64 pixel sums.
+	struct	BatchJob
+	{
+		GpuParams	params;
+		double*	sums;
+	};
+
+	// Device state of one context (grown on demand, kept between renders), and the
+	// captured pass graph with what it was captured for.
 	struct	WavefrontState
 	{
 		void*	mem = nullptr;
 		size_t	bytes = 0;
 		WaveBuffers	buf{};
+		float*	colors = nullptr;		// [global path][3]
+		PassJob*	jobs = nullptr;		// the current pass's table
+		PassJob*	plans = nullptr;	// every pass's table of a render
+		size_t	planCapacity = 0;
+		hipGraphExec_t	graph = nullptr;
+		std::vector<unsigned char>	graphKey;
+
+		void	dropGraph()
+		{
+			if (graph) (void)hipGraphExecDestroy(graph);
+			graph = nullptr;
+			graphKey.clear();
+		}
 
 		void	release()
 		{
+			dropGraph();
 			if (mem) (void)hipFree(mem);
+			if (plans) (void)hipFree(plans);
 			mem = nullptr;
+			plans = nullptr;
+			planCapacity = 0;
 			bytes = 0;
 		}
 
+		// The layout is fixed between reallocations (captured graphs hold its pointers):
+		// a smaller request keeps the larger layout.
 		void	ensure(std::uint32_t capacity)
 		{
+			if (mem && capacity <= buf.capacity)
+			{
+				return;
+			}
 			const size_t n = capacity;
 			const size_t spillBytes = static_cast<size_t>(kWaveStackCapacity - kWaveLdsStack) * kWaveGrid * kWaveBlock * 4;
-			const size_t need = n * (16 * 2 * 2 + 4 * 2 + 16 + 4 + 16 * 3 + 32 + 2 + 32 * 2) + kMaxBounceSlots * 4 * 4 + 64 * 16
-				+ spillBytes + 256;
+			const size_t need = n * (16 * 2 * 2 + 4 * 2 + 16 + 4 + 16 * 3 + 32 + 2 + 32 * 2 + 8 + 12) + kMaxBounceSlots * 4 * 4
+				+ sizeof(PassJob) * kMaxBatchScenes + 64 * 20 + spillBytes + 256;
 			if (need > bytes)
 			{
-				release();
+				dropGraph();
+				if (mem) (void)hipFree(mem);
+				mem = nullptr;
+				bytes = 0;
 				if (hipMalloc(&mem, need) != hipSuccess)
 				{
 					mem = nullptr;
@@ -579,101 +745,212 @@ namespace luzgpu
 			buf.visible = reinterpret_cast<std::uint8_t*>(take(2 * n));
 			buf.sOrigin = reinterpret_cast<float4*>(take(32 * n));
 			buf.sDir = reinterpret_cast<float4*>(take(32 * n));
+			buf.sLogU = reinterpret_cast<float*>(take(8 * n));
+			colors = reinterpret_cast<float*>(take(12 * n));
 			buf.counts = reinterpret_cast<std::uint32_t*>(take(kMaxBounceSlots * 4 * 4));
+			jobs = reinterpret_cast<PassJob*>(take(sizeof(PassJob) * kMaxBatchScenes));
 			buf.spill = reinterpret_cast<std::uint32_t*>(take(spillBytes));
 			buf.capacity = capacity;
 		}
-	};
 
-	// Launch shape of the traversal stages: persistent blocks per launch and the number of
-	// idle lanes at which a wave takes new rays (1..64).
-	struct	WaveTuning
-	{
-		unsigned	grid = 0;
-		int	extendRefill = 0;
-		int	shadowRefill = 0;
-
-		WaveTuning	resolved() const
+		void	ensurePlans(size_t entries)
 		{
-			WaveTuning t = *this;
-			if (t.grid == 0) t.grid = kWaveGrid;
-			if (t.extendRefill == 0) t.extendRefill = kDefaultRefill;
-			if (t.shadowRefill == 0) t.shadowRefill = kDefaultRefill;
-			if (t.grid > kWaveGrid || t.extendRefill < 1 || t.extendRefill > 64 || t.shadowRefill < 1 || t.shadowRefill > 64)
+			if (entries <= planCapacity) return;
+			if (plans) (void)hipFree(plans);
+			plans = nullptr;
+			planCapacity = 0;
+			if (hipMalloc(reinterpret_cast<void**>(&plans), entries * sizeof(PassJob)) != hipSuccess)
 			{
-				throw std::invalid_argument("wavefront tuning out of range: grid 1.." + std::to_string(kWaveGrid)
-					+ ", refill 1..64");
+				plans = nullptr;
+				throw std::runtime_error("wavefront pass-table allocation failed");
 			}
-			return t;
+			planCapacity = entries;
 		}
 	};
 
-	struct	StageTimes
+	inline void	checkHip(hipError_t e, const char* what)
 	{
-		double	extendMs = 0.0, shadeMs = 0.0, shadowMs = 0.0;
-		std::uint32_t	extendLaunches = 0;
-	};
+		if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+	}
 
-	// Enqueues one launch of the wavefront integrator. With `events`, each stage is
-	// bracketed by HIP events (on this stream) and the per-stage times are returned.
-	inline void	renderWavefront(WavefrontState& st, const SceneView& s, const GpuParams& p, float* sampleColors,
+	// Enqueues the launches of one pass (the table is already in st.jobs).
+	template <int Feat>
+	inline void	enqueuePassF(WavefrontState& st, const SceneView& s, std::uint32_t jobCount, int maxBounces, bool volumes,
+		unsigned long long* counters, hipStream_t stream, bool count, WaveTuning tune, std::vector<hipEvent_t>* marks)
+	{
+		const WaveBuffers& w = st.buf;
+		auto mark = [&]() {
+			if (!marks) return;
+			hipEvent_t e;
+			checkHip(hipEventCreate(&e), "hipEventCreate");
+			checkHip(hipEventRecord(e, stream), "hipEventRecord");
+			marks->push_back(e);
+		};
+		const std::uint32_t vol = volumes ? 1u : 0u;
+		wf_zero_counts<<<1, 4 * kMaxBounceSlots, 0, stream>>>(w.counts, 4 * kMaxBounceSlots, st.jobs, jobCount);
+		for (std::uint32_t j = 0; j < jobCount; j++)
+		{
+			wf_raygen<<<kWaveGrid, kWaveBlock, 0, stream>>>(st.jobs, j, w);
+		}
+		for (int b = 0; b <= maxBounces; b++)
+		{
+			mark();
+			if (count) wf_extend<true, Feat><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill);
+			else wf_extend<false, Feat><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill);
+			mark();
+			wf_shade<Feat><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, vol);
+			mark();
+			if (count) wf_shadow<true, Feat><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, vol);
+			else wf_shadow<false, Feat><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, vol);
+			mark();
+			wf_resolve<<<kWaveGrid, kWaveBlock, 0, stream>>>(w, b, st.colors);
+		}
+		for (std::uint32_t j = 0; j < jobCount; j++)
+		{
+			wf_accumulate<<<1024, 256, 0, stream>>>(st.colors, st.jobs, j);
+		}
+	}
+
+	inline void	enqueuePass(int features, WavefrontState& st, const SceneView& s, std::uint32_t jobCount, int maxBounces, bool volumes,
+		unsigned long long* counters, hipStream_t stream, bool count, WaveTuning tune, std::vector<hipEvent_t>* marks)
+	{
+		switch (features & kFeatAll)
+		{
+			case 0: enqueuePassF<0>(st, s, jobCount, maxBounces, volumes, counters, stream, count, tune, marks); break;
+			case kFeatAnalytic: enqueuePassF<kFeatAnalytic>(st, s, jobCount, maxBounces, volumes, counters, stream, count, tune, marks); break;
+			case kFeatRich: enqueuePassF<kFeatRich>(st, s, jobCount, maxBounces, volumes, counters, stream, count, tune, marks); break;
+			default: enqueuePassF<kFeatAll>(st, s, jobCount, maxBounces, volumes, counters, stream, count, tune, marks); break;
+		}
+	}
+
+	// Renders every job's sample range into its sums: passes of at most kMaxPassPaths paths
+	// over all scenes at once. Without stage timing or visit counting, one pass is captured
+	// as a hipGraph (its varying inputs are all in device memory) and replayed for every pass
+	// and every later render with the same shape.
+	inline void	renderWavefront(WavefrontState& st, const SceneView& s, int features, const std::vector<BatchJob>& jobs,
 		unsigned long long* counters, hipStream_t stream, bool count, StageTimes* times, WaveTuning tune)
 	{
 		tune = tune.resolved();
-		if (p.maxBounces + 2 > kMaxBounceSlots)
+		if (jobs.empty() || jobs.size() > kMaxBatchScenes)
 		{
-			throw std::runtime_error("Unsupported: more than 62 bounces in the wavefront integrator");
+			throw std::invalid_argument("wavefront: 1.." + std::to_string(kMaxBatchScenes) + " scenes per launch");
 		}
-		// passes of at most ~16M paths keep the SoA state near 4 GB
-		const std::uint32_t pixels = p.width * p.height;
-		const std::uint32_t perPass = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount, (16u << 20) / pixels));
-		st.ensure(pixels * perPass);
-		const WaveBuffers& w = st.buf;
-		std::vector<hipEvent_t> ev;
-		auto mark = [&]() {
-			if (!times) return;
-			hipEvent_t e;
-			if (hipEventCreate(&e) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
-			if (hipEventRecord(e, stream) != hipSuccess) throw std::runtime_error("hipEventRecord failed");
-			ev.push_back(e);
-		};
-		for (std::uint32_t s0 = 0; s0 < p.sampleCount; s0 += perPass)
+		int maxBounces = 0;
+		bool volumes = false;
+		std::uint64_t total = 0;
+		for (const BatchJob& j : jobs)
 		{
-			GpuParams q = p;
-			q.sampleBegin = p.sampleBegin + s0;
-			q.sampleCount = std::min(perPass, p.sampleCount - s0);
-			float* colors = sampleColors + static_cast<size_t>(s0) * pixels * 3;
-			wf_zero_counts<<<1, 4 * kMaxBounceSlots, 0, stream>>>(w.counts, 4 * kMaxBounceSlots, pixels * q.sampleCount);
-			wf_raygen<<<kWaveGrid, kWaveBlock, 0, stream>>>(q, w);
-			for (int b = 0; b <= q.maxBounces; b++)
+			if (j.params.maxBounces + 2 > kMaxBounceSlots)
 			{
-				mark();
-				if (count) wf_extend<true><<<tune.grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters, tune.extendRefill);
-				else wf_extend<false><<<tune.grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters, tune.extendRefill);
-				mark();
-				wf_shade<<<kWaveGrid, kWaveBlock, 0, stream>>>(s, q, w, b);
-				mark();
-				if (count) wf_shadow<true><<<tune.grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters, tune.shadowRefill);
-				else wf_shadow<false><<<tune.grid, kWaveBlock, 0, stream>>>(s, q, w, b, counters, tune.shadowRefill);
-				mark();
-				wf_resolve<<<kWaveGrid, kWaveBlock, 0, stream>>>(q, w, b, colors);
+				throw std::runtime_error("Unsupported: more than 62 bounces in the wavefront integrator");
+			}
+			const std::uint64_t pixels = static_cast<std::uint64_t>(j.params.width) * j.params.height;
+			if (pixels == 0 || pixels > kMaxPassPaths)
+			{
+				throw std::runtime_error("Unsupported: image of more than 16M pixels in the wavefront integrator");
+			}
+			maxBounces = std::max(maxBounces, j.params.maxBounces);
+			volumes = volumes || j.params.hasVolumes;
+			total += pixels * j.params.sampleCount;
+		}
+		const std::uint32_t capacity = static_cast<std::uint32_t>(std::min<std::uint64_t>(kMaxPassPaths, std::max<std::uint64_t>(total, 1)));
+		const std::uint32_t n = static_cast<std::uint32_t>(jobs.size());
+		// pass plan: each pass gives every scene with samples left as many as the budget holds
+		std::vector<PassJob> plan;
+		std::vector<std::uint32_t> done(n, 0);
+		for (;;)
+		{
+			bool any = false;
+			for (std::uint32_t j = 0; j < n; j++) any = any || done[j] < jobs[j].params.sampleCount;
+			if (!any) break;
+			std::uint32_t budget = capacity, base = 0;
+			for (std::uint32_t j = 0; j < n; j++)
+			{
+				const GpuParams& p = jobs[j].params;
+				const std::uint32_t pixels = p.width * p.height;
+				const std::uint32_t c = std::min(p.sampleCount - done[j], budget / pixels);
+				PassJob pj{};
+				pj.params = p;
+				pj.params.sampleBegin = p.sampleBegin + done[j];
+				pj.params.sampleCount = c;
+				pj.pathBase = base;
+				pj.pixels = pixels;
+				pj.sums = jobs[j].sums;
+				plan.push_back(pj);
+				done[j] += c;
+				base += pixels * c;
+				budget -= pixels * c;
+			}
+		}
+		const std::uint32_t passes = static_cast<std::uint32_t>(plan.size() / n);
+		st.ensure(capacity);
+		st.ensurePlans(plan.size());
+		// the host table dies with this call: copy it before returning (ordered on the stream)
+		checkHip(hipMemcpyAsync(st.plans, plan.data(), plan.size() * sizeof(PassJob), hipMemcpyHostToDevice, stream), "pass tables");
+		checkHip(hipStreamSynchronize(stream), "pass tables");
+		const bool useGraph = !count && !times;
+		if (useGraph)
+		{
+			// what the captured launches depend on besides device memory
+			std::vector<unsigned char> key(sizeof(SceneView) + sizeof(WaveTuning) + 4 * sizeof(std::uint32_t) + sizeof(void*) * 2);
+			unsigned char* k = key.data();
+			std::memcpy(k, &s, sizeof(SceneView)); k += sizeof(SceneView);
+			std::memcpy(k, &tune, sizeof(WaveTuning)); k += sizeof(WaveTuning);
+			const std::uint32_t shape[4] = {n, static_cast<std::uint32_t>(maxBounces), volumes ? 1u : 0u,
+				static_cast<std::uint32_t>(features)};
+			std::memcpy(k, shape, sizeof(shape)); k += sizeof(shape);
+			void* ptrs[2] = {st.mem, counters};
+			std::memcpy(k, ptrs, sizeof(ptrs));
+			// the buffer layout the launches were captured with
+			const unsigned char* lb = reinterpret_cast<const unsigned char*>(&st.buf);
+			key.insert(key.end(), lb, lb + sizeof(WaveBuffers));
+			const void* more[2] = {st.jobs, st.colors};
+			const unsigned char* mb = reinterpret_cast<const unsigned char*>(more);
+			key.insert(key.end(), mb, mb + sizeof(more));
+			if (!st.graph || st.graphKey != key)
+			{
+				st.dropGraph();
+				hipGraph_t g = nullptr;
+				checkHip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+				enqueuePass(features, st, s, n, maxBounces, volumes, counters, stream, false, tune, nullptr);
+				checkHip(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+				const hipError_t e = hipGraphInstantiate(&st.graph, g, nullptr, nullptr, 0);
+				(void)hipGraphDestroy(g);
+				checkHip(e, "hipGraphInstantiate");
+				st.graphKey = key;
+			}
+		}
+		std::vector<hipEvent_t> marks;
+		for (std::uint32_t k = 0; k < passes; k++)
+		{
+			checkHip(hipMemcpyAsync(st.jobs, st.plans + static_cast<size_t>(k) * n, n * sizeof(PassJob), hipMemcpyDeviceToDevice,
+				stream), "pass table");
+			if (useGraph)
+			{
+				checkHip(hipGraphLaunch(st.graph, stream), "hipGraphLaunch");
+				if (times) times->graphReplays++;
+			}
+			else
+			{
+				enqueuePass(features, st, s, n, maxBounces, volumes, counters, stream, count, tune, times ? &marks : nullptr);
 			}
 		}
 		if (times)
 		{
-			if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("wavefront stage timing failed");
-			for (size_t k = 0; k + 3 < ev.size(); k += 4)
+			times->passes += passes;
+			checkHip(hipStreamSynchronize(stream), "wavefront stage timing");
+			for (size_t k = 0; k + 3 < marks.size(); k += 4)
 			{
 				float a = 0, b2 = 0, c = 0;
-				(void)hipEventElapsedTime(&a, ev[k], ev[k + 1]);
-				(void)hipEventElapsedTime(&b2, ev[k + 1], ev[k + 2]);
-				(void)hipEventElapsedTime(&c, ev[k + 2], ev[k + 3]);
+				(void)hipEventElapsedTime(&a, marks[k], marks[k + 1]);
+				(void)hipEventElapsedTime(&b2, marks[k + 1], marks[k + 2]);
+				(void)hipEventElapsedTime(&c, marks[k + 2], marks[k + 3]);
 				times->extendMs += a;
 				times->shadeMs += b2;
 				times->shadowMs += c;
 				times->extendLaunches++;
 			}
-			for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+			for (hipEvent_t e : marks) (void)hipEventDestroy(e);
 		}
 	}
 }

@@ -1376,6 +1376,139 @@ namespace luzgpu
 		gp.metersPerUnit = static_cast<float>(s.metersPerUnit);
 		gp.unboundedCount = static_cast<std::uint32_t>(p.unbounded.size());
 		gp.hasVolumes = p.volumes.empty() ? 0u : 1u;
+		p.features = 0;
+		for (const Hittable& h : scene.hittables)
+		{
+			if (h.kind == HIT_SPHERE || h.kind == HIT_CUBE || h.kind == HIT_VOLUME || h.kind == HIT_PLANE || h.kind == HIT_DIRECTIONAL)
+			{
+				p.features |= kFeatAnalytic;
+			}
+		}
+		for (const Material& m : scene.materials)
+		{
+			if (m.kind == MAT_GLOSSY || m.kind == MAT_DIFFUSE_GLOSSY || m.kind == MAT_PRINCIPLED || m.kind == MAT_ISOTROPIC || m.kind == MAT_HG)
+			{
+				p.features |= kFeatRich;
+			}
+		}
+		gp.lightWeighted = p.lightCdf.empty() ? 0u : 1u;
+		gp.lightBase = 0;
+		gp.envBase = 0;
+		gp.unboundedBase = 0;
+		if (p.unbounded.size() > 0xffffu)
+		{
+			throw std::runtime_error("Unsupported for the device path: more than 65535 planes");
+		}
 		return (p);
+	}
+}
+
+namespace luzgpu
+{
+	// Concatenates packed scenes into one set of device arrays (a multi-scene launch), and
+	// returns each scene's parameters with its TLAS root and table bases in the merged arrays.
+	PackedScene	mergeScenes(const std::vector<const PackedScene*>& parts, std::vector<GpuParams>& params)
+	{
+		PackedScene m;
+		params.clear();
+		for (const PackedScene* part : parts)
+		{
+			const PackedScene& p = *part;
+			const std::uint32_t tlasBase = static_cast<std::uint32_t>(m.tlasWide.size());
+			const std::uint32_t blasBase = static_cast<std::uint32_t>(m.blasWide.size());
+			const std::uint32_t meshTriBase = static_cast<std::uint32_t>(m.meshTris.size());
+			const std::uint32_t normalSlotBase = static_cast<std::uint32_t>(m.meshNormals.size() / 9);
+			const std::uint32_t leafBase = static_cast<std::uint32_t>(m.tlasLeaf.size() / 2);
+			const std::uint32_t hittableBase = static_cast<std::uint32_t>(m.hittables.size());
+			const std::uint32_t materialBase = static_cast<std::uint32_t>(m.materials.size());
+			const std::uint32_t base[8] = {
+				static_cast<std::uint32_t>(m.topTris.size()), static_cast<std::uint32_t>(m.rects.size()),
+				static_cast<std::uint32_t>(m.instances.size()), static_cast<std::uint32_t>(m.spheres.size()),
+				static_cast<std::uint32_t>(m.cubeFaces.size() / 6), static_cast<std::uint32_t>(m.volumes.size()),
+				static_cast<std::uint32_t>(m.planes.size()), 0u};
+			auto rebaseKind = [&](std::uint32_t kindIndex) {
+				const std::uint32_t kind = kindIndex >> kKindShift;
+				const std::uint32_t idx = (kindIndex & kIndexMask) + base[kind];
+				if (idx > kIndexMask) throw std::runtime_error("Unsupported: batch too large for 28-bit primitive indices");
+				return (kind << kKindShift) | idx;
+			};
+			for (Bvh4Node n : p.tlasWide)
+			{
+				for (int k = 0; k < 4; k++)
+				{
+					if (n.count[k] == 0) n.child[k] += static_cast<std::int32_t>(tlasBase);
+					else if (n.count[k] != 255) n.child[k] += static_cast<std::int32_t>(leafBase);
+				}
+				m.tlasWide.push_back(n);
+			}
+			for (Bvh4Node n : p.blasWide)
+			{
+				for (int k = 0; k < 4; k++)
+				{
+					if (n.count[k] == 0) n.child[k] += static_cast<std::int32_t>(blasBase);
+					else if (n.count[k] != 255) n.child[k] += static_cast<std::int32_t>(meshTriBase);
+				}
+				m.blasWide.push_back(n);
+			}
+			m.meshTris.insert(m.meshTris.end(), p.meshTris.begin(), p.meshTris.end());
+			m.meshNormals.insert(m.meshNormals.end(), p.meshNormals.begin(), p.meshNormals.end());
+			for (std::size_t i = 0; i + 1 < p.tlasLeaf.size(); i += 2)
+			{
+				m.tlasLeaf.push_back(p.tlasLeaf[i] + hittableBase);
+				m.tlasLeaf.push_back(rebaseKind(p.tlasLeaf[i + 1]));
+			}
+			for (GpuHittable h : p.hittables)
+			{
+				h.kindIndex = rebaseKind(h.kindIndex);
+				h.material += materialBase;
+				m.hittables.push_back(h);
+			}
+			for (GpuTri t : p.topTris)
+			{
+				t.prim += static_cast<std::int32_t>(hittableBase);
+				m.topTris.push_back(t);
+			}
+			m.rects.insert(m.rects.end(), p.rects.begin(), p.rects.end());
+			for (GpuInstance in : p.instances)
+			{
+				in.blasRoot += static_cast<std::int32_t>(blasBase);
+				in.triBase += static_cast<std::int32_t>(meshTriBase);
+				if (in.normalBase >= 0) in.normalBase += static_cast<std::int32_t>(normalSlotBase);
+				m.instances.push_back(in);
+			}
+			m.spheres.insert(m.spheres.end(), p.spheres.begin(), p.spheres.end());
+			m.cubeFaces.insert(m.cubeFaces.end(), p.cubeFaces.begin(), p.cubeFaces.end());
+			m.volumes.insert(m.volumes.end(), p.volumes.begin(), p.volumes.end());
+			m.planes.insert(m.planes.end(), p.planes.begin(), p.planes.end());
+			m.materials.insert(m.materials.end(), p.materials.begin(), p.materials.end());
+			GpuParams gp = p.params;
+			gp.lightBase = static_cast<std::uint32_t>(m.lights.size());
+			for (GpuLight l : p.lights)
+			{
+				if (l.kind == GPU_HIT_TRIANGLE || l.kind == GPU_HIT_RECTANGLE || l.kind == GPU_HIT_SPHERE) l.index += base[l.kind];
+				l.hittable += hittableBase;
+				l.material += materialBase;
+				m.lights.push_back(l);
+			}
+			// lightCdf is indexed like the lights: pad an unweighted scene's span
+			if (p.lightCdf.empty()) m.lightCdf.insert(m.lightCdf.end(), p.lights.size(), 0.0);
+			else m.lightCdf.insert(m.lightCdf.end(), p.lightCdf.begin(), p.lightCdf.end());
+			gp.envBase = static_cast<std::uint32_t>(m.envPdf.size());
+			m.envPixels.insert(m.envPixels.end(), p.envPixels.begin(), p.envPixels.end());
+			m.envPdf.insert(m.envPdf.end(), p.envPdf.begin(), p.envPdf.end());
+			// envCdf is empty when the map has no weight: keep it texel-aligned with envPdf
+			if (p.envCdf.empty()) m.envCdf.insert(m.envCdf.end(), p.envPdf.size(), 0.0);
+			else m.envCdf.insert(m.envCdf.end(), p.envCdf.begin(), p.envCdf.end());
+			gp.unboundedBase = static_cast<std::uint32_t>(m.unbounded.size());
+			for (std::uint32_t u : p.unbounded) m.unbounded.push_back(u + hittableBase);
+			if (m.unbounded.size() > 0xffffu) throw std::runtime_error("Unsupported: more than 65535 planes in a batch");
+			gp.tlasRoot = p.params.tlasRoot >= 0 ? p.params.tlasRoot + static_cast<std::int32_t>(tlasBase) : -1;
+			params.push_back(gp);
+			m.features |= p.features;
+			m.tlasStackBound = std::max(m.tlasStackBound, p.tlasStackBound);
+			m.blasStackBound = std::max(m.blasStackBound, p.blasStackBound);
+			m.triangleCount += p.triangleCount;
+		}
+		return (m);
 	}
 }

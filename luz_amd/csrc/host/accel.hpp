@@ -66,6 +66,7 @@ namespace luzgpu
 		int	blasStackBound = 0;
 		Camera	camera;
 		double	metersPerUnit = 1.0;
+		int	features = 0;	// kFeat* classes present (kernel specialisation)
 	};
 
 	// RenderCamera for an image of width x height (RayHelper.cpp:50-82).
@@ -75,4 +76,8 @@ namespace luzgpu
 	// Mesh.cpp:217-521), repacks them into the 64-byte device layout and derives
 	// light and environment sampling tables. Throws on unsupported content.
 	PackedScene	packScene(const Scene& scene);
+
+	// Concatenates packed scenes into one set of device arrays for a multi-scene launch;
+	// params receives each scene's parameters with its root and table bases.
+	PackedScene	mergeScenes(const std::vector<const PackedScene*>& parts, std::vector<GpuParams>& params);
 }

@@ -15,6 +15,7 @@
 #include "device/wavefront.hpp"
 #include "host/accel.hpp"
 
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -92,7 +93,7 @@ namespace luzgpu
 	constexpr int	kStackDepth = 48;
 
 	// counters: [0] closest-hit rays, [1] shadow rays, [2] nodes visited, [3] primitives tested
-	template <bool Count>
+	template <bool Count, int Feat>
 	__global__ __launch_bounds__(kMegaBlock) void	render_mega(SceneView s, GpuParams p, float* __restrict__ sampleColors,
 		unsigned long long* __restrict__ counters)
 	{
@@ -106,7 +107,7 @@ namespace luzgpu
 			const std::uint32_t slot = static_cast<std::uint32_t>(gid / pixels);
 			const std::uint32_t pixel = static_cast<std::uint32_t>(gid % pixels);
 			LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
-			const f3 c = cleanColor(tracePath<Count>(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest, shadow, visits));
+			const f3 c = cleanColor(tracePath<Count, Feat>(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest, shadow, visits));
 			float* out = sampleColors + (static_cast<size_t>(slot) * pixels + pixel) * 3;
 			out[0] = c.x;
 			out[1] = c.y;
@@ -136,6 +137,26 @@ namespace luzgpu
 	}
 
 	// Adds this launch's samples into the FP64 pixel sums, in sample order.
+	template <bool Count>
+	void	launchMegaC(int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, float* colors,
+		unsigned long long* counters)
+	{
+		switch (features & kFeatAll)
+		{
+			case 0: render_mega<Count, 0><<<grid, kMegaBlock, 0, st>>>(s, p, colors, counters); break;
+			case kFeatAnalytic: render_mega<Count, kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, p, colors, counters); break;
+			case kFeatRich: render_mega<Count, kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, p, colors, counters); break;
+			default: render_mega<Count, kFeatAll><<<grid, kMegaBlock, 0, st>>>(s, p, colors, counters); break;
+		}
+	}
+
+	void	launchMega(bool count, int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, float* colors,
+		unsigned long long* counters)
+	{
+		if (count) launchMegaC<true>(features, grid, st, s, p, colors, counters);
+		else launchMegaC<false>(features, grid, st, s, p, colors, counters);
+	}
+
 	__global__ void	accumulate(const float* __restrict__ sampleColors, double* __restrict__ sums, std::uint32_t pixels, std::uint32_t count)
 	{
 		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -187,7 +208,7 @@ namespace luzgpu
 		const Ray ray = cameraRay(p, i % p.width, i / p.width, ps);
 		LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
 		HitInfo h;
-		if (!traverse<false>(s, p.tlasRoot, ray, kTMinF, INFINITY, h, stack, nullptr, volumeLogU(p, ps)))
+		if (!traverse<false>(s, p, ray, kTMinF, INFINITY, h, stack, nullptr, volumeLogU(p, ps)))
 		{
 			ids[2 * i] = -1;
 			ids[2 * i + 1] = -1;
@@ -234,12 +255,18 @@ struct luzgpu_ctx
 	luzgpu::DeviceBuffer	output;
 	luzgpu::DeviceBuffer	scratch;
 	luzgpu::WavefrontState	wave;
+	// merged arrays of the last multi-scene batch, keyed by the scenes' serials
+	std::vector<std::uint64_t>	batchKey;
+	std::unique_ptr<luzgpu::GpuScene>	batch;
+	std::vector<luzgpu::GpuParams>	batchParams;
+	luzgpu::DeviceBuffer	batchSums;
 };
 
 struct luzgpu_scene
 {
 	luzgpu_ctx*	ctx = nullptr;
 	luzgpu::GpuScene	gpu;
+	std::uint64_t	serial = 0;		// identity for the context's merged-batch cache
 };
 
 namespace luzgpu
@@ -320,16 +347,17 @@ namespace luzgpu
 			v.envPdf = g.envPdf.as<float>();
 			v.envCdf = g.envCdf.as<double>();
 			v.unbounded = g.unbounded.as<std::uint32_t>();
-			v.unboundedCount = static_cast<std::uint32_t>(h.unbounded.size());
 			v.spheres = g.spheres.as<GpuSphere>();
 			v.cubeFaces = g.cubeFaces.as<GpuRect>();
 			v.volumes = g.volumes.as<GpuVolume>();
 			v.planes = g.planes.as<GpuPlane>();
 		}
 
-		GpuParams	resolveParams(const GpuScene& g, const luzgpu_settings* s)
+		// The render parameters of one call: the scene's (base) parameters with the caller's
+		// settings applied (FlagsParser overrides, FlagsParser.cpp:87-117).
+		GpuParams	resolveParams(const GpuParams& base, const PackedScene& host, const luzgpu_settings* s)
 		{
-			GpuParams p = g.host.params;
+			GpuParams p = base;
 			if (s)
 			{
 				p.renderSeed = s->seed;
@@ -342,7 +370,7 @@ namespace luzgpu
 				{
 					p.width = s->width > 0 ? static_cast<std::uint32_t>(s->width) : p.width;
 					p.height = s->height > 0 ? static_cast<std::uint32_t>(s->height) : p.height;
-					p.camera = makeCamera(g.host.camera, p.width, p.height, g.host.metersPerUnit);
+					p.camera = makeCamera(host.camera, p.width, p.height, host.metersPerUnit);
 				}
 			}
 			else
@@ -352,6 +380,11 @@ namespace luzgpu
 				p.sampleCount = p.totalSamples;
 			}
 			return p;
+		}
+
+		GpuParams	resolveParams(const GpuScene& g, const luzgpu_settings* s)
+		{
+			return resolveParams(g.host.params, g.host, s);
 		}
 
 		unsigned	blocks(size_t n, unsigned b) { return static_cast<unsigned>((n + b - 1) / b); }
@@ -396,6 +429,8 @@ void	luzgpu_destroy(luzgpu_ctx* ctx)
 	(void)hipSetDevice(ctx->device);
 	(void)hipStreamSynchronize(ctx->stream);
 	ctx->wave.release();
+	ctx->batch.reset();
+	ctx->batchSums.release();
 	ctx->sampleColors.release();
 	ctx->sums.release();
 	ctx->counters.release();
@@ -420,6 +455,8 @@ int	luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out)
 		}
 		auto sc = std::make_unique<luzgpu_scene>();
 		sc->ctx = ctx;
+		static std::atomic<std::uint64_t> serials{0};
+		sc->serial = ++serials;
 		const Scene scene = readSceneFile(path);
 		sc->gpu.host = packScene(scene);
 		const PackedScene& h = sc->gpu.host;
@@ -517,48 +554,48 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		ctx->counters.reserve(8 * sizeof(unsigned long long));
 		zero_doubles<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(ctx->sums.as<double>(), static_cast<size_t>(pixels) * 3);
 		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
-		// samples per launch: bounded so the per-sample colour buffer stays within 4 GiB
-		const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount,
-			static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(pixels) * 12ull))));
-		ctx->sampleColors.reserve(static_cast<size_t>(pixels) * perLaunch * 3 * sizeof(float));
 		hipEvent_t e0, e1;
 		check(hipEventCreate(&e0), "hipEventCreate");
 		check(hipEventCreate(&e1), "hipEventCreate");
 		std::vector<hipEvent_t> ks, ke;
 		StageTimes stages;
 		check(hipEventRecord(e0, st), "hipEventRecord");
-		for (std::uint32_t done = 0; done < p.sampleCount; done += perLaunch)
+		if (integrator == LUZGPU_INTEGRATOR_MEGAKERNEL)
+		{
+			// samples per launch: bounded so the per-sample colour buffer stays within 4 GiB
+			const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount,
+				static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(pixels) * 12ull))));
+			ctx->sampleColors.reserve(static_cast<size_t>(pixels) * perLaunch * 3 * sizeof(float));
+			for (std::uint32_t done = 0; done < p.sampleCount; done += perLaunch)
+			{
+				ks.emplace_back();
+				ke.emplace_back();
+				check(hipEventCreate(&ks.back()), "hipEventCreate");
+				check(hipEventCreate(&ke.back()), "hipEventCreate");
+				check(hipEventRecord(ks.back(), st), "hipEventRecord");
+				GpuParams q = p;
+				q.sampleBegin = p.sampleBegin + done;
+				q.sampleCount = std::min(perLaunch, p.sampleCount - done);
+				const size_t threads = static_cast<size_t>(pixels) * q.sampleCount;
+				launchMega(settings && settings->count_visits, scene->gpu.host.features, blocks(threads, kMegaBlock), st, scene->gpu.view, q,
+					ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>());
+				check(hipGetLastError(), "render launch");
+				check(hipEventRecord(ke.back(), st), "hipEventRecord");
+				accumulate<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sums.as<double>(), pixels, q.sampleCount);
+			}
+		}
+		else
 		{
 			ks.emplace_back();
 			ke.emplace_back();
 			check(hipEventCreate(&ks.back()), "hipEventCreate");
 			check(hipEventCreate(&ke.back()), "hipEventCreate");
 			check(hipEventRecord(ks.back(), st), "hipEventRecord");
-			GpuParams q = p;
-			q.sampleBegin = p.sampleBegin + done;
-			q.sampleCount = std::min(perLaunch, p.sampleCount - done);
-			if (integrator == LUZGPU_INTEGRATOR_MEGAKERNEL)
-			{
-				const size_t threads = static_cast<size_t>(pixels) * q.sampleCount;
-				if (settings && settings->count_visits)
-				{
-					render_mega<true><<<blocks(threads, kMegaBlock), kMegaBlock, 0, st>>>(scene->gpu.view, q, ctx->sampleColors.as<float>(),
-						ctx->counters.as<unsigned long long>());
-				}
-				else
-				{
-					render_mega<false><<<blocks(threads, kMegaBlock), kMegaBlock, 0, st>>>(scene->gpu.view, q, ctx->sampleColors.as<float>(),
-						ctx->counters.as<unsigned long long>());
-				}
-			}
-			else
-			{
-				renderWavefront(ctx->wave, scene->gpu.view, q, ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>(), st,
-					settings && settings->count_visits, settings && settings->stage_times ? &stages : nullptr, tune);
-			}
+			const std::vector<BatchJob> jobs{BatchJob{p, ctx->sums.as<double>()}};
+			renderWavefront(ctx->wave, scene->gpu.view, scene->gpu.host.features, jobs, ctx->counters.as<unsigned long long>(), st,
+				settings && settings->count_visits, settings && settings->stage_times ? &stages : nullptr, tune);
 			check(hipGetLastError(), "render launch");
 			check(hipEventRecord(ke.back(), st), "hipEventRecord");
-			accumulate<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sums.as<double>(), pixels, q.sampleCount);
 		}
 		check(hipEventRecord(e1, st), "hipEventRecord");
 		float* dst = rgb_out;
@@ -605,6 +642,145 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			stats->shade_ms = stages.shadeMs;
 			stats->shadow_ms = stages.shadowMs;
 			stats->extend_launches = stages.extendLaunches;
+		}
+	});
+}
+
+int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes, const luzgpu_settings* settings,
+	float* const* rgb_out, luzgpu_stats* stats)
+{
+	if (!ctx || !scenes || !rgb_out || n == 0 || n > kMaxBatchScenes)
+	{
+		g_lastError = "luzgpu_render_batch: NULL argument or scene count outside 1.." + std::to_string(kMaxBatchScenes);
+		return LUZGPU_E_INVALID;
+	}
+	for (uint32_t i = 0; i < n; i++)
+	{
+		if (!scenes[i] || scenes[i]->ctx != ctx || !rgb_out[i])
+		{
+			g_lastError = "luzgpu_render_batch: scene " + std::to_string(i) + " is NULL, from another context, or has no output";
+			return LUZGPU_E_INVALID;
+		}
+	}
+	return guarded([&] {
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		hipStream_t st = ctx->stream;
+		// merged device arrays, rebuilt only when the scene list changes
+		std::vector<std::uint64_t> key(n);
+		for (uint32_t i = 0; i < n; i++) key[i] = scenes[i]->serial;
+		if (!ctx->batch || ctx->batchKey != key)
+		{
+			ctx->batch.reset();
+			ctx->wave.dropGraph();
+			std::vector<const PackedScene*> parts(n);
+			for (uint32_t i = 0; i < n; i++) parts[i] = &scenes[i]->gpu.host;
+			auto g = std::make_unique<GpuScene>();
+			g->host = mergeScenes(parts, ctx->batchParams);
+			if (g->host.tlasStackBound + 2 + g->host.blasStackBound > kWaveStackCapacity)
+			{
+				throw std::runtime_error("Unsupported for the device path: traversal stack bound of the batch");
+			}
+			uploadScene(ctx, *g);
+			ctx->batch = std::move(g);
+			ctx->batchKey = key;
+		}
+		std::vector<BatchJob> jobs(n);
+		std::vector<size_t> sumOffset(n);
+		size_t sumTotal = 0;
+		for (uint32_t i = 0; i < n; i++)
+		{
+			const luzgpu_settings* si = settings ? settings + i : nullptr;
+			if (si && si->integrator != LUZGPU_INTEGRATOR_WAVEFRONT)
+			{
+				throw Error{LUZGPU_E_INVALID, "luzgpu_render_batch: batches use the wavefront integrator"};
+			}
+			const GpuParams p = resolveParams(ctx->batchParams[i], scenes[i]->gpu.host, si);
+			if (p.sampleBegin + p.sampleCount > p.totalSamples || p.sampleCount == 0 || p.maxBounces < 0)
+			{
+				throw Error{LUZGPU_E_INVALID, "luzgpu_render_batch: scene " + std::to_string(i) + ": empty or out-of-range samples"};
+			}
+			jobs[i].params = p;
+			sumOffset[i] = sumTotal;
+			sumTotal += static_cast<size_t>(p.width) * p.height * 3;
+		}
+		ctx->batchSums.reserve(sumTotal * sizeof(double));
+		ctx->counters.reserve(8 * sizeof(unsigned long long));
+		zero_doubles<<<blocks(sumTotal, 256), 256, 0, st>>>(ctx->batchSums.as<double>(), sumTotal);
+		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
+		for (uint32_t i = 0; i < n; i++) jobs[i].sums = ctx->batchSums.as<double>() + sumOffset[i];
+		WaveTuning tune;
+		if (settings)
+		{
+			tune.grid = settings[0].traversal_grid;
+			tune.extendRefill = settings[0].extend_refill;
+			tune.shadowRefill = settings[0].shadow_refill;
+		}
+		hipEvent_t e0, e1;
+		check(hipEventCreate(&e0), "hipEventCreate");
+		check(hipEventCreate(&e1), "hipEventCreate");
+		const bool timed = settings && settings[0].stage_times;
+		StageTimes stages;
+		check(hipEventRecord(e0, st), "hipEventRecord");
+		renderWavefront(ctx->wave, ctx->batch->view, ctx->batch->host.features, jobs, ctx->counters.as<unsigned long long>(), st,
+			settings && settings[0].count_visits, timed ? &stages : nullptr, tune);
+		check(hipGetLastError(), "batch render launch");
+		check(hipEventRecord(e1, st), "hipEventRecord");
+		// per-scene resolve, into device outputs directly or through the staging buffer
+		size_t hostBytes = 0;
+		for (uint32_t i = 0; i < n; i++)
+		{
+			if (!(settings && settings[i].output_on_device)) hostBytes += static_cast<size_t>(jobs[i].params.width) * jobs[i].params.height * 3 * sizeof(float);
+		}
+		if (hostBytes) ctx->output.reserve(hostBytes);
+		size_t staged = 0;
+		std::vector<std::pair<float*, size_t>> copies;
+		for (uint32_t i = 0; i < n; i++)
+		{
+			const std::uint32_t pixels = jobs[i].params.width * jobs[i].params.height;
+			float* dst = rgb_out[i];
+			if (!(settings && settings[i].output_on_device))
+			{
+				dst = reinterpret_cast<float*>(static_cast<char*>(ctx->output.ptr) + staged);
+				copies.emplace_back(rgb_out[i], staged);
+				staged += static_cast<size_t>(pixels) * 3 * sizeof(float);
+			}
+			resolve<<<blocks(pixels, 256), 256, 0, st>>>(jobs[i].sums, dst, pixels, static_cast<double>(jobs[i].params.sampleCount));
+		}
+		check(hipGetLastError(), "resolve launch");
+		unsigned long long counters[6] = {0, 0, 0, 0, 0, 0};
+		check(hipMemcpyAsync(counters, ctx->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, st), "counter readback");
+		for (uint32_t i = 0, c = 0; i < n; i++)
+		{
+			if (settings && settings[i].output_on_device) continue;
+			const size_t bytes = static_cast<size_t>(jobs[i].params.width) * jobs[i].params.height * 3 * sizeof(float);
+			check(hipMemcpyAsync(copies[c].first, static_cast<char*>(ctx->output.ptr) + copies[c].second, bytes, hipMemcpyDeviceToHost, st),
+				"image readback");
+			c++;
+		}
+		check(hipStreamSynchronize(st), "batch render");
+		float ms = 0.0f;
+		check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		if (stats)
+		{
+			std::uint64_t samples = 0;
+			for (const BatchJob& j : jobs) samples += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
+			*stats = luzgpu_stats{};
+			stats->rendered_samples = samples;
+			stats->closest_hit_rays = counters[0];
+			stats->shadow_rays = counters[1];
+			stats->render_ms = ms;
+			stats->kernel_ms = ms;
+			stats->nodes_visited = counters[2] + counters[4];
+			stats->prims_tested = counters[3] + counters[5];
+			stats->closest_nodes = counters[2];
+			stats->closest_prims = counters[3];
+			stats->extend_ms = stages.extendMs;
+			stats->shade_ms = stages.shadeMs;
+			stats->shadow_ms = stages.shadowMs;
+			stats->extend_launches = stages.extendLaunches;
+			stats->kernel_launches = stages.passes;
 		}
 	});
 }

@@ -181,6 +181,40 @@ class Renderer:
                                 st.closest_nodes, st.closest_prims, st.extend_ms, st.shade_ms, st.shadow_ms)
 
     @staticmethod
+    def render_batch(scenes, seeds, samples=None, max_bounces=None, outs=None, resolutions=None, count_visits=False,
+                     stage_times=False, tuning=None):
+        """Renders several scenes in one multi-scene launch sequence (luzgpu_render_batch).
+
+        ``seeds``, ``samples``, ``max_bounces`` and ``resolutions`` are per-scene lists (or
+        None for the scene defaults); ``outs`` optional per-scene device tensors. Returns
+        (images, aggregate RenderStats)."""
+        n = len(scenes)
+        if n == 0:
+            raise ValueError("render_batch needs at least one scene")
+        ctx = scenes[0].ctx
+        arr = (_lib.Settings * n)()
+        images, ptrs = [], (ctypes.c_void_p * n)()
+        for i, sc in enumerate(scenes):
+            res = resolutions[i] if resolutions else None
+            arr[i] = _settings(seeds[i], samples[i] if samples else None, max_bounces[i] if max_bounces else None, 0, None,
+                               "wavefront", on_device=outs is not None, resolution=res,
+                               count_visits=count_visits, stage_times=stage_times, tuning=tuning)
+            w, h = res if res is not None else (sc.width, sc.height)
+            if outs is not None:
+                images.append(outs[i])
+                ptrs[i] = outs[i].data_ptr()
+            else:
+                img = np.zeros((h, w, 3), dtype=np.float32)
+                images.append(img)
+                ptrs[i] = img.ctypes.data
+        handles = (ctypes.c_void_p * n)(*[sc._h.value for sc in scenes])
+        st = _lib.Stats()
+        _lib.check(ctx._lib.luzgpu_render_batch(ctx.handle, n, handles, arr, ptrs, ctypes.byref(st)))
+        return images, RenderStats(st.rendered_samples, st.closest_hit_rays, st.shadow_rays, st.render_ms, st.kernel_ms,
+                                   st.kernel_launches, st.nodes_visited, st.prims_tested, st.extend_launches,
+                                   st.closest_nodes, st.closest_prims, st.extend_ms, st.shade_ms, st.shadow_ms)
+
+    @staticmethod
     def primary_hits(scene: Scene, seed: int, resolution=None):
         s = _settings(seed, None, None, 0, None, "megakernel", resolution=resolution)
         w, h = resolution if resolution is not None else (scene.width, scene.height)

@@ -24,6 +24,9 @@ Workloads
 * ``volume``       participating media: a Henyey-Greenstein box of fog (g 0.6) and an
                    isotropic sphere of smoke over a ground plane, lit by a directional sun and
                    a sphere light; 160x120, 32 spp, depth 16.
+* ``batch``        32 scenes of seeded displaced-sphere meshes with triangle counts
+                   log-uniform in [1e4, 5e5], resolution drawn from {256^2, 512^2, 768x432,
+                   1024^2}, spp uniform in {16..128}, depth 10, alternating sky / background.
 * ``terrain``      a 1,002,528-triangle displaced grid terrain inside a box of
                    Henyey-Greenstein fog (sigma_t 0.05, albedo 0.9, g 0.6) under a directional
                    light; 1280x720, 32 spp, depth 16.
@@ -465,6 +468,101 @@ def terrain_scene(directory: str, width=1280, height=720, spp=32, depth=16, quad
     _write_if_changed(path, ("\n".join(lines) + "\n").encode())
     return Workload("terrain", directory, path, width, height, spp, depth, render_seed, 2 * quads * quads,
                     {"instances": 1})
+
+
+BATCH_RESOLUTIONS = [(256, 256), (512, 512), (768, 432), (1024, 1024)]
+
+
+def _uv_sphere_obj(rng: random.Random, radius: float, slices: int, stacks: int) -> str:
+    """Displaced UV sphere with 2 * slices * (stacks - 1) triangles."""
+    waves = []
+    for _ in range(3):
+        a = rng.uniform(0.0, 2.0 * math.pi)
+        z = rng.uniform(-1.0, 1.0)
+        r = math.sqrt(1.0 - z * z)
+        waves.append((r * math.cos(a), z, r * math.sin(a), rng.uniform(2.0, 6.0), rng.uniform(0.0, 2 * math.pi)))
+    verts = []
+
+    def vertex(x, y, z):
+        d = sum(math.sin(f * (x * wx + y * wy + z * wz) + ph) for (wx, wy, wz, f, ph) in waves) / 3.0
+        sc = radius * (1.0 + 0.1 * d)
+        verts.append((x * sc, y * sc, z * sc))
+
+    vertex(0.0, 1.0, 0.0)
+    for k in range(1, stacks):
+        th = math.pi * k / stacks
+        for j in range(slices):
+            ph = 2.0 * math.pi * j / slices
+            vertex(math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph))
+    vertex(0.0, -1.0, 0.0)
+    south = len(verts)
+    out = ["v %.7g %.7g %.7g" % v for v in verts]
+
+    def ring(k, j):
+        return 2 + (k - 1) * slices + (j % slices)
+
+    for j in range(slices):
+        out.append(f"f 1 {ring(1, j + 1)} {ring(1, j)}")
+    for k in range(1, stacks - 1):
+        for j in range(slices):
+            a, b2 = ring(k, j), ring(k, j + 1)
+            c, d = ring(k + 1, j), ring(k + 1, j + 1)
+            out.append(f"f {a} {b2} {d}")
+            out.append(f"f {a} {d} {c}")
+    for j in range(slices):
+        out.append(f"f {south} {ring(stacks - 1, j)} {ring(stacks - 1, j + 1)}")
+    return "\n".join(out) + "\n"
+
+
+def batch_scenes(directory: str, count=32, seed=1003, depth=10, spp_range=(16, 128)) -> list:
+    """The uneven multi-scene batch: one Workload per scene."""
+    os.makedirs(directory, exist_ok=True)
+    rng = random.Random(seed)
+    sky = os.path.join(directory, "sky.hdr")
+    _write_if_changed(sky, sky_hdr(512, 256))
+    out = []
+    for si in range(count):
+        target = math.exp(rng.uniform(math.log(1e4), math.log(5e5)))
+        width, height = rng.choice(BATCH_RESOLUTIONS)
+        spp = rng.randint(spp_range[0], spp_range[1])
+        # meshes of 2 * 32 * 31 = 1984 triangles, instanced to reach the target count
+        slices, stacks = 32, 32
+        per = 2 * slices * (stacks - 1)
+        instances = max(1, int(round(target / per)))
+        mesh_count = min(4, instances)
+        meshes = []
+        for m in range(mesh_count):
+            name = f"s{si:02d}_mesh{m}.obj"
+            _write_if_changed(os.path.join(directory, name), _uv_sphere_obj(rng, 1.0, slices, stacks).encode())
+            meshes.append(name)
+        mats, objs = [], []
+        extent = 6.0 * instances ** (1.0 / 3.0)
+        for i in range(instances):
+            pick = rng.random()
+            if pick < 0.4:
+                mats += [f"material m{i} {{", "type=lambertian", f"color={_v([rng.uniform(0.1, 0.9) for _ in range(3)])}", "}"]
+            elif pick < 0.7:
+                mats += [f"material m{i} {{", "type=metal", f"color={_v([rng.uniform(0.5, 0.95) for _ in range(3)])}",
+                         f"roughness={_f(math.sqrt(rng.uniform(0.05, 0.5)))}", "}"]
+            else:
+                mats += [f"material m{i} {{", "type=dielectric", "color=(1,1,1)", "ior=1.5", "roughness=0", "}"]
+            pos = [rng.uniform(-extent, extent) for _ in range(3)]
+            objs += [f"object o{i} {{", f"file={meshes[i % mesh_count]}", f"position={_v(pos)}",
+                     f"scale={_v([rng.uniform(0.5, 2.0)] * 3)}", f"material=m{i}", "}"]
+        lines = ["[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER]
+        if si % 2 == 0:
+            lines.append("environment=sky.hdr")
+        else:
+            lines += ["sky=none", "background=(0.6,0.65,0.8)"]
+        lines += ["", "[materials]", *mats, "", "[scene]",
+                  "camera main {", f"position=(0,0,{_f(-3.2 * extent)})", "direction=(0,0,1)", "up=(0,1,0)",
+                  "focal_length_mm=30", "sensor_width_mm=36", "sensor_height_mm=24", "pinhole=1", "focus_distance=10", "}",
+                  *objs]
+        path = os.path.join(directory, f"scene_{si:02d}.luz")
+        _write_if_changed(path, ("\n".join(lines) + "\n").encode())
+        out.append(Workload(f"batch{si:02d}", directory, path, width, height, spp, depth, si + 1, instances * per,
+                            {"instances": instances}))
+    return out
 
 
 def _write_if_changed(path: str, data: bytes) -> None:

@@ -100,3 +100,22 @@ def test_sample_range_sharding_composes(ctx, cornell):
     lo, _ = Renderer.render(scene, seed=3, samples=8, sample_begin=0, sample_count=4, integrator="wavefront")
     hi, _ = Renderer.render(scene, seed=3, samples=8, sample_begin=4, sample_count=4, integrator="wavefront")
     assert np.allclose((lo.astype(np.float64) + hi) / 2.0, full, rtol=1e-5, atol=1e-6)
+
+
+def test_graph_replays_match_eager_launches(ctx, cornell):
+    """Renders without stage timing replay one captured pass graph; with stage timing the
+    same launches are enqueued one by one. Every replay must equal the eager render, with
+    eager work and synchronisations between the replays."""
+    from luz_amd import Renderer, SceneFile
+    scene = SceneFile.read(ctx, cornell.scene_path)
+    eager, _ = Renderer.render(scene, seed=11, samples=6, stage_times=True)
+    for _ in range(3):
+        replay, _ = Renderer.render(scene, seed=11, samples=6)
+        assert np.array_equal(replay, eager)
+        again, _ = Renderer.render(scene, seed=11, samples=6, stage_times=True)
+        assert np.array_equal(again, eager)
+    # a different seed reuses the graph with a new pass table
+    other, _ = Renderer.render(scene, seed=12, samples=6)
+    other_eager, _ = Renderer.render(scene, seed=12, samples=6, stage_times=True)
+    assert np.array_equal(other, other_eager)
+    assert not np.array_equal(other, eager)
