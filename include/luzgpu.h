@@ -87,6 +87,7 @@ typedef struct luzgpu_scene_info
 	uint32_t	tlas_nodes, blas_nodes;
 	uint32_t	env_width, env_height;
 	uint32_t	tlas_depth, blas_depth;
+	uint32_t	materials;
 } luzgpu_scene_info;
 
 /* Context: one HIP device + stream. (Threads.cpp:375-548 worker pool) */
@@ -119,6 +120,23 @@ int		luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* 
  * settings[0]. stats, if given, receives one aggregate record for the batch. */
 int		luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes, const luzgpu_settings* settings,
 			float* const* rgb_out, luzgpu_stats* stats);
+
+/* Path-replay adjoint (no counterpart in the reference): gradients of an image loss with
+ * respect to per-material parameters, for the render luzgpu_render produces with the same
+ * settings (samples, seed, range, resolution). dl_dimage[W*H*3] = dLoss/dImage (a device
+ * pointer when settings->output_on_device, else host). Outputs (host, any may be NULL),
+ * indexed by material in scene-file order: d_albedo[3*m+c] for Lambertian albedo,
+ * d_roughness[m] for metal GGX roughness, d_emission[3*m+c] for emitter radiance; other
+ * materials get 0. Sampling, MIS weights and Russian roulette are held fixed (detached). */
+int		luzgpu_adjoint(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings, const float* dl_dimage,
+			double* d_albedo, double* d_roughness, double* d_emission);
+
+/* Sets per-material parameters (count = the scene's material count; any array may be NULL
+ * to keep those values): Lambertian albedo[3*m..], metal roughness[m], emitter
+ * emission[3*m..] (radiance); other material kinds ignore them. Light selection weights are
+ * rebuilt (Scene::updateLights, Scene.cpp:615-638). */
+int		luzgpu_scene_set_materials(luzgpu_scene* scene, uint32_t count, const double* albedo, const double* roughness,
+			const double* emission);
 
 /* --- introspection used by the parity tests (no counterpart in the reference API) --- */
 

@@ -26,6 +26,7 @@ EXPORTED = [
     "luzgpu_create", "luzgpu_destroy", "luzgpu_last_error", "luzgpu_scene_load",
     "luzgpu_scene_release", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_primary_hits",
     "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas", "luzgpu_render_batch",
+    "luzgpu_adjoint", "luzgpu_scene_set_materials",
 ]
 
 
@@ -72,7 +73,7 @@ class SceneInfo(ctypes.Structure):
         ("max_bounces", ctypes.c_int32)] + [
         (n, ctypes.c_uint32) for n in ("hittables", "triangles", "meshes", "instances", "lights",
                                         "tlas_nodes", "blas_nodes", "env_width", "env_height",
-                                        "tlas_depth", "blas_depth")]
+                                        "tlas_depth", "blas_depth", "materials")]
 
 
 class LuzGpuError(RuntimeError):
@@ -102,6 +103,8 @@ def load() -> ctypes.CDLL:
     lib.luzgpu_scene_get_info.argtypes = [vp, ctypes.POINTER(SceneInfo)]
     lib.luzgpu_render.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, ctypes.POINTER(Stats)]
     lib.luzgpu_primary_hits.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, vp]
+    lib.luzgpu_adjoint.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, vp, vp, vp]
+    lib.luzgpu_scene_set_materials.argtypes = [vp, ctypes.c_uint32, vp, vp, vp]
     lib.luzgpu_render_batch.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(vp), ctypes.POINTER(Settings),
                                         ctypes.POINTER(vp), ctypes.POINTER(Stats)]
     lib.luzgpu_sampler_eval.argtypes = [vp, ctypes.c_size_t, vp, vp, vp]
@@ -109,6 +112,7 @@ def load() -> ctypes.CDLL:
     lib.luzgpu_scene_blas.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64)]
     for name in ("luzgpu_create", "luzgpu_scene_load", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_render_batch",
+                 "luzgpu_adjoint", "luzgpu_scene_set_materials",
                  "luzgpu_primary_hits", "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib

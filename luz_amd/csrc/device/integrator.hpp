@@ -76,6 +76,7 @@ namespace luzgpu
 		f3	direction;
 		float	tMax;
 		bool	active;
+		std::uint32_t	lightMaterial;	// emissive material of the sampled light
 	};
 
 	template <int Feat = kFeatAll>
@@ -94,6 +95,7 @@ namespace luzgpu
 		const float w = powerHeuristic(ls.pdf, spdf);
 		c.contribution = f * ls.emitted * (w / (ls.pdf * prob));
 		c.direction = ls.direction;
+		c.lightMaterial = ls.material;
 		c.tMax = ls.tMax - kTMinF;
 		c.active = c.tMax > kTMinF;	// isShadowOccluded: tMax <= T_MIN is never occluded
 		if (!c.active)
@@ -216,9 +218,21 @@ namespace luzgpu
 	}
 
 	// Whole path in one thread: the plain (megakernel) integrator.
-	template <bool Count, int Feat, typename Stack>
+	// Hooks of the path-replay adjoint; the forward render uses this no-op set.
+	struct	NoAdjoint
+	{
+		static constexpr bool	enabled = false;
+		__device__ __forceinline__ void	emission(std::uint32_t, f3, f3) {}
+		__device__ __forceinline__ void	arrival(f3, float) {}
+		__device__ __forceinline__ void	specular() {}
+		__device__ __forceinline__ void	vertex(const GpuMaterial&, std::uint32_t, const Surface&, const Scatter&, f3) {}
+		__device__ __forceinline__ void	connection(const GpuMaterial&, std::uint32_t, const Surface&, const Scatter&, f3, f3, std::int32_t) {}
+		__device__ __forceinline__ void	continuation(const GpuMaterial&, std::uint32_t, const Surface&, const Scatter&, f3, f3) {}
+	};
+
+	template <bool Count, int Feat, typename Stack, typename Adj = NoAdjoint>
 	__device__ f3	tracePath(const SceneView& s, const GpuParams& p, std::uint32_t x, std::uint32_t y, std::uint32_t sample,
-		Stack& stack, std::uint32_t& closestRays, std::uint32_t& shadowRays, VisitCounter& visits)
+		Stack& stack, std::uint32_t& closestRays, std::uint32_t& shadowRays, VisitCounter& visits, Adj* adj = nullptr)
 	{
 		PixelSampler ps;
 		ps.begin(p.renderSeed, x, y, sample, 0);
@@ -242,7 +256,9 @@ namespace luzgpu
 				}
 				f3 c = thr * missRadiance(s, p, ray.d, w);
 				if (b > 0 && hasConstantBackgroundLight(p)) c = clampConstantBackground(p, c);
-				return acc + clampRay(c);
+				c = clampRay(c);
+				if constexpr (Adj::enabled) adj->arrival(c, w);
+				return acc + c;
 			}
 			const Surface sf = surfaceAt<Feat>(s, ray, h);
 			const GpuMaterial& m = s.materials[sf.material];
@@ -252,7 +268,13 @@ namespace luzgpu
 			{
 				const float lp = (!prevSpecular && p.lightCount > 0) ? lightsPdf<Feat>(s, p, prevOrigin, ray.d) : 0.0f;
 				const float w = prevSpecular ? 1.0f : powerHeuristic(prevPdf, lp);
-				return acc + clampRay((thr * emitted) * w);
+				const f3 inc = clampRay((thr * emitted) * w);
+				if constexpr (Adj::enabled)
+				{
+					adj->emission(sf.material, inc, emitted);
+					adj->arrival(inc, w);
+				}
+				return acc + inc;
 			}
 			f3 att = sc.attenuation;
 			if (!sf.frontFace && sc.absorbing)
@@ -266,6 +288,7 @@ namespace luzgpu
 			if (sc.specular)
 			{
 				if (!leavesOpaque(sf, m, sc.direction)) return acc;
+				if constexpr (Adj::enabled) adj->specular();
 				thr = clampRay(thr * att);
 				if (terminated(thr) || !russianRoulette(thr, b, ps)) return acc;
 				ray = makeRay(sf.position, sc.direction);
@@ -274,6 +297,7 @@ namespace luzgpu
 				continue;
 			}
 			acc = acc + thr * emitted;
+			if constexpr (Adj::enabled) adj->vertex(m, sf.material, sf, sc, acc);
 			if (terminated(thr * att)) return acc;
 			if (p.lightCount > 0)
 			{
@@ -288,7 +312,12 @@ namespace luzgpu
 						occluded = traverse<true, Count, Feat>(s, p, makeRay(sf.position, c.direction), kTMinF, c.tMax, sh, stack, &visits,
 							logU);
 					}
-					if (!occluded) acc = acc + clampRay(thr * c.contribution);
+					if (!occluded)
+					{
+						const f3 inc = clampRay(thr * c.contribution);
+						acc = acc + inc;
+						if constexpr (Adj::enabled) adj->connection(m, sf.material, sf, sc, c.direction, inc, static_cast<std::int32_t>(c.lightMaterial));
+					}
 				}
 			}
 			if (sampledInfinite)
@@ -300,7 +329,9 @@ namespace luzgpu
 					shadowRays++;
 					if (!traverse<true, Count, Feat>(s, p, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack, &visits, logU))
 					{
-						acc = acc + clampRay(thr * c.contribution);
+						const f3 inc = clampRay(thr * c.contribution);
+						acc = acc + inc;
+						if constexpr (Adj::enabled) adj->connection(m, sf.material, sf, sc, c.direction, inc, -1);
 					}
 				}
 			}
@@ -308,6 +339,7 @@ namespace luzgpu
 			if (!leavesOpaque(sf, m, dir)) return acc;
 			const float pdf = scatterPdf<Feat>(m, sf, sc, dir);
 			if (pdf <= 0.0f || !isfinite(pdf)) return acc;
+			if constexpr (Adj::enabled) adj->continuation(m, sf.material, sf, sc, dir, acc);
 			thr = clampRay(thr * att);
 			if (terminated(thr) || !russianRoulette(thr, b, ps)) return acc;
 			ray = makeRay(sf.position, dir);

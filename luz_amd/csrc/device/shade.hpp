@@ -194,6 +194,25 @@ namespace luzgpu
 		return 2.0f * noW / den;
 	}
 
+	// d ln(D(noH) G1(noV) G1(noL)) / d alpha of the GGX reflection lobe (the path-replay
+	// adjoint's roughness derivative; D and G1 as ggxD / smithG1 above).
+	__device__ __forceinline__ float	ggxDlogAlpha(f3 n, f3 view, f3 l, float alpha)
+	{
+		alpha = fmaxf(1e-4f, alpha);
+		const float noV = clamp01(dot(n, view)), noL = clamp01(dot(n, l));
+		const f3 h = safeNormalize(view + l, n);
+		const float noH = clamp01(dot(n, h));
+		const float a2 = alpha * alpha;
+		const float den = noH * noH * (a2 - 1.0f) + 1.0f;
+		float d = 2.0f / alpha - 4.0f * alpha * noH * noH / den;
+		auto g1 = [&](float c) {
+			const float sq = sqrtf(a2 + (1.0f - a2) * c * c);
+			return -(alpha * (1.0f - c * c) / sq) / (c + sq);
+		};
+		d += g1(noV) + g1(noL);
+		return isfinite(d) ? d : 0.0f;
+	}
+
 	__device__ __forceinline__ f3	sampleGGXHalf(f3 n, float alpha, float sx, float sy)
 	{
 		alpha = fmaxf(1e-4f, alpha);
@@ -316,6 +335,7 @@ namespace luzgpu
 		f3	cosineW;		// cosine basis axis; Henyey-Greenstein: the phase direction
 		float	g;			// Henyey-Greenstein anisotropy
 		bool	absorbing;	// dielectric medium absorption
+		bool	mirrorFallback;	// rough metal: the GGX sample reflected below the horizon
 	};
 
 	// materialCanTransmitThroughGeometry (ColorHelper.cpp:327-355)
@@ -544,6 +564,7 @@ namespace luzgpu
 		sc.incident = inDir;
 		sc.absorbing = false;
 		sc.specular = false;
+		sc.mirrorFallback = false;
 		sc.pdfType = PDF_NONE;
 		if (m.kind == GPU_MAT_LAMBERTIAN)
 		{
@@ -655,9 +676,17 @@ namespace luzgpu
 			float sx, sy;
 			ps.sample2D(DIM_BSDF_DIRECTION, sx, sy);
 			f3 h = sampleGGXHalf(sf.normal, m.alpha, sx, sy);
-			if (dot(view, h) <= 0.0f) h = sf.normal;
+			if (dot(view, h) <= 0.0f)
+			{
+				h = sf.normal;
+				sc.mirrorFallback = true;
+			}
 			f3 dir = reflect(inDir, h);
-			if (dot(dir, sf.normal) <= 0.0f) dir = reflect(inDir, sf.normal);
+			if (dot(dir, sf.normal) <= 0.0f)
+			{
+				dir = reflect(inDir, sf.normal);
+				sc.mirrorFallback = true;
+			}
 			const float pdf = ggxReflectionPdf(sf.normal, view, dir, m.alpha);
 			const f3 f = metalBsdfCos(m, sf.normal, view, dir);
 			if (pdf <= 0.0f || !isfinite(pdf) || maxc(f) <= 0.0f) return false;
@@ -756,6 +785,7 @@ namespace luzgpu
 		float	pdf;
 		float	tMax;
 		bool	valid;
+		std::uint32_t	material;	// the light's emissive material (finite lights)
 	};
 
 	// Sphere::sampleLight (Sphere.cpp:382-442): cone sampling toward the sphere, FP64 like
@@ -834,6 +864,7 @@ namespace luzgpu
 			}
 		}
 		const GpuLight& L = s.lights[p.lightBase + idx];
+		ls.material = L.material;
 		if (!(L.selectionProbability > 0.0f)) return ls;
 		if ((Feat & kFeatAnalytic) && L.kind == GPU_HIT_DIRECTIONAL)
 		{
@@ -847,7 +878,9 @@ namespace luzgpu
 		}
 		if ((Feat & kFeatAnalytic) && L.kind == GPU_HIT_SPHERE)
 		{
-			return sampleSphereLight(s, L, origin, ps);
+			LightSample sl = sampleSphereLight(s, L, origin, ps);
+			sl.material = L.material;
+			return sl;
 		}
 		if (L.kind != GPU_HIT_RECTANGLE && L.kind != GPU_HIT_TRIANGLE)
 		{

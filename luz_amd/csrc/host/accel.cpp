@@ -963,6 +963,7 @@ namespace luzgpu
 				smoothTransmission = m.metallic <= 1e-6 && m.roughness <= 1e-4 && m.transmission >= 1.0 - 1e-6
 					&& m.clearcoat <= 1e-6 && m.sheen <= 1e-6;
 			}
+			p.materialColor.push_back(m.color);
 			g.flags = (smoothTransmission ? GPU_MATF_SMOOTH_TRANSMISSION : 0u) | (m.conductor ? GPU_MATF_CONDUCTOR : 0u)
 				| ((m.absorption[0] > 0.0 || m.absorption[1] > 0.0 || m.absorption[2] > 0.0) ? GPU_MATF_ABSORPTION : 0u);
 			p.materials.push_back(g);
@@ -1276,6 +1277,7 @@ namespace luzgpu
 				throw std::runtime_error("Unsupported for the device path: emissive mesh instance");
 			}
 			l.area = static_cast<float>(h.kind == HIT_DIRECTIONAL || h.kind == HIT_CUBE || h.kind == HIT_PLANE ? 0.0 : area);
+			p.lightScale.push_back(area);
 			double w = (area <= 0.0 || !std::isfinite(lum) || lum <= 0.0) ? 0.0 : area * lum;
 			if (!std::isfinite(w) || w <= 0.0) w = 0.0;
 			total += w;
@@ -1510,5 +1512,40 @@ namespace luzgpu
 			m.triangleCount += p.triangleCount;
 		}
 		return (m);
+	}
+}
+
+namespace luzgpu
+{
+	void	rebuildLights(PackedScene& p)
+	{
+		double total = 0.0;
+		p.lightCdf.clear();
+		for (std::size_t i = 0; i < p.lights.size(); i++)
+		{
+			GpuLight& l = p.lights[i];
+			const dvec3& c = p.materialColor[l.material];
+			for (int k = 0; k < 3; k++) l.emitted[k] = static_cast<float>(c[k]);
+			const double lum = luminance(c);
+			const double area = p.lightScale[i];
+			double w = (area <= 0.0 || !std::isfinite(lum) || lum <= 0.0) ? 0.0 : area * lum;
+			if (!std::isfinite(w) || w <= 0.0) w = 0.0;
+			total += w;
+			p.lightCdf.push_back(total);
+		}
+		if (total <= 0.0) p.lightCdf.clear();
+		for (std::size_t i = 0; i < p.lights.size(); i++)
+		{
+			double prob;
+			if (p.lightCdf.empty()) prob = 1.0 / static_cast<double>(p.lights.size());
+			else
+			{
+				const double prev = i == 0 ? 0.0 : p.lightCdf[i - 1];
+				const double w = p.lightCdf[i] - prev;
+				prob = (w <= 0.0 || total <= 0.0) ? 0.0 : w / total;
+			}
+			p.lights[i].selectionProbability = static_cast<float>(prob);
+		}
+		p.params.lightWeighted = p.lightCdf.empty() ? 0u : 1u;
 	}
 }

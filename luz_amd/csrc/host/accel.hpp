@@ -67,7 +67,13 @@ namespace luzgpu
 		Camera	camera;
 		double	metersPerUnit = 1.0;
 		int	features = 0;	// kFeat* classes present (kernel specialisation)
+		std::vector<double>	lightScale;	// light selection weight per unit luminance (area, or 1)
+		std::vector<dvec3>	materialColor;	// FP64 material colours (weights are rebuilt from these)
 	};
+
+	// Rebuilds light emission, selection weights, CDF and probabilities after material
+	// colours changed (Scene::updateLights, Scene.cpp:615-638).
+	void	rebuildLights(PackedScene& p);
 
 	// RenderCamera for an image of width x height (RayHelper.cpp:50-82).
 	GpuCamera	makeCamera(const Camera& cam, std::uint32_t width, std::uint32_t height, double metersPerUnit);

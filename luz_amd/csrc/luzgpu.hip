@@ -11,6 +11,7 @@
 
 #include "../../include/luzgpu.h"
 #include "common/layout.h"
+#include "device/adjoint.hpp"
 #include "device/integrator.hpp"
 #include "device/wavefront.hpp"
 #include "host/accel.hpp"
@@ -155,6 +156,57 @@ namespace luzgpu
 	{
 		if (count) launchMegaC<true>(features, grid, st, s, p, colors, counters);
 		else launchMegaC<false>(features, grid, st, s, p, colors, counters);
+	}
+
+	// Path-replay adjoint, one thread per (pixel, sample): dLdI is dLoss/dImage (W*H*3),
+	// grads receives kGradPerMaterial FP64 sums per material.
+	template <int Feat>
+	__global__ __launch_bounds__(kMegaBlock) void	adjoint_mega(SceneView s, GpuParams p, const float* __restrict__ dLdI,
+		double* __restrict__ grads, std::uint32_t materialCount)
+	{
+		__shared__ std::uint32_t stackMem[kStackDepth * kMegaBlock];
+		extern __shared__ float ldsGrads[];
+		const std::uint32_t n = materialCount * kGradPerMaterial;
+		for (std::uint32_t i = threadIdx.x; i < n; i += kMegaBlock) ldsGrads[i] = 0.0f;
+		__syncthreads();
+		const std::uint32_t pixels = p.width * p.height;
+		const size_t id = static_cast<size_t>(blockIdx.x) * kMegaBlock + threadIdx.x;
+		if (id < static_cast<size_t>(pixels) * p.sampleCount)
+		{
+			const std::uint32_t pixel = static_cast<std::uint32_t>(id % pixels), slot = static_cast<std::uint32_t>(id / pixels);
+			LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
+			AdjointRecorder rec;
+			rec.grads = ldsGrads;
+			rec.count = 0;
+			rec.pendingMaterial = -1;
+			rec.pendingScale = 0.0f;
+			rec.materials = s.materials;
+			const float inv = 1.0f / static_cast<float>(p.sampleCount);
+			rec.weight = mk(dLdI[3 * static_cast<size_t>(pixel)] * inv, dLdI[3 * static_cast<size_t>(pixel) + 1] * inv,
+				dLdI[3 * static_cast<size_t>(pixel) + 2] * inv);
+			std::uint32_t closest = 0, shadow = 0;
+			VisitCounter visits{0u, 0u};
+			const f3 L = tracePath<false, Feat>(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest, shadow,
+				visits, &rec);
+			rec.finish(L);
+		}
+		__syncthreads();
+		for (std::uint32_t i = threadIdx.x; i < n; i += kMegaBlock)
+		{
+			if (ldsGrads[i] != 0.0f) atomicAdd(grads + i, static_cast<double>(ldsGrads[i]));
+		}
+	}
+
+	void	launchAdjoint(int features, size_t grid, size_t shmem, hipStream_t st, const SceneView& s, const GpuParams& p,
+		const float* dLdI, double* grads, std::uint32_t materialCount)
+	{
+		switch (features & kFeatAll)
+		{
+			case 0: adjoint_mega<0><<<grid, kMegaBlock, shmem, st>>>(s, p, dLdI, grads, materialCount); break;
+			case kFeatAnalytic: adjoint_mega<kFeatAnalytic><<<grid, kMegaBlock, shmem, st>>>(s, p, dLdI, grads, materialCount); break;
+			case kFeatRich: adjoint_mega<kFeatRich><<<grid, kMegaBlock, shmem, st>>>(s, p, dLdI, grads, materialCount); break;
+			default: adjoint_mega<kFeatAll><<<grid, kMegaBlock, shmem, st>>>(s, p, dLdI, grads, materialCount); break;
+		}
 	}
 
 	__global__ void	accumulate(const float* __restrict__ sampleColors, double* __restrict__ sums, std::uint32_t pixels, std::uint32_t count)
@@ -489,6 +541,7 @@ int	luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out)
 		info.blas_nodes = static_cast<std::uint32_t>(h.blasWide.size());
 		info.env_width = h.params.envWidth;
 		info.env_height = h.params.envHeight;
+		info.materials = static_cast<std::uint32_t>(h.materials.size());
 		info.tlas_depth = static_cast<std::uint32_t>(tlasDepth);
 		info.blas_depth = static_cast<std::uint32_t>(blasDepth);
 		*out = sc.release();
@@ -782,6 +835,128 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 			stats->extend_launches = stages.extendLaunches;
 			stats->kernel_launches = stages.passes;
 		}
+	});
+}
+
+int	luzgpu_adjoint(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings, const float* dl_dimage,
+	double* d_albedo, double* d_roughness, double* d_emission)
+{
+	if (!ctx || !scene || !dl_dimage || scene->ctx != ctx)
+	{
+		g_lastError = "luzgpu_adjoint: NULL argument or scene from another context";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		const GpuParams p = resolveParams(scene->gpu, settings);
+		if (p.sampleBegin + p.sampleCount > p.totalSamples || p.sampleCount == 0 || p.maxBounces < 0)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_adjoint: empty or out-of-range sample range"};
+		}
+		if (p.maxBounces + 1 > kAdjointRecords)
+		{
+			throw std::runtime_error("Unsupported for the adjoint: more than " + std::to_string(kAdjointRecords - 1) + " bounces");
+		}
+		const std::uint32_t materials = static_cast<std::uint32_t>(scene->gpu.host.materials.size());
+		const size_t shmem = static_cast<size_t>(materials) * kGradPerMaterial * sizeof(float);
+		if (shmem > 96 * 1024)
+		{
+			throw std::runtime_error("Unsupported for the adjoint: more than " + std::to_string(96 * 1024 / (kGradPerMaterial * 4)) + " materials");
+		}
+		hipStream_t st = ctx->stream;
+		const std::uint32_t pixels = p.width * p.height;
+		const float* dev = dl_dimage;
+		if (!(settings && settings->output_on_device))
+		{
+			ctx->scratch.reserve(static_cast<size_t>(pixels) * 3 * sizeof(float));
+			check(hipMemcpyAsync(ctx->scratch.ptr, dl_dimage, static_cast<size_t>(pixels) * 3 * sizeof(float), hipMemcpyHostToDevice, st),
+				"dL/dimage upload");
+			dev = ctx->scratch.as<float>();
+		}
+		const size_t n = static_cast<size_t>(materials) * kGradPerMaterial;
+		ctx->sums.reserve(n * sizeof(double));
+		zero_doubles<<<blocks(n, 256), 256, 0, st>>>(ctx->sums.as<double>(), n);
+		const size_t threads = static_cast<size_t>(pixels) * p.sampleCount;
+		launchAdjoint(scene->gpu.host.features, blocks(threads, kMegaBlock), shmem, st, scene->gpu.view, p, dev, ctx->sums.as<double>(),
+			materials);
+		check(hipGetLastError(), "adjoint launch");
+		std::vector<double> g(n);
+		check(hipMemcpyAsync(g.data(), ctx->sums.ptr, n * sizeof(double), hipMemcpyDeviceToHost, st), "gradient readback");
+		check(hipStreamSynchronize(st), "adjoint");
+		for (std::uint32_t m = 0; m < materials; m++)
+		{
+			for (int c = 0; c < 3; c++)
+			{
+				if (d_albedo) d_albedo[3 * m + c] = g[m * kGradPerMaterial + c];
+				if (d_emission) d_emission[3 * m + c] = g[m * kGradPerMaterial + 4 + c];
+			}
+			if (d_roughness) d_roughness[m] = g[m * kGradPerMaterial + 3];
+		}
+	});
+}
+
+int	luzgpu_scene_set_materials(luzgpu_scene* scene, uint32_t count, const double* albedo, const double* roughness,
+	const double* emission)
+{
+	if (!scene || !scene->ctx)
+	{
+		g_lastError = "luzgpu_scene_set_materials: NULL scene or a host-only scene";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		PackedScene& h = scene->gpu.host;
+		if (count != h.materials.size())
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_scene_set_materials: count must equal the scene's material count ("
+				+ std::to_string(h.materials.size()) + ")"};
+		}
+		for (uint32_t i = 0; i < count; i++)
+		{
+			GpuMaterial& m = h.materials[i];
+			if (albedo && m.kind == GPU_MAT_LAMBERTIAN)
+			{
+				for (int c = 0; c < 3; c++)
+				{
+					h.materialColor[i][c] = albedo[3 * i + c];
+					m.color[c] = static_cast<float>(albedo[3 * i + c]);
+				}
+			}
+			if (roughness && m.kind == GPU_MAT_METAL)
+			{
+				const double r = roughness[i];
+				if (!std::isfinite(r) || r < 0.0 || r > 1.0)
+				{
+					throw Error{LUZGPU_E_INVALID, "Metal roughness must be finite and in [0,1]."};
+				}
+				m.roughness = static_cast<float>(r);
+				m.alpha = static_cast<float>(std::max(1e-4, r * r));
+			}
+			if (emission && m.kind == GPU_MAT_EMISSIVE)
+			{
+				for (int c = 0; c < 3; c++)
+				{
+					if (!std::isfinite(emission[3 * i + c]) || emission[3 * i + c] < 0.0)
+					{
+						throw Error{LUZGPU_E_INVALID, "Emission must be finite and non-negative."};
+					}
+					h.materialColor[i][c] = emission[3 * i + c];
+					m.color[c] = static_cast<float>(emission[3 * i + c]);
+				}
+			}
+		}
+		rebuildLights(h);
+		luzgpu_ctx* ctx = scene->ctx;
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		hipStream_t st = ctx->stream;
+		scene->gpu.materials.upload(h.materials, st);
+		scene->gpu.lights.upload(h.lights, st);
+		scene->gpu.lightCdf.upload(h.lightCdf, st);
+		check(hipStreamSynchronize(st), "material upload");
+		scene->gpu.view.materials = scene->gpu.materials.as<GpuMaterial>();
+		scene->gpu.view.lights = scene->gpu.lights.as<GpuLight>();
+		scene->gpu.view.lightCdf = h.lightCdf.empty() ? nullptr : scene->gpu.lightCdf.as<double>();
+		ctx->wave.dropGraph();
+		ctx->batchKey.clear();
 	});
 }
 

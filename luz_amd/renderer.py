@@ -121,6 +121,45 @@ class Scene:
                                                    None, None))
         return boxes, links, idx
 
+    def set_materials(self, albedo=None, roughness=None, emission=None) -> None:
+        """Per-material parameters (scene-file order): albedo (n, 3) for Lambertian, roughness
+        (n,) for metal, emission (n, 3) radiance for emitters; None keeps the current values."""
+        n = self.info["materials"]
+
+        def arr(x, shape):
+            if x is None:
+                return None
+            if hasattr(x, "detach"):
+                x = x.detach().cpu().numpy()
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(shape))
+            return a
+        a, r, e = arr(albedo, (n, 3)), arr(roughness, (n,)), arr(emission, (n, 3))
+        _lib.check(self._lib.luzgpu_scene_set_materials(self._h, n, a.ctypes.data if a is not None else None,
+                                                        r.ctypes.data if r is not None else None,
+                                                        e.ctypes.data if e is not None else None))
+
+    def adjoint(self, dl_dimage, seed: int, samples=None, max_bounces=None, sample_begin=0, sample_count=None,
+                resolution=None):
+        """Path-replay adjoint of Renderer.render with the same arguments. dl_dimage: dLoss/dImage
+        (H, W, 3), a numpy array or a device tensor. Returns (d_albedo (n,3), d_roughness (n,),
+        d_emission (n,3)) as float64 numpy arrays."""
+        n = self.info["materials"]
+        on_device = hasattr(dl_dimage, "data_ptr") and getattr(dl_dimage, "is_cuda", False)
+        s = _settings(seed, samples, max_bounces, sample_begin, sample_count, "megakernel", on_device=on_device,
+                      resolution=resolution)
+        if on_device:
+            g = dl_dimage.contiguous().float()
+            ptr = g.data_ptr()
+        else:
+            g = np.ascontiguousarray(np.asarray(dl_dimage, dtype=np.float32))
+            ptr = g.ctypes.data
+        da = np.zeros((n, 3), dtype=np.float64)
+        dr = np.zeros(n, dtype=np.float64)
+        de = np.zeros((n, 3), dtype=np.float64)
+        _lib.check(self._lib.luzgpu_adjoint(self.ctx.handle, self._h, ctypes.byref(s), ctypes.c_void_p(ptr), da.ctypes.data,
+                                            dr.ctypes.data, de.ctypes.data))
+        return da, dr, de
+
 
 class SceneFile:
     @staticmethod

@@ -27,6 +27,9 @@ Workloads
 * ``batch``        32 scenes of seeded displaced-sphere meshes with triangle counts
                    log-uniform in [1e4, 5e5], resolution drawn from {256^2, 512^2, 768x432,
                    1024^2}, spp uniform in {16..128}, depth 10, alternating sky / background.
+* ``diffcornell``  the differentiable Cornell-like box: 16 materials (12 Lambertian walls and
+                   box faces with albedo U[0.1, 0.9]^3, 4 GGX metal faces with roughness
+                   U[0.05, 0.8]) and one rectangle emitter; 512x512, 16 spp, depth 6.
 * ``terrain``      a 1,002,528-triangle displaced grid terrain inside a box of
                    Henyey-Greenstein fog (sigma_t 0.05, albedo 0.9, g 0.6) under a directional
                    light; 1280x720, 32 spp, depth 16.
@@ -563,6 +566,66 @@ def batch_scenes(directory: str, count=32, seed=1003, depth=10, spp_range=(16, 1
         out.append(Workload(f"batch{si:02d}", directory, path, width, height, spp, depth, si + 1, instances * per,
                             {"instances": instances}))
     return out
+
+
+# the 16 materials of the differentiable box: 5 walls, the 6 faces of the short box, the
+# top and bottom of the tall box (one material) and its 4 sides
+def diff_parameters(seed=1004, metals=True):
+    """Per-material parameters of the differentiable box: (kinds, albedo, roughness, emission).
+    kinds[i] in {"lambertian", "metal", "emissive"}; material 16 is the emitter."""
+    rng = random.Random(seed)
+    kinds, albedo, roughness = [], [], []
+    for i in range(16):
+        metal = metals and i in (6, 9, 12, 15)
+        kinds.append("metal" if metal else "lambertian")
+        albedo.append([rng.uniform(0.1, 0.9) for _ in range(3)])
+        roughness.append(rng.uniform(0.05, 0.8))
+    kinds.append("emissive")
+    albedo.append([0.0, 0.0, 0.0])
+    roughness.append(0.0)
+    emission = [[0.0, 0.0, 0.0]] * 16 + [[17.0, 12.0, 4.0]]
+    return kinds, albedo, roughness, emission
+
+
+def diff_scene_text(kinds, albedo, roughness, emission, width=512, height=512, spp=16, depth=6) -> str:
+    lines = ["[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+             "sky=none", "background=(0,0,0)", "", "[materials]"]
+    for i, k in enumerate(kinds):
+        if k == "lambertian":
+            lines += [f"material d{i} {{", "type=lambertian", f"color={_v(albedo[i])}", "}"]
+        elif k == "metal":
+            lines += [f"material d{i} {{", "type=metal", f"color={_v(albedo[i])}", f"roughness={_f(roughness[i])}", "}"]
+        else:
+            lum = sum(c * w for c, w in zip(emission[i], LUMINANCE_WEIGHTS))
+            lines += [f"material d{i} {{", "type=emissive", f"color={_v(emission[i])}", f"radiance={_f(lum)}", "}"]
+    lines += ["", "[scene]", "camera main {", "position=(278,273,-800)", "direction=(0,0,1)", "up=(0,1,0)",
+              "focal_length_mm=35", "sensor_width_mm=25", "sensor_height_mm=25", "pinhole=1", "focus_distance=10", "}",
+              "objects{"]
+    for wi, (_, _, q) in enumerate(_CORNELL_WALLS):
+        lines.append(f"triangle={_v(q[0])},{_v(q[1])},{_v(q[2])},material=d{wi}")
+        lines.append(f"triangle={_v(q[0])},{_v(q[2])},{_v(q[3])},material=d{wi}")
+    lines.append(f"rectangle=(278,554,279.5),(0,-1,0),130,105,material=d{len(kinds) - 1}")
+    faces = _box_quads(_SHORT_BOX_TOP)
+    for fi, q in enumerate(faces):
+        lines.append(f"triangle={_v(q[0])},{_v(q[1])},{_v(q[2])},material=d{5 + fi}")
+        lines.append(f"triangle={_v(q[0])},{_v(q[2])},{_v(q[3])},material=d{5 + fi}")
+    faces = _box_quads(_TALL_BOX_TOP)
+    for fi, q in enumerate(faces):
+        mat = 11 + max(0, fi - 1)  # top and bottom share material 11, the sides take 12..15
+        lines.append(f"triangle={_v(q[0])},{_v(q[1])},{_v(q[2])},material=d{mat}")
+        lines.append(f"triangle={_v(q[0])},{_v(q[2])},{_v(q[3])},material=d{mat}")
+    lines.append("}")
+    return "\n".join(lines) + "\n"
+
+
+def write_diff_cornell(directory: str, params=None, width=512, height=512, spp=16, depth=6, seed=1, metals=True,
+                       name="diffcornell.luz") -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    kinds, albedo, roughness, emission = params if params is not None else diff_parameters(metals=metals)
+    path = os.path.join(directory, name)
+    _write_if_changed(path, diff_scene_text(kinds, albedo, roughness, emission, width, height, spp, depth).encode())
+    return Workload("diffcornell", directory, path, width, height, spp, depth, seed, 34,
+                    {"kinds": kinds, "albedo": albedo, "roughness": roughness, "emission": emission})
 
 
 def _write_if_changed(path: str, data: bytes) -> None:

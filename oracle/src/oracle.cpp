@@ -609,6 +609,7 @@ namespace oracle
 		C	emitted;
 		double	area;
 		V	dir;	// directional: -direction
+		std::uint32_t	material = 0;
 	};
 
 	struct	Env
@@ -817,6 +818,7 @@ namespace oracle
 			const Top& t = s.top[i];
 			if (s.mats[t.material].kind != luzgpu::MAT_EMISSIVE) continue;
 			Light l{t.kind, t.index, s.mats[t.material].color, 0.0, V{}};
+			l.material = t.material;
 			const double lum = luminance(l.emitted);
 			double w;
 			if (t.kind == K_TRI || t.kind == K_RECT || t.kind == K_SPHERE)
@@ -1200,6 +1202,7 @@ namespace oracle
 		Onb	basis = Onb(V{0, 0, 1});
 		V	phaseDir;	// Henyey-Greenstein
 		double	g = 0.0;
+		bool	mirrorFallback = false;	// rough metal: the GGX sample reflected below the horizon
 	};
 
 	// ---- glossy / diffuse-glossy / principled (Glossy.cpp, DiffuseGlossy.cpp, Principled.cpp) -----
@@ -1477,9 +1480,17 @@ namespace oracle
 			float sx, sy;
 			ps.s2(D_BSDF_DIRECTION, sx, sy);
 			V half = ggxHalf(h.n, m.alpha, sx, sy);
-			if (dot(view, half) <= 0.0) half = h.n;
+			if (dot(view, half) <= 0.0)
+			{
+				half = h.n;
+				sc.mirrorFallback = true;
+			}
 			V dir = reflect(r.d, half);
-			if (dot(dir, h.n) <= 0.0) dir = reflect(r.d, h.n);
+			if (dot(dir, h.n) <= 0.0)
+			{
+				dir = reflect(r.d, h.n);
+				sc.mirrorFallback = true;
+			}
 			const double pdf = ggxReflPdf(h.n, view, dir, m.alpha);
 			const C f = metalBsdf(m, h.n, view, dir);
 			if (pdf <= 0.0 || !std::isfinite(pdf) || maxChannel(f) <= 0.0) return false;
@@ -1561,6 +1572,7 @@ namespace oracle
 		C	emitted;
 		double	pdf = 0, tMax = 0;
 		bool	valid = false;
+		std::int64_t	material = -1;
 	};
 
 	LS	sampleLight(const Scene& s, V origin, const Sampler& ps)	// ColorHelper.cpp:914-971
@@ -1584,6 +1596,7 @@ namespace oracle
 		const double sel = selectionProbability(s, idx);
 		if (sel <= 0.0 || !std::isfinite(sel)) return out;
 		const Light& L = s.lights[idx];
+		out.material = L.material;
 		if (L.kind == K_DIR)	// DirectionalLight::sampleLight (DirectionalLight.cpp:74-88)
 		{
 			out.dir = L.dir;
@@ -1854,7 +1867,8 @@ namespace oracle
 		return true;
 	}
 
-	C	direct(const Scene& s, const Mat& m, const Hit& h, const Scatter& sc, int b, const Sampler& ps)	// :973-1033
+	C	direct(const Scene& s, const Mat& m, const Hit& h, const Scatter& sc, int b, const Sampler& ps, std::int64_t* lightMaterial = nullptr,
+		V* dirOut = nullptr)	// :973-1033
 	{
 		if (s.lights.empty()) return {};
 		const double prob = b < 2 ? 1.0 : 0.5;
@@ -1866,10 +1880,12 @@ namespace oracle
 		if (sp <= 0.0 || !std::isfinite(sp) || maxChannel(f) <= 0.0) return {};
 		if (occluded(s, Ray::normalized(h.p, ls.dir), ls.tMax - T_MIN, &ps)) return {};
 		const double w = power(ls.pdf, sp);
+		if (lightMaterial) *lightMaterial = ls.material;
+		if (dirOut) *dirOut = ls.dir;
 		return f * ls.emitted * (w / (ls.pdf * prob));
 	}
 
-	C	environment(const Scene& s, const Mat& m, const Hit& h, const Scatter& sc, int b, const Sampler& ps)	// :1035-1098
+	C	environment(const Scene& s, const Mat& m, const Hit& h, const Scatter& sc, int b, const Sampler& ps, V* dirOut = nullptr)	// :1035-1098
 	{
 		const bool cb = hasConstBg(s);
 		if (cb && sc.type == PDF_BSDF) return {};
@@ -1883,8 +1899,181 @@ namespace oracle
 		const double w = power(ls.pdf, sp);
 		C c = f * ls.emitted * (w / (ls.pdf * prob));
 		if (cb) c = clampBg(s, c);
+		if (dirOut) *dirOut = ls.dir;
 		return c;
 	}
+
+	// ---- path-replay adjoint (detached sampling): per-vertex records resolved at the end --------
+	// Probability that the metal sampler (Metal.cpp:126-170) falls back to the mirror
+	// direction, for a view at cos mu and GGX alpha a, and its derivative in a. With the view
+	// at polar angle tv and a half vector at polar angle th, azimuth phi from the view, the
+	// fallback happens iff v.h <= cos(tv) / (2 cos th), i.e. cos(phi) <= -cot(tv) cot(2 th):
+	// never for th <= pi/4 - tv/2, always for th >= pi/4 + tv/2, and for a fraction
+	// 1 - acos(K)/pi of the azimuths in between. The sampler draws t = cos^2(th) with CDF
+	// C(t) = a^2 t / (1 + (a^2 - 1) t), so P = C(t2) + integral over the band of the fraction
+	// times the density; the band integral is Gauss-Legendre in s with th = mid + half sin(pi s / 2),
+	// which smooths the square-root behaviour of the fraction at the band edges.
+	struct	GaussLegendre
+	{
+		std::vector<double>	x, w;
+		explicit GaussLegendre(int n)
+		{
+			for (int i = 0; i < n; i++)
+			{
+				double z = std::cos(PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+				for (int it = 0; it < 100; it++)
+				{
+					double p0 = 1.0, p1 = z;
+					for (int k = 2; k <= n; k++)
+					{
+						const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+						p0 = p1;
+						p1 = p2;
+					}
+					dp = n * (z * p1 - p0) / (z * z - 1.0);
+					const double dz = p1 / dp;
+					z -= dz;
+					if (std::fabs(dz) < 1e-15) break;
+				}
+				x.push_back(z);
+				w.push_back(2.0 / ((1.0 - z * z) * dp * dp));
+			}
+		}
+	};
+	double	mirrorFallback(double mu, double a, double* dPda, const GaussLegendre& gl)
+	{
+		a = std::max(1e-4, a);
+		mu = std::min(1.0, std::max(0.0, mu));
+		const double tv = std::acos(mu), cotv = mu / std::sqrt(std::max(1e-300, 1.0 - mu * mu)), a2 = a * a;
+		const double th1 = 0.25 * PI - 0.5 * tv, th2 = 0.25 * PI + 0.5 * tv;
+		const double t2 = std::cos(th2) * std::cos(th2), d2 = 1.0 + (a2 - 1.0) * t2;
+		double P = a2 * t2 / d2, dP = 2.0 * a * t2 * (1.0 - t2) / (d2 * d2);
+		const double mid = 0.5 * (th1 + th2), half = 0.5 * (th2 - th1);
+		for (std::size_t i = 0; i < gl.x.size(); i++)
+		{
+			const double th = mid + half * std::sin(0.5 * PI * gl.x[i]);
+			const double jac = half * 0.5 * PI * std::cos(0.5 * PI * gl.x[i]);
+			const double t = std::cos(th) * std::cos(th), d = 1.0 + (a2 - 1.0) * t, s2 = std::sin(2.0 * th);
+			const double K = -cotv * std::cos(2.0 * th) / s2;
+			const double frac = 1.0 - std::acos(std::min(1.0, std::max(-1.0, K))) / PI;
+			P += gl.w[i] * jac * frac * (a2 / (d * d)) * s2;
+			dP += gl.w[i] * jac * frac * (2.0 * a * (1.0 - t - a2 * t) / (d * d * d)) * s2;
+		}
+		if (dPda) *dPda = dP;
+		return P;
+	}
+	double	mirrorFallback(double mu, double a, double* dPda)
+	{
+		static const GaussLegendre gl(64);
+		return mirrorFallback(mu, a, dPda, gl);
+	}
+	// d ln G1(c) / d alpha (Smith GGX, BSDF.cpp:220-236)
+	double	dlogG1(double c, double a)
+	{
+		c = clamp01(c);
+		a = std::max(1e-4, a);
+		const double sq = std::sqrt(a * a + (1.0 - a * a) * c * c);
+		const double d = -(a * (1.0 - c * c) / sq) / (c + sq);
+		return std::isfinite(d) ? d : 0.0;
+	}
+
+	// d ln(D G1(noV) G1(noL)) / d alpha of the GGX reflection lobe (BSDF.cpp:209-236)
+	double	ggxDlogAlpha(V n, V view, V l, double a)
+	{
+		a = std::max(1e-4, a);
+		const double noV = clamp01(dot(n, view)), noL = clamp01(dot(n, l));
+		const V hv = safeNormalize(view + l, n);
+		const double noH = clamp01(dot(n, hv));
+		const double a2 = a * a, den = noH * noH * (a2 - 1.0) + 1.0;
+		const double d = 2.0 / a - 4.0 * a * noH * noH / den + dlogG1(noV, a) + dlogG1(noL, a);
+		return std::isfinite(d) ? d : 0.0;
+	}
+
+	struct	Adjoint
+	{
+		C	weight;						// dLoss / dL of the sample
+		std::vector<double>*	grads;	// 7 per material: albedo rgb, roughness, emission rgb
+		const Scene*	scene;
+		struct	Rec { std::uint32_t m; bool rough; C value; double dlog; };
+		std::vector<Rec>	recs;
+		// after a mirror fallback: the next emitter's MIS weight w depends on the metal's alpha
+		// through the previous pdf; d ln w / dr = (1 - w) * pendingScale
+		std::int64_t	pendingMat = -1;
+		double	pendingScale = 0.0;
+
+		static bool	roughMetal(const Mat& m) { return m.kind == luzgpu::MAT_METAL && m.roughness > 1e-4; }
+		static double	dAlphaDr(const Mat& m) { const double r = clamp01(m.roughness); return r * r > 1e-4 ? 2.0 * r : 0.0; }
+		void	add(std::uint32_t m, int k, double v) { if (v != 0.0 && std::isfinite(v)) (*grads)[m * 7 + k] += v; }
+		double	wsum(C c) const { return weight.r * c.r + weight.g * c.g + weight.b * c.b; }
+		void	emission(std::uint32_t m, C inc, C le)
+		{
+			if (le.r > 0.0) add(m, 4, weight.r * inc.r / le.r);
+			if (le.g > 0.0) add(m, 5, weight.g * inc.g / le.g);
+			if (le.b > 0.0) add(m, 6, weight.b * inc.b / le.b);
+		}
+		// an emitter or sky reached by a BSDF-sampled ray with MIS weight w
+		void	arrival(C inc, double w)
+		{
+			if (pendingMat >= 0) add(static_cast<std::uint32_t>(pendingMat), 3, wsum(inc) * (1.0 - w) * pendingScale);
+			pendingMat = -1;
+		}
+		void	specular() { pendingMat = -1; }
+		void	vertex(const Mat& m, std::uint32_t mi, C acc)
+		{
+			pendingMat = -1;
+			if (m.kind == luzgpu::MAT_LAMBERTIAN) recs.push_back(Rec{mi, false, acc, 0.0});
+		}
+		void	connection(const Mat& m, std::uint32_t mi, const Hit& h, const Scatter& sc, V dir, C inc, std::int64_t lightMat)
+		{
+			if (lightMat >= 0) emission(static_cast<std::uint32_t>(lightMat), inc, scene->mats[static_cast<std::size_t>(lightMat)].color);
+			if (roughMetal(m))
+			{
+				const V view = safeNormalize(sc.incident * -1.0, h.n);
+				add(mi, 3, (weight.r * inc.r + weight.g * inc.g + weight.b * inc.b) * ggxDlogAlpha(h.n, view, dir, m.alpha) * dAlphaDr(m));
+			}
+		}
+		// The sampled continuation. A regular GGX sample differentiates f / p with p held: d ln f.
+		// A mirror fallback is a point mass of probability P(alpha) carrying f / p at the mirror
+		// direction (= F G1(v)^2, D cancels): its derivative is the score d ln P plus
+		// 2 d ln G1(v), and the next emitter's MIS weight moves with p(mirror) (arrival).
+		void	continuation(const Mat& m, std::uint32_t mi, const Hit& h, const Scatter& sc, V dir, C acc)
+		{
+			if (!roughMetal(m)) return;
+			const V view = safeNormalize(sc.incident * -1.0, h.n);
+			if (!sc.mirrorFallback)
+			{
+				recs.push_back(Rec{mi, true, acc, ggxDlogAlpha(h.n, view, dir, m.alpha) * dAlphaDr(m)});
+				return;
+			}
+			const double mu = clamp01(dot(h.n, view));
+			double dP = 0.0;
+			const double P = mirrorFallback(mu, m.alpha, &dP);
+			const double score = P > 0.0 ? dP / P : 0.0;
+			recs.push_back(Rec{mi, true, acc, (score + 2.0 * dlogG1(mu, m.alpha)) * dAlphaDr(m)});
+			const double a = std::max(1e-4, m.alpha);
+			const double noH = clamp01(dot(h.n, safeNormalize(view + dir, h.n)));
+			const double dlogD = 2.0 / a - 4.0 * a * noH * noH / (noH * noH * (a * a - 1.0) + 1.0);
+			pendingMat = mi;
+			pendingScale = 2.0 * dlogD * dAlphaDr(m);
+		}
+		void	finish(C L)
+		{
+			for (const Rec& r : recs)
+			{
+				const C rest = L + r.value * -1.0;
+				if (r.rough) add(r.m, 3, (weight.r * rest.r + weight.g * rest.g + weight.b * rest.b) * r.dlog);
+				else
+				{
+					const C rho = scene->mats[r.m].color;
+					if (rho.r > 1e-6) add(r.m, 0, weight.r * rest.r / rho.r);
+					if (rho.g > 1e-6) add(r.m, 1, weight.g * rest.g / rho.g);
+					if (rho.b > 1e-6) add(r.m, 2, weight.b * rest.b / rho.b);
+				}
+			}
+			recs.clear();
+			pendingMat = -1;
+		}
+	};
 
 	// henyeyGreensteinDirection (ColorHelper.cpp:115-148) for a non-zero phase direction
 	V	hgDirection(V phaseDir, double gIn, float sx, float sy)
@@ -1904,7 +2093,7 @@ namespace oracle
 		return Onb(pd).local(st * std::cos(phi), st * std::sin(phi), c);
 	}
 
-	C	trace(const Scene& s, Ray ray, Sampler& ps)	// calculateLightRaysColor, ColorHelper.cpp:1136-1326
+	C	trace(const Scene& s, Ray ray, Sampler& ps, Adjoint* adj = nullptr)	// calculateLightRaysColor, ColorHelper.cpp:1136-1326
 	{
 		C acc, thr = {1, 1, 1};
 		bool prevSpec = true, prevInf = false;
@@ -1927,7 +2116,9 @@ namespace oracle
 				else sky = s.background * w;
 				C c = thr * sky;
 				if (b > 0 && hasConstBg(s)) c = clampBg(s, c);
-				return acc + clampRay(c);
+				c = clampRay(c);
+				if (adj) adj->arrival(c, w);
+				return acc + c;
 			}
 			const Mat& m = s.mats[h.material];
 			const C emitted = m.kind == luzgpu::MAT_EMISSIVE ? m.color : C{};
@@ -1936,12 +2127,19 @@ namespace oracle
 			{
 				const double lp = (!prevSpec && !s.lights.empty()) ? lightsPdf(s, prevOrigin, ray.d) : 0.0;
 				const double w = prevSpec ? 1.0 : power(prevPdf, lp);
-				return acc + clampRay((thr * emitted) * w);
+				const C inc = clampRay((thr * emitted) * w);
+				if (adj)
+				{
+					adj->emission(h.material, inc, emitted);
+					adj->arrival(inc, w);
+				}
+				return acc + inc;
 			}
 			const C att = sc.att;
 			if (sc.specular)
 			{
 				if (!leavesOpaque(m, h, sc.dir)) return acc;
+				if (adj) adj->specular();
 				thr = clampRay(thr * att);
 				if (terminated(thr) || !roulette(thr, b, ps)) return acc;
 				ray = Ray::normalized(h.p, sc.dir);
@@ -1950,9 +2148,23 @@ namespace oracle
 				continue;
 			}
 			acc = acc + thr * emitted;
+			if (adj) adj->vertex(m, h.material, acc);
 			if (terminated(thr * att)) return acc;
-			if (!s.lights.empty()) acc = acc + clampRay(thr * direct(s, m, h, sc, b, ps));
-			if (sampledInf) acc = acc + clampRay(thr * environment(s, m, h, sc, b, ps));
+			if (!s.lights.empty())
+			{
+				std::int64_t lm = -1;
+				V ld;
+				const C inc = clampRay(thr * direct(s, m, h, sc, b, ps, &lm, &ld));
+				acc = acc + inc;
+				if (adj && maxChannel(inc) > 0.0) adj->connection(m, h.material, h, sc, ld, inc, lm);
+			}
+			if (sampledInf)
+			{
+				V ed;
+				const C inc = clampRay(thr * environment(s, m, h, sc, b, ps, &ed));
+				acc = acc + inc;
+				if (adj && maxChannel(inc) > 0.0) adj->connection(m, h.material, h, sc, ed, inc, -1);
+			}
 			V dir;
 			if (sc.type == PDF_COSINE)	// Sampler::cosineHemisphere (Sampler.cpp:239-246)
 			{
@@ -1987,6 +2199,7 @@ namespace oracle
 			if (!leavesOpaque(m, h, scattered.d)) return acc;
 			const double pdf = scatterPdf(m, h, sc, scattered.d);
 			if (pdf <= 0.0 || !std::isfinite(pdf)) return acc;
+			if (adj) adj->continuation(m, h.material, h, sc, scattered.d, acc);
 			thr = clampRay(thr * att);
 			if (terminated(thr) || !roulette(thr, b, ps)) return acc;
 			ray = scattered;
@@ -2123,6 +2336,64 @@ int	oracle_render(const char* path, std::uint32_t seed, std::uint32_t spp, std::
 	}
 }
 
+// Path-replay adjoint of oracle_render for dLoss/dImage = dldi (W*H*3): FP64 gradients per
+// material (7 each: albedo rgb, GGX roughness, emission rgb).
+int	oracle_adjoint(const char* path, std::uint32_t seed, std::uint32_t spp, std::uint32_t width, std::uint32_t height,
+	std::uint32_t max_bounces_plus1, std::uint32_t threads, const float* dldi, double* grads, std::uint32_t material_count)
+{
+	try
+	{
+		const luzgpu::Scene in = luzgpu::readSceneFile(path);
+		oracle::Scene s = oracle::build(in, width, height);
+		if (max_bounces_plus1 > 0) s.maxBounces = static_cast<int>(max_bounces_plus1) - 1;
+		if (material_count != s.mats.size()) throw std::runtime_error("oracle_adjoint: material count mismatch");
+		const std::size_t W = s.width, H = s.height, total = W * H;
+		std::atomic<std::size_t> next(0);
+		const std::uint32_t nt = std::max(1u, threads);
+		std::vector<std::vector<double>> partial(nt, std::vector<double>(s.mats.size() * 7, 0.0));
+		std::vector<std::thread> pool;
+		for (std::uint32_t t = 0; t < nt; t++)
+		{
+			pool.emplace_back([&, t]() {
+				oracle::Adjoint adj;
+				adj.grads = &partial[t];
+				adj.scene = &s;
+				while (true)
+				{
+					const std::size_t start = next.fetch_add(16);
+					if (start >= total) break;
+					for (std::size_t i = start; i < std::min(start + 16, total); i++)
+					{
+						const std::size_t x = i % W, y = i / W;
+						adj.weight = {dldi[3 * i] / static_cast<double>(spp), dldi[3 * i + 1] / static_cast<double>(spp),
+							dldi[3 * i + 2] / static_cast<double>(spp)};
+						for (std::uint32_t k = 0; k < spp; k++)
+						{
+							oracle::Sampler ps;
+							ps.begin(seed, x, y, k, 0);
+							const oracle::Ray r = oracle::cameraRay(s, x, y, ps);
+							adj.finish(oracle::trace(s, r, ps, &adj));
+						}
+					}
+				}
+			});
+		}
+		for (auto& th : pool) th.join();
+		for (std::size_t k = 0; k < s.mats.size() * 7; k++)
+		{
+			double v = 0.0;
+			for (const auto& p : partial) v += p[k];
+			grads[k] = v;
+		}
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		g_oracleError = e.what();
+		return -1;
+	}
+}
+
 // Primary-ray closest hit of sample 0 / stream 0 for every pixel: ids[2*i] = top-level
 // hittable, ids[2*i+1] = mesh triangle (or -1); t[i] (or -1).
 int	oracle_primary(const char* path, std::uint32_t seed, std::uint32_t width, std::uint32_t height, std::int64_t* ids, double* t)
@@ -2180,6 +2451,50 @@ double	oracle_ggx_ndf_integral(double alpha, std::uint32_t n)
 		sum += oracle::ggxD(z, alpha) * z;
 	}
 	return sum * (2.0 * oracle::PI) / static_cast<double>(n);
+}
+
+// d ln f / d alpha of the GGX reflection lobe used by the adjoint, for a view at cos mu_v and
+// a light at cos mu_l, azimuth phi apart; *fd receives the central difference (step h) of
+// ln of the oracle's own GGX reflection BSDF, the quantity it differentiates.
+double	oracle_ggx_dlog_alpha(double mu_v, double mu_l, double phi, double alpha, double h, double* fd)
+{
+	const oracle::V n = {0, 0, 1};
+	const oracle::V view = {std::sqrt(1.0 - mu_v * mu_v), 0.0, mu_v};
+	const double sl = std::sqrt(1.0 - mu_l * mu_l);
+	const oracle::V l = {sl * std::cos(phi), sl * std::sin(phi), mu_l};
+	const oracle::C one = {1.0, 1.0, 1.0};
+	const double fp = oracle::ggxReflBsdf(one, n, view, l, alpha + h).r;
+	const double fm = oracle::ggxReflBsdf(one, n, view, l, alpha - h).r;
+	*fd = (std::log(fp) - std::log(fm)) / (2.0 * h);
+	return oracle::ggxDlogAlpha(n, view, l, alpha);
+}
+
+// Mirror-fallback probability of the metal sampler (closed form + quadrature with `nodes`
+// Gauss-Legendre points, 0 = the adjoint's default) and its alpha derivative; *brute
+// receives the fallback frequency of the sampler itself over an n x 20n grid of (sx, sy).
+double	oracle_ggx_fallback(double mu, double alpha, std::uint32_t nodes, std::uint32_t n, double* dPda, double* brute)
+{
+	double P;
+	if (nodes == 0) P = oracle::mirrorFallback(mu, alpha, dPda);
+	else P = oracle::mirrorFallback(mu, alpha, dPda, oracle::GaussLegendre(static_cast<int>(nodes)));
+	if (n > 0)
+	{
+		const oracle::V nrm = {0, 0, 1};
+		const oracle::V view = {std::sqrt(1.0 - mu * mu), 0.0, mu};
+		std::size_t hits = 0;
+		for (std::uint32_t i = 0; i < n; i++)
+		{
+			for (std::uint32_t j = 0; j < 20 * n; j++)
+			{
+				const float sx = static_cast<float>((i + 0.5) / n), sy = static_cast<float>((j + 0.5) / (20.0 * n));
+				const oracle::V half = oracle::ggxHalf(nrm, alpha, sx, sy);
+				const oracle::V dir = oracle::reflect(view * -1.0, half);
+				if (oracle::dot(view, half) <= 0.0 || oracle::dot(dir, nrm) <= 0.0) hits++;
+			}
+		}
+		*brute = static_cast<double>(hits) / (20.0 * n * n);
+	}
+	return P;
 }
 
 double	oracle_dielectric_fresnel(double cos_theta, double eta_i, double eta_t)

@@ -97,6 +97,53 @@ def small_scene_fixtures(name, wl, tmp):
     primary_fixture(name, wl.scene_path, wl.seed, tmp)
 
 
+def fd_fixture(tmp, name="diff_fd_128_8spp_depth2", width=128, height=128, spp=8, depth=2, metals=False, h_albedo=0.01,
+               h_emission=0.1, seed=1):
+    """FP64 central finite differences of L = sum(w * image) on the reference renderer, with
+    the same seed on both sides, for every Lambertian albedo channel, metal roughness and
+    the emitter radiance of the differentiable box."""
+    kinds, albedo, roughness, emission = workloads.diff_parameters(metals=metals)
+    w = np.random.default_rng(7).uniform(-1.0, 1.0, (height, width, 3)).astype(np.float32)
+
+    def loss(a, r, e, tag):
+        d = os.path.join(tmp, "fd")
+        wl = workloads.write_diff_cornell(d, (kinds, a, r, e), width, height, spp, depth, name=f"{tag}.luz")
+        out = os.path.join(d, f"{tag}.bin")
+        run("render", wl.scene_path, seed, os.cpu_count() or 8, out)
+        img, _ = read_image(out)
+        return float((img * w.astype(np.float64)).sum()), img
+
+    base, img0 = loss(albedo, roughness, emission, "base")
+    n = len(kinds)
+    d_albedo = np.zeros((n, 3))
+    d_rough = np.zeros(n)
+    d_emission = np.zeros((n, 3))
+    for m, k in enumerate(kinds):
+        if k == "lambertian":
+            for c in range(3):
+                ap = [list(x) for x in albedo]
+                am = [list(x) for x in albedo]
+                ap[m][c] += h_albedo
+                am[m][c] -= h_albedo
+                d_albedo[m, c] = (loss(ap, roughness, emission, "p")[0] - loss(am, roughness, emission, "m")[0]) / (2 * h_albedo)
+        elif k == "metal":
+            rp, rm = list(roughness), list(roughness)
+            rp[m] += 0.01
+            rm[m] -= 0.01
+            d_rough[m] = (loss(albedo, rp, emission, "p")[0] - loss(albedo, rm, emission, "m")[0]) / 0.02
+        else:
+            for c in range(3):
+                ep = [list(x) for x in emission]
+                em = [list(x) for x in emission]
+                ep[m][c] += h_emission
+                em[m][c] -= h_emission
+                d_emission[m, c] = (loss(albedo, roughness, ep, "p")[0] - loss(albedo, roughness, em, "m")[0]) / (2 * h_emission)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), weights=w, d_albedo=d_albedo, d_roughness=d_rough,
+                        d_emission=d_emission, loss=base, image=img0.astype(np.float32), seed=seed, spp=spp, depth=depth,
+                        width=width, height=height, metals=metals)
+    print(name, "loss", base, "|d_albedo|", np.abs(d_albedo).sum(), "d_emission", d_emission[-1])
+
+
 def main():
     if not os.path.exists(LUZREF):
         sys.exit("build the reference harness first: make -C oracle ref")
@@ -110,11 +157,14 @@ def main():
             small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
         if "materials" in only:
             small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
+        if "fd" in only:
+            fd_fixture(tmp)
         return
     sampler_fixture(tmp)
     small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
     small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
     small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
+    fd_fixture(tmp)
 
     # Cornell box, 300x300, 50 spp, depth 8, seed 1 (the plain CPU-runnable config)
     c = workloads.write_cornell(os.path.join(tmp, "cornell"))

@@ -35,6 +35,12 @@ def load():
         lib.oracle_hg_pdf.restype = ctypes.c_double
         lib.oracle_hg_sample_cos.argtypes = [ctypes.c_double, ctypes.c_float, ctypes.c_float]
         lib.oracle_hg_sample_cos.restype = ctypes.c_double
+        lib.oracle_adjoint.argtypes = [ctypes.c_char_p] + [ctypes.c_uint32] * 6 + [vp, vp, ctypes.c_uint32]
+        lib.oracle_adjoint.restype = ctypes.c_int
+        lib.oracle_ggx_dlog_alpha.argtypes = [ctypes.c_double] * 5 + [vp]
+        lib.oracle_ggx_dlog_alpha.restype = ctypes.c_double
+        lib.oracle_ggx_fallback.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, vp, vp]
+        lib.oracle_ggx_fallback.restype = ctypes.c_double
         _lib = lib
     return _lib
 
@@ -68,3 +74,31 @@ def primary(path, seed, width, height):
     if rc != 0:
         raise RuntimeError(lib.oracle_last_error().decode())
     return ids, t
+
+
+def adjoint(path, seed, spp, width, height, weights, material_count, max_bounces=None, threads=None):
+    """Path-replay gradients of sum(weights * image): (material_count, 7) FP64, per material
+    albedo rgb, GGX roughness, emission rgb."""
+    lib = load()
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    assert w.shape == (height, width, 3)
+    grads = np.zeros((material_count, 7), dtype=np.float64)
+    threads = threads or min(16, os.cpu_count() or 1)
+    rc = lib.oracle_adjoint(path.encode(), seed, spp, width, height, 0 if max_bounces is None else max_bounces + 1,
+                            threads, w.ctypes.data, grads.ctypes.data, material_count)
+    if rc != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return grads
+
+
+def ggx_dlog_alpha(mu_v, mu_l, phi, alpha, h=1e-6):
+    fd = ctypes.c_double()
+    d = load().oracle_ggx_dlog_alpha(mu_v, mu_l, phi, alpha, h, ctypes.byref(fd))
+    return d, fd.value
+
+
+def ggx_fallback(mu, alpha, nodes=0, grid=0):
+    """(P, dP/dalpha, brute-force frequency or None) of the metal sampler's mirror fallback."""
+    dp, brute = ctypes.c_double(), ctypes.c_double()
+    p = load().oracle_ggx_fallback(mu, alpha, nodes, grid, ctypes.byref(dp), ctypes.byref(brute))
+    return p, dp.value, (brute.value if grid else None)
