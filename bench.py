@@ -1,6 +1,6 @@
 """Throughput benchmark of the radiance path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload spheres|cornell]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload spheres|cornell|batch|terrain|diff]
                     [--integrator megakernel|wavefront] [--no-cpu-baseline]
 
 A step is one full render of the workload at its configured resolution and spp
@@ -121,12 +121,78 @@ def bench_batch(args, wdir):
     ctx.close()
 
 
+def bench_diff(args, wdir):
+    """Differentiable box, 512x512, 16 materials, 16 spp, depth 6: one iteration = forward
+    render + L2 loss against a target rendered at perturbed parameters + path-replay adjoint
+    (through luz_amd.diff, a torch.autograd.Function). Reports the iteration time and its
+    ratio to the forward alone."""
+    import numpy as np
+    import torch
+    from luz_amd import Context, SceneFile, diff, workloads
+    kinds, albedo, roughness, emission = workloads.diff_parameters(metals=True)
+    wl = workloads.write_diff_cornell(os.path.join(wdir, "diff"), (kinds, albedo, roughness, emission), 512, 512, 16, 6)
+    ctx = Context(0)
+    scene = SceneFile.read(ctx, wl.scene_path)
+    rng = np.random.default_rng(args.seed)
+    ta = np.clip(np.array(albedo) * rng.uniform(0.8, 1.2, (len(kinds), 3)), 0.02, 0.98)
+    tr = np.clip(np.array(roughness) * rng.uniform(0.8, 1.2, len(kinds)), 0.02, 1.0)
+    dev = torch.device("cuda")
+    with torch.no_grad():
+        target = diff.render(scene, torch.tensor(ta), torch.tensor(tr), torch.tensor(emission), seed=args.seed + 1000)
+    a = torch.tensor(albedo, dtype=torch.float64, requires_grad=True)
+    r = torch.tensor(roughness, dtype=torch.float64, requires_grad=True)
+    e = torch.tensor(emission, dtype=torch.float64, requires_grad=True)
+
+    def forward():
+        with torch.no_grad():
+            return diff.render(scene, a, r, e, seed=args.seed)
+
+    def iteration():
+        a.grad = r.grad = e.grad = None
+        img = diff.render(scene, a, r, e, seed=args.seed)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        return loss
+
+    def timed(fn, label):
+        for i in range(args.warmup):
+            fn()
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            fn()
+        torch.cuda.synchronize(dev)
+        ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+        log(f"{label}: {ms:.2f} ms")
+        return ms
+
+    fwd_ms = timed(forward, "forward")
+    it_ms = timed(iteration, "forward + adjoint")
+    loss = iteration()
+    log(f"loss {loss.item():.6g}; |d albedo| {a.grad.abs().sum().item():.4g}, |d roughness| {r.grad.abs().sum().item():.4g}")
+    samples = 512 * 512 * 16
+    line = {
+        "metric": "fwd+adjoint ms/iter", "value": round(it_ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(it_ms, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "differentiable box: 512x512, 16 materials (12 Lambertian, 4 GGX metals) + 1 emitter, "
+                               "16 spp, depth 6, L2 loss against a target at perturbed parameters",
+                   "samples_per_iteration": samples},
+        "forward_ms": round(fwd_ms, 3), "iteration_over_forward": round(it_ms / fwd_ms, 3),
+        "forward_msamples_per_s": round(samples / fwd_ms / 1e3, 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    scene.release()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="spheres", choices=["spheres", "cornell", "batch", "terrain"])
+    ap.add_argument("--workload", default="spheres", choices=["spheres", "cornell", "batch", "terrain", "diff"])
     ap.add_argument("--integrator", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -146,6 +212,8 @@ def main():
     t0 = time.time()
     if args.workload == "batch":
         return bench_batch(args, wdir)
+    if args.workload == "diff":
+        return bench_diff(args, wdir)
     if args.workload == "terrain":
         wl = workloads.terrain_scene(os.path.join(wdir, "terrain"))
     elif args.workload == "spheres":

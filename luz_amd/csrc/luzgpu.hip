@@ -955,7 +955,8 @@ int	luzgpu_scene_set_materials(luzgpu_scene* scene, uint32_t count, const double
 		scene->gpu.view.materials = scene->gpu.materials.as<GpuMaterial>();
 		scene->gpu.view.lights = scene->gpu.lights.as<GpuLight>();
 		scene->gpu.view.lightCdf = h.lightCdf.empty() ? nullptr : scene->gpu.lightCdf.as<double>();
-		ctx->wave.dropGraph();
+		// same-sized uploads keep their buffers, so a captured pass graph (keyed on the
+		// scene's device pointers) stays valid; merged batch arrays hold copies: rebuild them
 		ctx->batchKey.clear();
 	});
 }

@@ -152,6 +152,61 @@ namespace
 		return (0);
 	}
 
+	// seeds SCENE FIRST COUNT THREADS OUT SPP W H: renders seeds FIRST..FIRST+COUNT-1 of one
+	// loaded scene; writes u32 count, u32 tiles (4), then per seed the per-channel means of
+	// each of 4 x 4 image tiles (row-major, f64).
+	int	cmdSeeds(int argc, char** argv)
+	{
+		if (argc < 10)
+		{
+			return (2);
+		}
+		Scene scene;
+		loadScene(scene, argv[2]);
+		const int first = std::stoi(argv[3]);
+		const std::uint32_t count = static_cast<std::uint32_t>(std::stoul(argv[4]));
+		scene.setRenderingThreads(static_cast<std::size_t>(std::stoul(argv[5])));
+		scene.setSampleCount(std::stoi(argv[7]));
+		std::ofstream out(argv[6], std::ios::binary);
+		const std::uint32_t tiles = 4;
+		put(out, count);
+		put(out, tiles);
+		for (std::uint32_t k = 0; k < count; k++)
+		{
+			scene.getImage()->setWidth(std::stoul(argv[8]));
+			scene.getImage()->setHeight(std::stoul(argv[9]));
+			scene.getImage()->initialize();
+			setRandomSeed(first + static_cast<int>(k));
+			if (!Renderer::render(scene))
+			{
+				return (1);
+			}
+			const auto& image = scene.getImage();
+			const std::size_t w = image->getWidth(), h = image->getHeight();
+			for (std::uint32_t ty = 0; ty < tiles; ty++)
+			{
+				for (std::uint32_t tx = 0; tx < tiles; tx++)
+				{
+					double sum[3] = {0.0, 0.0, 0.0};
+					std::size_t n = 0;
+					for (std::size_t y = ty * h / tiles; y < (ty + 1) * h / tiles; y++)
+					{
+						for (std::size_t x = tx * w / tiles; x < (tx + 1) * w / tiles; x++)
+						{
+							const Color c = image->pixels()[y * w + x];
+							sum[0] += c.getRed();
+							sum[1] += c.getGreen();
+							sum[2] += c.getBlue();
+							n++;
+						}
+					}
+					for (double v : sum) put(out, v / static_cast<double>(n));
+				}
+			}
+		}
+		return (0);
+	}
+
 	// record: u32 seed, x, y, sample, stream, bounce, dim; f64 s1; f32 s2x, s2y
 	int	cmdSampler(int argc, char** argv)
 	{
@@ -522,7 +577,7 @@ int	main(int argc, char** argv)
 {
 	if (argc < 2)
 	{
-		std::cerr << "usage: luzref render|sampler|blas|tlas|primary|env|lights ..." << std::endl;
+		std::cerr << "usage: luzref render|seeds|sampler|blas|tlas|primary|env|lights ..." << std::endl;
 		return (2);
 	}
 	try
@@ -535,6 +590,7 @@ int	main(int argc, char** argv)
 		if (cmd == "primary") return (cmdPrimary(argc, argv));
 		if (cmd == "env") return (cmdEnv(argc, argv));
 		if (cmd == "lights") return (cmdLights(argc, argv));
+		if (cmd == "seeds") return (cmdSeeds(argc, argv));
 	}
 	catch (const std::exception& e)
 	{

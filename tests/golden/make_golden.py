@@ -97,6 +97,34 @@ def small_scene_fixtures(name, wl, tmp):
     primary_fixture(name, wl.scene_path, wl.seed, tmp)
 
 
+ZSTAT_SEEDS = (1001, 64)  # reference seeds of the bias fixtures; device renders use 1..64
+
+
+def zstat_configs(tmp):
+    """(name, workload, spp, width, height) of the per-seed bias fixtures: every small scene
+    family at low spp, so 64 reference renders take seconds."""
+    return [
+        ("cornell", workloads.write_cornell(os.path.join(tmp, "z_cornell")), 4, 300, 300),
+        ("spheres", workloads.spheres_scene(os.path.join(tmp, "z_spheres")), 2, 192, 108),
+        ("primitives", workloads.write_primitives(os.path.join(tmp, "z_primitives")), 4, 160, 120),
+        ("materials", workloads.write_materials(os.path.join(tmp, "z_materials")), 4, 160, 120),
+        ("volume", workloads.write_volume(os.path.join(tmp, "z_volume")), 4, 160, 120),
+    ]
+
+
+def zstat_fixtures(tmp):
+    """Per-seed tile means of reference renders at 64 independent seeds (4 x 4 tiles, RGB)."""
+    first, count = ZSTAT_SEEDS
+    for name, wl, spp, w, h in zstat_configs(tmp):
+        out = os.path.join(tmp, f"z_{name}.bin")
+        run("seeds", wl.scene_path, first, count, os.cpu_count() or 8, out, spp, w, h)
+        head = np.fromfile(out, dtype="<u4", count=2)
+        means = np.fromfile(out, dtype="<f8", offset=8).reshape(int(head[0]), int(head[1]) ** 2, 3)
+        np.savez_compressed(os.path.join(HERE, f"zstats_{name}.npz"), tile_means=means, first_seed=first, spp=spp,
+                            width=w, height=h)
+        print("zstats", name, means.shape, means.mean((0, 1)))
+
+
 def fd_fixture(tmp, name="diff_fd_128_8spp_depth2", width=128, height=128, spp=8, depth=2, metals=False, h_albedo=0.01,
                h_emission=0.1, seed=1):
     """FP64 central finite differences of L = sum(w * image) on the reference renderer, with
@@ -159,12 +187,15 @@ def main():
             small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
         if "fd" in only:
             fd_fixture(tmp)
+        if "zstats" in only:
+            zstat_fixtures(tmp)
         return
     sampler_fixture(tmp)
     small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
     small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
     small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
     fd_fixture(tmp)
+    zstat_fixtures(tmp)
 
     # Cornell box, 300x300, 50 spp, depth 8, seed 1 (the plain CPU-runnable config)
     c = workloads.write_cornell(os.path.join(tmp, "cornell"))

@@ -122,3 +122,14 @@ def test_henyey_greenstein_closed_forms():
             sel = (mu >= edges[k]) & (mu <= edges[k + 1])
             mass = 2.0 * np.pi * np.trapezoid(p[sel], mu[sel])
             assert abs(hist[k] / n - mass) < 5e-3
+
+
+def test_bias_fixture_tiles_match_oracle(cornell):
+    """The 64-seed bias fixture's first seeds, recomputed by the oracle with the tests' tile
+    code, equal the reference harness's tile means (pins the fixture and the tiling)."""
+    from bias import tile_means
+    g = np.load(os.path.join(GOLDEN, "zstats_cornell.npz"))
+    for k in range(2):
+        img, _ = oracle_lib.render(cornell.scene_path, int(g["first_seed"]) + k, int(g["spp"]), int(g["width"]),
+                                   int(g["height"]))
+        assert np.allclose(tile_means(img), g["tile_means"][k], rtol=1e-12, atol=1e-15)
