@@ -91,6 +91,14 @@ def load() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run luz_amd/build.py (or __graft_entry__.build())")
+    # torch ships its own HIP runtime under this library's HIP soname (libamdhip64.so.7).
+    # Importing torch first makes the loader bind this library to that runtime, so the
+    # process holds one HIP/HSA runtime; loaded the other way round there are two, and the
+    # one initialised second finds no devices.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     vp = ctypes.c_void_p
     lib.luzgpu_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]

@@ -238,6 +238,7 @@ namespace luzgpu
 		ps.begin(p.renderSeed, x, y, sample, 0);
 		Ray ray = cameraRay(p, x, y, ps);
 		f3 acc = mk(0, 0, 0), thr = mk(1, 1, 1), prevOrigin = mk(0, 0, 0);
+		RayOrigin from = kNoOrigin;
 		bool prevSpecular = true, prevInfinite = false;
 		float prevPdf = 0.0f;
 		const bool sampledInfinite = hasEnvLight(p) || hasConstantBackgroundLight(p);
@@ -247,7 +248,7 @@ namespace luzgpu
 			const float logU = volumeLogU(p, ps);
 			HitInfo h;
 			closestRays++;
-			if (!traverse<false, Count, Feat>(s, p, ray, kTMinF, INFINITY, h, stack, &visits, logU))
+			if (!traverse<false, Count, Feat>(s, p, ray, kTMinF, INFINITY, h, stack, &visits, logU, from))
 			{
 				float w = 1.0f;
 				if (b > 0 && !prevSpecular && prevInfinite)
@@ -261,6 +262,7 @@ namespace luzgpu
 				return acc + c;
 			}
 			const Surface sf = surfaceAt<Feat>(s, ray, h);
+			from = originOf(s, h);
 			const GpuMaterial& m = s.materials[sf.material];
 			const f3 emitted = m.kind == GPU_MAT_EMISSIVE ? ld3(m.color) : mk(0, 0, 0);
 			Scatter sc;
@@ -310,7 +312,7 @@ namespace luzgpu
 						HitInfo sh;
 						shadowRays++;
 						occluded = traverse<true, Count, Feat>(s, p, makeRay(sf.position, c.direction), kTMinF, c.tMax, sh, stack, &visits,
-							logU);
+							logU, from);
 					}
 					if (!occluded)
 					{
@@ -327,7 +329,8 @@ namespace luzgpu
 				{
 					HitInfo sh;
 					shadowRays++;
-					if (!traverse<true, Count, Feat>(s, p, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack, &visits, logU))
+					if (!traverse<true, Count, Feat>(s, p, makeRay(sf.position, c.direction), kTMinF, INFINITY, sh, stack, &visits, logU,
+						from))
 					{
 						const f3 inc = clampRay(thr * c.contribution);
 						acc = acc + inc;

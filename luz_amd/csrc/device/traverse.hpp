@@ -280,6 +280,18 @@ namespace luzgpu
 		std::int32_t	tri;		// mesh triangle slot (global) for instances, -1 otherwise
 	};
 
+	// The primitive a ray leaves from. A planar one (triangle, rectangle, cube face, plane) is
+	// left out of the ray's traversal: a ray cannot meet the plane it starts on again, and
+	// the FP64 reference never does, but an fp32 hit position may sit a little behind that
+	// plane, so a grazing ray would meet it again beyond T_MIN. Curved and volume hits, and
+	// camera rays, exclude nothing (hittable 0xffffffff).
+	struct	RayOrigin
+	{
+		std::uint32_t	hittable;
+		std::int32_t	tri;		// instance: mesh triangle slot; cube: face; else -1
+	};
+	constexpr RayOrigin	kNoOrigin{0xffffffffu, -1};
+
 	// Per-thread traversal stack in LDS, laid out [depth][thread] so the 64 lanes of a
 	// wave touch 64 consecutive banks.
 	template <int Depth, int Block>
@@ -344,6 +356,7 @@ namespace luzgpu
 		std::uint32_t	unbounded;	// this ray's scene: unbounded list base << 16 | count
 		std::uint32_t	entry;
 		std::uint32_t	instHittable;
+		RayOrigin	from;	// planar primitive the ray leaves from
 		bool	inInstance;
 		bool	found;
 	};
@@ -351,10 +364,11 @@ namespace luzgpu
 	// Starts a traversal; returns false when there is nothing to traverse.
 	template <typename Stack>
 	__device__ __forceinline__ bool	traceBegin(TraceState& st, const SceneView& s, std::int32_t tlasRoot, std::uint32_t unbounded,
-		const Ray& worldRay, float tMax, float logU, Stack& stack)
+		const Ray& worldRay, float tMax, float logU, RayOrigin from, Stack& stack)
 	{
 		const Bvh4Node* tlas = s.tlas;
 		st.logU = logU;
+		st.from = from;
 		st.unbounded = unbounded;
 		st.world = worldRay;
 		st.ray = worldRay;
@@ -458,6 +472,7 @@ namespace luzgpu
 				if (Count) counter->prims++;
 				if (st.inInstance)
 				{
+					if (start + i == st.from.tri && st.instHittable == st.from.hittable) continue;
 					const GpuTri& tri = s.meshTris[start + i];
 					float t, u, v;
 					if (triHit(tri, st.ray, tMin, st.hit.t, t, u, v))
@@ -477,7 +492,7 @@ namespace luzgpu
 				if (kind == GPU_HIT_TRIANGLE)
 				{
 					float t, u, v;
-					if (triHit(s.topTris[idx], st.ray, tMin, st.hit.t, t, u, v))
+					if (h != st.from.hittable && triHit(s.topTris[idx], st.ray, tMin, st.hit.t, t, u, v))
 					{
 						st.found = true;
 						st.hit.t = t; st.hit.u = u; st.hit.v = v;
@@ -489,7 +504,7 @@ namespace luzgpu
 				else if (kind == GPU_HIT_RECTANGLE)
 				{
 					float t;
-					if (rectHit(s.rects[idx], st.ray, tMin, st.hit.t, t))
+					if (h != st.from.hittable && rectHit(s.rects[idx], st.ray, tMin, st.hit.t, t))
 					{
 						st.found = true;
 						st.hit.t = t;
@@ -516,7 +531,7 @@ namespace luzgpu
 					for (int f = 0; f < 6; f++)
 					{
 						float t;
-						if (rectHit(s.cubeFaces[6 * idx + f], st.ray, tMin, st.hit.t, t))
+						if ((h != st.from.hittable || f != st.from.tri) && rectHit(s.cubeFaces[6 * idx + f], st.ray, tMin, st.hit.t, t))
 						{
 							st.found = true;
 							st.hit.t = t;
@@ -567,7 +582,7 @@ namespace luzgpu
 				const std::uint32_t h = s.unbounded[ubBase + i];
 				const GpuPlane& pl = s.planes[s.hittables[h].kindIndex & kIndexMask];
 				float t;
-				if (planeHit(pl, st.world, tMin, st.hit.t, t))
+				if (h != st.from.hittable && planeHit(pl, st.world, tMin, st.hit.t, t))
 				{
 					st.found = true;
 					st.hit.t = t;
@@ -587,12 +602,21 @@ namespace luzgpu
 		return (p.unboundedBase << 16) | p.unboundedCount;
 	}
 
+	// The exclusion identity of a hit (see RayOrigin).
+	__device__ __forceinline__ RayOrigin	originOf(const SceneView& s, const HitInfo& h)
+	{
+		const std::uint32_t kind = s.hittables[h.hittable].kindIndex >> kKindShift;
+		const bool planar = kind == GPU_HIT_TRIANGLE || kind == GPU_HIT_RECTANGLE || kind == GPU_HIT_INSTANCE || kind == GPU_HIT_CUBE
+			|| kind == GPU_HIT_PLANE;
+		return planar ? RayOrigin{h.hittable, h.tri} : kNoOrigin;
+	}
+
 	template <bool AnyHit, bool Count = false, int Feat = kFeatAll, typename Stack>
 	__device__ __forceinline__ bool	traverse(const SceneView& s, const GpuParams& p, const Ray& worldRay, float tMin, float tMax,
-		HitInfo& hit, Stack& stack, VisitCounter* counter = nullptr, float logU = 0.0f)
+		HitInfo& hit, Stack& stack, VisitCounter* counter = nullptr, float logU = 0.0f, RayOrigin from = kNoOrigin)
 	{
 		TraceState st;
-		if (traceBegin(st, s, p.tlasRoot, unboundedOf(p), worldRay, tMax, logU, stack))
+		if (traceBegin(st, s, p.tlasRoot, unboundedOf(p), worldRay, tMax, logU, from, stack))
 		{
 			while (!traceStep<AnyHit, Count, Feat>(s, st, tMin, stack, counter))
 			{

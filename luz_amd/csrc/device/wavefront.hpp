@@ -50,7 +50,8 @@ namespace luzgpu
 	{
 		// queue-ordered (double-buffered by bounce parity)
 		float4*	qOrigin[2];
-		float4*	qDir[2];
+		float4*	qDir[2];		// xyz, w = hittable the ray leaves from (bits)
+		std::int32_t*	qFrom[2];	// its triangle slot / cube face (RayOrigin::tri)
 		std::uint32_t*	qPath[2];
 		float4*	qHit;		// t, u, v, hittable (bits)
 		std::int32_t*	qTri;
@@ -64,6 +65,7 @@ namespace luzgpu
 		float4*	sOrigin;	// xyz, w = tMax
 		float4*	sDir;		// xyz, w = scene << 25 | (2 * path + slot) (bits)
 		float*	sLogU;		// volume free-flight sample of the shadow ray (batches with volumes)
+		int2*	sFrom;		// RayOrigin of the shadow ray (hittable bits, tri)
 		// counters: [b] rays in queue b, [kMaxBounceSlots + b] shadow rays of bounce b,
 		// [2 * kMaxBounceSlots + b] / [3 * ...] extend / shadow work claims
 		std::uint32_t*	counts;
@@ -155,7 +157,8 @@ namespace luzgpu
 			const std::uint32_t g = base + id;
 			const float logU = p.hasVolumes ? volumeLogU(p, ps) : 0.0f;
 			w.qOrigin[0][g] = make_float4(r.o.x, r.o.y, r.o.z, logU);
-			w.qDir[0][g] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+			w.qDir[0][g] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(kNoOrigin.hittable));
+			w.qFrom[0][g] = kNoOrigin.tri;
 			w.qPath[0][g] = (job << kPathBits) | g;
 			w.throughput[g] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
 			w.radiance[g] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(PF_PREV_SPECULAR));
@@ -268,9 +271,10 @@ namespace luzgpu
 					slot = k;
 					Ray r;
 					float tMax, logU;
-					const std::uint32_t job = src.load(slot, r, tMax, logU);
+					RayOrigin from;
+					const std::uint32_t job = src.load(slot, r, tMax, logU, from);
 					const GpuParams& jp = jobs[job].params;
-					active = traceBegin(st, s, jp.tlasRoot, unboundedOf(jp), r, tMax, logU, stack);
+					active = traceBegin(st, s, jp.tlasRoot, unboundedOf(jp), r, tMax, logU, from, stack);
 					if (!active)
 					{
 						src.store(slot, st);
@@ -299,12 +303,13 @@ namespace luzgpu
 		std::uint32_t*	spill;
 		__device__ __forceinline__ std::uint32_t	count() const { return w.counts[b]; }
 		// returns the ray's scene job
-		__device__ __forceinline__ std::uint32_t	load(std::uint32_t i, Ray& r, float& tMax, float& logU) const
+		__device__ __forceinline__ std::uint32_t	load(std::uint32_t i, Ray& r, float& tMax, float& logU, RayOrigin& from) const
 		{
 			const float4 o = w.qOrigin[b & 1][i], d = w.qDir[b & 1][i];
 			r = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
 			tMax = INFINITY;
 			logU = o.w;
+			from = RayOrigin{__float_as_uint(d.w), w.qFrom[b & 1][i]};
 			return w.qPath[b & 1][i] >> kPathBits;
 		}
 		__device__ __forceinline__ void	store(std::uint32_t i, const TraceState& st) const
@@ -322,7 +327,7 @@ namespace luzgpu
 		std::uint32_t	volumes;	// the batch has volumes: shadow rays carry a sample
 		std::uint32_t	target;	// connection slot of the ray being traced
 		__device__ __forceinline__ std::uint32_t	count() const { return w.counts[kMaxBounceSlots + b]; }
-		__device__ __forceinline__ std::uint32_t	load(std::uint32_t i, Ray& r, float& tMax, float& logU)
+		__device__ __forceinline__ std::uint32_t	load(std::uint32_t i, Ray& r, float& tMax, float& logU, RayOrigin& from)
 		{
 			const float4 o = w.sOrigin[i], d = w.sDir[i];
 			r = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
@@ -330,6 +335,8 @@ namespace luzgpu
 			const std::uint32_t tag = __float_as_uint(d.w);
 			target = tag & kTargetMask;
 			logU = volumes ? w.sLogU[i] : 0.0f;
+			const int2 f = w.sFrom[i];
+			from = RayOrigin{static_cast<std::uint32_t>(f.x), f.y};
 			return tag >> kTargetBits;
 		}
 		__device__ __forceinline__ void	store(std::uint32_t, const TraceState& st) const
@@ -366,6 +373,7 @@ namespace luzgpu
 			Ray next{};
 			Connection cl{}, ce{};
 			f3 shadePoint = mk(0, 0, 0);
+			RayOrigin from = kNoOrigin;
 			const std::uint32_t qp = valid ? w.qPath[q][i] : 0u;
 			const std::uint32_t job = qp >> kPathBits;
 			const std::uint32_t path = qp & kPathMask;
@@ -421,6 +429,7 @@ namespace luzgpu
 					h.tri = w.qTri[i];
 					const Surface sf = surfaceAt<Feat>(s, ray, h);
 					shadePoint = sf.position;
+					from = originOf(s, h);
 					const GpuMaterial& m = s.materials[sf.material];
 					const f3 emitted = m.kind == GPU_MAT_EMISSIVE ? ld3(m.color) : mk(0, 0, 0);
 					Scatter sc;
@@ -533,7 +542,8 @@ namespace luzgpu
 			{
 				const int nq = (b + 1) & 1;
 				w.qOrigin[nq][pos] = make_float4(next.o.x, next.o.y, next.o.z, nextLogU);
-				w.qDir[nq][pos] = make_float4(next.d.x, next.d.y, next.d.z, 0.0f);
+				w.qDir[nq][pos] = make_float4(next.d.x, next.d.y, next.d.z, __uint_as_float(from.hittable));
+				w.qFrom[nq][pos] = from.tri;
 				w.qPath[nq][pos] = qp;
 			}
 			const std::uint32_t tagBase = job << kTargetBits;
@@ -542,6 +552,7 @@ namespace luzgpu
 			{
 				w.sOrigin[sl] = make_float4(shadePoint.x, shadePoint.y, shadePoint.z, cl.tMax);
 				w.sDir[sl] = make_float4(cl.direction.x, cl.direction.y, cl.direction.z, __uint_as_float(tagBase | (2u * path)));
+				w.sFrom[sl] = make_int2(static_cast<int>(from.hittable), from.tri);
 				if (volumes) w.sLogU[sl] = shadowLogU;
 			}
 			const std::uint32_t se = waveAppend(wantEnv, shadowCount);
@@ -549,6 +560,7 @@ namespace luzgpu
 			{
 				w.sOrigin[se] = make_float4(shadePoint.x, shadePoint.y, shadePoint.z, INFINITY);
 				w.sDir[se] = make_float4(ce.direction.x, ce.direction.y, ce.direction.z, __uint_as_float(tagBase | (2u * path + 1u)));
+				w.sFrom[se] = make_int2(static_cast<int>(from.hittable), from.tri);
 				if (volumes) w.sLogU[se] = shadowLogU;
 			}
 		}
@@ -703,6 +715,48 @@ namespace luzgpu
 			bytes = 0;
 		}
 
+		// Carves the state arrays for `capacity` paths out of `base` (each 256-byte aligned)
+		// and returns the bytes used; with base == nullptr it only measures.
+		size_t	carve(char* base, std::uint32_t capacity)
+		{
+			const size_t n = capacity;
+			const size_t spillBytes = static_cast<size_t>(kWaveStackCapacity - kWaveLdsStack) * kWaveGrid * kWaveBlock * 4;
+			size_t off = 0;
+			auto take = [&](size_t b) { char* r = base ? base + off : nullptr; off += (b + 255) & ~size_t(255); return r; };
+			WaveBuffers nb;
+			std::memset(&nb, 0, sizeof(nb));	// padding too: graph keys compare the raw bytes
+			for (int k = 0; k < 2; k++)
+			{
+				nb.qOrigin[k] = reinterpret_cast<float4*>(take(16 * n));
+				nb.qDir[k] = reinterpret_cast<float4*>(take(16 * n));
+				nb.qFrom[k] = reinterpret_cast<std::int32_t*>(take(4 * n));
+				nb.qPath[k] = reinterpret_cast<std::uint32_t*>(take(4 * n));
+			}
+			nb.qHit = reinterpret_cast<float4*>(take(16 * n));
+			nb.qTri = reinterpret_cast<std::int32_t*>(take(4 * n));
+			nb.throughput = reinterpret_cast<float4*>(take(16 * n));
+			nb.radiance = reinterpret_cast<float4*>(take(16 * n));
+			nb.prevOrigin = reinterpret_cast<float4*>(take(16 * n));
+			nb.connection = reinterpret_cast<float4*>(take(32 * n));
+			nb.visible = reinterpret_cast<std::uint8_t*>(take(2 * n));
+			nb.sOrigin = reinterpret_cast<float4*>(take(32 * n));
+			nb.sDir = reinterpret_cast<float4*>(take(32 * n));
+			nb.sLogU = reinterpret_cast<float*>(take(8 * n));
+			nb.sFrom = reinterpret_cast<int2*>(take(16 * n));
+			float* nc = reinterpret_cast<float*>(take(12 * n));
+			nb.counts = reinterpret_cast<std::uint32_t*>(take(kMaxBounceSlots * 4 * 4));
+			PassJob* nj = reinterpret_cast<PassJob*>(take(sizeof(PassJob) * kMaxBatchScenes));
+			nb.spill = reinterpret_cast<std::uint32_t*>(take(spillBytes));
+			nb.capacity = capacity;
+			if (base)
+			{
+				buf = nb;
+				colors = nc;
+				jobs = nj;
+			}
+			return off;
+		}
+
 		// The layout is fixed between reallocations (captured graphs hold its pointers):
 		// a smaller request keeps the larger layout.
 		void	ensure(std::uint32_t capacity)
@@ -711,10 +765,7 @@ namespace luzgpu
 			{
 				return;
 			}
-			const size_t n = capacity;
-			const size_t spillBytes = static_cast<size_t>(kWaveStackCapacity - kWaveLdsStack) * kWaveGrid * kWaveBlock * 4;
-			const size_t need = n * (16 * 2 * 2 + 4 * 2 + 16 + 4 + 16 * 3 + 32 + 2 + 32 * 2 + 8 + 12) + kMaxBounceSlots * 4 * 4
-				+ sizeof(PassJob) * kMaxBatchScenes + 64 * 20 + spillBytes + 256;
+			const size_t need = carve(nullptr, capacity);
 			if (need > bytes)
 			{
 				dropGraph();
@@ -728,29 +779,7 @@ namespace luzgpu
 				}
 				bytes = need;
 			}
-			char* c = static_cast<char*>(mem);
-			auto take = [&](size_t b) { char* r = c; c += (b + 255) & ~size_t(255); return r; };
-			for (int k = 0; k < 2; k++)
-			{
-				buf.qOrigin[k] = reinterpret_cast<float4*>(take(16 * n));
-				buf.qDir[k] = reinterpret_cast<float4*>(take(16 * n));
-				buf.qPath[k] = reinterpret_cast<std::uint32_t*>(take(4 * n));
-			}
-			buf.qHit = reinterpret_cast<float4*>(take(16 * n));
-			buf.qTri = reinterpret_cast<std::int32_t*>(take(4 * n));
-			buf.throughput = reinterpret_cast<float4*>(take(16 * n));
-			buf.radiance = reinterpret_cast<float4*>(take(16 * n));
-			buf.prevOrigin = reinterpret_cast<float4*>(take(16 * n));
-			buf.connection = reinterpret_cast<float4*>(take(32 * n));
-			buf.visible = reinterpret_cast<std::uint8_t*>(take(2 * n));
-			buf.sOrigin = reinterpret_cast<float4*>(take(32 * n));
-			buf.sDir = reinterpret_cast<float4*>(take(32 * n));
-			buf.sLogU = reinterpret_cast<float*>(take(8 * n));
-			colors = reinterpret_cast<float*>(take(12 * n));
-			buf.counts = reinterpret_cast<std::uint32_t*>(take(kMaxBounceSlots * 4 * 4));
-			jobs = reinterpret_cast<PassJob*>(take(sizeof(PassJob) * kMaxBatchScenes));
-			buf.spill = reinterpret_cast<std::uint32_t*>(take(spillBytes));
-			buf.capacity = capacity;
+			carve(static_cast<char*>(mem), capacity);
 		}
 
 		void	ensurePlans(size_t entries)

@@ -17,6 +17,7 @@
 #include "host/accel.hpp"
 
 #include <atomic>
+#include <unordered_set>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -294,6 +295,16 @@ namespace luzgpu
 			lights, lightCdf, envPixels, envPdf, envCdf, unbounded, spheres, cubeFaces, volumes, planes;
 		SceneView	view{};
 		luzgpu_scene_info	info{};
+
+		void	releaseDevice()
+		{
+			for (DeviceBuffer* b : {&tlas, &blas, &topTris, &meshTris, &meshNormals, &tlasPrims, &hittables, &rects, &instances,
+				&materials, &lights, &lightCdf, &envPixels, &envPdf, &envCdf, &unbounded, &spheres, &cubeFaces, &volumes, &planes})
+			{
+				b->release();
+			}
+			view = SceneView{};
+		}
 	};
 }
 
@@ -312,6 +323,7 @@ struct luzgpu_ctx
 	std::unique_ptr<luzgpu::GpuScene>	batch;
 	std::vector<luzgpu::GpuParams>	batchParams;
 	luzgpu::DeviceBuffer	batchSums;
+	std::unordered_set<luzgpu_scene*>	scenes;	// loaded on this context and not yet released
 };
 
 struct luzgpu_scene
@@ -480,6 +492,14 @@ void	luzgpu_destroy(luzgpu_ctx* ctx)
 	if (!ctx) return;
 	(void)hipSetDevice(ctx->device);
 	(void)hipStreamSynchronize(ctx->stream);
+	// scenes still loaded lose their device arrays now, while the device is current, and
+	// remain valid handles for host-side queries and luzgpu_scene_release
+	for (luzgpu_scene* sc : ctx->scenes)
+	{
+		sc->gpu.releaseDevice();
+		sc->ctx = nullptr;
+	}
+	ctx->scenes.clear();
 	ctx->wave.release();
 	ctx->batch.reset();
 	ctx->batchSums.release();
@@ -544,6 +564,7 @@ int	luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out)
 		info.materials = static_cast<std::uint32_t>(h.materials.size());
 		info.tlas_depth = static_cast<std::uint32_t>(tlasDepth);
 		info.blas_depth = static_cast<std::uint32_t>(blasDepth);
+		if (ctx) ctx->scenes.insert(sc.get());
 		*out = sc.release();
 	});
 }
@@ -555,6 +576,7 @@ void	luzgpu_scene_release(luzgpu_scene* scene)
 	{
 		(void)hipSetDevice(scene->ctx->device);
 		(void)hipStreamSynchronize(scene->ctx->stream);
+		scene->ctx->scenes.erase(scene);
 	}
 	delete scene;
 }

@@ -25,14 +25,16 @@ class RenderFunction(torch.autograd.Function):
         Renderer.render(scene, seed=seed, samples=samples, out=out)
         ctx.scene, ctx.seed, ctx.samples = scene, seed, samples
         ctx.dtypes = (albedo.dtype, roughness.dtype, emission.dtype)
+        ctx.devices = (albedo.device, roughness.device, emission.device)
         return out.view(scene.height, scene.width, 3)
 
     @staticmethod
     def backward(ctx, grad):
         da, dr, de = ctx.scene.adjoint(grad.contiguous(), seed=ctx.seed, samples=ctx.samples)
-        dev = grad.device
-        return (torch.as_tensor(da, dtype=ctx.dtypes[0], device=dev), torch.as_tensor(dr, dtype=ctx.dtypes[1], device=dev),
-                torch.as_tensor(de, dtype=ctx.dtypes[2], device=dev), None, None, None)
+        # each gradient lives where its parameter does
+        return (torch.as_tensor(da, dtype=ctx.dtypes[0], device=ctx.devices[0]),
+                torch.as_tensor(dr, dtype=ctx.dtypes[1], device=ctx.devices[1]),
+                torch.as_tensor(de, dtype=ctx.dtypes[2], device=ctx.devices[2]), None, None, None)
 
 
 def render(scene, albedo, roughness, emission, seed: int = 1, samples=None):

@@ -17,8 +17,12 @@ def tile_means(img, tiles=TILES):
     return out
 
 
-def welch_z(a, b):
-    """z of mean(a) - mean(b) along axis 0 for independent samples."""
+def welch_z(a, b, floor=1e-5):
+    """z of mean(a) - mean(b) along axis 0 for independent samples. The standard error has
+    a floor of `floor` times the mean magnitude, so a region that is (nearly) constant on
+    both sides, such as a background of one value, is compared at fp32 resolution rather
+    than dividing by a zero variance."""
     va, vb = a.var(0, ddof=1) / len(a), b.var(0, ddof=1) / len(b)
-    den = np.sqrt(va + vb)
+    scale = floor * np.maximum(np.abs(a.mean(0)), np.abs(b.mean(0)))
+    den = np.sqrt(va + vb + scale ** 2)
     return np.where(den > 0, (a.mean(0) - b.mean(0)) / np.where(den > 0, den, 1.0), 0.0)
