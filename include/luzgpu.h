@@ -59,7 +59,27 @@ typedef struct luzgpu_settings
 	uint32_t	traversal_grid;  /* persistent blocks per traversal launch, 1..2048                   */
 	int32_t		extend_refill;   /* idle lanes (1..64) at which a wave takes new closest-hit rays     */
 	int32_t		shadow_refill;   /* idle lanes (1..64) at which a wave takes new shadow rays          */
+	uint32_t	fetch_chunk;     /* rays a wave claims per queue atomic, 64..1024                     */
+	uint32_t	lds_stack;       /* LDS traversal-stack entries per lane: 8, 16 or 24                 */
+	uint32_t	autotune;        /* 1: use the context's tuned launch shape for these scenes and      *
+	                              * settings, tuning it first if there is none (luzgpu_autotune);     *
+	                              * the fields above are then ignored                                 */
 } luzgpu_settings;
+
+/* Launch shape chosen by luzgpu_autotune, with the calibration that chose it. */
+typedef struct luzgpu_tuning
+{
+	uint32_t	traversal_grid;
+	int32_t		extend_refill;
+	int32_t		shadow_refill;
+	uint32_t	fetch_chunk;
+	uint32_t	lds_stack;
+	uint32_t	trials;              /* candidate shapes timed                                  */
+	uint64_t	calibration_paths;   /* paths of the calibration render (one pass)              */
+	double		default_ms;          /* calibration render with the built-in defaults           */
+	double		tuned_ms;            /* ... with the chosen shape                               */
+	double		tune_ms;             /* wall time of the whole search                           */
+} luzgpu_tuning;
 
 typedef struct luzgpu_stats
 {
@@ -77,6 +97,7 @@ typedef struct luzgpu_stats
 	double		extend_ms;           /* wavefront, with stage_times                        */
 	double		shade_ms;
 	double		shadow_ms;
+	double		tune_ms;             /* autotune: search time spent by this call (not in render_ms) */
 } luzgpu_stats;
 
 typedef struct luzgpu_scene_info
@@ -122,6 +143,16 @@ int		luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* 
  * settings[0]. stats, if given, receives one aggregate record for the batch. */
 int		luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes, const luzgpu_settings* settings,
 			float* const* rgb_out, luzgpu_stats* stats);
+
+/* Autotuner (no counterpart in the reference, whose CPU workers have no launch shape):
+ * times a calibration render of these scenes and settings (one pass of at most 16M paths)
+ * under candidate traversal launch shapes (LDS stack depth, rays per queue claim,
+ * lane-refill thresholds, persistent grid) and returns the fastest; the context keeps it
+ * for later luzgpu_render / luzgpu_render_batch calls with the same scenes and settings
+ * and autotune = 1. n = 1 tunes the single-scene path, n > 1 the merged batch. The images
+ * do not depend on the launch shape. */
+int		luzgpu_autotune(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes, const luzgpu_settings* settings,
+			luzgpu_tuning* out);
 
 /* Path-replay adjoint (no counterpart in the reference): gradients of an image loss with
  * respect to per-material parameters, for the render luzgpu_render produces with the same

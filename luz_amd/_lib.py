@@ -26,7 +26,7 @@ EXPORTED = [
     "luzgpu_create", "luzgpu_destroy", "luzgpu_last_error", "luzgpu_scene_load",
     "luzgpu_scene_release", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_primary_hits",
     "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas", "luzgpu_render_batch",
-    "luzgpu_adjoint", "luzgpu_scene_set_materials",
+    "luzgpu_adjoint", "luzgpu_scene_set_materials", "luzgpu_autotune",
 ]
 
 
@@ -46,6 +46,24 @@ class Settings(ctypes.Structure):
         ("traversal_grid", ctypes.c_uint32),
         ("extend_refill", ctypes.c_int32),
         ("shadow_refill", ctypes.c_int32),
+        ("fetch_chunk", ctypes.c_uint32),
+        ("lds_stack", ctypes.c_uint32),
+        ("autotune", ctypes.c_uint32),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("traversal_grid", ctypes.c_uint32),
+        ("extend_refill", ctypes.c_int32),
+        ("shadow_refill", ctypes.c_int32),
+        ("fetch_chunk", ctypes.c_uint32),
+        ("lds_stack", ctypes.c_uint32),
+        ("trials", ctypes.c_uint32),
+        ("calibration_paths", ctypes.c_uint64),
+        ("default_ms", ctypes.c_double),
+        ("tuned_ms", ctypes.c_double),
+        ("tune_ms", ctypes.c_double),
     ]
 
 
@@ -65,6 +83,7 @@ class Stats(ctypes.Structure):
         ("extend_ms", ctypes.c_double),
         ("shade_ms", ctypes.c_double),
         ("shadow_ms", ctypes.c_double),
+        ("tune_ms", ctypes.c_double),
     ]
 
 
@@ -113,6 +132,7 @@ def load() -> ctypes.CDLL:
     lib.luzgpu_primary_hits.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, vp]
     lib.luzgpu_adjoint.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, vp, vp, vp]
     lib.luzgpu_scene_set_materials.argtypes = [vp, ctypes.c_uint32, vp, vp, vp]
+    lib.luzgpu_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(vp), ctypes.POINTER(Settings), ctypes.POINTER(Tuning)]
     lib.luzgpu_render_batch.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(vp), ctypes.POINTER(Settings),
                                         ctypes.POINTER(vp), ctypes.POINTER(Stats)]
     lib.luzgpu_sampler_eval.argtypes = [vp, ctypes.c_size_t, vp, vp, vp]

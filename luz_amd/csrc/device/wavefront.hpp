@@ -93,7 +93,8 @@ namespace luzgpu
 	};
 
 	constexpr int	kWaveBlock = 256;
-	constexpr int	kWaveLdsStack = 16;
+	constexpr int	kWaveLdsStack = 16;		// default LDS stack entries per lane (tunable: 8, 16, 24)
+	constexpr int	kWaveLdsStackMin = 8;
 	constexpr int	kWaveStackCapacity = 64;
 	constexpr int	kMaxBounceSlots = 64;
 	constexpr unsigned	kWaveGrid = 2048;	// 256 CUs x 8 waves per SIMD x 4 SIMDs / 4 waves per block
@@ -166,7 +167,8 @@ namespace luzgpu
 	}
 
 	// Queue slots handed to the lanes of one wave, claimed from the queue's fetch counter in
-	// chunks (one atomic per chunk). All fields are wave-uniform.
+	// chunks of `chunk` rays (one atomic per chunk; tunable 64..1024). All fields are
+	// wave-uniform.
 	constexpr std::uint32_t	kFetchChunk = 256;
 
 	struct	WaveFeed
@@ -177,7 +179,7 @@ namespace luzgpu
 
 		// Gives every lane in `idle` (ballot mask) the next queue slot, in lane order; lanes
 		// left without one (queue exhausted) get 0xffffffff.
-		__device__ __forceinline__ std::uint32_t	hand(unsigned long long idle, std::uint32_t n, std::uint32_t* fetch)
+		__device__ __forceinline__ std::uint32_t	hand(unsigned long long idle, std::uint32_t n, std::uint32_t* fetch, std::uint32_t chunk)
 		{
 			const std::uint32_t want = static_cast<std::uint32_t>(__popcll(idle));
 			const std::uint32_t rank = static_cast<std::uint32_t>(__popcll(idle & ((1ull << laneId()) - 1ull)));
@@ -192,10 +194,10 @@ namespace luzgpu
 			std::uint32_t c = 0;
 			if (laneId() == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(idle)) - 1))
 			{
-				c = atomicAdd(fetch, kFetchChunk);
+				c = atomicAdd(fetch, chunk);
 			}
 			c = __builtin_amdgcn_readfirstlane(__shfl(c, __ffsll(static_cast<long long>(idle)) - 1));
-			const std::uint32_t cEnd = c >= n ? c : min(c + kFetchChunk, n);
+			const std::uint32_t cEnd = c >= n ? c : min(c + chunk, n);
 			if (rank < avail) slot = next + rank;
 			else
 			{
@@ -245,13 +247,13 @@ namespace luzgpu
 	// ended takes the next ray of the queue once at least `refillAt` lanes of its wave are
 	// idle (refillAt = 64: the wave refills only when all its lanes are done).
 	// Source supplies the queue: count(), load(slot, ray, tMax) and store(slot, state).
-	template <bool AnyHit, bool Count, int Feat, typename Source>
+	template <bool AnyHit, bool Count, int Feat, int LdsDepth, typename Source>
 	__device__ __forceinline__ void	traceQueue(const SceneView& s, const PassJob* jobs, Source& src, std::uint32_t* fetch,
-		int refillAt, unsigned long long* counters, int counterSlot)
+		int refillAt, std::uint32_t chunk, unsigned long long* counters, int counterSlot)
 	{
-		__shared__ std::uint32_t stackMem[kWaveLdsStack * kWaveBlock];
+		__shared__ std::uint32_t stackMem[LdsDepth * kWaveBlock];
 		const std::uint32_t gtid = blockIdx.x * kWaveBlock + threadIdx.x;
-		SpillStack<kWaveLdsStack, kWaveBlock> stack{stackMem + threadIdx.x, src.spill + gtid, gridDim.x * kWaveBlock, 0,
+		SpillStack<LdsDepth, kWaveBlock> stack{stackMem + threadIdx.x, src.spill + gtid, gridDim.x * kWaveBlock, 0,
 			kWaveStackCapacity};
 		const std::uint32_t n = src.count();
 		WaveFeed feed{0u, 0u, false};
@@ -260,12 +262,15 @@ namespace luzgpu
 		TraceState st;
 		std::uint32_t slot = 0xffffffffu;
 		bool active = false;
+		// the last job's root and unbounded list: a single-scene pass loads them once per lane
+		std::uint32_t lastJob = 0xffffffffu, lastUnbounded = 0u;
+		std::int32_t lastRoot = -1;
 		while (true)
 		{
 			const unsigned long long idle = __ballot(!active);
 			if (!feed.drained && (__popcll(idle) >= refillAt || idle == ~0ull))
 			{
-				const std::uint32_t k = feed.hand(idle, n, fetch);
+				const std::uint32_t k = feed.hand(idle, n, fetch, chunk);
 				if (!active && k != 0xffffffffu)
 				{
 					slot = k;
@@ -273,8 +278,14 @@ namespace luzgpu
 					float tMax, logU;
 					RayOrigin from;
 					const std::uint32_t job = src.load(slot, r, tMax, logU, from);
-					const GpuParams& jp = jobs[job].params;
-					active = traceBegin(st, s, jp.tlasRoot, unboundedOf(jp), r, tMax, logU, from, stack);
+					if (job != lastJob)
+					{
+						const GpuParams& jp = jobs[job].params;
+						lastJob = job;
+						lastRoot = jp.tlasRoot;
+						lastUnbounded = unboundedOf(jp);
+					}
+					active = traceBegin(st, s, lastRoot, lastUnbounded, r, tMax, logU, from, stack);
 					if (!active)
 					{
 						src.store(slot, st);
@@ -346,12 +357,12 @@ namespace luzgpu
 	};
 
 	// closest hit of every ray of queue b
-	template <bool Count, int Feat>
+	template <bool Count, int Feat, int LdsDepth>
 	__global__ __launch_bounds__(kWaveBlock) void	wf_extend(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
-		unsigned long long* counters, int refillAt)
+		unsigned long long* counters, int refillAt, std::uint32_t chunk)
 	{
 		ExtendQueue src{w, b, w.spill};
-		traceQueue<false, Count, Feat>(s, jobs, src, w.counts + 2 * kMaxBounceSlots + b, refillAt, counters, 0);
+		traceQueue<false, Count, Feat, LdsDepth>(s, jobs, src, w.counts + 2 * kMaxBounceSlots + b, refillAt, chunk, counters, 0);
 	}
 
 	template <int Feat>
@@ -567,12 +578,12 @@ namespace luzgpu
 	}
 
 	// any-hit test of the shadow rays appended in bounce b
-	template <bool Count, int Feat>
+	template <bool Count, int Feat, int LdsDepth>
 	__global__ __launch_bounds__(kWaveBlock) void	wf_shadow(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
-		unsigned long long* counters, int refillAt, std::uint32_t volumes)
+		unsigned long long* counters, int refillAt, std::uint32_t chunk, std::uint32_t volumes)
 	{
 		ShadowQueue src{w, b, w.spill, volumes, 0u};
-		traceQueue<true, Count, Feat>(s, jobs, src, w.counts + 3 * kMaxBounceSlots + b, refillAt, counters, 1);
+		traceQueue<true, Count, Feat, LdsDepth>(s, jobs, src, w.counts + 3 * kMaxBounceSlots + b, refillAt, chunk, counters, 1);
 	}
 
 	// sampleColors is indexed by global path id
@@ -642,11 +653,15 @@ namespace luzgpu
 
 	// Launch shape of the traversal stages: persistent blocks per launch and the number of
 	// idle lanes at which a wave takes new rays (1..64).
+	// Launch shape of the traversal kernels; 0 = default. None of it changes results: every
+	// ray is traversed in the same order whichever lane, chunk or stack level holds it.
 	struct	WaveTuning
 	{
-		unsigned	grid = 0;
-		int	extendRefill = 0;
-		int	shadowRefill = 0;
+		unsigned	grid = 0;			// persistent blocks per traversal launch
+		int	extendRefill = 0;		// idle lanes at which a wave takes new closest-hit rays
+		int	shadowRefill = 0;		// idle lanes at which a wave takes new shadow rays
+		unsigned	chunk = 0;			// rays a wave claims per queue atomic
+		int	ldsStack = 0;			// LDS stack entries per lane (the rest spill to HBM)
 
 		WaveTuning	resolved() const
 		{
@@ -654,16 +669,20 @@ namespace luzgpu
 			if (t.grid == 0) t.grid = kWaveGrid;
 			if (t.extendRefill == 0) t.extendRefill = kDefaultRefill;
 			if (t.shadowRefill == 0) t.shadowRefill = kDefaultRefill;
-			if (t.grid > kWaveGrid || t.extendRefill < 1 || t.extendRefill > 64 || t.shadowRefill < 1 || t.shadowRefill > 64)
+			if (t.chunk == 0) t.chunk = kFetchChunk;
+			if (t.ldsStack == 0) t.ldsStack = kWaveLdsStack;
+			if (t.grid > kWaveGrid || t.extendRefill < 1 || t.extendRefill > 64 || t.shadowRefill < 1 || t.shadowRefill > 64
+				|| t.chunk < 64 || t.chunk > 1024 || (t.ldsStack != 8 && t.ldsStack != 16 && t.ldsStack != 24))
 			{
 				throw std::invalid_argument("wavefront tuning out of range: grid 1.." + std::to_string(kWaveGrid)
-					+ ", refill 1..64");
+					+ ", refill 1..64, chunk 64..1024, LDS stack 8, 16 or 24");
 			}
 			return t;
 		}
 		bool	operator==(const WaveTuning& o) const
 		{
-			return grid == o.grid && extendRefill == o.extendRefill && shadowRefill == o.shadowRefill;
+			return grid == o.grid && extendRefill == o.extendRefill && shadowRefill == o.shadowRefill && chunk == o.chunk
+				&& ldsStack == o.ldsStack;
 		}
 	};
 
@@ -720,7 +739,7 @@ namespace luzgpu
 		size_t	carve(char* base, std::uint32_t capacity)
 		{
 			const size_t n = capacity;
-			const size_t spillBytes = static_cast<size_t>(kWaveStackCapacity - kWaveLdsStack) * kWaveGrid * kWaveBlock * 4;
+			const size_t spillBytes = static_cast<size_t>(kWaveStackCapacity - kWaveLdsStackMin) * kWaveGrid * kWaveBlock * 4;
 			size_t off = 0;
 			auto take = [&](size_t b) { char* r = base ? base + off : nullptr; off += (b + 255) & ~size_t(255); return r; };
 			WaveBuffers nb;
@@ -824,13 +843,18 @@ namespace luzgpu
 		for (int b = 0; b <= maxBounces; b++)
 		{
 			mark();
-			if (count) wf_extend<true, Feat><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill);
-			else wf_extend<false, Feat><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill);
+			const unsigned ch = tune.chunk;
+			if (count) wf_extend<true, Feat, kWaveLdsStack><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
+			else if (tune.ldsStack == 8) wf_extend<false, Feat, 8><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
+			else if (tune.ldsStack == 24) wf_extend<false, Feat, 24><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
+			else wf_extend<false, Feat, 16><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
 			mark();
 			wf_shade<Feat><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, vol);
 			mark();
-			if (count) wf_shadow<true, Feat><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, vol);
-			else wf_shadow<false, Feat><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, vol);
+			if (count) wf_shadow<true, Feat, kWaveLdsStack><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
+			else if (tune.ldsStack == 8) wf_shadow<false, Feat, 8><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
+			else if (tune.ldsStack == 24) wf_shadow<false, Feat, 24><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
+			else wf_shadow<false, Feat, 16><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
 			mark();
 			wf_resolve<<<kWaveGrid, kWaveBlock, 0, stream>>>(w, b, st.colors);
 		}

@@ -324,6 +324,9 @@ struct luzgpu_ctx
 	std::vector<luzgpu::GpuParams>	batchParams;
 	luzgpu::DeviceBuffer	batchSums;
 	std::unordered_set<luzgpu_scene*>	scenes;	// loaded on this context and not yet released
+	// autotuned launch shapes, keyed by scene serials and per-scene (width, height, spp, depth)
+	std::vector<std::pair<std::vector<std::uint64_t>, luzgpu_tuning>>	tuned;
+	luzgpu::DeviceBuffer	tuneSums;
 };
 
 struct luzgpu_scene
@@ -452,10 +455,178 @@ namespace luzgpu
 		}
 
 		unsigned	blocks(size_t n, unsigned b) { return static_cast<unsigned>((n + b - 1) / b); }
+
+		WaveTuning	tuningOf(const luzgpu_settings* s)
+		{
+			WaveTuning t;
+			if (s)
+			{
+				t.grid = s->traversal_grid;
+				t.extendRefill = s->extend_refill;
+				t.shadowRefill = s->shadow_refill;
+				t.chunk = s->fetch_chunk;
+				t.ldsStack = static_cast<int>(s->lds_stack);
+			}
+			return t;
+		}
+
+		WaveTuning	tuningOf(const luzgpu_tuning& r)
+		{
+			WaveTuning t;
+			t.grid = r.traversal_grid;
+			t.extendRefill = r.extend_refill;
+			t.shadowRefill = r.shadow_refill;
+			t.chunk = r.fetch_chunk;
+			t.ldsStack = static_cast<int>(r.lds_stack);
+			return t.resolved();
+		}
+
+		std::vector<std::uint64_t>	tuneKey(const std::vector<std::uint64_t>& serials, const std::vector<BatchJob>& jobs)
+		{
+			std::vector<std::uint64_t> k = serials;
+			for (const BatchJob& j : jobs)
+			{
+				k.push_back((static_cast<std::uint64_t>(j.params.width) << 32) | j.params.height);
+				k.push_back((static_cast<std::uint64_t>(j.params.sampleCount) << 32) | static_cast<std::uint32_t>(j.params.maxBounces));
+			}
+			return k;
+		}
+
+		// Picks the traversal launch shape (LDS stack depth, rays claimed per queue atomic,
+		// lane-refill thresholds, persistent grid) by timing a calibration render of the same
+		// jobs: each scene's samples cut so the whole calibration is one full pass (at most
+		// 16M paths), coordinate descent over the knobs, two sweeps, each candidate timed on
+		// its second run (the first captures its graph). The image does not depend on any knob.
+		luzgpu_tuning	autotuneJobs(luzgpu_ctx* ctx, const SceneView& view, int features, const std::vector<BatchJob>& jobs)
+		{
+			const auto t0 = std::chrono::steady_clock::now();
+			hipStream_t st = ctx->stream;
+			std::uint64_t total = 0, sumLen = 0;
+			for (const BatchJob& j : jobs)
+			{
+				total += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
+				sumLen += static_cast<std::uint64_t>(j.params.width) * j.params.height * 3;
+			}
+			const double scale = std::min(1.0, static_cast<double>(kMaxPassPaths) / static_cast<double>(std::max<std::uint64_t>(total, 1)));
+			ctx->tuneSums.reserve(sumLen * sizeof(double));
+			std::vector<BatchJob> cal = jobs;
+			std::uint64_t off = 0;
+			for (BatchJob& j : cal)
+			{
+				j.params.sampleCount = std::max<std::uint32_t>(1u, static_cast<std::uint32_t>(j.params.sampleCount * scale));
+				j.sums = ctx->tuneSums.as<double>() + off;
+				off += static_cast<std::uint64_t>(j.params.width) * j.params.height * 3;
+			}
+			ctx->counters.reserve(8 * sizeof(unsigned long long));
+			hipEvent_t e0, e1;
+			check(hipEventCreate(&e0), "hipEventCreate");
+			check(hipEventCreate(&e1), "hipEventCreate");
+			std::uint32_t trials = 0;
+			auto timeOf = [&](WaveTuning t) {
+				float ms = 0.0f;
+				for (int rep = 0; rep < 2; rep++)
+				{
+					check(hipEventRecord(e0, st), "hipEventRecord");
+					renderWavefront(ctx->wave, view, features, cal, ctx->counters.as<unsigned long long>(), st, false, nullptr, t);
+					check(hipEventRecord(e1, st), "hipEventRecord");
+					check(hipEventSynchronize(e1), "autotune");
+					check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+				}
+				trials++;
+				return static_cast<double>(ms);
+			};
+			WaveTuning best = WaveTuning{}.resolved();
+			const double defaultMs = timeOf(best);
+			double bestMs = defaultMs;
+			for (int sweep = 0; sweep < 2; sweep++)
+			{
+				auto tryValues = [&](auto set, std::initializer_list<int> values) {
+					for (int v : values)
+					{
+						WaveTuning t = best;
+						set(t, v);
+						if (t == best) continue;
+						const double ms = timeOf(t.resolved());
+						if (ms < bestMs)
+						{
+							bestMs = ms;
+							best = t;
+						}
+					}
+				};
+				tryValues([](WaveTuning& t, int v) { t.ldsStack = v; }, {8, 16, 24});
+				tryValues([](WaveTuning& t, int v) { t.chunk = static_cast<unsigned>(v); }, {64, 256, 1024});
+				tryValues([](WaveTuning& t, int v) { t.extendRefill = v; }, {1, 16, 48});
+				tryValues([](WaveTuning& t, int v) { t.shadowRefill = v; }, {1, 16, 48});
+				tryValues([](WaveTuning& t, int v) { t.grid = static_cast<unsigned>(v); }, {1024, 2048});
+			}
+			(void)hipEventDestroy(e0);
+			(void)hipEventDestroy(e1);
+			luzgpu_tuning r{};
+			r.traversal_grid = best.grid;
+			r.extend_refill = best.extendRefill;
+			r.shadow_refill = best.shadowRefill;
+			r.fetch_chunk = best.chunk;
+			r.lds_stack = static_cast<std::uint32_t>(best.ldsStack);
+			r.trials = trials;
+			r.default_ms = defaultMs;
+			r.tuned_ms = bestMs;
+			r.calibration_paths = static_cast<std::uint64_t>(0);
+			for (const BatchJob& j : cal) r.calibration_paths += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
+			r.tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+			return r;
+		}
+
+		// The context's tuning for these jobs, tuned now if there is none yet.
+		WaveTuning	tunedFor(luzgpu_ctx* ctx, const std::vector<std::uint64_t>& serials, const SceneView& view, int features,
+			const std::vector<BatchJob>& jobs, luzgpu_tuning* report)
+		{
+			const std::vector<std::uint64_t> key = tuneKey(serials, jobs);
+			for (const auto& e : ctx->tuned)
+			{
+				if (e.first == key)
+				{
+					if (report) *report = e.second;
+					return tuningOf(e.second);
+				}
+			}
+			const luzgpu_tuning r = autotuneJobs(ctx, view, features, jobs);
+			ctx->tuned.emplace_back(key, r);
+			if (report) *report = r;
+			return tuningOf(r);
+		}
 	}
 }
 
 using namespace luzgpu;
+
+namespace
+{
+	// Merged device arrays of a multi-scene batch, rebuilt only when the scene list changes;
+	// returns the list's key (the scenes' serials).
+	std::vector<std::uint64_t>	ensureBatch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes)
+	{
+		std::vector<std::uint64_t> key(n);
+		for (uint32_t i = 0; i < n; i++) key[i] = scenes[i]->serial;
+		if (!ctx->batch || ctx->batchKey != key)
+		{
+			ctx->batch.reset();
+			ctx->wave.dropGraph();
+			std::vector<const PackedScene*> parts(n);
+			for (uint32_t i = 0; i < n; i++) parts[i] = &scenes[i]->gpu.host;
+			auto g = std::make_unique<GpuScene>();
+			g->host = mergeScenes(parts, ctx->batchParams);
+			if (g->host.tlasStackBound + 2 + g->host.blasStackBound > kWaveStackCapacity)
+			{
+				throw std::runtime_error("Unsupported for the device path: traversal stack bound of the batch");
+			}
+			uploadScene(ctx, *g);
+			ctx->batch = std::move(g);
+			ctx->batchKey = key;
+		}
+		return key;
+	}
+}
 
 extern "C" {
 
@@ -615,14 +786,16 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		{
 			throw Error{LUZGPU_E_INVALID, "luzgpu_render: unknown integrator"};
 		}
-		WaveTuning tune;
-		if (settings)
+		WaveTuning tune = tuningOf(settings).resolved();
+		double tuneMs = 0.0;
+		if (settings && settings->autotune && integrator == LUZGPU_INTEGRATOR_WAVEFRONT)
 		{
-			tune.grid = settings->traversal_grid;
-			tune.extendRefill = settings->extend_refill;
-			tune.shadowRefill = settings->shadow_refill;
+			luzgpu_tuning r{};
+			const std::vector<BatchJob> cal{BatchJob{p, nullptr}};
+			const size_t before = ctx->tuned.size();
+			tune = tunedFor(ctx, {scene->serial}, scene->gpu.view, scene->gpu.host.features, cal, &r);
+			if (ctx->tuned.size() != before) tuneMs = r.tune_ms;
 		}
-		tune = tune.resolved();
 		const std::uint32_t pixels = p.width * p.height;
 		hipStream_t st = ctx->stream;
 		ctx->sums.reserve(static_cast<size_t>(pixels) * 3 * sizeof(double));
@@ -717,7 +890,72 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			stats->shade_ms = stages.shadeMs;
 			stats->shadow_ms = stages.shadowMs;
 			stats->extend_launches = stages.extendLaunches;
+			stats->tune_ms = tuneMs;
 		}
+	});
+}
+
+int	luzgpu_autotune(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes, const luzgpu_settings* settings, luzgpu_tuning* out)
+{
+	if (!ctx || !scenes || !out || n == 0 || n > kMaxBatchScenes)
+	{
+		g_lastError = "luzgpu_autotune: NULL argument or scene count outside 1.." + std::to_string(kMaxBatchScenes);
+		return LUZGPU_E_INVALID;
+	}
+	for (uint32_t i = 0; i < n; i++)
+	{
+		if (!scenes[i] || scenes[i]->ctx != ctx)
+		{
+			g_lastError = "luzgpu_autotune: scene " + std::to_string(i) + " is NULL or from another context";
+			return LUZGPU_E_INVALID;
+		}
+	}
+	return guarded([&] {
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		std::vector<BatchJob> jobs(n);
+		std::vector<std::uint64_t> serials(n);
+		for (uint32_t i = 0; i < n; i++)
+		{
+			serials[i] = scenes[i]->serial;
+			const luzgpu_settings* si = settings ? settings + i : nullptr;
+			const GpuParams p = n == 1 ? resolveParams(scenes[i]->gpu, si) : GpuParams{};
+			jobs[i].params = p;
+		}
+		if (n == 1)
+		{
+			const GpuParams& p = jobs[0].params;
+			if (p.sampleBegin + p.sampleCount > p.totalSamples || p.sampleCount == 0 || p.maxBounces < 0)
+			{
+				throw Error{LUZGPU_E_INVALID, "luzgpu_autotune: empty or out-of-range samples"};
+			}
+			*out = autotuneJobs(ctx, scenes[0]->gpu.view, scenes[0]->gpu.host.features, jobs);
+		}
+		else
+		{
+			const std::vector<std::uint64_t> key = ensureBatch(ctx, n, scenes);
+			for (uint32_t i = 0; i < n; i++)
+			{
+				jobs[i].params = resolveParams(ctx->batchParams[i], scenes[i]->gpu.host, settings ? settings + i : nullptr);
+				const GpuParams& p = jobs[i].params;
+				if (p.sampleBegin + p.sampleCount > p.totalSamples || p.sampleCount == 0 || p.maxBounces < 0)
+				{
+					throw Error{LUZGPU_E_INVALID, "luzgpu_autotune: scene " + std::to_string(i) + ": empty or out-of-range samples"};
+				}
+			}
+			*out = autotuneJobs(ctx, ctx->batch->view, ctx->batch->host.features, jobs);
+		}
+		// later renders of these jobs with autotune = 1 use this result
+		const std::vector<std::uint64_t> key = tuneKey(serials, jobs);
+		bool replaced = false;
+		for (auto& e : ctx->tuned)
+		{
+			if (e.first == key)
+			{
+				e.second = *out;
+				replaced = true;
+			}
+		}
+		if (!replaced) ctx->tuned.emplace_back(key, *out);
 	});
 }
 
@@ -740,25 +978,7 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 	return guarded([&] {
 		check(hipSetDevice(ctx->device), "hipSetDevice");
 		hipStream_t st = ctx->stream;
-		// merged device arrays, rebuilt only when the scene list changes
-		std::vector<std::uint64_t> key(n);
-		for (uint32_t i = 0; i < n; i++) key[i] = scenes[i]->serial;
-		if (!ctx->batch || ctx->batchKey != key)
-		{
-			ctx->batch.reset();
-			ctx->wave.dropGraph();
-			std::vector<const PackedScene*> parts(n);
-			for (uint32_t i = 0; i < n; i++) parts[i] = &scenes[i]->gpu.host;
-			auto g = std::make_unique<GpuScene>();
-			g->host = mergeScenes(parts, ctx->batchParams);
-			if (g->host.tlasStackBound + 2 + g->host.blasStackBound > kWaveStackCapacity)
-			{
-				throw std::runtime_error("Unsupported for the device path: traversal stack bound of the batch");
-			}
-			uploadScene(ctx, *g);
-			ctx->batch = std::move(g);
-			ctx->batchKey = key;
-		}
+		const std::vector<std::uint64_t> key = ensureBatch(ctx, n, scenes);
 		std::vector<BatchJob> jobs(n);
 		std::vector<size_t> sumOffset(n);
 		size_t sumTotal = 0;
@@ -783,12 +1003,16 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 		zero_doubles<<<blocks(sumTotal, 256), 256, 0, st>>>(ctx->batchSums.as<double>(), sumTotal);
 		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
 		for (uint32_t i = 0; i < n; i++) jobs[i].sums = ctx->batchSums.as<double>() + sumOffset[i];
-		WaveTuning tune;
-		if (settings)
+		WaveTuning tune = tuningOf(settings).resolved();
+		double tuneMs = 0.0;
+		if (settings && settings[0].autotune)
 		{
-			tune.grid = settings[0].traversal_grid;
-			tune.extendRefill = settings[0].extend_refill;
-			tune.shadowRefill = settings[0].shadow_refill;
+			luzgpu_tuning r{};
+			const size_t before = ctx->tuned.size();
+			tune = tunedFor(ctx, key, ctx->batch->view, ctx->batch->host.features, jobs, &r);
+			if (ctx->tuned.size() != before) tuneMs = r.tune_ms;
+			// the calibration used the counters: clear them again
+			zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
 		}
 		hipEvent_t e0, e1;
 		check(hipEventCreate(&e0), "hipEventCreate");
@@ -855,6 +1079,7 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 			stats->shade_ms = stages.shadeMs;
 			stats->shadow_ms = stages.shadowMs;
 			stats->extend_launches = stages.extendLaunches;
+			stats->tune_ms = tuneMs;
 			stats->kernel_launches = stages.passes;
 		}
 	});

@@ -72,6 +72,32 @@ class RenderStats:
     extend_ms: float = 0.0
     shade_ms: float = 0.0
     shadow_ms: float = 0.0
+    tune_ms: float = 0.0
+
+    @staticmethod
+    def of(st) -> "RenderStats":
+        return RenderStats(st.rendered_samples, st.closest_hit_rays, st.shadow_rays, st.render_ms, st.kernel_ms,
+                           st.kernel_launches, st.nodes_visited, st.prims_tested, st.extend_launches,
+                           st.closest_nodes, st.closest_prims, st.extend_ms, st.shade_ms, st.shadow_ms, st.tune_ms)
+
+
+@dataclass
+class Tuning:
+    """Launch shape chosen by the autotuner (luzgpu_autotune) and its calibration."""
+    grid: int
+    extend_refill: int
+    shadow_refill: int
+    fetch_chunk: int
+    lds_stack: int
+    trials: int
+    calibration_paths: int
+    default_ms: float
+    tuned_ms: float
+    tune_ms: float
+
+    def as_dict(self) -> dict:
+        return {"grid": self.grid, "extend_refill": self.extend_refill, "shadow_refill": self.shadow_refill,
+                "fetch_chunk": self.fetch_chunk, "lds_stack": self.lds_stack}
 
 
 class Scene:
@@ -174,10 +200,14 @@ class SceneFile:
 def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=False, resolution=None,
               count_visits=False, stage_times=False, tuning=None):
     s = _lib.Settings()
-    if tuning:
+    if tuning == "auto":
+        s.autotune = 1
+    elif tuning:
         s.traversal_grid = int(tuning.get("grid", 0))
         s.extend_refill = int(tuning.get("extend_refill", 0))
         s.shadow_refill = int(tuning.get("shadow_refill", 0))
+        s.fetch_chunk = int(tuning.get("fetch_chunk", 0))
+        s.lds_stack = int(tuning.get("lds_stack", 0))
     s.count_visits = 1 if count_visits else 0
     s.stage_times = 1 if stage_times else 0
     if resolution is not None:
@@ -201,8 +231,9 @@ class Renderer:
                tuning=None):
         """Returns (image[H, W, 3] float32, RenderStats). ``out`` may be a device tensor
         (torch, float32, contiguous, H*W*3) that receives the image without a host copy.
-        ``tuning`` ({"grid", "extend_refill", "shadow_refill"}) sets the wavefront launch
-        shape; it never changes the image."""
+        ``tuning`` ({"grid", "extend_refill", "shadow_refill", "fetch_chunk", "lds_stack"}) sets
+        the wavefront launch shape, or "auto" uses the context's autotuned one (tuning it
+        first if needed); it never changes the image."""
         s = _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=out is not None,
                       resolution=resolution, count_visits=count_visits, stage_times=stage_times, tuning=tuning)
         st = _lib.Stats()
@@ -215,9 +246,7 @@ class Renderer:
             ptr = img.ctypes.data
         _lib.check(scene.ctx._lib.luzgpu_render(scene.ctx.handle, scene._h, ctypes.byref(s), ctypes.c_void_p(ptr),
                                                 ctypes.byref(st)))
-        return img, RenderStats(st.rendered_samples, st.closest_hit_rays, st.shadow_rays, st.render_ms, st.kernel_ms,
-                                st.kernel_launches, st.nodes_visited, st.prims_tested, st.extend_launches,
-                                st.closest_nodes, st.closest_prims, st.extend_ms, st.shade_ms, st.shadow_ms)
+        return img, RenderStats.of(st)
 
     @staticmethod
     def render_batch(scenes, seeds, samples=None, max_bounces=None, outs=None, resolutions=None, count_visits=False,
@@ -249,9 +278,24 @@ class Renderer:
         handles = (ctypes.c_void_p * n)(*[sc._h.value for sc in scenes])
         st = _lib.Stats()
         _lib.check(ctx._lib.luzgpu_render_batch(ctx.handle, n, handles, arr, ptrs, ctypes.byref(st)))
-        return images, RenderStats(st.rendered_samples, st.closest_hit_rays, st.shadow_rays, st.render_ms, st.kernel_ms,
-                                   st.kernel_launches, st.nodes_visited, st.prims_tested, st.extend_launches,
-                                   st.closest_nodes, st.closest_prims, st.extend_ms, st.shade_ms, st.shadow_ms)
+        return images, RenderStats.of(st)
+
+    @staticmethod
+    def autotune(scenes, seeds, samples=None, max_bounces=None, resolutions=None) -> Tuning:
+        """Autotunes the wavefront launch shape for these scenes and settings (one scene: the
+        single-scene path; several: the merged batch). Later renders with tuning="auto" and
+        the same scenes and settings use the result."""
+        n = len(scenes)
+        ctx = scenes[0].ctx
+        arr = (_lib.Settings * n)()
+        for i in range(n):
+            arr[i] = _settings(seeds[i], samples[i] if samples else None, max_bounces[i] if max_bounces else None, 0, None,
+                               "wavefront", resolution=resolutions[i] if resolutions else None)
+        handles = (ctypes.c_void_p * n)(*[sc._h.value for sc in scenes])
+        t = _lib.Tuning()
+        _lib.check(ctx._lib.luzgpu_autotune(ctx.handle, n, handles, arr, ctypes.byref(t)))
+        return Tuning(t.traversal_grid, t.extend_refill, t.shadow_refill, t.fetch_chunk, t.lds_stack, t.trials,
+                      t.calibration_paths, t.default_ms, t.tuned_ms, t.tune_ms)
 
     @staticmethod
     def primary_hits(scene: Scene, seed: int, resolution=None):
