@@ -83,13 +83,19 @@ def bench_batch(args, wdir):
     outs = [torch.empty(w.width * w.height * 3, dtype=torch.float32, device="cuda") for w in wls]
     total = sum(w.width * w.height * w.spp for w in wls)
     log(f"batch: {len(wls)} scenes, {sum(w.triangles for w in wls)} triangles, {total / 1e6:.1f} M samples per step")
+    tuning = None
+    if args.tuning == "auto":
+        tuned = Renderer.autotune(scenes, seeds)
+        tuning = "auto"
+        log(f"autotune: {tuned.as_dict()} in {tuned.tune_ms:.0f} ms over {tuned.trials} shapes "
+            f"(calibration pass {tuned.default_ms:.2f} -> {tuned.tuned_ms:.2f} ms)")
     for i in range(args.warmup):
-        _, st = Renderer.render_batch(scenes, seeds, outs=outs)
+        _, st = Renderer.render_batch(scenes, seeds, outs=outs, tuning=tuning)
         log(f"warmup {i}: {st.render_ms:.1f} ms")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        _, st = Renderer.render_batch(scenes, seeds, outs=outs)
+        _, st = Renderer.render_batch(scenes, seeds, outs=outs, tuning=tuning)
         log(f"step {i}: {st.render_ms:.1f} ms, {st.kernel_launches} passes")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -194,6 +200,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="spheres", choices=["spheres", "cornell", "batch", "terrain", "diff"])
     ap.add_argument("--integrator", default="wavefront", choices=["megakernel", "wavefront"])
+    ap.add_argument("--tuning", default="auto", choices=["auto", "default"],
+                    help="wavefront launch shape: autotuned (untimed calibration first) or built-in defaults")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=16)
@@ -248,14 +256,24 @@ def main():
         f"{cst.prims_tested / max(cst.rendered_samples, 1):.1f} primitives, "
         f"{(cst.closest_hit_rays + cst.shadow_rays) / max(cst.rendered_samples, 1):.2f} rays per sample")
 
+    tuning = None
+    tuned = None
+    if wavefront and args.tuning == "auto":
+        # the autotuner's calibration pass (untimed; its result is reused by every step)
+        tuned = Renderer.autotune([scene], [args.seed])
+        tuning = "auto"
+        log(f"autotune: {tuned.as_dict()} in {tuned.tune_ms:.0f} ms over {tuned.trials} shapes "
+            f"(calibration pass {tuned.default_ms:.2f} -> {tuned.tuned_ms:.2f} ms)")
     for i in range(args.warmup):
-        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out, stage_times=wavefront)
+        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out, stage_times=wavefront,
+                                tuning=tuning)
         log(f"warmup {i}: {st.render_ms:.1f} ms")
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out, stage_times=wavefront)
+        _, st = Renderer.render(scene, seed=args.seed, integrator=args.integrator, out=out, stage_times=wavefront,
+                                tuning=tuning)
         stats.append(st)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -305,6 +323,7 @@ def main():
                         f"{wl.width}x{wl.height}, {wl.spp} spp, depth {wl.depth}, seed {args.seed}",
             "width": wl.width, "height": wl.height, "spp": wl.spp, "max_depth": wl.depth,
             "triangles": wl.triangles, "integrator": args.integrator,
+            "launch_shape": tuned.as_dict() if tuned else "default",
         },
         "mrays_per_s": round(rays_total / elapsed / 1e6, 3),
         "roofline": {

@@ -443,8 +443,10 @@ namespace luzgpu
 			}
 			if (hits > 0)
 			{
-				// order by entry distance (insertion sort of <= 4), push far to near
-				for (int i = 1; i < hits; i++)
+				// closest hit: order by entry distance (insertion sort of <= 4), push far to
+				// near; an any-hit query ends at the first hit, so its order does not matter
+				// (BVHNode::hitAny is unordered, BVHNode.cpp:326-380)
+				for (int i = 1; i < (AnyHit ? 0 : hits); i++)
 				{
 					const float kt = tn[i];
 					const std::uint32_t ke = e[i];

@@ -98,7 +98,10 @@ namespace luzgpu
 	constexpr int	kWaveStackCapacity = 64;
 	constexpr int	kMaxBounceSlots = 64;
 	constexpr unsigned	kWaveGrid = 2048;	// 256 CUs x 8 waves per SIMD x 4 SIMDs / 4 waves per block
-	constexpr int	kDefaultRefill = 16;
+	// default traversal launch shape (the autotuner's usual choice on the sphere workloads):
+	// half the maximal grid, whole-wave claims of 64 rays
+	constexpr unsigned	kDefaultGrid = 1024;
+	constexpr int	kDefaultRefill = 64;
 
 	__device__ __forceinline__ std::uint32_t	laneId() { return threadIdx.x & 63u; }
 
@@ -169,7 +172,7 @@ namespace luzgpu
 	// Queue slots handed to the lanes of one wave, claimed from the queue's fetch counter in
 	// chunks of `chunk` rays (one atomic per chunk; tunable 64..1024). All fields are
 	// wave-uniform.
-	constexpr std::uint32_t	kFetchChunk = 256;
+	constexpr std::uint32_t	kFetchChunk = 64;
 
 	struct	WaveFeed
 	{
@@ -245,7 +248,8 @@ namespace luzgpu
 
 	// Persistent traversal over one ray queue with lane refill: a lane whose traversal has
 	// ended takes the next ray of the queue once at least `refillAt` lanes of its wave are
-	// idle (refillAt = 64: the wave refills only when all its lanes are done).
+	// idle (refillAt = 64: the wave claims 64 rays at a time and every lane traverses its
+	// ray to the end before the next claim).
 	// Source supplies the queue: count(), load(slot, ray, tMax) and store(slot, state).
 	template <bool AnyHit, bool Count, int Feat, int LdsDepth, typename Source>
 	__device__ __forceinline__ void	traceQueue(const SceneView& s, const PassJob* jobs, Source& src, std::uint32_t* fetch,
@@ -265,6 +269,43 @@ namespace luzgpu
 		// the last job's root and unbounded list: a single-scene pass loads them once per lane
 		std::uint32_t lastJob = 0xffffffffu, lastUnbounded = 0u;
 		std::int32_t lastRoot = -1;
+		if (refillAt >= 64)
+		{
+			// whole-wave cadence: the wave claims 64 rays, each lane runs its traversal to the
+			// end in a tight loop, then the wave claims the next 64
+			for (;;)
+			{
+				const std::uint32_t k = feed.hand(~0ull, n, fetch, chunk);
+				if (__ballot(k != 0xffffffffu) == 0ull)
+				{
+					break;
+				}
+				if (k != 0xffffffffu)
+				{
+					Ray r;
+					float tMax, logU;
+					RayOrigin from;
+					const std::uint32_t job = src.load(k, r, tMax, logU, from);
+					if (job != lastJob)
+					{
+						const GpuParams& jp = jobs[job].params;
+						lastJob = job;
+						lastRoot = jp.tlasRoot;
+						lastUnbounded = unboundedOf(jp);
+					}
+					if (traceBegin(st, s, lastRoot, lastUnbounded, r, tMax, logU, from, stack))
+					{
+						while (!traceStep<AnyHit, Count, Feat>(s, st, kTMinF, stack, &visits))
+						{
+						}
+					}
+					src.store(k, st);
+					rays++;
+				}
+			}
+			flushCounters(counters, counterSlot, rays, visits, Count);
+			return;
+		}
 		while (true)
 		{
 			const unsigned long long idle = __ballot(!active);
@@ -666,7 +707,7 @@ namespace luzgpu
 		WaveTuning	resolved() const
 		{
 			WaveTuning t = *this;
-			if (t.grid == 0) t.grid = kWaveGrid;
+			if (t.grid == 0) t.grid = kDefaultGrid;
 			if (t.extendRefill == 0) t.extendRefill = kDefaultRefill;
 			if (t.shadowRefill == 0) t.shadowRefill = kDefaultRefill;
 			if (t.chunk == 0) t.chunk = kFetchChunk;
