@@ -423,22 +423,56 @@ namespace luzgpu
 			std::uint32_t e[4];
 			float tn[4];
 			int hits = 0;
-			for (int k = 0; k < 4; k++)
+			const f3 inv = st.ray.inv;
+			if (inv.x != 0.0f && inv.y != 0.0f && inv.z != 0.0f)
 			{
-				const std::uint32_t cnt = (n3.z >> (8 * k)) & 0xffu;
-				if (cnt == 255u) continue;
-				float lo[3], hi[3];
-				for (int a = 0; a < 3; a++)
+				// Slab distances straight from the quantised planes: with s = 2^exp / d and
+				// b = (origin - o) / d per axis, a plane at q is at t = fma(q, s, b). Rounding
+				// differs from decoding the plane first by a few ulps, so the interval is
+				// widened by 2^-20 relative (the quantised boxes already enclose the FP64 ones).
+				const float s0 = sc[0] * inv.x, s1 = sc[1] * inv.y, s2 = sc[2] * inv.z;
+				const float b0 = (org[0] - st.ray.o.x) * inv.x, b1 = (org[1] - st.ray.o.y) * inv.y, b2 = (org[2] - st.ray.o.z) * inv.z;
+				for (int k = 0; k < 4; k++)
 				{
-					lo[a] = fmaf(static_cast<float>((qlo[a] >> (8 * k)) & 0xffu), sc[a], org[a]);
-					hi[a] = fmaf(static_cast<float>((qhi[a] >> (8 * k)) & 0xffu), sc[a], org[a]);
+					const std::uint32_t cnt = (n3.z >> (8 * k)) & 0xffu;
+					if (cnt == 255u) continue;
+					const float l0 = fmaf(static_cast<float>((qlo[0] >> (8 * k)) & 0xffu), s0, b0);
+					const float h0 = fmaf(static_cast<float>((qhi[0] >> (8 * k)) & 0xffu), s0, b0);
+					const float l1 = fmaf(static_cast<float>((qlo[1] >> (8 * k)) & 0xffu), s1, b1);
+					const float h1 = fmaf(static_cast<float>((qhi[1] >> (8 * k)) & 0xffu), s1, b1);
+					const float l2 = fmaf(static_cast<float>((qlo[2] >> (8 * k)) & 0xffu), s2, b2);
+					const float h2 = fmaf(static_cast<float>((qhi[2] >> (8 * k)) & 0xffu), s2, b2);
+					const float near = fmaxf(fmaxf(fminf(l0, h0), fminf(l1, h1)), fminf(l2, h2)) * (1.0f - 0x1p-20f);
+					const float far = fminf(fminf(fmaxf(l0, h0), fmaxf(l1, h1)), fmaxf(l2, h2)) * (1.0f + 0x1p-20f);
+					const float t = fmaxf(near, kTMinF);
+					if (fminf(far, st.hit.t) > t)
+					{
+						tn[hits] = t;
+						e[hits] = cnt > 0 ? leafEntry(static_cast<std::int32_t>(child[k]), static_cast<std::int32_t>(cnt)) : child[k];
+						hits++;
+					}
 				}
-				float t;
-				if (boxHit(st.ray, lo, hi, st.hit.t, t))
+			}
+			else
+			{
+				// an axis-parallel ray: AABB::hit tests the origin against that slab (AABB.cpp:37-80)
+				for (int k = 0; k < 4; k++)
 				{
-					tn[hits] = t;
-					e[hits] = cnt > 0 ? leafEntry(static_cast<std::int32_t>(child[k]), static_cast<std::int32_t>(cnt)) : child[k];
-					hits++;
+					const std::uint32_t cnt = (n3.z >> (8 * k)) & 0xffu;
+					if (cnt == 255u) continue;
+					float lo[3], hi[3];
+					for (int a = 0; a < 3; a++)
+					{
+						lo[a] = fmaf(static_cast<float>((qlo[a] >> (8 * k)) & 0xffu), sc[a], org[a]);
+						hi[a] = fmaf(static_cast<float>((qhi[a] >> (8 * k)) & 0xffu), sc[a], org[a]);
+					}
+					float t;
+					if (boxHit(st.ray, lo, hi, st.hit.t, t))
+					{
+						tn[hits] = t;
+						e[hits] = cnt > 0 ? leafEntry(static_cast<std::int32_t>(child[k]), static_cast<std::int32_t>(cnt)) : child[k];
+						hits++;
+					}
 				}
 			}
 			if (hits > 0)
