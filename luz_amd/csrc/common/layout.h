@@ -176,15 +176,22 @@ namespace luzgpu
 
 	struct alignas(16) GpuLight
 	{
-		std::uint32_t	kind;		// GPU_HIT_*: triangle, rectangle, sphere, directional; others never sample
-		std::uint32_t	index;		// into top-level triangles / rectangles / spheres
+		std::uint32_t	kind;		// GPU_HIT_*: triangle, rectangle, sphere, directional, instance; others never sample
+		std::uint32_t	index;		// into top-level triangles / rectangles / spheres / instances
 		std::uint32_t	hittable;
 		std::uint32_t	material;
 		float	emitted[3];
 		float	area;
 		float	selectionProbability;	// weight / total (or 1/n when unweighted)
 		float	direction[3];		// directional: the sampled direction (minus the light's)
+		// emissive mesh instance: its span of the triangle-area prefix sums (FP64) and of the
+		// matching triangle slots, and its local -> world map (rotate(p * scale) + position)
+		std::uint32_t	triBase;
+		std::uint32_t	triCount;
+		std::uint32_t	pad[2];
+		float	localToWorld[12];
 	};
+	static_assert(sizeof(GpuLight) == 112, "GpuLight must be 112 bytes");
 
 	// Everything the per-pixel camera needs, in FP64 exactly as the reference's
 	// RenderCamera (src/renderer/RendererInternal.hpp:13-26).

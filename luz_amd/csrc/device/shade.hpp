@@ -882,26 +882,62 @@ namespace luzgpu
 			sl.material = L.material;
 			return sl;
 		}
-		if (L.kind != GPU_HIT_RECTANGLE && L.kind != GPU_HIT_TRIANGLE)
-		{
-			return ls;	// Hittable::sampleLight default: never valid (pdfValue 0)
-		}
-		float sx, sy;
-		ps.sample2D(DIM_LIGHT_SURFACE_POINT, sx, sy);
 		f3 point, normal;
-		if (L.kind == GPU_HIT_RECTANGLE)
+		if ((Feat & kFeatAnalytic) && L.kind == GPU_HIT_INSTANCE)
 		{
-			const GpuRect& r = s.rects[L.index];
-			point = ld3(r.position) + ld3(r.widthAxis) * ((sx - 0.5f) * r.width) + ld3(r.heightAxis) * ((sy - 0.5f) * r.height);
-			normal = ld3(r.normal);
+			// MeshInstance::sampleLight / _sampleSurface (MeshInstance.cpp:409-466,510-546): a
+			// triangle by area (std::lower_bound over the FP64 prefix sums), a point on it
+			// (Triangle::sampleSurface, Triangle.cpp:320-341), mapped to world space
+			if (L.triCount == 0u || !(L.area > 0.0f)) return ls;
+			const double* cdf = s.lightTriCdf + L.triBase;
+			const double target = ps.sample1D(DIM_LIGHT_SURFACE_SELECTION) * cdf[L.triCount - 1];
+			std::uint32_t lo = 0, hi = L.triCount;
+			while (lo < hi)
+			{
+				const std::uint32_t mid = (lo + hi) >> 1;
+				if (cdf[mid] < target) lo = mid + 1;
+				else hi = mid;
+			}
+			const std::uint32_t k = lo < L.triCount - 1 ? lo : L.triCount - 1;
+			const double prev = k == 0 ? 0.0 : cdf[k - 1];
+			if (!(cdf[k] - prev > 0.0)) return ls;
+			const GpuTri& t = s.meshTris[s.lightTriSlot[L.triBase + k]];
+			float sx, sy;
+			ps.sample2D(DIM_LIGHT_SURFACE_POINT, sx, sy);
+			float u = sx, v = sy;
+			if (u + v > 1.0f) { u = 1.0f - u; v = 1.0f - v; }
+			const f3 lp = ld3(t.v0) + ld3(t.e1) * u + ld3(t.e2) * v;
+			const float* M = L.localToWorld;
+			point = mk(M[0] * lp.x + M[1] * lp.y + M[2] * lp.z + M[3], M[4] * lp.x + M[5] * lp.y + M[6] * lp.z + M[7],
+				M[8] * lp.x + M[9] * lp.y + M[10] * lp.z + M[11]);
+			const float* N = s.instances[L.index].normalToWorld;
+			const f3 fn = faceNormalOf(t);
+			normal = normalizeOrZero(mk(N[0] * fn.x + N[1] * fn.y + N[2] * fn.z, N[3] * fn.x + N[4] * fn.y + N[5] * fn.z,
+				N[6] * fn.x + N[7] * fn.y + N[8] * fn.z));
+			if (len2(normal) <= 0.0f) return ls;
 		}
 		else
 		{
-			const GpuTri& t = s.topTris[L.index];
-			float u = sx, v = sy;
-			if (u + v > 1.0f) { u = 1.0f - u; v = 1.0f - v; }
-			point = ld3(t.v0) + ld3(t.e1) * u + ld3(t.e2) * v;
-			normal = faceNormalOf(t);
+			if (L.kind != GPU_HIT_RECTANGLE && L.kind != GPU_HIT_TRIANGLE)
+			{
+				return ls;	// Hittable::sampleLight default: never valid (pdfValue 0)
+			}
+			float sx, sy;
+			ps.sample2D(DIM_LIGHT_SURFACE_POINT, sx, sy);
+			if (L.kind == GPU_HIT_RECTANGLE)
+			{
+				const GpuRect& r = s.rects[L.index];
+				point = ld3(r.position) + ld3(r.widthAxis) * ((sx - 0.5f) * r.width) + ld3(r.heightAxis) * ((sy - 0.5f) * r.height);
+				normal = ld3(r.normal);
+			}
+			else
+			{
+				const GpuTri& t = s.topTris[L.index];
+				float u = sx, v = sy;
+				if (u + v > 1.0f) { u = 1.0f - u; v = 1.0f - v; }
+				point = ld3(t.v0) + ld3(t.e1) * u + ld3(t.e2) * v;
+				normal = faceNormalOf(t);
+			}
 		}
 		const f3 d = point - origin;
 		const float d2 = len2(d);
@@ -938,6 +974,20 @@ namespace luzgpu
 			const double rr = static_cast<double>(sp.radius) * sp.radius;
 			const double cosMax = sqrt(1.0 - rr / (cx * cx + cy * cy + cz * cz));
 			return static_cast<float>(1.0 / (2.0 * 3.14159265358979323846 * (1.0 - cosMax)));
+		}
+		if ((Feat & kFeatAnalytic) && L.kind == GPU_HIT_INSTANCE)
+		{
+			// MeshInstance::pdfValue (MeshInstance.cpp:468-497): this instance alone
+			if (!(L.area > 0.0f) || !(vl2 > 0.0f)) return 0.0f;
+			HitInfo h;
+			if (!instanceClosest(s, L.index, L.hittable, r, INFINITY, h)) return 0.0f;
+			const float* N = s.instances[L.index].normalToWorld;
+			const f3 fn = faceNormalOf(s.meshTris[h.tri]);
+			const f3 gn = normalizeOrZero(mk(N[0] * fn.x + N[1] * fn.y + N[2] * fn.z, N[3] * fn.x + N[4] * fn.y + N[5] * fn.z,
+				N[6] * fn.x + N[7] * fn.y + N[8] * fn.z));
+			const float c = fabsf(dot(vec, gn) / sqrtf(vl2));
+			if (!(c > 0.0f)) return 0.0f;
+			return (h.t * h.t * vl2) / (c * L.area);
 		}
 		if (L.kind != GPU_HIT_RECTANGLE && L.kind != GPU_HIT_TRIANGLE)
 		{

@@ -36,6 +36,8 @@ namespace luzgpu
 		const GpuMaterial*	materials;
 		const GpuLight*	lights;
 		const double*	lightCdf;
+		const double*	lightTriCdf;		// emissive mesh instances: triangle-area prefix sums
+		const std::uint32_t*	lightTriSlot;	// ... and the mesh-triangle slot of each entry
 		const float4*	envPixels;
 		const float*	envPdf;
 		const double*	envCdf;
@@ -58,7 +60,9 @@ namespace luzgpu
 		{
 			return 0.0f;
 		}
-		const float r = 1.0f / c;
+		// hardware reciprocal (1 ulp): the inverse only feeds box tests, whose intervals are
+		// widened by far more than that
+		const float r = __builtin_amdgcn_rcpf(c);
 		return isfinite(r) ? r : 0.0f;
 	}
 
@@ -90,8 +94,9 @@ namespace luzgpu
 		slab(r.o.x, r.inv.x, lo[0], hi[0], n0, f0);
 		slab(r.o.y, r.inv.y, lo[1], hi[1], n1, f1);
 		slab(r.o.z, r.inv.z, lo[2], hi[2], n2, f2);
-		const float tn = fmaxf(fmaxf(n0, n1), fmaxf(n2, kTMinF));
-		const float tf = fminf(fminf(f0, f1), fminf(f2, tMax));
+		// widened by 2^-20 relative like the quantised-plane test (the inverse is 1 ulp)
+		const float tn = fmaxf(fmaxf(n0, n1) * (1.0f - 0x1p-20f), fmaxf(n2 * (1.0f - 0x1p-20f), kTMinF));
+		const float tf = fminf(fminf(f0, f1) * (1.0f + 0x1p-20f), fminf(f2 * (1.0f + 0x1p-20f), tMax));
 		tNear = tn;
 		return tf > tn;
 	}
@@ -114,7 +119,7 @@ namespace luzgpu
 		{
 			return false;
 		}
-		const float inv = 1.0f / det;
+		const float inv = __builtin_amdgcn_rcpf(det);	// 1 ulp, against the FP64 reference's 1 / det
 		u = dot(tv, p) * inv;
 		if (u < 0.0f || u > 1.0f)
 		{
@@ -304,6 +309,21 @@ namespace luzgpu
 		// every LDS access inside this thread's column whatever happens.
 		__device__ __forceinline__ void	push(std::uint32_t v) { base[min(sp, Depth - 1) * Block] = v; sp++; }
 		__device__ __forceinline__ std::uint32_t	pop() { sp--; return base[min(sp, Depth - 1) * Block]; }
+		__device__ __forceinline__ bool	empty() const { return sp == 0; }
+		__device__ __forceinline__ void	reset() { sp = 0; }
+	};
+
+	// A private traversal stack (scratch memory) for the rare traversals made outside the
+	// traversal kernels, such as re-intersecting one emissive mesh instance for its light pdf.
+	// Scenes are admitted only when a BLAS traversal fits (their bound is checked on load).
+	constexpr int	kLocalStackDepth = 48;
+	struct	LocalStack
+	{
+		std::uint32_t	v[kLocalStackDepth];
+		int	sp;
+
+		__device__ __forceinline__ void	push(std::uint32_t x) { v[min(sp, kLocalStackDepth - 1)] = x; sp++; }
+		__device__ __forceinline__ std::uint32_t	pop() { sp--; return v[min(sp, kLocalStackDepth - 1)]; }
 		__device__ __forceinline__ bool	empty() const { return sp == 0; }
 		__device__ __forceinline__ void	reset() { sp = 0; }
 	};
@@ -636,6 +656,36 @@ namespace luzgpu
 	__device__ __forceinline__ std::uint32_t	unboundedOf(const GpuParams& p)
 	{
 		return (p.unboundedBase << 16) | p.unboundedCount;
+	}
+
+	// Closest hit of a world ray against one mesh instance alone (MeshInstance::hit,
+	// MeshInstance.cpp:303-327): its BLAS traversed in local space, t shared with the world
+	// ray (the local direction is not renormalised).
+	__device__ inline bool	instanceClosest(const SceneView& s, std::uint32_t instance, std::uint32_t hittable, const Ray& world,
+		float tMax, HitInfo& hit)
+	{
+		const GpuInstance& in = s.instances[instance];
+		TraceState st;
+		st.world = world;
+		st.ray = toLocal(in, world);
+		st.hit.t = tMax;
+		st.hit.hittable = 0xffffffffu;
+		st.hit.tri = -1;
+		st.nodes = s.blas;
+		st.logU = 0.0f;
+		st.unbounded = 0u;
+		st.entry = static_cast<std::uint32_t>(in.blasRoot);
+		st.instHittable = hittable;
+		st.from = kNoOrigin;
+		st.inInstance = true;
+		st.found = false;
+		LocalStack stack;
+		stack.reset();
+		while (!traceStep<false, false, 0>(s, st, kTMinF, stack, nullptr))
+		{
+		}
+		hit = st.hit;
+		return st.found;
 	}
 
 	// The exclusion identity of a hit (see RayOrigin).

@@ -1272,9 +1272,62 @@ namespace luzgpu
 				l.kind = h.kind == HIT_CUBE ? GPU_HIT_CUBE : GPU_HIT_PLANE;
 				area = 1.0;
 			}
+			else if (h.kind == HIT_INSTANCE)
+			{
+				// MeshInstance::_surfaceArea / _transformedAreaPrefixSums / lightSelectionWeight
+				// (MeshInstance.cpp:345-407,589-603): a uniform scale keeps the mesh's own
+				// prefix sums (Mesh.cpp:525-532) and scales the total, any other scale sums the
+				// transformed triangle areas
+				const Instance& inst = scene.instances[h.index];
+				const Mesh& mesh = scene.meshes[inst.mesh];
+				const InstanceXf xf(inst);
+				const double ax = std::fabs(inst.scale[0]), ay = std::fabs(inst.scale[1]), az = std::fabs(inst.scale[2]);
+				const bool uniform = std::fabs(ax - ay) <= 1e-12 && std::fabs(ax - az) <= 1e-12;
+				const std::vector<std::size_t>& order = p.blas[inst.mesh].triangleIndices;
+				std::vector<std::uint32_t> slotOf(order.size());
+				for (std::size_t j = 0; j < order.size(); j++) slotOf[order[j]] = static_cast<std::uint32_t>(meshTriBase[inst.mesh] + j);
+				l.kind = GPU_HIT_INSTANCE;
+				l.index = p.hittables[i].kindIndex & kIndexMask;
+				l.triBase = static_cast<std::uint32_t>(p.lightTriCdf.size());
+				l.triCount = static_cast<std::uint32_t>(mesh.triangles.size());
+				double total = 0.0;
+				for (std::size_t k = 0; k < mesh.triangles.size(); k++)
+				{
+					const Triangle& t = mesh.triangles[k];
+					double a;
+					if (uniform)
+					{
+						const dvec3 c = cross(t.v1 - t.v0, t.v2 - t.v0);
+						const double ls = dot(c, c);
+						a = ls <= 1e-24 ? 0.0 : std::sqrt(ls) / 2.0;	// Triangle::_updateCache
+					}
+					else
+					{
+						const dvec3 w0 = xf.transformPoint(t.v0), w1 = xf.transformPoint(t.v1), w2 = xf.transformPoint(t.v2);
+						const dvec3 c = cross(w1 - w0, w2 - w0);
+						a = std::sqrt(dot(c, c)) / 2.0;
+						if (!std::isfinite(a) || a <= 0.0) a = 0.0;
+					}
+					total += a;
+					p.lightTriCdf.push_back(total);
+					p.lightTriSlot.push_back(slotOf[k]);
+				}
+				if (!std::isfinite(total) || total <= 0.0) area = 0.0;
+				else area = uniform ? total * (ax * ax) : total;
+				if (!std::isfinite(area) || area <= 0.0) area = 0.0;
+				// local -> world: rotate(p * scale) + position, as a 3x4 map
+				for (int c = 0; c < 3; c++)
+				{
+					dvec3 e{};
+					e[c] = inst.scale[c];
+					const dvec3 r = xf.rotate(e);
+					for (int row = 0; row < 3; row++) l.localToWorld[row * 4 + c] = static_cast<float>(r[row]);
+				}
+				for (int row = 0; row < 3; row++) l.localToWorld[row * 4 + 3] = static_cast<float>(inst.position[row]);
+			}
 			else
 			{
-				throw std::runtime_error("Unsupported for the device path: emissive mesh instance");
+				throw std::runtime_error("Unsupported for the device path: emissive volume");
 			}
 			l.area = static_cast<float>(h.kind == HIT_DIRECTIONAL || h.kind == HIT_CUBE || h.kind == HIT_PLANE ? 0.0 : area);
 			p.lightScale.push_back(area);
@@ -1381,9 +1434,10 @@ namespace luzgpu
 		p.features = 0;
 		for (const Hittable& h : scene.hittables)
 		{
-			if (h.kind == HIT_SPHERE || h.kind == HIT_CUBE || h.kind == HIT_VOLUME || h.kind == HIT_PLANE || h.kind == HIT_DIRECTIONAL)
+			if (h.kind == HIT_SPHERE || h.kind == HIT_CUBE || h.kind == HIT_VOLUME || h.kind == HIT_PLANE || h.kind == HIT_DIRECTIONAL
+				|| (h.kind == HIT_INSTANCE && scene.materials[h.material].kind == MAT_EMISSIVE))
 			{
-				p.features |= kFeatAnalytic;
+				p.features |= kFeatAnalytic;	// the class of rarer primitives and lights
 			}
 		}
 		for (const Material& m : scene.materials)
@@ -1485,13 +1539,20 @@ namespace luzgpu
 			m.materials.insert(m.materials.end(), p.materials.begin(), p.materials.end());
 			GpuParams gp = p.params;
 			gp.lightBase = static_cast<std::uint32_t>(m.lights.size());
+			const std::uint32_t lightTriBase = static_cast<std::uint32_t>(m.lightTriCdf.size());
 			for (GpuLight l : p.lights)
 			{
-				if (l.kind == GPU_HIT_TRIANGLE || l.kind == GPU_HIT_RECTANGLE || l.kind == GPU_HIT_SPHERE) l.index += base[l.kind];
+				if (l.kind == GPU_HIT_TRIANGLE || l.kind == GPU_HIT_RECTANGLE || l.kind == GPU_HIT_SPHERE || l.kind == GPU_HIT_INSTANCE)
+				{
+					l.index += base[l.kind];
+				}
+				if (l.kind == GPU_HIT_INSTANCE) l.triBase += lightTriBase;
 				l.hittable += hittableBase;
 				l.material += materialBase;
 				m.lights.push_back(l);
 			}
+			m.lightTriCdf.insert(m.lightTriCdf.end(), p.lightTriCdf.begin(), p.lightTriCdf.end());
+			for (std::uint32_t slot : p.lightTriSlot) m.lightTriSlot.push_back(slot + meshTriBase);
 			// lightCdf is indexed like the lights: pad an unweighted scene's span
 			if (p.lightCdf.empty()) m.lightCdf.insert(m.lightCdf.end(), p.lights.size(), 0.0);
 			else m.lightCdf.insert(m.lightCdf.end(), p.lightCdf.begin(), p.lightCdf.end());

@@ -52,6 +52,8 @@ namespace luzgpu
 		std::vector<GpuMaterial>	materials;
 		std::vector<GpuLight>	lights;
 		std::vector<double>	lightCdf;
+		std::vector<double>	lightTriCdf;			// emissive instances: triangle-area prefix sums
+		std::vector<std::uint32_t>	lightTriSlot;	// ... and each entry's meshTris slot
 		std::vector<float>	envPixels;	// float4 per texel
 		std::vector<float>	envPdf;		// (weight / total) / solid angle per texel
 		std::vector<double>	envCdf;

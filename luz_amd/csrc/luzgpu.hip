@@ -292,14 +292,15 @@ namespace luzgpu
 	{
 		PackedScene	host;
 		DeviceBuffer	tlas, blas, topTris, meshTris, meshNormals, tlasPrims, hittables, rects, instances, materials,
-			lights, lightCdf, envPixels, envPdf, envCdf, unbounded, spheres, cubeFaces, volumes, planes;
+			lights, lightCdf, lightTriCdf, lightTriSlot, envPixels, envPdf, envCdf, unbounded, spheres, cubeFaces, volumes, planes;
 		SceneView	view{};
 		luzgpu_scene_info	info{};
 
 		void	releaseDevice()
 		{
 			for (DeviceBuffer* b : {&tlas, &blas, &topTris, &meshTris, &meshNormals, &tlasPrims, &hittables, &rects, &instances,
-				&materials, &lights, &lightCdf, &envPixels, &envPdf, &envCdf, &unbounded, &spheres, &cubeFaces, &volumes, &planes})
+				&materials, &lights, &lightCdf, &lightTriCdf, &lightTriSlot, &envPixels, &envPdf, &envCdf, &unbounded, &spheres,
+				&cubeFaces, &volumes, &planes})
 			{
 				b->release();
 			}
@@ -388,6 +389,8 @@ namespace luzgpu
 			g.materials.upload(h.materials, st);
 			g.lights.upload(h.lights, st);
 			g.lightCdf.upload(h.lightCdf, st);
+			g.lightTriCdf.upload(h.lightTriCdf, st);
+			g.lightTriSlot.upload(h.lightTriSlot, st);
 			g.envPixels.upload(h.envPixels, st);
 			g.envPdf.upload(h.envPdf, st);
 			g.envCdf.upload(h.envCdf, st);
@@ -410,6 +413,8 @@ namespace luzgpu
 			v.materials = g.materials.as<GpuMaterial>();
 			v.lights = g.lights.as<GpuLight>();
 			v.lightCdf = h.lightCdf.empty() ? nullptr : g.lightCdf.as<double>();
+			v.lightTriCdf = h.lightTriCdf.empty() ? nullptr : g.lightTriCdf.as<double>();
+			v.lightTriSlot = h.lightTriSlot.empty() ? nullptr : g.lightTriSlot.as<std::uint32_t>();
 			v.envPixels = g.envPixels.as<float4>();
 			v.envPdf = g.envPdf.as<float>();
 			v.envCdf = g.envCdf.as<double>();

@@ -333,6 +333,52 @@ def primitives_scene_text(width=160, height=120, spp=32, depth=8) -> str:
     return "\n".join(lines) + "\n"
 
 
+_QUAD_OBJ = "v -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\nf 1 2 3\nf 1 3 4\n"
+
+
+def meshlight_scene_text(width=160, height=120, spp=32, depth=8) -> str:
+    """Emissive mesh instances: a displaced sphere under a non-uniform scale (transformed
+    area prefix sums), a quad under a uniform scale (the mesh's own prefix sums), a rectangle
+    light beside them, a lit mesh and a floor plane."""
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "sky=none", "background=(0.02,0.02,0.03)", "",
+        "[materials]",
+        "material floor {", "type=lambertian", "color=(0.55,0.55,0.5)", "}",
+        "material clay {", "type=lambertian", "color=(0.7,0.35,0.2)", "}",
+        "material steel {", "type=metal", "color=(0.8,0.8,0.85)", "roughness=0.3", "}",
+        "material bulb {", "type=emissive", "color=(1,0.8,0.55)", "radiance=9", "}",
+        "material sheet {", "type=emissive", "color=(0.55,0.7,1)", "radiance=3", "}",
+        "material strip {", "type=emissive", "color=(1,1,1)", "radiance=2", "}",
+        "",
+        "[scene]",
+        "camera main {", "position=(0,3,-12)", "direction=(0,-0.15,1)", "up=(0,1,0)", "focal_length_mm=35",
+        "sensor_width_mm=36", "sensor_height_mm=27", "pinhole=1", "focus_distance=10", "}",
+        "object bulb {", "file=bulb.obj", "position=(-1.5,4.5,3)", "rotation=(20,35,0)", "scale=(0.9,0.6,1.2)",
+        "material=bulb", "}",
+        "object sheet {", "file=quad.obj", "position=(3,3,5)", "rotation=(70,-20,0)", "scale=(1.5,1.5,1.5)",
+        "material=sheet", "}",
+        "object blob {", "file=blob.obj", "position=(1,1.2,2)", "rotation=(0,40,0)", "scale=(1,1,1)", "material=clay", "}",
+        "object ball {", "file=bulb.obj", "position=(-2.5,1,0.5)", "scale=(1,1,1)", "material=steel", "}",
+        "objects{",
+        "plane=0,(0,1,0),material=floor",
+        "rectangle=(0,6,7),(0,-1,-0.4),4,0.5,material=strip",
+        "}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def write_meshlight(directory: str, width=160, height=120, spp=32, depth=8, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    _write_if_changed(os.path.join(directory, "bulb.obj"), _sphere_obj(random.Random(1006), 1.0).encode())
+    _write_if_changed(os.path.join(directory, "blob.obj"), _sphere_obj(random.Random(1007), 1.1).encode())
+    _write_if_changed(os.path.join(directory, "quad.obj"), _QUAD_OBJ.encode())
+    path = os.path.join(directory, "meshlight.luz")
+    _write_if_changed(path, meshlight_scene_text(width, height, spp, depth).encode())
+    return Workload("meshlight", directory, path, width, height, spp, depth, seed,
+                    3 * 2 * SPHERE_SLICES * (SPHERE_STACKS - 1) + 2)
+
+
 def write_primitives(directory: str, width=160, height=120, spp=32, depth=8, seed=1) -> Workload:
     os.makedirs(directory, exist_ok=True)
     path = os.path.join(directory, "primitives.luz")

@@ -146,7 +146,7 @@ namespace oracle
 
 	enum Dim : std::uint32_t
 	{
-		D_CAMERA = 0, D_LENS = 1, D_LIGHT_STRATEGY = 2, D_LIGHT_SELECTION = 3, D_LIGHT_SURFACE_POINT = 5,
+		D_CAMERA = 0, D_LENS = 1, D_LIGHT_STRATEGY = 2, D_LIGHT_SELECTION = 3, D_LIGHT_SURFACE_SELECTION = 4, D_LIGHT_SURFACE_POINT = 5,
 		D_BSDF_DIRECTION = 6, D_MATERIAL_DECISION = 7, D_RR = 9, D_VOLUME_DISTANCE = 10, D_ENV_SELECTION = 12,
 		D_ENV_POINT = 13, D_ENV_STRATEGY = 14
 	};
@@ -564,6 +564,10 @@ namespace oracle
 			nz = (-y * sx) + (z * cx);
 			return {x, ny, nz};
 		}
+		V	transformPoint(V p) const	// :123-126
+		{
+			return rotate({p.x * scale.x, p.y * scale.y, p.z * scale.z}) + position;
+		}
 		V	transformNormal(V n) const	// :151-164
 		{
 			const V s = rotate({n.x / scale.x, n.y / scale.y, n.z / scale.z});
@@ -610,6 +614,9 @@ namespace oracle
 		double	area;
 		V	dir;	// directional: -direction
 		std::uint32_t	material = 0;
+		// emissive mesh instance: the triangle-area prefix sums its sampler searches (the
+		// mesh's own when the instance scales area uniformly, else the transformed ones)
+		std::vector<double>	triCdf;
 	};
 
 	struct	Env
@@ -834,7 +841,41 @@ namespace oracle
 				if (t.kind == K_DIR) l.dir = s.directionals[t.index] * -1.0;
 				w = (!std::isfinite(lum) || lum <= 0.0) ? 0.0 : lum;
 			}
-			else throw std::runtime_error("oracle: emissive mesh instances are not restated");
+			else if (t.kind == K_INST)
+			{
+				// MeshInstance::_surfaceArea / _transformedAreaPrefixSums / lightSelectionWeight
+				// (MeshInstance.cpp:345-407,589-603); Mesh prefix sums (Mesh.cpp:525-532)
+				const Instance& inst = s.instances[t.index];
+				const Mesh& m = s.meshes[inst.mesh];
+				const double ax = std::fabs(inst.scale.x), ay = std::fabs(inst.scale.y), az = std::fabs(inst.scale.z);
+				const bool uniform = std::fabs(ax - ay) <= 1e-12 && std::fabs(ax - az) <= 1e-12;
+				double total = 0.0;
+				if (uniform)
+				{
+					for (const Tri& tr : m.tris)
+					{
+						total += tr.area;
+						l.triCdf.push_back(total);
+					}
+					const double a = total * (ax * ax);
+					l.area = (std::isfinite(a) && a > 0.0) ? a : 0.0;
+					if (!(std::isfinite(total) && total > 0.0)) l.triCdf.clear();
+				}
+				else
+				{
+					for (const Tri& tr : m.tris)
+					{
+						const V w0 = inst.transformPoint(tr.v0), w1 = inst.transformPoint(tr.v1), w2 = inst.transformPoint(tr.v2);
+						const double a = std::sqrt(len2(cross(w1 - w0, w2 - w0))) / 2.0;
+						total += (std::isfinite(a) && a > 0.0) ? a : 0.0;
+						l.triCdf.push_back(total);
+					}
+					if (!(std::isfinite(total) && total > 0.0)) l.triCdf.clear();
+					l.area = l.triCdf.empty() ? 0.0 : total;
+				}
+				w = (!std::isfinite(lum) || lum <= 0.0 || l.area <= 0.0) ? 0.0 : l.area * lum;
+			}
+			else throw std::runtime_error("oracle: emissive volumes are not restated");
 			if (!std::isfinite(w) || w <= 0.0) w = 0.0;
 			s.lightTotal += w;
 			s.lightCdf.push_back(s.lightTotal);
@@ -1643,6 +1684,42 @@ namespace oracle
 			out.valid = maxChannel(out.emitted) > 0.0;
 			return out;
 		}
+		if (L.kind == K_INST)	// MeshInstance::sampleLight / _sampleSurface (MeshInstance.cpp:409-466,510-546)
+		{
+			const Instance& inst = s.instances[L.index];
+			const Mesh& m = s.meshes[inst.mesh];
+			if (L.triCdf.empty() || L.area <= 0.0) return out;
+			// Mesh::sampleSurface or the transformed prefix sums: lower_bound of u * total
+			const double target = ps.s1(D_LIGHT_SURFACE_SELECTION) * L.triCdf.back();
+			const auto it = std::lower_bound(L.triCdf.begin(), L.triCdf.end(), target);
+			const std::size_t k = std::min<std::size_t>(static_cast<std::size_t>(it - L.triCdf.begin()), L.triCdf.size() - 1);
+			const double prev = k == 0 ? 0.0 : L.triCdf[k - 1];
+			if (L.triCdf[k] - prev <= 0.0) return out;
+			const Tri& tr = m.tris[k];	// Triangle::sampleSurface (Triangle.cpp:320-341)
+			if (tr.degenerate || tr.area <= 0.0) return out;
+			float sx, sy;
+			ps.s2(D_LIGHT_SURFACE_POINT, sx, sy);
+			double u = sx, v = sy;
+			if (u + v > 1.0) { u = 1.0 - u; v = 1.0 - v; }
+			const V local = tr.v0 + ((tr.v1 - tr.v0) * u) + ((tr.v2 - tr.v0) * v);
+			const V position = inst.transformPoint(local);
+			const V normal = inst.transformNormal(tr.face);
+			if (!(std::isfinite(position.x) && std::isfinite(position.y) && std::isfinite(position.z)) || len2(normal) <= 0.0) return out;
+			const V d = position - origin;
+			const double d2 = len2(d);
+			if (!std::isfinite(d2) || d2 <= 0.0) return out;
+			const double dist = std::sqrt(d2);
+			out.dir = d / dist;
+			const double c = std::fabs(dot(out.dir, normal));
+			if (!std::isfinite(c) || c <= 0.0) return out;
+			const double pdf = d2 / (c * L.area);
+			if (!(std::isfinite(pdf) && pdf > 0.0 && std::isfinite(dist) && dist > T_MIN)) return out;
+			out.pdf = pdf * sel;
+			out.tMax = dist;
+			out.emitted = L.emitted;
+			out.valid = maxChannel(out.emitted) > 0.0;
+			return out;
+		}
 		if (L.kind != K_RECT && L.kind != K_TRI) return out;	// Hittable::sampleLight default
 		float sx, sy;
 		ps.s2(D_LIGHT_SURFACE_POINT, sx, sy);
@@ -1689,6 +1766,27 @@ namespace oracle
 			if (!sp.intersect(r, T_MIN, T_MAX, t)) return 0.0;
 			const double cosMax = std::sqrt(1.0 - sp.r * sp.r / len2(sp.c - origin));
 			return 1.0 / (2.0 * PI * (1.0 - cosMax));
+		}
+		if (L.kind == K_INST)	// MeshInstance::pdfValue (MeshInstance.cpp:468-497): this instance alone
+		{
+			if (L.area <= 0.0) return 0.0;
+			const double vl = std::sqrt(len2(vec));
+			if (!std::isfinite(vl) || vl <= 0.0) return 0.0;
+			const Instance& inst = s.instances[L.index];
+			Ray local;
+			double scale;
+			if (!inst.localRay(r, local, scale)) return 0.0;
+			const Mesh& m = s.meshes[inst.mesh];
+			std::uint32_t tri;
+			double lt, u, v;
+			if (!m.bvh.trace(m.tris, local, T_MIN * scale, T_MAX * scale, false, tri, lt, u, v)) return 0.0;
+			const double wt = lt / scale;
+			if (!std::isfinite(wt) || wt < T_MIN || wt > T_MAX) return 0.0;
+			const V gn = inst.transformNormal(m.tris[tri].face);
+			const double d2 = wt * wt * len2(vec);
+			const double c = std::fabs(dot(vec, gn) / vl);
+			if (!std::isfinite(d2) || !std::isfinite(c) || c <= 0.0) return 0.0;
+			return d2 / (c * L.area);
 		}
 		if (L.kind != K_RECT && L.kind != K_TRI) return 0.0;
 		if (L.kind == K_RECT)

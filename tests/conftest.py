@@ -42,6 +42,12 @@ def materials(workload_dir):
 
 
 @pytest.fixture(scope="session")
+def meshlight(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_meshlight(os.path.join(workload_dir, "meshlight"))
+
+
+@pytest.fixture(scope="session")
 def volume(workload_dir):
     from luz_amd import workloads
     return workloads.write_volume(os.path.join(workload_dir, "volume"))

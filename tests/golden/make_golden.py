@@ -185,6 +185,8 @@ def main():
             small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
         if "materials" in only:
             small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
+        if "meshlight" in only:
+            small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
         if "fd" in only:
             fd_fixture(tmp)
         if "zstats" in only:
@@ -194,6 +196,7 @@ def main():
     small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
     small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
     small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
+    small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
     fd_fixture(tmp)
     zstat_fixtures(tmp)
 

@@ -13,9 +13,9 @@ from test_gpu_parity import REL_MSE_BAR, _golden
 pytestmark = pytest.mark.gpu
 
 
-def test_batch_matches_single_scene_renders_and_goldens(ctx, cornell, primitives, volume, materials):
+def test_batch_matches_single_scene_renders_and_goldens(ctx, cornell, primitives, volume, materials, meshlight):
     from luz_amd import Renderer, SceneFile
-    wls = [cornell, primitives, volume, materials]
+    wls = [cornell, primitives, volume, materials, meshlight]
     goldens = ["cornell_300_50spp_seed1.npz"] + [f"{w.name}_{w.width}x{w.height}_{w.spp}spp_seed{w.seed}.npz"
                                                  for w in wls[1:]]
     scenes = [SceneFile.read(ctx, w.scene_path) for w in wls]
@@ -29,7 +29,7 @@ def test_batch_matches_single_scene_renders_and_goldens(ctx, cornell, primitives
         assert e_single < 1e-10
         assert e_golden <= REL_MSE_BAR
     # the merged arrays are cached: a second call with other seeds reuses them
-    again, _ = Renderer.render_batch(scenes, seeds=[w.seed + 1 for w in wls], samples=[4] * 4)
+    again, _ = Renderer.render_batch(scenes, seeds=[w.seed + 1 for w in wls], samples=[4] * len(wls))
     for w, sc, img in zip(wls, scenes, again):
         single, _ = Renderer.render(sc, seed=w.seed + 1, samples=4)
         assert rel_mse(img, single) < 1e-10

@@ -2,7 +2,9 @@
 
 Scenes: ``primitives`` (plane, spheres, cubes, sphere / rectangle / cube emitters, constant
 background), ``volume`` (Henyey-Greenstein box fog, isotropic sphere smoke, directional sun,
-sphere light) and ``materials`` (glossy, diffuse-glossy and principled variants). Same bars as test_gpu_parity: primary-hit IDs against the reference's
+sphere light), ``materials`` (glossy, diffuse-glossy and principled variants) and
+``meshlight`` (emissive mesh instances under uniform and non-uniform scales, sampled by
+triangle area and re-intersected for their MIS pdf). Same bars as test_gpu_parity: primary-hit IDs against the reference's
 FP64 brute force, images within relative MSE 1e-4 of the reference render at equal spp and
 seed (tests/golden/make_golden.py)."""
 
@@ -15,7 +17,7 @@ from test_gpu_parity import REL_MSE_BAR, _golden, _primary_check
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight"])
 def test_primary_hits(ctx, request, name):
     from luz_amd import SceneFile
     wl = request.getfixturevalue(name)
@@ -24,7 +26,7 @@ def test_primary_hits(ctx, request, name):
 
 
 @pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight"])
 def test_image_parity(ctx, request, name, integrator):
     from luz_amd import Renderer, SceneFile
     wl = request.getfixturevalue(name)

@@ -76,6 +76,14 @@ def test_primitive_and_volume_scenes_match_reference(primitives, volume):
     assert v.info["hittables"] == 8 and v.info["lights"] == 2
 
 
+def test_mesh_light_scene_matches_reference(meshlight):
+    """Emissive mesh instances are lights (MeshInstance::lightSelectionWeight) beside the
+    rectangle; the TLAS over four instances and a rectangle (the plane stays outside)."""
+    m = SceneFile.read(None, meshlight.scene_path)
+    assert m.tlas_topology() == open(os.path.join(GOLDEN, "meshlight_tlas.txt")).read().strip()
+    assert m.info["hittables"] == 6 and m.info["lights"] == 3 and m.info["instances"] == 4
+
+
 def test_volume_and_phase_validation(tmp_path):
     """HenyeyGreenstein.cpp:10-18 and Objects.cpp:491-497 reject these inputs."""
     import pytest
