@@ -230,12 +230,59 @@ namespace luzgpu
 		__device__ __forceinline__ void	continuation(const GpuMaterial&, std::uint32_t, const Surface&, const Scatter&, f3, f3) {}
 	};
 
+	// Denoising features of a primary hit or miss (ColorHelper.cpp:725-799), 11 values:
+	// pixel coordinates, hit flag, t / (1 + t), normal, log-normalised albedo, material type.
+	__device__ __forceinline__ float	normalizedColorChannel(float v)
+	{
+		if (!isfinite(v) || v <= 0.0f) return 0.0f;
+		return fminf(1.0f, log1pf(v) / log1pf(1.0e12f));
+	}
+
+	// MaterialType of the reference (MaterialTypes.hpp): LAMBERTIAN 1 ... PRINCIPLED 7, GLOSSY 8,
+	// DIFFUSE_GLOSSY 9
+	__device__ __forceinline__ float	referenceMaterialType(std::uint32_t kind)
+	{
+		const std::uint32_t t = kind == GPU_MAT_GLOSSY ? 8u : kind == GPU_MAT_DIFFUSE_GLOSSY ? 9u : kind == GPU_MAT_PRINCIPLED ? 7u : kind;
+		return static_cast<float>(t) / 7.0f;
+	}
+
+	__device__ __forceinline__ void	primaryFeatures(float* f, const GpuParams& p, std::uint32_t x, std::uint32_t y, bool hit, float t,
+		f3 normal, f3 albedo, std::uint32_t kind)
+	{
+		f[0] = p.width > 1u ? static_cast<float>(static_cast<double>(x) / (static_cast<double>(p.width) - 1.0)) : 0.0f;
+		f[1] = p.height > 1u ? static_cast<float>(static_cast<double>(y) / (static_cast<double>(p.height) - 1.0)) : 0.0f;
+		if (!hit)
+		{
+			f[2] = 0.0f;
+			f[3] = 1.0f;
+			f[4] = f[5] = f[6] = 0.5f;
+			f[10] = 0.0f;
+		}
+		else
+		{
+			const float l2 = len2(normal);
+			if (l2 > 0.0f) normal = normal * (1.0f / sqrtf(l2));
+			f[2] = 1.0f;
+			f[3] = t > 0.0f ? t / (1.0f + t) : 0.0f;
+			f[4] = normal.x * 0.5f + 0.5f;
+			f[5] = normal.y * 0.5f + 0.5f;
+			f[6] = normal.z * 0.5f + 0.5f;
+			f[10] = referenceMaterialType(kind);
+		}
+		f[7] = normalizedColorChannel(albedo.x);
+		f[8] = normalizedColorChannel(albedo.y);
+		f[9] = normalizedColorChannel(albedo.z);
+	}
+
+	// One path. With `features` (denoising), the primary hit's or miss's features are written
+	// there and the sampler runs on `stream` (1 / 2 for the two half-buffers, Threads.cpp:594-598).
 	template <bool Count, int Feat, typename Stack, typename Adj = NoAdjoint>
 	__device__ f3	tracePath(const SceneView& s, const GpuParams& p, std::uint32_t x, std::uint32_t y, std::uint32_t sample,
-		Stack& stack, std::uint32_t& closestRays, std::uint32_t& shadowRays, VisitCounter& visits, Adj* adj = nullptr)
+		Stack& stack, std::uint32_t& closestRays, std::uint32_t& shadowRays, VisitCounter& visits, Adj* adj = nullptr,
+		std::uint32_t stream = 0u, float* features = nullptr)
 	{
 		PixelSampler ps;
-		ps.begin(p.renderSeed, x, y, sample, 0);
+		ps.begin(p.renderSeed, x, y, sample, stream);
 		Ray ray = cameraRay(p, x, y, ps);
 		f3 acc = mk(0, 0, 0), thr = mk(1, 1, 1), prevOrigin = mk(0, 0, 0);
 		RayOrigin from = kNoOrigin;
@@ -255,6 +302,10 @@ namespace luzgpu
 				{
 					w = powerHeuristic(prevPdf, infiniteLightPdf(s, p, ray.d));
 				}
+				if (b == 0 && features)
+				{
+					primaryFeatures(features, p, x, y, false, 0.0f, mk(0, 0, 0), missRadiance(s, p, ray.d, 1.0f), 0u);
+				}
 				f3 c = thr * missRadiance(s, p, ray.d, w);
 				if (b > 0 && hasConstantBackgroundLight(p)) c = clampConstantBackground(p, c);
 				c = clampRay(c);
@@ -264,6 +315,10 @@ namespace luzgpu
 			const Surface sf = surfaceAt<Feat>(s, ray, h);
 			from = originOf(s, h);
 			const GpuMaterial& m = s.materials[sf.material];
+			if (b == 0 && features)
+			{
+				primaryFeatures(features, p, x, y, true, h.t, sf.normal, ld3(m.color), m.kind);
+			}
 			const f3 emitted = m.kind == GPU_MAT_EMISSIVE ? ld3(m.color) : mk(0, 0, 0);
 			Scatter sc;
 			if (!scatter<Feat>(m, sf, ray.d, ps, sc))

@@ -1,0 +1,519 @@
+// NFOR denoiser on the device (Denoise::applyNFOR, src/core/Denoise/NFOR.cpp:977-1102), in
+// FP64 like the reference.
+//
+// Inputs are the half-buffer statistics the integrator accumulates with denoising on
+// (Threads.cpp:274-372): mean colours of the even and odd samples (A / B), the per-pixel
+// colour variance, the mean primary features of each half (11 per pixel) and their variance.
+// Stages, one kernel each:
+//   firefly suppression of A, B and of their average (NFOR.cpp:145-211);
+//   feature prefiltering, each half guided by the other (NL-means weights, :403-464);
+//   first-order feature regression of A with B's features and of B with A's, at two
+//   bandwidths (k = 0.5, 1) (:739-832);
+//   MSE estimation, its and the bandwidth selection's cross-bilateral filtering (:834-936);
+//   per-pixel blend, then a final regression of the blend (:1074-1101).
+//
+// The regression solves a 12 x 12 weighted least-squares system per pixel. One wave of 64
+// lanes owns one pixel: for each window sample, lanes share the patch distance (a wave
+// reduction) and the 12-entry basis (lane shuffles), and each lane accumulates its own slice
+// of the two systems' upper triangles and right-hand sides in registers; the systems are
+// then solved by Gauss-Jordan elimination with partial pivoting in LDS, lanes working on
+// the elements of each pivot step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace luzgpu
+{
+	constexpr int	kNforFeatures = 11;
+	constexpr int	kNforBasis = kNforFeatures + 1;			// REGRESSION_SIZE
+	constexpr int	kNforUpper = kNforBasis * (kNforBasis + 1) / 2;	// 78 upper-triangle entries
+	constexpr int	kNforSystem = kNforUpper + kNforBasis * 3;		// + the RGB right-hand side
+
+	struct	NforParams
+	{
+		std::uint32_t	width, height;
+		std::uint32_t	prefilterRadius, regressionRadius, patchRadius;
+		double	minVariance;
+	};
+
+	__device__ __forceinline__ std::uint32_t	nforClamp(long long v, std::uint32_t limit)
+	{
+		return v < 0 ? 0u : (v >= static_cast<long long>(limit) ? limit - 1u : static_cast<std::uint32_t>(v));
+	}
+
+	// colorDistanceSquared, colorLuminance (NFOR.cpp:84-103)
+	__device__ __forceinline__ double	nforDist2(const double* a, const double* b)
+	{
+		const double r = a[0] - b[0], g = a[1] - b[1], bl = a[2] - b[2];
+		return ((r * r) + (g * g) + (bl * bl)) / 3.0;
+	}
+
+	__device__ __forceinline__ double	nforLuminance(const double* c)
+	{
+		const double l = c[0] * 0.2722287168 + c[1] * 0.6740817658 + c[2] * 0.0536895174;
+		return (!isfinite(l) || l <= 0.0) ? 0.0 : l;
+	}
+
+	// patchDistance (NFOR.cpp:236-311); border patches clamp their coordinates
+	__device__ inline double	nforPatchDistance(const double* guide, const double* variance, const NforParams& P, std::uint32_t x1,
+		std::uint32_t y1, std::uint32_t x2, std::uint32_t y2, std::uint32_t patchRadius)
+	{
+		if (x1 == x2 && y1 == y2) return 0.0;
+		const long long r = patchRadius;
+		double distance = 0.0;
+		unsigned samples = 0;
+		for (long long oy = -r; oy <= r; oy++)
+		{
+			for (long long ox = -r; ox <= r; ox++)
+			{
+				const std::uint32_t ia = nforClamp(static_cast<long long>(y1) + oy, P.height) * P.width + nforClamp(static_cast<long long>(x1) + ox, P.width);
+				const std::uint32_t ib = nforClamp(static_cast<long long>(y2) + oy, P.height) * P.width + nforClamp(static_cast<long long>(x2) + ox, P.width);
+				const double lv = variance[ia] + variance[ib] + P.minVariance;
+				distance += nforDist2(guide + 3 * ia, guide + 3 * ib) / lv;
+				samples++;
+			}
+		}
+		return distance / static_cast<double>(samples);
+	}
+
+	// colorWeightFromDistance (NFOR.cpp:313-318)
+	__device__ __forceinline__ double	nforWeight(double distance, double bandwidth, double minVariance)
+	{
+		return exp(-distance / fmax(minVariance, bandwidth * bandwidth));
+	}
+
+	// suppressDenoiseFireflies (NFOR.cpp:145-211): a bright pixel with fewer than two similar
+	// neighbours in its 5 x 5 window that stands 2.5x and 0.18 above the window's median
+	// luminance takes the luminance-median of its darker neighbours. `average` != 0: the
+	// input is the mean of a and b (averageColorBuffers) instead of a.
+	__global__ void	nfor_fireflies(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out,
+		NforParams P, int average)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= P.width * P.height) return;
+		auto colorAt = [&](std::uint32_t k, double* c) {
+			if (average)
+			{
+				for (int j = 0; j < 3; j++) c[j] = (a[3 * k + j] + b[3 * k + j]) * 0.5;
+			}
+			else
+			{
+				for (int j = 0; j < 3; j++) c[j] = a[3 * k + j];
+			}
+		};
+		double center[3];
+		colorAt(i, center);
+		for (int j = 0; j < 3; j++) out[3 * i + j] = center[j];
+		if (P.width < 3 || P.height < 3) return;
+		const std::uint32_t x = i % P.width, y = i / P.width;
+		const double lum = nforLuminance(center);
+		double neighbor[24];
+		std::uint32_t candidate[24];
+		double candidateLum[24];
+		int n = 0, nc = 0;
+		unsigned similar = 0;
+		for (int oy = -2; oy <= 2; oy++)
+		{
+			for (int ox = -2; ox <= 2; ox++)
+			{
+				if (ox == 0 && oy == 0) continue;
+				const long long sx = static_cast<long long>(x) + ox, sy = static_cast<long long>(y) + oy;
+				if (sx < 0 || sy < 0 || sx >= P.width || sy >= P.height) continue;
+				const std::uint32_t k = static_cast<std::uint32_t>(sy) * P.width + static_cast<std::uint32_t>(sx);
+				double c[3];
+				colorAt(k, c);
+				const double l = nforLuminance(c);
+				neighbor[n++] = l;
+				if (l >= lum * 0.65) similar++;
+				else
+				{
+					candidate[nc] = k;
+					candidateLum[nc] = l;
+					nc++;
+				}
+			}
+		}
+		// isDenoiseFirefly (NFOR.cpp:115-143)
+		if (nc == 0 || lum < 0.30 || n == 0 || similar >= 2u) return;
+		for (int p = 1; p < n; p++)
+		{
+			const double v = neighbor[p];
+			int q = p - 1;
+			while (q >= 0 && neighbor[q] > v)
+			{
+				neighbor[q + 1] = neighbor[q];
+				q--;
+			}
+			neighbor[q + 1] = v;
+		}
+		const double median = neighbor[n / 2];
+		if (!(lum >= median * 2.5 && lum >= median + 0.18)) return;
+		// medianColorByLuminance (NFOR.cpp:105-113)
+		for (int p = 1; p < nc; p++)
+		{
+			const double v = candidateLum[p];
+			const std::uint32_t k = candidate[p];
+			int q = p - 1;
+			while (q >= 0 && candidateLum[q] > v)
+			{
+				candidateLum[q + 1] = candidateLum[q];
+				candidate[q + 1] = candidate[q];
+				q--;
+			}
+			candidateLum[q + 1] = v;
+			candidate[q + 1] = k;
+		}
+		double c[3];
+		colorAt(candidate[nc / 2], c);
+		for (int j = 0; j < 3; j++) out[3 * i + j] = c[j];
+	}
+
+	// doubledVariance and averageFeatureVarianceMap (NFOR.cpp:331-353)
+	__global__ void	nfor_variances(const double* __restrict__ colorVariance, const double* __restrict__ featureVariance,
+		double* __restrict__ regressionVariance, double* __restrict__ featureVarianceAverage, std::uint32_t n)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= n) return;
+		regressionVariance[i] = colorVariance[i] * 2.0;
+		double v = 0.0;
+		for (int k = 0; k < kNforFeatures; k++) v += featureVariance[kNforFeatures * i + k];
+		featureVarianceAverage[i] = v / static_cast<double>(kNforFeatures);
+	}
+
+	// prefilterFeatures (NFOR.cpp:403-464): `image`'s features averaged with weights from
+	// `guide`'s feature distances and a spatial Gaussian.
+	__global__ void	nfor_prefilter(const double* __restrict__ image, const double* __restrict__ guide,
+		const double* __restrict__ varianceAverage, double* __restrict__ out, NforParams P)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= P.width * P.height) return;
+		const std::uint32_t x = i % P.width, y = i / P.width;
+		const long long r = P.prefilterRadius;
+		const double sigma = fmax(1.0, static_cast<double>(P.prefilterRadius));
+		double sum[kNforFeatures] = {};
+		double total = 0.0;
+		for (long long oy = -r; oy <= r; oy++)
+		{
+			for (long long ox = -r; ox <= r; ox++)
+			{
+				const std::uint32_t k = nforClamp(static_cast<long long>(y) + oy, P.height) * P.width + nforClamp(static_cast<long long>(x) + ox, P.width);
+				const double lv = varianceAverage[i] + varianceAverage[k] + P.minVariance;
+				double d = 0.0;
+				for (int f = 0; f < kNforFeatures; f++)
+				{
+					const double diff = guide[kNforFeatures * i + f] - guide[kNforFeatures * k + f];
+					d += diff * diff;
+				}
+				const double distance = (d / static_cast<double>(kNforFeatures)) / lv;
+				const double spatial = static_cast<double>(ox * ox + oy * oy);
+				const double w = exp(-distance / 0.25) * exp(-spatial / (2.0 * sigma * sigma));
+				for (int f = 0; f < kNforFeatures; f++) sum[f] += image[kNforFeatures * k + f] * w;
+				total += w;
+			}
+		}
+		for (int f = 0; f < kNforFeatures; f++)
+		{
+			out[kNforFeatures * i + f] = total <= 2.220446049250313e-16 ? image[kNforFeatures * i + f] : sum[f] / total;
+		}
+	}
+
+	// weightedAverageFallback (NFOR.cpp:518-553), when a system is singular
+	__device__ inline void	nforFallback(const double* image, const double* variance, const NforParams& P, std::uint32_t x, std::uint32_t y,
+		double* out)
+	{
+		const long long r = P.regressionRadius;
+		double acc[3] = {0.0, 0.0, 0.0}, total = 0.0;
+		for (long long oy = -r; oy <= r; oy++)
+		{
+			for (long long ox = -r; ox <= r; ox++)
+			{
+				const std::uint32_t k = nforClamp(static_cast<long long>(y) + oy, P.height) * P.width + nforClamp(static_cast<long long>(x) + ox, P.width);
+				const double spatial = static_cast<double>(ox * ox + oy * oy);
+				const double w = (1.0 / (variance[k] + P.minVariance)) * exp(-spatial / 8.0);
+				for (int j = 0; j < 3; j++) acc[j] += image[3 * k + j] * w;
+				total += w;
+			}
+		}
+		const std::uint32_t c = y * P.width + x;
+		for (int j = 0; j < 3; j++) out[j] = total <= 2.220446049250313e-16 ? image[3 * c + j] : acc[j] / total;
+	}
+
+	__device__ __forceinline__ double	nforWaveSum(double v)
+	{
+		for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+		return v;
+	}
+
+	// Solves the system in `m` (kNforBasis x kNforBasis, row-major) with right-hand side `rhs`
+	// (kNforBasis x 3) in place (solveLinearSystemRGB, NFOR.cpp:466-516); returns false when a
+	// pivot falls below 1e-10. All 64 lanes of the block's single wave take part.
+	__device__ inline bool	nforSolve(double* m, double* rhs, int* ok)
+	{
+		const int lane = static_cast<int>(threadIdx.x);
+		constexpr int W = kNforBasis + 3;	// columns incl. the right-hand side
+		for (int pivot = 0; pivot < kNforBasis; pivot++)
+		{
+			if (lane == 0)
+			{
+				int best = pivot;
+				double bestValue = fabs(m[pivot * kNforBasis + pivot]);
+				for (int row = pivot + 1; row < kNforBasis; row++)
+				{
+					const double v = fabs(m[row * kNforBasis + pivot]);
+					if (v > bestValue)
+					{
+						bestValue = v;
+						best = row;
+					}
+				}
+				ok[0] = bestValue < 1e-10 ? 0 : 1;
+				ok[1] = best;
+			}
+			__syncthreads();
+			if (!ok[0]) return false;
+			const int best = ok[1];
+			if (best != pivot && lane < W)
+			{
+				double* a = lane < kNforBasis ? m + pivot * kNforBasis + lane : rhs + pivot * 3 + (lane - kNforBasis);
+				double* b = lane < kNforBasis ? m + best * kNforBasis + lane : rhs + best * 3 + (lane - kNforBasis);
+				const double t = *a;
+				*a = *b;
+				*b = t;
+			}
+			__syncthreads();
+			const double pivotValue = m[pivot * kNforBasis + pivot];
+			__syncthreads();
+			if (lane < W && (lane >= kNforBasis || lane >= pivot))
+			{
+				double* a = lane < kNforBasis ? m + pivot * kNforBasis + lane : rhs + pivot * 3 + (lane - kNforBasis);
+				*a /= pivotValue;
+			}
+			__syncthreads();
+			// each lane: a set of (row, column) elements; factors are read before any update
+			double factor[3];
+			int rows[3], cols[3], cnt = 0;
+			for (int e = lane; e < kNforBasis * W; e += 64)
+			{
+				const int row = e / W, col = e % W;
+				if (row == pivot || (col < kNforBasis && col < pivot)) continue;
+				rows[cnt] = row;
+				cols[cnt] = col;
+				factor[cnt] = m[row * kNforBasis + pivot];
+				cnt++;
+			}
+			__syncthreads();
+			for (int k = 0; k < cnt; k++)
+			{
+				const int row = rows[k], col = cols[k];
+				if (col < kNforBasis) m[row * kNforBasis + col] -= factor[k] * m[pivot * kNforBasis + col];
+				else rhs[row * 3 + (col - kNforBasis)] -= factor[k] * rhs[pivot * 3 + (col - kNforBasis)];
+			}
+			__syncthreads();
+		}
+		return true;
+	}
+
+	// regressionPass / regressionPairPass (NFOR.cpp:679-832): one 64-lane block per pixel.
+	// `second` null: one bandwidth (b0) into `first`; else b0 into `first` and b1 into `second`.
+	__global__ __launch_bounds__(64) void	nfor_regression(const double* __restrict__ image, const double* __restrict__ features,
+		const double* __restrict__ variance, double* __restrict__ first, double* __restrict__ second, NforParams P, double b0,
+		double b1)
+	{
+		__shared__ double m[2][kNforBasis * kNforBasis];
+		__shared__ double rhs[2][kNforBasis * 3];
+		__shared__ int ok[2];
+		const std::uint32_t c = blockIdx.x;
+		if (c >= P.width * P.height) return;
+		const std::uint32_t x = c % P.width, y = c / P.width;
+		const int lane = static_cast<int>(threadIdx.x);
+		const int systems = second ? 2 : 1;
+		// this lane's entries: index e < systems * kNforSystem; e % kNforSystem < kNforUpper is
+		// an upper-triangle matrix element, the rest right-hand-side elements
+		double acc[4] = {0.0, 0.0, 0.0, 0.0};
+		int er[4] = {0, 0, 0, 0}, ec[4] = {0, 0, 0, 0}, es[4] = {0, 0, 0, 0}, ne = 0;
+		for (int e = lane; e < systems * kNforSystem; e += 64)
+		{
+			const int sys = e / kNforSystem, k = e % kNforSystem;
+			int row, col;
+			if (k < kNforUpper)
+			{
+				row = 0;
+				int rem = k;
+				while (rem >= kNforBasis - row)
+				{
+					rem -= kNforBasis - row;
+					row++;
+				}
+				col = row + rem;
+			}
+			else
+			{
+				row = (k - kNforUpper) / 3;
+				col = kNforBasis + (k - kNforUpper) % 3;	// right-hand side component
+			}
+			er[ne] = row;
+			ec[ne] = col;
+			es[ne] = sys;
+			acc[ne] = (col == row) ? 1e-4 : 0.0;	// initializeRegressionMatrix: ridge 1e-4
+			ne++;
+		}
+		const long long r = P.regressionRadius;
+		const long long pr = P.patchRadius;
+		const int patchTerms = static_cast<int>((2 * pr + 1) * (2 * pr + 1));
+		for (long long oy = -r; oy <= r; oy++)
+		{
+			for (long long ox = -r; ox <= r; ox++)
+			{
+				const std::uint32_t sx = nforClamp(static_cast<long long>(x) + ox, P.width);
+				const std::uint32_t sy = nforClamp(static_cast<long long>(y) + oy, P.height);
+				const std::uint32_t s = sy * P.width + sx;
+				// patch distance: lanes share its terms, then a wave sum
+				double term = 0.0;
+				if (!(sx == x && sy == y))
+				{
+					for (int t = lane; t < patchTerms; t += 64)
+					{
+						const long long px = t % (2 * pr + 1) - pr, py = t / (2 * pr + 1) - pr;
+						const std::uint32_t ia = nforClamp(static_cast<long long>(y) + py, P.height) * P.width + nforClamp(static_cast<long long>(x) + px, P.width);
+						const std::uint32_t ib = nforClamp(static_cast<long long>(sy) + py, P.height) * P.width + nforClamp(static_cast<long long>(sx) + px, P.width);
+						term += nforDist2(image + 3 * ia, image + 3 * ib) / (variance[ia] + variance[ib] + P.minVariance);
+					}
+				}
+				const double distance = (sx == x && sy == y) ? 0.0 : nforWaveSum(term) / static_cast<double>(patchTerms);
+				const double w0 = nforWeight(distance, b0, P.minVariance);
+				const double w1 = second ? nforWeight(distance, b1, P.minVariance) : 0.0;
+				// the basis (lanes 0..11) and the sample's colour (lanes 12..14)
+				double v = 0.0;
+				if (lane == 0) v = 1.0;
+				else if (lane < kNforBasis) v = features[kNforFeatures * s + (lane - 1)] - features[kNforFeatures * c + (lane - 1)];
+				else if (lane < kNforBasis + 3) v = image[3 * s + (lane - kNforBasis)];
+				// every lane runs all four shuffle pairs (a lane with fewer entries reads lane 0
+				// for the spare ones), so the shuffles stay wave-uniform
+				for (int k = 0; k < 4; k++)
+				{
+					const double vr = __shfl(v, er[k]);
+					const double vc = __shfl(v, ec[k]);
+					const double w = es[k] == 0 ? w0 : w1;
+					if (k < ne) acc[k] += (w * vr) * vc;
+				}
+			}
+		}
+		// completeRegressionMatrix + solve, one system at a time
+		for (int k = 0; k < ne; k++)
+		{
+			if (ec[k] < kNforBasis)
+			{
+				m[es[k]][er[k] * kNforBasis + ec[k]] = acc[k];
+				m[es[k]][ec[k] * kNforBasis + er[k]] = acc[k];
+			}
+			else rhs[es[k]][er[k] * 3 + (ec[k] - kNforBasis)] = acc[k];
+		}
+		__syncthreads();
+		for (int sys = 0; sys < systems; sys++)
+		{
+			double* out = (sys == 0 ? first : second) + 3 * static_cast<size_t>(c);
+			if (nforSolve(m[sys], rhs[sys], ok))
+			{
+				if (lane < 3) out[lane] = rhs[sys][lane];
+			}
+			else if (lane == 0)
+			{
+				nforFallback(image, variance, P, x, y, out);
+			}
+			__syncthreads();
+		}
+	}
+
+	// estimateMSE (NFOR.cpp:875-893) of the first and the second bandwidth
+	__global__ void	nfor_mse(const double* __restrict__ colorA, const double* __restrict__ colorB, const double* __restrict__ a0,
+		const double* __restrict__ b0, const double* __restrict__ a1, const double* __restrict__ b1,
+		const double* __restrict__ variance, double* __restrict__ mse0, double* __restrict__ mse1, std::uint32_t n)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= n) return;
+		auto mse = [&](const double* fa, const double* fb) {
+			const double mseA = nforDist2(colorB + 3 * i, fa + 3 * i) - 2.0 * variance[i];
+			const double mseB = nforDist2(colorA + 3 * i, fb + 3 * i) - 2.0 * variance[i];
+			const double residual = nforDist2(fa + 3 * i, fb + 3 * i) * 0.25;
+			return fmax(0.0, ((mseA + mseB) * 0.5) - residual);
+		};
+		mse0[i] = mse(a0, b0);
+		mse1[i] = mse(a1, b1);
+	}
+
+	// filterScalarMapPair (NFOR.cpp:834-873): radius-3 cross-bilateral filter of two maps,
+	// weights from 3 x 3 colour patches of `guide` (bandwidth 1). `select` != 0: the maps are
+	// first turned into bandwidthSelectionMap(first, second) and (second, first) (:907-919).
+	__global__ void	nfor_filter_pair(const double* __restrict__ first, const double* __restrict__ second,
+		const double* __restrict__ guide, const double* __restrict__ guideVariance, double* __restrict__ out0,
+		double* __restrict__ out1, NforParams P, int select)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= P.width * P.height) return;
+		const std::uint32_t x = i % P.width, y = i / P.width;
+		auto value = [&](std::uint32_t k, double& v0, double& v1) {
+			if (select)
+			{
+				v0 = first[k] <= second[k] ? 1.0 : 0.0;
+				v1 = second[k] <= first[k] ? 1.0 : 0.0;
+			}
+			else
+			{
+				v0 = first[k];
+				v1 = second[k];
+			}
+		};
+		double f0 = 0.0, f1 = 0.0, total = 0.0;
+		for (long long oy = -3; oy <= 3; oy++)
+		{
+			for (long long ox = -3; ox <= 3; ox++)
+			{
+				const std::uint32_t sx = nforClamp(static_cast<long long>(x) + ox, P.width);
+				const std::uint32_t sy = nforClamp(static_cast<long long>(y) + oy, P.height);
+				const std::uint32_t k = sy * P.width + sx;
+				const double w = nforWeight(nforPatchDistance(guide, guideVariance, P, x, y, sx, sy, 1u), 1.0, P.minVariance);
+				double v0, v1;
+				value(k, v0, v1);
+				f0 += v0 * w;
+				f1 += v1 * w;
+				total += w;
+			}
+		}
+		if (total > 2.220446049250313e-16)
+		{
+			out0[i] = f0 / total;
+			out1[i] = f1 / total;
+		}
+		else
+		{
+			value(i, out0[i], out1[i]);
+		}
+	}
+
+	// The per-pixel blend of applyNFOR (NFOR.cpp:1072-1087) and combinedResultVariance
+	// (:895-905): combined colour, blended features and the final regression's variance.
+	__global__ void	nfor_combine(const double* __restrict__ sel0, const double* __restrict__ sel1, const double* __restrict__ mse0,
+		const double* __restrict__ mse1, const double* __restrict__ a0, const double* __restrict__ a1, const double* __restrict__ b0,
+		const double* __restrict__ b1, const double* __restrict__ featA, const double* __restrict__ featB,
+		double* __restrict__ combined, double* __restrict__ features, double* __restrict__ variance, NforParams P)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= P.width * P.height) return;
+		const double total = sel0[i] + sel1[i];
+		const double w0 = total > P.minVariance ? sel0[i] / total : (mse0[i] <= mse1[i] ? 1.0 : 0.0);
+		for (int j = 0; j < 3; j++)
+		{
+			const double ra = a0[3 * i + j] * w0 + a1[3 * i + j] * (1.0 - w0);
+			const double rb = b0[3 * i + j] * w0 + b1[3 * i + j] * (1.0 - w0);
+			combined[3 * i + j] = (ra + rb) * 0.5;
+		}
+		for (int f = 0; f < kNforFeatures; f++)
+		{
+			features[kNforFeatures * i + f] = featA[kNforFeatures * i + f] * 0.5 + featB[kNforFeatures * i + f] * (1.0 - 0.5);
+		}
+		variance[i] = nforDist2(a1 + 3 * i, b1 + 3 * i) * 0.25;
+	}
+}

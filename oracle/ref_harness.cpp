@@ -12,6 +12,7 @@
 //   luzref lights  <scene.luz>
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -46,6 +47,7 @@
 #include "Defaults.hpp"
 #include "Utilities.hpp"
 #include "OBJReader.hpp"
+#include "Denoise/NFOR.hpp"
 #include "Hittables/BVHNode.hpp"
 #include "Hittables/Mesh.hpp"
 #include "Hittables/MeshInstance.hpp"
@@ -204,6 +206,70 @@ namespace
 				}
 			}
 		}
+		return (0);
+	}
+
+	// denoise SCENE SEED THREADS OUT SPP W H PREFILTER REGRESSION PATCH: the worker loop of
+	// Threads.cpp:375-548 with denoising on (half-buffers on sample streams 1 / 2, primary
+	// features), then Denoise::applyNFOR with the given radii. Writes u32 W, H, then f64
+	// arrays: image (W*H*3, the raw mean), colorA, colorB (W*H*3), colorVariance (W*H),
+	// featuresA, featuresB, featureVariance (W*H*11), denoised (W*H*3).
+	int	cmdDenoise(int argc, char** argv)
+	{
+		if (argc < 12)
+		{
+			return (2);
+		}
+		Scene scene;
+		loadScene(scene, argv[2]);
+		setRandomSeed(std::stoi(argv[3]));
+		const std::size_t threads = static_cast<std::size_t>(std::stoul(argv[4]));
+		scene.setSampleCount(std::stoi(argv[6]));
+		scene.getImage()->setWidth(std::stoul(argv[7]));
+		scene.getImage()->setHeight(std::stoul(argv[8]));
+		scene.getImage()->initialize();
+		scene.setDenoise(true);
+		scene.updateLights();
+		scene.updateAccelerationStructure();
+		const std::size_t w = scene.getImage()->getWidth(), h = scene.getImage()->getHeight();
+		const Renderer::internal::RenderCamera camera = Renderer::internal::_prepareRenderCamera(scene);
+		Sampler::setRenderSeed(static_cast<std::uint32_t>(randomSeedValue()));
+		scene.initializeDenoiseBuffers(w, h);
+		std::atomic<std::size_t> next(0);
+		std::vector<std::thread> pool;
+		for (std::size_t t = 0; t < std::max<std::size_t>(1, threads); t++)
+		{
+			pool.emplace_back([&]() {
+				for (std::size_t i = next++; i < w * h; i = next++)
+				{
+					Renderer::internal::_threadRender(scene, camera, i % w, i / w);
+				}
+			});
+		}
+		for (std::thread& th : pool) th.join();
+		Denoise::NFORSettings settings;
+		settings.featurePrefilterRadius = static_cast<unsigned int>(std::stoul(argv[9]));
+		settings.regressionRadius = static_cast<unsigned int>(std::stoul(argv[10]));
+		settings.patchRadius = static_cast<unsigned int>(std::stoul(argv[11]));
+		settings.threadCount = std::max<std::size_t>(1, threads);
+		const Denoise::NFORBuffers& b = *scene.getDenoiseBuffers();
+		std::unique_ptr<Image> denoised = Denoise::applyNFOR(b, settings);
+		std::ofstream out(argv[5], std::ios::binary);
+		put(out, static_cast<std::uint32_t>(w));
+		put(out, static_cast<std::uint32_t>(h));
+		auto putColor = [&](const Color& c) { put(out, c.getRed()); put(out, c.getGreen()); put(out, c.getBlue()); };
+		for (std::size_t i = 0; i < w * h; i++) putColor(scene.getImage()->pixels()[i]);
+		for (const Color& c : b.colorA) putColor(c);
+		for (const Color& c : b.colorB) putColor(c);
+		for (double v : b.colorVariance) put(out, v);
+		for (const auto* fs : {&b.featuresA, &b.featuresB, &b.featureVariance})
+		{
+			for (const Denoise::FeatureVector& f : *fs)
+			{
+				for (std::size_t k = 0; k < Denoise::NFOR_FEATURE_COUNT; k++) put(out, f[k]);
+			}
+		}
+		for (std::size_t i = 0; i < w * h; i++) putColor(denoised->pixels()[i]);
 		return (0);
 	}
 
@@ -577,7 +643,7 @@ int	main(int argc, char** argv)
 {
 	if (argc < 2)
 	{
-		std::cerr << "usage: luzref render|seeds|sampler|blas|tlas|primary|env|lights ..." << std::endl;
+		std::cerr << "usage: luzref render|seeds|denoise|sampler|blas|tlas|primary|env|lights ..." << std::endl;
 		return (2);
 	}
 	try
@@ -591,6 +657,7 @@ int	main(int argc, char** argv)
 		if (cmd == "env") return (cmdEnv(argc, argv));
 		if (cmd == "lights") return (cmdLights(argc, argv));
 		if (cmd == "seeds") return (cmdSeeds(argc, argv));
+		if (cmd == "denoise") return (cmdDenoise(argc, argv));
 	}
 	catch (const std::exception& e)
 	{
