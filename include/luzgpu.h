@@ -99,7 +99,34 @@ typedef struct luzgpu_stats
 	double		shade_ms;
 	double		shadow_ms;
 	double		tune_ms;             /* autotune: search time spent by this call (not in render_ms) */
+	double		denoise_ms;          /* luzgpu_render_denoised: NFOR (SceneRenderStats::denoiseMS) */
 } luzgpu_stats;
+
+/* Denoise::NFORSettings (NFOR.hpp:39-48); NULL = the reference's defaults 1, 2, 1, 1e-5. */
+typedef struct luzgpu_nfor_settings
+{
+	uint32_t	feature_prefilter_radius;
+	uint32_t	regression_radius;     /* 1..32 */
+	uint32_t	patch_radius;          /* 1..8 */
+	uint32_t	pad;
+	double		min_variance;
+} luzgpu_nfor_settings;
+
+/* Denoise::NFORBuffers (NFOR.hpp:23-37) as host FP64 arrays of width x height pixels:
+ * colours 3 per pixel, colour variance 1, features (NFOR_FEATURE_COUNT = 11) per pixel;
+ * `image` is the raw mean image of the same render (an output only). Any pointer may be
+ * NULL where the buffers are an output. */
+typedef struct luzgpu_nfor_buffers
+{
+	uint32_t	width, height;
+	double*		image;
+	double*		color_a;
+	double*		color_b;
+	double*		color_variance;
+	double*		features_a;
+	double*		features_b;
+	double*		feature_variance;
+} luzgpu_nfor_buffers;
 
 typedef struct luzgpu_scene_info
 {
@@ -154,6 +181,24 @@ int		luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scene
  * do not depend on the launch shape. */
 int		luzgpu_autotune(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes, const luzgpu_settings* settings,
 			luzgpu_tuning* out);
+
+/* Renders with denoising on, as Renderer::render does for a scene with denoise enabled
+ * (Threads.cpp:375-548): sample k draws from sampler stream (k % 2) + 1, the even and odd
+ * samples fill two half-buffers with their colours and primary-hit features
+ * (_threadRender, Threads.cpp:585-633), and Denoise::applyNFOR (NFOR.cpp:977-1102) runs on
+ * the device. rgb_out receives Scene::getImage() (the raw mean), denoised_out the denoised
+ * image (Scene::getDenoisedImage(), before post-processing); both W*H*3 fp32, host or
+ * device pointers as settings->output_on_device says. buffers_out (optional, host FP64)
+ * receives the NFOR inputs. The whole sample range is rendered in one call, on the plain
+ * kernel. */
+int		luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings,
+			const luzgpu_nfor_settings* nfor, float* rgb_out, float* denoised_out, luzgpu_nfor_buffers* buffers_out,
+			luzgpu_stats* stats);
+
+/* Denoise::applyNFOR on caller-supplied buffers (host FP64, `image` unused); denoised_out
+ * receives W*H*3 FP64 on the host, elapsed_ms (optional) the device time. */
+int		luzgpu_nfor(luzgpu_ctx* ctx, const luzgpu_nfor_buffers* in, const luzgpu_nfor_settings* nfor, double* denoised_out,
+			double* elapsed_ms);
 
 /* Path-replay adjoint (no counterpart in the reference): gradients of an image loss with
  * respect to per-material parameters, for the render luzgpu_render produces with the same

@@ -26,7 +26,7 @@ EXPORTED = [
     "luzgpu_create", "luzgpu_destroy", "luzgpu_last_error", "luzgpu_scene_load",
     "luzgpu_scene_release", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_primary_hits",
     "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas", "luzgpu_render_batch",
-    "luzgpu_adjoint", "luzgpu_scene_set_materials", "luzgpu_autotune",
+    "luzgpu_adjoint", "luzgpu_scene_set_materials", "luzgpu_autotune", "luzgpu_render_denoised", "luzgpu_nfor",
 ]
 
 
@@ -84,7 +84,24 @@ class Stats(ctypes.Structure):
         ("shade_ms", ctypes.c_double),
         ("shadow_ms", ctypes.c_double),
         ("tune_ms", ctypes.c_double),
+        ("denoise_ms", ctypes.c_double),
     ]
+
+
+class NforSettings(ctypes.Structure):
+    _fields_ = [
+        ("feature_prefilter_radius", ctypes.c_uint32),
+        ("regression_radius", ctypes.c_uint32),
+        ("patch_radius", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("min_variance", ctypes.c_double),
+    ]
+
+
+class NforBuffers(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32)] + [
+        (n, ctypes.c_void_p) for n in ("image", "color_a", "color_b", "color_variance", "features_a", "features_b",
+                                       "feature_variance")]
 
 
 class SceneInfo(ctypes.Structure):
@@ -133,6 +150,9 @@ def load() -> ctypes.CDLL:
     lib.luzgpu_adjoint.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, vp, vp, vp]
     lib.luzgpu_scene_set_materials.argtypes = [vp, ctypes.c_uint32, vp, vp, vp]
     lib.luzgpu_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(vp), ctypes.POINTER(Settings), ctypes.POINTER(Tuning)]
+    lib.luzgpu_render_denoised.argtypes = [vp, vp, ctypes.POINTER(Settings), ctypes.POINTER(NforSettings), vp, vp,
+                                           ctypes.POINTER(NforBuffers), ctypes.POINTER(Stats)]
+    lib.luzgpu_nfor.argtypes = [vp, ctypes.POINTER(NforBuffers), ctypes.POINTER(NforSettings), vp, ctypes.POINTER(ctypes.c_double)]
     lib.luzgpu_render_batch.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(vp), ctypes.POINTER(Settings),
                                         ctypes.POINTER(vp), ctypes.POINTER(Stats)]
     lib.luzgpu_sampler_eval.argtypes = [vp, ctypes.c_size_t, vp, vp, vp]

@@ -245,11 +245,12 @@ namespace luzgpu
 		std::uint32_t	lightWeighted;	// lightCdf holds weights (else uniform selection)
 		std::uint32_t	envBase;		// envPixels / envPdf / envCdf texel
 		std::uint32_t	unboundedBase;
+		std::uint32_t	denoiseStreams;	// 1: sample k draws from stream (k % 2) + 1 (Threads.cpp:594-598)
 	};
 
 	// Scene content classes the kernels are specialised on (a scene without them runs
 	// kernels that do not carry their code or registers).
-	constexpr int	kFeatAnalytic = 1;	// spheres, cubes, volumes, planes, directional lights
+	constexpr int	kFeatAnalytic = 1;	// spheres, cubes, volumes, planes, directional and mesh-instance lights
 	constexpr int	kFeatRich = 2;		// glossy, diffuse-glossy, principled, isotropic, Henyey-Greenstein
 	constexpr int	kFeatAll = 3;
 

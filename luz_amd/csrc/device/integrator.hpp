@@ -274,15 +274,22 @@ namespace luzgpu
 		f[9] = normalizedColorChannel(albedo.z);
 	}
 
+	// The sampler stream of a sample: 0, or with denoising (k % 2) + 1, the two half-buffers
+	// (Threads.cpp:594-598).
+	__device__ __forceinline__ std::uint32_t	sampleStream(const GpuParams& p, std::uint32_t sample)
+	{
+		return p.denoiseStreams ? (sample & 1u) + 1u : 0u;
+	}
+
 	// One path. With `features` (denoising), the primary hit's or miss's features are written
-	// there and the sampler runs on `stream` (1 / 2 for the two half-buffers, Threads.cpp:594-598).
+	// there.
 	template <bool Count, int Feat, typename Stack, typename Adj = NoAdjoint>
 	__device__ f3	tracePath(const SceneView& s, const GpuParams& p, std::uint32_t x, std::uint32_t y, std::uint32_t sample,
 		Stack& stack, std::uint32_t& closestRays, std::uint32_t& shadowRays, VisitCounter& visits, Adj* adj = nullptr,
-		std::uint32_t stream = 0u, float* features = nullptr)
+		float* features = nullptr)
 	{
 		PixelSampler ps;
-		ps.begin(p.renderSeed, x, y, sample, stream);
+		ps.begin(p.renderSeed, x, y, sample, sampleStream(p, sample));
 		Ray ray = cameraRay(p, x, y, ps);
 		f3 acc = mk(0, 0, 0), thr = mk(1, 1, 1), prevOrigin = mk(0, 0, 0);
 		RayOrigin from = kNoOrigin;

@@ -140,7 +140,7 @@ namespace luzgpu
 	__device__ __forceinline__ float	pathVolumeLogU(const GpuParams& p, std::uint32_t x, std::uint32_t y, std::uint32_t sample, int b)
 	{
 		PixelSampler ps;
-		ps.begin(p.renderSeed, x, y, sample, 0);
+		ps.begin(p.renderSeed, x, y, sample, sampleStream(p, sample));
 		ps.bounce = static_cast<std::uint32_t>(b);
 		return volumeLogU(p, ps);
 	}
@@ -156,7 +156,7 @@ namespace luzgpu
 			const std::uint32_t pixel = id % pixels, slot = id / pixels;
 			const std::uint32_t x = pixel % p.width, y = pixel / p.width;
 			PixelSampler ps;
-			ps.begin(p.renderSeed, x, y, p.sampleBegin + slot, 0);
+			ps.begin(p.renderSeed, x, y, p.sampleBegin + slot, sampleStream(p, p.sampleBegin + slot));
 			const Ray r = cameraRay(p, x, y, ps);
 			const std::uint32_t g = base + id;
 			const float logU = p.hasVolumes ? volumeLogU(p, ps) : 0.0f;
@@ -446,7 +446,7 @@ namespace luzgpu
 				const std::uint32_t pixel = local % J.pixels, slot = local / J.pixels;
 				const std::uint32_t px = pixel % p.width, py = pixel / p.width;
 				PixelSampler ps;
-				ps.begin(p.renderSeed, px, py, p.sampleBegin + slot, 0);
+				ps.begin(p.renderSeed, px, py, p.sampleBegin + slot, sampleStream(p, p.sampleBegin + slot));
 				ps.bounce = static_cast<std::uint32_t>(b);
 				const float4 hv = w.qHit[i];
 				const float4 o = w.qOrigin[q][i], d = w.qDir[q][i];

@@ -902,10 +902,6 @@ namespace luzgpu
 		{
 			throw std::runtime_error("Unsupported for the device path: adaptive sampling (set adaptive=0)");
 		}
-		if (s.denoise)
-		{
-			throw std::runtime_error("Unsupported for the device path: denoise (set denoise=0)");
-		}
 
 		// ---- materials -------------------------------------------------------------
 		for (const Material& m : scene.materials)
@@ -1431,6 +1427,9 @@ namespace luzgpu
 		gp.metersPerUnit = static_cast<float>(s.metersPerUnit);
 		gp.unboundedCount = static_cast<std::uint32_t>(p.unbounded.size());
 		gp.hasVolumes = p.volumes.empty() ? 0u : 1u;
+		// a scene with denoising on samples on streams 1 / 2 (Threads.cpp:594-598), also when
+		// only its raw image is rendered
+		gp.denoiseStreams = s.denoise ? 1u : 0u;
 		p.features = 0;
 		for (const Hittable& h : scene.hittables)
 		{

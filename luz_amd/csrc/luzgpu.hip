@@ -13,6 +13,7 @@
 #include "common/layout.h"
 #include "device/adjoint.hpp"
 #include "device/integrator.hpp"
+#include "device/nfor.hpp"
 #include "device/wavefront.hpp"
 #include "host/accel.hpp"
 
@@ -159,6 +160,132 @@ namespace luzgpu
 		else launchMegaC<false>(features, grid, st, s, p, colors, counters);
 	}
 
+	// The plain kernel with denoising on: each (pixel, sample) also writes its primary
+	// features (kNforFeatures floats, RenderSample of _calculatePixelSample, ColorHelper.cpp:1119-1126).
+	template <int Feat>
+	__global__ __launch_bounds__(kMegaBlock) void	render_mega_denoise(SceneView s, GpuParams p, float* __restrict__ sampleColors,
+		float* __restrict__ sampleFeatures, unsigned long long* __restrict__ counters)
+	{
+		__shared__ std::uint32_t stackMem[kStackDepth * kMegaBlock];
+		const std::uint32_t pixels = p.width * p.height;
+		const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * kMegaBlock + threadIdx.x;
+		std::uint32_t closest = 0, shadow = 0;
+		VisitCounter visits{0u, 0u};
+		if (gid < static_cast<std::uint64_t>(pixels) * p.sampleCount)
+		{
+			const std::uint32_t slot = static_cast<std::uint32_t>(gid / pixels);
+			const std::uint32_t pixel = static_cast<std::uint32_t>(gid % pixels);
+			LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
+			float f[kNforFeatures];
+			const f3 c = cleanColor(tracePath<false, Feat>(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest,
+				shadow, visits, static_cast<NoAdjoint*>(nullptr), f));
+			const size_t k = static_cast<size_t>(slot) * pixels + pixel;
+			sampleColors[3 * k] = c.x;
+			sampleColors[3 * k + 1] = c.y;
+			sampleColors[3 * k + 2] = c.z;
+			for (int j = 0; j < kNforFeatures; j++) sampleFeatures[kNforFeatures * k + j] = f[j];
+		}
+		for (int off = 32; off > 0; off >>= 1)
+		{
+			closest += __shfl_down(closest, off);
+			shadow += __shfl_down(shadow, off);
+		}
+		if ((threadIdx.x & 63) == 0)
+		{
+			if (closest) atomicAdd(counters + 0, static_cast<unsigned long long>(closest));
+			if (shadow) atomicAdd(counters + 1, static_cast<unsigned long long>(shadow));
+		}
+	}
+
+	void	launchMegaDenoise(int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, float* colors,
+		float* feats, unsigned long long* counters)
+	{
+		switch (features & kFeatAll)
+		{
+			case 0: render_mega_denoise<0><<<grid, kMegaBlock, 0, st>>>(s, p, colors, feats, counters); break;
+			case kFeatAnalytic: render_mega_denoise<kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, p, colors, feats, counters); break;
+			case kFeatRich: render_mega_denoise<kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, p, colors, feats, counters); break;
+			default: render_mega_denoise<kFeatAll><<<grid, kMegaBlock, 0, st>>>(s, p, colors, feats, counters); break;
+		}
+	}
+
+	// Per-pixel FP64 statistics of a denoised render, in sample order (_threadRender with
+	// denoise buffers, Threads.cpp:585-627): [0,3) half A colour sum, [3,6) half B, [6,9) all
+	// colours, [9,12) colour squares, then kNforFeatures each of half A, half B, all features
+	// and feature squares.
+	constexpr int	kDenoiseStats = 12 + 4 * kNforFeatures;
+
+	__global__ void	denoise_accumulate(const float* __restrict__ colors, const float* __restrict__ feats, double* __restrict__ acc,
+		std::uint32_t pixels, std::uint32_t count, std::uint32_t firstSample)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= pixels) return;
+		double a[kDenoiseStats];
+		for (int j = 0; j < kDenoiseStats; j++) a[j] = acc[static_cast<size_t>(kDenoiseStats) * i + j];
+		for (std::uint32_t k = 0; k < count; k++)
+		{
+			const size_t idx = static_cast<size_t>(k) * pixels + i;
+			const int half = ((firstSample + k) & 1u) ? 3 : 0;
+			for (int j = 0; j < 3; j++)
+			{
+				const double c = colors[3 * idx + j];
+				a[half + j] += c;
+				a[6 + j] += c;
+				a[9 + j] += c * c;
+			}
+			const int fh = ((firstSample + k) & 1u) ? 12 + kNforFeatures : 12;
+			for (int j = 0; j < kNforFeatures; j++)
+			{
+				const double f = feats[kNforFeatures * idx + j];
+				a[fh + j] += f;
+				a[12 + 2 * kNforFeatures + j] += f;
+				a[12 + 3 * kNforFeatures + j] += f * f;
+			}
+		}
+		for (int j = 0; j < kDenoiseStats; j++) acc[static_cast<size_t>(kDenoiseStats) * i + j] = a[j];
+	}
+
+	// sampleMeanVariance (Threads.cpp:150-161)
+	__device__ __forceinline__ double	sampleMeanVariance(double sum, double square, std::uint32_t n)
+	{
+		if (n <= 1u) return 0.0;
+		const double m = static_cast<double>(n);
+		return fmax(0.0, ((square - (sum * sum / m)) / (m - 1.0)) / m);
+	}
+
+	// storeDenoisePixel (Threads.cpp:340-372) and the pixel's raw mean (:628-631): the NFOR
+	// input buffers (FP64) and the image.
+	__global__ void	denoise_finalize(const double* __restrict__ acc, std::uint32_t pixels, std::uint32_t n, float* __restrict__ image,
+		double* __restrict__ meanImage, double* __restrict__ colorA, double* __restrict__ colorB, double* __restrict__ colorVariance,
+		double* __restrict__ featA, double* __restrict__ featB, double* __restrict__ featVariance)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= pixels) return;
+		const double* a = acc + static_cast<size_t>(kDenoiseStats) * i;
+		const std::uint32_t nA = (n + 1u) / 2u, nB = n / 2u;	// samples 0, 2, 4... and 1, 3, 5...
+		const double dn = static_cast<double>(n);
+		double cv = 0.0;
+		for (int j = 0; j < 3; j++)
+		{
+			const double fallback = n == 0u ? 0.0 : a[6 + j] / dn;
+			colorA[3 * i + j] = nA == 0u ? fallback : a[j] / static_cast<double>(nA);
+			colorB[3 * i + j] = nB == 0u ? fallback : a[3 + j] / static_cast<double>(nB);
+			cv += sampleMeanVariance(a[6 + j], a[9 + j], n);
+			double m = a[6 + j] / dn;
+			if (!isfinite(m)) m = 0.0;
+			meanImage[3 * i + j] = m;
+			image[3 * i + j] = static_cast<float>(m);
+		}
+		colorVariance[i] = cv / 3.0;
+		for (int j = 0; j < kNforFeatures; j++)
+		{
+			const double fallback = n == 0u ? 0.0 : a[12 + 2 * kNforFeatures + j] / dn;
+			featA[kNforFeatures * i + j] = nA == 0u ? fallback : a[12 + j] / static_cast<double>(nA);
+			featB[kNforFeatures * i + j] = nB == 0u ? fallback : a[12 + kNforFeatures + j] / static_cast<double>(nB);
+			featVariance[kNforFeatures * i + j] = sampleMeanVariance(a[12 + 2 * kNforFeatures + j], a[12 + 3 * kNforFeatures + j], n);
+		}
+	}
+
 	// Path-replay adjoint, one thread per (pixel, sample): dLdI is dLoss/dImage (W*H*3),
 	// grads receives kGradPerMaterial FP64 sums per material.
 	template <int Feat>
@@ -236,6 +363,12 @@ namespace luzgpu
 			const double v = sums[3 * i + k] / count;
 			out[3 * i + k] = isfinite(v) ? static_cast<float>(v) : 0.0f;
 		}
+	}
+
+	__global__ void	to_float(const double* __restrict__ in, float* __restrict__ out, size_t n)
+	{
+		const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+		if (i < n) out[i] = static_cast<float>(in[i]);
 	}
 
 	__global__ void	zero_doubles(double* p, size_t n)
@@ -328,6 +461,8 @@ struct luzgpu_ctx
 	// autotuned launch shapes, keyed by scene serials and per-scene (width, height, spp, depth)
 	std::vector<std::pair<std::vector<std::uint64_t>, luzgpu_tuning>>	tuned;
 	luzgpu::DeviceBuffer	tuneSums;
+	// denoising: per-sample features, per-pixel statistics, NFOR inputs and temporaries
+	luzgpu::DeviceBuffer	sampleFeatures, denoiseStats, nforBuffers, nforTemp;
 };
 
 struct luzgpu_scene
@@ -580,6 +715,92 @@ namespace luzgpu
 			for (const BatchJob& j : cal) r.calibration_paths += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
 			r.tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 			return r;
+		}
+
+		// NFOR input buffers of W x H pixels in one FP64 allocation (field offsets in doubles).
+		struct	NforLayout
+		{
+			size_t	mean, colorA, colorB, colorVariance, featA, featB, featVariance, total;
+			explicit NforLayout(size_t n)
+			{
+				mean = 0;
+				colorA = mean + 3 * n;
+				colorB = colorA + 3 * n;
+				colorVariance = colorB + 3 * n;
+				featA = colorVariance + n;
+				featB = featA + kNforFeatures * n;
+				featVariance = featB + kNforFeatures * n;
+				total = featVariance + kNforFeatures * n;
+			}
+		};
+
+		NforParams	nforParams(std::uint32_t w, std::uint32_t h, const luzgpu_nfor_settings* ns)
+		{
+			NforParams P{w, h, 1u, 2u, 1u, 1e-5};	// NFORSettings defaults (NFOR.hpp:39-48)
+			if (ns)
+			{
+				P.prefilterRadius = ns->feature_prefilter_radius;
+				P.regressionRadius = ns->regression_radius;
+				P.patchRadius = ns->patch_radius;
+				P.minVariance = ns->min_variance;
+			}
+			if (P.regressionRadius == 0 || P.patchRadius == 0)
+			{
+				throw Error{LUZGPU_E_INVALID, "NFOR radii must be positive."};	// NFOR.cpp:985-988
+			}
+			if (P.regressionRadius > 32 || P.patchRadius > 8 || P.prefilterRadius > 32 || !(P.minVariance >= 0.0))
+			{
+				throw Error{LUZGPU_E_INVALID, "NFOR settings out of range: regression radius <= 32, patch radius <= 8, prefilter radius <= 32"};
+			}
+			return P;
+		}
+
+		// Denoise::applyNFOR (NFOR.cpp:977-1102) on device buffers, into `out` (W x H x 3, FP64).
+		void	runNfor(luzgpu_ctx* ctx, const NforParams& P, const double* colorA, const double* colorB, const double* colorVariance,
+			const double* featA, const double* featB, const double* featVariance, double* out)
+		{
+			hipStream_t st = ctx->stream;
+			const size_t n = static_cast<size_t>(P.width) * P.height;
+			// temporaries, in doubles per pixel: 3 x 3 suppressed colours, 2 variances, 2 x 11
+			// filtered features, 4 x 3 regressions, 6 scalar maps, 3 + 11 + 1 for the blend
+			const size_t per = 9 + 2 + 2 * kNforFeatures + 12 + 6 + 3 + kNforFeatures + 1;
+			ctx->nforTemp.reserve(per * n * sizeof(double));
+			double* t = ctx->nforTemp.as<double>();
+			double* supA = t; t += 3 * n;
+			double* supB = t; t += 3 * n;
+			double* color = t; t += 3 * n;
+			double* regVar = t; t += n;
+			double* featVarAvg = t; t += n;
+			double* ffA = t; t += kNforFeatures * n;
+			double* ffB = t; t += kNforFeatures * n;
+			double* a0 = t; t += 3 * n;
+			double* a1 = t; t += 3 * n;
+			double* b0 = t; t += 3 * n;
+			double* b1 = t; t += 3 * n;
+			double* m0 = t; t += n;
+			double* m1 = t; t += n;
+			double* f0 = t; t += n;
+			double* f1 = t; t += n;
+			double* s0 = t; t += n;
+			double* s1 = t; t += n;
+			double* combined = t; t += 3 * n;
+			double* finalFeat = t; t += kNforFeatures * n;
+			double* finalVar = t; t += n;
+			const unsigned g = blocks(n, 256);
+			nfor_fireflies<<<g, 256, 0, st>>>(colorA, colorA, supA, P, 0);
+			nfor_fireflies<<<g, 256, 0, st>>>(colorB, colorB, supB, P, 0);
+			nfor_fireflies<<<g, 256, 0, st>>>(supA, supB, color, P, 1);
+			nfor_variances<<<g, 256, 0, st>>>(colorVariance, featVariance, regVar, featVarAvg, static_cast<std::uint32_t>(n));
+			nfor_prefilter<<<g, 256, 0, st>>>(featA, featB, featVarAvg, ffA, P);
+			nfor_prefilter<<<g, 256, 0, st>>>(featB, featA, featVarAvg, ffB, P);
+			nfor_regression<<<static_cast<unsigned>(n), 64, 0, st>>>(supA, ffB, regVar, a0, a1, P, 0.5, 1.0);
+			nfor_regression<<<static_cast<unsigned>(n), 64, 0, st>>>(supB, ffA, regVar, b0, b1, P, 0.5, 1.0);
+			nfor_mse<<<g, 256, 0, st>>>(supA, supB, a0, b0, a1, b1, regVar, m0, m1, static_cast<std::uint32_t>(n));
+			nfor_filter_pair<<<g, 256, 0, st>>>(m0, m1, color, colorVariance, f0, f1, P, 0);
+			nfor_filter_pair<<<g, 256, 0, st>>>(f0, f1, color, colorVariance, s0, s1, P, 1);
+			nfor_combine<<<g, 256, 0, st>>>(s0, s1, f0, f1, a0, a1, b0, b1, ffA, ffB, combined, finalFeat, finalVar, P);
+			nfor_regression<<<static_cast<unsigned>(n), 64, 0, st>>>(combined, finalFeat, finalVar, out, nullptr, P, 1.0, 1.0);
+			check(hipGetLastError(), "NFOR launch");
 		}
 
 		// The context's tuning for these jobs, tuned now if there is none yet.
@@ -897,6 +1118,149 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			stats->extend_launches = stages.extendLaunches;
 			stats->tune_ms = tuneMs;
 		}
+	});
+}
+
+int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings, const luzgpu_nfor_settings* nfor,
+	float* rgb_out, float* denoised_out, luzgpu_nfor_buffers* buffers_out, luzgpu_stats* stats)
+{
+	if (!ctx || !scene || !rgb_out || !denoised_out || scene->ctx != ctx)
+	{
+		g_lastError = "luzgpu_render_denoised: NULL argument or scene from another context";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		GpuParams p = resolveParams(scene->gpu, settings);
+		p.denoiseStreams = 1u;
+		if (p.sampleBegin != 0u || p.sampleCount != p.totalSamples || p.sampleCount == 0u)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render_denoised: the whole sample range [0, samples) is rendered at once"};
+		}
+		if (p.maxBounces < 0)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render_denoised: negative max_bounces"};
+		}
+		const NforParams P = nforParams(p.width, p.height, nfor);
+		hipStream_t st = ctx->stream;
+		const std::uint32_t pixels = p.width * p.height;
+		const NforLayout L(pixels);
+		ctx->denoiseStats.reserve(static_cast<size_t>(pixels) * kDenoiseStats * sizeof(double));
+		ctx->nforBuffers.reserve((L.total + 3 * static_cast<size_t>(pixels)) * sizeof(double));
+		ctx->counters.reserve(8 * sizeof(unsigned long long));
+		zero_doubles<<<blocks(static_cast<size_t>(pixels) * kDenoiseStats, 256), 256, 0, st>>>(ctx->denoiseStats.as<double>(),
+			static_cast<size_t>(pixels) * kDenoiseStats);
+		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
+		hipEvent_t e0, e1, e2;
+		check(hipEventCreate(&e0), "hipEventCreate");
+		check(hipEventCreate(&e1), "hipEventCreate");
+		check(hipEventCreate(&e2), "hipEventCreate");
+		check(hipEventRecord(e0, st), "hipEventRecord");
+		// samples per launch: per-sample colour + feature buffers within 4 GiB
+		const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount,
+			static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(pixels) * (12ull + 4ull * kNforFeatures)))));
+		ctx->sampleColors.reserve(static_cast<size_t>(pixels) * perLaunch * 3 * sizeof(float));
+		ctx->sampleFeatures.reserve(static_cast<size_t>(pixels) * perLaunch * kNforFeatures * sizeof(float));
+		for (std::uint32_t done = 0; done < p.sampleCount; done += perLaunch)
+		{
+			GpuParams q = p;
+			q.sampleBegin = done;
+			q.sampleCount = std::min(perLaunch, p.sampleCount - done);
+			const size_t threads = static_cast<size_t>(pixels) * q.sampleCount;
+			launchMegaDenoise(scene->gpu.host.features, blocks(threads, kMegaBlock), st, scene->gpu.view, q, ctx->sampleColors.as<float>(),
+				ctx->sampleFeatures.as<float>(), ctx->counters.as<unsigned long long>());
+			check(hipGetLastError(), "denoise render launch");
+			denoise_accumulate<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sampleFeatures.as<float>(),
+				ctx->denoiseStats.as<double>(), pixels, q.sampleCount, done);
+		}
+		double* nb = ctx->nforBuffers.as<double>();
+		ctx->output.reserve(static_cast<size_t>(pixels) * 3 * sizeof(float) * 2);
+		float* image = ctx->output.as<float>();
+		float* denoised = image + static_cast<size_t>(pixels) * 3;
+		denoise_finalize<<<blocks(pixels, 256), 256, 0, st>>>(ctx->denoiseStats.as<double>(), pixels, p.sampleCount, image, nb + L.mean,
+			nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB, nb + L.featVariance);
+		check(hipEventRecord(e1, st), "hipEventRecord");
+		double* out = nb + L.total;
+		runNfor(ctx, P, nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB, nb + L.featVariance, out);
+		to_float<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(out, denoised, static_cast<size_t>(pixels) * 3);
+		check(hipEventRecord(e2, st), "hipEventRecord");
+		const size_t imageBytes = static_cast<size_t>(pixels) * 3 * sizeof(float);
+		const hipMemcpyKind kind = (settings && settings->output_on_device) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+		check(hipMemcpyAsync(rgb_out, image, imageBytes, kind, st), "image copy");
+		check(hipMemcpyAsync(denoised_out, denoised, imageBytes, kind, st), "denoised copy");
+		if (buffers_out)
+		{
+			buffers_out->width = p.width;
+			buffers_out->height = p.height;
+			const std::pair<double*, size_t> fields[] = {{buffers_out->image, L.mean}, {buffers_out->color_a, L.colorA},
+				{buffers_out->color_b, L.colorB}, {buffers_out->color_variance, L.colorVariance}, {buffers_out->features_a, L.featA},
+				{buffers_out->features_b, L.featB}, {buffers_out->feature_variance, L.featVariance}};
+			const size_t sizes[] = {3, 3, 3, 1, kNforFeatures, kNforFeatures, kNforFeatures};
+			for (int k = 0; k < 7; k++)
+			{
+				if (!fields[k].first) continue;
+				check(hipMemcpyAsync(fields[k].first, nb + fields[k].second, sizes[k] * pixels * sizeof(double), hipMemcpyDeviceToHost, st),
+					"buffer readback");
+			}
+		}
+		unsigned long long counters[2] = {0, 0};
+		check(hipMemcpyAsync(counters, ctx->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, st), "counter readback");
+		check(hipStreamSynchronize(st), "denoised render");
+		float renderMs = 0.0f, denoiseMs = 0.0f;
+		check(hipEventElapsedTime(&renderMs, e0, e1), "hipEventElapsedTime");
+		check(hipEventElapsedTime(&denoiseMs, e1, e2), "hipEventElapsedTime");
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		(void)hipEventDestroy(e2);
+		if (stats)
+		{
+			*stats = luzgpu_stats{};
+			stats->rendered_samples = static_cast<std::uint64_t>(pixels) * p.sampleCount;
+			stats->closest_hit_rays = counters[0];
+			stats->shadow_rays = counters[1];
+			stats->render_ms = renderMs;
+			stats->kernel_ms = renderMs;
+			stats->denoise_ms = denoiseMs;
+		}
+	});
+}
+
+int	luzgpu_nfor(luzgpu_ctx* ctx, const luzgpu_nfor_buffers* in, const luzgpu_nfor_settings* nfor, double* denoised_out, double* elapsed_ms)
+{
+	if (!ctx || !in || !denoised_out || !in->color_a || !in->color_b || !in->color_variance || !in->features_a || !in->features_b
+		|| !in->feature_variance || in->width == 0 || in->height == 0)
+	{
+		g_lastError = "luzgpu_nfor: NULL argument or empty image";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		const NforParams P = nforParams(in->width, in->height, nfor);
+		hipStream_t st = ctx->stream;
+		const size_t n = static_cast<size_t>(in->width) * in->height;
+		const NforLayout L(n);
+		ctx->nforBuffers.reserve((L.total + 3 * n) * sizeof(double));
+		double* nb = ctx->nforBuffers.as<double>();
+		const std::pair<const double*, size_t> fields[] = {{in->color_a, L.colorA}, {in->color_b, L.colorB},
+			{in->color_variance, L.colorVariance}, {in->features_a, L.featA}, {in->features_b, L.featB}, {in->feature_variance, L.featVariance}};
+		const size_t sizes[] = {3, 3, 1, kNforFeatures, kNforFeatures, kNforFeatures};
+		for (int k = 0; k < 6; k++)
+		{
+			check(hipMemcpyAsync(nb + fields[k].second, fields[k].first, sizes[k] * n * sizeof(double), hipMemcpyHostToDevice, st), "buffer upload");
+		}
+		hipEvent_t e0, e1;
+		check(hipEventCreate(&e0), "hipEventCreate");
+		check(hipEventCreate(&e1), "hipEventCreate");
+		check(hipEventRecord(e0, st), "hipEventRecord");
+		runNfor(ctx, P, nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB, nb + L.featVariance, nb + L.total);
+		check(hipEventRecord(e1, st), "hipEventRecord");
+		check(hipMemcpyAsync(denoised_out, nb + L.total, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st), "denoised readback");
+		check(hipStreamSynchronize(st), "NFOR");
+		float ms = 0.0f;
+		check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		if (elapsed_ms) *elapsed_ms = ms;
 	});
 }
 

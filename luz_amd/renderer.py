@@ -73,12 +73,14 @@ class RenderStats:
     shade_ms: float = 0.0
     shadow_ms: float = 0.0
     tune_ms: float = 0.0
+    denoise_ms: float = 0.0
 
     @staticmethod
     def of(st) -> "RenderStats":
         return RenderStats(st.rendered_samples, st.closest_hit_rays, st.shadow_rays, st.render_ms, st.kernel_ms,
                            st.kernel_launches, st.nodes_visited, st.prims_tested, st.extend_launches,
-                           st.closest_nodes, st.closest_prims, st.extend_ms, st.shade_ms, st.shadow_ms, st.tune_ms)
+                           st.closest_nodes, st.closest_prims, st.extend_ms, st.shade_ms, st.shadow_ms, st.tune_ms,
+                           st.denoise_ms)
 
 
 @dataclass
@@ -296,6 +298,64 @@ class Renderer:
         _lib.check(ctx._lib.luzgpu_autotune(ctx.handle, n, handles, arr, ctypes.byref(t)))
         return Tuning(t.traversal_grid, t.extend_refill, t.shadow_refill, t.fetch_chunk, t.lds_stack, t.trials,
                       t.calibration_paths, t.default_ms, t.tuned_ms, t.tune_ms)
+
+    NFOR_FIELDS = (("color_a", 3), ("color_b", 3), ("color_variance", 1), ("features_a", 11), ("features_b", 11),
+                   ("feature_variance", 11))
+
+    @staticmethod
+    def _nfor_settings(nfor):
+        if nfor is None:
+            return None
+        ns = _lib.NforSettings()
+        ns.feature_prefilter_radius = int(nfor.get("feature_prefilter_radius", 1))
+        ns.regression_radius = int(nfor.get("regression_radius", 2))
+        ns.patch_radius = int(nfor.get("patch_radius", 1))
+        ns.min_variance = float(nfor.get("min_variance", 1e-5))
+        return ns
+
+    @staticmethod
+    def render_denoised(scene: Scene, seed: int, samples=None, max_bounces=None, resolution=None, nfor=None,
+                        buffers=False):
+        """Render with denoising on (luzgpu_render_denoised): returns (raw image, denoised image,
+        RenderStats, NFOR buffers or None). ``nfor``: {"feature_prefilter_radius",
+        "regression_radius", "patch_radius", "min_variance"} (defaults as NFORSettings)."""
+        s = _settings(seed, samples, max_bounces, 0, None, "megakernel", resolution=resolution)
+        w, h = resolution if resolution is not None else (scene.width, scene.height)
+        img = np.zeros((h, w, 3), dtype=np.float32)
+        den = np.zeros((h, w, 3), dtype=np.float32)
+        st = _lib.Stats()
+        out = None
+        bp = None
+        if buffers:
+            out = {"image": np.zeros((h, w, 3))}
+            out.update({k: np.zeros((h, w, c)) for k, c in Renderer.NFOR_FIELDS})
+            bp = _lib.NforBuffers()
+            for k, a in out.items():
+                setattr(bp, k, a.ctypes.data)
+        ns = Renderer._nfor_settings(nfor)
+        _lib.check(scene.ctx._lib.luzgpu_render_denoised(scene.ctx.handle, scene._h, ctypes.byref(s),
+                                                         ctypes.byref(ns) if ns is not None else None, img.ctypes.data,
+                                                         den.ctypes.data, ctypes.byref(bp) if bp is not None else None,
+                                                         ctypes.byref(st)))
+        return img, den, RenderStats.of(st), out
+
+    @staticmethod
+    def nfor(ctx, buffers: dict, nfor=None):
+        """Denoise::applyNFOR on the device for host FP64 buffers {color_a, color_b,
+        color_variance, features_a, features_b, feature_variance} (H, W, c); returns
+        (denoised (H, W, 3) FP64, device ms)."""
+        h, w = buffers["color_a"].shape[:2]
+        arrs = {k: np.ascontiguousarray(buffers[k], dtype=np.float64).reshape(h, w, c) for k, c in Renderer.NFOR_FIELDS}
+        bp = _lib.NforBuffers()
+        bp.width, bp.height = w, h
+        for k, a in arrs.items():
+            setattr(bp, k, a.ctypes.data)
+        out = np.zeros((h, w, 3))
+        ms = ctypes.c_double()
+        ns = Renderer._nfor_settings(nfor)
+        _lib.check(ctx._lib.luzgpu_nfor(ctx.handle, ctypes.byref(bp), ctypes.byref(ns) if ns is not None else None,
+                                        out.ctypes.data, ctypes.byref(ms)))
+        return out, ms.value
 
     @staticmethod
     def primary_hits(scene: Scene, seed: int, resolution=None):

@@ -125,6 +125,37 @@ def zstat_fixtures(tmp):
         print("zstats", name, means.shape, means.mean((0, 1)))
 
 
+NFOR_CASES = [
+    # (name, workload writer, spp, width, height, (prefilter, regression, patch) radii)
+    ("nfor_cornell_48x48_8spp", "cornell", 8, 48, 48, (1, 2, 1)),
+    ("nfor_primitives_64x48_8spp_r9", "primitives", 8, 64, 48, (1, 9, 1)),
+]
+
+
+def nfor_fixtures(tmp):
+    """The reference's denoise pipeline: its NFOR input buffers (half-buffer means and
+    variances, primary features) and Denoise::applyNFOR's output for them, plus the raw
+    mean image, at seed 1 (oracle/ref_harness.cpp, denoise command)."""
+    writers = {"cornell": workloads.write_cornell, "primitives": workloads.write_primitives}
+    for name, scene, spp, w, h, radii in NFOR_CASES:
+        wl = writers[scene](os.path.join(tmp, f"n_{scene}"))
+        out = os.path.join(tmp, f"{name}.bin")
+        run("denoise", wl.scene_path, 1, os.cpu_count() or 8, out, spp, w, h, *radii)
+        head = np.fromfile(out, dtype="<u4", count=2)
+        assert head[0] == w and head[1] == h
+        data = np.fromfile(out, dtype="<f8", offset=8)
+        n = w * h
+        fields, off = {}, 0
+        for k, c in (("image", 3), ("color_a", 3), ("color_b", 3), ("color_variance", 1), ("features_a", 11),
+                     ("features_b", 11), ("feature_variance", 11), ("denoised", 3)):
+            fields[k] = data[off:off + n * c].reshape(h, w, c)
+            off += n * c
+        assert off == data.size
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), seed=1, spp=spp, scene=scene,
+                            radii=np.array(radii), **fields)
+        print(name, {k: float(np.abs(v).mean()) for k, v in fields.items()})
+
+
 def fd_fixture(tmp, name="diff_fd_128_8spp_depth2", width=128, height=128, spp=8, depth=2, metals=False, h_albedo=0.01,
                h_emission=0.1, seed=1):
     """FP64 central finite differences of L = sum(w * image) on the reference renderer, with
@@ -191,6 +222,8 @@ def main():
             fd_fixture(tmp)
         if "zstats" in only:
             zstat_fixtures(tmp)
+        if "nfor" in only:
+            nfor_fixtures(tmp)
         return
     sampler_fixture(tmp)
     small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
@@ -199,6 +232,7 @@ def main():
     small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
     fd_fixture(tmp)
     zstat_fixtures(tmp)
+    nfor_fixtures(tmp)
 
     # Cornell box, 300x300, 50 spp, depth 8, seed 1 (the plain CPU-runnable config)
     c = workloads.write_cornell(os.path.join(tmp, "cornell"))

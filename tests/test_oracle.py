@@ -133,3 +133,18 @@ def test_bias_fixture_tiles_match_oracle(cornell):
         img, _ = oracle_lib.render(cornell.scene_path, int(g["first_seed"]) + k, int(g["spp"]), int(g["width"]),
                                    int(g["height"]))
         assert np.allclose(tile_means(img), g["tile_means"][k], rtol=1e-12, atol=1e-15)
+
+
+def test_nfor_fixture_is_consistent():
+    """The denoise fixtures' layout: the raw mean image is the sample-weighted mean of the
+    half-buffers (even samples in A, odd in B), and features stay in their ranges."""
+    for name in ("nfor_cornell_48x48_8spp", "nfor_primitives_64x48_8spp_r9"):
+        g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+        n = int(g["spp"])
+        na, nb = (n + 1) // 2, n // 2
+        mean = (g["color_a"] * na + g["color_b"] * nb) / n
+        assert np.allclose(mean, g["image"], rtol=1e-12, atol=1e-12)
+        for k in ("features_a", "features_b"):
+            f = g[k]
+            assert f.min() >= 0.0 and f.max() <= 1.0 + 1e-12
+        assert (g["color_variance"] >= 0).all() and (g["feature_variance"] >= 0).all()
