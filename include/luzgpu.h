@@ -65,7 +65,22 @@ typedef struct luzgpu_settings
 	uint32_t	autotune;        /* 1: use the context's tuned launch shape for these scenes and      *
 	                              * settings, tuning it first if there is none (luzgpu_autotune);     *
 	                              * the fields above are then ignored                                 */
+	/* adaptive per-pixel stopping (_threadRender, Threads.cpp:551-583; rule :194-262). It applies
+	 * when enabled and adaptive_min_samples < samples (adaptiveSamplingCanStop, :265-271), and
+	 * then needs the whole sample range in one call. */
+	uint32_t	adaptive;        /* LUZGPU_ADAPTIVE_*: the scene's adaptive= setting, or forced       */
+	uint32_t	adaptive_min_samples;    /* 0: scene value (adaptiveminsamples, default 128)      */
+	uint32_t	adaptive_check_interval; /* 0: scene value (adaptivecheckinterval, default 32)    */
+	uint32_t	reserved0;
+	double		adaptive_threshold;      /* <= 0: scene value (adaptivethreshold, default 0.03)   */
 } luzgpu_settings;
+
+enum
+{
+	LUZGPU_ADAPTIVE_SCENE = 0,   /* as the scene file says (--adaptive / --no-adaptive unset) */
+	LUZGPU_ADAPTIVE_OFF = 1,     /* --no-adaptive (FlagsParser)                               */
+	LUZGPU_ADAPTIVE_ON = 2       /* --adaptive                                                */
+};
 
 /* Launch shape chosen by luzgpu_autotune, with the calibration that chose it. */
 typedef struct luzgpu_tuning
@@ -156,15 +171,19 @@ int		luzgpu_scene_get_info(const luzgpu_scene* scene, luzgpu_scene_info* out);
 
 /* Renders samples [sample_begin, sample_begin + sample_count) and writes
  * rgb_out[W*H*3] (fp32, row-major, y down) = mean over those samples of the cleaned
- * per-sample radiance — Scene::getImage() after _threadRender with adaptive sampling and
- * denoising off and a raw view transform (Threads.cpp:551-583, 77-88). */
+ * per-sample radiance — Scene::getImage() after _threadRender with denoising off and a raw
+ * view transform (Threads.cpp:551-583, 77-88). With adaptive sampling in effect (the
+ * settings' adaptive fields) each pixel stops where _threadRender's loop stops it and its
+ * mean is over its own samples; stats->rendered_samples is then the sum of the per-pixel
+ * counts (SceneRenderStats::renderedSamples, Threads.cpp:439,491). */
 int		luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings,
 			float* rgb_out, luzgpu_stats* stats);
 
 /* Renders n scenes (1..127) in one multi-scene wavefront launch sequence: the scenes'
  * device arrays are merged (and cached while the same scenes are passed again), every pass
  * covers a slice of samples of every scene, and the pass's launches are replayed from a
- * captured hipGraph. Replaces n concurrent Renderer::render calls, which the reference
+ * captured hipGraph. Scenes with adaptive sampling in effect are rendered in rounds over
+ * their still-active pixels, all scenes' rounds sharing the passes. Replaces n concurrent Renderer::render calls, which the reference
  * cannot run (its render seed is global, Sampler.cpp:28). settings[i] (or NULL: scene
  * defaults) and rgb_out[i] are per scene, as for luzgpu_render; the integrator must be the
  * wavefront one, and launch tuning, visit counting and stage timing are taken from
@@ -218,6 +237,10 @@ int		luzgpu_scene_set_materials(luzgpu_scene* scene, uint32_t count, const doubl
 			const double* emission);
 
 /* --- introspection used by the parity tests (no counterpart in the reference API) --- */
+
+/* Per-pixel sample counts (the values _threadRender returns, Threads.cpp:582,640) of the
+ * context's last luzgpu_render or luzgpu_render_denoised: count = W*H of that render. */
+int		luzgpu_sample_counts(luzgpu_ctx* ctx, uint32_t* counts_out, size_t count);
 
 /* Primary-ray closest hits of sample 0, stream 0 (RayHelper.cpp:85-106 + HitHelper.cpp:5-32):
  * ids_out[W*H*2] = (top-level hittable index in Scene::getHittables() order, triangle index

@@ -27,6 +27,7 @@ EXPORTED = [
     "luzgpu_scene_release", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_primary_hits",
     "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas", "luzgpu_render_batch",
     "luzgpu_adjoint", "luzgpu_scene_set_materials", "luzgpu_autotune", "luzgpu_render_denoised", "luzgpu_nfor",
+    "luzgpu_sample_counts",
 ]
 
 
@@ -49,7 +50,15 @@ class Settings(ctypes.Structure):
         ("fetch_chunk", ctypes.c_uint32),
         ("lds_stack", ctypes.c_uint32),
         ("autotune", ctypes.c_uint32),
+        ("adaptive", ctypes.c_uint32),
+        ("adaptive_min_samples", ctypes.c_uint32),
+        ("adaptive_check_interval", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("adaptive_threshold", ctypes.c_double),
     ]
+
+
+ADAPTIVE_SCENE, ADAPTIVE_OFF, ADAPTIVE_ON = 0, 1, 2
 
 
 class Tuning(ctypes.Structure):
@@ -156,6 +165,7 @@ def load() -> ctypes.CDLL:
     lib.luzgpu_render_batch.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(vp), ctypes.POINTER(Settings),
                                         ctypes.POINTER(vp), ctypes.POINTER(Stats)]
     lib.luzgpu_sampler_eval.argtypes = [vp, ctypes.c_size_t, vp, vp, vp]
+    lib.luzgpu_sample_counts.argtypes = [vp, vp, ctypes.c_size_t]
     lib.luzgpu_scene_tlas_topology.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     lib.luzgpu_scene_blas.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64)]

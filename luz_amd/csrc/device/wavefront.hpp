@@ -26,6 +26,7 @@
 // the host captures once as a hipGraph and replays for every pass.
 #pragma once
 
+#include "adaptive.hpp"
 #include "integrator.hpp"
 
 #include <hip/hip_runtime.h>
@@ -88,9 +89,17 @@ namespace luzgpu
 	{
 		GpuParams	params;		// sampleBegin / sampleCount: this pass's slice
 		std::uint32_t	pathBase;	// first global path id of the slice
-		std::uint32_t	pixels;
+		std::uint32_t	pixels;		// pixels rendered: the image, or the entries of pixelList
 		double*	sums;			// the scene's FP64 pixel sums
+		const std::uint32_t*	pixelList;	// adaptive rounds: the active pixels (nullptr: every pixel)
+		double*	moments;		// adaptive rounds: per-pixel AdaptiveAccumulator (or nullptr)
 	};
+
+	// Image pixel of entry i of a pass's pixel range.
+	__device__ __forceinline__ std::uint32_t	passPixel(const PassJob& J, std::uint32_t i)
+	{
+		return J.pixelList ? J.pixelList[i] : i;
+	}
 
 	constexpr int	kWaveBlock = 256;
 	constexpr int	kWaveLdsStack = 16;		// default LDS stack entries per lane (tunable: 8, 16, 24)
@@ -147,13 +156,14 @@ namespace luzgpu
 
 	__global__ __launch_bounds__(kWaveBlock) void	wf_raygen(const PassJob* jobs, std::uint32_t job, WaveBuffers w)
 	{
-		const GpuParams& p = jobs[job].params;
-		const std::uint32_t pixels = jobs[job].pixels;
-		const std::uint32_t base = jobs[job].pathBase;
+		const PassJob& J = jobs[job];
+		const GpuParams& p = J.params;
+		const std::uint32_t pixels = J.pixels;
+		const std::uint32_t base = J.pathBase;
 		const std::uint32_t n = pixels * p.sampleCount;
 		for (std::uint32_t id = blockIdx.x * kWaveBlock + threadIdx.x; id < n; id += gridDim.x * kWaveBlock)
 		{
-			const std::uint32_t pixel = id % pixels, slot = id / pixels;
+			const std::uint32_t pixel = passPixel(J, id % pixels), slot = id / pixels;
 			const std::uint32_t x = pixel % p.width, y = pixel / p.width;
 			PixelSampler ps;
 			ps.begin(p.renderSeed, x, y, p.sampleBegin + slot, sampleStream(p, p.sampleBegin + slot));
@@ -443,7 +453,7 @@ namespace luzgpu
 				const GpuParams& p = J.params;
 				const bool sampledInfinite = hasEnvLight(p) || hasConstantBackgroundLight(p);
 				const std::uint32_t local = path - J.pathBase;
-				const std::uint32_t pixel = local % J.pixels, slot = local / J.pixels;
+				const std::uint32_t pixel = passPixel(J, local % J.pixels), slot = local / J.pixels;
 				const std::uint32_t px = pixel % p.width, py = pixel / p.width;
 				PixelSampler ps;
 				ps.begin(p.renderSeed, px, py, p.sampleBegin + slot, sampleStream(p, p.sampleBegin + slot));
@@ -671,24 +681,37 @@ namespace luzgpu
 	}
 
 	// Adds a pass's sample colours of one scene into its FP64 pixel sums, in sample order
-	// (the per-pixel sum of _threadRender, Threads.cpp:551-583).
+	// (the per-pixel sum of _threadRender, Threads.cpp:551-583), and in adaptive rounds into
+	// the pixel's AdaptiveAccumulator (Threads.cpp:571-576).
 	__global__ void	wf_accumulate(const float* __restrict__ colors, const PassJob* jobs, std::uint32_t job)
 	{
 		const PassJob& J = jobs[job];
 		const std::uint32_t count = J.params.sampleCount;
 		for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < J.pixels; i += gridDim.x * blockDim.x)
 		{
-			double r = J.sums[3 * static_cast<size_t>(i)], g = J.sums[3 * static_cast<size_t>(i) + 1], b = J.sums[3 * static_cast<size_t>(i) + 2];
+			const size_t px = passPixel(J, i);
+			double r = J.sums[3 * px], g = J.sums[3 * px + 1], b = J.sums[3 * px + 2];
+			double m[kAdaptiveMoments];
+			if (J.moments)
+			{
+				for (int k = 0; k < kAdaptiveMoments; k++) m[k] = J.moments[kAdaptiveMoments * px + k];
+			}
 			for (std::uint32_t k = 0; k < count; k++)
 			{
 				const float* c = colors + (static_cast<size_t>(J.pathBase) + static_cast<size_t>(k) * J.pixels + i) * 3;
-				r += static_cast<double>(c[0]);
-				g += static_cast<double>(c[1]);
-				b += static_cast<double>(c[2]);
+				const double cr = c[0], cg = c[1], cb = c[2];
+				r += cr;
+				g += cg;
+				b += cb;
+				if (J.moments) adaptiveAdd(m, cr, cg, cb);
 			}
-			J.sums[3 * static_cast<size_t>(i)] = r;
-			J.sums[3 * static_cast<size_t>(i) + 1] = g;
-			J.sums[3 * static_cast<size_t>(i) + 2] = b;
+			J.sums[3 * px] = r;
+			J.sums[3 * px + 1] = g;
+			J.sums[3 * px + 2] = b;
+			if (J.moments)
+			{
+				for (int k = 0; k < kAdaptiveMoments; k++) J.moments[kAdaptiveMoments * px + k] = m[k];
+			}
 		}
 	}
 
@@ -741,7 +764,15 @@ namespace luzgpu
 	{
 		GpuParams	params;
 		double*	sums;
+		const std::uint32_t*	pixelList = nullptr;	// adaptive round: the active pixels
+		std::uint32_t	listCount = 0;				// ... and how many
+		double*	moments = nullptr;				// adaptive round: per-pixel accumulators
 	};
+
+	inline std::uint32_t	jobPixels(const BatchJob& j)
+	{
+		return j.pixelList ? j.listCount : j.params.width * j.params.height;
+	}
 
 	// Device state of one context (grown on demand, kept between renders), and the
 	// captured pass graph with what it was captured for.
@@ -938,7 +969,7 @@ namespace luzgpu
 			{
 				throw std::runtime_error("Unsupported: more than 62 bounces in the wavefront integrator");
 			}
-			const std::uint64_t pixels = static_cast<std::uint64_t>(j.params.width) * j.params.height;
+			const std::uint64_t pixels = j.pixelList ? j.listCount : static_cast<std::uint64_t>(j.params.width) * j.params.height;
 			if (pixels == 0 || pixels > kMaxPassPaths)
 			{
 				throw std::runtime_error("Unsupported: image of more than 16M pixels in the wavefront integrator");
@@ -961,7 +992,7 @@ namespace luzgpu
 			for (std::uint32_t j = 0; j < n; j++)
 			{
 				const GpuParams& p = jobs[j].params;
-				const std::uint32_t pixels = p.width * p.height;
+				const std::uint32_t pixels = jobPixels(jobs[j]);
 				const std::uint32_t c = std::min(p.sampleCount - done[j], budget / pixels);
 				PassJob pj{};
 				pj.params = p;
@@ -970,6 +1001,8 @@ namespace luzgpu
 				pj.pathBase = base;
 				pj.pixels = pixels;
 				pj.sums = jobs[j].sums;
+				pj.pixelList = jobs[j].pixelList;
+				pj.moments = jobs[j].moments;
 				plan.push_back(pj);
 				done[j] += c;
 				base += pixels * c;

@@ -898,10 +898,10 @@ namespace luzgpu
 	{
 		PackedScene p;
 		const Settings& s = scene.settings;
-		if (s.adaptive)
-		{
-			throw std::runtime_error("Unsupported for the device path: adaptive sampling (set adaptive=0)");
-		}
+		p.adaptive.enabled = s.adaptive;
+		p.adaptive.minSamples = s.adaptiveMinSamples;
+		p.adaptive.checkInterval = s.adaptiveCheckInterval;
+		p.adaptive.threshold = s.adaptiveThreshold;
 
 		// ---- materials -------------------------------------------------------------
 		for (const Material& m : scene.materials)

@@ -31,6 +31,16 @@ namespace luzgpu
 
 	Blas	buildBlas(const Mesh& mesh);
 
+	// The scene file's adaptive sampling settings (Settings.cpp:210-261, defaults
+	// Defaults.hpp:13-16); whether a render stops pixels early is decided per call.
+	struct	AdaptiveSettings
+	{
+		bool	enabled = true;
+		std::uint32_t	minSamples = 128;
+		std::uint32_t	checkInterval = 32;
+		double	threshold = 0.03;
+	};
+
 	struct	PackedScene
 	{
 		std::vector<BvhNode>	tlasNodes;	// binary device form (intermediate)
@@ -71,6 +81,7 @@ namespace luzgpu
 		int	features = 0;	// kFeat* classes present (kernel specialisation)
 		std::vector<double>	lightScale;	// light selection weight per unit luminance (area, or 1)
 		std::vector<dvec3>	materialColor;	// FP64 material colours (weights are rebuilt from these)
+		AdaptiveSettings	adaptive;
 	};
 
 	// Rebuilds light emission, selection weights, CDF and probabilities after material

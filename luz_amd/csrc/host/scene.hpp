@@ -182,6 +182,9 @@ namespace luzgpu
 		std::uint32_t	samples = 48;
 		std::int32_t	maxBounces = 6;
 		bool	adaptive = true;
+		std::uint32_t	adaptiveMinSamples = 128;	// Defaults.hpp:13-16
+		std::uint32_t	adaptiveCheckInterval = 32;
+		double	adaptiveThreshold = 0.03;
 		bool	denoise = true;
 		SkyKind	sky = SKY_NONE;
 		bool	skySet = false;

@@ -209,7 +209,59 @@ namespace luzgpu
 					}
 					else if (key == "adaptive" || key == "adaptivesampling" || key == "adaptive_sampling")
 					{
-						s.adaptive = parseDouble(value, "adaptive setting") != 0.0;
+						// Settings.cpp:210-224: an integer, 0 or 1
+						int v = 0;
+						if (std::sscanf(value.c_str(), "%d", &v) != 1)
+						{
+							throw std::runtime_error("Invalid adaptive setting. Use adaptive=0 or adaptive=1.");
+						}
+						if (v != 0 && v != 1)
+						{
+							throw std::runtime_error("Invalid adaptive setting. Use 0 or 1.");
+						}
+						s.adaptive = v != 0;
+					}
+					else if (key == "adaptiveminsamples" || key == "adaptive_min_samples")
+					{
+						// Settings.cpp:225-237, Scene.cpp:193-200
+						int v = 0;
+						if (std::sscanf(value.c_str(), "%d", &v) != 1)
+						{
+							throw std::runtime_error("Invalid adaptive minimum samples setting. Use adaptiveminsamples=N.");
+						}
+						if (v <= 0)
+						{
+							throw std::runtime_error("Adaptive minimum samples must be positive.");
+						}
+						s.adaptiveMinSamples = static_cast<std::uint32_t>(v);
+					}
+					else if (key == "adaptivethreshold" || key == "adaptive_threshold")
+					{
+						// Settings.cpp:238-249, Scene.cpp:221-228
+						double v = 0.0;
+						if (std::sscanf(value.c_str(), "%lf", &v) != 1)
+						{
+							throw std::runtime_error("Invalid adaptive threshold setting. Use adaptivethreshold=F.");
+						}
+						if (!std::isfinite(v) || v <= 0.0)
+						{
+							throw std::runtime_error("Adaptive threshold must be positive.");
+						}
+						s.adaptiveThreshold = v;
+					}
+					else if (key == "adaptivecheckinterval" || key == "adaptive_check_interval")
+					{
+						// Settings.cpp:250-261, Scene.cpp:207-214
+						int v = 0;
+						if (std::sscanf(value.c_str(), "%d", &v) != 1)
+						{
+							throw std::runtime_error("Invalid adaptive check interval setting. Use adaptivecheckinterval=N.");
+						}
+						if (v <= 0)
+						{
+							throw std::runtime_error("Adaptive check interval must be positive.");
+						}
+						s.adaptiveCheckInterval = static_cast<std::uint32_t>(v);
 					}
 					else if (key == "denoise")
 					{

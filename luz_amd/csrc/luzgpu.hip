@@ -95,23 +95,25 @@ namespace luzgpu
 	constexpr int	kMegaBlock = 128;
 	constexpr int	kStackDepth = 48;
 
-	// counters: [0] closest-hit rays, [1] shadow rays, [2] nodes visited, [3] primitives tested
+	// counters: [0] closest-hit rays, [1] shadow rays, [2] nodes visited, [3] primitives tested.
+	// Renders samples [sampleBegin, sampleBegin + sampleCount) of `pixels` pixels: entries of
+	// `list` (adaptive rounds), or the whole image when `list` is nullptr.
 	template <bool Count, int Feat>
-	__global__ __launch_bounds__(kMegaBlock) void	render_mega(SceneView s, GpuParams p, float* __restrict__ sampleColors,
-		unsigned long long* __restrict__ counters)
+	__global__ __launch_bounds__(kMegaBlock) void	render_mega(SceneView s, GpuParams p, const std::uint32_t* __restrict__ list,
+		std::uint32_t pixels, float* __restrict__ sampleColors, unsigned long long* __restrict__ counters)
 	{
 		__shared__ std::uint32_t stackMem[kStackDepth * kMegaBlock];
-		const std::uint32_t pixels = p.width * p.height;
 		const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * kMegaBlock + threadIdx.x;
 		std::uint32_t closest = 0, shadow = 0;
 		VisitCounter visits{0u, 0u};
 		if (gid < static_cast<std::uint64_t>(pixels) * p.sampleCount)
 		{
 			const std::uint32_t slot = static_cast<std::uint32_t>(gid / pixels);
-			const std::uint32_t pixel = static_cast<std::uint32_t>(gid % pixels);
+			const std::uint32_t entry = static_cast<std::uint32_t>(gid % pixels);
+			const std::uint32_t pixel = list ? list[entry] : entry;
 			LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
 			const f3 c = cleanColor(tracePath<Count, Feat>(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest, shadow, visits));
-			float* out = sampleColors + (static_cast<size_t>(slot) * pixels + pixel) * 3;
+			float* out = sampleColors + (static_cast<size_t>(slot) * pixels + entry) * 3;
 			out[0] = c.x;
 			out[1] = c.y;
 			out[2] = c.z;
@@ -139,47 +141,46 @@ namespace luzgpu
 		}
 	}
 
-	// Adds this launch's samples into the FP64 pixel sums, in sample order.
 	template <bool Count>
-	void	launchMegaC(int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, float* colors,
-		unsigned long long* counters)
+	void	launchMegaC(int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, const std::uint32_t* list,
+		std::uint32_t pixels, float* colors, unsigned long long* counters)
 	{
 		switch (features & kFeatAll)
 		{
-			case 0: render_mega<Count, 0><<<grid, kMegaBlock, 0, st>>>(s, p, colors, counters); break;
-			case kFeatAnalytic: render_mega<Count, kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, p, colors, counters); break;
-			case kFeatRich: render_mega<Count, kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, p, colors, counters); break;
-			default: render_mega<Count, kFeatAll><<<grid, kMegaBlock, 0, st>>>(s, p, colors, counters); break;
+			case 0: render_mega<Count, 0><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, counters); break;
+			case kFeatAnalytic: render_mega<Count, kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, counters); break;
+			case kFeatRich: render_mega<Count, kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, counters); break;
+			default: render_mega<Count, kFeatAll><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, counters); break;
 		}
 	}
 
-	void	launchMega(bool count, int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, float* colors,
-		unsigned long long* counters)
+	void	launchMega(bool count, int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p,
+		const std::uint32_t* list, std::uint32_t pixels, float* colors, unsigned long long* counters)
 	{
-		if (count) launchMegaC<true>(features, grid, st, s, p, colors, counters);
-		else launchMegaC<false>(features, grid, st, s, p, colors, counters);
+		if (count) launchMegaC<true>(features, grid, st, s, p, list, pixels, colors, counters);
+		else launchMegaC<false>(features, grid, st, s, p, list, pixels, colors, counters);
 	}
 
 	// The plain kernel with denoising on: each (pixel, sample) also writes its primary
 	// features (kNforFeatures floats, RenderSample of _calculatePixelSample, ColorHelper.cpp:1119-1126).
 	template <int Feat>
-	__global__ __launch_bounds__(kMegaBlock) void	render_mega_denoise(SceneView s, GpuParams p, float* __restrict__ sampleColors,
-		float* __restrict__ sampleFeatures, unsigned long long* __restrict__ counters)
+	__global__ __launch_bounds__(kMegaBlock) void	render_mega_denoise(SceneView s, GpuParams p, const std::uint32_t* __restrict__ list,
+		std::uint32_t pixels, float* __restrict__ sampleColors, float* __restrict__ sampleFeatures, unsigned long long* __restrict__ counters)
 	{
 		__shared__ std::uint32_t stackMem[kStackDepth * kMegaBlock];
-		const std::uint32_t pixels = p.width * p.height;
 		const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * kMegaBlock + threadIdx.x;
 		std::uint32_t closest = 0, shadow = 0;
 		VisitCounter visits{0u, 0u};
 		if (gid < static_cast<std::uint64_t>(pixels) * p.sampleCount)
 		{
 			const std::uint32_t slot = static_cast<std::uint32_t>(gid / pixels);
-			const std::uint32_t pixel = static_cast<std::uint32_t>(gid % pixels);
+			const std::uint32_t entry = static_cast<std::uint32_t>(gid % pixels);
+			const std::uint32_t pixel = list ? list[entry] : entry;
 			LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
 			float f[kNforFeatures];
 			const f3 c = cleanColor(tracePath<false, Feat>(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest,
 				shadow, visits, static_cast<NoAdjoint*>(nullptr), f));
-			const size_t k = static_cast<size_t>(slot) * pixels + pixel;
+			const size_t k = static_cast<size_t>(slot) * pixels + entry;
 			sampleColors[3 * k] = c.x;
 			sampleColors[3 * k + 1] = c.y;
 			sampleColors[3 * k + 2] = c.z;
@@ -197,15 +198,15 @@ namespace luzgpu
 		}
 	}
 
-	void	launchMegaDenoise(int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, float* colors,
-		float* feats, unsigned long long* counters)
+	void	launchMegaDenoise(int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, const std::uint32_t* list,
+		std::uint32_t pixels, float* colors, float* feats, unsigned long long* counters)
 	{
 		switch (features & kFeatAll)
 		{
-			case 0: render_mega_denoise<0><<<grid, kMegaBlock, 0, st>>>(s, p, colors, feats, counters); break;
-			case kFeatAnalytic: render_mega_denoise<kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, p, colors, feats, counters); break;
-			case kFeatRich: render_mega_denoise<kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, p, colors, feats, counters); break;
-			default: render_mega_denoise<kFeatAll><<<grid, kMegaBlock, 0, st>>>(s, p, colors, feats, counters); break;
+			case 0: render_mega_denoise<0><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, feats, counters); break;
+			case kFeatAnalytic: render_mega_denoise<kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, feats, counters); break;
+			case kFeatRich: render_mega_denoise<kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, feats, counters); break;
+			default: render_mega_denoise<kFeatAll><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, feats, counters); break;
 		}
 	}
 
@@ -215,13 +216,23 @@ namespace luzgpu
 	// and feature squares.
 	constexpr int	kDenoiseStats = 12 + 4 * kNforFeatures;
 
+	// `list` / `pixels`: as render_mega_denoise; `moments` (adaptive rounds, or nullptr):
+	// the pixels' AdaptiveAccumulator.
 	__global__ void	denoise_accumulate(const float* __restrict__ colors, const float* __restrict__ feats, double* __restrict__ acc,
-		std::uint32_t pixels, std::uint32_t count, std::uint32_t firstSample)
+		const std::uint32_t* __restrict__ list, std::uint32_t pixels, std::uint32_t count, std::uint32_t firstSample,
+		double* __restrict__ moments)
 	{
+		#pragma clang fp contract(off)
 		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 		if (i >= pixels) return;
+		const size_t px = list ? list[i] : i;
 		double a[kDenoiseStats];
-		for (int j = 0; j < kDenoiseStats; j++) a[j] = acc[static_cast<size_t>(kDenoiseStats) * i + j];
+		for (int j = 0; j < kDenoiseStats; j++) a[j] = acc[static_cast<size_t>(kDenoiseStats) * px + j];
+		double m[kAdaptiveMoments];
+		if (moments)
+		{
+			for (int j = 0; j < kAdaptiveMoments; j++) m[j] = moments[kAdaptiveMoments * px + j];
+		}
 		for (std::uint32_t k = 0; k < count; k++)
 		{
 			const size_t idx = static_cast<size_t>(k) * pixels + i;
@@ -233,6 +244,7 @@ namespace luzgpu
 				a[6 + j] += c;
 				a[9 + j] += c * c;
 			}
+			if (moments) adaptiveAdd(m, colors[3 * idx], colors[3 * idx + 1], colors[3 * idx + 2]);
 			const int fh = ((firstSample + k) & 1u) ? 12 + kNforFeatures : 12;
 			for (int j = 0; j < kNforFeatures; j++)
 			{
@@ -242,7 +254,11 @@ namespace luzgpu
 				a[12 + 3 * kNforFeatures + j] += f * f;
 			}
 		}
-		for (int j = 0; j < kDenoiseStats; j++) acc[static_cast<size_t>(kDenoiseStats) * i + j] = a[j];
+		for (int j = 0; j < kDenoiseStats; j++) acc[static_cast<size_t>(kDenoiseStats) * px + j] = a[j];
+		if (moments)
+		{
+			for (int j = 0; j < kAdaptiveMoments; j++) moments[kAdaptiveMoments * px + j] = m[j];
+		}
 	}
 
 	// sampleMeanVariance (Threads.cpp:150-161)
@@ -255,12 +271,15 @@ namespace luzgpu
 
 	// storeDenoisePixel (Threads.cpp:340-372) and the pixel's raw mean (:628-631): the NFOR
 	// input buffers (FP64) and the image.
-	__global__ void	denoise_finalize(const double* __restrict__ acc, std::uint32_t pixels, std::uint32_t n, float* __restrict__ image,
-		double* __restrict__ meanImage, double* __restrict__ colorA, double* __restrict__ colorB, double* __restrict__ colorVariance,
-		double* __restrict__ featA, double* __restrict__ featB, double* __restrict__ featVariance)
+	// `used` (adaptive renders, or nullptr): each pixel's sample count instead of `n`.
+	__global__ void	denoise_finalize(const double* __restrict__ acc, std::uint32_t pixels, std::uint32_t uniformCount,
+		const std::uint32_t* __restrict__ used, float* __restrict__ image, double* __restrict__ meanImage, double* __restrict__ colorA,
+		double* __restrict__ colorB, double* __restrict__ colorVariance, double* __restrict__ featA, double* __restrict__ featB,
+		double* __restrict__ featVariance)
 	{
 		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 		if (i >= pixels) return;
+		const std::uint32_t n = used ? used[i] : uniformCount;
 		const double* a = acc + static_cast<size_t>(kDenoiseStats) * i;
 		const std::uint32_t nA = (n + 1u) / 2u, nB = n / 2u;	// samples 0, 2, 4... and 1, 3, 5...
 		const double dn = static_cast<double>(n);
@@ -337,21 +356,51 @@ namespace luzgpu
 		}
 	}
 
-	__global__ void	accumulate(const float* __restrict__ sampleColors, double* __restrict__ sums, std::uint32_t pixels, std::uint32_t count)
+	// Adds one launch's samples (render_mega's layout) into the FP64 pixel sums in sample
+	// order, and in adaptive rounds into the pixels' AdaptiveAccumulator.
+	__global__ void	accumulate(const float* __restrict__ sampleColors, double* __restrict__ sums, const std::uint32_t* __restrict__ list,
+		std::uint32_t pixels, std::uint32_t count, double* __restrict__ moments)
 	{
 		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 		if (i >= pixels) return;
-		double r = sums[3 * i], g = sums[3 * i + 1], b = sums[3 * i + 2];
+		const size_t px = list ? list[i] : i;
+		double r = sums[3 * px], g = sums[3 * px + 1], b = sums[3 * px + 2];
+		double m[kAdaptiveMoments];
+		if (moments)
+		{
+			for (int j = 0; j < kAdaptiveMoments; j++) m[j] = moments[kAdaptiveMoments * px + j];
+		}
 		for (std::uint32_t k = 0; k < count; k++)
 		{
 			const float* c = sampleColors + (static_cast<size_t>(k) * pixels + i) * 3;
-			r += static_cast<double>(c[0]);
-			g += static_cast<double>(c[1]);
-			b += static_cast<double>(c[2]);
+			const double cr = c[0], cg = c[1], cb = c[2];
+			r += cr;
+			g += cg;
+			b += cb;
+			if (moments) adaptiveAdd(m, cr, cg, cb);
 		}
-		sums[3 * i] = r;
-		sums[3 * i + 1] = g;
-		sums[3 * i + 2] = b;
+		sums[3 * px] = r;
+		sums[3 * px + 1] = g;
+		sums[3 * px + 2] = b;
+		if (moments)
+		{
+			for (int j = 0; j < kAdaptiveMoments; j++) moments[kAdaptiveMoments * px + j] = m[j];
+		}
+	}
+
+	// The image of an adaptive render: each pixel's sum over its own sample count,
+	// cleanColor'd (Threads.cpp:580-581).
+	__global__ void	resolve_used(const double* __restrict__ sums, const std::uint32_t* __restrict__ used, float* __restrict__ out,
+		std::uint32_t pixels)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= pixels) return;
+		const double n = static_cast<double>(used[i]);
+		for (int k = 0; k < 3; k++)
+		{
+			const double v = sums[3 * i + k] / n;
+			out[3 * i + k] = isfinite(v) ? static_cast<float>(v) : 0.0f;
+		}
 	}
 
 	__global__ void	resolve(const double* __restrict__ sums, float* __restrict__ out, std::uint32_t pixels, double count)
@@ -375,6 +424,12 @@ namespace luzgpu
 	{
 		const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 		if (i < n) p[i] = 0.0;
+	}
+
+	__global__ void	zero_u32(std::uint32_t* p, std::uint32_t n)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i < n) p[i] = 0u;
 	}
 
 	__global__ void	zero_u64(unsigned long long* p, size_t n)
@@ -463,6 +518,13 @@ struct luzgpu_ctx
 	luzgpu::DeviceBuffer	tuneSums;
 	// denoising: per-sample features, per-pixel statistics, NFOR inputs and temporaries
 	luzgpu::DeviceBuffer	sampleFeatures, denoiseStats, nforBuffers, nforTemp;
+	// adaptive rounds: per-pixel accumulators, sample counts, two active-pixel lists, counters
+	luzgpu::DeviceBuffer	adaptiveMem;
+	// per-pixel sample counts of the last single-scene render: `countsUniform` samples each,
+	// or (countsUniform == 0) the adaptive rounds' counts in adaptiveMem
+	std::uint32_t	countsPixels = 0;
+	std::uint32_t	countsUniform = 0;
+	std::uint32_t*	countsDevice = nullptr;
 };
 
 struct luzgpu_scene
@@ -821,6 +883,152 @@ namespace luzgpu
 			if (report) *report = r;
 			return tuningOf(r);
 		}
+
+		// The stopping rule of a render, or false when every pixel takes every sample: the
+		// scene's adaptive settings with the caller's overrides (FlagsParser's --adaptive
+		// options), in effect when enabled and min samples < samples (adaptiveSamplingCanStop,
+		// Threads.cpp:265-271).
+		bool	adaptiveRule(const PackedScene& h, const GpuParams& p, const luzgpu_settings* s, AdaptiveRule& rule)
+		{
+			AdaptiveSettings a = h.adaptive;
+			if (s)
+			{
+				if (s->adaptive == LUZGPU_ADAPTIVE_OFF) a.enabled = false;
+				else if (s->adaptive == LUZGPU_ADAPTIVE_ON) a.enabled = true;
+				else if (s->adaptive != LUZGPU_ADAPTIVE_SCENE) throw Error{LUZGPU_E_INVALID, "unknown adaptive mode"};
+				if (s->adaptive_min_samples) a.minSamples = s->adaptive_min_samples;
+				if (s->adaptive_check_interval) a.checkInterval = s->adaptive_check_interval;
+				if (s->adaptive_threshold > 0.0)
+				{
+					if (!std::isfinite(s->adaptive_threshold)) throw Error{LUZGPU_E_INVALID, "Adaptive threshold must be positive."};
+					a.threshold = s->adaptive_threshold;
+				}
+			}
+			const std::uint32_t total = p.totalSamples;
+			if (!a.enabled || a.minSamples >= total) return false;
+			if (p.sampleBegin != 0u || p.sampleCount != total)
+			{
+				throw Error{LUZGPU_E_INVALID, "adaptive sampling renders the whole sample range [0, samples) in one call "
+					"(turn it off to split a render by sample range)"};
+			}
+			rule.maxSamples = total;
+			rule.minSamples = a.minSamples;
+			rule.checkInterval = a.checkInterval;
+			// darkMinimumSamples (Threads.cpp:183-192)
+			const std::uint64_t dark = std::max<std::uint64_t>(256u, static_cast<std::uint64_t>(a.minSamples) + 3ull * a.checkInterval);
+			rule.darkMinimum = static_cast<std::uint32_t>(std::min<std::uint64_t>(total, std::max<std::uint64_t>(a.minSamples, dark)));
+			rule.threshold = a.threshold;
+			return true;
+		}
+
+		// One image of an adaptive render and its round state. A job whose rule cannot stop
+		// early (minSamples == maxSamples) renders every sample in its first round.
+		struct	AdaptiveJob
+		{
+			AdaptiveRule	rule;
+			std::uint32_t	pixels;
+			size_t	offset;			// first pixel in the context's per-pixel arrays
+			std::uint32_t	first = 0;	// sample index of the job's first sample
+			std::uint32_t	used = 0;	// samples every active pixel has
+			std::uint32_t	active = 0;
+			std::uint32_t	round = 0;	// samples of the coming round
+			int	list = -1;			// active-pixel list in use (-1: every pixel)
+		};
+
+		// One round's work for one job: samples [begin, begin + count) of `active` pixels,
+		// `list` (or every pixel when nullptr), accumulated into the job's sums and `moments`.
+		struct	AdaptiveRound
+		{
+			std::uint32_t	job;
+			const std::uint32_t*	list;
+			std::uint32_t	active;
+			std::uint32_t	begin;
+			std::uint32_t	count;
+			double*	moments;
+		};
+
+		struct	AdaptiveArrays
+		{
+			double*	moments;
+			std::uint32_t*	used;
+			std::uint32_t*	lists[2];
+			std::uint32_t*	counters;
+		};
+
+		AdaptiveArrays	adaptiveArrays(luzgpu_ctx* ctx, size_t pixels, size_t jobs)
+		{
+			const size_t momentBytes = ((pixels * kAdaptiveMoments * sizeof(double)) + 255) & ~size_t(255);
+			const size_t listBytes = ((pixels * sizeof(std::uint32_t)) + 255) & ~size_t(255);
+			ctx->adaptiveMem.reserve(momentBytes + 3 * listBytes + jobs * sizeof(std::uint32_t));
+			char* b = static_cast<char*>(ctx->adaptiveMem.ptr);
+			AdaptiveArrays a;
+			a.moments = reinterpret_cast<double*>(b);
+			a.used = reinterpret_cast<std::uint32_t*>(b + momentBytes);
+			a.lists[0] = reinterpret_cast<std::uint32_t*>(b + momentBytes + listBytes);
+			a.lists[1] = reinterpret_cast<std::uint32_t*>(b + momentBytes + 2 * listBytes);
+			a.counters = reinterpret_cast<std::uint32_t*>(b + momentBytes + 3 * listBytes);
+			return a;
+		}
+
+		// Runs the rounds of every job until each pixel has stopped (adaptive.hpp); `render`
+		// enqueues one round of every job that still has active pixels. Returns the samples
+		// rendered (the sum of the per-pixel counts).
+		template <typename Render>
+		std::uint64_t	runAdaptive(luzgpu_ctx* ctx, std::vector<AdaptiveJob>& jobs, const AdaptiveArrays& A, Render&& render)
+		{
+			hipStream_t st = ctx->stream;
+			size_t pixels = 0;
+			for (AdaptiveJob& j : jobs)
+			{
+				j.used = 0;
+				j.active = j.pixels;
+				j.round = j.rule.minSamples;
+				j.list = -1;
+				pixels += j.pixels;
+			}
+			zero_doubles<<<blocks(pixels * kAdaptiveMoments, 256), 256, 0, st>>>(A.moments, pixels * kAdaptiveMoments);
+			std::uint64_t rendered = 0;
+			std::vector<std::uint32_t> next(jobs.size());
+			for (;;)
+			{
+				std::vector<AdaptiveRound> work;
+				for (std::uint32_t k = 0; k < jobs.size(); k++)
+				{
+					const AdaptiveJob& j = jobs[k];
+					if (j.active == 0u) continue;
+					work.push_back(AdaptiveRound{k, j.list < 0 ? nullptr : A.lists[j.list] + j.offset, j.active, j.first + j.used, j.round,
+						A.moments + j.offset * kAdaptiveMoments});
+				}
+				if (work.empty()) break;
+				render(work);
+				zero_u32<<<1, 256, 0, st>>>(A.counters, static_cast<std::uint32_t>(jobs.size()));
+				for (const AdaptiveRound& w : work)
+				{
+					const AdaptiveJob& j = jobs[w.job];
+					const int to = j.list < 0 ? 0 : 1 - j.list;
+					adaptive_check<<<blocks(w.active, 256), 256, 0, st>>>(j.rule, w.list, w.active, j.used + j.round, w.moments,
+						A.used + j.offset, A.lists[to] + j.offset, A.counters + w.job);
+				}
+				check(hipGetLastError(), "adaptive check launch");
+				check(hipMemcpyAsync(next.data(), A.counters, next.size() * sizeof(std::uint32_t), hipMemcpyDeviceToHost, st),
+					"adaptive counters");
+				check(hipStreamSynchronize(st), "adaptive round");
+				for (const AdaptiveRound& w : work)
+				{
+					AdaptiveJob& j = jobs[w.job];
+					rendered += static_cast<std::uint64_t>(j.active) * j.round;
+					j.used += j.round;
+					j.active = next[w.job];
+					j.list = j.list < 0 ? 0 : 1 - j.list;
+					j.round = std::min(j.rule.checkInterval, j.rule.maxSamples - j.used);
+					if (j.active > 0u && j.round == 0u)
+					{
+						throw Error{LUZGPU_E_DEVICE, "adaptive rounds: pixels left active at the sample total"};
+					}
+				}
+			}
+			return rendered;
+		}
 	}
 }
 
@@ -1022,54 +1230,84 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			tune = tunedFor(ctx, {scene->serial}, scene->gpu.view, scene->gpu.host.features, cal, &r);
 			if (ctx->tuned.size() != before) tuneMs = r.tune_ms;
 		}
+		AdaptiveRule rule{};
+		const bool adaptive = adaptiveRule(scene->gpu.host, p, settings, rule);
 		const std::uint32_t pixels = p.width * p.height;
 		hipStream_t st = ctx->stream;
 		ctx->sums.reserve(static_cast<size_t>(pixels) * 3 * sizeof(double));
 		ctx->counters.reserve(8 * sizeof(unsigned long long));
 		zero_doubles<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(ctx->sums.as<double>(), static_cast<size_t>(pixels) * 3);
 		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
+		ctx->countsPixels = 0;
 		hipEvent_t e0, e1;
 		check(hipEventCreate(&e0), "hipEventCreate");
 		check(hipEventCreate(&e1), "hipEventCreate");
 		std::vector<hipEvent_t> ks, ke;
 		StageTimes stages;
-		check(hipEventRecord(e0, st), "hipEventRecord");
-		if (integrator == LUZGPU_INTEGRATOR_MEGAKERNEL)
-		{
-			// samples per launch: bounded so the per-sample colour buffer stays within 4 GiB
-			const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount,
-				static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(pixels) * 12ull))));
-			ctx->sampleColors.reserve(static_cast<size_t>(pixels) * perLaunch * 3 * sizeof(float));
-			for (std::uint32_t done = 0; done < p.sampleCount; done += perLaunch)
-			{
-				ks.emplace_back();
-				ke.emplace_back();
-				check(hipEventCreate(&ks.back()), "hipEventCreate");
-				check(hipEventCreate(&ke.back()), "hipEventCreate");
-				check(hipEventRecord(ks.back(), st), "hipEventRecord");
-				GpuParams q = p;
-				q.sampleBegin = p.sampleBegin + done;
-				q.sampleCount = std::min(perLaunch, p.sampleCount - done);
-				const size_t threads = static_cast<size_t>(pixels) * q.sampleCount;
-				launchMega(settings && settings->count_visits, scene->gpu.host.features, blocks(threads, kMegaBlock), st, scene->gpu.view, q,
-					ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>());
-				check(hipGetLastError(), "render launch");
-				check(hipEventRecord(ke.back(), st), "hipEventRecord");
-				accumulate<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sums.as<double>(), pixels, q.sampleCount);
-			}
-		}
-		else
-		{
+		auto markStart = [&]() {
 			ks.emplace_back();
 			ke.emplace_back();
 			check(hipEventCreate(&ks.back()), "hipEventCreate");
 			check(hipEventCreate(&ke.back()), "hipEventCreate");
 			check(hipEventRecord(ks.back(), st), "hipEventRecord");
-			const std::vector<BatchJob> jobs{BatchJob{p, ctx->sums.as<double>()}};
-			renderWavefront(ctx->wave, scene->gpu.view, scene->gpu.host.features, jobs, ctx->counters.as<unsigned long long>(), st,
-				settings && settings->count_visits, settings && settings->stage_times ? &stages : nullptr, tune);
-			check(hipGetLastError(), "render launch");
-			check(hipEventRecord(ke.back(), st), "hipEventRecord");
+		};
+		// samples [q.sampleBegin, + q.sampleCount) of `active` pixels (`list`, or all) into the sums
+		auto renderRange = [&](const GpuParams& q, const std::uint32_t* list, std::uint32_t active, double* moments) {
+			if (integrator == LUZGPU_INTEGRATOR_MEGAKERNEL)
+			{
+				// samples per launch: bounded so the per-sample colour buffer stays within 4 GiB
+				const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(q.sampleCount,
+					static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(active) * 12ull))));
+				ctx->sampleColors.reserve(static_cast<size_t>(active) * perLaunch * 3 * sizeof(float));
+				for (std::uint32_t done = 0; done < q.sampleCount; done += perLaunch)
+				{
+					markStart();
+					GpuParams r = q;
+					r.sampleBegin = q.sampleBegin + done;
+					r.sampleCount = std::min(perLaunch, q.sampleCount - done);
+					const size_t threads = static_cast<size_t>(active) * r.sampleCount;
+					launchMega(settings && settings->count_visits, scene->gpu.host.features, blocks(threads, kMegaBlock), st, scene->gpu.view, r,
+						list, active, ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>());
+					check(hipGetLastError(), "render launch");
+					check(hipEventRecord(ke.back(), st), "hipEventRecord");
+					accumulate<<<blocks(active, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sums.as<double>(), list, active,
+						r.sampleCount, moments);
+				}
+			}
+			else
+			{
+				markStart();
+				BatchJob job{q, ctx->sums.as<double>()};
+				job.pixelList = list;
+				job.listCount = active;
+				job.moments = moments;
+				renderWavefront(ctx->wave, scene->gpu.view, scene->gpu.host.features, {job}, ctx->counters.as<unsigned long long>(), st,
+					settings && settings->count_visits, settings && settings->stage_times ? &stages : nullptr, tune);
+				check(hipGetLastError(), "render launch");
+				check(hipEventRecord(ke.back(), st), "hipEventRecord");
+			}
+		};
+		check(hipEventRecord(e0, st), "hipEventRecord");
+		std::uint64_t rendered = static_cast<std::uint64_t>(pixels) * p.sampleCount;
+		AdaptiveArrays A{};
+		if (adaptive)
+		{
+			A = adaptiveArrays(ctx, pixels, 1);
+			std::vector<AdaptiveJob> aj(1);
+			aj[0].rule = rule;
+			aj[0].pixels = pixels;
+			aj[0].offset = 0;
+			rendered = runAdaptive(ctx, aj, A, [&](const std::vector<AdaptiveRound>& work) {
+				const AdaptiveRound& w = work[0];
+				GpuParams q = p;
+				q.sampleBegin = w.begin;
+				q.sampleCount = w.count;
+				renderRange(q, w.list, w.active, w.moments);
+			});
+		}
+		else
+		{
+			renderRange(p, nullptr, pixels, nullptr);
 		}
 		check(hipEventRecord(e1, st), "hipEventRecord");
 		float* dst = rgb_out;
@@ -1078,7 +1316,8 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			ctx->output.reserve(static_cast<size_t>(pixels) * 3 * sizeof(float));
 			dst = ctx->output.as<float>();
 		}
-		resolve<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), dst, pixels, static_cast<double>(p.sampleCount));
+		if (adaptive) resolve_used<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), A.used, dst, pixels);
+		else resolve<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), dst, pixels, static_cast<double>(p.sampleCount));
 		check(hipGetLastError(), "resolve launch");
 		unsigned long long counters[6] = {0, 0, 0, 0, 0, 0};
 		check(hipMemcpyAsync(counters, ctx->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, st), "counter readback");
@@ -1100,9 +1339,12 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		}
 		(void)hipEventDestroy(e0);
 		(void)hipEventDestroy(e1);
+		ctx->countsPixels = pixels;
+		ctx->countsUniform = adaptive ? 0u : p.sampleCount;
+		ctx->countsDevice = adaptive ? A.used : nullptr;
 		if (stats)
 		{
-			stats->rendered_samples = static_cast<std::uint64_t>(pixels) * p.sampleCount;
+			stats->rendered_samples = rendered;
 			stats->closest_hit_rays = counters[0];
 			stats->shadow_rays = counters[1];
 			stats->render_ms = ms;
@@ -1142,8 +1384,11 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 			throw Error{LUZGPU_E_INVALID, "luzgpu_render_denoised: negative max_bounces"};
 		}
 		const NforParams P = nforParams(p.width, p.height, nfor);
+		AdaptiveRule rule{};
+		const bool adaptive = adaptiveRule(scene->gpu.host, p, settings, rule);
 		hipStream_t st = ctx->stream;
 		const std::uint32_t pixels = p.width * p.height;
+		ctx->countsPixels = 0;
 		const NforLayout L(pixels);
 		ctx->denoiseStats.reserve(static_cast<size_t>(pixels) * kDenoiseStats * sizeof(double));
 		ctx->nforBuffers.reserve((L.total + 3 * static_cast<size_t>(pixels)) * sizeof(double));
@@ -1156,29 +1401,51 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 		check(hipEventCreate(&e1), "hipEventCreate");
 		check(hipEventCreate(&e2), "hipEventCreate");
 		check(hipEventRecord(e0, st), "hipEventRecord");
-		// samples per launch: per-sample colour + feature buffers within 4 GiB
-		const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount,
-			static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(pixels) * (12ull + 4ull * kNforFeatures)))));
-		ctx->sampleColors.reserve(static_cast<size_t>(pixels) * perLaunch * 3 * sizeof(float));
-		ctx->sampleFeatures.reserve(static_cast<size_t>(pixels) * perLaunch * kNforFeatures * sizeof(float));
-		for (std::uint32_t done = 0; done < p.sampleCount; done += perLaunch)
+		// samples [begin, begin + count) of `active` pixels (`list`, or all) into the statistics
+		auto renderRange = [&](std::uint32_t begin, std::uint32_t count, const std::uint32_t* list, std::uint32_t active, double* moments) {
+			// samples per launch: per-sample colour + feature buffers within 4 GiB
+			const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(count,
+				static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(active) * (12ull + 4ull * kNforFeatures)))));
+			ctx->sampleColors.reserve(static_cast<size_t>(active) * perLaunch * 3 * sizeof(float));
+			ctx->sampleFeatures.reserve(static_cast<size_t>(active) * perLaunch * kNforFeatures * sizeof(float));
+			for (std::uint32_t done = 0; done < count; done += perLaunch)
+			{
+				GpuParams q = p;
+				q.sampleBegin = begin + done;
+				q.sampleCount = std::min(perLaunch, count - done);
+				const size_t threads = static_cast<size_t>(active) * q.sampleCount;
+				launchMegaDenoise(scene->gpu.host.features, blocks(threads, kMegaBlock), st, scene->gpu.view, q, list, active,
+					ctx->sampleColors.as<float>(), ctx->sampleFeatures.as<float>(), ctx->counters.as<unsigned long long>());
+				check(hipGetLastError(), "denoise render launch");
+				denoise_accumulate<<<blocks(active, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sampleFeatures.as<float>(),
+					ctx->denoiseStats.as<double>(), list, active, q.sampleCount, q.sampleBegin, moments);
+			}
+		};
+		std::uint64_t rendered = static_cast<std::uint64_t>(pixels) * p.sampleCount;
+		AdaptiveArrays A{};
+		if (adaptive)
 		{
-			GpuParams q = p;
-			q.sampleBegin = done;
-			q.sampleCount = std::min(perLaunch, p.sampleCount - done);
-			const size_t threads = static_cast<size_t>(pixels) * q.sampleCount;
-			launchMegaDenoise(scene->gpu.host.features, blocks(threads, kMegaBlock), st, scene->gpu.view, q, ctx->sampleColors.as<float>(),
-				ctx->sampleFeatures.as<float>(), ctx->counters.as<unsigned long long>());
-			check(hipGetLastError(), "denoise render launch");
-			denoise_accumulate<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sampleFeatures.as<float>(),
-				ctx->denoiseStats.as<double>(), pixels, q.sampleCount, done);
+			A = adaptiveArrays(ctx, pixels, 1);
+			std::vector<AdaptiveJob> aj(1);
+			aj[0].rule = rule;
+			aj[0].pixels = pixels;
+			aj[0].offset = 0;
+			rendered = runAdaptive(ctx, aj, A, [&](const std::vector<AdaptiveRound>& work) {
+				const AdaptiveRound& w = work[0];
+				renderRange(w.begin, w.count, w.list, w.active, w.moments);
+			});
+		}
+		else
+		{
+			renderRange(0u, p.sampleCount, nullptr, pixels, nullptr);
 		}
 		double* nb = ctx->nforBuffers.as<double>();
 		ctx->output.reserve(static_cast<size_t>(pixels) * 3 * sizeof(float) * 2);
 		float* image = ctx->output.as<float>();
 		float* denoised = image + static_cast<size_t>(pixels) * 3;
-		denoise_finalize<<<blocks(pixels, 256), 256, 0, st>>>(ctx->denoiseStats.as<double>(), pixels, p.sampleCount, image, nb + L.mean,
-			nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB, nb + L.featVariance);
+		denoise_finalize<<<blocks(pixels, 256), 256, 0, st>>>(ctx->denoiseStats.as<double>(), pixels, p.sampleCount,
+			adaptive ? A.used : nullptr, image, nb + L.mean, nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB,
+			nb + L.featVariance);
 		check(hipEventRecord(e1, st), "hipEventRecord");
 		double* out = nb + L.total;
 		runNfor(ctx, P, nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB, nb + L.featVariance, out);
@@ -1212,10 +1479,13 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 		(void)hipEventDestroy(e0);
 		(void)hipEventDestroy(e1);
 		(void)hipEventDestroy(e2);
+		ctx->countsPixels = pixels;
+		ctx->countsUniform = adaptive ? 0u : p.sampleCount;
+		ctx->countsDevice = adaptive ? A.used : nullptr;
 		if (stats)
 		{
 			*stats = luzgpu_stats{};
-			stats->rendered_samples = static_cast<std::uint64_t>(pixels) * p.sampleCount;
+			stats->rendered_samples = rendered;
 			stats->closest_hit_rays = counters[0];
 			stats->shadow_rays = counters[1];
 			stats->render_ms = renderMs;
@@ -1351,6 +1621,11 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 		std::vector<BatchJob> jobs(n);
 		std::vector<size_t> sumOffset(n);
 		size_t sumTotal = 0;
+		// adaptive rounds for the whole batch when any scene stops pixels early; the others
+		// then take all their samples in their first round
+		std::vector<AdaptiveJob> aj(n);
+		bool adaptive = false;
+		size_t pixelTotal = 0;
 		for (uint32_t i = 0; i < n; i++)
 		{
 			const luzgpu_settings* si = settings ? settings + i : nullptr;
@@ -1366,6 +1641,20 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 			jobs[i].params = p;
 			sumOffset[i] = sumTotal;
 			sumTotal += static_cast<size_t>(p.width) * p.height * 3;
+			AdaptiveRule rule{};
+			if (adaptiveRule(scenes[i]->gpu.host, p, si, rule))
+			{
+				adaptive = true;
+			}
+			else
+			{
+				rule = AdaptiveRule{p.sampleCount, p.sampleCount, 1u, p.sampleCount, 1.0};
+				aj[i].first = p.sampleBegin;
+			}
+			aj[i].rule = rule;
+			aj[i].pixels = p.width * p.height;
+			aj[i].offset = pixelTotal;
+			pixelTotal += aj[i].pixels;
 		}
 		ctx->batchSums.reserve(sumTotal * sizeof(double));
 		ctx->counters.reserve(8 * sizeof(unsigned long long));
@@ -1389,8 +1678,33 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 		const bool timed = settings && settings[0].stage_times;
 		StageTimes stages;
 		check(hipEventRecord(e0, st), "hipEventRecord");
-		renderWavefront(ctx->wave, ctx->batch->view, ctx->batch->host.features, jobs, ctx->counters.as<unsigned long long>(), st,
-			settings && settings[0].count_visits, timed ? &stages : nullptr, tune);
+		std::uint64_t rendered = 0;
+		for (const BatchJob& j : jobs) rendered += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
+		AdaptiveArrays A{};
+		if (adaptive)
+		{
+			A = adaptiveArrays(ctx, pixelTotal, n);
+			rendered = runAdaptive(ctx, aj, A, [&](const std::vector<AdaptiveRound>& work) {
+				std::vector<BatchJob> round;
+				for (const AdaptiveRound& w : work)
+				{
+					BatchJob b = jobs[w.job];
+					b.params.sampleBegin = w.begin;
+					b.params.sampleCount = w.count;
+					b.pixelList = w.list;
+					b.listCount = w.active;
+					b.moments = w.moments;
+					round.push_back(b);
+				}
+				renderWavefront(ctx->wave, ctx->batch->view, ctx->batch->host.features, round, ctx->counters.as<unsigned long long>(), st,
+					settings && settings[0].count_visits, timed ? &stages : nullptr, tune);
+			});
+		}
+		else
+		{
+			renderWavefront(ctx->wave, ctx->batch->view, ctx->batch->host.features, jobs, ctx->counters.as<unsigned long long>(), st,
+				settings && settings[0].count_visits, timed ? &stages : nullptr, tune);
+		}
 		check(hipGetLastError(), "batch render launch");
 		check(hipEventRecord(e1, st), "hipEventRecord");
 		// per-scene resolve, into device outputs directly or through the staging buffer
@@ -1412,7 +1726,8 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 				copies.emplace_back(rgb_out[i], staged);
 				staged += static_cast<size_t>(pixels) * 3 * sizeof(float);
 			}
-			resolve<<<blocks(pixels, 256), 256, 0, st>>>(jobs[i].sums, dst, pixels, static_cast<double>(jobs[i].params.sampleCount));
+			if (adaptive) resolve_used<<<blocks(pixels, 256), 256, 0, st>>>(jobs[i].sums, A.used + aj[i].offset, dst, pixels);
+			else resolve<<<blocks(pixels, 256), 256, 0, st>>>(jobs[i].sums, dst, pixels, static_cast<double>(jobs[i].params.sampleCount));
 		}
 		check(hipGetLastError(), "resolve launch");
 		unsigned long long counters[6] = {0, 0, 0, 0, 0, 0};
@@ -1430,12 +1745,11 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 		check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
 		(void)hipEventDestroy(e0);
 		(void)hipEventDestroy(e1);
+		if (adaptive) ctx->countsPixels = 0;	// the per-pixel counts of the last single render are gone
 		if (stats)
 		{
-			std::uint64_t samples = 0;
-			for (const BatchJob& j : jobs) samples += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
 			*stats = luzgpu_stats{};
-			stats->rendered_samples = samples;
+			stats->rendered_samples = rendered;
 			stats->closest_hit_rays = counters[0];
 			stats->shadow_rays = counters[1];
 			stats->render_ms = ms;
@@ -1574,6 +1888,34 @@ int	luzgpu_scene_set_materials(luzgpu_scene* scene, uint32_t count, const double
 		// same-sized uploads keep their buffers, so a captured pass graph (keyed on the
 		// scene's device pointers) stays valid; merged batch arrays hold copies: rebuild them
 		ctx->batchKey.clear();
+	});
+}
+
+int	luzgpu_sample_counts(luzgpu_ctx* ctx, uint32_t* counts_out, size_t count)
+{
+	if (!ctx || !counts_out)
+	{
+		g_lastError = "luzgpu_sample_counts: NULL argument";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		if (ctx->countsPixels == 0u)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_sample_counts: no single-scene render on this context since its last adaptive batch"};
+		}
+		if (count != ctx->countsPixels)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_sample_counts: count must be the last render's W*H = " + std::to_string(ctx->countsPixels)};
+		}
+		if (ctx->countsUniform != 0u)
+		{
+			std::fill(counts_out, counts_out + count, ctx->countsUniform);
+			return;
+		}
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		check(hipMemcpyAsync(counts_out, ctx->countsDevice, count * sizeof(std::uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+			"sample count readback");
+		check(hipStreamSynchronize(ctx->stream), "sample count readback");
 	});
 }
 

@@ -23,6 +23,7 @@ class Context:
     def __init__(self, device: int = 0):
         self._lib = _lib.load()
         self._h = ctypes.c_void_p()
+        self._last_resolution = None  # (W, H) of the last single-scene render
         _lib.check(self._lib.luzgpu_create(device, ctypes.byref(self._h)))
 
     @property
@@ -200,8 +201,21 @@ class SceneFile:
 
 
 def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=False, resolution=None,
-              count_visits=False, stage_times=False, tuning=None):
+              count_visits=False, stage_times=False, tuning=None, adaptive=None):
     s = _lib.Settings()
+    # adaptive: None = the scene's adaptive= setting, False / True = --no-adaptive / --adaptive,
+    # a dict {"min_samples", "check_interval", "threshold"} = on with those values
+    if adaptive is False:
+        s.adaptive = _lib.ADAPTIVE_OFF
+    elif adaptive is True:
+        s.adaptive = _lib.ADAPTIVE_ON
+    elif isinstance(adaptive, dict):
+        s.adaptive = _lib.ADAPTIVE_ON
+        s.adaptive_min_samples = int(adaptive.get("min_samples", 0))
+        s.adaptive_check_interval = int(adaptive.get("check_interval", 0))
+        s.adaptive_threshold = float(adaptive.get("threshold", 0.0))
+    elif adaptive is not None:
+        raise ValueError("adaptive: None, a bool, or a dict of min_samples / check_interval / threshold")
     if tuning == "auto":
         s.autotune = 1
     elif tuning:
@@ -230,14 +244,18 @@ class Renderer:
     @staticmethod
     def render(scene: Scene, seed: int, samples=None, max_bounces=None, sample_begin=0, sample_count=None,
                integrator: str = "wavefront", out=None, resolution=None, count_visits=False, stage_times=False,
-               tuning=None):
+               tuning=None, adaptive=None):
         """Returns (image[H, W, 3] float32, RenderStats). ``out`` may be a device tensor
         (torch, float32, contiguous, H*W*3) that receives the image without a host copy.
         ``tuning`` ({"grid", "extend_refill", "shadow_refill", "fetch_chunk", "lds_stack"}) sets
         the wavefront launch shape, or "auto" uses the context's autotuned one (tuning it
-        first if needed); it never changes the image."""
+        first if needed); it never changes the image. ``adaptive``: None follows the scene
+        file, False / True turn per-pixel stopping off / on, a dict {"min_samples",
+        "check_interval", "threshold"} turns it on with those values;
+        ``Renderer.sample_counts`` then gives each pixel's sample count."""
         s = _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=out is not None,
-                      resolution=resolution, count_visits=count_visits, stage_times=stage_times, tuning=tuning)
+                      resolution=resolution, count_visits=count_visits, stage_times=stage_times, tuning=tuning,
+                      adaptive=adaptive)
         st = _lib.Stats()
         w, h = resolution if resolution is not None else (scene.width, scene.height)
         if out is not None:
@@ -248,16 +266,17 @@ class Renderer:
             ptr = img.ctypes.data
         _lib.check(scene.ctx._lib.luzgpu_render(scene.ctx.handle, scene._h, ctypes.byref(s), ctypes.c_void_p(ptr),
                                                 ctypes.byref(st)))
+        scene.ctx._last_resolution = (w, h)
         return img, RenderStats.of(st)
 
     @staticmethod
     def render_batch(scenes, seeds, samples=None, max_bounces=None, outs=None, resolutions=None, count_visits=False,
-                     stage_times=False, tuning=None):
+                     stage_times=False, tuning=None, adaptive=None):
         """Renders several scenes in one multi-scene launch sequence (luzgpu_render_batch).
 
-        ``seeds``, ``samples``, ``max_bounces`` and ``resolutions`` are per-scene lists (or
-        None for the scene defaults); ``outs`` optional per-scene device tensors. Returns
-        (images, aggregate RenderStats)."""
+        ``seeds``, ``samples``, ``max_bounces``, ``resolutions`` and ``adaptive`` are
+        per-scene lists (or None for the scene defaults); ``outs`` optional per-scene device
+        tensors. Returns (images, aggregate RenderStats)."""
         n = len(scenes)
         if n == 0:
             raise ValueError("render_batch needs at least one scene")
@@ -268,7 +287,8 @@ class Renderer:
             res = resolutions[i] if resolutions else None
             arr[i] = _settings(seeds[i], samples[i] if samples else None, max_bounces[i] if max_bounces else None, 0, None,
                                "wavefront", on_device=outs is not None, resolution=res,
-                               count_visits=count_visits, stage_times=stage_times, tuning=tuning)
+                               count_visits=count_visits, stage_times=stage_times, tuning=tuning,
+                               adaptive=adaptive[i] if adaptive else None)
             w, h = res if res is not None else (sc.width, sc.height)
             if outs is not None:
                 images.append(outs[i])
@@ -315,11 +335,12 @@ class Renderer:
 
     @staticmethod
     def render_denoised(scene: Scene, seed: int, samples=None, max_bounces=None, resolution=None, nfor=None,
-                        buffers=False):
+                        buffers=False, adaptive=None):
         """Render with denoising on (luzgpu_render_denoised): returns (raw image, denoised image,
         RenderStats, NFOR buffers or None). ``nfor``: {"feature_prefilter_radius",
-        "regression_radius", "patch_radius", "min_variance"} (defaults as NFORSettings)."""
-        s = _settings(seed, samples, max_bounces, 0, None, "megakernel", resolution=resolution)
+        "regression_radius", "patch_radius", "min_variance"} (defaults as NFORSettings);
+        ``adaptive`` as for ``render``."""
+        s = _settings(seed, samples, max_bounces, 0, None, "megakernel", resolution=resolution, adaptive=adaptive)
         w, h = resolution if resolution is not None else (scene.width, scene.height)
         img = np.zeros((h, w, 3), dtype=np.float32)
         den = np.zeros((h, w, 3), dtype=np.float32)
@@ -337,6 +358,7 @@ class Renderer:
                                                          ctypes.byref(ns) if ns is not None else None, img.ctypes.data,
                                                          den.ctypes.data, ctypes.byref(bp) if bp is not None else None,
                                                          ctypes.byref(st)))
+        scene.ctx._last_resolution = (w, h)
         return img, den, RenderStats.of(st), out
 
     @staticmethod
@@ -356,6 +378,16 @@ class Renderer:
         _lib.check(ctx._lib.luzgpu_nfor(ctx.handle, ctypes.byref(bp), ctypes.byref(ns) if ns is not None else None,
                                         out.ctypes.data, ctypes.byref(ms)))
         return out, ms.value
+
+    @staticmethod
+    def sample_counts(ctx) -> np.ndarray:
+        """Per-pixel sample counts (H, W) of the context's last render / render_denoised."""
+        if ctx._last_resolution is None:
+            raise RuntimeError("sample_counts: no render on this context yet")
+        w, h = ctx._last_resolution
+        out = np.zeros((h, w), dtype=np.uint32)
+        _lib.check(ctx._lib.luzgpu_sample_counts(ctx.handle, out.ctypes.data, w * h))
+        return out
 
     @staticmethod
     def primary_hits(scene: Scene, seed: int, resolution=None):

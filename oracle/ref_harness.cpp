@@ -229,12 +229,21 @@ namespace
 		scene.getImage()->setHeight(std::stoul(argv[8]));
 		scene.getImage()->initialize();
 		scene.setDenoise(true);
+		const bool adaptive = argc >= 15;	// optional MIN INTERVAL THRESHOLD: adaptive sampling on
+		if (adaptive)
+		{
+			scene.setAdaptiveSampling(true);
+			scene.setAdaptiveMinSamples(std::stoi(argv[12]));
+			scene.setAdaptiveCheckInterval(std::stoi(argv[13]));
+			scene.setAdaptiveThreshold(std::stod(argv[14]));
+		}
 		scene.updateLights();
 		scene.updateAccelerationStructure();
 		const std::size_t w = scene.getImage()->getWidth(), h = scene.getImage()->getHeight();
 		const Renderer::internal::RenderCamera camera = Renderer::internal::_prepareRenderCamera(scene);
 		Sampler::setRenderSeed(static_cast<std::uint32_t>(randomSeedValue()));
 		scene.initializeDenoiseBuffers(w, h);
+		std::vector<std::uint32_t> counts(w * h, 0);
 		std::atomic<std::size_t> next(0);
 		std::vector<std::thread> pool;
 		for (std::size_t t = 0; t < std::max<std::size_t>(1, threads); t++)
@@ -242,7 +251,7 @@ namespace
 			pool.emplace_back([&]() {
 				for (std::size_t i = next++; i < w * h; i = next++)
 				{
-					Renderer::internal::_threadRender(scene, camera, i % w, i / w);
+					counts[i] = Renderer::internal::_threadRender(scene, camera, i % w, i / w);
 				}
 			});
 		}
@@ -270,6 +279,63 @@ namespace
 			}
 		}
 		for (std::size_t i = 0; i < w * h; i++) putColor(denoised->pixels()[i]);
+		if (adaptive)
+		{
+			for (std::uint32_t c : counts) put(out, c);
+		}
+		return (0);
+	}
+
+	// adaptive SCENE SEED THREADS OUT SPP W H MIN INTERVAL THRESHOLD: the per-pixel loop of
+	// _threadRender with adaptive sampling on and denoising off; writes u32 W, H, then W*H
+	// u32 sample counts (its return values) and the W*H*3 f64 image.
+	int	cmdAdaptive(int argc, char** argv)
+	{
+		if (argc < 11)
+		{
+			return (2);
+		}
+		Scene scene;
+		loadScene(scene, argv[2]);
+		setRandomSeed(std::stoi(argv[3]));
+		const std::size_t threads = static_cast<std::size_t>(std::stoul(argv[4]));
+		scene.setSampleCount(std::stoi(argv[6]));
+		scene.getImage()->setWidth(std::stoul(argv[7]));
+		scene.getImage()->setHeight(std::stoul(argv[8]));
+		scene.getImage()->initialize();
+		scene.setAdaptiveSampling(true);
+		scene.setAdaptiveMinSamples(std::stoi(argv[9]));
+		scene.setAdaptiveCheckInterval(std::stoi(argv[10]));
+		scene.setAdaptiveThreshold(argc >= 12 ? std::stod(argv[11]) : 0.03);
+		scene.updateLights();
+		scene.updateAccelerationStructure();
+		const std::size_t w = scene.getImage()->getWidth(), h = scene.getImage()->getHeight();
+		const Renderer::internal::RenderCamera camera = Renderer::internal::_prepareRenderCamera(scene);
+		Sampler::setRenderSeed(static_cast<std::uint32_t>(randomSeedValue()));
+		std::vector<std::uint32_t> counts(w * h, 0);
+		std::atomic<std::size_t> next(0);
+		std::vector<std::thread> pool;
+		for (std::size_t t = 0; t < std::max<std::size_t>(1, threads); t++)
+		{
+			pool.emplace_back([&]() {
+				for (std::size_t i = next++; i < w * h; i = next++)
+				{
+					counts[i] = Renderer::internal::_threadRender(scene, camera, i % w, i / w);
+				}
+			});
+		}
+		for (std::thread& th : pool) th.join();
+		std::ofstream out(argv[5], std::ios::binary);
+		put(out, static_cast<std::uint32_t>(w));
+		put(out, static_cast<std::uint32_t>(h));
+		for (std::uint32_t c : counts) put(out, c);
+		for (std::size_t i = 0; i < w * h; i++)
+		{
+			const Color c = scene.getImage()->pixels()[i];
+			put(out, c.getRed());
+			put(out, c.getGreen());
+			put(out, c.getBlue());
+		}
 		return (0);
 	}
 
@@ -643,7 +709,7 @@ int	main(int argc, char** argv)
 {
 	if (argc < 2)
 	{
-		std::cerr << "usage: luzref render|seeds|denoise|sampler|blas|tlas|primary|env|lights ..." << std::endl;
+		std::cerr << "usage: luzref render|seeds|denoise|adaptive|sampler|blas|tlas|primary|env|lights ..." << std::endl;
 		return (2);
 	}
 	try
@@ -658,6 +724,7 @@ int	main(int argc, char** argv)
 		if (cmd == "lights") return (cmdLights(argc, argv));
 		if (cmd == "seeds") return (cmdSeeds(argc, argv));
 		if (cmd == "denoise") return (cmdDenoise(argc, argv));
+		if (cmd == "adaptive") return (cmdAdaptive(argc, argv));
 	}
 	catch (const std::exception& e)
 	{

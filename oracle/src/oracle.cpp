@@ -2434,6 +2434,137 @@ int	oracle_render(const char* path, std::uint32_t seed, std::uint32_t spp, std::
 	}
 }
 
+namespace oracle
+{
+	// AdaptiveAccumulator (Threads.cpp:31-53)
+	struct	AdaptiveAcc
+	{
+		C	sum, square;
+		double	lumSum = 0.0, lumSquare = 0.0, maxLum = 0.0;
+
+		void	add(const C& c)
+		{
+			// sampleLuminance (Threads.cpp:139-148)
+			double lum = luminance(c);
+			if (!std::isfinite(lum)) lum = 0.0;
+			sum = sum + c;
+			square = square + C{c.r * c.r, c.g * c.g, c.b * c.b};
+			lumSum += lum;
+			lumSquare += lum * lum;
+			maxLum = std::max(maxLum, lum);
+		}
+	};
+
+	// confidenceInterval95 over sampleMeanVariance (Threads.cpp:150-172)
+	double	ci95(double sum, double square, unsigned count)
+	{
+		double mv = 0.0;
+		if (count > 1)
+		{
+			const double n = static_cast<double>(count);
+			mv = std::max(0.0, ((square - (sum * sum / n)) / (n - 1.0)) / n);
+		}
+		if (!std::isfinite(mv)) return std::numeric_limits<double>::infinity();
+		return 1.96 * std::sqrt(mv);
+	}
+
+	// channelConverged (Threads.cpp:174-180)
+	bool	channelOk(double mean, double ci, double threshold)
+	{
+		return ci <= threshold * std::max(std::fabs(mean), 0.02);
+	}
+
+	struct	AdaptiveRuleC
+	{
+		unsigned	maxSamples, minSamples, checkInterval, darkMinimum;
+		double	threshold;
+	};
+
+	// adaptiveSampleConverged (Threads.cpp:194-262)
+	bool	adaptiveStop(const AdaptiveRuleC& r, unsigned used, const AdaptiveAcc& a)
+	{
+		if (used >= r.maxSamples) return true;
+		if (used < r.minSamples) return false;
+		if ((used - r.minSamples) % r.checkInterval != 0) return false;
+		if (used <= 1) return false;
+		const double n = static_cast<double>(used);
+		if (a.maxLum < 0.02 && used < r.darkMinimum) return false;
+		return channelOk(a.lumSum / n, ci95(a.lumSum, a.lumSquare, used), r.threshold)
+			&& channelOk(a.sum.r / n, ci95(a.sum.r, a.square.r, used), r.threshold)
+			&& channelOk(a.sum.g / n, ci95(a.sum.g, a.square.g, used), r.threshold)
+			&& channelOk(a.sum.b / n, ci95(a.sum.b, a.square.b, used), r.threshold);
+	}
+}
+
+// FP64 render with adaptive per-pixel stopping: _threadRender's loop (Threads.cpp:551-583)
+// with the rule of adaptiveSampleConverged. min_samples / check_interval / threshold: 0 (or
+// <= 0) takes the scene file's value. out: H*W*3 doubles, counts: H*W samples per pixel.
+// Returns 1 without rendering when the rule cannot stop a pixel (adaptiveSamplingCanStop,
+// Threads.cpp:265-271, false).
+int	oracle_render_adaptive(const char* path, std::uint32_t seed, std::uint32_t spp, std::uint32_t width, std::uint32_t height,
+	std::uint32_t max_bounces_plus1, std::uint32_t threads, std::uint32_t min_samples, std::uint32_t check_interval,
+	double threshold, double* out, std::uint32_t* counts)
+{
+	try
+	{
+		const luzgpu::Scene in = luzgpu::readSceneFile(path);
+		oracle::Scene s = oracle::build(in, width, height);
+		if (max_bounces_plus1 > 0) s.maxBounces = static_cast<int>(max_bounces_plus1) - 1;
+		const unsigned total = spp > 0 ? spp : in.settings.samples;
+		oracle::AdaptiveRuleC rule{};
+		rule.maxSamples = total;
+		rule.minSamples = std::min(min_samples ? min_samples : in.settings.adaptiveMinSamples, total);
+		rule.checkInterval = check_interval ? check_interval : in.settings.adaptiveCheckInterval;
+		rule.threshold = threshold > 0.0 ? threshold : in.settings.adaptiveThreshold;
+		if ((min_samples ? min_samples : in.settings.adaptiveMinSamples) >= total) return 1;
+		// darkMinimumSamples (Threads.cpp:183-192)
+		rule.darkMinimum = std::min(total, std::max(rule.minSamples, std::max(256u, rule.minSamples + 3u * rule.checkInterval)));
+		const std::size_t W = s.width, H = s.height, N = W * H;
+		std::atomic<std::size_t> next(0);
+		std::vector<std::thread> pool;
+		for (std::uint32_t t = 0; t < std::max(1u, threads); t++)
+		{
+			pool.emplace_back([&]() {
+				while (true)
+				{
+					const std::size_t start = next.fetch_add(16);
+					if (start >= N) break;
+					for (std::size_t i = start; i < std::min(start + 16, N); i++)
+					{
+						const std::size_t x = i % W, y = i / W;
+						oracle::C sum;
+						oracle::AdaptiveAcc acc;
+						unsigned used = 0;
+						for (unsigned k = 0; k < total; k++)
+						{
+							oracle::Sampler ps;
+							ps.begin(seed, x, y, k, 0);
+							const oracle::Ray r = oracle::cameraRay(s, x, y, ps);
+							const oracle::C c = oracle::clean(oracle::trace(s, r, ps));
+							sum = sum + c;
+							used = k + 1;
+							acc.add(c);
+							if (oracle::adaptiveStop(rule, used, acc)) break;
+						}
+						const oracle::C mean = oracle::clean(sum / static_cast<double>(used));
+						out[3 * i] = mean.r;
+						out[3 * i + 1] = mean.g;
+						out[3 * i + 2] = mean.b;
+						counts[i] = used;
+					}
+				}
+			});
+		}
+		for (auto& t : pool) t.join();
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		g_oracleError = e.what();
+		return -1;
+	}
+}
+
 // Path-replay adjoint of oracle_render for dLoss/dImage = dldi (W*H*3): FP64 gradients per
 // material (7 each: albedo rgb, GGX roughness, emission rgb).
 int	oracle_adjoint(const char* path, std::uint32_t seed, std::uint32_t spp, std::uint32_t width, std::uint32_t height,

@@ -129,31 +129,70 @@ NFOR_CASES = [
     # (name, workload writer, spp, width, height, (prefilter, regression, patch) radii)
     ("nfor_cornell_48x48_8spp", "cornell", 8, 48, 48, (1, 2, 1)),
     ("nfor_primitives_64x48_8spp_r9", "primitives", 8, 64, 48, (1, 9, 1)),
+    # adaptive sampling on: (min samples, check interval, threshold)
+    ("nfor_adaptive_cornell_48x48_48spp", "cornell", 48, 48, 48, (1, 2, 1), (8, 4, 0.2)),
 ]
 
 
 def nfor_fixtures(tmp):
     """The reference's denoise pipeline: its NFOR input buffers (half-buffer means and
     variances, primary features) and Denoise::applyNFOR's output for them, plus the raw
-    mean image, at seed 1 (oracle/ref_harness.cpp, denoise command)."""
+    mean image, at seed 1 (oracle/ref_harness.cpp, denoise command); with adaptive
+    sampling on (min samples, check interval, threshold given) also each pixel's sample
+    count."""
     writers = {"cornell": workloads.write_cornell, "primitives": workloads.write_primitives}
-    for name, scene, spp, w, h, radii in NFOR_CASES:
+    for name, scene, spp, w, h, radii, *rule in NFOR_CASES:
         wl = writers[scene](os.path.join(tmp, f"n_{scene}"))
         out = os.path.join(tmp, f"{name}.bin")
-        run("denoise", wl.scene_path, 1, os.cpu_count() or 8, out, spp, w, h, *radii)
+        run("denoise", wl.scene_path, 1, os.cpu_count() or 8, out, spp, w, h, *radii,
+            *([rule[0][0], rule[0][1], repr(rule[0][2])] if rule else []))
         head = np.fromfile(out, dtype="<u4", count=2)
         assert head[0] == w and head[1] == h
-        data = np.fromfile(out, dtype="<f8", offset=8)
         n = w * h
+        nf = n * (3 + 3 + 3 + 1 + 11 + 11 + 11 + 3)
+        data = np.fromfile(out, dtype="<f8", count=nf, offset=8)
         fields, off = {}, 0
         for k, c in (("image", 3), ("color_a", 3), ("color_b", 3), ("color_variance", 1), ("features_a", 11),
                      ("features_b", 11), ("feature_variance", 11), ("denoised", 3)):
             fields[k] = data[off:off + n * c].reshape(h, w, c)
             off += n * c
-        assert off == data.size
+        extra = {}
+        if rule:
+            counts = np.fromfile(out, dtype="<u4", offset=8 + 8 * nf).reshape(h, w)
+            extra = dict(counts=counts.astype(np.uint16), min_samples=rule[0][0], check_interval=rule[0][1],
+                         threshold=rule[0][2])
+        else:
+            assert os.path.getsize(out) == 8 + 8 * nf
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), seed=1, spp=spp, scene=scene,
-                            radii=np.array(radii), **fields)
+                            radii=np.array(radii), **fields, **extra)
         print(name, {k: float(np.abs(v).mean()) for k, v in fields.items()})
+
+
+ADAPTIVE_CASES = [
+    # (name, workload writer, spp, width, height, min samples, check interval, threshold)
+    ("adaptive_primitives_64x48_64spp", "primitives", 64, 64, 48, 8, 4, 0.1),
+    ("adaptive_cornell_64x48_96spp", "cornell", 96, 64, 48, 16, 8, 0.2),
+    # dark pixels (outside the box) stop only from darkMinimumSamples = 256 on
+    ("adaptive_cornell_dark_40x30_288spp", "cornell", 288, 40, 30, 16, 16, 0.2),
+]
+
+
+def adaptive_fixtures(tmp):
+    """The reference's per-pixel loop with adaptive sampling on (denoising off) at seed 1:
+    each pixel's sample count (_threadRender's return value) and the image
+    (oracle/ref_harness.cpp, adaptive command)."""
+    writers = {"cornell": workloads.write_cornell, "primitives": workloads.write_primitives}
+    for name, scene, spp, w, h, mn, interval, threshold in ADAPTIVE_CASES:
+        wl = writers[scene](os.path.join(tmp, f"a_{scene}"))
+        out = os.path.join(tmp, f"{name}.bin")
+        run("adaptive", wl.scene_path, 1, os.cpu_count() or 8, out, spp, w, h, mn, interval, repr(threshold))
+        head = np.fromfile(out, dtype="<u4", count=2)
+        assert head[0] == w and head[1] == h
+        counts = np.fromfile(out, dtype="<u4", count=w * h, offset=8).reshape(h, w)
+        image = np.fromfile(out, dtype="<f8", offset=8 + 4 * w * h).reshape(h, w, 3)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), seed=1, spp=spp, scene=scene, min_samples=mn,
+                            check_interval=interval, threshold=threshold, counts=counts.astype(np.uint16), image=image)
+        print(name, "counts", np.unique(counts, return_counts=True))
 
 
 def fd_fixture(tmp, name="diff_fd_128_8spp_depth2", width=128, height=128, spp=8, depth=2, metals=False, h_albedo=0.01,
@@ -224,6 +263,8 @@ def main():
             zstat_fixtures(tmp)
         if "nfor" in only:
             nfor_fixtures(tmp)
+        if "adaptive" in only:
+            adaptive_fixtures(tmp)
         return
     sampler_fixture(tmp)
     small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
@@ -233,6 +274,7 @@ def main():
     fd_fixture(tmp)
     zstat_fixtures(tmp)
     nfor_fixtures(tmp)
+    adaptive_fixtures(tmp)
 
     # Cornell box, 300x300, 50 spp, depth 8, seed 1 (the plain CPU-runnable config)
     c = workloads.write_cornell(os.path.join(tmp, "cornell"))

@@ -41,6 +41,8 @@ def load():
         lib.oracle_ggx_dlog_alpha.restype = ctypes.c_double
         lib.oracle_ggx_fallback.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, vp, vp]
         lib.oracle_ggx_fallback.restype = ctypes.c_double
+        lib.oracle_render_adaptive.argtypes = ([ctypes.c_char_p] + [ctypes.c_uint32] * 8 + [ctypes.c_double, vp, vp])
+        lib.oracle_render_adaptive.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -64,6 +66,22 @@ def render(path, seed, spp, width, height, sample_begin=0, max_bounces=None, thr
     if rc != 0:
         raise RuntimeError(lib.oracle_last_error().decode())
     return out, ms.value
+
+
+def render_adaptive(path, seed, spp, width, height, min_samples=0, check_interval=0, threshold=0.0,
+                    max_bounces=None, threads=None):
+    """Adaptive per-pixel stopping (0 / 0.0: the scene file's values): (image, counts), or None
+    when the rule cannot stop a pixel early."""
+    lib = load()
+    out = np.zeros((height, width, 3), dtype=np.float64)
+    counts = np.zeros((height, width), dtype=np.uint32)
+    threads = threads or min(16, os.cpu_count() or 1)
+    rc = lib.oracle_render_adaptive(path.encode(), seed, spp, width, height,
+                                    0 if max_bounces is None else max_bounces + 1, threads, min_samples,
+                                    check_interval, threshold, out.ctypes.data, counts.ctypes.data)
+    if rc < 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return None if rc == 1 else (out, counts)
 
 
 def primary(path, seed, width, height):

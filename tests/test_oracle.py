@@ -138,9 +138,10 @@ def test_bias_fixture_tiles_match_oracle(cornell):
 def test_nfor_fixture_is_consistent():
     """The denoise fixtures' layout: the raw mean image is the sample-weighted mean of the
     half-buffers (even samples in A, odd in B), and features stay in their ranges."""
-    for name in ("nfor_cornell_48x48_8spp", "nfor_primitives_64x48_8spp_r9"):
+    for name in ("nfor_cornell_48x48_8spp", "nfor_primitives_64x48_8spp_r9", "nfor_adaptive_cornell_48x48_48spp"):
         g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
-        n = int(g["spp"])
+        # adaptive sampling: each pixel's own sample count
+        n = g["counts"].astype(np.int64)[..., None] if "counts" in g else int(g["spp"])
         na, nb = (n + 1) // 2, n // 2
         mean = (g["color_a"] * na + g["color_b"] * nb) / n
         assert np.allclose(mean, g["image"], rtol=1e-12, atol=1e-12)
