@@ -362,10 +362,11 @@ def main():
     if args.workload == "terrain":
         # the denoise stage of the volume workload: the reference's NFOR with a radius-9
         # regression window over the two half-buffers (untimed for the headline)
-        _, _, dst, _ = Renderer.render_denoised(scene, seed=args.seed,
-                                                nfor={"feature_prefilter_radius": 1, "regression_radius": 9,
-                                                      "patch_radius": 1})
-        log(f"denoised render: {dst.render_ms:.1f} ms render (plain kernel, half-buffers), NFOR {dst.denoise_ms:.1f} ms")
+        nfor = {"feature_prefilter_radius": 1, "regression_radius": 9, "patch_radius": 1}
+        Renderer.render_denoised(scene, seed=args.seed, nfor=nfor)  # first call: graph capture, buffers
+        _, _, dst, _ = Renderer.render_denoised(scene, seed=args.seed, nfor=nfor)
+        log(f"denoised render: {dst.render_ms:.1f} ms render (wavefront, half-buffers + features), "
+            f"NFOR {dst.denoise_ms:.1f} ms")
         line["denoise"] = {"nfor_ms": round(dst.denoise_ms, 3), "render_ms": round(dst.render_ms, 3),
                            "regression_radius": 9, "patch_radius": 1, "feature_prefilter_radius": 1}
     scene.release()

@@ -208,8 +208,8 @@ int		luzgpu_autotune(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes, c
  * the device. rgb_out receives Scene::getImage() (the raw mean), denoised_out the denoised
  * image (Scene::getDenoisedImage(), before post-processing); both W*H*3 fp32, host or
  * device pointers as settings->output_on_device says. buffers_out (optional, host FP64)
- * receives the NFOR inputs. The whole sample range is rendered in one call, on the plain
- * kernel. */
+ * receives the NFOR inputs. The whole sample range is rendered in one call, by the
+ * integrator settings->integrator names (features come from each path's primary hit). */
 int		luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings,
 			const luzgpu_nfor_settings* nfor, float* rgb_out, float* denoised_out, luzgpu_nfor_buffers* buffers_out,
 			luzgpu_stats* stats);

@@ -32,6 +32,35 @@ namespace luzgpu
 	constexpr int	kNforUpper = kNforBasis * (kNforBasis + 1) / 2;	// 78 upper-triangle entries
 	constexpr int	kNforSystem = kNforUpper + kNforBasis * 3;		// + the RGB right-hand side
 
+	// Per-pixel FP64 statistics of a denoised render, in sample order (_threadRender with
+	// denoise buffers, Threads.cpp:585-627): [0,3) half A colour sum, [3,6) half B, [6,9) all
+	// colours, [9,12) colour squares, then kNforFeatures each of half A, half B, all features
+	// and feature squares.
+	constexpr int	kDenoiseStats = 12 + 4 * kNforFeatures;
+
+	// Adds one sample (cleaned colour, primary features) to a pixel's statistics `a`;
+	// `odd`: the sample index is odd (half B, sampler stream 2).
+	__device__ __forceinline__ void	denoiseAddSample(double* a, const float* color, const float* feat, bool odd)
+	{
+		#pragma clang fp contract(off)
+		const int half = odd ? 3 : 0;
+		for (int j = 0; j < 3; j++)
+		{
+			const double c = color[j];
+			a[half + j] += c;
+			a[6 + j] += c;
+			a[9 + j] += c * c;
+		}
+		const int fh = odd ? 12 + kNforFeatures : 12;
+		for (int j = 0; j < kNforFeatures; j++)
+		{
+			const double f = feat[j];
+			a[fh + j] += f;
+			a[12 + 2 * kNforFeatures + j] += f;
+			a[12 + 3 * kNforFeatures + j] += f * f;
+		}
+	}
+
 	struct	NforParams
 	{
 		std::uint32_t	width, height;
@@ -424,6 +453,153 @@ namespace luzgpu
 				nforFallback(image, variance, P, x, y, out);
 			}
 			__syncthreads();
+		}
+	}
+
+	// regressionPass (NFOR.cpp:679-737) with one lane per pixel: the window is visited in the
+	// reference's raster order and every sum is formed in its order, the system lives in
+	// registers (78 upper-triangle entries + 36 right-hand-side entries while accumulating,
+	// the completed 12 x 15 system while solving, every index static after unrolling) and is
+	// solved by the reference's Gauss-Jordan elimination with partial pivoting
+	// (solveLinearSystemRGB, NFOR.cpp:466-516). A pair pass (regressionPairPass, :739-832)
+	// is two launches, one per bandwidth: each pair of sums is formed as in the reference.
+	__global__ __launch_bounds__(64) void	nfor_regression_lane(const double* __restrict__ image, const double* __restrict__ features,
+		const double* __restrict__ variance, double* __restrict__ out, NforParams P, double bandwidth)
+	{
+		#pragma clang fp contract(off)
+		const std::uint32_t c = blockIdx.x * 64u + threadIdx.x;
+		if (c >= P.width * P.height) return;
+		const std::uint32_t x = c % P.width, y = c / P.width;
+		double upper[kNforUpper];
+		double rhs[kNforBasis][3];
+		{
+			int k = 0;
+			#pragma unroll
+			for (int row = 0; row < kNforBasis; row++)
+			{
+				#pragma unroll
+				for (int col = row; col < kNforBasis; col++) upper[k++] = (col == row) ? 1e-4 : 0.0;	// ridge
+				rhs[row][0] = rhs[row][1] = rhs[row][2] = 0.0;
+			}
+		}
+		double center[kNforFeatures];
+		#pragma unroll
+		for (int j = 0; j < kNforFeatures; j++) center[j] = features[kNforFeatures * static_cast<size_t>(c) + j];
+		const double bw2 = fmax(P.minVariance, bandwidth * bandwidth);
+		const long long r = P.regressionRadius;
+		for (long long oy = -r; oy <= r; oy++)
+		{
+			const std::uint32_t sy = nforClamp(static_cast<long long>(y) + oy, P.height);
+			for (long long ox = -r; ox <= r; ox++)
+			{
+				const std::uint32_t sx = nforClamp(static_cast<long long>(x) + ox, P.width);
+				const size_t s = static_cast<size_t>(sy) * P.width + sx;
+				const double distance = nforPatchDistance(image, variance, P, x, y, sx, sy, P.patchRadius);
+				const double w = exp(-distance / bw2);	// colorWeightFromDistance (NFOR.cpp:313-318)
+				double basis[kNforBasis];
+				basis[0] = 1.0;
+				#pragma unroll
+				for (int j = 0; j < kNforFeatures; j++) basis[j + 1] = features[kNforFeatures * s + j] - center[j];
+				const double cr = image[3 * s], cg = image[3 * s + 1], cb = image[3 * s + 2];
+				// addRegressionSample (NFOR.cpp:582-600)
+				int k = 0;
+				#pragma unroll
+				for (int row = 0; row < kNforBasis; row++)
+				{
+					const double wb = w * basis[row];
+					#pragma unroll
+					for (int col = row; col < kNforBasis; col++) upper[k++] += wb * basis[col];
+					rhs[row][0] += wb * cr;
+					rhs[row][1] += wb * cg;
+					rhs[row][2] += wb * cb;
+				}
+			}
+		}
+		// completeRegressionMatrix (NFOR.cpp:628-637), then solveLinearSystemRGB
+		double m[kNforBasis][kNforBasis];
+		{
+			int k = 0;
+			#pragma unroll
+			for (int row = 0; row < kNforBasis; row++)
+			{
+				#pragma unroll
+				for (int col = row; col < kNforBasis; col++)
+				{
+					m[row][col] = upper[k];
+					m[col][row] = upper[k];
+					k++;
+				}
+			}
+		}
+		bool solved = true;
+		#pragma unroll
+		for (int pivot = 0; pivot < kNforBasis; pivot++)
+		{
+			int best = pivot;
+			double bestValue = fabs(m[pivot][pivot]);
+			#pragma unroll
+			for (int row = pivot + 1; row < kNforBasis; row++)
+			{
+				const double v = fabs(m[row][pivot]);
+				if (v > bestValue)
+				{
+					bestValue = v;
+					best = row;
+				}
+			}
+			if (bestValue < 1e-10)
+			{
+				solved = false;
+				break;
+			}
+			// row swap (columns below the pivot are zero in every row already)
+			#pragma unroll
+			for (int row = pivot + 1; row < kNforBasis; row++)
+			{
+				if (row == best)
+				{
+					#pragma unroll
+					for (int col = pivot; col < kNforBasis; col++)
+					{
+						const double t = m[pivot][col];
+						m[pivot][col] = m[row][col];
+						m[row][col] = t;
+					}
+					#pragma unroll
+					for (int j = 0; j < 3; j++)
+					{
+						const double t = rhs[pivot][j];
+						rhs[pivot][j] = rhs[row][j];
+						rhs[row][j] = t;
+					}
+				}
+			}
+			const double pivotValue = m[pivot][pivot];
+			#pragma unroll
+			for (int col = pivot; col < kNforBasis; col++) m[pivot][col] /= pivotValue;
+			#pragma unroll
+			for (int j = 0; j < 3; j++) rhs[pivot][j] /= pivotValue;
+			#pragma unroll
+			for (int row = 0; row < kNforBasis; row++)
+			{
+				if (row == pivot) continue;
+				const double factor = m[row][pivot];
+				#pragma unroll
+				for (int col = pivot; col < kNforBasis; col++) m[row][col] -= factor * m[pivot][col];
+				#pragma unroll
+				for (int j = 0; j < 3; j++) rhs[row][j] -= factor * rhs[pivot][j];
+			}
+		}
+		double* o = out + 3 * static_cast<size_t>(c);
+		if (solved)
+		{
+			o[0] = rhs[0][0];
+			o[1] = rhs[0][1];
+			o[2] = rhs[0][2];
+		}
+		else
+		{
+			nforFallback(image, variance, P, x, y, o);
 		}
 	}
 

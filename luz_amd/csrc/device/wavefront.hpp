@@ -28,6 +28,7 @@
 
 #include "adaptive.hpp"
 #include "integrator.hpp"
+#include "nfor.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -93,6 +94,8 @@ namespace luzgpu
 		double*	sums;			// the scene's FP64 pixel sums
 		const std::uint32_t*	pixelList;	// adaptive rounds: the active pixels (nullptr: every pixel)
 		double*	moments;		// adaptive rounds: per-pixel AdaptiveAccumulator (or nullptr)
+		float*	features;		// denoising: primary features per global path id (or nullptr)
+		double*	denoise;		// denoising: per-pixel statistics (kDenoiseStats) instead of sums
 	};
 
 	// Image pixel of entry i of a pass's pixel range.
@@ -480,6 +483,11 @@ namespace luzgpu
 					f3 c = thr * missRadiance(s, p, ray.d, wgt);
 					if (b > 0 && hasConstantBackgroundLight(p)) c = clampConstantBackground(p, c);
 					acc = acc + clampRay(c);
+					if (b == 0 && J.features)
+					{
+						primaryFeatures(J.features + static_cast<size_t>(kNforFeatures) * path, p, px, py, false, 0.0f, mk(0, 0, 0),
+							missRadiance(s, p, ray.d, 1.0f), 0u);
+					}
 				}
 				else
 				{
@@ -493,6 +501,11 @@ namespace luzgpu
 					shadePoint = sf.position;
 					from = originOf(s, h);
 					const GpuMaterial& m = s.materials[sf.material];
+					if (b == 0 && J.features)
+					{
+						primaryFeatures(J.features + static_cast<size_t>(kNforFeatures) * path, p, px, py, true, h.t, sf.normal,
+							ld3(m.color), m.kind);
+					}
 					const f3 emitted = m.kind == GPU_MAT_EMISSIVE ? ld3(m.color) : mk(0, 0, 0);
 					Scatter sc;
 					if (!scatter<Feat>(m, sf, ray.d, ps, sc))
@@ -687,6 +700,33 @@ namespace luzgpu
 	{
 		const PassJob& J = jobs[job];
 		const std::uint32_t count = J.params.sampleCount;
+		if (J.denoise)
+		{
+			// denoising: half-buffer and feature statistics (Threads.cpp:585-627)
+			for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < J.pixels; i += gridDim.x * blockDim.x)
+			{
+				const size_t px = passPixel(J, i);
+				double a[kDenoiseStats];
+				for (int j = 0; j < kDenoiseStats; j++) a[j] = J.denoise[kDenoiseStats * px + j];
+				double m[kAdaptiveMoments];
+				if (J.moments)
+				{
+					for (int k = 0; k < kAdaptiveMoments; k++) m[k] = J.moments[kAdaptiveMoments * px + k];
+				}
+				for (std::uint32_t k = 0; k < count; k++)
+				{
+					const size_t g = static_cast<size_t>(J.pathBase) + static_cast<size_t>(k) * J.pixels + i;
+					denoiseAddSample(a, colors + 3 * g, J.features + kNforFeatures * g, ((J.params.sampleBegin + k) & 1u) != 0u);
+					if (J.moments) adaptiveAdd(m, colors[3 * g], colors[3 * g + 1], colors[3 * g + 2]);
+				}
+				for (int j = 0; j < kDenoiseStats; j++) J.denoise[kDenoiseStats * px + j] = a[j];
+				if (J.moments)
+				{
+					for (int k = 0; k < kAdaptiveMoments; k++) J.moments[kAdaptiveMoments * px + k] = m[k];
+				}
+			}
+			return;
+		}
 		for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < J.pixels; i += gridDim.x * blockDim.x)
 		{
 			const size_t px = passPixel(J, i);
@@ -767,6 +807,7 @@ namespace luzgpu
 		const std::uint32_t*	pixelList = nullptr;	// adaptive round: the active pixels
 		std::uint32_t	listCount = 0;				// ... and how many
 		double*	moments = nullptr;				// adaptive round: per-pixel accumulators
+		double*	denoise = nullptr;				// denoising: per-pixel statistics (sums unused)
 	};
 
 	inline std::uint32_t	jobPixels(const BatchJob& j)
@@ -782,6 +823,8 @@ namespace luzgpu
 		size_t	bytes = 0;
 		WaveBuffers	buf{};
 		float*	colors = nullptr;		// [global path][3]
+		float*	features = nullptr;		// denoising: [global path][kNforFeatures], on demand
+		std::uint32_t	featureCapacity = 0;
 		PassJob*	jobs = nullptr;		// the current pass's table
 		PassJob*	plans = nullptr;	// every pass's table of a render
 		size_t	planCapacity = 0;
@@ -800,6 +843,9 @@ namespace luzgpu
 			dropGraph();
 			if (mem) (void)hipFree(mem);
 			if (plans) (void)hipFree(plans);
+			if (features) (void)hipFree(features);
+			features = nullptr;
+			featureCapacity = 0;
 			mem = nullptr;
 			plans = nullptr;
 			planCapacity = 0;
@@ -871,6 +917,22 @@ namespace luzgpu
 				bytes = need;
 			}
 			carve(static_cast<char*>(mem), capacity);
+		}
+
+		// The per-path feature buffer of denoising passes (not part of the captured layout:
+		// passes find it through their job table).
+		void	ensureFeatures(std::uint32_t capacity)
+		{
+			if (features && capacity <= featureCapacity) return;
+			if (features) (void)hipFree(features);
+			features = nullptr;
+			featureCapacity = 0;
+			if (hipMalloc(reinterpret_cast<void**>(&features), static_cast<size_t>(capacity) * kNforFeatures * sizeof(float)) != hipSuccess)
+			{
+				features = nullptr;
+				throw std::runtime_error("wavefront feature buffer allocation failed");
+			}
+			featureCapacity = capacity;
 		}
 
 		void	ensurePlans(size_t entries)
@@ -1003,6 +1065,7 @@ namespace luzgpu
 				pj.sums = jobs[j].sums;
 				pj.pixelList = jobs[j].pixelList;
 				pj.moments = jobs[j].moments;
+				pj.denoise = jobs[j].denoise;
 				plan.push_back(pj);
 				done[j] += c;
 				base += pixels * c;
@@ -1011,6 +1074,13 @@ namespace luzgpu
 		}
 		const std::uint32_t passes = static_cast<std::uint32_t>(plan.size() / n);
 		st.ensure(capacity);
+		bool denoising = false;
+		for (const BatchJob& j : jobs) denoising = denoising || j.denoise != nullptr;
+		if (denoising)
+		{
+			st.ensureFeatures(st.buf.capacity);
+			for (PassJob& pj : plan) pj.features = pj.denoise ? st.features : nullptr;
+		}
 		st.ensurePlans(plan.size());
 		// the host table dies with this call: copy it before returning (ordered on the stream)
 		checkHip(hipMemcpyAsync(st.plans, plan.data(), plan.size() * sizeof(PassJob), hipMemcpyHostToDevice, stream), "pass tables");

@@ -210,12 +210,6 @@ namespace luzgpu
 		}
 	}
 
-	// Per-pixel FP64 statistics of a denoised render, in sample order (_threadRender with
-	// denoise buffers, Threads.cpp:585-627): [0,3) half A colour sum, [3,6) half B, [6,9) all
-	// colours, [9,12) colour squares, then kNforFeatures each of half A, half B, all features
-	// and feature squares.
-	constexpr int	kDenoiseStats = 12 + 4 * kNforFeatures;
-
 	// `list` / `pixels`: as render_mega_denoise; `moments` (adaptive rounds, or nullptr):
 	// the pixels' AdaptiveAccumulator.
 	__global__ void	denoise_accumulate(const float* __restrict__ colors, const float* __restrict__ feats, double* __restrict__ acc,
@@ -236,23 +230,8 @@ namespace luzgpu
 		for (std::uint32_t k = 0; k < count; k++)
 		{
 			const size_t idx = static_cast<size_t>(k) * pixels + i;
-			const int half = ((firstSample + k) & 1u) ? 3 : 0;
-			for (int j = 0; j < 3; j++)
-			{
-				const double c = colors[3 * idx + j];
-				a[half + j] += c;
-				a[6 + j] += c;
-				a[9 + j] += c * c;
-			}
+			denoiseAddSample(a, colors + 3 * idx, feats + kNforFeatures * idx, ((firstSample + k) & 1u) != 0u);
 			if (moments) adaptiveAdd(m, colors[3 * idx], colors[3 * idx + 1], colors[3 * idx + 2]);
-			const int fh = ((firstSample + k) & 1u) ? 12 + kNforFeatures : 12;
-			for (int j = 0; j < kNforFeatures; j++)
-			{
-				const double f = feats[kNforFeatures * idx + j];
-				a[fh + j] += f;
-				a[12 + 2 * kNforFeatures + j] += f;
-				a[12 + 3 * kNforFeatures + j] += f * f;
-			}
 		}
 		for (int j = 0; j < kDenoiseStats; j++) acc[static_cast<size_t>(kDenoiseStats) * px + j] = a[j];
 		if (moments)
@@ -855,13 +834,16 @@ namespace luzgpu
 			nfor_variances<<<g, 256, 0, st>>>(colorVariance, featVariance, regVar, featVarAvg, static_cast<std::uint32_t>(n));
 			nfor_prefilter<<<g, 256, 0, st>>>(featA, featB, featVarAvg, ffA, P);
 			nfor_prefilter<<<g, 256, 0, st>>>(featB, featA, featVarAvg, ffB, P);
-			nfor_regression<<<static_cast<unsigned>(n), 64, 0, st>>>(supA, ffB, regVar, a0, a1, P, 0.5, 1.0);
-			nfor_regression<<<static_cast<unsigned>(n), 64, 0, st>>>(supB, ffA, regVar, b0, b1, P, 0.5, 1.0);
+			const unsigned lanes = blocks(n, 64);
+			nfor_regression_lane<<<lanes, 64, 0, st>>>(supA, ffB, regVar, a0, P, 0.5);
+			nfor_regression_lane<<<lanes, 64, 0, st>>>(supA, ffB, regVar, a1, P, 1.0);
+			nfor_regression_lane<<<lanes, 64, 0, st>>>(supB, ffA, regVar, b0, P, 0.5);
+			nfor_regression_lane<<<lanes, 64, 0, st>>>(supB, ffA, regVar, b1, P, 1.0);
 			nfor_mse<<<g, 256, 0, st>>>(supA, supB, a0, b0, a1, b1, regVar, m0, m1, static_cast<std::uint32_t>(n));
 			nfor_filter_pair<<<g, 256, 0, st>>>(m0, m1, color, colorVariance, f0, f1, P, 0);
 			nfor_filter_pair<<<g, 256, 0, st>>>(f0, f1, color, colorVariance, s0, s1, P, 1);
 			nfor_combine<<<g, 256, 0, st>>>(s0, s1, f0, f1, a0, a1, b0, b1, ffA, ffB, combined, finalFeat, finalVar, P);
-			nfor_regression<<<static_cast<unsigned>(n), 64, 0, st>>>(combined, finalFeat, finalVar, out, nullptr, P, 1.0, 1.0);
+			nfor_regression_lane<<<lanes, 64, 0, st>>>(combined, finalFeat, finalVar, out, P, 1.0);
 			check(hipGetLastError(), "NFOR launch");
 		}
 
@@ -1401,8 +1383,29 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 		check(hipEventCreate(&e1), "hipEventCreate");
 		check(hipEventCreate(&e2), "hipEventCreate");
 		check(hipEventRecord(e0, st), "hipEventRecord");
+		const std::uint32_t integrator = settings ? settings->integrator : LUZGPU_INTEGRATOR_WAVEFRONT;
+		if (integrator > LUZGPU_INTEGRATOR_MEGAKERNEL)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render_denoised: unknown integrator"};
+		}
+		const WaveTuning tune = tuningOf(settings).resolved();
 		// samples [begin, begin + count) of `active` pixels (`list`, or all) into the statistics
 		auto renderRange = [&](std::uint32_t begin, std::uint32_t count, const std::uint32_t* list, std::uint32_t active, double* moments) {
+			if (integrator == LUZGPU_INTEGRATOR_WAVEFRONT)
+			{
+				GpuParams q = p;
+				q.sampleBegin = begin;
+				q.sampleCount = count;
+				BatchJob job{q, nullptr};
+				job.pixelList = list;
+				job.listCount = active;
+				job.moments = moments;
+				job.denoise = ctx->denoiseStats.as<double>();
+				renderWavefront(ctx->wave, scene->gpu.view, scene->gpu.host.features, {job}, ctx->counters.as<unsigned long long>(), st,
+					false, nullptr, tune);
+				check(hipGetLastError(), "denoise render launch");
+				return;
+			}
 			// samples per launch: per-sample colour + feature buffers within 4 GiB
 			const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(count,
 				static_cast<std::uint32_t>((4ull << 30) / (static_cast<unsigned long long>(active) * (12ull + 4ull * kNforFeatures)))));

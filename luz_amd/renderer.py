@@ -335,12 +335,12 @@ class Renderer:
 
     @staticmethod
     def render_denoised(scene: Scene, seed: int, samples=None, max_bounces=None, resolution=None, nfor=None,
-                        buffers=False, adaptive=None):
+                        buffers=False, adaptive=None, integrator: str = "wavefront"):
         """Render with denoising on (luzgpu_render_denoised): returns (raw image, denoised image,
         RenderStats, NFOR buffers or None). ``nfor``: {"feature_prefilter_radius",
         "regression_radius", "patch_radius", "min_variance"} (defaults as NFORSettings);
         ``adaptive`` as for ``render``."""
-        s = _settings(seed, samples, max_bounces, 0, None, "megakernel", resolution=resolution, adaptive=adaptive)
+        s = _settings(seed, samples, max_bounces, 0, None, integrator, resolution=resolution, adaptive=adaptive)
         w, h = resolution if resolution is not None else (scene.width, scene.height)
         img = np.zeros((h, w, 3), dtype=np.float32)
         den = np.zeros((h, w, 3), dtype=np.float32)

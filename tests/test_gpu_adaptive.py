@@ -114,7 +114,8 @@ def test_adaptive_in_batch_matches_single(ctx, cornell, primitives):
     assert st.rendered_samples == int(counts_a.sum()) + 16 * 40 * 40
 
 
-def test_adaptive_denoised_render_matches_reference(ctx, cornell):
+@pytest.mark.parametrize("integrator", ["wavefront", "megakernel"])
+def test_adaptive_denoised_render_matches_reference(ctx, cornell, integrator):
     from luz_amd import Renderer, SceneFile
     g = np.load(os.path.join(GOLDEN, "nfor_adaptive_cornell_48x48_48spp.npz"))
     scene = SceneFile.read(ctx, cornell.scene_path)
@@ -122,9 +123,9 @@ def test_adaptive_denoised_render_matches_reference(ctx, cornell):
     r = [int(v) for v in g["radii"]]
     nfor = {"feature_prefilter_radius": r[0], "regression_radius": r[1], "patch_radius": r[2], "min_variance": 1e-5}
     img, den, st, buf = Renderer.render_denoised(scene, seed=1, samples=int(g["spp"]), resolution=(w, h), nfor=nfor,
-                                                 buffers=True, adaptive=_rule(g))
+                                                 buffers=True, adaptive=_rule(g), integrator=integrator)
     counts = Renderer.sample_counts(ctx)
-    _compare("adaptive denoise", img, counts, g, st)
+    _compare(f"adaptive denoise [{integrator}]", img, counts, g, st)
     same = counts == g["counts"]
     for k in ("color_a", "color_b", "features_a", "features_b"):
         e = rel_mse(buf[k][same], g[k][same])

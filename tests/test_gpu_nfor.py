@@ -45,18 +45,19 @@ def test_nfor_on_reference_buffers(ctx, name):
     assert e < 1e-12
 
 
+@pytest.mark.parametrize("integrator", ["wavefront", "megakernel"])
 @pytest.mark.parametrize("name", CASES)
-def test_denoised_render_matches_reference(ctx, request, name):
+def test_denoised_render_matches_reference(ctx, request, name, integrator):
     from luz_amd import Renderer, SceneFile
     g = _fixture(name)
     wl = request.getfixturevalue(str(g["scene"]))
     scene = SceneFile.read(ctx, wl.scene_path)
     h, w = g["image"].shape[:2]
     img, den, st, buf = Renderer.render_denoised(scene, seed=int(g["seed"]), samples=int(g["spp"]), resolution=(w, h),
-                                                 nfor=_settings(g), buffers=True)
+                                                 nfor=_settings(g), buffers=True, integrator=integrator)
     errors = {k: rel_mse(buf[k], g[k]) for k in ("image", "color_a", "color_b", "features_a", "features_b")}
     errors["denoised"] = rel_mse(den, g["denoised"])
-    print(f"{name}: " + ", ".join(f"{k} {v:.2e}" for k, v in errors.items()) + f"; render {st.render_ms:.1f} ms, "
+    print(f"{name} [{integrator}]: " + ", ".join(f"{k} {v:.2e}" for k, v in errors.items()) + f"; render {st.render_ms:.1f} ms, "
           f"NFOR {st.denoise_ms:.2f} ms")
     assert np.array_equal(img, buf["image"].astype(np.float32))
     for k in ("image", "color_a", "color_b", "features_a", "features_b"):
