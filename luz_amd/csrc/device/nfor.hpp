@@ -12,12 +12,10 @@
 //   MSE estimation, its and the bandwidth selection's cross-bilateral filtering (:834-936);
 //   per-pixel blend, then a final regression of the blend (:1074-1101).
 //
-// The regression solves a 12 x 12 weighted least-squares system per pixel. One wave of 64
-// lanes owns one pixel: for each window sample, lanes share the patch distance (a wave
-// reduction) and the 12-entry basis (lane shuffles), and each lane accumulates its own slice
-// of the two systems' upper triangles and right-hand sides in registers; the systems are
-// then solved by Gauss-Jordan elimination with partial pivoting in LDS, lanes working on
-// the elements of each pivot step.
+// The regression solves a 12 x 12 weighted least-squares system per pixel. Each lane owns
+// one pixel: it walks the window in the reference's raster order, so every sum is formed in
+// the reference's order, keeps the system in registers and solves it by the reference's
+// Gauss-Jordan elimination with partial pivoting.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -268,192 +266,6 @@ namespace luzgpu
 		}
 		const std::uint32_t c = y * P.width + x;
 		for (int j = 0; j < 3; j++) out[j] = total <= 2.220446049250313e-16 ? image[3 * c + j] : acc[j] / total;
-	}
-
-	__device__ __forceinline__ double	nforWaveSum(double v)
-	{
-		for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-		return v;
-	}
-
-	// Solves the system in `m` (kNforBasis x kNforBasis, row-major) with right-hand side `rhs`
-	// (kNforBasis x 3) in place (solveLinearSystemRGB, NFOR.cpp:466-516); returns false when a
-	// pivot falls below 1e-10. All 64 lanes of the block's single wave take part.
-	__device__ inline bool	nforSolve(double* m, double* rhs, int* ok)
-	{
-		const int lane = static_cast<int>(threadIdx.x);
-		constexpr int W = kNforBasis + 3;	// columns incl. the right-hand side
-		for (int pivot = 0; pivot < kNforBasis; pivot++)
-		{
-			if (lane == 0)
-			{
-				int best = pivot;
-				double bestValue = fabs(m[pivot * kNforBasis + pivot]);
-				for (int row = pivot + 1; row < kNforBasis; row++)
-				{
-					const double v = fabs(m[row * kNforBasis + pivot]);
-					if (v > bestValue)
-					{
-						bestValue = v;
-						best = row;
-					}
-				}
-				ok[0] = bestValue < 1e-10 ? 0 : 1;
-				ok[1] = best;
-			}
-			__syncthreads();
-			if (!ok[0]) return false;
-			const int best = ok[1];
-			if (best != pivot && lane < W)
-			{
-				double* a = lane < kNforBasis ? m + pivot * kNforBasis + lane : rhs + pivot * 3 + (lane - kNforBasis);
-				double* b = lane < kNforBasis ? m + best * kNforBasis + lane : rhs + best * 3 + (lane - kNforBasis);
-				const double t = *a;
-				*a = *b;
-				*b = t;
-			}
-			__syncthreads();
-			const double pivotValue = m[pivot * kNforBasis + pivot];
-			__syncthreads();
-			if (lane < W && (lane >= kNforBasis || lane >= pivot))
-			{
-				double* a = lane < kNforBasis ? m + pivot * kNforBasis + lane : rhs + pivot * 3 + (lane - kNforBasis);
-				*a /= pivotValue;
-			}
-			__syncthreads();
-			// each lane: a set of (row, column) elements; factors are read before any update
-			double factor[3];
-			int rows[3], cols[3], cnt = 0;
-			for (int e = lane; e < kNforBasis * W; e += 64)
-			{
-				const int row = e / W, col = e % W;
-				if (row == pivot || (col < kNforBasis && col < pivot)) continue;
-				rows[cnt] = row;
-				cols[cnt] = col;
-				factor[cnt] = m[row * kNforBasis + pivot];
-				cnt++;
-			}
-			__syncthreads();
-			for (int k = 0; k < cnt; k++)
-			{
-				const int row = rows[k], col = cols[k];
-				if (col < kNforBasis) m[row * kNforBasis + col] -= factor[k] * m[pivot * kNforBasis + col];
-				else rhs[row * 3 + (col - kNforBasis)] -= factor[k] * rhs[pivot * 3 + (col - kNforBasis)];
-			}
-			__syncthreads();
-		}
-		return true;
-	}
-
-	// regressionPass / regressionPairPass (NFOR.cpp:679-832): one 64-lane block per pixel.
-	// `second` null: one bandwidth (b0) into `first`; else b0 into `first` and b1 into `second`.
-	__global__ __launch_bounds__(64) void	nfor_regression(const double* __restrict__ image, const double* __restrict__ features,
-		const double* __restrict__ variance, double* __restrict__ first, double* __restrict__ second, NforParams P, double b0,
-		double b1)
-	{
-		__shared__ double m[2][kNforBasis * kNforBasis];
-		__shared__ double rhs[2][kNforBasis * 3];
-		__shared__ int ok[2];
-		const std::uint32_t c = blockIdx.x;
-		if (c >= P.width * P.height) return;
-		const std::uint32_t x = c % P.width, y = c / P.width;
-		const int lane = static_cast<int>(threadIdx.x);
-		const int systems = second ? 2 : 1;
-		// this lane's entries: index e < systems * kNforSystem; e % kNforSystem < kNforUpper is
-		// an upper-triangle matrix element, the rest right-hand-side elements
-		double acc[4] = {0.0, 0.0, 0.0, 0.0};
-		int er[4] = {0, 0, 0, 0}, ec[4] = {0, 0, 0, 0}, es[4] = {0, 0, 0, 0}, ne = 0;
-		for (int e = lane; e < systems * kNforSystem; e += 64)
-		{
-			const int sys = e / kNforSystem, k = e % kNforSystem;
-			int row, col;
-			if (k < kNforUpper)
-			{
-				row = 0;
-				int rem = k;
-				while (rem >= kNforBasis - row)
-				{
-					rem -= kNforBasis - row;
-					row++;
-				}
-				col = row + rem;
-			}
-			else
-			{
-				row = (k - kNforUpper) / 3;
-				col = kNforBasis + (k - kNforUpper) % 3;	// right-hand side component
-			}
-			er[ne] = row;
-			ec[ne] = col;
-			es[ne] = sys;
-			acc[ne] = (col == row) ? 1e-4 : 0.0;	// initializeRegressionMatrix: ridge 1e-4
-			ne++;
-		}
-		const long long r = P.regressionRadius;
-		const long long pr = P.patchRadius;
-		const int patchTerms = static_cast<int>((2 * pr + 1) * (2 * pr + 1));
-		for (long long oy = -r; oy <= r; oy++)
-		{
-			for (long long ox = -r; ox <= r; ox++)
-			{
-				const std::uint32_t sx = nforClamp(static_cast<long long>(x) + ox, P.width);
-				const std::uint32_t sy = nforClamp(static_cast<long long>(y) + oy, P.height);
-				const std::uint32_t s = sy * P.width + sx;
-				// patch distance: lanes share its terms, then a wave sum
-				double term = 0.0;
-				if (!(sx == x && sy == y))
-				{
-					for (int t = lane; t < patchTerms; t += 64)
-					{
-						const long long px = t % (2 * pr + 1) - pr, py = t / (2 * pr + 1) - pr;
-						const std::uint32_t ia = nforClamp(static_cast<long long>(y) + py, P.height) * P.width + nforClamp(static_cast<long long>(x) + px, P.width);
-						const std::uint32_t ib = nforClamp(static_cast<long long>(sy) + py, P.height) * P.width + nforClamp(static_cast<long long>(sx) + px, P.width);
-						term += nforDist2(image + 3 * ia, image + 3 * ib) / (variance[ia] + variance[ib] + P.minVariance);
-					}
-				}
-				const double distance = (sx == x && sy == y) ? 0.0 : nforWaveSum(term) / static_cast<double>(patchTerms);
-				const double w0 = nforWeight(distance, b0, P.minVariance);
-				const double w1 = second ? nforWeight(distance, b1, P.minVariance) : 0.0;
-				// the basis (lanes 0..11) and the sample's colour (lanes 12..14)
-				double v = 0.0;
-				if (lane == 0) v = 1.0;
-				else if (lane < kNforBasis) v = features[kNforFeatures * s + (lane - 1)] - features[kNforFeatures * c + (lane - 1)];
-				else if (lane < kNforBasis + 3) v = image[3 * s + (lane - kNforBasis)];
-				// every lane runs all four shuffle pairs (a lane with fewer entries reads lane 0
-				// for the spare ones), so the shuffles stay wave-uniform
-				for (int k = 0; k < 4; k++)
-				{
-					const double vr = __shfl(v, er[k]);
-					const double vc = __shfl(v, ec[k]);
-					const double w = es[k] == 0 ? w0 : w1;
-					if (k < ne) acc[k] += (w * vr) * vc;
-				}
-			}
-		}
-		// completeRegressionMatrix + solve, one system at a time
-		for (int k = 0; k < ne; k++)
-		{
-			if (ec[k] < kNforBasis)
-			{
-				m[es[k]][er[k] * kNforBasis + ec[k]] = acc[k];
-				m[es[k]][ec[k] * kNforBasis + er[k]] = acc[k];
-			}
-			else rhs[es[k]][er[k] * 3 + (ec[k] - kNforBasis)] = acc[k];
-		}
-		__syncthreads();
-		for (int sys = 0; sys < systems; sys++)
-		{
-			double* out = (sys == 0 ? first : second) + 3 * static_cast<size_t>(c);
-			if (nforSolve(m[sys], rhs[sys], ok))
-			{
-				if (lane < 3) out[lane] = rhs[sys][lane];
-			}
-			else if (lane == 0)
-			{
-				nforFallback(image, variance, P, x, y, out);
-			}
-			__syncthreads();
-		}
 	}
 
 	// regressionPass (NFOR.cpp:679-737) with one lane per pixel: the window is visited in the
