@@ -268,20 +268,42 @@ namespace luzgpu
 		for (int j = 0; j < 3; j++) out[j] = total <= 2.220446049250313e-16 ? image[3 * c + j] : acc[j] / total;
 	}
 
-	// regressionPass (NFOR.cpp:679-737) with one lane per pixel: the window is visited in the
-	// reference's raster order and every sum is formed in its order, the system lives in
-	// registers (78 upper-triangle entries + 36 right-hand-side entries while accumulating,
-	// the completed 12 x 15 system while solving, every index static after unrolling) and is
-	// solved by the reference's Gauss-Jordan elimination with partial pivoting
-	// (solveLinearSystemRGB, NFOR.cpp:466-516). A pair pass (regressionPairPass, :739-832)
-	// is two launches, one per bandwidth: each pair of sums is formed as in the reference.
-	__global__ __launch_bounds__(64) void	nfor_regression_lane(const double* __restrict__ image, const double* __restrict__ features,
-		const double* __restrict__ variance, double* __restrict__ out, NforParams P, double bandwidth)
+	constexpr int	kNforTile = 16;						// pixels per tile side (one lane each)
+	constexpr int	kNforMaxPatchRadius = 8;
+	constexpr int	kNforApronMax = kNforTile + 2 * kNforMaxPatchRadius;
+
+	// regressionPass (NFOR.cpp:679-737) with one lane per pixel, a 16 x 16 tile per block.
+	// The window is visited in the reference's raster order and every sum is formed in its
+	// order. The system lives in registers (78 upper-triangle entries + 36 right-hand-side
+	// entries while accumulating, the completed 12 x 15 system while solving, every index
+	// static after unrolling) and is solved by the reference's Gauss-Jordan elimination with
+	// partial pivoting (solveLinearSystemRGB, NFOR.cpp:466-516). A pair pass
+	// (regressionPairPass, :739-832) is two launches, one per bandwidth.
+	//
+	// Patch distances: for a window displacement d, the term of patch offset o between pixel
+	// p and its sample p + d is D(p + o, p + o + d) (patchDistance, NFOR.cpp:240-311), which
+	// depends on p + o only. The block evaluates D(q, q + d) once for every q of its tile and
+	// apron into LDS, and each lane sums its (2r+1)^2 terms from there in the reference's
+	// order. Lanes whose patch or sample would be clamped at the image border take the
+	// direct, clamped form.
+	__global__ __launch_bounds__(kNforTile * kNforTile) void	nfor_regression_tile(const double* __restrict__ image,
+		const double* __restrict__ features, const double* __restrict__ variance, double* __restrict__ out, NforParams P,
+		double bandwidth)
 	{
 		#pragma clang fp contract(off)
-		const std::uint32_t c = blockIdx.x * 64u + threadIdx.x;
-		if (c >= P.width * P.height) return;
-		const std::uint32_t x = c % P.width, y = c / P.width;
+		__shared__ double pairTerms[2][kNforApronMax * kNforApronMax];
+		const int tx = static_cast<int>(threadIdx.x) % kNforTile, ty = static_cast<int>(threadIdx.x) / kNforTile;
+		const std::uint32_t tilesX = (P.width + kNforTile - 1) / kNforTile;
+		const long long x0 = static_cast<long long>(blockIdx.x % tilesX) * kNforTile;
+		const long long y0 = static_cast<long long>(blockIdx.x / tilesX) * kNforTile;
+		const std::uint32_t x = static_cast<std::uint32_t>(x0) + tx, y = static_cast<std::uint32_t>(y0) + ty;
+		const bool active = x < P.width && y < P.height;
+		const std::uint32_t c = active ? y * P.width + x : 0u;
+		const long long pr = P.patchRadius;
+		const int apron = kNforTile + 2 * static_cast<int>(pr);
+		const double patchTerms = static_cast<double>((2 * pr + 1) * (2 * pr + 1));
+		const bool patchInside = active && static_cast<long long>(x) >= pr && static_cast<long long>(y) >= pr
+			&& static_cast<long long>(x) + pr < static_cast<long long>(P.width) && static_cast<long long>(y) + pr < static_cast<long long>(P.height);
 		double upper[kNforUpper];
 		double rhs[kNforBasis][3];
 		{
@@ -299,14 +321,49 @@ namespace luzgpu
 		for (int j = 0; j < kNforFeatures; j++) center[j] = features[kNforFeatures * static_cast<size_t>(c) + j];
 		const double bw2 = fmax(P.minVariance, bandwidth * bandwidth);
 		const long long r = P.regressionRadius;
+		int buf = 0;
 		for (long long oy = -r; oy <= r; oy++)
 		{
-			const std::uint32_t sy = nforClamp(static_cast<long long>(y) + oy, P.height);
 			for (long long ox = -r; ox <= r; ox++)
 			{
-				const std::uint32_t sx = nforClamp(static_cast<long long>(x) + ox, P.width);
+				// D(q, q + d) over the tile and its apron (entries off the image are clamped
+				// reads whose values no lane uses); double-buffered, so one barrier per step
+				for (int i = static_cast<int>(threadIdx.x); i < apron * apron; i += kNforTile * kNforTile)
+				{
+					const long long qx = x0 - pr + i % apron, qy = y0 - pr + i / apron;
+					const std::uint32_t qa = nforClamp(qy, P.height) * P.width + nforClamp(qx, P.width);
+					const std::uint32_t qb = nforClamp(qy + oy, P.height) * P.width + nforClamp(qx + ox, P.width);
+					pairTerms[buf][i] = nforDist2(image + 3 * static_cast<size_t>(qa), image + 3 * static_cast<size_t>(qb))
+						/ (variance[qa] + variance[qb] + P.minVariance);
+				}
+				__syncthreads();
+				const double* terms = pairTerms[buf];
+				buf ^= 1;
+				if (!active) continue;
+				const long long ux = static_cast<long long>(x) + ox, uy = static_cast<long long>(y) + oy;
+				const std::uint32_t sx = nforClamp(ux, P.width), sy = nforClamp(uy, P.height);
 				const size_t s = static_cast<size_t>(sy) * P.width + sx;
-				const double distance = nforPatchDistance(image, variance, P, x, y, sx, sy, P.patchRadius);
+				double distance = 0.0;
+				if (sx == x && sy == y)
+				{
+					distance = 0.0;
+				}
+				else if (patchInside && ux == sx && uy == sy && ux >= pr && uy >= pr && ux + pr < static_cast<long long>(P.width)
+					&& uy + pr < static_cast<long long>(P.height))
+				{
+					for (long long py = -pr; py <= pr; py++)
+					{
+						for (long long px = -pr; px <= pr; px++)
+						{
+							distance += terms[(ty + pr + py) * apron + (tx + pr + px)];
+						}
+					}
+					distance = distance / patchTerms;
+				}
+				else
+				{
+					distance = nforPatchDistance(image, variance, P, x, y, sx, sy, P.patchRadius);
+				}
 				const double w = exp(-distance / bw2);	// colorWeightFromDistance (NFOR.cpp:313-318)
 				double basis[kNforBasis];
 				basis[0] = 1.0;
@@ -327,6 +384,7 @@ namespace luzgpu
 				}
 			}
 		}
+		if (!active) return;
 		// completeRegressionMatrix (NFOR.cpp:628-637), then solveLinearSystemRGB
 		double m[kNforBasis][kNforBasis];
 		{

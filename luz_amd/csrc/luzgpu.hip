@@ -789,7 +789,8 @@ namespace luzgpu
 			{
 				throw Error{LUZGPU_E_INVALID, "NFOR radii must be positive."};	// NFOR.cpp:985-988
 			}
-			if (P.regressionRadius > 32 || P.patchRadius > 8 || P.prefilterRadius > 32 || !(P.minVariance >= 0.0))
+			if (P.regressionRadius > 32 || P.patchRadius > static_cast<std::uint32_t>(kNforMaxPatchRadius) || P.prefilterRadius > 32
+				|| !(P.minVariance >= 0.0))
 			{
 				throw Error{LUZGPU_E_INVALID, "NFOR settings out of range: regression radius <= 32, patch radius <= 8, prefilter radius <= 32"};
 			}
@@ -834,16 +835,17 @@ namespace luzgpu
 			nfor_variances<<<g, 256, 0, st>>>(colorVariance, featVariance, regVar, featVarAvg, static_cast<std::uint32_t>(n));
 			nfor_prefilter<<<g, 256, 0, st>>>(featA, featB, featVarAvg, ffA, P);
 			nfor_prefilter<<<g, 256, 0, st>>>(featB, featA, featVarAvg, ffB, P);
-			const unsigned lanes = blocks(n, 64);
-			nfor_regression_lane<<<lanes, 64, 0, st>>>(supA, ffB, regVar, a0, P, 0.5);
-			nfor_regression_lane<<<lanes, 64, 0, st>>>(supA, ffB, regVar, a1, P, 1.0);
-			nfor_regression_lane<<<lanes, 64, 0, st>>>(supB, ffA, regVar, b0, P, 0.5);
-			nfor_regression_lane<<<lanes, 64, 0, st>>>(supB, ffA, regVar, b1, P, 1.0);
+			const unsigned tiles = blocks(P.width, kNforTile) * blocks(P.height, kNforTile);
+			constexpr unsigned tb = kNforTile * kNforTile;
+			nfor_regression_tile<<<tiles, tb, 0, st>>>(supA, ffB, regVar, a0, P, 0.5);
+			nfor_regression_tile<<<tiles, tb, 0, st>>>(supA, ffB, regVar, a1, P, 1.0);
+			nfor_regression_tile<<<tiles, tb, 0, st>>>(supB, ffA, regVar, b0, P, 0.5);
+			nfor_regression_tile<<<tiles, tb, 0, st>>>(supB, ffA, regVar, b1, P, 1.0);
 			nfor_mse<<<g, 256, 0, st>>>(supA, supB, a0, b0, a1, b1, regVar, m0, m1, static_cast<std::uint32_t>(n));
 			nfor_filter_pair<<<g, 256, 0, st>>>(m0, m1, color, colorVariance, f0, f1, P, 0);
 			nfor_filter_pair<<<g, 256, 0, st>>>(f0, f1, color, colorVariance, s0, s1, P, 1);
 			nfor_combine<<<g, 256, 0, st>>>(s0, s1, f0, f1, a0, a1, b0, b1, ffA, ffB, combined, finalFeat, finalVar, P);
-			nfor_regression_lane<<<lanes, 64, 0, st>>>(combined, finalFeat, finalVar, out, P, 1.0);
+			nfor_regression_tile<<<tiles, tb, 0, st>>>(combined, finalFeat, finalVar, out, P, 1.0);
 			check(hipGetLastError(), "NFOR launch");
 		}
 
