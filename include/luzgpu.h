@@ -73,6 +73,10 @@ typedef struct luzgpu_settings
 	uint32_t	adaptive_check_interval; /* 0: scene value (adaptivecheckinterval, default 32)    */
 	uint32_t	reserved0;
 	double		adaptive_threshold;      /* <= 0: scene value (adaptivethreshold, default 0.03)   */
+	uint32_t	postprocess;     /* 1: rgb_out (and the denoised image) receive the scene's display  *
+	                              * post-processing of the raw image, as Renderer::render leaves     *
+	                              * Scene::getImage() (applyPostProcessing, Threads.cpp:77-99)       */
+	uint32_t	reserved1;
 } luzgpu_settings;
 
 enum
@@ -116,6 +120,25 @@ typedef struct luzgpu_stats
 	double		tune_ms;             /* autotune: search time spent by this call (not in render_ms) */
 	double		denoise_ms;          /* luzgpu_render_denoised: NFOR (SceneRenderStats::denoiseMS) */
 } luzgpu_stats;
+
+/* Display post-processing (applyPostProcessing, Threads.cpp:77-99; Scene defaults
+ * Scene.cpp:109-139): exposure in stops, bloom on / off, the view transform and the display
+ * contrast (1 = the default curve-free path). */
+typedef struct luzgpu_post_settings
+{
+	double		exposure;
+	double		contrast;
+	uint32_t	bloom;
+	uint32_t	view_transform;   /* LUZGPU_VIEW_* */
+} luzgpu_post_settings;
+
+enum
+{
+	LUZGPU_VIEW_STANDARD = 0,
+	LUZGPU_VIEW_AGX = 1,         /* the reference's default */
+	LUZGPU_VIEW_ACES = 2,
+	LUZGPU_VIEW_RAW = 3
+};
 
 /* Denoise::NFORSettings (NFOR.hpp:39-48); NULL = the reference's defaults 1, 2, 1, 1e-5. */
 typedef struct luzgpu_nfor_settings
@@ -218,6 +241,19 @@ int		luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_s
  * receives W*H*3 FP64 on the host, elapsed_ms (optional) the device time. */
 int		luzgpu_nfor(luzgpu_ctx* ctx, const luzgpu_nfor_buffers* in, const luzgpu_nfor_settings* nfor, double* denoised_out,
 			double* elapsed_ms);
+
+/* The scene file's post-processing settings (view_transform, bloom, exposure /
+ * photographic_exposure, contrast keys; Settings.cpp:274-323). */
+int		luzgpu_scene_post_settings(const luzgpu_scene* scene, luzgpu_post_settings* out);
+
+/* applyPostProcessing (Threads.cpp:77-99) on a W x H x 3 FP64 scene-linear image:
+ * Image::applyExposure, bloom (Image::extractBloom + Gaussian::blur(9, 2.0) added back at
+ * 0.25, Threads.cpp:55-75), then unless the view is raw the view transform to display sRGB
+ * (Image::applyViewTransformAndEncodeSRGB, or applyViewTransform + applyContrast +
+ * gammaCorrect when contrast != 1) and Image::suppressIsolatedFireflies. image_in and
+ * image_out are host pointers, or device pointers when on_device = 1; they may be equal. */
+int		luzgpu_postprocess(luzgpu_ctx* ctx, const luzgpu_post_settings* post, uint32_t width, uint32_t height,
+			const double* image_in, double* image_out, uint32_t on_device);
 
 /* Path-replay adjoint (no counterpart in the reference): gradients of an image loss with
  * respect to per-material parameters, for the render luzgpu_render produces with the same

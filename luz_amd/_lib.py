@@ -27,7 +27,7 @@ EXPORTED = [
     "luzgpu_scene_release", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_primary_hits",
     "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas", "luzgpu_render_batch",
     "luzgpu_adjoint", "luzgpu_scene_set_materials", "luzgpu_autotune", "luzgpu_render_denoised", "luzgpu_nfor",
-    "luzgpu_sample_counts",
+    "luzgpu_sample_counts", "luzgpu_scene_post_settings", "luzgpu_postprocess",
 ]
 
 
@@ -55,7 +55,22 @@ class Settings(ctypes.Structure):
         ("adaptive_check_interval", ctypes.c_uint32),
         ("reserved0", ctypes.c_uint32),
         ("adaptive_threshold", ctypes.c_double),
+        ("postprocess", ctypes.c_uint32),
+        ("reserved1", ctypes.c_uint32),
     ]
+
+
+class PostSettings(ctypes.Structure):
+    _fields_ = [
+        ("exposure", ctypes.c_double),
+        ("contrast", ctypes.c_double),
+        ("bloom", ctypes.c_uint32),
+        ("view_transform", ctypes.c_uint32),
+    ]
+
+
+VIEW_STANDARD, VIEW_AGX, VIEW_ACES, VIEW_RAW = 0, 1, 2, 3
+VIEW_NAMES = {"standard": VIEW_STANDARD, "agx": VIEW_AGX, "aces": VIEW_ACES, "raw": VIEW_RAW}
 
 
 ADAPTIVE_SCENE, ADAPTIVE_OFF, ADAPTIVE_ON = 0, 1, 2
@@ -166,6 +181,9 @@ def load() -> ctypes.CDLL:
                                         ctypes.POINTER(vp), ctypes.POINTER(Stats)]
     lib.luzgpu_sampler_eval.argtypes = [vp, ctypes.c_size_t, vp, vp, vp]
     lib.luzgpu_sample_counts.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.luzgpu_scene_post_settings.argtypes = [vp, ctypes.POINTER(PostSettings)]
+    lib.luzgpu_postprocess.argtypes = [vp, ctypes.POINTER(PostSettings), ctypes.c_uint32, ctypes.c_uint32, vp, vp,
+                                       ctypes.c_uint32]
     lib.luzgpu_scene_tlas_topology.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     lib.luzgpu_scene_blas.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64)]

@@ -902,6 +902,10 @@ namespace luzgpu
 		p.adaptive.minSamples = s.adaptiveMinSamples;
 		p.adaptive.checkInterval = s.adaptiveCheckInterval;
 		p.adaptive.threshold = s.adaptiveThreshold;
+		p.post.viewTransform = s.viewTransform;
+		p.post.bloom = s.bloom;
+		p.post.exposure = s.exposure;
+		p.post.contrast = s.contrast;
 
 		// ---- materials -------------------------------------------------------------
 		for (const Material& m : scene.materials)

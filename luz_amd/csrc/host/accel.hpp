@@ -41,6 +41,16 @@ namespace luzgpu
 		double	threshold = 0.03;
 	};
 
+	// The scene file's display post-processing settings (Settings.cpp:274-323), applied by
+	// the caller's request after a render (applyPostProcessing, Threads.cpp:77-99).
+	struct	PostSettings
+	{
+		int	viewTransform = 1;	// 0 standard, 1 AgX, 2 ACES, 3 raw
+		bool	bloom = true;
+		double	exposure = 0.0;
+		double	contrast = 1.0;
+	};
+
 	struct	PackedScene
 	{
 		std::vector<BvhNode>	tlasNodes;	// binary device form (intermediate)
@@ -82,6 +92,7 @@ namespace luzgpu
 		std::vector<double>	lightScale;	// light selection weight per unit luminance (area, or 1)
 		std::vector<dvec3>	materialColor;	// FP64 material colours (weights are rebuilt from these)
 		AdaptiveSettings	adaptive;
+		PostSettings	post;
 	};
 
 	// Rebuilds light emission, selection weights, CDF and probabilities after material

@@ -186,6 +186,12 @@ namespace luzgpu
 		std::uint32_t	adaptiveCheckInterval = 32;
 		double	adaptiveThreshold = 0.03;
 		bool	denoise = true;
+		// display post-processing (Scene.cpp:99-139 defaults): view transform 0 standard,
+		// 1 AgX, 2 ACES, 3 raw
+		int	viewTransform = 1;
+		bool	bloom = true;
+		double	exposure = 0.0;
+		double	contrast = 1.0;
 		SkyKind	sky = SKY_NONE;
 		bool	skySet = false;
 		dvec3	background{};

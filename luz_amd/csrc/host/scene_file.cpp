@@ -267,23 +267,86 @@ namespace luzgpu
 					{
 						s.denoise = parseDouble(value, "denoise setting") != 0.0;
 					}
-					else if (key == "view_transform" || key == "outputfilename")
+					else if (key == "outputfilename")
 					{
-						// display-only: the device path produces the raw scene-linear image
+						// the output file is the caller's (image writers are outside the device path)
+					}
+					else if (key == "view_transform")
+					{
+						// parseViewTransform (Settings.cpp:36-57)
+						const std::string v = lower(trim(value));
+						if (v == "standard") s.viewTransform = 0;
+						else if (v == "agx") s.viewTransform = 1;
+						else if (v == "aces") s.viewTransform = 2;
+						else if (v == "raw") s.viewTransform = 3;
+						else
+						{
+							throw std::runtime_error("Invalid view_transform setting. Use standard, agx, aces, or raw.");
+						}
 					}
 					else if (key == "bloom")
 					{
-						if (parseDouble(value, "bloom setting") != 0.0)
+						// Settings.cpp:281-291: an integer, 0 or 1
+						int v = 0;
+						if (std::sscanf(value.c_str(), "%d", &v) != 1)
 						{
-							throw std::runtime_error("Unsupported setting for the device path: bloom=1");
+							throw std::runtime_error("Invalid bloom setting. Use bloom=0 or bloom=1.");
 						}
+						if (v != 0 && v != 1)
+						{
+							throw std::runtime_error("Invalid bloom setting. Use 0 or 1.");
+						}
+						s.bloom = v != 0;
 					}
 					else if (key == "exposure")
 					{
-						if (parseDouble(value, "exposure setting") != 0.0)
+						// Settings.cpp:292-301, Scene.cpp:261-268
+						double v = 0.0;
+						if (std::sscanf(value.c_str(), "%lf", &v) != 1)
 						{
-							throw std::runtime_error("Unsupported setting for the device path: exposure != 0");
+							throw std::runtime_error("Invalid exposure setting. Use exposure=F.");
 						}
+						if (!std::isfinite(v) || !std::isfinite(std::pow(2.0, v)))
+						{
+							throw std::runtime_error("Exposure must be finite.");
+						}
+						s.exposure = v;
+					}
+					else if (key == "photographic_exposure" || key == "photographicexposure" || key == "camera_exposure"
+						|| key == "cameraexposure")
+					{
+						// applyPhotographicExposure (Settings.cpp:102-117), Scene.cpp:270-286:
+						// exposure = log2(shutter * (ISO / 100) / N^2)
+						double n = 0.0, t = 0.0, iso = 0.0;
+						char tail = 0;
+						if (std::sscanf(value.c_str(), "%lf , %lf , %lf %c", &n, &t, &iso, &tail) != 3)
+						{
+							throw std::runtime_error("Invalid photographic exposure setting. Use photographic_exposure=F_NUMBER,SHUTTER_SECONDS,ISO.");
+						}
+						if (!std::isfinite(n) || n <= 0.0 || !std::isfinite(t) || t <= 0.0 || !std::isfinite(iso) || iso <= 0.0)
+						{
+							throw std::runtime_error("Photographic exposure values must be finite and positive.");
+						}
+						const double v = std::log2((t * (iso / 100.0)) / (n * n));
+						if (!std::isfinite(v) || !std::isfinite(std::pow(2.0, v)))
+						{
+							throw std::runtime_error("Exposure must be finite.");
+						}
+						s.exposure = v;
+					}
+					else if (key == "contrast")
+					{
+						// Settings.cpp:314-323, Scene.cpp:293-300
+						double v = 0.0;
+						if (std::sscanf(value.c_str(), "%lf", &v) != 1)
+						{
+							throw std::runtime_error("Invalid contrast setting. Use contrast=F.");
+						}
+						if (!std::isfinite(v) || v < 0.0)
+						{
+							throw std::runtime_error("Contrast must be non-negative.");
+						}
+						s.contrast = v;
 					}
 					else if (key == "sky")
 					{

@@ -14,6 +14,7 @@
 #include "device/adjoint.hpp"
 #include "device/integrator.hpp"
 #include "device/nfor.hpp"
+#include "device/post.hpp"
 #include "device/wavefront.hpp"
 #include "host/accel.hpp"
 
@@ -393,6 +394,21 @@ namespace luzgpu
 		}
 	}
 
+	// The FP64 mean image (cleanColor'd, Threads.cpp:580-581) for post-processing: each
+	// pixel's sum over `used[i]` samples, or over `count` when used is nullptr.
+	__global__ void	resolve_double(const double* __restrict__ sums, const std::uint32_t* __restrict__ used, double count,
+		double* __restrict__ out, std::uint32_t pixels)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= pixels) return;
+		const double n = used ? static_cast<double>(used[i]) : count;
+		for (int k = 0; k < 3; k++)
+		{
+			const double v = sums[3 * i + k] / n;
+			out[3 * i + k] = isfinite(v) ? v : 0.0;
+		}
+	}
+
 	__global__ void	to_float(const double* __restrict__ in, float* __restrict__ out, size_t n)
 	{
 		const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -499,6 +515,8 @@ struct luzgpu_ctx
 	luzgpu::DeviceBuffer	sampleFeatures, denoiseStats, nforBuffers, nforTemp;
 	// adaptive rounds: per-pixel accumulators, sample counts, two active-pixel lists, counters
 	luzgpu::DeviceBuffer	adaptiveMem;
+	// display post-processing: the FP64 mean image and three working images
+	luzgpu::DeviceBuffer	postMem;
 	// per-pixel sample counts of the last single-scene render: `countsUniform` samples each,
 	// or (countsUniform == 0) the adaptive rounds' counts in adaptiveMem
 	std::uint32_t	countsPixels = 0;
@@ -866,6 +884,81 @@ namespace luzgpu
 			ctx->tuned.emplace_back(key, r);
 			if (report) *report = r;
 			return tuningOf(r);
+		}
+
+		luzgpu_post_settings	postOf(const PackedScene& h)
+		{
+			luzgpu_post_settings r{};
+			r.exposure = h.post.exposure;
+			r.contrast = h.post.contrast;
+			r.bloom = h.post.bloom ? 1u : 0u;
+			r.view_transform = static_cast<std::uint32_t>(h.post.viewTransform);
+			return r;
+		}
+
+		// Kernel parameters of a post-processing run, validated as the reference's setters do
+		// (Scene.cpp:261-300); the exposure scale and the blur taps are formed on the host
+		// exactly as Image::applyExposure and createSeparableKernel form them.
+		PostParams	postParams(const luzgpu_post_settings& s, std::uint32_t width, std::uint32_t height)
+		{
+			PostParams P{};
+			P.width = width;
+			P.height = height;
+			if (!std::isfinite(s.exposure) || !std::isfinite(std::pow(2.0, s.exposure)))
+			{
+				throw Error{LUZGPU_E_INVALID, "Exposure must be finite."};
+			}
+			if (!std::isfinite(s.contrast) || s.contrast < 0.0)
+			{
+				throw Error{LUZGPU_E_INVALID, "Contrast must be non-negative."};
+			}
+			if (s.view_transform > LUZGPU_VIEW_RAW || s.bloom > 1u)
+			{
+				throw Error{LUZGPU_E_INVALID, "post-processing: unknown view transform or bloom flag"};
+			}
+			P.exposureScale = std::pow(2.0, s.exposure);
+			P.contrast = s.contrast;
+			P.view = static_cast<int>(s.view_transform);
+			double sum = 0.0;
+			for (int x = -4; x <= 4; x++)
+			{
+				P.blur[x + 4] = std::exp(-(static_cast<double>(x * x)) / (2.0 * 2.0 * 2.0));
+				sum += P.blur[x + 4];
+			}
+			for (double& v : P.blur) v /= sum;
+			return P;
+		}
+
+		// applyPostProcessing (Threads.cpp:77-99) from the FP64 image `in` into `out` (device
+		// pointers, W x H x 3; they may be equal).
+		void	runPost(luzgpu_ctx* ctx, const luzgpu_post_settings& s, std::uint32_t width, std::uint32_t height, const double* in,
+			double* out)
+		{
+			const PostParams P = postParams(s, width, height);
+			const size_t n = static_cast<size_t>(width) * height;
+			hipStream_t st = ctx->stream;
+			ctx->postMem.reserve(4 * n * 3 * sizeof(double));
+			double* img = ctx->postMem.as<double>() + n * 3;
+			double* a = img + n * 3;
+			double* b = a + n * 3;
+			const unsigned g = blocks(n, 256);
+			post_exposure<<<g, 256, 0, st>>>(in, img, P);
+			if (s.bloom)
+			{
+				post_bloom_extract<<<g, 256, 0, st>>>(img, a, P);
+				post_blur_h<<<g, 256, 0, st>>>(a, b, P);
+				post_blur_v_add<<<g, 256, 0, st>>>(b, img, P);
+			}
+			if (P.view == POST_VIEW_RAW)
+			{
+				check(hipMemcpyAsync(out, img, n * 3 * sizeof(double), hipMemcpyDeviceToDevice, st), "post copy");
+			}
+			else
+			{
+				post_view<<<g, 256, 0, st>>>(img, a, P);
+				post_fireflies<<<g, 256, 0, st>>>(a, out, P);
+			}
+			check(hipGetLastError(), "post-processing launch");
 		}
 
 		// The stopping rule of a render, or false when every pixel takes every sample: the
@@ -1300,7 +1393,16 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			ctx->output.reserve(static_cast<size_t>(pixels) * 3 * sizeof(float));
 			dst = ctx->output.as<float>();
 		}
-		if (adaptive) resolve_used<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), A.used, dst, pixels);
+		if (settings && settings->postprocess)
+		{
+			ctx->postMem.reserve(4 * static_cast<size_t>(pixels) * 3 * sizeof(double));
+			double* mean = ctx->postMem.as<double>();
+			resolve_double<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), adaptive ? A.used : nullptr,
+				static_cast<double>(p.sampleCount), mean, pixels);
+			runPost(ctx, postOf(scene->gpu.host), p.width, p.height, mean, mean);
+			post_to_float<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(mean, dst, static_cast<size_t>(pixels) * 3);
+		}
+		else if (adaptive) resolve_used<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), A.used, dst, pixels);
 		else resolve<<<blocks(pixels, 256), 256, 0, st>>>(ctx->sums.as<double>(), dst, pixels, static_cast<double>(p.sampleCount));
 		check(hipGetLastError(), "resolve launch");
 		unsigned long long counters[6] = {0, 0, 0, 0, 0, 0};
@@ -1454,7 +1556,20 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 		check(hipEventRecord(e1, st), "hipEventRecord");
 		double* out = nb + L.total;
 		runNfor(ctx, P, nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB, nb + L.featVariance, out);
-		to_float<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(out, denoised, static_cast<size_t>(pixels) * 3);
+		if (settings && settings->postprocess)
+		{
+			// the reference post-processes both images (Threads.cpp:530-546)
+			ctx->postMem.reserve(4 * static_cast<size_t>(pixels) * 3 * sizeof(double));
+			double* tmp = ctx->postMem.as<double>();
+			runPost(ctx, postOf(scene->gpu.host), p.width, p.height, nb + L.mean, tmp);
+			to_float<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(tmp, image, static_cast<size_t>(pixels) * 3);
+			runPost(ctx, postOf(scene->gpu.host), p.width, p.height, out, tmp);
+			to_float<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(tmp, denoised, static_cast<size_t>(pixels) * 3);
+		}
+		else
+		{
+			to_float<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(out, denoised, static_cast<size_t>(pixels) * 3);
+		}
 		check(hipEventRecord(e2, st), "hipEventRecord");
 		const size_t imageBytes = static_cast<size_t>(pixels) * 3 * sizeof(float);
 		const hipMemcpyKind kind = (settings && settings->output_on_device) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
@@ -1731,7 +1846,16 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 				copies.emplace_back(rgb_out[i], staged);
 				staged += static_cast<size_t>(pixels) * 3 * sizeof(float);
 			}
-			if (adaptive) resolve_used<<<blocks(pixels, 256), 256, 0, st>>>(jobs[i].sums, A.used + aj[i].offset, dst, pixels);
+			if (settings && settings[i].postprocess)
+			{
+				ctx->postMem.reserve(4 * static_cast<size_t>(pixels) * 3 * sizeof(double));
+				double* mean = ctx->postMem.as<double>();
+				resolve_double<<<blocks(pixels, 256), 256, 0, st>>>(jobs[i].sums, adaptive ? A.used + aj[i].offset : nullptr,
+					static_cast<double>(jobs[i].params.sampleCount), mean, pixels);
+				runPost(ctx, postOf(scenes[i]->gpu.host), jobs[i].params.width, jobs[i].params.height, mean, mean);
+				post_to_float<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(mean, dst, static_cast<size_t>(pixels) * 3);
+			}
+			else if (adaptive) resolve_used<<<blocks(pixels, 256), 256, 0, st>>>(jobs[i].sums, A.used + aj[i].offset, dst, pixels);
 			else resolve<<<blocks(pixels, 256), 256, 0, st>>>(jobs[i].sums, dst, pixels, static_cast<double>(jobs[i].params.sampleCount));
 		}
 		check(hipGetLastError(), "resolve launch");
@@ -1893,6 +2017,46 @@ int	luzgpu_scene_set_materials(luzgpu_scene* scene, uint32_t count, const double
 		// same-sized uploads keep their buffers, so a captured pass graph (keyed on the
 		// scene's device pointers) stays valid; merged batch arrays hold copies: rebuild them
 		ctx->batchKey.clear();
+	});
+}
+
+int	luzgpu_scene_post_settings(const luzgpu_scene* scene, luzgpu_post_settings* out)
+{
+	if (!scene || !out)
+	{
+		g_lastError = "luzgpu_scene_post_settings: NULL argument";
+		return LUZGPU_E_INVALID;
+	}
+	*out = postOf(scene->gpu.host);
+	return LUZGPU_OK;
+}
+
+int	luzgpu_postprocess(luzgpu_ctx* ctx, const luzgpu_post_settings* post, uint32_t width, uint32_t height, const double* image_in,
+	double* image_out, uint32_t on_device)
+{
+	if (!ctx || !post || !image_in || !image_out || width == 0 || height == 0)
+	{
+		g_lastError = "luzgpu_postprocess: NULL argument or empty image";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		hipStream_t st = ctx->stream;
+		const size_t n = static_cast<size_t>(width) * height;
+		const size_t bytes = n * 3 * sizeof(double);
+		if (on_device)
+		{
+			runPost(ctx, *post, width, height, image_in, image_out);
+			check(hipStreamSynchronize(st), "post-processing");
+			return;
+		}
+		(void)postParams(*post, width, height);	// validate before any copy
+		ctx->postMem.reserve(4 * bytes);
+		double* buf = ctx->postMem.as<double>();
+		check(hipMemcpyAsync(buf, image_in, bytes, hipMemcpyHostToDevice, st), "image upload");
+		runPost(ctx, *post, width, height, buf, buf);
+		check(hipMemcpyAsync(image_out, buf, bytes, hipMemcpyDeviceToHost, st), "image readback");
+		check(hipStreamSynchronize(st), "post-processing");
 	});
 }
 

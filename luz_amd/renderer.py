@@ -150,6 +150,12 @@ class Scene:
                                                    None, None))
         return boxes, links, idx
 
+    def post_settings(self):
+        """The scene file's post-processing settings (a luzgpu_post_settings)."""
+        ps = _lib.PostSettings()
+        _lib.check(self._lib.luzgpu_scene_post_settings(self._h, ctypes.byref(ps)))
+        return ps
+
     def set_materials(self, albedo=None, roughness=None, emission=None) -> None:
         """Per-material parameters (scene-file order): albedo (n, 3) for Lambertian, roughness
         (n,) for metal, emission (n, 3) radiance for emitters; None keeps the current values."""
@@ -201,8 +207,9 @@ class SceneFile:
 
 
 def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=False, resolution=None,
-              count_visits=False, stage_times=False, tuning=None, adaptive=None):
+              count_visits=False, stage_times=False, tuning=None, adaptive=None, postprocess=False):
     s = _lib.Settings()
+    s.postprocess = 1 if postprocess else 0
     # adaptive: None = the scene's adaptive= setting, False / True = --no-adaptive / --adaptive,
     # a dict {"min_samples", "check_interval", "threshold"} = on with those values
     if adaptive is False:
@@ -238,13 +245,22 @@ def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator
     return s
 
 
+def _post_settings(post: dict):
+    ps = _lib.PostSettings()
+    ps.exposure = float(post.get("exposure", 0.0))
+    ps.contrast = float(post.get("contrast", 1.0))
+    ps.bloom = 1 if post.get("bloom", True) else 0
+    ps.view_transform = _lib.VIEW_NAMES[post.get("view_transform", "agx")]
+    return ps
+
+
 class Renderer:
     """``Renderer::render`` for the device path."""
 
     @staticmethod
     def render(scene: Scene, seed: int, samples=None, max_bounces=None, sample_begin=0, sample_count=None,
                integrator: str = "wavefront", out=None, resolution=None, count_visits=False, stage_times=False,
-               tuning=None, adaptive=None):
+               tuning=None, adaptive=None, postprocess=False):
         """Returns (image[H, W, 3] float32, RenderStats). ``out`` may be a device tensor
         (torch, float32, contiguous, H*W*3) that receives the image without a host copy.
         ``tuning`` ({"grid", "extend_refill", "shadow_refill", "fetch_chunk", "lds_stack"}) sets
@@ -252,10 +268,12 @@ class Renderer:
         first if needed); it never changes the image. ``adaptive``: None follows the scene
         file, False / True turn per-pixel stopping off / on, a dict {"min_samples",
         "check_interval", "threshold"} turns it on with those values;
-        ``Renderer.sample_counts`` then gives each pixel's sample count."""
+        ``Renderer.sample_counts`` then gives each pixel's sample count. ``postprocess``:
+        the image is the scene's display post-processing of the raw image (exposure, bloom,
+        view transform, firefly suppression), as Renderer::render leaves Scene::getImage()."""
         s = _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=out is not None,
                       resolution=resolution, count_visits=count_visits, stage_times=stage_times, tuning=tuning,
-                      adaptive=adaptive)
+                      adaptive=adaptive, postprocess=postprocess)
         st = _lib.Stats()
         w, h = resolution if resolution is not None else (scene.width, scene.height)
         if out is not None:
@@ -335,12 +353,13 @@ class Renderer:
 
     @staticmethod
     def render_denoised(scene: Scene, seed: int, samples=None, max_bounces=None, resolution=None, nfor=None,
-                        buffers=False, adaptive=None, integrator: str = "wavefront"):
+                        buffers=False, adaptive=None, integrator: str = "wavefront", postprocess=False):
         """Render with denoising on (luzgpu_render_denoised): returns (raw image, denoised image,
         RenderStats, NFOR buffers or None). ``nfor``: {"feature_prefilter_radius",
         "regression_radius", "patch_radius", "min_variance"} (defaults as NFORSettings);
         ``adaptive`` as for ``render``."""
-        s = _settings(seed, samples, max_bounces, 0, None, integrator, resolution=resolution, adaptive=adaptive)
+        s = _settings(seed, samples, max_bounces, 0, None, integrator, resolution=resolution, adaptive=adaptive,
+                      postprocess=postprocess)
         w, h = resolution if resolution is not None else (scene.width, scene.height)
         img = np.zeros((h, w, 3), dtype=np.float32)
         den = np.zeros((h, w, 3), dtype=np.float32)
@@ -378,6 +397,18 @@ class Renderer:
         _lib.check(ctx._lib.luzgpu_nfor(ctx.handle, ctypes.byref(bp), ctypes.byref(ns) if ns is not None else None,
                                         out.ctypes.data, ctypes.byref(ms)))
         return out, ms.value
+
+    @staticmethod
+    def postprocess(ctx, image: np.ndarray, post) -> np.ndarray:
+        """applyPostProcessing on the device for an (H, W, 3) scene-linear image (FP64);
+        ``post``: a Scene (its file's settings) or {"exposure", "contrast", "bloom",
+        "view_transform" ("standard" | "agx" | "aces" | "raw")}."""
+        ps = post.post_settings() if isinstance(post, Scene) else _post_settings(post)
+        img = np.ascontiguousarray(image, dtype=np.float64)
+        h, w = img.shape[:2]
+        out = np.zeros_like(img)
+        _lib.check(ctx._lib.luzgpu_postprocess(ctx.handle, ctypes.byref(ps), w, h, img.ctypes.data, out.ctypes.data, 0))
+        return out
 
     @staticmethod
     def sample_counts(ctx) -> np.ndarray:

@@ -56,6 +56,8 @@
 #include "Hittables/Sphere.hpp"
 #include "Materials/Lambertian.hpp"
 #include "EnvironmentMap.hpp"
+#include "Image.hpp"
+#include "Blur/Gaussian.hpp"
 #undef private
 #undef protected
 
@@ -332,6 +334,74 @@ namespace
 		for (std::size_t i = 0; i < w * h; i++)
 		{
 			const Color c = scene.getImage()->pixels()[i];
+			put(out, c.getRed());
+			put(out, c.getGreen());
+			put(out, c.getBlue());
+		}
+		return (0);
+	}
+
+	// post IN OUT EXPOSURE BLOOM VIEW CONTRAST: the reference's display post-processing of an
+	// FP64 image (u32 W, H, then W*H*3 f64, written back in the same layout). The steps are
+	// applyPostProcessing's (Threads.cpp:55-99, an unexported function there), called through
+	// the public Image / Gaussian API; VIEW: standard, agx, aces or raw.
+	int	cmdPost(int argc, char** argv)
+	{
+		if (argc < 8)
+		{
+			return (2);
+		}
+		std::ifstream in(argv[2], std::ios::binary);
+		std::uint32_t w = 0, h = 0;
+		in.read(reinterpret_cast<char*>(&w), 4);
+		in.read(reinterpret_cast<char*>(&h), 4);
+		Image image(w, h);
+		image.initialize();
+		for (std::size_t i = 0; i < static_cast<std::size_t>(w) * h; i++)
+		{
+			double c[3];
+			in.read(reinterpret_cast<char*>(c), sizeof(c));
+			image.pixels()[i] = Color(c[0], c[1], c[2]);
+		}
+		const double exposure = std::stod(argv[4]);
+		const bool bloom = std::stoi(argv[5]) != 0;
+		const std::string view = argv[6];
+		const double contrast = std::stod(argv[7]);
+		const ViewTransform vt = view == "standard" ? ViewTransform::Standard : view == "aces" ? ViewTransform::ACES
+			: view == "raw" ? ViewTransform::Raw : ViewTransform::AgX;
+		image.applyExposure(exposure);
+		if (bloom)
+		{
+			auto brightness = image.extractBloom(D_BLOOM_THRESHOLD, D_BLOOM_SOFT_KNEE);
+			Image blurred(image.getWidth(), image.getHeight());
+			blurred.initialize();
+			Gaussian::blur(*brightness, blurred, 9, 2.0);
+			for (std::size_t i = 0; i < static_cast<std::size_t>(w) * h; i++)
+			{
+				image.pixels()[i] += blurred.pixels()[i] * D_BLOOM_INTENSITY;
+			}
+		}
+		if (vt != ViewTransform::Raw)
+		{
+			if (contrast == D_CONTRAST)
+			{
+				image.applyViewTransformAndEncodeSRGB(vt);
+				image.suppressIsolatedFireflies();
+			}
+			else
+			{
+				image.applyViewTransform(vt);
+				image.applyContrast(contrast);
+				image.gammaCorrect();
+				image.suppressIsolatedFireflies();
+			}
+		}
+		std::ofstream out(argv[3], std::ios::binary);
+		put(out, w);
+		put(out, h);
+		for (std::size_t i = 0; i < static_cast<std::size_t>(w) * h; i++)
+		{
+			const Color c = image.pixels()[i];
 			put(out, c.getRed());
 			put(out, c.getGreen());
 			put(out, c.getBlue());
@@ -709,7 +779,7 @@ int	main(int argc, char** argv)
 {
 	if (argc < 2)
 	{
-		std::cerr << "usage: luzref render|seeds|denoise|adaptive|sampler|blas|tlas|primary|env|lights ..." << std::endl;
+		std::cerr << "usage: luzref render|seeds|denoise|adaptive|post|sampler|blas|tlas|primary|env|lights ..." << std::endl;
 		return (2);
 	}
 	try
@@ -725,6 +795,7 @@ int	main(int argc, char** argv)
 		if (cmd == "seeds") return (cmdSeeds(argc, argv));
 		if (cmd == "denoise") return (cmdDenoise(argc, argv));
 		if (cmd == "adaptive") return (cmdAdaptive(argc, argv));
+		if (cmd == "post") return (cmdPost(argc, argv));
 	}
 	catch (const std::exception& e)
 	{

@@ -195,6 +195,45 @@ def adaptive_fixtures(tmp):
         print(name, "counts", np.unique(counts, return_counts=True))
 
 
+POST_CASES = [
+    # (exposure, bloom, view transform, contrast)
+    (0.0, 1, "agx", 1.0),          # the reference's defaults
+    (1.0, 0, "aces", 1.0),
+    (-0.5, 1, "standard", 1.3),    # the contrast branch (applyViewTransform + applyContrast + gammaCorrect)
+    (0.25, 1, "raw", 1.0),         # bloom on a raw view
+    (2.0, 0, "agx", 0.8),
+    (1.5, 0, "raw", 1.0),          # exposure alone: max(x, 0) * 2^1.5
+]
+
+
+def post_fixture(tmp):
+    """The reference's display post-processing (oracle/ref_harness.cpp, post command) of a
+    64 x 64 crop of the Cornell reference render around the light, with isolated fireflies
+    and a saturated patch planted, under six settings."""
+    src = np.load(os.path.join(HERE, "cornell_300_50spp_seed1.npz"))["image"].astype(np.float64)
+    img = np.ascontiguousarray(src[0:64, 118:182])
+    h, w = img.shape[:2]
+    img[40, 10] = (30.0, 30.0, 30.0)      # isolated firefly (bloom bright-pass and display)
+    img[20, 50] = (5.0, 4.0, 3.0)         # bright outlier
+    img[50, 30] = (20.0, 20.0, 20.0)      # saturated display pixel ...
+    img[50, 31] = (20.0, 20.0, 20.0)      # ... with one saturated neighbour
+    img[10, 10] = (-1.0, np.nan, np.inf)  # non-finite and negative input
+    inp = os.path.join(tmp, "post_in.bin")
+    with open(inp, "wb") as fh:
+        fh.write(struct.pack("<II", w, h))
+        fh.write(img.astype("<f8").tobytes())
+    outs = []
+    for exposure, bloom, view, contrast in POST_CASES:
+        out = os.path.join(tmp, "post_out.bin")
+        run("post", inp, out, repr(exposure), bloom, view, repr(contrast))
+        data = np.fromfile(out, dtype="<f8", offset=8).reshape(h, w, 3)
+        outs.append(data)
+    np.savez_compressed(os.path.join(HERE, "post_cornell_64x64.npz"), image=img, outputs=np.stack(outs),
+                        exposure=np.array([c[0] for c in POST_CASES]), bloom=np.array([c[1] for c in POST_CASES]),
+                        view=np.array([c[2] for c in POST_CASES]), contrast=np.array([c[3] for c in POST_CASES]))
+    print("post", [float(np.abs(o).mean()) for o in outs])
+
+
 def fd_fixture(tmp, name="diff_fd_128_8spp_depth2", width=128, height=128, spp=8, depth=2, metals=False, h_albedo=0.01,
                h_emission=0.1, seed=1):
     """FP64 central finite differences of L = sum(w * image) on the reference renderer, with
@@ -265,6 +304,8 @@ def main():
             nfor_fixtures(tmp)
         if "adaptive" in only:
             adaptive_fixtures(tmp)
+        if "post" in only:
+            post_fixture(tmp)
         return
     sampler_fixture(tmp)
     small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
@@ -288,6 +329,7 @@ def main():
         run("tlas", c.scene_path, tl)
         fh.write(open(tl).read())
     primary_fixture("cornell", c.scene_path, 1, tmp)
+    post_fixture(tmp)
 
     # 512 displaced spheres: topology hashes, a reduced-resolution render, primary hits
     s = workloads.spheres_scene(os.path.join(tmp, "spheres"))
