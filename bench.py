@@ -110,22 +110,32 @@ def bench_batch(args, wdir):
     batch_ms = elapsed * 1000.0 / args.steps
     log(f"one at a time: megakernel {plain_ms:.0f} ms, wavefront {wave_ms:.0f} ms; batched {batch_ms:.0f} ms "
         f"({plain_ms / batch_ms:.2f}x / {wave_ms / batch_ms:.2f}x)")
-    # the small-scene regime: the same 32 scenes at 128x128 and 4 spp, where one scene cannot
-    # fill the GPU. Device time (HIP events) of the plain kernel per scene, summed, against the
-    # batched launch sequence's device time: launch gaps and host syncs are left out of both.
-    small = [(128, 128)] * len(wls)
-    small_spp = [4] * len(wls)
-    for _ in range(2):
-        _, sst = Renderer.render_batch(scenes, seeds, samples=small_spp, resolutions=small, tuning=tuning)
-    small_batch_ms = min(Renderer.render_batch(scenes, seeds, samples=small_spp, resolutions=small,
-                                               tuning=tuning)[1].render_ms for _ in range(3))
-    small_plain_ms = 0.0
-    for sc, sd in zip(scenes, seeds):
-        Renderer.render(sc, seed=sd, samples=4, resolution=(128, 128), integrator="megakernel")
-        small_plain_ms += min(Renderer.render(sc, seed=sd, samples=4, resolution=(128, 128),
-                                              integrator="megakernel")[1].render_ms for _ in range(3))
-    log(f"small scenes (32 x 128x128 x 4 spp): plain kernel one at a time {small_plain_ms:.2f} ms of device time, "
-        f"batched {small_batch_ms:.2f} ms ({small_plain_ms / small_batch_ms:.2f}x)")
+    # The small-scene regime: the same 32 scenes at sizes where one scene cannot fill the GPU.
+    # Device time (HIP events) of the plain kernel per scene, summed, against one batched
+    # render on the multi-scene wavefront and on the batched plain kernel (every scene's paths
+    # in one launch): launch gaps and host syncs are left out of all three.
+    small = []
+    for (sw, sh), sspp in (((128, 128), 4), ((64, 64), 1), ((32, 32), 1)):
+        n = len(wls)
+        kw = dict(samples=[sspp] * n, resolutions=[(sw, sh)] * n)
+        batched = {}
+        for integ in ("wavefront", "megakernel"):
+            for _ in range(2):
+                Renderer.render_batch(scenes, seeds, integrator=integ, **kw)
+            batched[integ] = min(Renderer.render_batch(scenes, seeds, integrator=integ, **kw)[1].render_ms for _ in range(3))
+        plain = 0.0
+        for sc, sd in zip(scenes, seeds):
+            Renderer.render(sc, seed=sd, samples=sspp, resolution=(sw, sh), integrator="megakernel")
+            plain += min(Renderer.render(sc, seed=sd, samples=sspp, resolution=(sw, sh),
+                                         integrator="megakernel")[1].render_ms for _ in range(3))
+        log(f"small scenes ({n} x {sw}x{sh} x {sspp} spp): plain kernel one at a time {plain:.3f} ms of device time; "
+            f"batched wavefront {batched['wavefront']:.3f} ms ({plain / batched['wavefront']:.2f}x), "
+            f"batched plain kernel {batched['megakernel']:.3f} ms ({plain / batched['megakernel']:.2f}x)")
+        small.append({"scenes": n, "resolution": f"{sw}x{sh}", "spp": sspp, "plain_megakernel_device_ms": round(plain, 3),
+                      "batched_wavefront_device_ms": round(batched["wavefront"], 3),
+                      "batched_plain_kernel_device_ms": round(batched["megakernel"], 3),
+                      "batched_wavefront_vs_plain": round(plain / batched["wavefront"], 3),
+                      "batched_plain_kernel_vs_plain": round(plain / batched["megakernel"], 3)})
     line = {
         "metric": "Msamples/s", "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(batch_ms, 3), "higher_is_better": True, "scaling": "weak",
@@ -135,10 +145,7 @@ def bench_batch(args, wdir):
                    "scenes": len(wls), "samples_per_step": total, "integrator": "wavefront, multi-scene, hipGraph passes"},
         "batched_vs_plain_megakernel": round(plain_ms / batch_ms, 3),
         "batched_vs_single_scene_wavefront": round(wave_ms / batch_ms, 3),
-        "small_scenes": {"scenes": len(wls), "resolution": "128x128", "spp": 4,
-                         "plain_megakernel_device_ms": round(small_plain_ms, 3),
-                         "batched_device_ms": round(small_batch_ms, 3),
-                         "batched_vs_plain_megakernel": round(small_plain_ms / small_batch_ms, 3)},
+        "small_scenes": small,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

@@ -208,9 +208,11 @@ int		luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* 
  * captured hipGraph. Scenes with adaptive sampling in effect are rendered in rounds over
  * their still-active pixels, all scenes' rounds sharing the passes. Replaces n concurrent Renderer::render calls, which the reference
  * cannot run (its render seed is global, Sampler.cpp:28). settings[i] (or NULL: scene
- * defaults) and rgb_out[i] are per scene, as for luzgpu_render; the integrator must be the
- * wavefront one, and launch tuning, visit counting and stage timing are taken from
- * settings[0]. stats, if given, receives one aggregate record for the batch. */
+ * defaults) and rgb_out[i] are per scene, as for luzgpu_render; launch tuning, visit
+ * counting and stage timing are taken from settings[0]. The integrator (the same for every
+ * scene) is the wavefront one, or the plain kernel: then every scene's paths run in one
+ * launch, one scene per wave, which suits scenes too small to fill the GPU alone (no
+ * adaptive sampling there). stats, if given, receives one aggregate record for the batch. */
 int		luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes, const luzgpu_settings* settings,
 			float* const* rgb_out, luzgpu_stats* stats);
 

@@ -212,6 +212,7 @@ namespace luzgpu
 	{
 		GPU_SKY_NONE = 0,
 		GPU_SKY_ENVIRONMENT = 1,
+		GPU_SKY_LINEAR = 2,
 	};
 
 	// Scalars of one render: all device pointers live in GpuSceneView.

@@ -134,6 +134,13 @@ namespace luzgpu
 	__device__ __forceinline__ f3	missRadiance(const SceneView& s, const GpuParams& p, f3 dir, float envWeight)
 	{
 		f3 sky;
+		if (p.sky == GPU_SKY_LINEAR)
+		{
+			// _calculateSkyInterpolation (Atmospherics.cpp:44-53), skyline 0.5 (Scene.cpp:117); no
+			// MIS weight: the linear sky is never sampled as a light (ColorHelper.cpp:1196-1198)
+			const float t = 0.5f * (dir.y + 1.0f);
+			return mk(1.0f - t + 0.5f * t, 1.0f - t + 0.7f * t, 1.0f - t + t);
+		}
 		if (p.sky == GPU_SKY_ENVIRONMENT && p.hasEnvironment && p.envStrength > 0.0f)
 		{
 			sky = envLookup(s, p, dir) * p.envStrength;

@@ -898,6 +898,11 @@ namespace luzgpu
 	{
 		PackedScene p;
 		const Settings& s = scene.settings;
+		if (s.sky == SKY_ATMOSPHERE)
+		{
+			throw std::runtime_error(std::string("Unsupported for the device path: the atmosphere sky")
+				+ (s.skySet ? "" : " (the default when the scene sets neither sky= nor environment=)"));
+		}
 		p.adaptive.enabled = s.adaptive;
 		p.adaptive.minSamples = s.adaptiveMinSamples;
 		p.adaptive.checkInterval = s.adaptiveCheckInterval;
@@ -1424,7 +1429,7 @@ namespace luzgpu
 		gp.height = s.height;
 		gp.maxBounces = s.maxBounces;
 		gp.totalSamples = s.samples;
-		gp.sky = s.sky == SKY_ENVIRONMENT ? GPU_SKY_ENVIRONMENT : GPU_SKY_NONE;
+		gp.sky = s.sky == SKY_ENVIRONMENT ? GPU_SKY_ENVIRONMENT : s.sky == SKY_LINEAR ? GPU_SKY_LINEAR : GPU_SKY_NONE;
 		for (int k = 0; k < 3; k++) gp.background[k] = static_cast<float>(s.background[k]);
 		gp.backgroundLuminance = static_cast<float>(luminance(s.background));
 		gp.lightCount = static_cast<std::uint32_t>(p.lights.size());

@@ -150,6 +150,8 @@ namespace luzgpu
 	{
 		SKY_NONE = 0,
 		SKY_ENVIRONMENT = 1,
+		SKY_LINEAR = 2,		// white-to-blue gradient (Atmospherics.cpp:44-53)
+		SKY_ATMOSPHERE = 3,	// the reference's default (Scene.cpp:116); outside the device path
 	};
 
 	struct	Camera
@@ -192,7 +194,7 @@ namespace luzgpu
 		bool	bloom = true;
 		double	exposure = 0.0;
 		double	contrast = 1.0;
-		SkyKind	sky = SKY_NONE;
+		SkyKind	sky = SKY_ATMOSPHERE;
 		bool	skySet = false;
 		dvec3	background{};
 		double	metersPerUnit = 1.0;

@@ -70,7 +70,7 @@ namespace luzgpu
 			}
 			if (t.size() < 2 || t.front() != '(' || t.back() != ')')
 			{
-				throw std::runtime_error("Unsupported " + what + " value (only (x,y,z) / acescg(...) are supported): " + text);
+				throw std::runtime_error("Unsupported " + what + " value (only (x,y,z), acescg(...), srgb(...) and linear_srgb(...) are supported): " + text);
 			}
 			const std::string body = t.substr(1, t.size() - 2);
 			dvec3 out{};
@@ -86,6 +86,58 @@ namespace luzgpu
 				start = comma + 1;
 			}
 			return (out);
+		}
+
+		// ColorManagement::acescgFromLinearSRGB (ColorManagement.cpp:222-225): the linear-sRGB
+		// to ACEScg matrix (:121-125) after clamping to non-negative finite values
+		dvec3	acescgFromLinearSRGB(const dvec3& c)
+		{
+			static const double M[3][3] = {
+				{0.6131324224, 0.3395380158, 0.0474166960},
+				{0.0701243808, 0.9163940113, 0.0134515239},
+				{0.0205876575, 0.1095745716, 0.8697854040}
+			};
+			auto nonNegative = [](double v) { return (std::isfinite(v) && v > 0.0) ? v : 0.0; };
+			const double r = nonNegative(c[0]), g = nonNegative(c[1]), b = nonNegative(c[2]);
+			dvec3 o;
+			for (int i = 0; i < 3; i++) o[i] = M[i][0] * r + M[i][1] * g + M[i][2] * b;
+			return (o);
+		}
+
+		// ColorManagement::srgbToLinear (ColorManagement.cpp:186-201)
+		double	srgbToLinear(double v)
+		{
+			if (!std::isfinite(v) || v <= 0.0) return (0.0);
+			if (v >= 1.0) return (1.0);
+			if (v <= 0.04045) return (v / 12.92);
+			return (std::pow((v + 0.055) / 1.055, 2.4));
+		}
+
+		// A colour value (Parsing.cpp:227-264): (r,g,b) and acescg(r,g,b) are ACEScg as given;
+		// srgb / display_srgb / display(r,g,b) are sRGB-encoded and linear_srgb / linearsrgb /
+		// lin_srgb(r,g,b) linear sRGB, both converted to ACEScg.
+		dvec3	parseColor(const std::string& text, const std::string& what)
+		{
+			const std::string t = trim(text);
+			const std::string lt = lower(t);
+			for (const char* name : {"srgb", "display_srgb", "display"})
+			{
+				const std::string prefix = std::string(name) + "(";
+				if (lt.rfind(prefix, 0) == 0)
+				{
+					const dvec3 e = parseTriple(t.substr(prefix.size() - 1), what);
+					return (acescgFromLinearSRGB(dvec3{srgbToLinear(e[0]), srgbToLinear(e[1]), srgbToLinear(e[2])}));
+				}
+			}
+			for (const char* name : {"linear_srgb", "linearsrgb", "lin_srgb"})
+			{
+				const std::string prefix = std::string(name) + "(";
+				if (lt.rfind(prefix, 0) == 0)
+				{
+					return (acescgFromLinearSRGB(parseTriple(t.substr(prefix.size() - 1), what)));
+				}
+			}
+			return (parseTriple(t, what));
 		}
 
 		bool	splitAssignment(const std::string& line, std::string& key, std::string& value)
@@ -267,6 +319,15 @@ namespace luzgpu
 					{
 						s.denoise = parseDouble(value, "denoise setting") != 0.0;
 					}
+					else if (key == "distanceblueness")
+					{
+						// Settings.cpp:646-655: 0 or 1; the renderer never reads it (ColorHelper.cpp:1155)
+						int v = 0;
+						if (std::sscanf(value.c_str(), "%d", &v) != 1 || (v != 0 && v != 1))
+						{
+							throw std::runtime_error("Invalid distanceblueness setting. Use distanceblueness=0 or distanceblueness=1.");
+						}
+					}
 					else if (key == "outputfilename")
 					{
 						// the output file is the caller's (image writers are outside the device path)
@@ -350,24 +411,18 @@ namespace luzgpu
 					}
 					else if (key == "sky")
 					{
+						// Settings.cpp:440-470
 						const std::string v = lower(value);
-						if (v == "none")
-						{
-							s.sky = SKY_NONE;
-						}
-						else if (v == "environment")
-						{
-							s.sky = SKY_ENVIRONMENT;
-						}
-						else
-						{
-							throw std::runtime_error("Unsupported sky for the device path: " + value);
-						}
+						if (v == "none") s.sky = SKY_NONE;
+						else if (v == "environment") s.sky = SKY_ENVIRONMENT;
+						else if (v == "linear") s.sky = SKY_LINEAR;
+						else if (v == "atmosphere") s.sky = SKY_ATMOSPHERE;
+						else throw std::runtime_error("Unknown sky setting: " + value);
 						s.skySet = true;
 					}
 					else if (key == "background" || key == "backgroundcolor" || key == "background_color")
 					{
-						s.background = parseTriple(value, "background color");
+						s.background = parseColor(value, "background color");
 					}
 					else if (key == "environment" || key == "environmentmap" || key == "environment_map")
 					{
@@ -580,7 +635,7 @@ namespace luzgpu
 						throw std::runtime_error("Unknown material property: " + line);
 					}
 					if (key == "type") type = lower(value);
-					else if (key == "color" || key == "basecolor" || key == "base_color") color = parseTriple(value, "material color");
+					else if (key == "color" || key == "basecolor" || key == "base_color") color = parseColor(value, "material color");
 					else if (key == "roughness") roughness = parseDouble(value, "roughness");
 					else if (key == "fuzz") fuzz = parseDouble(value, "fuzz");
 					else if (key == "ior" || key == "refractiveindex" || key == "refractive_index") ior = parseDouble(value, "ior");
@@ -598,7 +653,7 @@ namespace luzgpu
 					else if (key == "alpha") alphaKey = parseDouble(value, key);
 					else if (key == "glossy_color" || key == "glossycolor" || key == "specular_color" || key == "specularcolor")
 					{
-						glossyColor = parseTriple(value, "glossy color");
+						glossyColor = parseColor(value, "glossy color");
 						hasGlossyColor = true;
 					}
 					else if (key == "glossy_weight" || key == "glossyweight" || key == "glossy_mix" || key == "glossymix")
@@ -878,7 +933,7 @@ namespace luzgpu
 					else if (key == "radius") v.radius = parseDouble(value, key);
 					else if (key == "density" || key == "extinction" || key == "sigma_t") density = parseDouble(value, key);
 					else if (key == "color" || key == "albedo" || key == "scatteringcolor" || key == "scattering_color")
-						color = parseTriple(value, "volume color");
+						color = parseColor(value, "volume color");
 					else if (key == "anisotropy" || key == "g") g = parseDouble(value, key);
 					else if (key == "material")
 					{
@@ -949,7 +1004,7 @@ namespace luzgpu
 						throw std::runtime_error("Invalid directional light property: " + t);
 					}
 					if (key == "direction") direction = parseTriple(value, "directional light direction");
-					else if (key == "color") color = parseTriple(value, "directional light color");
+					else if (key == "color") color = parseColor(value, "directional light color");
 					else if (key == "irradiance" || key == "w_m2" || key == "wm2" || key == "illuminance" || key == "lux" || key == "lx")
 					{
 						const double v = parseDouble(value, key);

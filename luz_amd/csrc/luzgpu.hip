@@ -162,6 +162,115 @@ namespace luzgpu
 		else launchMegaC<false>(features, grid, st, s, p, list, pixels, colors, counters);
 	}
 
+	// One scene of a batched plain-kernel launch. Its (pixel, sample) threads start on a wave
+	// boundary, so every wave renders one scene and reads that scene's parameters with
+	// scalar loads.
+	struct	MegaJob
+	{
+		GpuParams	params;			// sampleBegin / sampleCount: this launch's slice
+		std::uint32_t	firstWave;	// first wave of the slice in the launch
+		std::uint32_t	pixels;
+		std::uint32_t	count;		// pixels * sampleCount
+		std::uint32_t	pixelBase;	// first pixel of the scene in the launch's pixel numbering
+		std::uint64_t	colorBase;	// first sample colour of the slice
+		double*	sums;			// the scene's FP64 pixel sums
+	};
+
+	// Several scenes in one launch (render_batch on the plain kernel): small scenes that
+	// cannot fill the GPU alone run side by side, each path still in one thread.
+	template <bool Count, int Feat>
+	__global__ __launch_bounds__(kMegaBlock) void	render_mega_batch(SceneView s, const MegaJob* __restrict__ jobs, std::uint32_t jobCount,
+		float* __restrict__ sampleColors, unsigned long long* __restrict__ counters)
+	{
+		__shared__ std::uint32_t stackMem[kStackDepth * kMegaBlock];
+		const std::uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kMegaBlock + threadIdx.x) >> 6);
+		std::uint32_t lo = 0, hi = jobCount;
+		while (hi - lo > 1u)
+		{
+			const std::uint32_t mid = (lo + hi) >> 1;
+			if (jobs[mid].firstWave <= wave) lo = mid;
+			else hi = mid;
+		}
+		const MegaJob& J = jobs[__builtin_amdgcn_readfirstlane(lo)];
+		const std::uint32_t local = (wave - J.firstWave) * 64u + (threadIdx.x & 63u);
+		std::uint32_t closest = 0, shadow = 0;
+		VisitCounter visits{0u, 0u};
+		if (wave >= J.firstWave && local < J.count)
+		{
+			const GpuParams& p = J.params;
+			const std::uint32_t slot = local / J.pixels, pixel = local % J.pixels;
+			LdsStack<kStackDepth, kMegaBlock> stack{stackMem + threadIdx.x, 0};
+			const f3 c = cleanColor(tracePath<Count, Feat>(s, p, pixel % p.width, pixel / p.width, p.sampleBegin + slot, stack, closest,
+				shadow, visits));
+			float* out = sampleColors + (J.colorBase + local) * 3;
+			out[0] = c.x;
+			out[1] = c.y;
+			out[2] = c.z;
+		}
+		for (int off = 32; off > 0; off >>= 1)
+		{
+			closest += __shfl_down(closest, off);
+			shadow += __shfl_down(shadow, off);
+			if (Count)
+			{
+				visits.nodes += __shfl_down(visits.nodes, off);
+				visits.prims += __shfl_down(visits.prims, off);
+			}
+		}
+		if ((threadIdx.x & 63) == 0)
+		{
+			atomicAdd(counters + 0, static_cast<unsigned long long>(closest));
+			atomicAdd(counters + 1, static_cast<unsigned long long>(shadow));
+			if (Count)
+			{
+				atomicAdd(counters + 2, static_cast<unsigned long long>(visits.nodes));
+				atomicAdd(counters + 3, static_cast<unsigned long long>(visits.prims));
+			}
+		}
+	}
+
+	// Adds a batched launch's sample colours into every scene's FP64 pixel sums, in sample
+	// order: one thread per pixel of the launch.
+	__global__ void	accumulate_batch(const float* __restrict__ colors, const MegaJob* __restrict__ jobs, std::uint32_t jobCount,
+		std::uint32_t totalPixels)
+	{
+		const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+		if (i >= totalPixels) return;
+		std::uint32_t lo = 0, hi = jobCount;
+		while (hi - lo > 1u)
+		{
+			const std::uint32_t mid = (lo + hi) >> 1;
+			if (jobs[mid].pixelBase <= i) lo = mid;
+			else hi = mid;
+		}
+		const MegaJob& J = jobs[lo];
+		const std::uint32_t px = i - J.pixelBase;
+		double r = J.sums[3 * static_cast<size_t>(px)], g = J.sums[3 * static_cast<size_t>(px) + 1], b = J.sums[3 * static_cast<size_t>(px) + 2];
+		for (std::uint32_t k = 0; k < J.params.sampleCount; k++)
+		{
+			const float* c = colors + (J.colorBase + static_cast<std::uint64_t>(k) * J.pixels + px) * 3;
+			r += static_cast<double>(c[0]);
+			g += static_cast<double>(c[1]);
+			b += static_cast<double>(c[2]);
+		}
+		J.sums[3 * static_cast<size_t>(px)] = r;
+		J.sums[3 * static_cast<size_t>(px) + 1] = g;
+		J.sums[3 * static_cast<size_t>(px) + 2] = b;
+	}
+
+	template <bool Count>
+	void	launchMegaBatchC(int features, size_t grid, hipStream_t st, const SceneView& s, const MegaJob* jobs, std::uint32_t n,
+		float* colors, unsigned long long* counters)
+	{
+		switch (features & kFeatAll)
+		{
+			case 0: render_mega_batch<Count, 0><<<grid, kMegaBlock, 0, st>>>(s, jobs, n, colors, counters); break;
+			case kFeatAnalytic: render_mega_batch<Count, kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, jobs, n, colors, counters); break;
+			case kFeatRich: render_mega_batch<Count, kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, jobs, n, colors, counters); break;
+			default: render_mega_batch<Count, kFeatAll><<<grid, kMegaBlock, 0, st>>>(s, jobs, n, colors, counters); break;
+		}
+	}
+
 	// The plain kernel with denoising on: each (pixel, sample) also writes its primary
 	// features (kNforFeatures floats, RenderSample of _calculatePixelSample, ColorHelper.cpp:1119-1126).
 	template <int Feat>
@@ -517,6 +626,8 @@ struct luzgpu_ctx
 	luzgpu::DeviceBuffer	adaptiveMem;
 	// display post-processing: the FP64 mean image and three working images
 	luzgpu::DeviceBuffer	postMem;
+	// batched plain-kernel launches: their job tables
+	luzgpu::DeviceBuffer	megaJobs;
 	// per-pixel sample counts of the last single-scene render: `countsUniform` samples each,
 	// or (countsUniform == 0) the adaptive rounds' counts in adaptiveMem
 	std::uint32_t	countsPixels = 0;
@@ -1749,9 +1860,13 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 		for (uint32_t i = 0; i < n; i++)
 		{
 			const luzgpu_settings* si = settings ? settings + i : nullptr;
-			if (si && si->integrator != LUZGPU_INTEGRATOR_WAVEFRONT)
+			if (si && si->integrator != (settings ? settings[0].integrator : 0u))
 			{
-				throw Error{LUZGPU_E_INVALID, "luzgpu_render_batch: batches use the wavefront integrator"};
+				throw Error{LUZGPU_E_INVALID, "luzgpu_render_batch: every scene of a batch uses the same integrator"};
+			}
+			if (si && si->integrator > LUZGPU_INTEGRATOR_MEGAKERNEL)
+			{
+				throw Error{LUZGPU_E_INVALID, "luzgpu_render_batch: unknown integrator"};
 			}
 			const GpuParams p = resolveParams(ctx->batchParams[i], scenes[i]->gpu.host, si);
 			if (p.sampleBegin + p.sampleCount > p.totalSamples || p.sampleCount == 0 || p.maxBounces < 0)
@@ -1783,7 +1898,7 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 		for (uint32_t i = 0; i < n; i++) jobs[i].sums = ctx->batchSums.as<double>() + sumOffset[i];
 		WaveTuning tune = tuningOf(settings).resolved();
 		double tuneMs = 0.0;
-		if (settings && settings[0].autotune)
+		if (settings && settings[0].autotune && settings[0].integrator == LUZGPU_INTEGRATOR_WAVEFRONT)
 		{
 			luzgpu_tuning r{};
 			const size_t before = ctx->tuned.size();
@@ -1801,7 +1916,69 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 		std::uint64_t rendered = 0;
 		for (const BatchJob& j : jobs) rendered += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
 		AdaptiveArrays A{};
-		if (adaptive)
+		const bool plainKernel = settings && settings[0].integrator == LUZGPU_INTEGRATOR_MEGAKERNEL;
+		if (plainKernel && adaptive)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render_batch: adaptive sampling in a batch runs on the wavefront integrator"};
+		}
+		if (plainKernel)
+		{
+			// launches of at most 2^26 paths, each giving every scene with samples left a slice
+			constexpr std::uint64_t budget = 1ull << 26;
+			std::vector<std::uint32_t> done(n, 0);
+			std::uint32_t totalPixels = 0;
+			for (uint32_t i = 0; i < n; i++) totalPixels += jobs[i].params.width * jobs[i].params.height;
+			ctx->megaJobs.reserve(n * sizeof(MegaJob));
+			for (;;)
+			{
+				std::vector<MegaJob> table;
+				std::uint64_t paths = 0;
+				std::uint32_t waves = 0, pixelBase = 0;
+				for (uint32_t i = 0; i < n; i++)
+				{
+					const GpuParams& p = jobs[i].params;
+					const std::uint32_t pixels = p.width * p.height;
+					MegaJob m{};
+					m.params = p;
+					m.pixels = pixels;
+					m.pixelBase = pixelBase;
+					m.sums = jobs[i].sums;
+					pixelBase += pixels;
+					const std::uint64_t room = budget > paths ? (budget - paths) / pixels : 0;
+					const std::uint32_t c = static_cast<std::uint32_t>(std::min<std::uint64_t>(p.sampleCount - done[i], room));
+					m.params.sampleBegin = p.sampleBegin + done[i];
+					m.params.sampleCount = c;
+					m.count = pixels * c;
+					m.firstWave = waves;
+					m.colorBase = paths;
+					waves += (m.count + 63u) / 64u;
+					paths += m.count;
+					done[i] += c;
+					table.push_back(m);
+				}
+				if (paths == 0) break;
+				ctx->sampleColors.reserve(static_cast<size_t>(paths) * 3 * sizeof(float));
+				check(hipMemcpyAsync(ctx->megaJobs.ptr, table.data(), table.size() * sizeof(MegaJob), hipMemcpyHostToDevice, st),
+					"batch job table");
+				const size_t grid = (static_cast<size_t>(waves) * 64 + kMegaBlock - 1) / kMegaBlock;
+				if (settings[0].count_visits)
+				{
+					launchMegaBatchC<true>(ctx->batch->host.features, grid, st, ctx->batch->view, ctx->megaJobs.as<MegaJob>(), n,
+						ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>());
+				}
+				else
+				{
+					launchMegaBatchC<false>(ctx->batch->host.features, grid, st, ctx->batch->view, ctx->megaJobs.as<MegaJob>(), n,
+						ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>());
+				}
+				check(hipGetLastError(), "batch render launch");
+				accumulate_batch<<<blocks(totalPixels, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->megaJobs.as<MegaJob>(), n,
+					totalPixels);
+				// the table is rewritten by the next launch's copy: wait for this one
+				check(hipStreamSynchronize(st), "batch launch");
+			}
+		}
+		else if (adaptive)
 		{
 			A = adaptiveArrays(ctx, pixelTotal, n);
 			rendered = runAdaptive(ctx, aj, A, [&](const std::vector<AdaptiveRound>& work) {

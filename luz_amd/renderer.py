@@ -289,12 +289,14 @@ class Renderer:
 
     @staticmethod
     def render_batch(scenes, seeds, samples=None, max_bounces=None, outs=None, resolutions=None, count_visits=False,
-                     stage_times=False, tuning=None, adaptive=None):
+                     stage_times=False, tuning=None, adaptive=None, integrator: str = "wavefront"):
         """Renders several scenes in one multi-scene launch sequence (luzgpu_render_batch).
 
         ``seeds``, ``samples``, ``max_bounces``, ``resolutions`` and ``adaptive`` are
         per-scene lists (or None for the scene defaults); ``outs`` optional per-scene device
-        tensors. Returns (images, aggregate RenderStats)."""
+        tensors. ``integrator``: "wavefront" (multi-scene passes) or "megakernel" (every
+        scene's paths in one plain-kernel launch, for scenes too small to fill the GPU).
+        Returns (images, aggregate RenderStats)."""
         n = len(scenes)
         if n == 0:
             raise ValueError("render_batch needs at least one scene")
@@ -304,7 +306,7 @@ class Renderer:
         for i, sc in enumerate(scenes):
             res = resolutions[i] if resolutions else None
             arr[i] = _settings(seeds[i], samples[i] if samples else None, max_bounces[i] if max_bounces else None, 0, None,
-                               "wavefront", on_device=outs is not None, resolution=res,
+                               integrator, on_device=outs is not None, resolution=res,
                                count_visits=count_visits, stage_times=stage_times, tuning=tuning,
                                adaptive=adaptive[i] if adaptive else None)
             w, h = res if res is not None else (sc.width, sc.height)
