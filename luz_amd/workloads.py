@@ -333,6 +333,25 @@ def primitives_scene_text(width=160, height=120, spp=32, depth=8) -> str:
     return "\n".join(lines) + "\n"
 
 
+def skylinear_scene_text(width=128, height=96, spp=16, depth=6) -> str:
+    """The primitives scene under the linear gradient sky, with colours given as sRGB-encoded,
+    display and linear-sRGB values (converted to ACEScg by the loader)."""
+    text = primitives_scene_text(width, height, spp, depth)
+    text = text.replace("sky=none\nbackground=(0.05,0.06,0.08)\n", "sky=linear\ndistanceblueness=0\n")
+    text = text.replace("color=(0.6,0.6,0.6)", "color=srgb(0.8,0.78,0.75)")
+    text = text.replace("color=(0.7,0.15,0.1)", "color=linear_srgb(0.7,0.15,0.1)")
+    text = text.replace("color=(0.85,0.8,0.7)", "color=display_srgb(0.9,0.85,0.7)")
+    text = text.replace("color=(0.6,0.7,1)", "color=lin_srgb(0.6,0.7,1)")
+    return text
+
+
+def write_skylinear(directory: str, width=128, height=96, spp=16, depth=6, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "skylinear.luz")
+    _write_if_changed(path, skylinear_scene_text(width, height, spp, depth).encode())
+    return Workload("skylinear", directory, path, width, height, spp, depth, seed, 0)
+
+
 _QUAD_OBJ = "v -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\nf 1 2 3\nf 1 3 4\n"
 
 

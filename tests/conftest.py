@@ -48,6 +48,12 @@ def meshlight(workload_dir):
 
 
 @pytest.fixture(scope="session")
+def skylinear(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_skylinear(os.path.join(workload_dir, "skylinear"))
+
+
+@pytest.fixture(scope="session")
 def volume(workload_dir):
     from luz_amd import workloads
     return workloads.write_volume(os.path.join(workload_dir, "volume"))

@@ -294,6 +294,8 @@ def main():
             small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
         if "materials" in only:
             small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
+        if "skylinear" in only:
+            small_scene_fixtures("skylinear", workloads.write_skylinear(os.path.join(tmp, "skylinear")), tmp)
         if "meshlight" in only:
             small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
         if "fd" in only:
@@ -312,6 +314,7 @@ def main():
     small_scene_fixtures("volume", workloads.write_volume(os.path.join(tmp, "volume")), tmp)
     small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
     small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
+    small_scene_fixtures("skylinear", workloads.write_skylinear(os.path.join(tmp, "skylinear")), tmp)
     fd_fixture(tmp)
     zstat_fixtures(tmp)
     nfor_fixtures(tmp)

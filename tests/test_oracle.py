@@ -89,11 +89,13 @@ def test_ggx_reflection_pdf_mass():
         assert 0.5 < lib.oracle_ggx_pdf_integral(alpha, mu, 800) <= 1.0
 
 
-def test_oracle_primitive_volume_material_images_match_reference(primitives, volume, materials, meshlight):
-    """Spheres, cubes, planes, Henyey-Greenstein / isotropic media, delta lights and the
-    glossy / diffuse-glossy / principled BSDFs: the restatement reproduces the reference's
-    FP64 renders and brute-force primary hits."""
-    for name, wl in (("primitives", primitives), ("volume", volume), ("materials", materials), ("meshlight", meshlight)):
+def test_oracle_primitive_volume_material_images_match_reference(primitives, volume, materials, meshlight, skylinear):
+    """Spheres, cubes, planes, Henyey-Greenstein / isotropic media, delta lights, the
+    glossy / diffuse-glossy / principled BSDFs, and the linear sky with sRGB / linear-sRGB
+    colour values: the restatement reproduces the reference's FP64 renders and brute-force
+    primary hits."""
+    for name, wl in (("primitives", primitives), ("volume", volume), ("materials", materials), ("meshlight", meshlight),
+                     ("skylinear", skylinear)):
         ref = np.load(os.path.join(GOLDEN, f"{name}_{wl.width}x{wl.height}_{wl.spp}spp_seed{wl.seed}.npz"))["image"]
         img, _ = oracle_lib.render(wl.scene_path, wl.seed, wl.spp, wl.width, wl.height)
         e = rel_mse(img, ref)
