@@ -922,6 +922,7 @@ namespace oracle
 				e.weights.assign(e.W * e.H, 0.0);
 			}
 		}
+		if (in.settings.sky == luzgpu::SKY_ATMOSPHERE) throw std::runtime_error("the atmosphere sky is outside the oracle");
 		s.sky = in.settings.sky;
 		s.background = fromC(in.settings.background);
 		s.maxBounces = in.settings.maxBounces;
@@ -2210,7 +2211,13 @@ namespace oracle
 					w = power(prevPdf, hasEnvLight(s) ? envPdf(s.env, ray.d) : (hasConstBg(s) ? 1.0 / (4.0 * PI) : 0.0));
 				}
 				C sky;
-				if (s.sky == luzgpu::SKY_ENVIRONMENT && s.hasEnv && s.env.strength > 0.0) sky = envLookup(s.env, ray.d) * s.env.strength * w;
+				if (s.sky == luzgpu::SKY_LINEAR)
+				{
+					// _calculateSkyInterpolation (Atmospherics.cpp:44-53): skyline 0.5, no MIS weight
+					const double t = 0.5 * (normalize(ray.d).y + 1.0);
+					sky = C{1.0, 1.0, 1.0} * (1.0 - t) + C{0.5, 0.7, 1.0} * t;
+				}
+				else if (s.sky == luzgpu::SKY_ENVIRONMENT && s.hasEnv && s.env.strength > 0.0) sky = envLookup(s.env, ray.d) * s.env.strength * w;
 				else sky = s.background * w;
 				C c = thr * sky;
 				if (b > 0 && hasConstBg(s)) c = clampBg(s, c);
