@@ -61,6 +61,29 @@ def cpu_baseline(scene_path: str, seed: int, spp: int, threads: int):
     }
 
 
+def cpu_nfor_baseline(scene_path: str, threads: int, width: int, height: int, full_pixels: int):
+    """The reference's own Denoise::applyNFOR (regression radius 9) on the host cores, on a
+    bounded crop of the same scene (2 spp half-buffers from the reference's denoising
+    render, untimed): its time per pixel, scaled to the full frame (the regression's cost is
+    linear in the pixel count)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "luzref")
+    if not os.path.exists(exe):
+        return None
+    out = os.path.join(tempfile.mkdtemp(prefix="luz_nfor_"), "nfor.bin")
+    proc = subprocess.run([exe, "denoise", scene_path, "1", str(threads), out, "2", str(width), str(height), "1", "9", "1"],
+                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=900)
+    text = proc.stdout.decode(errors="replace")
+    if proc.returncode != 0 or "nfor_ms=" not in text:
+        log("cpu NFOR baseline failed:", text[-500:])
+        return None
+    ms = float(text.split("nfor_ms=")[1].split()[0])
+    frame_ms = ms * full_pixels / (width * height)
+    log(f"cpu NFOR baseline: {width}x{height} in {ms:.0f} ms on {threads} threads -> {frame_ms:.0f} ms per full frame")
+    return {"nfor_ms_per_frame": round(frame_ms, 1), "cores": threads, "kind": "reference",
+            "sample": f"reference Denoise::applyNFOR, regression radius 9, on a {width}x{height} render of the same scene "
+                      f"(2 spp), --threads {threads}, scaled by pixel count to the full frame"}
+
+
 def load_traffic(workload: str, kernel: str):
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
     if not os.path.exists(path):
@@ -376,6 +399,9 @@ def main():
             f"NFOR {dst.denoise_ms:.1f} ms")
         line["denoise"] = {"nfor_ms": round(dst.denoise_ms, 3), "render_ms": round(dst.render_ms, 3),
                            "regression_radius": 9, "patch_radius": 1, "feature_prefilter_radius": 1}
+        if not args.no_cpu_baseline:
+            line["denoise"]["cpu_baseline"] = cpu_nfor_baseline(wl.scene_path, min(16, os.cpu_count() or 1), 320, 180,
+                                                                wl.width * wl.height)
     scene.release()
     ctx.close()
     if not args.no_cpu_baseline:

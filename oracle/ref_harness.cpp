@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <atomic>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -264,7 +265,10 @@ namespace
 		settings.patchRadius = static_cast<unsigned int>(std::stoul(argv[11]));
 		settings.threadCount = std::max<std::size_t>(1, threads);
 		const Denoise::NFORBuffers& b = *scene.getDenoiseBuffers();
+		const auto nforStart = std::chrono::steady_clock::now();
 		std::unique_ptr<Image> denoised = Denoise::applyNFOR(b, settings);
+		std::cout << "nfor_ms=" << std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - nforStart).count()
+			<< std::endl;
 		std::ofstream out(argv[5], std::ios::binary);
 		put(out, static_cast<std::uint32_t>(w));
 		put(out, static_cast<std::uint32_t>(h));
