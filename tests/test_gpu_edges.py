@@ -1,0 +1,56 @@
+"""Edge cases of the render entry points on the device (GPU), against the oracle (the FP64
+restatement pinned to the reference's renders): single-pixel and single-row images, no
+bounce (camera rays only: emitters and sky), one sample, odd sizes that leave partial waves,
+both integrators; and the argument errors the entry points must raise instead of running."""
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from conftest import rel_mse
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((1, 1), 4, 8), ((7, 1), 3, 8), ((1, 9), 2, 8), ((33, 17), 1, 8), ((64, 48), 8, 0), ((31, 29), 5, 1)]
+
+
+@pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("res,spp,depth", CASES)
+def test_small_shapes_match_oracle(ctx, primitives, res, spp, depth, integrator):
+    from luz_amd import Renderer, SceneFile
+    scene = SceneFile.read(ctx, primitives.scene_path)
+    img, st = Renderer.render(scene, seed=5, samples=spp, max_bounces=depth, resolution=res, integrator=integrator)
+    ref, _ = oracle_lib.render(primitives.scene_path, 5, spp, res[0], res[1], max_bounces=depth)
+    assert img.shape == (res[1], res[0], 3)
+    assert st.rendered_samples == res[0] * res[1] * spp
+    assert np.isfinite(img).all()
+    e = rel_mse(img, ref) if np.abs(ref).sum() > 0 else float(np.abs(img).max())
+    print(f"{res} x {spp} spp, depth {depth} [{integrator}]: rel-MSE {e:.3e}")
+    assert e <= 1e-4
+
+
+def test_depth_zero_casts_one_closest_ray_per_path(ctx, primitives):
+    """maxlightbounces=0: the bounce loop runs once (ColorHelper.cpp:1165), so one
+    closest-hit query per path; its light connections still cast shadow rays."""
+    from luz_amd import Renderer, SceneFile
+    scene = SceneFile.read(ctx, primitives.scene_path)
+    for integrator in ("megakernel", "wavefront"):
+        _, st = Renderer.render(scene, seed=1, samples=2, max_bounces=0, resolution=(40, 30), integrator=integrator)
+        assert st.closest_hit_rays == 40 * 30 * 2
+        assert 0 < st.shadow_rays <= 2 * 40 * 30 * 2
+
+
+def test_argument_errors(ctx, primitives):
+    from luz_amd import Renderer, SceneFile, _lib
+    scene = SceneFile.read(ctx, primitives.scene_path)
+    with pytest.raises(_lib.LuzGpuError):
+        Renderer.render(scene, seed=1, samples=8, sample_begin=6, sample_count=4)  # beyond the total
+    with pytest.raises(_lib.LuzGpuError):
+        Renderer.render(scene, seed=1, samples=8, sample_begin=0, sample_count=0)  # empty range
+    with pytest.raises(_lib.LuzGpuError):
+        Renderer.render_denoised(scene, seed=1, samples=4, resolution=(16, 16), nfor={"regression_radius": 0})
+    with pytest.raises(_lib.LuzGpuError):
+        Renderer.postprocess(ctx, np.zeros((4, 4, 3)), {"contrast": -1.0})
+    # the context stays usable after every refused call
+    img, _ = Renderer.render(scene, seed=1, samples=1, resolution=(8, 8))
+    assert np.isfinite(img).all()
