@@ -157,21 +157,30 @@ namespace luzgpu
 		return volumeLogU(p, ps);
 	}
 
-	__global__ __launch_bounds__(kWaveBlock) void	wf_raygen(const PassJob* jobs, std::uint32_t job, WaveBuffers w)
+	// Camera rays of every job of the pass in one launch (queue 0 holds the pass's paths in
+	// job order, counts[0] of them).
+	__global__ __launch_bounds__(kWaveBlock) void	wf_raygen(const PassJob* jobs, std::uint32_t jobCount, WaveBuffers w)
 	{
-		const PassJob& J = jobs[job];
-		const GpuParams& p = J.params;
-		const std::uint32_t pixels = J.pixels;
-		const std::uint32_t base = J.pathBase;
-		const std::uint32_t n = pixels * p.sampleCount;
-		for (std::uint32_t id = blockIdx.x * kWaveBlock + threadIdx.x; id < n; id += gridDim.x * kWaveBlock)
+		const std::uint32_t total = w.counts[0];
+		for (std::uint32_t g = blockIdx.x * kWaveBlock + threadIdx.x; g < total; g += gridDim.x * kWaveBlock)
 		{
+			std::uint32_t lo = 0, hi = jobCount;	// the last job whose slice starts at or before g
+			while (hi - lo > 1u)
+			{
+				const std::uint32_t mid = (lo + hi) >> 1;
+				if (jobs[mid].pathBase <= g) lo = mid;
+				else hi = mid;
+			}
+			const std::uint32_t job = lo;
+			const PassJob& J = jobs[job];
+			const GpuParams& p = J.params;
+			const std::uint32_t pixels = J.pixels;
+			const std::uint32_t id = g - J.pathBase;
 			const std::uint32_t pixel = passPixel(J, id % pixels), slot = id / pixels;
 			const std::uint32_t x = pixel % p.width, y = pixel / p.width;
 			PixelSampler ps;
 			ps.begin(p.renderSeed, x, y, p.sampleBegin + slot, sampleStream(p, p.sampleBegin + slot));
 			const Ray r = cameraRay(p, x, y, ps);
-			const std::uint32_t g = base + id;
 			const float logU = p.hasVolumes ? volumeLogU(p, ps) : 0.0f;
 			w.qOrigin[0][g] = make_float4(r.o.x, r.o.y, r.o.z, logU);
 			w.qDir[0][g] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(kNoOrigin.hittable));
@@ -273,6 +282,13 @@ namespace luzgpu
 		SpillStack<LdsDepth, kWaveBlock> stack{stackMem + threadIdx.x, src.spill + gtid, gridDim.x * kWaveBlock, 0,
 			kWaveStackCapacity};
 		const std::uint32_t n = src.count();
+		// Waves beyond what the queue can feed leave before touching the claim counter: a
+		// whole grid of claims on one counter costs ~50 us when the queue is short. The waves
+		// that stay claim until the queue is exhausted, so every ray is still traced.
+		if (static_cast<std::uint64_t>(gtid >> 6) * chunk >= n)
+		{
+			return;
+		}
 		WaveFeed feed{0u, 0u, false};
 		std::uint32_t rays = 0;
 		VisitCounter visits{0u, 0u};
@@ -970,10 +986,7 @@ namespace luzgpu
 		};
 		const std::uint32_t vol = volumes ? 1u : 0u;
 		wf_zero_counts<<<1, 4 * kMaxBounceSlots, 0, stream>>>(w.counts, 4 * kMaxBounceSlots, st.jobs, jobCount);
-		for (std::uint32_t j = 0; j < jobCount; j++)
-		{
-			wf_raygen<<<kWaveGrid, kWaveBlock, 0, stream>>>(st.jobs, j, w);
-		}
+		wf_raygen<<<kWaveGrid, kWaveBlock, 0, stream>>>(st.jobs, jobCount, w);
 		for (int b = 0; b <= maxBounces; b++)
 		{
 			mark();
