@@ -1929,9 +1929,17 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 			std::uint32_t totalPixels = 0;
 			for (uint32_t i = 0; i < n; i++) totalPixels += jobs[i].params.width * jobs[i].params.height;
 			ctx->megaJobs.reserve(n * sizeof(MegaJob));
+			std::uint64_t totalPaths = 0;
+			for (uint32_t i = 0; i < n; i++) totalPaths += static_cast<std::uint64_t>(jobs[i].params.width) * jobs[i].params.height * jobs[i].params.sampleCount;
+			// no launch holds more than the budget: one allocation before the first launch
+			ctx->sampleColors.reserve(static_cast<size_t>(std::min(totalPaths, budget)) * 3 * sizeof(float));
+			// every launch's host table stays alive until the call's final synchronisation (the
+			// copies are stream-ordered after the previous launch's kernels)
+			std::vector<std::vector<MegaJob>> tables;
 			for (;;)
 			{
-				std::vector<MegaJob> table;
+				tables.emplace_back();
+				std::vector<MegaJob>& table = tables.back();
 				std::uint64_t paths = 0;
 				std::uint32_t waves = 0, pixelBase = 0;
 				for (uint32_t i = 0; i < n; i++)
@@ -1957,7 +1965,6 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 					table.push_back(m);
 				}
 				if (paths == 0) break;
-				ctx->sampleColors.reserve(static_cast<size_t>(paths) * 3 * sizeof(float));
 				check(hipMemcpyAsync(ctx->megaJobs.ptr, table.data(), table.size() * sizeof(MegaJob), hipMemcpyHostToDevice, st),
 					"batch job table");
 				const size_t grid = (static_cast<size_t>(waves) * 64 + kMegaBlock - 1) / kMegaBlock;
@@ -1974,8 +1981,6 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 				check(hipGetLastError(), "batch render launch");
 				accumulate_batch<<<blocks(totalPixels, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->megaJobs.as<MegaJob>(), n,
 					totalPixels);
-				// the table is rewritten by the next launch's copy: wait for this one
-				check(hipStreamSynchronize(st), "batch launch");
 			}
 		}
 		else if (adaptive)
