@@ -175,8 +175,25 @@ namespace luzgpu
 			const PassJob& J = jobs[job];
 			const GpuParams& p = J.params;
 			const std::uint32_t pixels = J.pixels;
+			// queue entry g holds the job's id-th camera ray; the queue visits the image in
+			// bands of 8 rows, column by column, so a wave's 64 rays cover an 8 x 8 block
+			// (coherent first traversals). The path keeps its row-major id, which the later
+			// stages and the accumulation decode.
 			const std::uint32_t id = g - J.pathBase;
-			const std::uint32_t pixel = passPixel(J, id % pixels), slot = id / pixels;
+			const std::uint32_t slot = id / pixels, order = id % pixels;
+			std::uint32_t pixel;
+			if (J.pixelList)
+			{
+				pixel = J.pixelList[order];
+			}
+			else
+			{
+				const std::uint32_t band = order / (8u * p.width);
+				const std::uint32_t bandRows = min(8u, p.height - band * 8u);
+				const std::uint32_t within = order - band * 8u * p.width;
+				pixel = (band * 8u + within % bandRows) * p.width + within / bandRows;
+			}
+			const std::uint32_t path = J.pathBase + slot * pixels + (J.pixelList ? order : pixel);
 			const std::uint32_t x = pixel % p.width, y = pixel / p.width;
 			PixelSampler ps;
 			ps.begin(p.renderSeed, x, y, p.sampleBegin + slot, sampleStream(p, p.sampleBegin + slot));
@@ -185,9 +202,9 @@ namespace luzgpu
 			w.qOrigin[0][g] = make_float4(r.o.x, r.o.y, r.o.z, logU);
 			w.qDir[0][g] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(kNoOrigin.hittable));
 			w.qFrom[0][g] = kNoOrigin.tri;
-			w.qPath[0][g] = (job << kPathBits) | g;
-			w.throughput[g] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-			w.radiance[g] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(PF_PREV_SPECULAR));
+			w.qPath[0][g] = (job << kPathBits) | path;
+			w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+			w.radiance[path] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(PF_PREV_SPECULAR));
 		}
 	}
 
