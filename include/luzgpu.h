@@ -34,7 +34,11 @@ extern "C" {
 #define LUZGPU_E_IO (-5)           /* file missing or malformed (reference: runtime_error) */
 
 #define LUZGPU_INTEGRATOR_WAVEFRONT 0u
-#define LUZGPU_INTEGRATOR_MEGAKERNEL 1u
+#define LUZGPU_INTEGRATOR_MEGAKERNEL 1u   /* the plain kernel: one thread per path                */
+#define LUZGPU_INTEGRATOR_AUTO 2u         /* the plain kernel for calls of at most 4M paths        *
+                                           * (pixels x samples, summed over a batch's scenes),     *
+                                           * where per-bounce launches cost more than they save;   *
+                                           * the wavefront above                                    */
 
 typedef struct luzgpu_ctx luzgpu_ctx;
 typedef struct luzgpu_scene luzgpu_scene;

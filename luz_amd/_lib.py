@@ -21,6 +21,7 @@ E_IO = -5
 
 INTEGRATOR_WAVEFRONT = 0
 INTEGRATOR_MEGAKERNEL = 1
+INTEGRATOR_AUTO = 2
 
 EXPORTED = [
     "luzgpu_create", "luzgpu_destroy", "luzgpu_last_error", "luzgpu_scene_load",

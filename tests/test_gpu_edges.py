@@ -54,3 +54,19 @@ def test_argument_errors(ctx, primitives):
     # the context stays usable after every refused call
     img, _ = Renderer.render(scene, seed=1, samples=1, resolution=(8, 8))
     assert np.isfinite(img).all()
+
+
+def test_auto_integrator_picks_by_size(ctx, primitives, cornell):
+    """LUZGPU_INTEGRATOR_AUTO runs the plain kernel up to 4M paths: the images equal the
+    plain kernel's, single and batched."""
+    from luz_amd import Renderer, SceneFile
+    a = SceneFile.read(ctx, primitives.scene_path)
+    b = SceneFile.read(ctx, cornell.scene_path)
+    auto, _ = Renderer.render(a, seed=2, samples=4, resolution=(64, 48), integrator="auto")
+    plain, _ = Renderer.render(a, seed=2, samples=4, resolution=(64, 48), integrator="megakernel")
+    assert np.array_equal(auto, plain)
+    imgs, _ = Renderer.render_batch([a, b], seeds=[2, 3], samples=[4, 2], resolutions=[(64, 48), (40, 40)],
+                                    integrator="auto")
+    ref, _ = Renderer.render_batch([a, b], seeds=[2, 3], samples=[4, 2], resolutions=[(64, 48), (40, 40)],
+                                   integrator="megakernel")
+    assert all(np.array_equal(x, y) for x, y in zip(imgs, ref))
