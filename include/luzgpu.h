@@ -60,7 +60,7 @@ typedef struct luzgpu_settings
 	uint32_t	count_visits;    /* 1: also count BVH nodes / primitives visited (slower)             */
 	uint32_t	stage_times;     /* 1: time each wavefront stage with HIP events                      */
 	/* wavefront launch shape (0 = built-in default); no effect on the image */
-	uint32_t	traversal_grid;  /* persistent blocks per traversal launch, 1..2048 (default 1024)    */
+	uint32_t	traversal_grid;  /* persistent blocks per traversal launch, 1..2048 (default 1280)    */
 	int32_t		extend_refill;   /* idle lanes (1..64) at which a wave takes new closest-hit rays;    *
 	                              * 64 (default): whole-wave claims, each lane traverses to the end   */
 	int32_t		shadow_refill;   /* the same for shadow rays (default 64)                             */

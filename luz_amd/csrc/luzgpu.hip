@@ -868,7 +868,7 @@ namespace luzgpu
 				tryValues([](WaveTuning& t, int v) { t.chunk = static_cast<unsigned>(v); }, {64, 256, 1024});
 				tryValues([](WaveTuning& t, int v) { t.extendRefill = v; }, {1, 16, 48, 64});
 				tryValues([](WaveTuning& t, int v) { t.shadowRefill = v; }, {1, 16, 48, 64});
-				tryValues([](WaveTuning& t, int v) { t.grid = static_cast<unsigned>(v); }, {1024, 2048});
+				tryValues([](WaveTuning& t, int v) { t.grid = static_cast<unsigned>(v); }, {1024, 1280, 2048});
 			}
 			(void)hipEventDestroy(e0);
 			(void)hipEventDestroy(e1);
