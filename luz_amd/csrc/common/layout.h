@@ -213,6 +213,23 @@ namespace luzgpu
 		GPU_SKY_NONE = 0,
 		GPU_SKY_ENVIRONMENT = 1,
 		GPU_SKY_LINEAR = 2,
+		GPU_SKY_ATMOSPHERE = 3,
+	};
+
+	// The atmosphere sky's parameters (Atmosphere.hpp:52-63) with the derived values its
+	// march uses: unit sun direction, sun radiance x its scale, 1 / scale heights.
+	struct	GpuAtmosphere
+	{
+		double	sunDirection[3];
+		double	sunRadiance[3];
+		double	earthRadius;		// metres
+		double	atmosphereRadius;	// metres
+		double	inverseHR;
+		double	inverseHM;
+		double	starsBrightness;
+		double	metersPerUnit;
+		std::uint32_t	samples;
+		std::uint32_t	lightSamples;
 	};
 
 	// Scalars of one render: all device pointers live in GpuSceneView.
@@ -247,6 +264,7 @@ namespace luzgpu
 		std::uint32_t	envBase;		// envPixels / envPdf / envCdf texel
 		std::uint32_t	unboundedBase;
 		std::uint32_t	denoiseStreams;	// 1: sample k draws from stream (k % 2) + 1 (Threads.cpp:594-598)
+		GpuAtmosphere	atmosphere;		// sky == GPU_SKY_ATMOSPHERE
 	};
 
 	// Scene content classes the kernels are specialised on (a scene without them runs

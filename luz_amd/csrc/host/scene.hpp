@@ -151,7 +151,7 @@ namespace luzgpu
 		SKY_NONE = 0,
 		SKY_ENVIRONMENT = 1,
 		SKY_LINEAR = 2,		// white-to-blue gradient (Atmospherics.cpp:44-53)
-		SKY_ATMOSPHERE = 3,	// the reference's default (Scene.cpp:116); outside the device path
+		SKY_ATMOSPHERE = 3,	// the reference's default (Scene.cpp:118)
 	};
 
 	struct	Camera
@@ -196,6 +196,16 @@ namespace luzgpu
 		double	contrast = 1.0;
 		SkyKind	sky = SKY_ATMOSPHERE;
 		bool	skySet = false;
+		// the atmosphere sky (Atmosphere.cpp:67-82 defaults, Defaults.hpp:22-25)
+		double	sunAngle = -0.4;
+		double	earthRadius = 6360e3;
+		double	atmosphereRadius = 6420e3;
+		double	atmosphereHR = 7994.0;
+		double	atmosphereHM = 1200.0;
+		int	atmosphereSamples = 16;
+		int	atmosphereLightSamples = 8;
+		double	starsBrightness = 0.5;
+		double	sunRadianceScale = 1.0;
 		dvec3	background{};
 		double	metersPerUnit = 1.0;
 	};

@@ -420,6 +420,52 @@ namespace luzgpu
 						else throw std::runtime_error("Unknown sky setting: " + value);
 						s.skySet = true;
 					}
+					else if (key == "atmosphere")
+					{
+						// Settings.cpp:612-628 with the checks of Atmosphere's constructor and
+						// setters (Atmosphere.cpp:84-113, 186-300)
+						if (s.sky != SKY_ATMOSPHERE)
+						{
+							throw std::runtime_error("Atmosphere settings require sky=atmosphere.");
+						}
+						double sun, earth, outer, hr, hm, stars;
+						int samples, lightSamples;
+						if (std::sscanf(lower(value).c_str(), "%lf,%lf,%lf,%lf,%lf,%d,%d,%lf", &sun, &earth, &outer, &hr, &hm, &samples,
+								&lightSamples, &stars) != 8)
+						{
+							throw std::runtime_error("Invalid atmosphere setting. Use atmosphere=SUN,EARTH_RADIUS,ATMOSPHERE_RADIUS,HR,HM,SAMPLES,LIGHT_SAMPLES,STARS.");
+						}
+						if (!std::isfinite(sun)) throw std::runtime_error("Sun angle must be finite.");
+						if (!std::isfinite(earth) || earth <= 0.0) throw std::runtime_error("Earth radius must be positive.");
+						if (!std::isfinite(outer) || outer <= earth)
+						{
+							throw std::runtime_error("Atmosphere radius must be larger than Earth radius.");
+						}
+						if (!std::isfinite(hr) || hr <= 0.0) throw std::runtime_error("Atmosphere HR must be positive.");
+						if (!std::isfinite(hm) || hm <= 0.0) throw std::runtime_error("Atmosphere HM must be positive.");
+						if (samples <= 0) throw std::runtime_error("Atmosphere sample count must be positive.");
+						if (lightSamples <= 0) throw std::runtime_error("Atmosphere light sample count must be positive.");
+						if (!std::isfinite(stars) || stars < 0.0) throw std::runtime_error("Stars brightness must be non-negative.");
+						s.sunAngle = sun;
+						s.earthRadius = earth;
+						s.atmosphereRadius = outer;
+						s.atmosphereHR = hr;
+						s.atmosphereHM = hm;
+						s.atmosphereSamples = samples;
+						s.atmosphereLightSamples = lightSamples;
+						s.starsBrightness = stars;
+					}
+					else if (key == "atmosphere_sun_scale" || key == "atmospheresunscale" || key == "atmosphere_sun_multiplier"
+						|| key == "atmospheresunmultiplier")
+					{
+						// Settings.cpp:629-643, Atmosphere.cpp:207-214
+						const double v = parseDouble(value, "Atmosphere sun scale");
+						if (!std::isfinite(v) || v < 0.0)
+						{
+							throw std::runtime_error("Atmosphere sun radiance scale must be finite and non-negative.");
+						}
+						s.sunRadianceScale = v;
+					}
 					else if (key == "background" || key == "backgroundcolor" || key == "background_color")
 					{
 						s.background = parseColor(value, "background color");

@@ -352,6 +352,41 @@ def write_skylinear(directory: str, width=128, height=96, spp=16, depth=6, seed=
     return Workload("skylinear", directory, path, width, height, spp, depth, seed, 0)
 
 
+def atmosphere_scene_text(width=96, height=64, spp=8, depth=6) -> str:
+    """The atmosphere sky over a small landscape 2 km above a planet of the default size:
+    scene units of 1000 km (meters_per_unit=1e6), a custom atmosphere= line (12 view and 6
+    sun steps, sun 45 degrees up) and a scaled sun; half the frame is sky, the rest shows
+    the aerial perspective over 50-300 km."""
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "meters_per_unit=1000000", "sky=atmosphere", "atmosphere_sun_scale=0.02",
+        "atmosphere=0.25,6360000,6420000,7994,1200,12,6,0.5", "",
+        "[materials]",
+        "material ground {", "type=lambertian", "color=(0.35,0.3,0.22)", "}",
+        "material clay {", "type=lambertian", "color=(0.7,0.35,0.2)", "}",
+        "material steel {", "type=metal", "color=(0.8,0.8,0.85)", "roughness=0.2", "}",
+        "material glow {", "type=emissive", "color=(1,0.8,0.5)", "radiance=6", "}",
+        "",
+        "[scene]",
+        "camera main {", "position=(0,6.362,0)", "direction=(0,-0.04,1)", "up=(0,1,0)", "focal_length_mm=35",
+        "sensor_width_mm=36", "sensor_height_mm=24", "pinhole=1", "focus_distance=1", "}",
+        "objects{",
+        "plane=6.3605,(0,1,0),material=ground",
+        "sphere=(-0.04,6.372,0.15),0.012,material=clay",
+        "sphere=(0.05,6.375,0.3),0.015,material=steel",
+        "sphere=(0.0,6.364,0.08),0.004,material=glow",
+        "}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def write_atmosphere(directory: str, width=96, height=64, spp=8, depth=6, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "atmosphere.luz")
+    _write_if_changed(path, atmosphere_scene_text(width, height, spp, depth).encode())
+    return Workload("atmosphere", directory, path, width, height, spp, depth, seed, 0)
+
+
 _QUAD_OBJ = "v -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\nf 1 2 3\nf 1 3 4\n"
 
 

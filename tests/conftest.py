@@ -54,6 +54,12 @@ def skylinear(workload_dir):
 
 
 @pytest.fixture(scope="session")
+def atmosphere(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_atmosphere(os.path.join(workload_dir, "atmosphere"))
+
+
+@pytest.fixture(scope="session")
 def volume(workload_dir):
     from luz_amd import workloads
     return workloads.write_volume(os.path.join(workload_dir, "volume"))

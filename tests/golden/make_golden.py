@@ -296,6 +296,8 @@ def main():
             small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
         if "skylinear" in only:
             small_scene_fixtures("skylinear", workloads.write_skylinear(os.path.join(tmp, "skylinear")), tmp)
+        if "atmosphere" in only:
+            small_scene_fixtures("atmosphere", workloads.write_atmosphere(os.path.join(tmp, "atmosphere")), tmp)
         if "meshlight" in only:
             small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
         if "fd" in only:
@@ -315,6 +317,7 @@ def main():
     small_scene_fixtures("materials", workloads.write_materials(os.path.join(tmp, "materials")), tmp)
     small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
     small_scene_fixtures("skylinear", workloads.write_skylinear(os.path.join(tmp, "skylinear")), tmp)
+    small_scene_fixtures("atmosphere", workloads.write_atmosphere(os.path.join(tmp, "atmosphere")), tmp)
     fd_fixture(tmp)
     zstat_fixtures(tmp)
     nfor_fixtures(tmp)

@@ -31,7 +31,7 @@ def test_device_sampler_bit_exact(ctx):
     assert sel.sum() == 1 and s1[sel][0] > 0.5
 
 
-def _primary_check(scene, golden, resolution=None):
+def _primary_check(scene, golden, resolution=None, t_rtol=2e-5):
     from luz_amd import Renderer
     ids, t = Renderer.primary_hits(scene, seed=1, resolution=resolution)
     g = _golden(golden)
@@ -44,7 +44,7 @@ def _primary_check(scene, golden, resolution=None):
         print("unlisted:", list(zip(xs.tolist(), ys.tolist()))[:20])
     assert (~same).sum() <= max(1, 1e-4 * n)
     hit = g["top"] >= 0
-    assert np.allclose(t[hit & same], g["t"][hit & same], rtol=2e-5, atol=1e-4)
+    assert np.allclose(t[hit & same], g["t"][hit & same], rtol=t_rtol, atol=1e-4)
     return int(bad.sum())
 
 

@@ -17,16 +17,20 @@ from test_gpu_parity import REL_MSE_BAR, _golden, _primary_check
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere"])
 def test_primary_hits(ctx, request, name):
     from luz_amd import SceneFile
     wl = request.getfixturevalue(name)
     scene = SceneFile.read(ctx, wl.scene_path)
-    assert _primary_check(scene, f"{name}_primary.npz") <= 2
+    # the atmosphere scene sits 6.36 units from the origin, where an FP32 origin carries
+    # 4.8e-7 of rounding; over the ground plane's grazing rays (|d.y| down to ~0.005) that is
+    # up to 1e-4 of t, so its distances are held to 2e-3 (its hit ids stay exact)
+    t_rtol = 2e-3 if name == "atmosphere" else 2e-5
+    assert _primary_check(scene, f"{name}_primary.npz", t_rtol=t_rtol) <= 2
 
 
 @pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere"])
 def test_image_parity(ctx, request, name, integrator):
     from luz_amd import Renderer, SceneFile
     wl = request.getfixturevalue(name)

@@ -7,6 +7,7 @@ import hashlib
 import os
 
 import numpy as np
+import pytest
 
 from conftest import GOLDEN
 from luz_amd import SceneFile
@@ -50,7 +51,9 @@ def test_unsupported_feature_is_reported(tmp_path):
     import pytest
     from luz_amd import _lib
     p = tmp_path / "bad.luz"
-    p.write_text("[settings]\nsky=atmosphere\n\n[scene]\ncamera c {\nposition=(0,0,0)\n}\n")
+    # the atmosphere sky with a directional light as its sun (Scene.cpp:455-473) stays refused
+    p.write_text("[settings]\nsky=atmosphere\n\n[scene]\ncamera c {\nposition=(0,0,0)\n}\n"
+                 "directional_light sun {\ndirection=(0,-1,0)\ncolor=(1,1,1)\nirradiance=2\n}\n")
     with pytest.raises(_lib.LuzGpuError) as e:
         SceneFile.read(None, str(p))
     assert e.value.code == _lib.E_UNSUPPORTED
@@ -121,3 +124,36 @@ def test_material_validation(tmp_path):
         with pytest.raises(_lib.LuzGpuError) as e:
             SceneFile.read(None, str(f))
         assert e.value.code == code, props
+
+
+ATMOSPHERE_BAD = [
+    # Tests.cpp:1461-1463
+    "sky=atmosphere\natmosphere=0,1,2,3,4,0,1,0",
+    "sky=atmosphere\natmosphere=0,2,1,3,4,1,1,0",
+    "sky=atmosphere\natmosphere_sun_scale=-1",
+    # Settings.cpp:618-625: the key needs the atmosphere sky, and all eight values
+    "sky=linear\natmosphere=0.25,12840000.0,12910000.0,7994.0,1200.0,12,6,0.1",
+    "sky=atmosphere\natmosphere=0.25,12840000.0,12910000.0",
+    "sky=atmosphere\natmosphere=0.25,12840000.0,12910000.0,0,1200.0,12,6,0.1",
+    "sky=atmosphere\natmosphere=0.25,12840000.0,12910000.0,7994.0,1200.0,12,6,-1",
+]
+
+
+@pytest.mark.parametrize("settings", ATMOSPHERE_BAD)
+def test_scene_file_rejects_invalid_atmosphere(tmp_path, settings):
+    import pytest as _pytest
+    p = tmp_path / "bad.luz"
+    p.write_text(f"[settings]\n{settings}\n\n[scene]\ncamera c {{\nposition=(0,0,0)\n}}\n")
+    with _pytest.raises(Exception):
+        SceneFile.read(None, str(p))
+
+
+def test_scene_file_accepts_atmosphere(tmp_path, atmosphere):
+    """Tests.cpp:1466-1496: the default sky and a custom atmosphere= line load; the sun scale
+    keys are accepted in all four spellings."""
+    for settings in ("", "sky=atmosphere\natmosphere=0.25,12840000.0,12910000.0,7994.0,1200.0,12,6,0.1",
+                     "atmospheresunscale=0.5", "atmosphere_sun_multiplier=2", "atmospheresunmultiplier=0"):
+        p = tmp_path / "ok.luz"
+        p.write_text(f"[settings]\n{settings}\n\n[scene]\ncamera c {{\nposition=(0,0,0)\n}}\n")
+        SceneFile.read(None, str(p)).release()
+    SceneFile.read(None, atmosphere.scene_path).release()
