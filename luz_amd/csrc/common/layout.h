@@ -270,7 +270,7 @@ namespace luzgpu
 	// Scene content classes the kernels are specialised on (a scene without them runs
 	// kernels that do not carry their code or registers).
 	constexpr int	kFeatAnalytic = 1;	// spheres, cubes, volumes, planes, directional and mesh-instance lights
-	constexpr int	kFeatRich = 2;		// glossy, diffuse-glossy, principled, isotropic, Henyey-Greenstein
+	constexpr int	kFeatRich = 2;		// glossy, diffuse-glossy, principled, isotropic, Henyey-Greenstein; the atmosphere sky
 	constexpr int	kFeatAll = 3;
 
 	// Per-scene traversal entry of a multi-scene launch: TLAS root and the unbounded

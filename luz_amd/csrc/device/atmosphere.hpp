@@ -4,7 +4,9 @@
 // stars drawn where no environment is visible.
 //
 // Rays carry FP32 origins and unit directions; the march widens them to FP64 and works in
-// metres (scene units x metersPerUnit) around a planet centred at the world origin.
+// metres (scene units x metersPerUnit) around a planet centred at the world origin. Only
+// the kernels specialised for rich scenes (kFeatRich) carry this code: an atmosphere scene
+// is classed as rich, so the FP64 loops add no registers to the kernels of other scenes.
 #pragma once
 
 #include "sampler.hpp"

@@ -132,15 +132,19 @@ namespace luzgpu
 		return c;
 	}
 
+	template <int Feat>
 	__device__ __forceinline__ f3	missRadiance(const SceneView& s, const GpuParams& p, const Ray& ray, const PixelSampler& ps,
 		float envWeight)
 	{
 		const f3 dir = ray.d;
 		f3 sky;
-		if (p.sky == GPU_SKY_ATMOSPHERE)
+		if constexpr ((Feat & kFeatRich) != 0)
 		{
-			// without an environment map the stars are not MIS-weighted (ColorHelper.cpp:590-605)
-			return atmosphereSky(p.atmosphere, ray.o, dir, ps);
+			if (p.sky == GPU_SKY_ATMOSPHERE)
+			{
+				// without an environment map the stars are not MIS-weighted (ColorHelper.cpp:590-605)
+				return atmosphereSky(p.atmosphere, ray.o, dir, ps);
+			}
 		}
 		if (p.sky == GPU_SKY_LINEAR)
 		{
@@ -326,9 +330,9 @@ namespace luzgpu
 				}
 				if (b == 0 && features)
 				{
-					primaryFeatures(features, p, x, y, false, 0.0f, mk(0, 0, 0), missRadiance(s, p, ray, ps, 1.0f), 0u);
+					primaryFeatures(features, p, x, y, false, 0.0f, mk(0, 0, 0), missRadiance<Feat>(s, p, ray, ps, 1.0f), 0u);
 				}
-				f3 c = thr * missRadiance(s, p, ray, ps, w);
+				f3 c = thr * missRadiance<Feat>(s, p, ray, ps, w);
 				if (b > 0 && hasConstantBackgroundLight(p)) c = clampConstantBackground(p, c);
 				c = clampRay(c);
 				if constexpr (Adj::enabled) adj->arrival(c, w);
@@ -341,10 +345,13 @@ namespace luzgpu
 			{
 				primaryFeatures(features, p, x, y, true, h.t, sf.normal, ld3(m.color), m.kind);
 			}
-			if (b == 0 && p.sky == GPU_SKY_ATMOSPHERE)
+			if constexpr ((Feat & kFeatRich) != 0)
 			{
-				atmospherePrimary(p.atmosphere, ray.o, ray.d, h.t, acc, thr);
-				if (terminated(thr)) return acc;
+				if (b == 0 && p.sky == GPU_SKY_ATMOSPHERE)
+				{
+					atmospherePrimary(p.atmosphere, ray.o, ray.d, h.t, acc, thr);
+					if (terminated(thr)) return acc;
+				}
 			}
 			const f3 emitted = m.kind == GPU_MAT_EMISSIVE ? ld3(m.color) : mk(0, 0, 0);
 			Scatter sc;

@@ -514,13 +514,13 @@ namespace luzgpu
 					{
 						wgt = powerHeuristic(prevPdf, infiniteLightPdf(s, p, ray.d));
 					}
-					f3 c = thr * missRadiance(s, p, ray, ps, wgt);
+					f3 c = thr * missRadiance<Feat>(s, p, ray, ps, wgt);
 					if (b > 0 && hasConstantBackgroundLight(p)) c = clampConstantBackground(p, c);
 					acc = acc + clampRay(c);
 					if (b == 0 && J.features)
 					{
 						primaryFeatures(J.features + static_cast<size_t>(kNforFeatures) * path, p, px, py, false, 0.0f, mk(0, 0, 0),
-							missRadiance(s, p, ray, ps, 1.0f), 0u);
+							missRadiance<Feat>(s, p, ray, ps, 1.0f), 0u);
 					}
 				}
 				else
@@ -541,10 +541,13 @@ namespace luzgpu
 							ld3(m.color), m.kind);
 					}
 					bool alive = true;
-					if (b == 0 && p.sky == GPU_SKY_ATMOSPHERE)
+					if constexpr ((Feat & kFeatRich) != 0)
 					{
-						atmospherePrimary(p.atmosphere, ray.o, ray.d, h.t, acc, thr);
-						alive = !terminated(thr);
+						if (b == 0 && p.sky == GPU_SKY_ATMOSPHERE)
+						{
+							atmospherePrimary(p.atmosphere, ray.o, ray.d, h.t, acc, thr);
+							alive = !terminated(thr);
+						}
 					}
 					const f3 emitted = m.kind == GPU_MAT_EMISSIVE ? ld3(m.color) : mk(0, 0, 0);
 					Scatter sc;

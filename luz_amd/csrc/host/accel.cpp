@@ -1535,6 +1535,7 @@ namespace luzgpu
 				p.features |= kFeatRich;
 			}
 		}
+		if (s.sky == SKY_ATMOSPHERE) p.features |= kFeatRich;	// the atmosphere march lives in the rich kernels
 		gp.lightWeighted = p.lightCdf.empty() ? 0u : 1u;
 		gp.lightBase = 0;
 		gp.envBase = 0;
