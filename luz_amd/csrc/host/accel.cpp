@@ -896,49 +896,12 @@ namespace luzgpu
 
 	namespace
 	{
-		// The CIE 1931 colour-matching fits ColorScience.cpp:19-42 uses (three / two / two
-		// piecewise Gaussians).
-		double	cmfGaussian(double v) { return std::exp(-0.5 * v * v); }
-		double	cmfX(double w)
-		{
-			return 0.362 * cmfGaussian((w - 442.0) * (w < 442.0 ? 0.0624 : 0.0374))
-				+ 1.056 * cmfGaussian((w - 599.8) * (w < 599.8 ? 0.0264 : 0.0323))
-				- 0.065 * cmfGaussian((w - 501.1) * (w < 501.1 ? 0.0490 : 0.0382));
-		}
-		double	cmfY(double w)
-		{
-			return 0.821 * cmfGaussian((w - 568.8) * (w < 568.8 ? 0.0213 : 0.0247))
-				+ 0.286 * cmfGaussian((w - 530.9) * (w < 530.9 ? 0.0613 : 0.0322));
-		}
-		double	cmfZ(double w)
-		{
-			return 1.217 * cmfGaussian((w - 437.0) * (w < 437.0 ? 0.0845 : 0.0278))
-				+ 0.681 * cmfGaussian((w - 459.0) * (w < 459.0 ? 0.0385 : 0.0725));
-		}
-
 		// The atmosphere's default sun: LightUnits::solarDirectionalIrradiance of
-		// ColorScience::solar() (Atmosphere.cpp:76, LightUnits.cpp:35-50 and 113-117): a 5778 K
-		// Planck spectrum summed against the colour-matching fits every 5 nm over 360-830 nm
-		// (ColorScience.cpp:77-88, 196-219), XYZ -> ACEScg (ColorManagement.cpp:139-143),
-		// normalised to a maximum channel of 1, then scaled to a luminance of 1361 W/m^2.
+		// ColorScience::solar() (Atmosphere.cpp:76, LightUnits.cpp:35-50 and 113-117), the
+		// 5778 K blackbody colour scaled to a luminance of 1361 W/m^2.
 		dvec3	solarSunRadiance()
 		{
-			double x = 0.0, y = 0.0, z = 0.0;
-			for (double w = 360.0; w <= 830.0; w += 5.0)
-			{
-				const double meters = w * 1.0e-9;
-				const double exponent = 1.438776877e-2 / (meters * 5778.0);
-				const double r = exponent > 700.0 ? 0.0 : 1.0 / (std::pow(meters, 5.0) * std::expm1(exponent));
-				x += r * cmfX(w);
-				y += r * cmfY(w);
-				z += r * cmfZ(w);
-			}
-			dvec3 c{1.6410233797 * x + -0.3248032942 * y + -0.2364246952 * z,
-				-0.6636628587 * x + 1.6153315917 * y + 0.0167563477 * z,
-				0.0117218943 * x + -0.0082844420 * y + 0.9883948585 * z};
-			for (double& v : c) v = std::isfinite(v) ? std::max(0.0, v) : 0.0;
-			const double top = std::max(c[0], std::max(c[1], c[2]));
-			for (double& v : c) v /= top;
+			dvec3 c = blackbodyColor(5778.0);
 			const double scale = 1361.0 / luminance(c);
 			for (double& v : c) v *= scale;
 			return c;

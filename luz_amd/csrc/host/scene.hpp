@@ -236,4 +236,8 @@ namespace luzgpu
 	Mesh	readObjMesh(const std::string& path);
 	Environment	readHDREnvironment(const std::string& path);
 	double	luminance(const dvec3& c);
+	// ColorScience::blackbody / ::wavelength (ColorScience.cpp:182-214): ACEScg colours of
+	// unit maximum channel; throw outside 1000-40000 K / 360-830 nm
+	dvec3	blackbodyColor(double kelvin);
+	dvec3	wavelengthColor(double nanometers);
 }

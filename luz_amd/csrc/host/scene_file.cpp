@@ -33,6 +33,75 @@ namespace luzgpu
 
 	namespace
 	{
+		// The CIE 1931 colour-matching fits ColorScience.cpp:19-42 uses (three / two / two
+		// piecewise Gaussians).
+		double	cmfGaussian(double v) { return std::exp(-0.5 * v * v); }
+		double	cmfX(double w)
+		{
+			return 0.362 * cmfGaussian((w - 442.0) * (w < 442.0 ? 0.0624 : 0.0374))
+				+ 1.056 * cmfGaussian((w - 599.8) * (w < 599.8 ? 0.0264 : 0.0323))
+				- 0.065 * cmfGaussian((w - 501.1) * (w < 501.1 ? 0.0490 : 0.0382));
+		}
+		double	cmfY(double w)
+		{
+			return 0.821 * cmfGaussian((w - 568.8) * (w < 568.8 ? 0.0213 : 0.0247))
+				+ 0.286 * cmfGaussian((w - 530.9) * (w < 530.9 ? 0.0613 : 0.0322));
+		}
+		double	cmfZ(double w)
+		{
+			return 1.217 * cmfGaussian((w - 437.0) * (w < 437.0 ? 0.0845 : 0.0278))
+				+ 0.681 * cmfGaussian((w - 459.0) * (w < 459.0 ? 0.0385 : 0.0725));
+		}
+
+		// XYZ -> ACEScg (ColorManagement.cpp:139-143), then non-negative channels scaled to a
+		// maximum of 1 (normalizeVisibleColor, ColorScience.cpp:49-60)
+		dvec3	visibleFromXYZ(double x, double y, double z)
+		{
+			dvec3 c{1.6410233797 * x + -0.3248032942 * y + -0.2364246952 * z,
+				-0.6636628587 * x + 1.6153315917 * y + 0.0167563477 * z,
+				0.0117218943 * x + -0.0082844420 * y + 0.9883948585 * z};
+			for (double& v : c) v = std::isfinite(v) ? std::max(0.0, v) : 0.0;
+			const double top = std::max(c[0], std::max(c[1], c[2]));
+			if (top <= 0.0 || !std::isfinite(top))
+			{
+				throw std::runtime_error("Spectral color converted outside the visible RGB gamut.");
+			}
+			for (double& v : c) v /= top;
+			return c;
+		}
+	}
+
+	dvec3	blackbodyColor(double kelvin)
+	{
+		if (!std::isfinite(kelvin) || kelvin < 1000.0 || kelvin > 40000.0)
+		{
+			throw std::runtime_error("Blackbody temperature must be finite and within 1000-40000 K.");
+		}
+		// relativePlanckRadiance summed every 5 nm over 360-830 nm (ColorScience.cpp:77-88, 196-214)
+		double x = 0.0, y = 0.0, z = 0.0;
+		for (double w = 360.0; w <= 830.0; w += 5.0)
+		{
+			const double meters = w * 1.0e-9;
+			const double exponent = 1.438776877e-2 / (meters * kelvin);
+			const double r = exponent > 700.0 ? 0.0 : 1.0 / (std::pow(meters, 5.0) * std::expm1(exponent));
+			x += r * cmfX(w);
+			y += r * cmfY(w);
+			z += r * cmfZ(w);
+		}
+		return visibleFromXYZ(x, y, z);
+	}
+
+	dvec3	wavelengthColor(double nanometers)
+	{
+		if (!std::isfinite(nanometers) || nanometers < 360.0 || nanometers > 830.0)
+		{
+			throw std::runtime_error("Wavelength must be finite and within 360-830 nm.");
+		}
+		return visibleFromXYZ(cmfX(nanometers), cmfY(nanometers), cmfZ(nanometers));
+	}
+
+	namespace
+	{
 		std::string	trim(const std::string& s)
 		{
 			std::size_t a = 0, b = s.size();
@@ -137,6 +206,36 @@ namespace luzgpu
 					return (acescgFromLinearSRGB(parseTriple(t.substr(prefix.size() - 1), what)));
 				}
 			}
+			// single-argument colour functions with an optional unit suffix (Parsing.cpp:108-135, 191-210)
+			auto argument = [&](const std::string& name, const std::string& suffix) -> double
+			{
+				const std::string prefix = name + "(";
+				if (lt.back() != ')') throw std::runtime_error("Invalid " + what + " function.");
+				std::string a = trim(t.substr(prefix.size(), t.size() - prefix.size() - 1));
+				const std::string la = lower(a);
+				if (la.size() > suffix.size() && la.compare(la.size() - suffix.size(), suffix.size(), suffix) == 0)
+				{
+					a = trim(a.substr(0, a.size() - suffix.size()));
+				}
+				std::size_t used = 0;
+				double v = 0.0;
+				try
+				{
+					v = std::stod(a, &used);
+				}
+				catch (const std::exception&)
+				{
+					throw std::runtime_error("Invalid " + what + " function value.");
+				}
+				if (used != a.size() || !std::isfinite(v)) throw std::runtime_error("Invalid " + what + " function value.");
+				return v;
+			};
+			if (lt.rfind("wavelength(", 0) == 0) return (wavelengthColor(argument("wavelength", "nm")));
+			for (const char* name : {"blackbody", "temperature", "color_temperature"})
+			{
+				if (lt.rfind(std::string(name) + "(", 0) == 0) return (blackbodyColor(argument(name, "k")));
+			}
+			if (lt == "solar" || lt == "sun") return (blackbodyColor(5778.0));
 			return (parseTriple(t, what));
 		}
 

@@ -387,6 +387,26 @@ def write_atmosphere(directory: str, width=96, height=64, spp=8, depth=6, seed=1
     return Workload("atmosphere", directory, path, width, height, spp, depth, seed, 0)
 
 
+def spectral_scene_text(width=96, height=72, spp=8, depth=6) -> str:
+    """The primitives scene with colours given as spectral functions: blackbody temperatures
+    (with and without the K suffix, under all three names), a single wavelength and the solar
+    colour."""
+    text = primitives_scene_text(width, height, spp, depth)
+    text = text.replace("color=(0.6,0.6,0.6)", "color=temperature(6500K)")
+    text = text.replace("color=(0.7,0.15,0.1)", "color=wavelength(610nm)")
+    text = text.replace("color=(0.85,0.8,0.7)", "color=solar")
+    text = text.replace("color=(1,0.85,0.6)", "color=blackbody(2700)")
+    text = text.replace("color=(0.6,0.7,1)", "color=color_temperature(9000 k)")
+    return text
+
+
+def write_spectral(directory: str, width=96, height=72, spp=8, depth=6, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "spectral.luz")
+    _write_if_changed(path, spectral_scene_text(width, height, spp, depth).encode())
+    return Workload("spectral", directory, path, width, height, spp, depth, seed, 0)
+
+
 _QUAD_OBJ = "v -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\nf 1 2 3\nf 1 3 4\n"
 
 

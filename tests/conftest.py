@@ -79,3 +79,9 @@ def rel_mse(x, r):
     x = x.astype(np.float64)
     r = r.astype(np.float64)
     return float(((x - r) ** 2).sum() / max((r ** 2).sum(), 1e-300))
+
+
+@pytest.fixture(scope="session")
+def spectral(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_spectral(os.path.join(workload_dir, "spectral"))
