@@ -173,6 +173,17 @@ namespace luzgpu
 			static_cast<float>(a.in[2] + a.tr[2] * star));
 	}
 
+	// sampleAtmosphereSky with a visible environment (ColorHelper.cpp:590-605): the
+	// (MIS-weighted) environment radiance seen through the atmosphere
+	__device__ inline f3	atmosphereOver(const GpuAtmosphere& A, f3 origin, f3 dir, f3 background)
+	{
+#pragma clang fp contract(off)
+		const AtmosphereSegment a = atmosphereSegment(A, origin, dir, kAtmosphereTMax);
+		return mk(static_cast<float>(a.in[0] + a.tr[0] * static_cast<double>(background.x)),
+			static_cast<float>(a.in[1] + a.tr[1] * static_cast<double>(background.y)),
+			static_cast<float>(a.in[2] + a.tr[2] * static_cast<double>(background.z)));
+	}
+
 	// compositePrimaryAtmosphereSegment with primaryAtmosphereTMax (ColorHelper.cpp:624-659):
 	// the aerial perspective between the camera and the first surface (extended to the
 	// ground when the planet surface lies beyond it).

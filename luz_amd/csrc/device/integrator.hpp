@@ -67,7 +67,7 @@ namespace luzgpu
 
 	__device__ __forceinline__ bool	samplesVisibleInfinite(const GpuParams& p, const Scatter& sc)
 	{
-		return (p.sky == GPU_SKY_ENVIRONMENT && hasEnvLight(p)) || (hasConstantBackgroundLight(p) && sc.pdfType != PDF_BSDF);
+		return ((p.sky == GPU_SKY_ENVIRONMENT || p.sky == GPU_SKY_ATMOSPHERE) && hasEnvLight(p)) || (hasConstantBackgroundLight(p) && sc.pdfType != PDF_BSDF);
 	}
 
 	// Unoccluded contribution of one light sample and the shadow ray that gates it.
@@ -142,7 +142,11 @@ namespace luzgpu
 		{
 			if (p.sky == GPU_SKY_ATMOSPHERE)
 			{
-				// without an environment map the stars are not MIS-weighted (ColorHelper.cpp:590-605)
+				// a visible environment map replaces the stars (ColorHelper.cpp:590-605)
+				if (p.hasEnvironment && p.envStrength > 0.0f)
+				{
+					return atmosphereOver(p.atmosphere, ray.o, dir, envLookup(s, p, dir) * p.envStrength * envWeight);
+				}
 				return atmosphereSky(p.atmosphere, ray.o, dir, ps);
 			}
 		}

@@ -936,10 +936,6 @@ namespace luzgpu
 	{
 		PackedScene p;
 		const Settings& s = scene.settings;
-		if (s.sky == SKY_ATMOSPHERE && scene.hasEnvironment)
-		{
-			throw std::runtime_error("Unsupported for the device path: the atmosphere sky combined with an environment map");
-		}
 		if (s.sky == SKY_ATMOSPHERE && !scene.directionals.empty())
 		{
 			throw std::runtime_error("Unsupported for the device path: a directional light as the atmosphere's sun");

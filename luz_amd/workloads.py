@@ -407,6 +407,19 @@ def write_spectral(directory: str, width=96, height=72, spp=8, depth=6, seed=1) 
     return Workload("spectral", directory, path, width, height, spp, depth, seed, 0)
 
 
+def write_atmosphere_env(directory: str, width=96, height=64, spp=8, depth=6, seed=1) -> Workload:
+    """A small spheres scene under the atmosphere sky with its environment map both seen
+    through the atmosphere and sampled as a light: a 30 m planet at the origin inside a
+    220 m atmosphere with scale heights of 8 m / 2 m."""
+    wl = spheres_scene(directory, width=width, height=height, spp=spp, depth=depth, count=24, render_seed=seed)
+    text = open(wl.scene_path).read()
+    text = text.replace("environment=sky.hdr", "sky=atmosphere\natmosphere_sun_scale=0.5\n"
+                        "atmosphere=0.3,30,220,8,2,8,4,0.5\nenvironment=sky.hdr", 1)
+    path = os.path.join(directory, "atmosphere_env.luz")
+    _write_if_changed(path, text.encode())
+    return Workload("atmosphere_env", directory, path, width, height, spp, depth, seed, wl.triangles)
+
+
 _QUAD_OBJ = "v -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\nf 1 2 3\nf 1 3 4\n"
 
 

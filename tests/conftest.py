@@ -85,3 +85,9 @@ def rel_mse(x, r):
 def spectral(workload_dir):
     from luz_amd import workloads
     return workloads.write_spectral(os.path.join(workload_dir, "spectral"))
+
+
+@pytest.fixture(scope="session")
+def atmosphere_env(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_atmosphere_env(os.path.join(workload_dir, "atmosphere_env"))

@@ -300,6 +300,8 @@ def main():
             small_scene_fixtures("atmosphere", workloads.write_atmosphere(os.path.join(tmp, "atmosphere")), tmp)
         if "spectral" in only:
             small_scene_fixtures("spectral", workloads.write_spectral(os.path.join(tmp, "spectral")), tmp)
+        if "atmosphere_env" in only:
+            small_scene_fixtures("atmosphere_env", workloads.write_atmosphere_env(os.path.join(tmp, "atmosphere_env")), tmp)
         if "meshlight" in only:
             small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
         if "fd" in only:
@@ -321,6 +323,7 @@ def main():
     small_scene_fixtures("skylinear", workloads.write_skylinear(os.path.join(tmp, "skylinear")), tmp)
     small_scene_fixtures("atmosphere", workloads.write_atmosphere(os.path.join(tmp, "atmosphere")), tmp)
     small_scene_fixtures("spectral", workloads.write_spectral(os.path.join(tmp, "spectral")), tmp)
+    small_scene_fixtures("atmosphere_env", workloads.write_atmosphere_env(os.path.join(tmp, "atmosphere_env")), tmp)
     fd_fixture(tmp)
     zstat_fixtures(tmp)
     nfor_fixtures(tmp)

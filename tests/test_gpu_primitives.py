@@ -17,7 +17,7 @@ from test_gpu_parity import REL_MSE_BAR, _golden, _primary_check
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env"])
 def test_primary_hits(ctx, request, name):
     from luz_amd import SceneFile
     wl = request.getfixturevalue(name)
@@ -30,7 +30,7 @@ def test_primary_hits(ctx, request, name):
 
 
 @pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env"])
 def test_image_parity(ctx, request, name, integrator):
     from luz_amd import Renderer, SceneFile
     wl = request.getfixturevalue(name)
