@@ -184,7 +184,7 @@ typedef struct luzgpu_scene_info
 /* Context: one HIP device + stream. (Threads.cpp:375-548 worker pool)
  * luzgpu_destroy frees the device arrays of scenes still loaded on the context; those
  * handles stay valid for host-side queries and luzgpu_scene_release. */
-int		luzgpu_create(int device, luzgpu_ctx** out);
+int		luzgpu_create(luzgpu_ctx** out, int device);
 void	luzgpu_destroy(luzgpu_ctx* ctx);
 const char*	luzgpu_last_error(void);
 

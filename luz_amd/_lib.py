@@ -162,7 +162,7 @@ def load() -> ctypes.CDLL:
         pass
     lib = ctypes.CDLL(LIB_PATH)
     vp = ctypes.c_void_p
-    lib.luzgpu_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.luzgpu_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
     lib.luzgpu_destroy.argtypes = [vp]
     lib.luzgpu_destroy.restype = None
     lib.luzgpu_last_error.restype = ctypes.c_char_p

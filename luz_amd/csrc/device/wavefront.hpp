@@ -1011,8 +1011,8 @@ namespace luzgpu
 			if (!marks) return;
 			hipEvent_t e;
 			checkHip(hipEventCreate(&e), "hipEventCreate");
-			checkHip(hipEventRecord(e, stream), "hipEventRecord");
 			marks->push_back(e);
+			checkHip(hipEventRecord(e, stream), "hipEventRecord");
 		};
 		const std::uint32_t vol = volumes ? 1u : 0u;
 		wf_zero_counts<<<1, 4 * kMaxBounceSlots, 0, stream>>>(w.counts, 4 * kMaxBounceSlots, st.jobs, jobCount);
@@ -1160,7 +1160,13 @@ namespace luzgpu
 				st.graphKey = key;
 			}
 		}
-		std::vector<hipEvent_t> marks;
+		// stage-timing events, destroyed on every exit (an error thrown mid-pass included)
+		struct Marks
+		{
+			std::vector<hipEvent_t>	v;
+			~Marks() { for (hipEvent_t e : v) (void)hipEventDestroy(e); }
+		} held;
+		std::vector<hipEvent_t>& marks = held.v;
 		for (std::uint32_t k = 0; k < passes; k++)
 		{
 			checkHip(hipMemcpyAsync(st.jobs, st.plans + static_cast<size_t>(k) * n, n * sizeof(PassJob), hipMemcpyDeviceToDevice,
@@ -1190,7 +1196,6 @@ namespace luzgpu
 				times->shadowMs += c;
 				times->extendLaunches++;
 			}
-			for (hipEvent_t e : marks) (void)hipEventDestroy(e);
 		}
 	}
 }

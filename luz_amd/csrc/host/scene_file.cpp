@@ -33,6 +33,10 @@ namespace luzgpu
 
 	namespace
 	{
+		// Largest atmosphere march (view samples, and light samples per view sample) the device
+		// path accepts: 4096 x 4096 FP64 steps is already seconds per miss-ray wave.
+		constexpr int	kMaxAtmosphereSamples = 4096;
+
 		// The CIE 1931 colour-matching fits ColorScience.cpp:19-42 uses (three / two / two
 		// piecewise Gaussians).
 		double	cmfGaussian(double v) { return std::exp(-0.5 * v * v); }
@@ -545,6 +549,14 @@ namespace luzgpu
 						if (samples <= 0) throw std::runtime_error("Atmosphere sample count must be positive.");
 						if (lightSamples <= 0) throw std::runtime_error("Atmosphere light sample count must be positive.");
 						if (!std::isfinite(stars) || stars < 0.0) throw std::runtime_error("Stars brightness must be non-negative.");
+						// The device marches samples x lightSamples steps per miss ray inside one
+						// kernel launch; the reference (CPU) only slows down, a GPU launch of that
+						// length would outlast any watchdog.
+						if (samples > kMaxAtmosphereSamples || lightSamples > kMaxAtmosphereSamples)
+						{
+							throw std::runtime_error("Unsupported for the device path: atmosphere SAMPLES and LIGHT_SAMPLES above "
+								+ std::to_string(kMaxAtmosphereSamples));
+						}
 						s.sunAngle = sun;
 						s.earthRadius = earth;
 						s.atmosphereRadius = outer;

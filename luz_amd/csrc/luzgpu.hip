@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -84,6 +85,27 @@ namespace luzgpu
 
 			template <typename T> T*	as() const { return static_cast<T*>(ptr); }
 		};
+
+		// A HIP event destroyed with its scope, so refused or failing calls leak none.
+		struct	Event
+		{
+			hipEvent_t	e = nullptr;
+
+			Event() { check(hipEventCreate(&e), "hipEventCreate"); }
+			Event(const Event&) = delete;
+			Event& operator=(const Event&) = delete;
+			~Event() { if (e) (void)hipEventDestroy(e); }
+
+			void	record(hipStream_t st) { check(hipEventRecord(e, st), "hipEventRecord"); }
+			operator hipEvent_t() const { return e; }
+		};
+
+		float	elapsedMs(const Event& a, const Event& b)
+		{
+			float ms = 0.0f;
+			check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+			return ms;
+		}
 	}
 
 	// Scalars the kernels need beyond GpuParams.
@@ -94,6 +116,8 @@ namespace luzgpu
 	};
 
 	constexpr int	kMegaBlock = 128;
+	constexpr std::uint32_t	kMaxImageSide = 65536;
+	constexpr std::uint64_t	kMaxImagePixels = 1ull << 28;
 	constexpr int	kStackDepth = 48;
 
 	// counters: [0] closest-hit rays, [1] shadow rays, [2] nodes visited, [3] primitives tested.
@@ -756,6 +780,17 @@ namespace luzgpu
 				p.sampleBegin = 0;
 				p.sampleCount = p.totalSamples;
 			}
+			// 64-bit checks: a sample range or an image size that wraps in 32 bits would
+			// otherwise size buffers and sampler indices wrongly
+			if (static_cast<std::uint64_t>(p.sampleBegin) + p.sampleCount > p.totalSamples)
+			{
+				throw Error{LUZGPU_E_INVALID, "sample range [sample_begin, sample_begin + sample_count) outside [0, samples)"};
+			}
+			if (p.width == 0 || p.height == 0 || p.width > kMaxImageSide || p.height > kMaxImageSide
+				|| static_cast<std::uint64_t>(p.width) * p.height > kMaxImagePixels)
+			{
+				throw Error{LUZGPU_E_INVALID, "image size outside 1..65536 per side and 2^28 pixels"};
+			}
 			return p;
 		}
 
@@ -828,19 +863,17 @@ namespace luzgpu
 				off += static_cast<std::uint64_t>(j.params.width) * j.params.height * 3;
 			}
 			ctx->counters.reserve(8 * sizeof(unsigned long long));
-			hipEvent_t e0, e1;
-			check(hipEventCreate(&e0), "hipEventCreate");
-			check(hipEventCreate(&e1), "hipEventCreate");
+			Event e0, e1;
 			std::uint32_t trials = 0;
 			auto timeOf = [&](WaveTuning t) {
 				float ms = 0.0f;
 				for (int rep = 0; rep < 2; rep++)
 				{
-					check(hipEventRecord(e0, st), "hipEventRecord");
+					e0.record(st);
 					renderWavefront(ctx->wave, view, features, cal, ctx->counters.as<unsigned long long>(), st, false, nullptr, t);
-					check(hipEventRecord(e1, st), "hipEventRecord");
+					e1.record(st);
 					check(hipEventSynchronize(e1), "autotune");
-					check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+					ms = elapsedMs(e0, e1);
 				}
 				trials++;
 				return static_cast<double>(ms);
@@ -870,8 +903,6 @@ namespace luzgpu
 				tryValues([](WaveTuning& t, int v) { t.shadowRefill = v; }, {1, 16, 48, 64});
 				tryValues([](WaveTuning& t, int v) { t.grid = static_cast<unsigned>(v); }, {1024, 1280, 2048});
 			}
-			(void)hipEventDestroy(e0);
-			(void)hipEventDestroy(e1);
 			luzgpu_tuning r{};
 			r.traversal_grid = best.grid;
 			r.extend_refill = best.extendRefill;
@@ -1273,7 +1304,7 @@ const char*	luzgpu_last_error(void)
 	return g_lastError.c_str();
 }
 
-int	luzgpu_create(int device, luzgpu_ctx** out)
+int	luzgpu_create(luzgpu_ctx** out, int device)
 {
 	if (!out)
 	{
@@ -1440,17 +1471,13 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		zero_doubles<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(ctx->sums.as<double>(), static_cast<size_t>(pixels) * 3);
 		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
 		ctx->countsPixels = 0;
-		hipEvent_t e0, e1;
-		check(hipEventCreate(&e0), "hipEventCreate");
-		check(hipEventCreate(&e1), "hipEventCreate");
-		std::vector<hipEvent_t> ks, ke;
+		Event e0, e1;
+		std::deque<Event> ks, ke;
 		StageTimes stages;
 		auto markStart = [&]() {
 			ks.emplace_back();
 			ke.emplace_back();
-			check(hipEventCreate(&ks.back()), "hipEventCreate");
-			check(hipEventCreate(&ke.back()), "hipEventCreate");
-			check(hipEventRecord(ks.back(), st), "hipEventRecord");
+			ks.back().record(st);
 		};
 		// samples [q.sampleBegin, + q.sampleCount) of `active` pixels (`list`, or all) into the sums
 		auto renderRange = [&](const GpuParams& q, const std::uint32_t* list, std::uint32_t active, double* moments) {
@@ -1470,7 +1497,7 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 					launchMega(settings && settings->count_visits, scene->gpu.host.features, blocks(threads, kMegaBlock), st, scene->gpu.view, r,
 						list, active, ctx->sampleColors.as<float>(), ctx->counters.as<unsigned long long>());
 					check(hipGetLastError(), "render launch");
-					check(hipEventRecord(ke.back(), st), "hipEventRecord");
+					ke.back().record(st);
 					accumulate<<<blocks(active, 256), 256, 0, st>>>(ctx->sampleColors.as<float>(), ctx->sums.as<double>(), list, active,
 						r.sampleCount, moments);
 				}
@@ -1485,10 +1512,10 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 				renderWavefront(ctx->wave, scene->gpu.view, scene->gpu.host.features, {job}, ctx->counters.as<unsigned long long>(), st,
 					settings && settings->count_visits, settings && settings->stage_times ? &stages : nullptr, tune);
 				check(hipGetLastError(), "render launch");
-				check(hipEventRecord(ke.back(), st), "hipEventRecord");
+				ke.back().record(st);
 			}
 		};
-		check(hipEventRecord(e0, st), "hipEventRecord");
+		e0.record(st);
 		std::uint64_t rendered = static_cast<std::uint64_t>(pixels) * p.sampleCount;
 		AdaptiveArrays A{};
 		if (adaptive)
@@ -1510,7 +1537,7 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 		{
 			renderRange(p, nullptr, pixels, nullptr);
 		}
-		check(hipEventRecord(e1, st), "hipEventRecord");
+		e1.record(st);
 		float* dst = rgb_out;
 		if (!(settings && settings->output_on_device))
 		{
@@ -1536,19 +1563,9 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			check(hipMemcpyAsync(rgb_out, dst, static_cast<size_t>(pixels) * 3 * sizeof(float), hipMemcpyDeviceToHost, st), "image readback");
 		}
 		check(hipStreamSynchronize(st), "render");
-		float ms = 0.0f;
-		check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+		const float ms = elapsedMs(e0, e1);
 		double kernelMs = 0.0;
-		for (size_t i = 0; i < ks.size(); i++)
-		{
-			float k = 0.0f;
-			check(hipEventElapsedTime(&k, ks[i], ke[i]), "hipEventElapsedTime");
-			kernelMs += k;
-			(void)hipEventDestroy(ks[i]);
-			(void)hipEventDestroy(ke[i]);
-		}
-		(void)hipEventDestroy(e0);
-		(void)hipEventDestroy(e1);
+		for (size_t i = 0; i < ks.size(); i++) kernelMs += elapsedMs(ks[i], ke[i]);
 		ctx->countsPixels = pixels;
 		ctx->countsUniform = adaptive ? 0u : p.sampleCount;
 		ctx->countsDevice = adaptive ? A.used : nullptr;
@@ -1606,11 +1623,8 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 		zero_doubles<<<blocks(static_cast<size_t>(pixels) * kDenoiseStats, 256), 256, 0, st>>>(ctx->denoiseStats.as<double>(),
 			static_cast<size_t>(pixels) * kDenoiseStats);
 		zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
-		hipEvent_t e0, e1, e2;
-		check(hipEventCreate(&e0), "hipEventCreate");
-		check(hipEventCreate(&e1), "hipEventCreate");
-		check(hipEventCreate(&e2), "hipEventCreate");
-		check(hipEventRecord(e0, st), "hipEventRecord");
+		Event e0, e1, e2;
+		e0.record(st);
 		const std::uint32_t integrator = resolveIntegrator(settings ? settings->integrator : LUZGPU_INTEGRATOR_WAVEFRONT,
 			static_cast<std::uint64_t>(p.width) * p.height * p.sampleCount, "luzgpu_render_denoised");
 		const WaveTuning tune = tuningOf(settings).resolved();
@@ -1674,7 +1688,7 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 		denoise_finalize<<<blocks(pixels, 256), 256, 0, st>>>(ctx->denoiseStats.as<double>(), pixels, p.sampleCount,
 			adaptive ? A.used : nullptr, image, nb + L.mean, nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB,
 			nb + L.featVariance);
-		check(hipEventRecord(e1, st), "hipEventRecord");
+		e1.record(st);
 		double* out = nb + L.total;
 		runNfor(ctx, P, nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB, nb + L.featVariance, out);
 		if (settings && settings->postprocess)
@@ -1691,7 +1705,7 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 		{
 			to_float<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(out, denoised, static_cast<size_t>(pixels) * 3);
 		}
-		check(hipEventRecord(e2, st), "hipEventRecord");
+		e2.record(st);
 		const size_t imageBytes = static_cast<size_t>(pixels) * 3 * sizeof(float);
 		const hipMemcpyKind kind = (settings && settings->output_on_device) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
 		check(hipMemcpyAsync(rgb_out, image, imageBytes, kind, st), "image copy");
@@ -1717,9 +1731,6 @@ int	luzgpu_render_denoised(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_se
 		float renderMs = 0.0f, denoiseMs = 0.0f;
 		check(hipEventElapsedTime(&renderMs, e0, e1), "hipEventElapsedTime");
 		check(hipEventElapsedTime(&denoiseMs, e1, e2), "hipEventElapsedTime");
-		(void)hipEventDestroy(e0);
-		(void)hipEventDestroy(e1);
-		(void)hipEventDestroy(e2);
 		ctx->countsPixels = pixels;
 		ctx->countsUniform = adaptive ? 0u : p.sampleCount;
 		ctx->countsDevice = adaptive ? A.used : nullptr;
@@ -1759,18 +1770,14 @@ int	luzgpu_nfor(luzgpu_ctx* ctx, const luzgpu_nfor_buffers* in, const luzgpu_nfo
 		{
 			check(hipMemcpyAsync(nb + fields[k].second, fields[k].first, sizes[k] * n * sizeof(double), hipMemcpyHostToDevice, st), "buffer upload");
 		}
-		hipEvent_t e0, e1;
-		check(hipEventCreate(&e0), "hipEventCreate");
-		check(hipEventCreate(&e1), "hipEventCreate");
-		check(hipEventRecord(e0, st), "hipEventRecord");
+		Event e0, e1;
+		e0.record(st);
 		runNfor(ctx, P, nb + L.colorA, nb + L.colorB, nb + L.colorVariance, nb + L.featA, nb + L.featB, nb + L.featVariance, nb + L.total);
-		check(hipEventRecord(e1, st), "hipEventRecord");
+		e1.record(st);
 		check(hipMemcpyAsync(denoised_out, nb + L.total, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st), "denoised readback");
 		check(hipStreamSynchronize(st), "NFOR");
 		float ms = 0.0f;
 		check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-		(void)hipEventDestroy(e0);
-		(void)hipEventDestroy(e1);
 		if (elapsed_ms) *elapsed_ms = ms;
 	});
 }
@@ -1917,12 +1924,10 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 			// the calibration used the counters: clear them again
 			zero_u64<<<1, 64, 0, st>>>(ctx->counters.as<unsigned long long>(), 8);
 		}
-		hipEvent_t e0, e1;
-		check(hipEventCreate(&e0), "hipEventCreate");
-		check(hipEventCreate(&e1), "hipEventCreate");
+		Event e0, e1;
 		const bool timed = settings && settings[0].stage_times;
 		StageTimes stages;
-		check(hipEventRecord(e0, st), "hipEventRecord");
+		e0.record(st);
 		std::uint64_t rendered = 0;
 		for (const BatchJob& j : jobs) rendered += static_cast<std::uint64_t>(j.params.width) * j.params.height * j.params.sampleCount;
 		AdaptiveArrays A{};
@@ -2024,7 +2029,7 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 				settings && settings[0].count_visits, timed ? &stages : nullptr, tune);
 		}
 		check(hipGetLastError(), "batch render launch");
-		check(hipEventRecord(e1, st), "hipEventRecord");
+		e1.record(st);
 		// per-scene resolve, into device outputs directly or through the staging buffer
 		size_t hostBytes = 0;
 		for (uint32_t i = 0; i < n; i++)
@@ -2070,8 +2075,6 @@ int	luzgpu_render_batch(luzgpu_ctx* ctx, uint32_t n, luzgpu_scene* const* scenes
 		check(hipStreamSynchronize(st), "batch render");
 		float ms = 0.0f;
 		check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-		(void)hipEventDestroy(e0);
-		(void)hipEventDestroy(e1);
 		if (adaptive) ctx->countsPixels = 0;	// the per-pixel counts of the last single render are gone
 		if (stats)
 		{
@@ -2109,6 +2112,12 @@ int	luzgpu_adjoint(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* 
 		if (p.sampleBegin + p.sampleCount > p.totalSamples || p.sampleCount == 0 || p.maxBounces < 0)
 		{
 			throw Error{LUZGPU_E_INVALID, "luzgpu_adjoint: empty or out-of-range sample range"};
+		}
+		AdaptiveRule rule{};
+		if (adaptiveRule(scene->gpu.host, p, settings, rule))
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_adjoint: adaptive sampling is in effect for these settings, but the adjoint replays "
+				"every sample of every pixel; render and differentiate with adaptive = LUZGPU_ADAPTIVE_OFF"};
 		}
 		if (p.maxBounces + 1 > kAdjointRecords)
 		{

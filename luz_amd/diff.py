@@ -22,7 +22,8 @@ class RenderFunction(torch.autograd.Function):
     def forward(ctx, albedo, roughness, emission, scene, seed, samples):
         scene.set_materials(albedo, roughness, emission)
         out = torch.empty(scene.height * scene.width * 3, dtype=torch.float32, device="cuda")
-        Renderer.render(scene, seed=seed, samples=samples, out=out)
+        # adaptive sampling off: the adjoint differentiates the image of every sample
+        Renderer.render(scene, seed=seed, samples=samples, out=out, adaptive=False)
         ctx.scene, ctx.seed, ctx.samples = scene, seed, samples
         ctx.dtypes = (albedo.dtype, roughness.dtype, emission.dtype)
         ctx.devices = (albedo.device, roughness.device, emission.device)
@@ -30,7 +31,7 @@ class RenderFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad):
-        da, dr, de = ctx.scene.adjoint(grad.contiguous(), seed=ctx.seed, samples=ctx.samples)
+        da, dr, de = ctx.scene.adjoint(grad.contiguous(), seed=ctx.seed, samples=ctx.samples, adaptive=False)
         # each gradient lives where its parameter does
         return (torch.as_tensor(da, dtype=ctx.dtypes[0], device=ctx.devices[0]),
                 torch.as_tensor(dr, dtype=ctx.dtypes[1], device=ctx.devices[1]),

@@ -23,8 +23,9 @@ class Context:
     def __init__(self, device: int = 0):
         self._lib = _lib.load()
         self._h = ctypes.c_void_p()
+        self.device = int(device)
         self._last_resolution = None  # (W, H) of the last single-scene render
-        _lib.check(self._lib.luzgpu_create(device, ctypes.byref(self._h)))
+        _lib.check(self._lib.luzgpu_create(ctypes.byref(self._h), device))
 
     @property
     def handle(self):
@@ -174,19 +175,24 @@ class Scene:
                                                         e.ctypes.data if e is not None else None))
 
     def adjoint(self, dl_dimage, seed: int, samples=None, max_bounces=None, sample_begin=0, sample_count=None,
-                resolution=None):
+                resolution=None, adaptive=None):
         """Path-replay adjoint of Renderer.render with the same arguments. dl_dimage: dLoss/dImage
         (H, W, 3), a numpy array or a device tensor. Returns (d_albedo (n,3), d_roughness (n,),
-        d_emission (n,3)) as float64 numpy arrays."""
+        d_emission (n,3)) as float64 numpy arrays. The adjoint replays every sample of every
+        pixel, so it is refused (LuzGpuError) where adaptive sampling would be in effect: pass
+        adaptive=False, to this call and to the render it differentiates."""
         n = self.info["materials"]
         on_device = hasattr(dl_dimage, "data_ptr") and getattr(dl_dimage, "is_cuda", False)
         s = _settings(seed, samples, max_bounces, sample_begin, sample_count, "megakernel", on_device=on_device,
-                      resolution=resolution)
+                      resolution=resolution, adaptive=adaptive)
+        w, h = resolution if resolution is not None else (self.width, self.height)
         if on_device:
             g = dl_dimage.contiguous().float()
-            ptr = g.data_ptr()
+            ptr = _device_buffer(g, self.ctx, w * h * 3, "Scene.adjoint(dl_dimage)")
         else:
             g = np.ascontiguousarray(np.asarray(dl_dimage, dtype=np.float32))
+            if g.size != w * h * 3:
+                raise ValueError(f"Scene.adjoint(dl_dimage): expected {w * h * 3} elements (H*W*3), got {g.size}")
             ptr = g.ctypes.data
         da = np.zeros((n, 3), dtype=np.float64)
         dr = np.zeros(n, dtype=np.float64)
@@ -204,6 +210,24 @@ class SceneFile:
         lib = _lib.load()
         _lib.check(lib.luzgpu_scene_load(ctx.handle if ctx is not None else None, path.encode(), ctypes.byref(h)))
         return Scene(ctx, h)
+
+
+def _device_buffer(t, ctx, count: int, what: str) -> int:
+    """The device address of a caller tensor the library writes or reads ``count`` floats
+    through, after checking that it is one: a GPU tensor on the context's device, float32,
+    contiguous, with exactly ``count`` elements. Anything else raises ValueError here instead
+    of reaching a kernel as a bad pointer."""
+    if not hasattr(t, "data_ptr") or not getattr(t, "is_cuda", False):
+        raise ValueError(f"{what}: expected a GPU tensor (torch, device cuda:{ctx.device})")
+    if str(t.dtype) != "torch.float32":
+        raise ValueError(f"{what}: expected float32, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: expected a contiguous tensor")
+    if t.numel() != count:
+        raise ValueError(f"{what}: expected {count} elements (H*W*3), got {t.numel()}")
+    if t.device.index != ctx.device:
+        raise ValueError(f"{what}: tensor on cuda:{t.device.index}, context on cuda:{ctx.device}")
+    return t.data_ptr()
 
 
 def _settings(seed, samples, max_bounces, sample_begin, sample_count, integrator, on_device=False, resolution=None,
@@ -278,7 +302,7 @@ class Renderer:
         st = _lib.Stats()
         w, h = resolution if resolution is not None else (scene.width, scene.height)
         if out is not None:
-            ptr = out.data_ptr()
+            ptr = _device_buffer(out, scene.ctx, w * h * 3, "Renderer.render(out=...)")
             img = out
         else:
             img = np.zeros((h, w, 3), dtype=np.float32)
@@ -313,7 +337,7 @@ class Renderer:
             w, h = res if res is not None else (sc.width, sc.height)
             if outs is not None:
                 images.append(outs[i])
-                ptrs[i] = outs[i].data_ptr()
+                ptrs[i] = _device_buffer(outs[i], ctx, w * h * 3, f"Renderer.render_batch(outs[{i}])")
             else:
                 img = np.zeros((h, w, 3), dtype=np.float32)
                 images.append(img)

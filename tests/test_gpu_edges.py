@@ -70,3 +70,91 @@ def test_auto_integrator_picks_by_size(ctx, primitives, cornell):
     ref, _ = Renderer.render_batch([a, b], seeds=[2, 3], samples=[4, 2], resolutions=[(64, 48), (40, 40)],
                                    integrator="megakernel")
     assert all(np.array_equal(x, y) for x, y in zip(imgs, ref))
+
+
+def test_wrapping_sample_range_and_size_are_refused(ctx, primitives):
+    """A sample_begin near 2^32 must not wrap round the 32-bit range check; an image whose
+    pixel count overflows 32 bits must be refused, not sized modulo 2^32."""
+    from luz_amd import Renderer, SceneFile, _lib
+    scene = SceneFile.read(ctx, primitives.scene_path)
+    with pytest.raises(_lib.LuzGpuError):
+        Renderer.render(scene, seed=1, samples=8, sample_begin=0xFFFFFFF0, sample_count=32)
+    with pytest.raises(_lib.LuzGpuError):
+        Renderer.render(scene, seed=1, samples=1, resolution=(65536, 65537))
+    img, _ = Renderer.render(scene, seed=1, samples=1, resolution=(8, 8))
+    assert np.isfinite(img).all()
+
+
+def test_bad_output_buffers_raise_before_the_call(ctx, primitives):
+    """Caller tensors that are not float32, contiguous, H*W*3 on the context's device raise
+    ValueError in Python; none reaches a kernel."""
+    import torch
+    from luz_amd import Renderer, SceneFile
+    scene = SceneFile.read(ctx, primitives.scene_path)
+    n = 8 * 8 * 3
+    bad = [torch.zeros(n, dtype=torch.float32),                               # host memory
+           torch.zeros(n, dtype=torch.float64, device="cuda"),                # wrong dtype
+           torch.zeros(n * 2, dtype=torch.float32, device="cuda")[::2],       # not contiguous
+           torch.zeros(n - 3, dtype=torch.float32, device="cuda")]            # too small
+    for t in bad:
+        with pytest.raises(ValueError):
+            Renderer.render(scene, seed=1, samples=1, resolution=(8, 8), out=t)
+        with pytest.raises(ValueError):
+            Renderer.render_batch([scene], seeds=[1], samples=[1], resolutions=[(8, 8)], outs=[t])
+    with pytest.raises(ValueError):
+        scene.adjoint(np.zeros(n - 3, dtype=np.float32), seed=1, samples=1, resolution=(8, 8), adaptive=False)
+    with pytest.raises(ValueError):
+        scene.adjoint(torch.zeros(n - 3, device="cuda"), seed=1, samples=1, resolution=(8, 8), adaptive=False)
+    good = torch.zeros(n, dtype=torch.float32, device="cuda")
+    Renderer.render(scene, seed=1, samples=1, resolution=(8, 8), out=good)
+    torch.cuda.synchronize()
+    assert torch.isfinite(good).all()
+
+
+def test_adjoint_refuses_adaptive_renders(ctx, workload_dir):
+    """With adaptive sampling in effect (the scene default is on, 128 minimum samples) the
+    forward image averages each pixel over its own sample count; the adjoint replays every
+    sample, so it must refuse rather than return gradients of another image. With
+    adaptive=False both agree, and the autograd wrapper passes that itself."""
+    import os
+    import torch
+    from luz_amd import Renderer, SceneFile, _lib, diff, workloads
+    d = os.path.join(workload_dir, "diff_adaptive")
+    os.makedirs(d, exist_ok=True)
+    kinds, albedo, roughness, emission = workloads.diff_parameters()
+    text = workloads.diff_scene_text(kinds, albedo, roughness, emission, 16, 16, 200, 3)
+    path = os.path.join(d, "adaptive_on.luz")
+    with open(path, "w") as fh:
+        fh.write(text.replace("adaptive=0", "adaptive=1"))
+    scene = SceneFile.read(ctx, path)
+    g = np.ones((16, 16, 3), dtype=np.float32)
+    with pytest.raises(_lib.LuzGpuError):
+        scene.adjoint(g, seed=1, samples=200)
+    da, dr, de = scene.adjoint(g, seed=1, samples=200, adaptive=False)
+    assert np.isfinite(da).all() and np.isfinite(dr).all() and np.isfinite(de).all()
+    n = scene.info["materials"]
+    alb = torch.full((n, 3), 0.5, dtype=torch.float64, requires_grad=True)
+    rough = torch.full((n,), 0.3, dtype=torch.float64, requires_grad=True)
+    emi = torch.tensor(emission, dtype=torch.float64, requires_grad=True)
+    img = diff.render(scene, alb, rough, emi, seed=1, samples=200)
+    img.sum().backward()
+    assert torch.isfinite(alb.grad).all()
+    ref, _ = Renderer.render(scene, seed=1, samples=200, adaptive=False)
+    assert np.array_equal(img.detach().cpu().numpy(), ref)
+
+
+def test_scene_outlives_its_context():
+    """luzgpu_destroy frees the device arrays of scenes still loaded; the handles stay valid
+    for host-side queries and release (a path that once crashed the host process)."""
+    from luz_amd import Context, SceneFile, workloads
+    import tempfile
+    d = tempfile.mkdtemp(prefix="luz_outlive_")
+    w = workloads.write_cornell(d)
+    c = Context(0)
+    scene = SceneFile.read(c, w.scene_path)
+    topo = scene.tlas_topology()
+    c.close()
+    assert scene.tlas_topology() == topo
+    assert scene.post_settings().view_transform == 3
+    scene.release()
+    scene.release()

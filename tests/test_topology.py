@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN
-from luz_amd import SceneFile
+from luz_amd import SceneFile, _lib
 
 
 def test_cornell_tlas_matches_reference(cornell):
@@ -157,3 +157,51 @@ def test_scene_file_accepts_atmosphere(tmp_path, atmosphere):
         p.write_text(f"[settings]\n{settings}\n\n[scene]\ncamera c {{\nposition=(0,0,0)\n}}\n")
         SceneFile.read(None, str(p)).release()
     SceneFile.read(None, atmosphere.scene_path).release()
+
+
+def test_atmosphere_march_length_is_capped(tmp_path):
+    """The device marches SAMPLES x LIGHT_SAMPLES FP64 steps per miss ray in one launch; the
+    loader refuses values above 4096 each as unsupported (the reference would only slow
+    down) and accepts the cap itself."""
+    line = "sky=atmosphere\natmosphere=0.25,12840000.0,12910000.0,7994.0,1200.0,{},{},0.1"
+    for s, l in ((4097, 6), (12, 5000)):
+        p = tmp_path / "long.luz"
+        p.write_text(f"[settings]\n{line.format(s, l)}\n\n[scene]\ncamera c {{\nposition=(0,0,0)\n}}\n")
+        with pytest.raises(_lib.LuzGpuError) as e:
+            SceneFile.read(None, str(p))
+        assert e.value.code == _lib.E_UNSUPPORTED
+    p = tmp_path / "cap.luz"
+    p.write_text(f"[settings]\n{line.format(4096, 4096)}\n\n[scene]\ncamera c {{\nposition=(0,0,0)\n}}\n")
+    SceneFile.read(None, str(p)).release()
+
+
+class _FakeTensor:
+    """Stands in for a device tensor on a machine without a GPU."""
+
+    def __init__(self, is_cuda=True, dtype="torch.float32", contiguous=True, numel=12, index=0):
+        import types
+        self.is_cuda, self.dtype, self._c, self._n = is_cuda, dtype, contiguous, numel
+        self.device = types.SimpleNamespace(index=index)
+
+    def data_ptr(self):
+        return 4096
+
+    def is_contiguous(self):
+        return self._c
+
+    def numel(self):
+        return self._n
+
+
+def test_device_buffer_checks_caller_tensors():
+    """Renderer.render(out=), render_batch(outs=) and Scene.adjoint(dl_dimage) pass caller
+    tensors to kernels as raw pointers; each property that makes such a pointer bad raises
+    ValueError first."""
+    import types
+    from luz_amd.renderer import _device_buffer
+    ctx = types.SimpleNamespace(device=0)
+    assert _device_buffer(_FakeTensor(), ctx, 12, "out") == 4096
+    for bad in (_FakeTensor(is_cuda=False), _FakeTensor(dtype="torch.float64"), _FakeTensor(contiguous=False),
+                _FakeTensor(numel=9), _FakeTensor(index=1), np.zeros(12, dtype=np.float32)):
+        with pytest.raises(ValueError):
+            _device_buffer(bad, ctx, 12, "out")
