@@ -206,6 +206,20 @@ int		luzgpu_scene_get_info(const luzgpu_scene* scene, luzgpu_scene_info* out);
 int		luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings,
 			float* rgb_out, luzgpu_stats* stats);
 
+/* luzgpu_render on the plain kernel without allocating anything. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). Its sums, counters and
+ * per-sample colours live in one fixed device array per GPU (about 48 MiB, held until the
+ * process ends), so an image has at most 1,048,576 pixels (LUZGPU_E_INVALID beyond), and
+ * adaptive sampling and post-processing (which need per-pixel state) are refused with
+ * LUZGPU_E_UNSUPPORTED, as is settings->integrator = LUZGPU_INTEGRATOR_WAVEFRONT. The image
+ * equals luzgpu_render's with LUZGPU_INTEGRATOR_MEGAKERNEL. Every call on one GPU, from any
+ * context, shares that array: calls that overlap on one GPU race. stats->render_ms is host
+ * wall time (no HIP events are created). With output_on_device = 0 the image is copied to
+ * host memory through the runtime's staging; pass a device pointer to avoid it. */
+int		luzgpu_render_noalloc(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings,
+			float* rgb_out, luzgpu_stats* stats);
+
 /* Renders n scenes (1..127) in one multi-scene wavefront launch sequence: the scenes'
  * device arrays are merged (and cached while the same scenes are passed again), every pass
  * covers a slice of samples of every scene, and the pass's launches are replayed from a

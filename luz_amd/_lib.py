@@ -28,7 +28,7 @@ EXPORTED = [
     "luzgpu_scene_release", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_primary_hits",
     "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas", "luzgpu_render_batch",
     "luzgpu_adjoint", "luzgpu_scene_set_materials", "luzgpu_autotune", "luzgpu_render_denoised", "luzgpu_nfor",
-    "luzgpu_sample_counts", "luzgpu_scene_post_settings", "luzgpu_postprocess",
+    "luzgpu_sample_counts", "luzgpu_scene_post_settings", "luzgpu_postprocess", "luzgpu_render_noalloc",
 ]
 
 
@@ -171,6 +171,7 @@ def load() -> ctypes.CDLL:
     lib.luzgpu_scene_release.restype = None
     lib.luzgpu_scene_get_info.argtypes = [vp, ctypes.POINTER(SceneInfo)]
     lib.luzgpu_render.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, ctypes.POINTER(Stats)]
+    lib.luzgpu_render_noalloc.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, ctypes.POINTER(Stats)]
     lib.luzgpu_primary_hits.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, vp]
     lib.luzgpu_adjoint.argtypes = [vp, vp, ctypes.POINTER(Settings), vp, vp, vp, vp]
     lib.luzgpu_scene_set_materials.argtypes = [vp, ctypes.c_uint32, vp, vp, vp]
@@ -188,8 +189,8 @@ def load() -> ctypes.CDLL:
     lib.luzgpu_scene_tlas_topology.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     lib.luzgpu_scene_blas.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64)]
-    for name in ("luzgpu_create", "luzgpu_scene_load", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_render_batch",
-                 "luzgpu_adjoint", "luzgpu_scene_set_materials",
+    for name in ("luzgpu_create", "luzgpu_scene_load", "luzgpu_scene_get_info", "luzgpu_render", "luzgpu_render_noalloc",
+                 "luzgpu_render_batch", "luzgpu_adjoint", "luzgpu_scene_set_materials",
                  "luzgpu_primary_hits", "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib

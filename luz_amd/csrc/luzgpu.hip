@@ -25,6 +25,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -34,6 +35,13 @@ namespace luzgpu
 	namespace
 	{
 		thread_local std::string	g_lastError;
+
+		// Device addresses of `scratch` (luzgpu_render_noalloc), one per device, looked up by
+		// luzgpu_create with that device current: in host code the array's name is only a
+		// stand-in, not a device address.
+		constexpr int	kScratchDevices = 64;
+		std::mutex	g_scratchMutex;
+		void*	g_scratchPtr[kScratchDevices] = {};
 
 		struct	Error
 		{
@@ -119,6 +127,21 @@ namespace luzgpu
 	constexpr std::uint32_t	kMaxImageSide = 65536;
 	constexpr std::uint64_t	kMaxImagePixels = 1ull << 28;
 	constexpr int	kStackDepth = 48;
+
+	// Device memory of luzgpu_render_noalloc, which allocates nothing: one fixed array per
+	// device, held until the process ends. Layout: 8 ray / visit counters, the FP64 pixel
+	// sums of at most kNoAllocMaxPixels pixels, then the per-sample colours of one plain-kernel
+	// launch (kNoAllocStagingSamples of them), which also hold the resolved fp32 image before
+	// a readback to host memory. Every call on a device shares that device's copy.
+	constexpr std::uint32_t	kNoAllocMaxPixels = 1u << 20;
+	constexpr std::uint32_t	kNoAllocStagingSamples = 2u << 20;
+	constexpr size_t	kNoAllocCountersBytes = 256;
+	constexpr size_t	kNoAllocSumsBytes = static_cast<size_t>(kNoAllocMaxPixels) * 3 * sizeof(double);
+	constexpr size_t	kNoAllocStagingBytes = static_cast<size_t>(kNoAllocStagingSamples) * 3 * sizeof(float);
+	constexpr size_t	kNoAllocScratchBytes = kNoAllocCountersBytes + kNoAllocSumsBytes + kNoAllocStagingBytes;
+	static_assert(kNoAllocStagingSamples >= kNoAllocMaxPixels, "the staging area must hold one fp32 image");
+
+	static __device__ __align__(256) unsigned char	scratch[kNoAllocScratchBytes];
 
 	// counters: [0] closest-hit rays, [1] shadow rays, [2] nodes visited, [3] primitives tested.
 	// Renders samples [sampleBegin, sampleBegin + sampleCount) of `pixels` pixels: entries of
@@ -1320,6 +1343,17 @@ int	luzgpu_create(luzgpu_ctx** out, int device)
 			throw Error{LUZGPU_E_INVALID, "luzgpu_create: device " + std::to_string(device) + " not present"};
 		}
 		check(hipSetDevice(device), "hipSetDevice");
+		if (device < kScratchDevices)
+		{
+			// once per device, with the device current: the device address of its `scratch`
+			std::lock_guard<std::mutex> lock(g_scratchMutex);
+			if (!g_scratchPtr[device])
+			{
+				void* ptr = nullptr;
+				check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+				g_scratchPtr[device] = ptr;
+			}
+		}
 		auto ctx = std::make_unique<luzgpu_ctx>();
 		ctx->device = device;
 		check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -1586,6 +1620,106 @@ int	luzgpu_render(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* s
 			stats->shadow_ms = stages.shadowMs;
 			stats->extend_launches = stages.extendLaunches;
 			stats->tune_ms = tuneMs;
+		}
+	});
+}
+
+int	luzgpu_render_noalloc(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* settings, float* rgb_out, luzgpu_stats* stats)
+{
+	if (!ctx || !scene || !rgb_out || scene->ctx != ctx)
+	{
+		g_lastError = "luzgpu_render_noalloc: NULL argument or scene from another context";
+		return LUZGPU_E_INVALID;
+	}
+	return guarded([&] {
+		check(hipSetDevice(ctx->device), "hipSetDevice");
+		void* base = ctx->device < kScratchDevices ? g_scratchPtr[ctx->device] : nullptr;
+		if (!base)
+		{
+			throw Error{LUZGPU_E_DEVICE, "luzgpu_render_noalloc: no scratch address for device " + std::to_string(ctx->device)};
+		}
+		const GpuParams p = resolveParams(scene->gpu, settings);
+		if (p.sampleCount == 0)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render_noalloc: empty sample range"};
+		}
+		if (p.maxBounces < 0)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render_noalloc: negative max_bounces"};
+		}
+		const std::uint32_t requested = settings ? settings->integrator : LUZGPU_INTEGRATOR_MEGAKERNEL;
+		if (requested != LUZGPU_INTEGRATOR_MEGAKERNEL && requested != LUZGPU_INTEGRATOR_AUTO)
+		{
+			throw Error{LUZGPU_E_UNSUPPORTED, "luzgpu_render_noalloc: only the plain kernel (LUZGPU_INTEGRATOR_MEGAKERNEL) renders without allocating"};
+		}
+		const std::uint32_t pixels = p.width * p.height;
+		if (pixels > kNoAllocMaxPixels)
+		{
+			throw Error{LUZGPU_E_INVALID, "luzgpu_render_noalloc: " + std::to_string(pixels) + " pixels exceed the fixed scratch's limit of "
+				+ std::to_string(kNoAllocMaxPixels) + " (1024 x 1024)"};
+		}
+		AdaptiveRule rule{};
+		if (adaptiveRule(scene->gpu.host, p, settings, rule))
+		{
+			throw Error{LUZGPU_E_UNSUPPORTED, "luzgpu_render_noalloc: adaptive sampling needs per-pixel state; pass adaptive = LUZGPU_ADAPTIVE_OFF"};
+		}
+		if (settings && settings->postprocess)
+		{
+			throw Error{LUZGPU_E_UNSUPPORTED, "luzgpu_render_noalloc: display post-processing is not available without allocation"};
+		}
+		unsigned char* bytes = static_cast<unsigned char*>(base);
+		unsigned long long* counters = reinterpret_cast<unsigned long long*>(bytes);
+		double* sums = reinterpret_cast<double*>(bytes + kNoAllocCountersBytes);
+		float* staging = reinterpret_cast<float*>(bytes + kNoAllocCountersBytes + kNoAllocSumsBytes);
+		hipStream_t st = ctx->stream;
+		const auto t0 = std::chrono::steady_clock::now();
+		// the previous call's contents stay in the array: clear what starts at zero
+		zero_doubles<<<blocks(static_cast<size_t>(pixels) * 3, 256), 256, 0, st>>>(sums, static_cast<size_t>(pixels) * 3);
+		zero_u64<<<1, 64, 0, st>>>(counters, 8);
+		// samples per launch: the per-sample colours of one launch fit the staging area
+		const std::uint32_t perLaunch = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(p.sampleCount, kNoAllocStagingSamples / pixels));
+		std::uint32_t launches = 0;
+		for (std::uint32_t done = 0; done < p.sampleCount; done += perLaunch)
+		{
+			GpuParams r = p;
+			r.sampleBegin = p.sampleBegin + done;
+			r.sampleCount = std::min(perLaunch, p.sampleCount - done);
+			const size_t threads = static_cast<size_t>(pixels) * r.sampleCount;
+			launchMega(settings && settings->count_visits, scene->gpu.host.features, blocks(threads, kMegaBlock), st, scene->gpu.view, r,
+				nullptr, pixels, staging, counters);
+			check(hipGetLastError(), "render launch");
+			accumulate<<<blocks(pixels, 256), 256, 0, st>>>(staging, sums, nullptr, pixels, r.sampleCount, nullptr);
+			launches++;
+		}
+		const bool onDevice = settings && settings->output_on_device;
+		float* dst = onDevice ? rgb_out : staging;
+		resolve<<<blocks(pixels, 256), 256, 0, st>>>(sums, dst, pixels, static_cast<double>(p.sampleCount));
+		check(hipGetLastError(), "resolve launch");
+		unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
+		check(hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, st), "counter readback");
+		if (!onDevice)
+		{
+			check(hipMemcpyAsync(rgb_out, staging, static_cast<size_t>(pixels) * 3 * sizeof(float), hipMemcpyDeviceToHost, st), "image readback");
+		}
+		check(hipStreamSynchronize(st), "render");
+		const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		// the per-pixel counts of this render are uniform; they no longer point at adaptive state
+		ctx->countsPixels = pixels;
+		ctx->countsUniform = p.sampleCount;
+		ctx->countsDevice = nullptr;
+		if (stats)
+		{
+			*stats = luzgpu_stats{};
+			stats->rendered_samples = static_cast<std::uint64_t>(pixels) * p.sampleCount;
+			stats->closest_hit_rays = c[0];
+			stats->shadow_rays = c[1];
+			stats->render_ms = ms;
+			stats->kernel_ms = ms;
+			stats->kernel_launches = launches;
+			stats->nodes_visited = c[2] + c[4];
+			stats->prims_tested = c[3] + c[5];
+			stats->closest_nodes = c[2];
+			stats->closest_prims = c[3];
 		}
 	});
 }
