@@ -593,6 +593,8 @@ namespace luzgpu
 		}
 		if (m.kind == GPU_MAT_PRINCIPLED)
 		{
+			// setMediumAbsorption (Principled.cpp:129-140): transmissive with a positive coefficient
+			sc.absorbing = (m.flags & GPU_MATF_ABSORPTION) != 0 && m.transmission > 0.0f;
 			const f3 base = clampColor01(ld3(m.color));
 			if (m.flags & GPU_MATF_SMOOTH_TRANSMISSION)
 			{

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <cmath>
 #include <functional>
 #include <limits>
@@ -959,7 +960,8 @@ namespace luzgpu
 				g.color[i] = static_cast<float>(m.color[i]);
 				g.eta[i] = static_cast<float>(m.eta[i]);
 				g.k[i] = static_cast<float>(m.k[i]);
-				g.absorption[i] = static_cast<float>(m.absorption[i]);
+				// a fully opaque interior (coefficient T_MAX) stays infinite in fp32
+				g.absorption[i] = m.absorption[i] > static_cast<double>(FLT_MAX) ? INFINITY : static_cast<float>(m.absorption[i]);
 			}
 			// BSDF.cpp:54-58: alpha = max(1e-4, clamp01(r)^2)
 			const double r = std::isfinite(m.roughness) ? std::max(0.0, std::min(1.0, m.roughness)) : 0.0;

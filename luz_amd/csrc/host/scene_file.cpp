@@ -19,7 +19,9 @@
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <limits>
 #include <map>
+#include <optional>
 #include <sstream>
 #include <stdexcept>
 
@@ -302,6 +304,416 @@ namespace luzgpu
 			}
 			const double len = std::sqrt(ls);
 			return (n * (1.0 / len));
+		}
+
+		// --- materials: measured presets, glass dispersion and the material builder ---
+
+		// MeasuredMaterials tables (MeasuredMaterials.cpp:53-150): conductor eta / k per RGB
+		// channel, glass nd / Abbe vd with Sellmeier B / C (micrometres^2) where known. The
+		// rendered colours are defined by these numbers, so they are the same numbers.
+		struct	ConductorPreset { const char* name; dvec3 eta, k; };
+		struct	GlassPreset { const char* name; double nd, vd; bool sellmeier; dvec3 b, c; };
+
+		const ConductorPreset	kConductors[] = {
+			{"aluminum", {1.44, 0.97, 0.62}, {7.38, 6.40, 5.30}},
+			{"aluminium", {1.44, 0.97, 0.62}, {7.38, 6.40, 5.30}},
+			{"copper", {0.27, 0.68, 1.34}, {3.57, 2.62, 2.29}},
+			{"cu", {0.27, 0.68, 1.34}, {3.57, 2.62, 2.29}},
+			{"gold", {0.17, 0.47, 1.54}, {3.14, 2.37, 1.91}},
+			{"au", {0.17, 0.47, 1.54}, {3.14, 2.37, 1.91}},
+			{"silver", {0.14, 0.16, 0.13}, {4.10, 3.14, 2.30}},
+			{"ag", {0.14, 0.16, 0.13}, {4.10, 3.14, 2.30}},
+			{"iron", {2.91, 2.95, 2.82}, {3.09, 2.93, 2.81}},
+			{"fe", {2.91, 2.95, 2.82}, {3.09, 2.93, 2.81}},
+			{"nickel", {1.92, 1.83, 1.62}, {3.67, 3.38, 3.04}},
+			{"ni", {1.92, 1.83, 1.62}, {3.67, 3.38, 3.04}},
+			{"chromium", {3.21, 3.18, 2.99}, {3.30, 3.32, 3.33}},
+			{"chrome", {3.21, 3.18, 2.99}, {3.30, 3.32, 3.33}},
+		};
+
+		const GlassPreset	kGlasses[] = {
+			{"bk7", 1.5168, 64.17, true, {1.03961212, 0.231792344, 1.01046945}, {0.00600069867, 0.0200179144, 103.560653}},
+			{"borosilicate", 1.474, 65.0, false, {}, {}},
+			{"fusedsilica", 1.4585, 67.82, true, {0.6961663, 0.4079426, 0.8974794}, {0.00467914826, 0.0135120631, 97.9340025}},
+			{"silica", 1.4585, 67.82, true, {0.6961663, 0.4079426, 0.8974794}, {0.00467914826, 0.0135120631, 97.9340025}},
+			{"water", 1.333, 55.8, false, {}, {}},
+			{"diamond", 2.417, 55.3, false, {}, {}},
+			{"sapphire", 1.773, 72.2, false, {}, {}},
+		};
+
+		// preset names compare lower-case without '_', '-' or spaces (MeasuredMaterials.cpp:36-51)
+		std::string	presetKey(const std::string& name)
+		{
+			std::string r;
+			for (char ch : name)
+			{
+				const unsigned char u = static_cast<unsigned char>(ch);
+				if (ch == '_' || ch == '-' || std::isspace(u)) continue;
+				r.push_back(static_cast<char>(std::tolower(u)));
+			}
+			return (r);
+		}
+
+		template <typename T, std::size_t N>
+		const T&	findPreset(const T (&table)[N], const std::string& name, const char* what)
+		{
+			const std::string key = presetKey(name);
+			for (const T& t : table)
+			{
+				if (presetKey(t.name) == key) return (t);
+			}
+			std::string names;
+			for (const T& t : table) names += (names.empty() ? "" : ", ") + std::string(t.name);
+			throw std::invalid_argument(std::string("Unknown ") + what + " preset '" + name + "'. Available presets: " + names + ".");
+		}
+
+		// parseNanometers (Materials.cpp:95-115): "587.56" or "587.56nm"
+		double	parseNanometers(std::string value, const std::string& label)
+		{
+			value = trim(value);
+			const std::string l = lower(value);
+			if (l.size() > 2 && l.compare(l.size() - 2, 2, "nm") == 0) value = trim(value.substr(0, value.size() - 2));
+			std::size_t used = 0;
+			const double nm = std::stod(value, &used);
+			if (used != value.size() || !std::isfinite(nm) || nm <= 0.0)
+			{
+				throw std::runtime_error(label + " must be a finite positive wavelength in nm.");
+			}
+			return (nm);
+		}
+
+		// refractiveIndexFromSellmeier (MeasuredMaterials.cpp:254-283):
+		// n^2 = 1 + sum_i B_i l^2 / (l^2 - C_i), l in micrometres
+		double	sellmeierIndex(const dvec3& b, const dvec3& c, double nm)
+		{
+			if (!std::isfinite(nm) || nm <= 0.0) throw std::invalid_argument("Sellmeier wavelength must be finite and positive.");
+			const double um = nm * 0.001;
+			const double l2 = um * um;
+			double n2 = 1.0;
+			for (int i = 0; i < 3; i++)
+			{
+				if (!std::isfinite(b[i]) || !std::isfinite(c[i]) || c[i] < 0.0)
+				{
+					throw std::invalid_argument("Sellmeier coefficients must be finite, with non-negative C terms.");
+				}
+				const double den = l2 - c[i];
+				if (std::fabs(den) <= 1e-12) throw std::invalid_argument("Sellmeier wavelength hits a resonance denominator.");
+				n2 += (b[i] * l2) / den;
+			}
+			if (!std::isfinite(n2) || n2 <= 0.0) throw std::invalid_argument("Sellmeier coefficients produced an invalid refractive index.");
+			return (std::sqrt(n2));
+		}
+
+		// refractiveIndexFromAbbe (MeasuredMaterials.cpp:285-304): the two-term Cauchy law
+		// n(l) = A + B / l^2 through n_d at the d line with (n_d - 1) / V_d = n_F - n_C.
+		double	abbeIndex(double nd, double vd, double nm)
+		{
+			auto positive = [](double v, const char* label) {
+				if (!std::isfinite(v) || v <= 0.0) throw std::invalid_argument(std::string(label) + " must be finite and positive.");
+			};
+			positive(nd, "Abbe refractive index");
+			positive(vd, "Abbe number");
+			positive(nm, "Abbe wavelength");
+			const double lF = 486.1327, lD = 587.5618, lC = 656.2725;
+			const double dispersion = (nd - 1.0) / vd;
+			const double b = dispersion / ((1.0 / (lF * lF)) - (1.0 / (lC * lC)));
+			const double a = nd - (b / (lD * lD));
+			return (a + (b / (nm * nm)));
+		}
+
+		// The keys of one material block, as parseMaterialProperty collects them
+		// (MaterialBuilder, Materials.cpp:26-75).
+		struct	MaterialKeys
+		{
+			std::string	type;
+			dvec3	color = {0.6, 0.6, 0.6};
+			dvec3	emissionColor = {1.0, 1.0, 1.0};
+			bool	hasEmissionColor = false, hasProperties = false;
+			double	roughness = 0.0, fuzz = -1.0;
+			double	ior = 1.52;	// RI_GLASS (RefractiveIndexes.hpp:3)
+			bool	hasIor = false;
+			double	metallic = 0.0, transmission = 0.0, clearcoat = 0.0, clearcoatRoughness = 0.03, sheen = 0.0;
+			double	glossyWeight = 0.5, alphaKey = 1.0;
+			bool	hasSubsurface = false;
+			double	subsurface = 0.0;
+			dvec3	glossyColor = {1.0, 1.0, 1.0};
+			bool	hasGlossyColor = false;
+			std::optional<double>	radiance, lum;
+			double	anisotropy = 0.0;
+			std::optional<dvec3>	eta, extinction, absorption, transmittance;
+			std::string	materialPreset, conductorPreset, glassPreset;
+			std::optional<double>	abbe;
+			std::optional<dvec3>	sellmeierB, sellmeierC;
+			double	iorWavelength = 587.5618;
+			bool	hasIorWavelength = false;
+			double	transmittanceDistance = 1.0;
+			bool	hasTransmittanceDistance = false;
+		};
+
+		// materialRefractiveIndex (Materials.cpp:251-320)
+		double	materialIor(const MaterialKeys& k, const std::string& type)
+		{
+			const GlassPreset* glass = nullptr;
+			if (!k.glassPreset.empty()) glass = &findPreset(kGlasses, k.glassPreset, "glass");
+			else if (!k.materialPreset.empty() && (type == "dielectric" || type == "principled")) glass = &findPreset(kGlasses, k.materialPreset, "glass");
+			if (glass && k.hasIor) throw std::runtime_error("Material defines both glass preset and explicit ior.");
+			if (glass && k.abbe) throw std::runtime_error("Material defines both glass preset and explicit Abbe number.");
+			if (glass && (k.sellmeierB || k.sellmeierC)) throw std::runtime_error("Material defines both glass preset and explicit Sellmeier coefficients.");
+			if (k.abbe && (k.sellmeierB || k.sellmeierC)) throw std::runtime_error("Material defines both Abbe and Sellmeier dispersion.");
+			if (k.sellmeierB || k.sellmeierC)
+			{
+				if (!k.sellmeierB || !k.sellmeierC) throw std::runtime_error("Sellmeier dispersion requires both sellmeier_b and sellmeier_c.");
+				return (sellmeierIndex(*k.sellmeierB, *k.sellmeierC, k.iorWavelength));
+			}
+			if (glass && glass->sellmeier) return (sellmeierIndex(glass->b, glass->c, k.iorWavelength));
+			if (k.abbe) return (abbeIndex(glass ? glass->nd : k.ior, *k.abbe, k.iorWavelength));
+			if (glass)
+			{
+				if (k.hasIorWavelength && glass->vd > 0.0) return (abbeIndex(glass->nd, glass->vd, k.iorWavelength));
+				if (k.hasIorWavelength) throw std::runtime_error("ior_wavelength requires Sellmeier or Abbe dispersion.");
+				return (glass->nd);
+			}
+			if (k.hasIorWavelength) throw std::runtime_error("ior_wavelength requires a glass preset, Abbe number, or Sellmeier coefficients.");
+			return (k.ior);
+		}
+
+		// absorptionFromTransmittance (Dielectric.cpp:42-53, Principled.cpp): sigma_a = -ln(T) / d,
+		// T = 0 -> T_MAX
+		dvec3	absorptionOf(const MaterialKeys& k, const char* who)
+		{
+			if (k.absorption && k.transmittance) throw std::runtime_error(std::string(who) + " material defines both absorption and transmittance.");
+			if (k.hasTransmittanceDistance && !k.transmittance) throw std::runtime_error(std::string(who) + " attenuation distance requires transmittance.");
+			if (k.absorption)
+			{
+				const dvec3 a = *k.absorption;
+				for (int i = 0; i < 3; i++)
+				{
+					if (!std::isfinite(a[i]) || a[i] < 0.0) throw std::invalid_argument(std::string(who) + " absorption coefficient must have finite non-negative channels.");
+				}
+				return (a);
+			}
+			if (k.transmittance)
+			{
+				if (!std::isfinite(k.transmittanceDistance) || k.transmittanceDistance <= 0.0)
+				{
+					throw std::invalid_argument(std::string(who) + " transmittance distance must be finite and positive.");
+				}
+				dvec3 a{};
+				for (int i = 0; i < 3; i++)
+				{
+					const double t = (*k.transmittance)[i];
+					if (!std::isfinite(t) || t < 0.0 || t > 1.0) throw std::invalid_argument(std::string(who) + " transmittance channels must be finite and in [0,1].");
+					a[i] = t <= 0.0 ? std::numeric_limits<double>::max() : -std::log(t) / k.transmittanceDistance;
+				}
+				return (a);
+			}
+			return {0.0, 0.0, 0.0};
+		}
+
+		double	unitValue(double v, const std::string& label)
+		{
+			if (!std::isfinite(v) || v < 0.0 || v > 1.0) throw std::invalid_argument(label + " must be finite and in [0,1].");
+			return (v);
+		}
+
+		void	conductorFrom(Material& m, const dvec3& eta, const dvec3& k, double roughness)
+		{
+			// Metal(eta, k, roughness), Metal.cpp:82-90, 118-125
+			for (int i = 0; i < 3; i++)
+			{
+				if (!std::isfinite(eta[i]) || eta[i] <= 0.0) throw std::invalid_argument("Conductor eta must have finite positive channels.");
+				if (!std::isfinite(k[i]) || k[i] < 0.0) throw std::invalid_argument("Conductor extinction coefficient must have finite non-negative channels.");
+			}
+			m.kind = MAT_METAL;
+			m.color = {1.0, 1.0, 1.0};
+			m.conductor = true;
+			m.eta = eta;
+			m.k = k;
+			m.roughness = unitValue(roughness, "Metal roughness");
+		}
+
+		// buildMaterial (Materials.cpp:757-889) with buildMetalMaterial (:660-691),
+		// buildDielectricMaterial (:323-346), buildPrincipledMaterial (:693-755) and
+		// buildEmissiveMaterial (:210-224, LightUnits.cpp:36-73)
+		Material	buildMaterial(const MaterialKeys& k)
+		{
+			if (!k.hasProperties) throw std::runtime_error("Material block ended before a material was defined.");
+			const std::string type = k.type.empty() ? "lambertian" : k.type;
+			if (!k.materialPreset.empty() && type != "metal" && type != "dielectric" && type != "principled")
+			{
+				throw std::runtime_error("Material preset is only valid for metal, dielectric, or principled materials.");
+			}
+			if (!k.conductorPreset.empty() && type != "metal") throw std::runtime_error("Conductor preset is only valid for metal materials.");
+			if ((!k.glassPreset.empty() || k.abbe || k.sellmeierB || k.sellmeierC || k.hasIorWavelength) && type != "dielectric" && type != "principled")
+			{
+				throw std::runtime_error("Glass dispersion properties are only valid for dielectric or principled materials.");
+			}
+			if (k.hasSubsurface && type != "principled") throw std::runtime_error("Subsurface properties are only valid for principled materials.");
+			Material m;
+			m.color = k.color;
+			if (type == "principled")
+			{
+				if (k.eta || k.extinction) throw std::runtime_error("Principled conductor metals use metallic color; use type=metal for eta/k.");
+				m.absorption = absorptionOf(k, "Principled");
+				if (k.hasSubsurface && k.subsurface != 0.0) throw std::runtime_error("Unsupported for the device path: principled subsurface");
+				m.kind = MAT_PRINCIPLED;
+				m.metallic = unitValue(k.metallic, "Principled metallic");
+				m.roughness = unitValue(k.roughness, "Principled roughness");
+				m.transmission = unitValue(std::max(k.transmission, 1.0 - k.alphaKey), "Principled transmission");
+				const double ior = materialIor(k, "principled");
+				if (!std::isfinite(ior) || ior <= 0.0) throw std::invalid_argument("Principled refractive index must be finite and positive.");
+				m.ior = ior;
+				m.clearcoat = unitValue(k.clearcoat, "Principled clearcoat");
+				m.clearcoatRoughness = unitValue(k.clearcoatRoughness, "Principled clearcoat roughness");
+				m.sheen = unitValue(k.sheen, "Principled sheen");
+				return (m);
+			}
+			if (type == "glossy")
+			{
+				m.kind = MAT_GLOSSY;
+				m.roughness = unitValue(k.roughness, "Glossy roughness");
+				return (m);
+			}
+			if (type == "diffuse_glossy" || type == "diffuseglossy")
+			{
+				// DiffuseGlossy(color, glossyColor or color, weight, roughness)
+				m.kind = MAT_DIFFUSE_GLOSSY;
+				m.glossyColor = k.hasGlossyColor ? k.glossyColor : k.color;
+				m.glossyWeight = unitValue(k.glossyWeight, "Diffuse-glossy weight");
+				m.roughness = unitValue(k.roughness, "Diffuse-glossy roughness");
+				return (m);
+			}
+			if (type == "lambertian")
+			{
+				m.kind = MAT_LAMBERTIAN;
+				return (m);
+			}
+			if (type == "metal")
+			{
+				const double r = k.fuzz >= 0.0 ? k.fuzz : k.roughness;
+				const std::string preset = !k.conductorPreset.empty() ? k.conductorPreset : k.materialPreset;
+				if (!preset.empty())
+				{
+					if (k.eta || k.extinction) throw std::runtime_error("Metal material defines both conductor preset and explicit eta/k.");
+					const ConductorPreset& c = findPreset(kConductors, preset, "conductor");
+					conductorFrom(m, c.eta, c.k, r);
+					return (m);
+				}
+				if (k.eta || k.extinction)
+				{
+					if (!k.eta || !k.extinction) throw std::runtime_error("Conductor metal requires both eta and k.");
+					conductorFrom(m, *k.eta, *k.extinction, r);
+					return (m);
+				}
+				m.kind = MAT_METAL;
+				m.roughness = unitValue(r, "Metal roughness");
+				return (m);
+			}
+			if (type == "dielectric")
+			{
+				m.absorption = absorptionOf(k, "Dielectric");
+				const double ior = materialIor(k, "dielectric");
+				if (!std::isfinite(ior) || ior <= 0.0) throw std::invalid_argument("Dielectric refractive index must be finite and positive.");
+				m.kind = MAT_DIELECTRIC;
+				m.ior = ior;
+				m.roughness = unitValue(k.roughness, "Dielectric roughness");
+				return (m);
+			}
+			if (type == "emissive")
+			{
+				const dvec3 c = k.hasEmissionColor ? k.emissionColor : k.color;
+				double target;
+				if (k.radiance) target = *k.radiance;
+				else if (k.lum) target = *k.lum / 683.0;
+				else throw std::runtime_error("Emissive material requires radiance or luminance.");
+				if (!std::isfinite(target) || target < 0.0) throw std::invalid_argument("Light radiance must be finite and non-negative.");
+				m.kind = MAT_EMISSIVE;
+				if (target == 0.0)
+				{
+					m.color = {0.0, 0.0, 0.0};
+				}
+				else
+				{
+					const double cl = luminance(c);
+					if (!(cl > 0.0)) throw std::runtime_error("Light color must have positive luminance for physical units.");
+					const double f = target / cl;
+					m.color = {c[0] * f, c[1] * f, c[2] * f};
+				}
+				return (m);
+			}
+			if (type == "isotropic")
+			{
+				m.kind = MAT_ISOTROPIC;
+				return (m);
+			}
+			if (type == "volume" || type == "phase" || type == "henyey_greenstein" || type == "henyeygreenstein" || type == "hg")
+			{
+				// HenyeyGreenstein.cpp:10-18
+				if (!std::isfinite(k.anisotropy) || k.anisotropy < -0.99 || k.anisotropy > 0.99)
+				{
+					throw std::invalid_argument("Volume anisotropy must be finite and between -0.99 and 0.99.");
+				}
+				m.kind = MAT_HG;
+				m.anisotropy = k.anisotropy;
+				return (m);
+			}
+			throw std::runtime_error("Unknown material type: " + type);
+		}
+
+		// parseDirectMaterialLine (Materials.cpp:117-196): `lambertian=(r,g,b)`,
+		// `metal=(r,g,b),fuzz`, `glossy=(r,g,b),roughness`, `dielectric=(r,g,b)`,
+		// `isotropic=(r,g,b)`, `hg=(r,g,b),g`; false when the line is none of them
+		bool	directMaterialLine(const std::string& line, Material& m)
+		{
+			const std::string l = lower(trim(line));
+			double r = 0, g = 0, b = 0, x = 0;
+			auto starts = [&](const char* p) { return l.rfind(p, 0) == 0; };
+			m = Material{};
+			if (starts("lambertian="))
+			{
+				if (std::sscanf(l.c_str(), "lambertian=(%lf,%lf,%lf)", &r, &g, &b) != 3) throw std::runtime_error("Invalid lambertian material: " + line);
+				m.kind = MAT_LAMBERTIAN;
+			}
+			else if (starts("metal="))
+			{
+				if (std::sscanf(l.c_str(), "metal=(%lf,%lf,%lf),%lf", &r, &g, &b, &x) != 4) throw std::runtime_error("Invalid metal material: " + line);
+				m.kind = MAT_METAL;
+				m.roughness = unitValue(x, "Metal roughness");
+			}
+			else if (starts("glossy="))
+			{
+				if (std::sscanf(l.c_str(), "glossy=(%lf,%lf,%lf),%lf", &r, &g, &b, &x) != 4) throw std::runtime_error("Invalid glossy material: " + line);
+				m.kind = MAT_GLOSSY;
+				m.roughness = unitValue(x, "Glossy roughness");
+			}
+			else if (starts("dielectric="))
+			{
+				if (std::sscanf(l.c_str(), "dielectric=(%lf,%lf,%lf)", &r, &g, &b) != 3) throw std::runtime_error("Invalid dielectric material: " + line);
+				m.kind = MAT_DIELECTRIC;
+				m.ior = 1.52;
+			}
+			else if (starts("emissive="))
+			{
+				throw std::runtime_error("Direct emissive material syntax is not physical. Use type=emissive with radiance or luminance.");
+			}
+			else if (starts("isotropic="))
+			{
+				if (std::sscanf(l.c_str(), "isotropic=(%lf,%lf,%lf)", &r, &g, &b) != 3) throw std::runtime_error("Invalid isotropic material: " + line);
+				m.kind = MAT_ISOTROPIC;
+			}
+			else if (starts("henyey_greenstein=") || starts("henyeygreenstein=") || starts("hg="))
+			{
+				if (std::sscanf(l.c_str(), "%*[^=]=(%lf,%lf,%lf),%lf", &r, &g, &b, &x) != 4) throw std::runtime_error("Invalid Henyey-Greenstein material: " + line);
+				if (!std::isfinite(x) || x < -0.99 || x > 0.99) throw std::invalid_argument("Volume anisotropy must be finite and between -0.99 and 0.99.");
+				m.kind = MAT_HG;
+				m.anisotropy = x;
+			}
+			else
+			{
+				return (false);
+			}
+			m.color = {r, g, b};
+			return (true);
 		}
 
 		struct	Reader
@@ -629,21 +1041,23 @@ namespace luzgpu
 				}
 			}
 
+			// One `material NAME { ... }` block: readBraceMaterialBlock / parseMaterialProperty /
+			// buildMaterial (SceneFile/Materials.cpp:368-640, 757-889). Keys are collected, then
+			// the material is built with the reference's checks; measured presets and glass
+			// dispersion are evaluated here, so the device sees constants only.
 			void	materialBlock(const std::string& name)
 			{
 				std::string line;
-				std::string type = "lambertian";
-				dvec3 color = {0.6, 0.6, 0.6};
-				double roughness = 0.0;
-				double fuzz = -1.0;
-				double ior = 1.52;	// RI_GLASS (RefractiveIndexes.hpp:3)
-				double metallic = 0.0, transmission = 0.0, clearcoat = 0.0, clearcoatRoughness = 0.03, sheen = 0.0;
-				double subsurface = 0.0, glossyWeight = 0.5, alphaKey = 1.0;
-				dvec3 glossyColor = {1.0, 1.0, 1.0};
-				bool hasGlossyColor = false;
-				double radiance = -1.0;
-				double lum = -1.0;
-				double anisotropy = 0.0;
+				MaterialKeys k;
+				std::optional<Material> direct;
+				auto emissionQuantity = [&](std::optional<double>& dst, const std::string& value) {
+					// assignSingleEmissionQuantity (Materials.cpp:198-208)
+					if (k.radiance || k.lum)
+					{
+						throw std::runtime_error("Material block defines multiple emission unit quantities.");
+					}
+					dst = std::stod(value);
+				};
 				while (next(line))
 				{
 					const std::string t = trim(line);
@@ -653,131 +1067,8 @@ namespace luzgpu
 					}
 					if (t == "}")
 					{
-						Material m;
+						Material m = direct ? *direct : buildMaterial(k);
 						m.name = name;
-						if (type == "lambertian")
-						{
-							m.kind = MAT_LAMBERTIAN;
-							m.color = color;
-						}
-						else if (type == "metal")
-						{
-							m.kind = MAT_METAL;
-							m.color = color;
-							m.roughness = fuzz >= 0.0 ? fuzz : roughness;
-							if (!(m.roughness >= 0.0 && m.roughness <= 1.0))
-							{
-								throw std::runtime_error("Metal roughness must be finite and in [0,1].");
-							}
-						}
-						else if (type == "dielectric")
-						{
-							m.kind = MAT_DIELECTRIC;
-							m.color = color;
-							m.ior = ior;
-							m.roughness = roughness;
-							if (!(ior > 0.0))
-							{
-								throw std::runtime_error("Dielectric refractive index must be finite and positive.");
-							}
-						}
-						else if (type == "emissive")
-						{
-							// Emissive::fromRadiance / fromLuminance (LightUnits.cpp:36-73)
-							double target;
-							if (radiance >= 0.0)
-							{
-								target = radiance;
-							}
-							else if (lum >= 0.0)
-							{
-								target = lum / 683.0;
-							}
-							else
-							{
-								throw std::runtime_error("Emissive material requires radiance or luminance.");
-							}
-							m.kind = MAT_EMISSIVE;
-							if (target == 0.0)
-							{
-								m.color = {0.0, 0.0, 0.0};
-							}
-							else
-							{
-								const double cl = luminance(color);
-								if (!(cl > 0.0))
-								{
-									throw std::runtime_error("Light color must have positive luminance for physical units.");
-								}
-								const double f = target / cl;
-								m.color = {color[0] * f, color[1] * f, color[2] * f};
-							}
-						}
-						else if (type == "glossy" || type == "diffuse_glossy" || type == "diffuseglossy" || type == "principled")
-						{
-							auto unit = [](double v, const std::string& label) {
-								if (!std::isfinite(v) || v < 0.0 || v > 1.0)
-								{
-									throw std::invalid_argument(label + " must be finite and in [0,1].");
-								}
-								return (v);
-							};
-							m.color = color;
-							if (type == "glossy")
-							{
-								m.kind = MAT_GLOSSY;
-								m.roughness = unit(roughness, "Glossy roughness");
-							}
-							else if (type != "principled")
-							{
-								// DiffuseGlossy(color, glossyColor or color, weight, roughness), Materials.cpp:831-840
-								m.kind = MAT_DIFFUSE_GLOSSY;
-								m.glossyColor = hasGlossyColor ? glossyColor : color;
-								m.glossyWeight = unit(glossyWeight, "Diffuse-glossy weight");
-								m.roughness = unit(roughness, "Diffuse-glossy roughness");
-							}
-							else
-							{
-								// buildPrincipledMaterial (Materials.cpp:693-760)
-								if (subsurface != 0.0)
-								{
-									throw std::runtime_error("Unsupported for the device path: principled subsurface");
-								}
-								m.kind = MAT_PRINCIPLED;
-								m.metallic = unit(metallic, "Principled metallic");
-								m.roughness = unit(roughness, "Principled roughness");
-								m.transmission = unit(std::max(transmission, 1.0 - alphaKey), "Principled transmission");
-								if (!std::isfinite(ior) || ior <= 0.0)
-								{
-									throw std::invalid_argument("Principled refractive index must be finite and positive.");
-								}
-								m.ior = ior;
-								m.clearcoat = unit(clearcoat, "Principled clearcoat");
-								m.clearcoatRoughness = unit(clearcoatRoughness, "Principled clearcoat roughness");
-								m.sheen = unit(sheen, "Principled sheen");
-							}
-						}
-						else if (type == "isotropic")
-						{
-							m.kind = MAT_ISOTROPIC;
-							m.color = color;
-						}
-						else if (type == "volume" || type == "phase" || type == "henyey_greenstein" || type == "henyeygreenstein"
-							|| type == "hg")
-						{
-							// HenyeyGreenstein.cpp:10-18
-							if (!std::isfinite(anisotropy) || anisotropy < -0.99 || anisotropy > 0.99)
-							{
-								throw std::invalid_argument("Volume anisotropy must be finite and between -0.99 and 0.99.");
-							}
-							m.kind = MAT_HG;
-							m.color = color;
-							m.anisotropy = anisotropy;
-						}
-						else
-						{
-							throw std::runtime_error("Unsupported material type for the device path: " + type);
-						}
 						if (materialIndex.count(name))
 						{
 							throw std::runtime_error("Duplicate material name: " + name);
@@ -786,36 +1077,100 @@ namespace luzgpu
 						scene.materials.push_back(m);
 						return;
 					}
-					std::string key, value;
-					if (!splitAssignment(t, key, value))
+					Material dm;
+					if (directMaterialLine(t, dm))
 					{
-						throw std::runtime_error("Unknown material property: " + line);
+						if (direct || k.hasProperties)
+						{
+							throw std::runtime_error("Material block defines more than one material.");
+						}
+						direct = dm;
+						continue;
 					}
-					if (key == "type") type = lower(value);
-					else if (key == "color" || key == "basecolor" || key == "base_color") color = parseColor(value, "material color");
-					else if (key == "roughness") roughness = parseDouble(value, "roughness");
-					else if (key == "fuzz") fuzz = parseDouble(value, "fuzz");
-					else if (key == "ior" || key == "refractiveindex" || key == "refractive_index") ior = parseDouble(value, "ior");
-					else if (key == "radiance" || key == "surfaceradiance" || key == "surface_radiance") radiance = parseDouble(value, "radiance");
-					else if (key == "luminance" || key == "nits" || key == "cd_m2") lum = parseDouble(value, "luminance");
-					else if (key == "anisotropy" || key == "g") anisotropy = parseDouble(value, "anisotropy");
-					else if (key == "metallic") metallic = parseDouble(value, key);
-					else if (key == "transmission") transmission = parseDouble(value, key);
-					else if (key == "clearcoat" || key == "clear_coat" || key == "coat") clearcoat = parseDouble(value, key);
+					std::string key, value;
+					if (!splitAssignment(t, key, value) || key.empty() || value.empty())
+					{
+						throw std::runtime_error("Invalid material property: " + line);
+					}
+					if (direct)
+					{
+						throw std::runtime_error("Material block mixes direct material syntax with property syntax.");
+					}
+					k.hasProperties = true;
+					if (key == "type") k.type = lower(value);
+					else if (key == "preset" || key == "material_preset" || key == "materialpreset") k.materialPreset = value;
+					else if (key == "color" || key == "basecolor" || key == "base_color") k.color = parseColor(value, "material color");
+					else if (key == "emission" || key == "emissioncolor" || key == "emission_color")
+					{
+						k.emissionColor = parseColor(value, "emission color");
+						k.hasEmissionColor = true;
+					}
+					else if (key == "fuzz") k.fuzz = std::stod(value);
+					else if (key == "roughness") k.roughness = std::stod(value);
+					else if (key == "metallic") k.metallic = std::stod(value);
+					else if (key == "eta" || key == "conductor_eta") k.eta = parseColor(value, key);
+					else if (key == "metal_preset" || key == "metalpreset" || key == "conductor" || key == "conductor_preset"
+						|| key == "conductorpreset") k.conductorPreset = value;
+					else if (key == "k" || key == "extinction" || key == "extinctioncoefficient" || key == "extinction_coefficient"
+						|| key == "conductor_k") k.extinction = parseColor(value, key);
+					else if (key == "transmission") k.transmission = std::stod(value);
+					else if (key == "clearcoat" || key == "clear_coat" || key == "coat") k.clearcoat = std::stod(value);
 					else if (key == "clearcoatroughness" || key == "clearcoat_roughness" || key == "clear_coat_roughness"
-						|| key == "coatroughness" || key == "coat_roughness") clearcoatRoughness = parseDouble(value, key);
-					else if (key == "sheen") sheen = parseDouble(value, key);
+						|| key == "coatroughness" || key == "coat_roughness") k.clearcoatRoughness = std::stod(value);
+					else if (key == "sheen") k.sheen = std::stod(value);
 					else if (key == "subsurface" || key == "subsurface_weight" || key == "subsurfaceweight" || key == "sss")
-						subsurface = parseDouble(value, key);
-					else if (key == "alpha") alphaKey = parseDouble(value, key);
+					{
+						k.subsurface = std::stod(value);
+						k.hasSubsurface = true;
+					}
+					else if (key.rfind("subsurface_", 0) == 0 || key.rfind("subsurface", 0) == 0 || key.rfind("sss", 0) == 0)
+					{
+						throw std::runtime_error("Unsupported for the device path: principled subsurface (" + key + ")");
+					}
 					else if (key == "glossy_color" || key == "glossycolor" || key == "specular_color" || key == "specularcolor")
 					{
-						glossyColor = parseColor(value, "glossy color");
-						hasGlossyColor = true;
+						k.glossyColor = parseColor(value, "glossy color");
+						k.hasGlossyColor = true;
 					}
 					else if (key == "glossy_weight" || key == "glossyweight" || key == "glossy_mix" || key == "glossymix")
-						glossyWeight = parseDouble(value, key);
-					else throw std::runtime_error("Unsupported material property for the device path: " + line);
+						k.glossyWeight = std::stod(value);
+					else if (key == "ior" || key == "refractiveindex" || key == "refractive_index")
+					{
+						k.ior = std::stod(value);
+						k.hasIor = true;
+					}
+					else if (key == "glass" || key == "glass_preset" || key == "glasspreset" || key == "dielectric_preset"
+						|| key == "dielectricpreset") k.glassPreset = value;
+					else if (key == "abbe" || key == "abbe_number" || key == "abbenumber" || key == "vd") k.abbe = std::stod(value);
+					else if (key == "sellmeier_b" || key == "sellmeierb") k.sellmeierB = parseTriple(value, key);
+					else if (key == "sellmeier_c" || key == "sellmeierc") k.sellmeierC = parseTriple(value, key);
+					else if (key == "ior_wavelength" || key == "iorwavelength" || key == "refractive_index_wavelength"
+						|| key == "refractiveindexwavelength")
+					{
+						k.iorWavelength = parseNanometers(value, "IOR wavelength");
+						k.hasIorWavelength = true;
+					}
+					else if (key == "absorption" || key == "absorptioncoefficient" || key == "absorption_coefficient" || key == "sigma_a"
+						|| key == "sigmaa") k.absorption = parseColor(value, key);
+					else if (key == "transmittance" || key == "transmittancecolor" || key == "transmittance_color" || key == "attenuation"
+						|| key == "attenuation_color" || key == "attenuationcolor") k.transmittance = parseColor(value, key);
+					else if (key == "attenuationdistance" || key == "attenuation_distance" || key == "absorptiondistance"
+						|| key == "absorption_distance")
+					{
+						k.transmittanceDistance = std::stod(value);
+						k.hasTransmittanceDistance = true;
+					}
+					else if (key == "alpha") k.alphaKey = std::stod(value);
+					else if (key == "radiance" || key == "surfaceradiance" || key == "surface_radiance" || key == "emissionradiance"
+						|| key == "emission_radiance") emissionQuantity(k.radiance, value);
+					else if (key == "luminance" || key == "nits" || key == "cd_m2" || key == "cdm2" || key == "emissionluminance"
+						|| key == "emission_luminance") emissionQuantity(k.lum, value);
+					else if (key == "anisotropy" || key == "g") k.anisotropy = std::stod(value);
+					else if (key == "texture" || key == "basecolortexture" || key == "base_color_texture" || key == "albedo")
+					{
+						throw std::runtime_error("Unsupported for the device path: material textures (" + key + ")");
+					}
+					else throw std::runtime_error("Unknown material property: " + line);
 				}
 				throw std::runtime_error("Material block '" + name + "' is missing a closing }.");
 			}
@@ -853,10 +1208,25 @@ namespace luzgpu
 				return (it->second);
 			}
 
+			// addCameraBlock (SceneFile/Scene.cpp:49-214): keys are collected, then applied at
+			// the block's end in the reference's order (focal length, sensor, f-number or
+			// aperture, pinhole, focus, photographic exposure). The scene renders from its first
+			// camera (Scene::getActiveCamera, Scene.cpp:586-594).
 			void	cameraBlock()
 			{
 				std::string line;
-				Camera& c = scene.camera;
+				Camera c;
+				c.position = {0.0, 0.0, 15.0};	// Camera.cpp:21-33 defaults
+				std::optional<double> focal, sensorW, sensorH, fNumber, aperture, focus, shutter, iso;
+				std::optional<bool> pinhole;
+				auto positive = [](const std::string& v, const std::string& label, double scale) {
+					const double x = parseDouble(v, label);
+					if (!std::isfinite(x) || x <= 0.0)
+					{
+						throw std::runtime_error(label + " must be finite and positive.");
+					}
+					return x * scale;
+				};
 				while (next(line))
 				{
 					const std::string t = trim(line);
@@ -866,7 +1236,51 @@ namespace luzgpu
 					}
 					if (t == "}")
 					{
-						scene.hasCamera = true;
+						if (focal) c.focalLength = *focal;
+						if (sensorW) c.sensorWidth = *sensorW;
+						if (sensorH) c.sensorHeight = *sensorH;
+						if (fNumber && aperture)
+						{
+							throw std::runtime_error("Camera block defines both f_stop and aperture_diameter.");
+						}
+						if (fNumber)
+						{
+							c.fNumber = *fNumber;
+							c.pinhole = false;
+						}
+						if (aperture)
+						{
+							c.fNumber = c.focalLength / *aperture;
+							c.pinhole = false;
+						}
+						if (pinhole)
+						{
+							if (*pinhole && aperture)
+							{
+								throw std::runtime_error("Camera block defines pinhole=1 and aperture_diameter.");
+							}
+							c.pinhole = *pinhole;
+						}
+						if (focus) c.focusDistance = *focus;
+						if (shutter || iso)
+						{
+							if (!shutter || !iso)
+							{
+								throw std::runtime_error("Camera block photographic exposure requires shutter and iso.");
+							}
+							// Scene::setPhotographicExposure (Scene.cpp:270-286) with the camera's f-number
+							const double v = std::log2((*shutter * (*iso / 100.0)) / (c.fNumber * c.fNumber));
+							if (!std::isfinite(v) || !std::isfinite(std::pow(2.0, v)))
+							{
+								throw std::runtime_error("Exposure must be finite.");
+							}
+							scene.settings.exposure = v;
+						}
+						if (!scene.hasCamera)
+						{
+							scene.camera = c;
+							scene.hasCamera = true;
+						}
 						return;
 					}
 					std::string key, value;
@@ -876,21 +1290,31 @@ namespace luzgpu
 					}
 					if (key == "position") c.position = parseTriple(value, "camera position");
 					else if (key == "direction") c.direction = parseTriple(value, "camera direction");
-					else if (key == "up") c.up = parseTriple(value, "camera up");
-					else if (key == "focal_length") c.focalLength = parseDouble(value, key);
-					else if (key == "focal_length_mm") c.focalLength = parseDouble(value, key) / 1000.0;
-					else if (key == "sensor_width") c.sensorWidth = parseDouble(value, key);
-					else if (key == "sensor_width_mm") c.sensorWidth = parseDouble(value, key) / 1000.0;
-					else if (key == "sensor_height") c.sensorHeight = parseDouble(value, key);
-					else if (key == "sensor_height_mm") c.sensorHeight = parseDouble(value, key) / 1000.0;
+					else if (key == "up" || key == "updirection" || key == "up_direction") c.up = parseTriple(value, "camera up");
+					else if (key == "focal_length") focal = positive(value, "Camera focal_length", 1.0);
+					else if (key == "focal_length_mm") focal = positive(value, "Camera focal_length_mm", 0.001);
+					else if (key == "sensor_width") sensorW = positive(value, "Camera sensor_width", 1.0);
+					else if (key == "sensor_width_mm") sensorW = positive(value, "Camera sensor_width_mm", 0.001);
+					else if (key == "sensor_height") sensorH = positive(value, "Camera sensor_height", 1.0);
+					else if (key == "sensor_height_mm") sensorH = positive(value, "Camera sensor_height_mm", 0.001);
 					else if (key == "f_stop" || key == "fstop" || key == "f_number" || key == "fnumber")
+						fNumber = positive(value, "Camera f_stop", 1.0);
+					else if (key == "aperture_diameter") aperture = positive(value, "Camera aperture_diameter", 1.0);
+					else if (key == "aperture_diameter_mm") aperture = positive(value, "Camera aperture_diameter_mm", 0.001);
+					else if (key == "pinhole")
 					{
-						c.fNumber = parseDouble(value, key);
-						c.pinhole = false;
+						const int v = std::stoi(value);
+						if (v != 0 && v != 1)
+						{
+							throw std::runtime_error("Camera pinhole must be 0 or 1.");
+						}
+						pinhole = v == 1;
 					}
-					else if (key == "pinhole") c.pinhole = parseDouble(value, key) != 0.0;
-					else if (key == "focus_distance") c.focusDistance = parseDouble(value, key);
-					else throw std::runtime_error("Unsupported camera property for the device path: " + line);
+					else if (key == "focus_distance") focus = positive(value, "Camera focus_distance", 1.0);
+					else if (key == "shutter" || key == "shutter_seconds" || key == "shutterseconds" || key == "shutter_speed"
+						|| key == "shutterspeed") shutter = positive(value, "Camera shutter", 1.0);
+					else if (key == "iso") iso = positive(value, "Camera iso", 1.0);
+					else throw std::runtime_error("Unknown camera property: " + line);
 				}
 				throw std::runtime_error("Camera block is missing a closing }.");
 			}

@@ -518,6 +518,61 @@ def write_materials(directory: str, width=160, height=120, spp=32, depth=8, seed
     return Workload("materials", directory, path, width, height, spp, depth, seed, 0)
 
 
+def optics_scene_text(width=160, height=120, spp=128, depth=10) -> str:
+    """The reference's default thin-lens camera (f/2.8, focused at 12 units) over the
+    materials no other fixture reaches: rough dielectrics, conductor-Fresnel metals (measured
+    presets and explicit eta / k, smooth and rough), glass presets with Sellmeier / Abbe
+    dispersion, interior absorption (coefficient and transmittance forms) in a dielectric and
+    a transmissive principled material, and direct material lines."""
+    mats = [
+        ("frost", ["type=dielectric", "color=(1,1,1)", "ior=1.5", "roughness=0.3"]),
+        ("gold", ["type=metal", "preset=gold", "roughness=0.2"]),
+        ("copper", ["type=metal", "eta=(0.27,0.68,1.34)", "k=(3.57,2.62,2.29)", "roughness=0"]),
+        ("chrome", ["type=metal", "conductor=Chrome", "fuzz=0.45"]),
+        ("bk7", ["type=dielectric", "color=(1,1,1)", "glass=BK7", "ior_wavelength=550nm", "roughness=0.08",
+                 "transmittance=(0.6,0.85,0.95)", "attenuation_distance=0.5"]),
+        ("amber", ["type=dielectric", "color=(1,1,1)", "ior=1.55", "abbe=40", "ior_wavelength=500",
+                   "absorption=(0.2,0.9,2.5)"]),
+        ("tinted", ["type=principled", "color=(0.95,0.95,0.95)", "transmission=1", "roughness=0.15",
+                    "preset=fused_silica", "absorption=(1.5,0.4,0.2)"]),
+        ("brushed", ["metal=(0.8,0.75,0.7),0.3"]),
+    ]
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "sky=none", "background=(0.06,0.07,0.09)", "",
+        "[materials]",
+        "material floor {", "type=lambertian", "color=(0.6,0.58,0.55)", "}",
+        "material panel {", "type=emissive", "color=(1,0.95,0.85)", "radiance=9", "}",
+    ]
+    for name, props in mats:
+        lines += [f"material {name} {{", *props, "}"]
+    lines += [
+        "",
+        "[scene]",
+        "camera main {", "position=(0,3.2,-12)", "direction=(0,-0.22,1)", "up=(0,1,0)", "focal_length_mm=35",
+        "sensor_width_mm=36", "sensor_height_mm=27", "f_stop=2.8", "focus_distance=12", "}",
+        "objects{",
+        "plane=0,(0,1,0),material=floor",
+        "rectangle=(0,7,3),(0,-1,0),7,3,material=panel",
+    ]
+    for i, (name, _) in enumerate(mats):
+        x = -5.25 + 1.5 * i
+        z = 0.5 + (i % 3) * 2.0
+        if name == "bk7":
+            lines.append(f"cube=({_f(x)},0.75,{_f(z)}),(0,0,-1),1.4,1.5,1.4,material={name}")
+        else:
+            lines.append(f"sphere=({_f(x)},0.7,{_f(z)}),0.7,material={name}")
+    lines.append("}")
+    return "\n".join(lines) + "\n"
+
+
+def write_optics(directory: str, width=160, height=120, spp=128, depth=10, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "optics.luz")
+    _write_if_changed(path, optics_scene_text(width, height, spp, depth).encode())
+    return Workload("optics", directory, path, width, height, spp, depth, seed, 0)
+
+
 def volume_scene_text(width=160, height=120, spp=32, depth=16) -> str:
     lines = [
         "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
@@ -717,6 +772,21 @@ def diff_parameters(seed=1004, metals=True):
     albedo.append([0.0, 0.0, 0.0])
     roughness.append(0.0)
     emission = [[0.0, 0.0, 0.0]] * 16 + [[17.0, 12.0, 4.0]]
+    return kinds, albedo, roughness, emission
+
+
+DIFF_ROUGH_METALS = ((3, 0.25), (2, 0.3), (5, 0.2), (12, 0.4))
+
+
+def diff_rough_parameters():
+    """The differentiable box with four rough GGX metals on large, visible surfaces (the
+    left wall, back wall, short box top and a tall box side), the rest Lambertian:
+    (kinds, albedo, roughness, emission) as diff_parameters returns them."""
+    kinds, albedo, roughness, emission = diff_parameters(metals=False)
+    kinds, roughness = list(kinds), list(roughness)
+    for m, r in DIFF_ROUGH_METALS:
+        kinds[m] = "metal"
+        roughness[m] = r
     return kinds, albedo, roughness, emission
 
 

@@ -604,6 +604,9 @@ namespace oracle
 		double	roughness = 0, alpha = 1e-4, ior = 1.5, g = 0.0;
 		C	glossyColor = {1, 1, 1};
 		double	glossyWeight = 0.5, metallic = 0, transmission = 0, clearcoat = 0, clearcoatRoughness = 0.03, sheen = 0;
+		bool	conductor = false;	// Metal::_useConductorFresnel (Metal.cpp:118-125)
+		C	eta = {1, 1, 1}, k = {0, 0, 0};
+		C	absorption = {0, 0, 0};	// dielectric / principled interior absorption, 1/m
 	};
 
 	struct	Light
@@ -704,6 +707,10 @@ namespace oracle
 			o.clearcoat = m.clearcoat;
 			o.clearcoatRoughness = m.clearcoatRoughness;
 			o.sheen = m.sheen;
+			o.conductor = m.conductor;
+			o.eta = fromC(m.eta);
+			o.k = fromC(m.k);
+			o.absorption = fromC(m.absorption);
 			s.mats.push_back(o);
 		}
 		for (const auto& mesh : in.meshes)
@@ -1172,10 +1179,28 @@ namespace oracle
 		if (!std::isfinite(sc) || sc <= 0.0) return {};
 		return clamp01(F) * sc;
 	}
-	C	metalBsdf(const Mat& m, V n, V view, V l)	// :306-323
+	C	conductorFresnel(C eta, C k, double c)	// BSDF.cpp:127-174
+	{
+		auto channel = [](double e, double kk, double cv) {
+			if (!std::isfinite(e) || !std::isfinite(kk) || e <= 0.0 || kk < 0.0) return 0.0;
+			const double c2 = cv * cv;
+			const double ek = e * e + kk * kk;
+			const double tec = 2.0 * e * cv;
+			const double dPar = ek * c2 + tec + 1.0;
+			const double dPerp = ek + tec + c2;
+			if (std::fabs(dPar) <= 1e-10 || std::fabs(dPerp) <= 1e-10) return 0.0;	// MIN_DENOMINATOR (BSDF.cpp:11)
+			const double rPar = (ek * c2 - tec + 1.0) / dPar;
+			const double rPerp = (ek - tec + c2) / dPerp;
+			return clamp01(0.5 * (rPar + rPerp));
+		};
+		c = clamp01(c);
+		return {channel(eta.r, k.r, c), channel(eta.g, k.g, c), channel(eta.b, k.b, c)};
+	}
+	C	metalBsdf(const Mat& m, V n, V view, V l)	// Metal.cpp:172-200: BSDF.cpp:291-323
 	{
 		const V h = safeNormalize(view + l, n);
-		return ggxReflBsdf(schlick(clamp01(m.color), std::max(0.0, dot(view, h))), n, view, l, m.alpha);
+		const double voH = std::max(0.0, dot(view, h));
+		return ggxReflBsdf(m.conductor ? conductorFresnel(m.eta, m.k, voH) : schlick(clamp01(m.color), voH), n, view, l, m.alpha);
 	}
 	bool	refractDir(V i, V mn, double eta, V& out)	// :325-350
 	{
@@ -1245,6 +1270,8 @@ namespace oracle
 		V	phaseDir;	// Henyey-Greenstein
 		double	g = 0.0;
 		bool	mirrorFallback = false;	// rough metal: the GGX sample reflected below the horizon
+		bool	absorbing = false;	// ScatterRecord::hasMediumAbsorption
+		C	absorption;			// ScatterRecord::mediumAbsorptionCoefficient
 	};
 
 	// ---- glossy / diffuse-glossy / principled (Glossy.cpp, DiffuseGlossy.cpp, Principled.cpp) -----
@@ -1444,6 +1471,9 @@ namespace oracle
 		}
 		if (m.kind == luzgpu::MAT_PRINCIPLED)	// Principled.cpp:668-761
 		{
+			// setMediumAbsorption (:129-140)
+			sc.absorbing = m.transmission > 0.0 && (m.absorption.r > 0.0 || m.absorption.g > 0.0 || m.absorption.b > 0.0);
+			sc.absorption = m.absorption;
 			const C base = clamp01(m.color);
 			if (m.metallic <= 1e-6 && m.roughness <= 1e-4 && m.transmission >= 1.0 - 1e-6 && m.clearcoat <= 1e-6 && m.sheen <= 1e-6)
 			{
@@ -1515,7 +1545,7 @@ namespace oracle
 			if (m.roughness <= 1e-4)
 			{
 				sc.dir = Ray::make(h.p, reflect(r.d, h.n)).d;
-				sc.att = schlick(clamp01(m.color), ct);
+				sc.att = m.conductor ? conductorFresnel(m.eta, m.k, ct) : schlick(clamp01(m.color), ct);
 				sc.specular = true;
 				return true;
 			}
@@ -1543,6 +1573,8 @@ namespace oracle
 		}
 		if (m.kind == luzgpu::MAT_DIELECTRIC)	// Dielectric.cpp:127-208
 		{
+			sc.absorbing = m.absorption.r > 0.0 || m.absorption.g > 0.0 || m.absorption.b > 0.0;	// :129-134
+			sc.absorption = m.absorption;
 			const V view = safeNormalize(r.d * -1.0, h.n);
 			const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0, ratio = ei / et;
 			if (m.roughness <= 1e-4)
@@ -2240,7 +2272,17 @@ namespace oracle
 				}
 				return acc + inc;
 			}
-			const C att = sc.att;
+			// effectiveScatterAttenuation / mediumTransmittance (ColorHelper.cpp:695-723): a back-face
+			// hit attenuates by Beer-Lambert over the segment's length in metres
+			C att = sc.att;
+			if (!h.front && sc.absorbing)
+			{
+				const double dm = h.t * s.mpu;
+				if (std::isfinite(dm) && dm > 0.0)
+				{
+					att = att * C{std::exp(-sc.absorption.r * dm), std::exp(-sc.absorption.g * dm), std::exp(-sc.absorption.b * dm)};
+				}
+			}
 			if (sc.specular)
 			{
 				if (!leavesOpaque(m, h, sc.dir)) return acc;
@@ -2736,6 +2778,12 @@ double	oracle_ggx_fallback(double mu, double alpha, std::uint32_t nodes, std::ui
 double	oracle_dielectric_fresnel(double cos_theta, double eta_i, double eta_t)
 {
 	return oracle::dielectricFresnel(cos_theta, eta_i, eta_t);
+}
+
+// conductor Fresnel of one channel (BSDF.cpp:127-174)
+double	oracle_conductor_fresnel(double cos_theta, double eta, double k)
+{
+	return oracle::conductorFresnel(oracle::C{eta, eta, eta}, oracle::C{k, k, k}, cos_theta).r;
 }
 
 }

@@ -91,3 +91,24 @@ def spectral(workload_dir):
 def atmosphere_env(workload_dir):
     from luz_amd import workloads
     return workloads.write_atmosphere_env(os.path.join(workload_dir, "atmosphere_env"))
+
+
+@pytest.fixture(scope="session")
+def optics(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_optics(os.path.join(workload_dir, "optics"))
+
+
+@pytest.fixture(scope="session")
+def terrain(workload_dir):
+    """The volume workload's scene: 1,002,528-triangle terrain, haze, sun (rendered at
+    reduced sizes by the tests)."""
+    from luz_amd import workloads
+    return workloads.terrain_scene(os.path.join(workload_dir, "terrain"))
+
+
+@pytest.fixture(scope="session")
+def batch(workload_dir):
+    """The uneven 32-scene batch (rendered at reduced sizes by the tests)."""
+    from luz_amd import workloads
+    return workloads.batch_scenes(os.path.join(workload_dir, "batch"))

@@ -281,6 +281,111 @@ def fd_fixture(tmp, name="diff_fd_128_8spp_depth2", width=128, height=128, spp=8
     print(name, "loss", base, "|d_albedo|", np.abs(d_albedo).sum(), "d_emission", d_emission[-1])
 
 
+FD_ROUGH = dict(width=64, height=64, spp=8, depth=6, h=0.02, seeds=64)
+
+
+def fd_rough_fixture(tmp):
+    """Roughness gradients of the reference renderer at depth 6 (Russian roulette, mirror
+    fallbacks, MIS): FP64 central differences (h = 0.02) of L_j = sum(w_j * image) for each
+    metal j and each of 64 seeds, both sides of each difference at the same seed. A
+    fixed-seed estimator jumps where a sampled direction crosses an edge or a discrete
+    decision flips, so a single seed's difference is noisy; the per-seed values are kept so a
+    test can form the mean over seeds and its standard error. Metal j's weight image w_j is
+    the normalised change of a 256-spp render when that metal's roughness is scaled by 0.7,
+    so each loss looks at that metal's highlight."""
+    c = FD_ROUGH
+    kinds, albedo, roughness, emission = workloads.diff_rough_parameters()
+    metals = [m for m, _ in workloads.DIFF_ROUGH_METALS]
+    d = os.path.join(tmp, "fdr")
+
+    def render(r, seed, spp, tag):
+        wl = workloads.write_diff_cornell(d, (kinds, albedo, r, emission), c["width"], c["height"], spp, c["depth"],
+                                          name=f"{tag}.luz")
+        out = os.path.join(d, f"{tag}.bin")
+        run("render", wl.scene_path, seed, os.cpu_count() or 8, out)
+        return read_image(out)[0]
+
+    base = render(roughness, 999, 256, "base")
+    w = []
+    for m in metals:
+        r = list(roughness)
+        r[m] *= 0.7
+        delta = render(r, 999, 256, "sharp") - base
+        w.append(delta / np.abs(delta).max())
+    w = np.stack(w)
+    fd = np.zeros((c["seeds"], len(metals)))
+    for si in range(c["seeds"]):
+        for j, m in enumerate(metals):
+            rp, rm = list(roughness), list(roughness)
+            rp[m] += c["h"]
+            rm[m] -= c["h"]
+            fd[si, j] = ((render(rp, si + 1, c["spp"], "p") - render(rm, si + 1, c["spp"], "m")) * w[j]).sum() / (2 * c["h"])
+    np.savez_compressed(os.path.join(HERE, "diff_fd_rough_64_8spp_depth6.npz"), weights=w.astype(np.float32), fd=fd,
+                        seeds=np.arange(1, c["seeds"] + 1), metals=np.array(metals), h=c["h"], spp=c["spp"],
+                        depth=c["depth"], width=c["width"], height=c["height"])
+    sem = fd.std(0, ddof=1) / np.sqrt(c["seeds"])
+    print("fd_rough mean", fd.mean(0), "sem", sem)
+
+
+TERRAIN_GOLDEN = dict(width=160, height=90, spp=8)
+TERRAIN_NFOR = dict(width=96, height=64, spp=8, radii=(1, 9, 1))
+BATCH_GOLDEN = dict(width=96, height=64, spp=8)
+
+
+def batch_golden_indices(wls):
+    """The batch scenes the reduced fixture renders: the one with the most triangles and
+    three more spread over the triangle range (the smallest, and two between)."""
+    order = sorted(range(len(wls)), key=lambda i: wls[i].triangles)
+    return sorted({order[-1], order[0], order[len(order) // 3], order[2 * len(order) // 3]})
+
+
+def terrain_fixtures(tmp):
+    """The volume workload's own scene (1,002,528-triangle terrain in Henyey-Greenstein haze
+    under a directional sun) at a reduced size the reference renders in seconds: its render
+    and primary hits, and its denoise pipeline with a radius-9 regression window (NFOR
+    inputs and the reference's applyNFOR output)."""
+    wl = workloads.terrain_scene(os.path.join(tmp, "terrain"))
+    c = TERRAIN_GOLDEN
+    out = os.path.join(tmp, "terrain.bin")
+    run("render", wl.scene_path, 1, os.cpu_count() or 8, out, c["spp"], c["width"], c["height"])
+    img, ms = read_image(out)
+    np.savez_compressed(os.path.join(HERE, f"terrain_{c['width']}x{c['height']}_{c['spp']}spp_seed1.npz"),
+                        image=img.astype(np.float32), render_ms=ms)
+    primary_fixture("terrain", wl.scene_path, 1, tmp, c["width"], c["height"])
+    n = TERRAIN_NFOR
+    name = f"nfor_terrain_{n['width']}x{n['height']}_{n['spp']}spp_r9"
+    out = os.path.join(tmp, f"{name}.bin")
+    run("denoise", wl.scene_path, 1, os.cpu_count() or 8, out, n["spp"], n["width"], n["height"], *n["radii"])
+    w, h = n["width"], n["height"]
+    px = w * h
+    data = np.fromfile(out, dtype="<f8", count=px * 48, offset=8)
+    fields, off = {}, 0
+    for k, ch in (("image", 3), ("color_a", 3), ("color_b", 3), ("color_variance", 1), ("features_a", 11),
+                  ("features_b", 11), ("feature_variance", 11), ("denoised", 3)):
+        fields[k] = data[off:off + px * ch].reshape(h, w, ch)
+        off += px * ch
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), seed=1, spp=n["spp"], scene="terrain",
+                        radii=np.array(n["radii"]), **fields)
+    print("terrain fixtures", float(img.mean()), {k: float(np.abs(v).mean()) for k, v in fields.items()})
+
+
+def batch_fixtures(tmp):
+    """Four of the uneven batch's scenes (its largest, 500k triangles, and three spread below
+    it), each rendered by the reference at a reduced size with the seed the batch gives it."""
+    wls = workloads.batch_scenes(os.path.join(tmp, "batch"))
+    c = BATCH_GOLDEN
+    images = []
+    picks = batch_golden_indices(wls)
+    for i in picks:
+        out = os.path.join(tmp, f"batch_{i}.bin")
+        run("render", wls[i].scene_path, wls[i].seed, os.cpu_count() or 8, out, c["spp"], c["width"], c["height"])
+        images.append(read_image(out)[0].astype(np.float32))
+    np.savez_compressed(os.path.join(HERE, f"batch_{c['width']}x{c['height']}_{c['spp']}spp.npz"), scenes=np.array(picks),
+                        seeds=np.array([wls[i].seed for i in picks]), triangles=np.array([wls[i].triangles for i in picks]),
+                        images=np.stack(images))
+    print("batch fixtures", picks, [wls[i].triangles for i in picks])
+
+
 def main():
     if not os.path.exists(LUZREF):
         sys.exit("build the reference harness first: make -C oracle ref")
@@ -302,10 +407,18 @@ def main():
             small_scene_fixtures("spectral", workloads.write_spectral(os.path.join(tmp, "spectral")), tmp)
         if "atmosphere_env" in only:
             small_scene_fixtures("atmosphere_env", workloads.write_atmosphere_env(os.path.join(tmp, "atmosphere_env")), tmp)
+        if "optics" in only:
+            small_scene_fixtures("optics", workloads.write_optics(os.path.join(tmp, "optics")), tmp)
         if "meshlight" in only:
             small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
         if "fd" in only:
             fd_fixture(tmp)
+        if "fd_rough" in only:
+            fd_rough_fixture(tmp)
+        if "terrain" in only:
+            terrain_fixtures(tmp)
+        if "batch" in only:
+            batch_fixtures(tmp)
         if "zstats" in only:
             zstat_fixtures(tmp)
         if "nfor" in only:
@@ -324,7 +437,11 @@ def main():
     small_scene_fixtures("atmosphere", workloads.write_atmosphere(os.path.join(tmp, "atmosphere")), tmp)
     small_scene_fixtures("spectral", workloads.write_spectral(os.path.join(tmp, "spectral")), tmp)
     small_scene_fixtures("atmosphere_env", workloads.write_atmosphere_env(os.path.join(tmp, "atmosphere_env")), tmp)
+    small_scene_fixtures("optics", workloads.write_optics(os.path.join(tmp, "optics")), tmp)
     fd_fixture(tmp)
+    fd_rough_fixture(tmp)
+    terrain_fixtures(tmp)
+    batch_fixtures(tmp)
     zstat_fixtures(tmp)
     nfor_fixtures(tmp)
     adaptive_fixtures(tmp)

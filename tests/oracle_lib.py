@@ -31,6 +31,8 @@ def load():
         lib.oracle_ggx_ndf_integral.restype = ctypes.c_double
         lib.oracle_dielectric_fresnel.argtypes = [ctypes.c_double] * 3
         lib.oracle_dielectric_fresnel.restype = ctypes.c_double
+        lib.oracle_conductor_fresnel.argtypes = [ctypes.c_double] * 3
+        lib.oracle_conductor_fresnel.restype = ctypes.c_double
         lib.oracle_hg_pdf.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.oracle_hg_pdf.restype = ctypes.c_double
         lib.oracle_hg_sample_cos.argtypes = [ctypes.c_double, ctypes.c_float, ctypes.c_float]

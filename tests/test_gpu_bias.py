@@ -4,7 +4,13 @@ For every small scene family, 64 device renders at seeds 1..64 against 64 refere
 at the independent seeds 1001..1064 (tests/golden/make_golden.py, zstat_fixtures): the
 Welch z of the per-channel image mean must satisfy |z| < 3, and of each of the 16 tile
 means per channel |z| < 4 (48 comparisons per scene, so a per-test 3-sigma bar would fire
-by chance)."""
+by chance).
+
+The optics scene is not among them: its rough dielectrics make the reference's own estimator
+heavy-tailed (single samples reach 1e8 where the transmission half vector the reference
+forms, normalize(view + (eta_i / eta_t) * l), is nearly perpendicular to the normal), so a
+64-seed mean is dominated by whichever seeds draw such a sample. It is compared at equal
+seeds instead (test_gpu_primitives), where both sides draw the same samples."""
 
 import os
 

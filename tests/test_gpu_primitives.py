@@ -4,7 +4,9 @@ Scenes: ``primitives`` (plane, spheres, cubes, sphere / rectangle / cube emitter
 background), ``volume`` (Henyey-Greenstein box fog, isotropic sphere smoke, directional sun,
 sphere light), ``materials`` (glossy, diffuse-glossy and principled variants) and
 ``meshlight`` (emissive mesh instances under uniform and non-uniform scales, sampled by
-triangle area and re-intersected for their MIS pdf). Same bars as test_gpu_parity: primary-hit IDs against the reference's
+triangle area and re-intersected for their MIS pdf) and ``optics`` (the reference's default
+thin-lens camera, rough dielectrics, conductor-Fresnel metals, dispersive glass presets and
+interior absorption). Same bars as test_gpu_parity: primary-hit IDs against the reference's
 FP64 brute force, images within relative MSE 1e-4 of the reference render at equal spp and
 seed (tests/golden/make_golden.py)."""
 
@@ -17,7 +19,7 @@ from test_gpu_parity import REL_MSE_BAR, _golden, _primary_check
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics"])
 def test_primary_hits(ctx, request, name):
     from luz_amd import SceneFile
     wl = request.getfixturevalue(name)
@@ -25,12 +27,13 @@ def test_primary_hits(ctx, request, name):
     # the atmosphere scene sits 6.36 units from the origin, where an FP32 origin carries
     # 4.8e-7 of rounding; over the ground plane's grazing rays (|d.y| down to ~0.005) that is
     # up to 1e-4 of t, so its distances are held to 2e-3 (its hit ids stay exact)
-    t_rtol = 2e-3 if name == "atmosphere" else 2e-5
+    # (the optics camera sees the floor plane out to the horizon: the same grazing rays)
+    t_rtol = 2e-3 if name in ("atmosphere", "optics") else 2e-5
     assert _primary_check(scene, f"{name}_primary.npz", t_rtol=t_rtol) <= 2
 
 
 @pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics"])
 def test_image_parity(ctx, request, name, integrator):
     from luz_amd import Renderer, SceneFile
     wl = request.getfixturevalue(name)

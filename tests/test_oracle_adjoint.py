@@ -110,3 +110,42 @@ def test_oracle_roughness_gradient_is_unbiased(workload_dir):
     # the noise of the difference (without the mirror-fallback score term the adjoint is off
     # by about ten of these sigmas on two metals)
     assert seen.sum() >= 2 and np.max(np.abs(d[:, seen, 0].mean(0)) / sem[seen]) > 4.0
+
+
+def _fd_rough():
+    return np.load(os.path.join(GOLDEN, "diff_fd_rough_64_8spp_depth6.npz"))
+
+
+def paired_z(adjoint, fd):
+    """z of the mean per-seed difference between adjoint gradients and the reference's
+    finite differences at the same seeds (per metal)."""
+    d = adjoint - fd
+    sem = d.std(0, ddof=1) / np.sqrt(d.shape[0])
+    return d.mean(0) / np.maximum(sem, 1e-12), sem
+
+
+def test_oracle_roughness_gradients_match_reference_finite_differences(workload_dir):
+    """Depth-6 roughness gradients (Russian roulette, mirror fallbacks, MIS) of four rough
+    metals against the reference's FP64 central differences (h = 0.02) at 64 seeds
+    (make_golden.py fd_rough_fixture). Per seed the two differ by the jumps a finite
+    difference of a fixed-seed estimator picks up; over the seeds both estimate the same
+    derivative of the expected loss, so their paired mean difference must be within 4
+    standard errors, while every metal's reference gradient is itself at least 3 standard
+    errors from zero (the test can see a missing or mis-signed term)."""
+    from luz_amd import workloads
+    g = _fd_rough()
+    params = workloads.diff_rough_parameters()
+    w, h, spp, depth = int(g["width"]), int(g["height"]), int(g["spp"]), int(g["depth"])
+    wl = workloads.write_diff_cornell(os.path.join(workload_dir, "diff_rough_oracle"), params, w, h, spp, depth)
+    fd = g["fd"]
+    metals = g["metals"]
+    adj = np.zeros_like(fd)
+    for si, seed in enumerate(g["seeds"]):
+        for j, m in enumerate(metals):
+            grads = oracle_lib.adjoint(wl.scene_path, int(seed), spp, w, h, g["weights"][j], 17)
+            adj[si, j] = grads[m, 3]
+    z, sem = paired_z(adj, fd)
+    fd_sem = fd.std(0, ddof=1) / np.sqrt(fd.shape[0])
+    print("reference FD", fd.mean(0), "+-", fd_sem, "\noracle adjoint", adj.mean(0), "\npaired z", z)
+    assert np.all(np.abs(fd.mean(0)) > 3 * fd_sem)
+    assert np.all(np.abs(z) < 4), z
