@@ -1,7 +1,7 @@
 """Throughput benchmark of the radiance path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload spheres|cornell|batch|terrain|diff]
-                    [--integrator megakernel|wavefront] [--no-cpu-baseline]
+                    [--integrator megakernel|wavefront] [--no-cpu-baseline] [--no-extras]
 
 A step is one full render of the workload at its configured resolution and spp
 (Renderer::render worker loop, Threads.cpp:375-548), inputs resident in HBM and the image
@@ -11,8 +11,13 @@ triangles), 40/30/30 Lambertian / GGX metal / dielectric, procedural HDR sky, 19
 W*H*spp*steps / seconds (docker/benchmark/benchmark.sh:150-151), with Mrays/s beside it.
 
 The CPU baseline is the reference renderer itself (oracle/_ref, built from the reference
-sources by oracle/Makefile) on this host's cores for a bounded sample of the same
-workload (16 spp at full resolution), timed by its own render clock.
+sources by oracle/Makefile) on every CPU this process may use, for a bounded sample of the
+same workload (16 spp at full resolution), timed by its own render clock; the host's CPU
+model and core counts are reported with it, and oracle/calibration.json says how the
+reference's release flags compare. The same 16 samples rendered on the device are checked
+against that reference image (`quality`: relative MSE, bar 1e-4). The default line also
+carries the differentiable box's forward + adjoint iteration (`diff`) and the uneven
+32-scene batch with its small-scene regime (`batch`), measured after the timed steps.
 """
 
 from __future__ import annotations
@@ -36,29 +41,97 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene_path: str, seed: int, spp: int, threads: int):
+def host_cpu():
+    """The host's CPU model and its physical / logical core counts (lscpu), and the threads
+    this process may use (its CPU affinity)."""
+    info = {"model": None, "sockets": None, "physical_cores": None, "logical_cpus": os.cpu_count(),
+            "threads_available": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=30).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        sockets = int(kv.get("Socket(s)", "0") or 0)
+        cores = int(kv.get("Core(s) per socket", "0") or 0)
+        info["sockets"] = sockets or None
+        info["physical_cores"] = sockets * cores or None
+        info["logical_cpus"] = int(kv.get("CPU(s)", info["logical_cpus"]) or info["logical_cpus"])
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def calibration(workload: str):
+    """Release-build speed ratio of the reference on its calibration host
+    (oracle/calibration.json, written by oracle/calibrate.py) for this workload, or None."""
+    path = os.path.join(ROOT, "oracle", "calibration.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        cal = json.load(open(path))
+        case = cal["cases"][workload]
+        return {"release_speedup": case["release_speedup"], "host_cpu": cal.get("host_cpu"), "threads": cal.get("threads"),
+                "faithful_ms": case["faithful_ms"], "release_ms": case["release_ms"],
+                "release_image_rel_mse": case["release_image_rel_mse"],
+                "case": f"{case['width']}x{case['height']}x{case['spp']}spp"}
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def cpu_baseline(scene_path: str, seed: int, spp: int, workload: str):
+    """The reference's own Renderer::render (oracle/_ref/luzref, built from its sources) on
+    every CPU this process may use, for `spp` of the workload's samples at full resolution,
+    timed by its render clock. Returns (baseline dict, the reference's FP64 image)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "luzref")
     if not os.path.exists(exe):
-        return None
+        return None, None
+    host = host_cpu()
+    threads = host["threads_available"]
     out = os.path.join(tempfile.mkdtemp(prefix="luz_cpu_"), "img.bin")
     t0 = time.time()
     proc = subprocess.run([exe, "render", scene_path, str(seed), str(threads), out, str(spp)],
                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=900)
     if proc.returncode != 0:
         log("cpu baseline failed:", proc.stdout.decode(errors="replace")[-500:])
-        return None
-    data = open(out, "rb").read(16)
+        return None, None
+    import numpy as np
+    data = open(out, "rb").read()
     w, h = struct.unpack("<II", data[:8])
     render_ms = struct.unpack("<d", data[8:16])[0]
-    log(f"cpu baseline: {w}x{h}x{spp} in {render_ms:.0f} ms render ({time.time() - t0:.1f} s wall incl. load)")
-    return {
-        "value": round(w * h * spp / (render_ms / 1000.0) / 1e6, 4),
+    image = np.frombuffer(data[16:], dtype="<f8").reshape(h, w, 3)
+    measured = w * h * spp / (render_ms / 1000.0) / 1e6
+    log(f"cpu baseline: {w}x{h}x{spp} in {render_ms:.0f} ms render on {threads} threads "
+        f"({time.time() - t0:.1f} s wall incl. load): {measured:.3f} Msamples/s; host {host}")
+    base = {
+        "value": round(measured, 4),
         "unit": "Msamples/s",
         "cores": threads,
         "kind": "reference",
         "sample": f"reference Renderer::render of the same scene, {w}x{h} at {spp} spp (of the workload's full spp), "
-                  f"--threads {threads}, seed {seed}",
+                  f"--threads {threads} (every CPU of this process), seed {seed}; faithful build (-O3, no fast-math / "
+                  f"native / LTO)",
+        "host": host,
     }
+    cal = calibration(workload)
+    if cal:
+        base["calibration"] = cal
+        if cal["release_image_rel_mse"] > 1e-4:
+            # -ffast-math changes what the release build renders for this workload: its time
+            # is not the time of the same work, so the faithful build's figure stands
+            base["sample"] += (f"; the reference's release build (-ffast-math -march=native -flto) renders this workload "
+                               f"differently (image rel-MSE {cal['release_image_rel_mse']:.3g} against the faithful build, "
+                               f"oracle/calibration.json), so its speed is not used")
+        elif abs(cal["release_speedup"] - 1.0) > 0.05:
+            # the reference's release flags (-ffast-math -march=native -flto) were this much
+            # faster on the calibration host: report the baseline scaled to them
+            base["faithful_build_value"] = base["value"]
+            base["value"] = round(measured * cal["release_speedup"], 4)
+            base["sample"] += (f"; value scaled by the release build's measured speedup x{cal['release_speedup']:.3f} "
+                               f"({cal['case']} on {cal['threads']} threads of {cal['host_cpu']}, oracle/calibration.json)")
+    return base, image
 
 
 def cpu_nfor_baseline(scene_path: str, threads: int, width: int, height: int, full_pixels: int):
@@ -94,9 +167,12 @@ def load_traffic(workload: str, kernel: str):
         return None
 
 
-def bench_batch(args, wdir):
+def bench_batch(args, wdir, steps=None, warmup=None):
     """32 uneven scenes in one multi-scene launch sequence against the same scenes rendered
-    one at a time by the plain single-scene megakernel (and by the single-scene wavefront)."""
+    one at a time by the plain single-scene megakernel (and by the single-scene wavefront).
+    Returns the result line."""
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     import torch
     from luz_amd import Context, Renderer, SceneFile, workloads
     wls = workloads.batch_scenes(os.path.join(wdir, "batch"))
@@ -112,17 +188,17 @@ def bench_batch(args, wdir):
         tuning = "auto"
         log(f"autotune: {tuned.as_dict()} in {tuned.tune_ms:.0f} ms over {tuned.trials} shapes "
             f"(calibration pass {tuned.default_ms:.2f} -> {tuned.tuned_ms:.2f} ms)")
-    for i in range(args.warmup):
+    for i in range(warmup):
         _, st = Renderer.render_batch(scenes, seeds, outs=outs, tuning=tuning)
         log(f"warmup {i}: {st.render_ms:.1f} ms")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         _, st = Renderer.render_batch(scenes, seeds, outs=outs, tuning=tuning)
         log(f"step {i}: {st.render_ms:.1f} ms, {st.kernel_launches} passes")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    value = total * args.steps / elapsed / 1e6
+    value = total * steps / elapsed / 1e6
     # the same scenes one by one (untimed for the headline; each timed by its own events)
     plain_ms = wave_ms = 0.0
     for w, sc, sd, o in zip(wls, scenes, seeds, outs):
@@ -130,7 +206,7 @@ def bench_batch(args, wdir):
         _, b = Renderer.render(sc, seed=sd, integrator="wavefront", out=o)
         plain_ms += a.render_ms
         wave_ms += b.render_ms
-    batch_ms = elapsed * 1000.0 / args.steps
+    batch_ms = elapsed * 1000.0 / steps
     log(f"one at a time: megakernel {plain_ms:.0f} ms, wavefront {wave_ms:.0f} ms; batched {batch_ms:.0f} ms "
         f"({plain_ms / batch_ms:.2f}x / {wave_ms / batch_ms:.2f}x)")
     # The small-scene regime: the same 32 scenes at sizes where one scene cannot fill the GPU.
@@ -160,8 +236,8 @@ def bench_batch(args, wdir):
                       "batched_wavefront_vs_plain": round(plain / batched["wavefront"], 3),
                       "batched_plain_kernel_vs_plain": round(plain / batched["megakernel"], 3)})
     line = {
-        "metric": "Msamples/s", "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(batch_ms, 3), "higher_is_better": True, "scaling": "weak",
+        "metric": "Msamples/s", "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(batch_ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"batch: {len(wls)} scenes of displaced sphere meshes, triangles log-uniform [1e4, 5e5], "
                                f"resolutions from 256^2 to 1024^2, spp 16..128, depth 10",
@@ -171,17 +247,19 @@ def bench_batch(args, wdir):
         "small_scenes": small,
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
     for sc in scenes:
         sc.release()
     ctx.close()
+    return line
 
 
-def bench_diff(args, wdir):
+def bench_diff(args, wdir, steps=None, warmup=None):
     """Differentiable box, 512x512, 16 materials, 16 spp, depth 6: one iteration = forward
     render + L2 loss against a target rendered at perturbed parameters + path-replay adjoint
-    (through luz_amd.diff, a torch.autograd.Function). Reports the iteration time and its
-    ratio to the forward alone."""
+    (through luz_amd.diff, a torch.autograd.Function). Returns the result line: the
+    iteration time and its ratio to the forward alone."""
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     import numpy as np
     import torch
     from luz_amd import Context, SceneFile, diff, workloads
@@ -211,14 +289,14 @@ def bench_diff(args, wdir):
         return loss
 
     def timed(fn, label):
-        for i in range(args.warmup):
+        for i in range(warmup):
             fn()
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
-        for i in range(args.steps):
+        for i in range(steps):
             fn()
         torch.cuda.synchronize(dev)
-        ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
         log(f"{label}: {ms:.2f} ms")
         return ms
 
@@ -228,8 +306,8 @@ def bench_diff(args, wdir):
     log(f"loss {loss.item():.6g}; |d albedo| {a.grad.abs().sum().item():.4g}, |d roughness| {r.grad.abs().sum().item():.4g}")
     samples = 512 * 512 * 16
     line = {
-        "metric": "fwd+adjoint ms/iter", "value": round(it_ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(it_ms, 3), "higher_is_better": False, "scaling": "weak",
+        "metric": "fwd+adjoint ms/iter", "value": round(it_ms, 3), "unit": "ms", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(it_ms, 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "differentiable box: 512x512, 16 materials (12 Lambertian, 4 GGX metals) + 1 emitter, "
                                "16 spp, depth 6, L2 loss against a target at perturbed parameters",
@@ -238,9 +316,9 @@ def bench_diff(args, wdir):
         "forward_msamples_per_s": round(samples / fwd_ms / 1e3, 3),
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
     scene.release()
     ctx.close()
+    return line
 
 
 def main():
@@ -255,6 +333,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=16)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the forward+adjoint and batch figures attached to the default line")
     ap.add_argument("--probe-copy", action="store_true", help="run one plain 1 GiB copy kernel (PMC scale probe)")
     args = ap.parse_args()
 
@@ -269,9 +349,11 @@ def main():
     wdir = os.path.join(tempfile.gettempdir(), "luz_bench_workloads")
     t0 = time.time()
     if args.workload == "batch":
-        return bench_batch(args, wdir)
+        print(json.dumps(bench_batch(args, wdir)), flush=True)
+        return
     if args.workload == "diff":
-        return bench_diff(args, wdir)
+        print(json.dumps(bench_diff(args, wdir)), flush=True)
+        return
     if args.workload == "terrain":
         wl = workloads.terrain_scene(os.path.join(wdir, "terrain"))
     elif args.workload == "spheres":
@@ -383,6 +465,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "traffic_source": (f"profiles/traffic_{kernel}_{wl.name}.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                               "per dispatch of this kernel on this workload, scaled by a 1 GiB copy probe")
+            if traffic else None,
             "kernel": kernel,
             "kernel_ms_per_launch": round(kernel_s * 1000.0 / max(launches, 1), 3),
             "algorithmic_bytes_per_launch": round(bytes_per_launch),
@@ -400,13 +485,39 @@ def main():
         line["denoise"] = {"nfor_ms": round(dst.denoise_ms, 3), "render_ms": round(dst.render_ms, 3),
                            "regression_radius": 9, "patch_radius": 1, "feature_prefilter_radius": 1}
         if not args.no_cpu_baseline:
-            line["denoise"]["cpu_baseline"] = cpu_nfor_baseline(wl.scene_path, min(16, os.cpu_count() or 1), 320, 180,
+            line["denoise"]["cpu_baseline"] = cpu_nfor_baseline(wl.scene_path, len(os.sched_getaffinity(0)), 320, 180,
                                                                 wl.width * wl.height)
+    # the quality gate's device image: samples [0, q) of the same frame (untimed), set
+    # against the reference's render of those samples from the CPU leg below
+    qspp = min(args.cpu_spp, wl.spp)
+    quality_img = None
+    if not args.no_cpu_baseline:
+        quality_img, _ = Renderer.render(scene, seed=args.seed, samples=qspp, integrator=args.integrator, tuning=tuning)
     scene.release()
     ctx.close()
+    if args.workload == "spheres" and not args.no_extras:
+        # the other two headline figures, outside the timed region: the differentiable box's
+        # forward + adjoint iteration and the uneven batch (with its small-scene regime)
+        d = bench_diff(args, wdir, steps=5, warmup=2)
+        line["diff"] = {k: d[k] for k in ("metric", "value", "unit", "forward_ms", "iteration_over_forward",
+                                          "forward_msamples_per_s", "steps", "warmup")}
+        line["diff"]["config"] = d["config"]["workload"]
+        b = bench_batch(args, wdir, steps=2, warmup=1)
+        line["batch"] = {k: b[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "warmup",
+                                           "batched_vs_plain_megakernel", "batched_vs_single_scene_wavefront",
+                                           "small_scenes")}
+        line["batch"]["config"] = b["config"]["workload"]
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(wl.scene_path, args.seed, min(args.cpu_spp, wl.spp), threads)
+        base, ref_img = cpu_baseline(wl.scene_path, args.seed, qspp, wl.name)
+        line["cpu_baseline"] = base
+        if ref_img is not None and quality_img is not None:
+            import numpy as np
+            r = ref_img.astype(np.float64)
+            e = float(((quality_img.astype(np.float64) - r) ** 2).sum() / max((r ** 2).sum(), 1e-300))
+            line["quality"] = {"rel_mse": e, "bar": 1e-4, "pass": e <= 1e-4,
+                               "against": f"the reference's FP64 render of samples [0, {qspp}) of the same frame "
+                                          f"({wl.width}x{wl.height}, seed {args.seed}) from the CPU leg"}
+            log(f"quality: rel-MSE {e:.3e} against the reference's {wl.width}x{wl.height}x{qspp} render")
     print(json.dumps(line), flush=True)
 
 
