@@ -194,6 +194,148 @@ const char*	luzgpu_last_error(void);
  * structures built on the host only (for inspection; it cannot be rendered). */
 int		luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out);
 void	luzgpu_scene_release(luzgpu_scene* scene);
+
+/* --- an in-memory Scene, flattened (no file) ---
+ * The seam Renderer::render(Scene&) (Renderer.hpp:7) -> _manageThreads(Scene&)
+ * (Threads.cpp:375) holds a populated Scene, not a path: scenes built in code (the
+ * reference's tests, --benchmark scenes, SceneHelpers.cpp:602) never see a file. These
+ * records carry what that Scene holds, in its own units and FP64, so the host builds the
+ * same acceleration structures and light tables as luzgpu_scene_load does from a file. */
+
+enum	/* material kinds (the reference's Material subclasses) */
+{
+	LUZGPU_MAT_LAMBERTIAN = 1,        /* Lambertian(color)                                        */
+	LUZGPU_MAT_METAL = 2,             /* Metal(color, roughness) or Metal(eta, k, roughness)      */
+	LUZGPU_MAT_DIELECTRIC = 3,        /* Dielectric(color, ior, roughness) + absorption           */
+	LUZGPU_MAT_EMISSIVE = 4,          /* Emissive: color = the emitted radiance                   */
+	LUZGPU_MAT_ISOTROPIC = 5,         /* Isotropic(color)                                         */
+	LUZGPU_MAT_HENYEY_GREENSTEIN = 6, /* HenyeyGreenstein(color, anisotropy)                      */
+	LUZGPU_MAT_GLOSSY = 7,            /* Glossy(color, roughness)                                 */
+	LUZGPU_MAT_DIFFUSE_GLOSSY = 8,    /* DiffuseGlossy(color, glossy color, weight, roughness)    */
+	LUZGPU_MAT_PRINCIPLED = 9         /* Principled without subsurface                            */
+};
+
+enum	/* top-level hittable kinds (Scene::getHittables() entries) */
+{
+	LUZGPU_HIT_TRIANGLE = 0,
+	LUZGPU_HIT_RECTANGLE = 1,
+	LUZGPU_HIT_INSTANCE = 2,          /* MeshInstance of meshes[mesh]                             */
+	LUZGPU_HIT_SPHERE = 3,
+	LUZGPU_HIT_CUBE = 4,
+	LUZGPU_HIT_VOLUME = 5,            /* ConstantVolume over a sphere or box boundary             */
+	LUZGPU_HIT_PLANE = 6,
+	LUZGPU_HIT_DIRECTIONAL = 7        /* DirectionalLight                                         */
+};
+
+enum	/* Scene::getRenderSky() (SkyTypes.hpp) */
+{
+	LUZGPU_SKY_NONE = 0,              /* the background colour                                    */
+	LUZGPU_SKY_ENVIRONMENT = 1,       /* the environment map                                      */
+	LUZGPU_SKY_LINEAR = 2,            /* the white-to-blue gradient                               */
+	LUZGPU_SKY_ATMOSPHERE = 3         /* the physical atmosphere (the reference's default)        */
+};
+
+/* One material: the fields its kind reads (the others are ignored). */
+typedef struct luzgpu_material_desc
+{
+	uint32_t	kind;                /* LUZGPU_MAT_*                                             */
+	uint32_t	conductor;           /* metal: 1 = conductor Fresnel from eta / k                */
+	double		color[3];            /* albedo / Schlick f0 / tint / emitted radiance / base     */
+	double		roughness;           /* metal, dielectric, glossy, diffuse-glossy, principled    */
+	double		ior;                 /* dielectric, principled                                   */
+	double		eta[3], k[3];        /* conductor metal                                          */
+	double		absorption[3];       /* dielectric / principled interior, 1/m                    */
+	double		anisotropy;          /* Henyey-Greenstein g                                      */
+	double		glossy_color[3];     /* diffuse-glossy                                           */
+	double		glossy_weight;
+	double		metallic, transmission, clearcoat, clearcoat_roughness, sheen;  /* principled   */
+} luzgpu_material_desc;
+
+/* One mesh (Mesh::_triangles, after its degenerate-triangle removal): vertices[9*t..] =
+ * v0, v1, v2 of triangle t; normals[9*t..] the vertex normals, or NULL. */
+typedef struct luzgpu_mesh_desc
+{
+	uint32_t		triangle_count;
+	uint32_t		pad;
+	const double*	vertices;
+	const double*	normals;
+} luzgpu_mesh_desc;
+
+/* One top-level hittable; per kind:
+ *   TRIANGLE     a, b, c = vertices; normals = vertex normals when flags & 1
+ *   RECTANGLE    a = position, b = orientation (normal), size[0..1] = width, height
+ *   INSTANCE     mesh; a = position, c = scale; cos_rotation / sin_rotation of the X, Y, Z
+ *                Euler angles (MeshInstance::InstanceTransform)
+ *   SPHERE       a = centre, radius
+ *   CUBE         a = position, size = width, height, depth
+ *   VOLUME       flags & 1: sphere boundary (a = centre, radius), else box (a = position,
+ *                size); density (per scene unit); material = the phase function
+ *   PLANE        plane_y, b = orientation
+ *   DIRECTIONAL  a = the light's direction of travel (normalised) */
+typedef struct luzgpu_hittable_desc
+{
+	uint32_t	kind;                /* LUZGPU_HIT_*                                             */
+	uint32_t	material;            /* index into materials                                     */
+	uint32_t	mesh;                /* INSTANCE: index into meshes                              */
+	uint32_t	flags;
+	double		a[3], b[3], c[3];
+	double		normals[9];
+	double		size[3];
+	double		radius;
+	double		plane_y;
+	double		density;
+	double		cos_rotation[3];
+	double		sin_rotation[3];
+} luzgpu_hittable_desc;
+
+/* Scene::getActiveCamera() (Camera.hpp), lengths in metres. */
+typedef struct luzgpu_camera_desc
+{
+	double		position[3], direction[3], up[3];
+	double		focal_length, sensor_width, sensor_height, f_number, focus_distance;
+	uint32_t	pinhole;
+	uint32_t	pad;
+} luzgpu_camera_desc;
+
+typedef struct luzgpu_scene_desc
+{
+	/* Scene getters (Scene.hpp:33-131) and the image size */
+	uint32_t	width, height, samples;
+	int32_t		max_bounces;
+	uint32_t	adaptive, adaptive_min_samples, adaptive_check_interval, denoise;
+	double		adaptive_threshold;
+	double		meters_per_unit;
+	uint32_t	sky;                 /* LUZGPU_SKY_*                                             */
+	uint32_t	view_transform;      /* LUZGPU_VIEW_*                                            */
+	uint32_t	bloom;
+	uint32_t	pad0;
+	double		exposure, contrast;
+	double		background[3];
+	/* Scene::getAtmosphere() (Atmosphere.hpp): sun angle (radians), radii and scale heights
+	 * (metres), march samples, stars brightness, and the sun's radiance scale */
+	double		sun_angle, earth_radius, atmosphere_radius, rayleigh_height, mie_height;
+	double		stars_brightness, sun_radiance_scale;
+	int32_t		atmosphere_samples, atmosphere_light_samples;
+	/* Scene::getEnvironmentMap(): env_width x env_height texels, 3 doubles each, scene-linear,
+	 * row 0 at the top (EnvironmentMap::_pixels); strength, rotation (degrees), lighting */
+	uint32_t	env_width, env_height;
+	const double*	env_texels;
+	double		env_strength, env_rotation;
+	uint32_t	env_lighting;
+	uint32_t	pad1;
+	luzgpu_camera_desc	camera;
+	uint32_t	material_count, mesh_count, hittable_count;
+	uint32_t	pad2;
+	const luzgpu_material_desc*	materials;
+	const luzgpu_mesh_desc*		meshes;
+	const luzgpu_hittable_desc*	hittables;
+} luzgpu_scene_desc;
+
+/* The same as luzgpu_scene_load for a Scene given as records: the scene's pre-render sync
+ * (Renderer.cpp:18-20: updateLights, updateAccelerationStructure) and the upload. All
+ * arrays are read during the call only. With ctx == NULL the scene is built on the host
+ * only. Replaces the SceneFile::read step for callers that hold a Scene (Renderer.hpp:7). */
+int		luzgpu_scene_create(luzgpu_ctx* ctx, const luzgpu_scene_desc* desc, luzgpu_scene** out);
 int		luzgpu_scene_get_info(const luzgpu_scene* scene, luzgpu_scene_info* out);
 
 /* Renders samples [sample_begin, sample_begin + sample_count) and writes

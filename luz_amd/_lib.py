@@ -29,6 +29,7 @@ EXPORTED = [
     "luzgpu_sampler_eval", "luzgpu_scene_tlas_topology", "luzgpu_scene_blas", "luzgpu_render_batch",
     "luzgpu_adjoint", "luzgpu_scene_set_materials", "luzgpu_autotune", "luzgpu_render_denoised", "luzgpu_nfor",
     "luzgpu_sample_counts", "luzgpu_scene_post_settings", "luzgpu_postprocess", "luzgpu_render_noalloc",
+    "luzgpu_scene_create",
 ]
 
 

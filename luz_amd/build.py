@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libluzgpu.so")
 BUILD = os.path.join(HERE, "_build")
 
-HOST_SOURCES = ["host/scene_file.cpp", "host/accel.cpp"]
+HOST_SOURCES = ["host/scene_file.cpp", "host/accel.cpp", "host/scene_desc.cpp"]
 DEVICE_SOURCES = ["luzgpu.hip"]
 ARCH = "gfx950"
 

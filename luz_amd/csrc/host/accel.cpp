@@ -248,6 +248,13 @@ namespace luzgpu
 			explicit InstanceXf(const Instance& inst) : position(inst.position), scale(inst.scale)
 			{
 				// MeshInstance.cpp:48-75
+				if (inst.hasTrig)
+				{
+					cx = inst.cosRotation[0]; sx = inst.sinRotation[0];
+					cy = inst.cosRotation[1]; sy = inst.sinRotation[1];
+					cz = inst.cosRotation[2]; sz = inst.sinRotation[2];
+					return;
+				}
 				const double d2r = 3.14159265358979323846 / 180.0;
 				cx = std::cos(inst.rotationDegrees[0] * d2r); sx = std::sin(inst.rotationDegrees[0] * d2r);
 				cy = std::cos(inst.rotationDegrees[1] * d2r); sy = std::sin(inst.rotationDegrees[1] * d2r);

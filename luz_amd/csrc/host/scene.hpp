@@ -9,6 +9,8 @@
 #include <string>
 #include <vector>
 
+struct	luzgpu_scene_desc;	// include/luzgpu.h
+
 namespace luzgpu
 {
 	using dvec3 = std::array<double, 3>;
@@ -136,6 +138,11 @@ namespace luzgpu
 		dvec3	position{};
 		dvec3	rotationDegrees{};
 		dvec3	scale = {1.0, 1.0, 1.0};
+		// an instance handed over from an in-memory Scene carries the cosines and sines its
+		// MeshInstance::InstanceTransform holds (used instead of rotationDegrees)
+		bool	hasTrig = false;
+		dvec3	cosRotation = {1.0, 1.0, 1.0};
+		dvec3	sinRotation{};
 	};
 
 	// One entry per reference top-level hittable, in Scene::getHittables() order.
@@ -229,6 +236,10 @@ namespace luzgpu
 		Environment	environment;
 		bool	hasEnvironment = false;
 	};
+
+	// Builds the host scene from the C ABI's flattened records (luzgpu_scene_desc,
+	// include/luzgpu.h); throws std::invalid_argument on inconsistent records.
+	Scene	sceneFromDesc(const ::luzgpu_scene_desc& desc);
 
 	// Reads the subset of the reference's .luz format listed in scene_file.cpp.
 	// Throws std::runtime_error with the reference's style of message on bad input.

@@ -1320,6 +1320,55 @@ namespace
 	}
 }
 
+namespace
+{
+	// The pre-render sync of Renderer::render (Renderer.cpp:18-20: updateLights,
+	// updateAccelerationStructure) for a parsed or handed-over scene, then the upload.
+	luzgpu_scene*	finishScene(luzgpu_ctx* ctx, const Scene& scene)
+	{
+		auto sc = std::make_unique<luzgpu_scene>();
+		sc->ctx = ctx;
+		static std::atomic<std::uint64_t> serials{0};
+		sc->serial = ++serials;
+		sc->gpu.host = packScene(scene);
+		const PackedScene& h = sc->gpu.host;
+		// Worst-case stack use: deferred TLAS entries, a leaf-resume entry and the instance
+		// exit marker, then deferred mesh-BVH entries. Scenes that could exceed the smaller
+		// of the two kernels' stacks are refused rather than traversed incompletely.
+		const int tlasDepth = h.tlasDeviceDepth;
+		const int blasDepth = h.blasDeviceDepth;
+		const int bound = h.tlasStackBound + 2 + h.blasStackBound;
+		if (bound > kStackDepth)
+		{
+			throw std::runtime_error("Unsupported for the device path: traversal stack bound " + std::to_string(bound)
+				+ " exceeds " + std::to_string(kStackDepth));
+		}
+		if (ctx)
+		{
+			uploadScene(ctx, sc->gpu);
+		}
+		luzgpu_scene_info& info = sc->gpu.info;
+		info.width = h.params.width;
+		info.height = h.params.height;
+		info.samples = h.params.totalSamples;
+		info.max_bounces = h.params.maxBounces;
+		info.hittables = static_cast<std::uint32_t>(h.hittables.size());
+		info.triangles = static_cast<std::uint32_t>(h.triangleCount);
+		info.meshes = static_cast<std::uint32_t>(h.blas.size());
+		info.instances = static_cast<std::uint32_t>(h.instances.size());
+		info.lights = static_cast<std::uint32_t>(h.lights.size());
+		info.tlas_nodes = static_cast<std::uint32_t>(h.tlasWide.size());
+		info.blas_nodes = static_cast<std::uint32_t>(h.blasWide.size());
+		info.env_width = h.params.envWidth;
+		info.env_height = h.params.envHeight;
+		info.materials = static_cast<std::uint32_t>(h.materials.size());
+		info.tlas_depth = static_cast<std::uint32_t>(tlasDepth);
+		info.blas_depth = static_cast<std::uint32_t>(blasDepth);
+		if (ctx) ctx->scenes.insert(sc.get());
+		return sc.release();
+	}
+}
+
 extern "C" {
 
 const char*	luzgpu_last_error(void)
@@ -1399,47 +1448,24 @@ int	luzgpu_scene_load(luzgpu_ctx* ctx, const char* path, luzgpu_scene** out)
 		{
 			check(hipSetDevice(ctx->device), "hipSetDevice");
 		}
-		auto sc = std::make_unique<luzgpu_scene>();
-		sc->ctx = ctx;
-		static std::atomic<std::uint64_t> serials{0};
-		sc->serial = ++serials;
-		const Scene scene = readSceneFile(path);
-		sc->gpu.host = packScene(scene);
-		const PackedScene& h = sc->gpu.host;
-		// Worst-case stack use: deferred TLAS entries, a leaf-resume entry and the instance
-		// exit marker, then deferred mesh-BVH entries. Scenes that could exceed the smaller
-		// of the two kernels' stacks are refused rather than traversed incompletely.
-		const int tlasDepth = h.tlasDeviceDepth;
-		const int blasDepth = h.blasDeviceDepth;
-		const int bound = h.tlasStackBound + 2 + h.blasStackBound;
-		if (bound > kStackDepth)
-		{
-			throw std::runtime_error("Unsupported for the device path: traversal stack bound " + std::to_string(bound)
-				+ " exceeds " + std::to_string(kStackDepth));
-		}
+		*out = finishScene(ctx, readSceneFile(path));
+	});
+}
+
+int	luzgpu_scene_create(luzgpu_ctx* ctx, const luzgpu_scene_desc* desc, luzgpu_scene** out)
+{
+	if (!desc || !out)
+	{
+		g_lastError = "luzgpu_scene_create: NULL argument";
+		return LUZGPU_E_INVALID;
+	}
+	*out = nullptr;
+	return guarded([&] {
 		if (ctx)
 		{
-			uploadScene(ctx, sc->gpu);
+			check(hipSetDevice(ctx->device), "hipSetDevice");
 		}
-		luzgpu_scene_info& info = sc->gpu.info;
-		info.width = h.params.width;
-		info.height = h.params.height;
-		info.samples = h.params.totalSamples;
-		info.max_bounces = h.params.maxBounces;
-		info.hittables = static_cast<std::uint32_t>(h.hittables.size());
-		info.triangles = static_cast<std::uint32_t>(h.triangleCount);
-		info.meshes = static_cast<std::uint32_t>(h.blas.size());
-		info.instances = static_cast<std::uint32_t>(h.instances.size());
-		info.lights = static_cast<std::uint32_t>(h.lights.size());
-		info.tlas_nodes = static_cast<std::uint32_t>(h.tlasWide.size());
-		info.blas_nodes = static_cast<std::uint32_t>(h.blasWide.size());
-		info.env_width = h.params.envWidth;
-		info.env_height = h.params.envHeight;
-		info.materials = static_cast<std::uint32_t>(h.materials.size());
-		info.tlas_depth = static_cast<std::uint32_t>(tlasDepth);
-		info.blas_depth = static_cast<std::uint32_t>(blasDepth);
-		if (ctx) ctx->scenes.insert(sc.get());
-		*out = sc.release();
+		*out = finishScene(ctx, sceneFromDesc(*desc));
 	});
 }
 

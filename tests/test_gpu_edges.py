@@ -158,3 +158,31 @@ def test_scene_outlives_its_context():
     assert scene.post_settings().view_transform == 3
     scene.release()
     scene.release()
+
+
+def test_render_of_a_scene_whose_context_is_gone_is_refused(primitives):
+    """After luzgpu_destroy the scene's info and topology still answer, a render of it is
+    refused (LUZGPU_E_INVALID) rather than touching freed device memory, and a new context
+    on the device works."""
+    import ctypes
+    from luz_amd import Context, SceneFile, _lib
+    ctx = Context(0)
+    scene = SceneFile.read(ctx, primitives.scene_path)
+    topology = scene.tlas_topology()
+    ctx.close()
+    info = _lib.SceneInfo()
+    _lib.check(scene._lib.luzgpu_scene_get_info(scene._h, ctypes.byref(info)))
+    assert info.hittables == scene.info["hittables"]
+    assert scene.tlas_topology() == topology
+    out = np.zeros(16 * 16 * 3, dtype=np.float32)
+    s = _lib.Settings()
+    s.seed, s.samples, s.max_bounces, s.sample_count, s.width, s.height = 1, 1, -1, -1, 16, 16
+    assert scene._lib.luzgpu_render(None, scene._h, ctypes.byref(s), out.ctypes.data, None) == _lib.E_INVALID
+    scene.release()
+    # a new context on the same device works after all this
+    with Context(0) as ctx2:
+        sc2 = SceneFile.read(ctx2, primitives.scene_path)
+        from luz_amd import Renderer
+        img, _ = Renderer.render(sc2, seed=1, samples=1, resolution=(16, 16))
+        assert np.isfinite(img).all()
+        sc2.release()

@@ -1,0 +1,30 @@
+"""The reference-side binding end to end on the device (GPU): a Scene built in code (no
+file) rendered through luzgpu_glue::manageThreadsOnDevice -- flattened, handed over with
+luzgpu_scene_create, rendered, the image left in the Scene -- against the same Scene rendered
+by the reference's own Renderer::render on the host with the same seed (oracle/glue_driver.cpp,
+prebuilt by `make -C oracle glue` where the reference sources are)."""
+
+import os
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "glue_driver")
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(DRIVER), reason="glue driver not built (make -C oracle glue)")]
+
+
+def test_scene_built_in_code_renders_on_device_like_the_reference():
+    print("glue_driver render-inmemory 0 160 120 16 7", flush=True)
+    out = subprocess.run([DRIVER, "render-inmemory", "0", "160", "120", "16", "7"], stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True, timeout=300)
+    print(out.stdout, out.stderr[-2000:])
+    assert out.returncode == 0
+    tokens = out.stdout.split()
+    rel = float(tokens[tokens.index("rel_mse") + 1])
+    i = tokens.index("samples")
+    assert tokens[i + 1] == tokens[i + 2] == str(160 * 120 * 16)
+    assert rel <= 1e-4
