@@ -73,16 +73,39 @@ namespace luzgpu
 		std::uint32_t*	counts;
 		std::uint32_t	capacity;
 		std::uint32_t*	spill;		// traversal stack spill, [entry][grid thread]
+		// Queued extension rays carry their scene above the global path id in qPath
+		// (scene << pathBits | path), shadow rays above the connection slot in their tag
+		// (scene << (pathBits + 1) | 2 * path + slot). pathBits = 31 - the bits of the largest
+		// scene index of the render: 31 for one scene, 24 for 127.
+		std::uint32_t	pathBits;
+
+		__device__ __forceinline__ std::uint32_t	sceneOfPath(std::uint32_t qp) const { return qp >> pathBits; }
+		__device__ __forceinline__ std::uint32_t	pathOf(std::uint32_t qp) const { return qp & ((1u << pathBits) - 1u); }
+		__device__ __forceinline__ std::uint32_t	queuedPath(std::uint32_t scene, std::uint32_t path) const { return (scene << pathBits) | path; }
+		__device__ __forceinline__ std::uint32_t	shadowTag(std::uint32_t scene, std::uint32_t slot) const
+		{
+			return pathBits < 31u ? (scene << (pathBits + 1u)) | slot : slot;
+		}
+		__device__ __forceinline__ std::uint32_t	sceneOfTag(std::uint32_t tag) const { return pathBits < 31u ? tag >> (pathBits + 1u) : 0u; }
+		__device__ __forceinline__ std::uint32_t	slotOfTag(std::uint32_t tag) const
+		{
+			return pathBits < 31u ? tag & ((2u << pathBits) - 1u) : tag;
+		}
 	};
 
-	// Queued extension rays carry their scene in qPath (scene << 24 | global path id), and
-	// their volume sample in qOrigin.w.
-	constexpr std::uint32_t	kPathBits = 24;
-	constexpr std::uint32_t	kPathMask = (1u << kPathBits) - 1u;
-	constexpr std::uint32_t	kTargetBits = 25;
-	constexpr std::uint32_t	kTargetMask = (1u << kTargetBits) - 1u;
+	// Queued extension rays carry their volume sample in qOrigin.w.
 	constexpr std::uint32_t	kMaxBatchScenes = 127;
-	constexpr std::uint32_t	kMaxPassPaths = 1u << 24;
+	// Paths per pass: 2^27 (~39 GB of path state, 1920 x 1080 x 64 samples in one pass), fewer
+	// when a batch's scene bits leave less room in the 32-bit queue entries (2^24 at 127 scenes).
+	constexpr std::uint32_t	kMaxPassPaths = 1u << 27;
+
+	// pathBits of a render of n scenes (see WaveBuffers).
+	inline std::uint32_t	pathBitsFor(std::uint32_t n)
+	{
+		std::uint32_t sceneBits = 0;
+		while ((1u << sceneBits) < n) sceneBits++;
+		return 31u - sceneBits;
+	}
 
 	// One scene of a pass. The table lives in device memory: a pass is replayed with a new
 	// table and the same launches.
@@ -203,7 +226,7 @@ namespace luzgpu
 			w.qOrigin[0][g] = make_float4(r.o.x, r.o.y, r.o.z, logU);
 			w.qDir[0][g] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(kNoOrigin.hittable));
 			w.qFrom[0][g] = kNoOrigin.tri;
-			w.qPath[0][g] = (job << kPathBits) | path;
+			w.qPath[0][g] = w.queuedPath(job, path);
 			w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
 			w.radiance[path] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(PF_PREV_SPECULAR));
 		}
@@ -409,7 +432,7 @@ namespace luzgpu
 			tMax = INFINITY;
 			logU = o.w;
 			from = RayOrigin{__float_as_uint(d.w), w.qFrom[b & 1][i]};
-			return w.qPath[b & 1][i] >> kPathBits;
+			return w.sceneOfPath(w.qPath[b & 1][i]);
 		}
 		__device__ __forceinline__ void	store(std::uint32_t i, const TraceState& st) const
 		{
@@ -432,11 +455,11 @@ namespace luzgpu
 			r = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
 			tMax = o.w;
 			const std::uint32_t tag = __float_as_uint(d.w);
-			target = tag & kTargetMask;
+			target = w.slotOfTag(tag);
 			logU = volumes ? w.sLogU[i] : 0.0f;
 			const int2 f = w.sFrom[i];
 			from = RayOrigin{static_cast<std::uint32_t>(f.x), f.y};
-			return tag >> kTargetBits;
+			return w.sceneOfTag(tag);
 		}
 		__device__ __forceinline__ void	store(std::uint32_t, const TraceState& st) const
 		{
@@ -474,8 +497,8 @@ namespace luzgpu
 			f3 shadePoint = mk(0, 0, 0);
 			RayOrigin from = kNoOrigin;
 			const std::uint32_t qp = valid ? w.qPath[q][i] : 0u;
-			const std::uint32_t job = qp >> kPathBits;
-			const std::uint32_t path = qp & kPathMask;
+			const std::uint32_t job = w.sceneOfPath(qp);
+			const std::uint32_t path = w.pathOf(qp);
 			float nextLogU = 0.0f, shadowLogU = 0.0f;
 			// one scene per iteration, so the scene's parameters are wave-uniform (scalar loads)
 			unsigned long long pending = __ballot(valid);
@@ -667,12 +690,12 @@ namespace luzgpu
 				w.qFrom[nq][pos] = from.tri;
 				w.qPath[nq][pos] = qp;
 			}
-			const std::uint32_t tagBase = job << kTargetBits;
+			const std::uint32_t tagScene = job;
 			const std::uint32_t sl = waveAppend(wantLight, shadowCount);
 			if (wantLight)
 			{
 				w.sOrigin[sl] = make_float4(shadePoint.x, shadePoint.y, shadePoint.z, cl.tMax);
-				w.sDir[sl] = make_float4(cl.direction.x, cl.direction.y, cl.direction.z, __uint_as_float(tagBase | (2u * path)));
+				w.sDir[sl] = make_float4(cl.direction.x, cl.direction.y, cl.direction.z, __uint_as_float(w.shadowTag(tagScene, 2u * path)));
 				w.sFrom[sl] = make_int2(static_cast<int>(from.hittable), from.tri);
 				if (volumes) w.sLogU[sl] = shadowLogU;
 			}
@@ -680,7 +703,7 @@ namespace luzgpu
 			if (wantEnv)
 			{
 				w.sOrigin[se] = make_float4(shadePoint.x, shadePoint.y, shadePoint.z, INFINITY);
-				w.sDir[se] = make_float4(ce.direction.x, ce.direction.y, ce.direction.z, __uint_as_float(tagBase | (2u * path + 1u)));
+				w.sDir[se] = make_float4(ce.direction.x, ce.direction.y, ce.direction.z, __uint_as_float(w.shadowTag(tagScene, 2u * path + 1u)));
 				w.sFrom[se] = make_int2(static_cast<int>(from.hittable), from.tri);
 				if (volumes) w.sLogU[se] = shadowLogU;
 			}
@@ -703,7 +726,7 @@ namespace luzgpu
 		const int q = b & 1;
 		for (std::uint32_t i = blockIdx.x * kWaveBlock + threadIdx.x; i < n; i += gridDim.x * kWaveBlock)
 		{
-			const std::uint32_t path = w.qPath[q][i] & kPathMask;
+			const std::uint32_t path = w.pathOf(w.qPath[q][i]);
 			float4 r4 = w.radiance[path];
 			std::uint32_t flags = __float_as_uint(r4.w);
 			f3 acc = mk(r4.x, r4.y, r4.z);
@@ -1075,16 +1098,18 @@ namespace luzgpu
 				throw std::runtime_error("Unsupported: more than 62 bounces in the wavefront integrator");
 			}
 			const std::uint64_t pixels = j.pixelList ? j.listCount : static_cast<std::uint64_t>(j.params.width) * j.params.height;
-			if (pixels == 0 || pixels > kMaxPassPaths)
+			if (pixels == 0 || pixels > std::min<std::uint64_t>(kMaxPassPaths, 1ull << pathBitsFor(static_cast<std::uint32_t>(jobs.size()))))
 			{
-				throw std::runtime_error("Unsupported: image of more than 16M pixels in the wavefront integrator");
+				throw std::runtime_error("Unsupported: image of more pixels than a wavefront pass holds (2^27, fewer in large batches)");
 			}
 			maxBounces = std::max(maxBounces, j.params.maxBounces);
 			volumes = volumes || j.params.hasVolumes;
 			total += pixels * j.params.sampleCount;
 		}
-		const std::uint32_t capacity = static_cast<std::uint32_t>(std::min<std::uint64_t>(kMaxPassPaths, std::max<std::uint64_t>(total, 1)));
 		const std::uint32_t n = static_cast<std::uint32_t>(jobs.size());
+		const std::uint32_t pathBits = pathBitsFor(n);
+		const std::uint32_t capacity = static_cast<std::uint32_t>(std::min<std::uint64_t>(std::min<std::uint64_t>(kMaxPassPaths, 1ull << pathBits),
+			std::max<std::uint64_t>(total, 1)));
 		// pass plan: each pass gives every scene with samples left as many as the budget holds
 		std::vector<PassJob> plan;
 		std::vector<std::uint32_t> done(n, 0);
@@ -1117,6 +1142,7 @@ namespace luzgpu
 		}
 		const std::uint32_t passes = static_cast<std::uint32_t>(plan.size() / n);
 		st.ensure(capacity);
+		st.buf.pathBits = pathBits;
 		bool denoising = false;
 		for (const BatchJob& j : jobs) denoising = denoising || j.denoise != nullptr;
 		if (denoising)
