@@ -3,8 +3,14 @@
 // produces the same FP64 scene on both sides:
 //   sections and blank-line termination  src/io/SceneFile/SceneFile.cpp:256-321
 //   [settings] keys                       src/io/SceneFile/Settings.cpp:162-640
-//   material blocks                       src/io/SceneFile/Materials.cpp:26-70,760-870
+//   [spectra] + reflectance(...) colours  src/io/SceneFile/Parsing.cpp:136-174,292-368,
+//                                         src/core/ColorScience.cpp:222-321
+//   material blocks                       src/io/SceneFile/Materials.cpp:26-70,760-870,927-960
+//   [meshes]                              src/io/SceneFile/Meshes.cpp:4-74
 //   primitives / object blocks            src/io/SceneFile/Objects.cpp:357-406,800-880,1502-1620
+//   sphere / area / sphere-light blocks   src/io/SceneFile/Objects.cpp:420-660,880-1160
+//   volume blocks and volume presets      src/io/SceneFile/Objects.cpp:690-800,1224-1330,
+//                                         src/core/MeasuredMaterials.cpp:139-152,225-330
 //   OBJ faces (fan triangulation)         src/io/OBJReader.cpp:576-641
 //   degenerate-triangle removal           src/io/OBJReader.cpp:418-445, Mesh.cpp:360-385
 //   RGBE decode + sRGB->ACEScg            src/core/EnvironmentMap.cpp:140-150,376-454
@@ -24,6 +30,8 @@
 #include <optional>
 #include <sstream>
 #include <stdexcept>
+#include <utility>
+#include <vector>
 
 namespace luzgpu
 {
@@ -59,13 +67,19 @@ namespace luzgpu
 				+ 0.681 * cmfGaussian((w - 459.0) * (w < 459.0 ? 0.0385 : 0.0725));
 		}
 
-		// XYZ -> ACEScg (ColorManagement.cpp:139-143), then non-negative channels scaled to a
-		// maximum of 1 (normalizeVisibleColor, ColorScience.cpp:49-60)
-		dvec3	visibleFromXYZ(double x, double y, double z)
+		// XYZ -> ACEScg (ColorManagement.cpp:139-143, 272-275)
+		dvec3	acescgFromXYZ(double x, double y, double z)
 		{
-			dvec3 c{1.6410233797 * x + -0.3248032942 * y + -0.2364246952 * z,
+			return dvec3{1.6410233797 * x + -0.3248032942 * y + -0.2364246952 * z,
 				-0.6636628587 * x + 1.6153315917 * y + 0.0167563477 * z,
 				0.0117218943 * x + -0.0082844420 * y + 0.9883948585 * z};
+		}
+
+		// XYZ -> ACEScg, then non-negative channels scaled to a maximum of 1
+		// (normalizeVisibleColor, ColorScience.cpp:49-60)
+		dvec3	visibleFromXYZ(double x, double y, double z)
+		{
+			dvec3 c = acescgFromXYZ(x, y, z);
 			for (double& v : c) v = std::isfinite(v) ? std::max(0.0, v) : 0.0;
 			const double top = std::max(c[0], std::max(c[1], c[2]));
 			if (top <= 0.0 || !std::isfinite(top))
@@ -188,6 +202,163 @@ namespace luzgpu
 			return (std::pow((v + 0.055) / 1.055, 2.4));
 		}
 
+		// ColorScience::reflectanceCurve (ColorScience.cpp:222-267): the samples (nm, value),
+		// sorted and linearly interpolated, held flat past either end, are weighted every 5 nm
+		// over 360-830 nm by a 6504 K Planck illuminant and the CIE fits; the ACEScg colour of
+		// that sum is divided channel by channel by the illuminant's own, each ratio clamped
+		// to [0,1].
+		dvec3	reflectanceColor(std::vector<std::pair<double, double>> samples)
+		{
+			if (samples.size() < 2)
+			{
+				throw std::invalid_argument("Spectral reflectance curves require at least two samples.");
+			}
+			for (const auto& s : samples)
+			{
+				if (!std::isfinite(s.first) || s.first < 360.0 || s.first > 830.0)
+				{
+					throw std::invalid_argument("Spectral reflectance wavelengths must be finite and within 360-830 nm.");
+				}
+				if (!std::isfinite(s.second) || s.second < 0.0 || s.second > 1.0)
+				{
+					throw std::invalid_argument("Spectral reflectance values must be finite and in [0,1].");
+				}
+			}
+			std::sort(samples.begin(), samples.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+			for (std::size_t i = 1; i < samples.size(); i++)
+			{
+				if (samples[i].first <= samples[i - 1].first)
+				{
+					throw std::invalid_argument("Spectral reflectance wavelengths must be unique.");
+				}
+			}
+			auto at = [&](double w) -> double
+			{
+				if (w <= samples.front().first) return samples.front().second;
+				if (w >= samples.back().first) return samples.back().second;
+				std::size_t hi = 1;
+				while (samples[hi].first <= w) hi++;	// first sample strictly above w
+				const auto& a = samples[hi - 1];
+				const auto& b = samples[hi];
+				const double span = b.first - a.first;
+				const double t = span > 0.0 ? (w - a.first) / span : 0.0;
+				return (a.second * (1.0 - t) + b.second * t);
+			};
+			double x = 0.0, y = 0.0, z = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+			for (double w = 360.0; w <= 830.0; w += 5.0)
+			{
+				const double meters = w * 1.0e-9;
+				const double exponent = 1.438776877e-2 / (meters * 6504.0);
+				const double light = exponent > 700.0 ? 0.0 : 1.0 / (std::pow(meters, 5.0) * std::expm1(exponent));
+				const double r = at(w);
+				const double ex = light * cmfX(w), ey = light * cmfY(w), ez = light * cmfZ(w);
+				x += r * ex;
+				y += r * ey;
+				z += r * ez;
+				wx += ex;
+				wy += ey;
+				wz += ez;
+			}
+			const dvec3 c = acescgFromXYZ(x, y, z);
+			const dvec3 white = acescgFromXYZ(wx, wy, wz);
+			dvec3 out{};
+			for (int i = 0; i < 3; i++)
+			{
+				if (!(white[i] > 0.0) || !std::isfinite(white[i]))
+				{
+					throw std::invalid_argument("Spectral reflectance illuminant produced an invalid white point.");
+				}
+				const double v = c[i] / white[i];
+				out[i] = (!std::isfinite(v) || v <= 0.0) ? 0.0 : (v >= 1.0 ? 1.0 : v);
+			}
+			return (out);
+		}
+
+		// One "nm value" line of a spectrum ('#' starts a comment, ',' and ';' read as
+		// spaces; Parsing.cpp:322-366, ColorScience.cpp:279-312). False for an empty line;
+		// throws when the line is not exactly two numbers.
+		bool	spectrumSample(std::string line, std::pair<double, double>& sample, const std::string& where)
+		{
+			const std::size_t hash = line.find('#');
+			if (hash != std::string::npos) line.erase(hash);
+			std::replace(line.begin(), line.end(), ',', ' ');
+			std::replace(line.begin(), line.end(), ';', ' ');
+			line = trim(line);
+			if (line.empty()) return (false);
+			std::istringstream in(line);
+			std::string trailing;
+			if (!(in >> sample.first >> sample.second) || (in >> trailing))
+			{
+				throw std::runtime_error("Invalid spectral reflectance sample in " + where + ". Use wavelength_nm,value.");
+			}
+			return (true);
+		}
+
+		// ColorScience::loadReflectanceCurve (ColorScience.cpp:269-321): a text file of samples.
+		dvec3	reflectanceFileColor(const std::string& path)
+		{
+			std::ifstream in(path);
+			if (!in)
+			{
+				throw std::runtime_error("Spectral reflectance curve could not be opened: " + path);
+			}
+			std::vector<std::pair<double, double>> samples;
+			std::string line;
+			std::size_t number = 0;
+			while (std::getline(in, line))
+			{
+				number++;
+				std::pair<double, double> s;
+				if (spectrumSample(line, s, path + ":" + std::to_string(number))) samples.push_back(s);
+			}
+			try
+			{
+				return (reflectanceColor(samples));
+			}
+			catch (const std::exception& e)
+			{
+				throw std::runtime_error("Invalid spectral reflectance curve '" + path + "': " + e.what());
+			}
+		}
+
+		// What a colour value may refer to beyond its own text: the [spectra] section read so
+		// far and the scene file's folder (Parsing.cpp:161-174). Set while a file is read.
+		struct	ColorContext
+		{
+			std::map<std::string, dvec3>	spectra;
+			std::filesystem::path	base;
+		};
+		thread_local const ColorContext*	tColorContext = nullptr;
+
+		std::string	resolvePath(const std::filesystem::path& base, const std::string& file);
+
+		// reflectance / reflectance_curve / spectrum / spectral(NAME or PATH), the name or path
+		// optionally quoted: a [spectra] entry of that name, else a curve file beside the scene
+		// (Parsing.cpp:136-174).
+		dvec3	spectralColor(const std::string& t, const std::string& function, const std::string& what)
+		{
+			const std::string prefix = function + "(";
+			if (t.back() != ')')
+			{
+				throw std::runtime_error("Invalid " + what + " color function.");
+			}
+			std::string a = trim(t.substr(prefix.size(), t.size() - prefix.size() - 1));
+			if (a.size() >= 2 && ((a.front() == '"' && a.back() == '"') || (a.front() == '\'' && a.back() == '\'')))
+			{
+				a = a.substr(1, a.size() - 2);
+			}
+			if (a.empty())
+			{
+				throw std::runtime_error("Invalid " + what + " spectral curve reference.");
+			}
+			if (tColorContext)
+			{
+				const auto it = tColorContext->spectra.find(a);
+				if (it != tColorContext->spectra.end()) return (it->second);
+			}
+			return (reflectanceFileColor(resolvePath(tColorContext ? tColorContext->base : std::filesystem::path(), a)));
+		}
+
 		// A colour value (Parsing.cpp:227-264): (r,g,b) and acescg(r,g,b) are ACEScg as given;
 		// srgb / display_srgb / display(r,g,b) are sRGB-encoded and linear_srgb / linearsrgb /
 		// lin_srgb(r,g,b) linear sRGB, both converted to ACEScg.
@@ -195,6 +366,10 @@ namespace luzgpu
 		{
 			const std::string t = trim(text);
 			const std::string lt = lower(t);
+			for (const char* name : {"reflectance", "reflectance_curve", "spectrum", "spectral"})
+			{
+				if (lt.rfind(std::string(name) + "(", 0) == 0) return (spectralColor(t, name, what));
+			}
 			for (const char* name : {"srgb", "display_srgb", "display"})
 			{
 				const std::string prefix = std::string(name) + "(";
@@ -264,8 +439,9 @@ namespace luzgpu
 			{
 				return (false);
 			}
+			// _parseNamedBlockHeader (Parsing.cpp:42-65): one word, no blanks or braces
 			name = trim(t.substr(keyword.size(), t.size() - keyword.size() - 1));
-			return (!name.empty());
+			return (!name.empty() && name.find_first_of(" \t{}") == std::string::npos);
 		}
 
 		std::string	resolvePath(const std::filesystem::path& base, const std::string& file)
@@ -339,6 +515,20 @@ namespace luzgpu
 			{"water", 1.333, 55.8, false, {}, {}},
 			{"diamond", 2.417, 55.3, false, {}, {}},
 			{"sapphire", 1.773, 72.2, false, {}, {}},
+		};
+
+		// MeasuredMaterials volume table (MeasuredMaterials.cpp:139-152): scattering and
+		// absorption coefficients (1/m) per RGB channel and the phase anisotropy.
+		struct	VolumePreset { const char* name; dvec3 sigmaS, sigmaA; double g; };
+
+		const VolumePreset	kVolumes[] = {
+			{"clearair", {3.8e-6, 13.5e-6, 33.1e-6}, {0.0, 0.0, 0.0}, 0.0},
+			{"air", {3.8e-6, 13.5e-6, 33.1e-6}, {0.0, 0.0, 0.0}, 0.0},
+			{"haze", {7.5e-4, 8.5e-4, 1.0e-3}, {1.0e-4, 1.0e-4, 1.2e-4}, 0.55},
+			{"mist", {0.025, 0.027, 0.030}, {0.001, 0.001, 0.001}, 0.72},
+			{"fog", {0.050, 0.052, 0.055}, {0.002, 0.002, 0.002}, 0.15},
+			{"smoke", {0.16, 0.15, 0.14}, {0.36, 0.34, 0.32}, 0.35},
+			{"cloud", {2.0, 2.0, 2.0}, {0.02, 0.02, 0.02}, 0.85},
 		};
 
 		// preset names compare lower-case without '_', '-' or spaces (MeasuredMaterials.cpp:36-51)
@@ -716,6 +906,117 @@ namespace luzgpu
 			return (true);
 		}
 
+		// A light block's physical quantity (PhysicalLightUnits, Objects.cpp:420-431): at most
+		// one per block.
+		struct	LightQuantity
+		{
+			enum Kind { NONE, RADIANCE, LUMINANCE, POWER, LUMENS, INTENSITY, CANDELA };
+			Kind	kind = NONE;
+			double	value = 0.0;
+		};
+
+		// parseSurfaceLightUnit / parseSphericalLightUnit (Objects.cpp:514-581): false when
+		// `key` names none of the block's quantities (intensities only where `spherical`).
+		bool	lightQuantity(LightQuantity& q, const std::string& key, const std::string& value, bool spherical)
+		{
+			LightQuantity::Kind k = LightQuantity::NONE;
+			if (key == "radiance" || key == "surface_radiance" || key == "surfaceradiance") k = LightQuantity::RADIANCE;
+			else if (key == "luminance" || key == "nits" || key == "cd_m2" || key == "cdm2") k = LightQuantity::LUMINANCE;
+			else if (key == "power" || key == "watts" || key == "radiant_power" || key == "radiantpower") k = LightQuantity::POWER;
+			else if (key == "lumens" || key == "lumen" || key == "lm" || key == "luminous_flux" || key == "luminousflux") k = LightQuantity::LUMENS;
+			else if (spherical && (key == "radiant_intensity" || key == "radiantintensity" || key == "w_sr" || key == "wsr")) k = LightQuantity::INTENSITY;
+			else if (spherical && (key == "candela" || key == "cd" || key == "luminous_intensity" || key == "luminousintensity")) k = LightQuantity::CANDELA;
+			else return (false);
+			if (q.kind != LightQuantity::NONE)
+			{
+				throw std::runtime_error("Light block defines multiple physical light quantities.");
+			}
+			q.kind = k;
+			q.value = std::stod(value);
+			return (true);
+		}
+
+		// surfaceLightEmission (Objects.cpp:603-630) over LightUnits.cpp:36-99: the colour
+		// scaled so that its luminance is the radiance the quantity gives an emitter of
+		// `area` square metres (luminous units divided by 683 lm/W).
+		dvec3	lightEmission(const dvec3& color, const LightQuantity& q, double area, const char* missing)
+		{
+			auto nonNegative = [](double v, const std::string& what) {
+				if (!std::isfinite(v) || v < 0.0) throw std::invalid_argument(what + " must be finite and non-negative.");
+			};
+			auto positiveArea = [](double a, const std::string& what) {
+				if (!std::isfinite(a) || a <= 0.0) throw std::invalid_argument(what + " must be finite and positive.");
+			};
+			double radiance = 0.0;
+			switch (q.kind)
+			{
+				case LightQuantity::RADIANCE:
+					radiance = q.value;
+					break;
+				case LightQuantity::LUMINANCE:
+					nonNegative(q.value, "Surface luminance");
+					radiance = q.value / 683.0;
+					break;
+				case LightQuantity::POWER:
+					nonNegative(q.value, "Radiant power");
+					positiveArea(area, "Emitter area");
+					radiance = q.value / (kPi * area);
+					break;
+				case LightQuantity::LUMENS:
+				{
+					nonNegative(q.value, "Luminous flux");
+					positiveArea(area, "Emitter area");
+					const double nits = q.value / (kPi * area);
+					nonNegative(nits, "Surface luminance");
+					radiance = nits / 683.0;
+					break;
+				}
+				case LightQuantity::INTENSITY:
+					nonNegative(q.value, "Radiant intensity");
+					positiveArea(area, "Emitter surface area");
+					radiance = 4.0 * q.value / area;
+					break;
+				case LightQuantity::CANDELA:
+				{
+					nonNegative(q.value, "Luminous intensity");
+					positiveArea(area, "Emitter surface area");
+					const double nits = 4.0 * q.value / area;
+					nonNegative(nits, "Surface luminance");
+					radiance = nits / 683.0;
+					break;
+				}
+				default:
+					throw std::runtime_error(missing);
+			}
+			nonNegative(radiance, "Surface radiance");
+			// scaleToLuminance (LightUnits.cpp:36-52)
+			nonNegative(color[0], "Light color red channel");
+			nonNegative(color[1], "Light color green channel");
+			nonNegative(color[2], "Light color blue channel");
+			if (radiance == 0.0)
+			{
+				return dvec3{0.0, 0.0, 0.0};
+			}
+			const double lum = luminance(color);
+			if (!std::isfinite(lum) || lum <= 0.0)
+			{
+				throw std::invalid_argument("Light color must have positive luminance for physical units.");
+			}
+			const double f = radiance / lum;
+			return dvec3{color[0] * f, color[1] * f, color[2] * f};
+		}
+
+		// "visible=0|1" of sphere blocks and sphere lights (Objects.cpp:924-932, 1127-1135)
+		bool	visibleFlag(const std::string& value, const std::string& what)
+		{
+			int v = 0;
+			if (std::sscanf(value.c_str(), "%d", &v) != 1 || (v != 0 && v != 1))
+			{
+				throw std::runtime_error(what + " visible must be 0 or 1.");
+			}
+			return (v != 0);
+		}
+
 		struct	Reader
 		{
 			std::ifstream	in;
@@ -723,6 +1024,8 @@ namespace luzgpu
 			Scene	scene;
 			std::map<std::string, std::uint32_t>	materialIndex;
 			std::map<std::string, std::uint32_t>	meshByPath;
+			std::map<std::string, std::string>	meshPaths;	// [meshes]: name -> resolved file
+			ColorContext	colors;
 
 			bool	next(std::string& line)
 			{
@@ -1352,7 +1655,7 @@ namespace luzgpu
 			void	objectBlock(const std::string& name)
 			{
 				std::string line;
-				std::string file;
+				std::string file, meshName;
 				dvec3 position{}, rotation{}, scale = {1.0, 1.0, 1.0};
 				std::int64_t mat = -1;
 				while (next(line))
@@ -1364,9 +1667,27 @@ namespace luzgpu
 					}
 					if (t == "}")
 					{
-						if (file.empty())
+						if (!file.empty() && !meshName.empty())
+						{
+							throw std::runtime_error("Object block '" + name + "' defines both mesh and file.");
+						}
+						if (file.empty() && meshName.empty())
 						{
 							throw std::runtime_error("Object block '" + name + "' ended before mesh or file was defined.");
+						}
+						std::string path;
+						if (!meshName.empty())
+						{
+							const auto it = meshPaths.find(meshName);
+							if (it == meshPaths.end())
+							{
+								throw std::runtime_error("Unknown mesh name: " + meshName);
+							}
+							path = it->second;
+						}
+						else
+						{
+							path = resolvePath(base, file);
 						}
 						if (mat < 0)
 						{
@@ -1374,7 +1695,7 @@ namespace luzgpu
 							mat = static_cast<std::int64_t>(scene.materials.size());
 							scene.materials.push_back(m);
 						}
-						addInstance(resolvePath(base, file), position, rotation, scale, static_cast<std::uint32_t>(mat));
+						addInstance(path, position, rotation, scale, static_cast<std::uint32_t>(mat));
 						return;
 					}
 					std::string key, value;
@@ -1382,7 +1703,8 @@ namespace luzgpu
 					{
 						throw std::runtime_error("Invalid object property: " + t);
 					}
-					if (key == "file" || key == "path") file = value;
+					if (key == "mesh") meshName = value;
+					else if (key == "file" || key == "path") file = value;
 					else if (key == "position") position = parseTriple(value, "object position");
 					else if (key == "rotation") rotation = parseTriple(value, "object rotation");
 					else if (key == "scale") scale = parseTriple(value, "object scale");
@@ -1414,12 +1736,53 @@ namespace luzgpu
 				return (parts);
 			}
 
+			// readPrimitiveMaterial (Objects.cpp:154-186): `material=NAME`, or `material[` and
+			// then one direct material line (lambertian=(r,g,b) ...) closed by `]` on its own
+			// line (_readMaterialSubSection, Materials.cpp:927-960).
 			std::uint32_t	primitiveMaterial(const std::string& part, const std::string& line)
 			{
-				std::string key, value;
-				if (!splitAssignment(part, key, value) || key != "material")
+				const std::string p = trim(part);
+				if (lower(p).rfind("material", 0) == 0 && trim(p.substr(8)) == "[")
 				{
-					throw std::runtime_error("Unsupported primitive material syntax for the device path (use material=NAME): " + line);
+					std::string l;
+					std::optional<Material> m;
+					while (next(l))
+					{
+						const std::string t = trim(l);
+						if (t.empty() || t[0] == '#')
+						{
+							continue;
+						}
+						if (t == "]")
+						{
+							if (!m)
+							{
+								throw std::runtime_error("Material block ended before a material was defined.");
+							}
+							scene.materials.push_back(*m);
+							return (static_cast<std::uint32_t>(scene.materials.size() - 1));
+						}
+						Material parsed;
+						if (!directMaterialLine(t, parsed))
+						{
+							throw std::runtime_error("Unknown material line: " + l);
+						}
+						if (m)
+						{
+							throw std::runtime_error("Material block defines more than one material.");
+						}
+						m = parsed;
+					}
+					throw std::runtime_error("Material block is missing a closing ].");
+				}
+				std::string key, value;
+				if (!splitAssignment(p, key, value) || key != "material")
+				{
+					throw std::runtime_error("Invalid primitive material. Use material[ for an inline material block or material=NAME for a named material: " + line);
+				}
+				if (value.empty() || value.find_first_of(" \t{}[]") != std::string::npos)
+				{
+					throw std::runtime_error("Invalid primitive material name: " + line);
 				}
 				return (material(value));
 			}
@@ -1442,6 +1805,10 @@ namespace luzgpu
 				dvec3 color = {0.72, 0.78, 0.86};
 				double g = 0.0;
 				std::int64_t mat = -1;
+				bool hasDensity = false, hasColor = false, hasAnisotropy = false;
+				std::string preset;
+				double densityScale = 1.0;
+				std::optional<dvec3> sigmaS, sigmaA;
 				while (next(line))
 				{
 					const std::string t = trim(line);
@@ -1451,6 +1818,56 @@ namespace luzgpu
 					}
 					if (t == "}")
 					{
+						// applyMeasuredVolumeData (Objects.cpp:746-800): a preset or explicit
+						// sigma_s / sigma_a (scaled by density_scale) set the density to the
+						// luminance of the extinction and the colour to sigma_s / density
+						// (MeasuredMaterials.cpp:306-330).
+						auto measured = [&](dvec3 s, dvec3 a) {
+							if (!std::isfinite(densityScale) || densityScale <= 0.0)
+							{
+								throw std::runtime_error("Volume '" + name + "' density_scale must be finite and positive.");
+							}
+							for (int i = 0; i < 3; i++)
+							{
+								s[i] = s[i] * densityScale;
+								a[i] = a[i] * densityScale;
+							}
+							for (int i = 0; i < 3; i++)
+							{
+								if (!std::isfinite(s[i]) || s[i] < 0.0 || !std::isfinite(a[i]) || a[i] < 0.0)
+								{
+									throw std::invalid_argument("Volume scattering and absorption coefficients must be finite and non-negative.");
+								}
+							}
+							density = luminance(dvec3{s[0] + a[0], s[1] + a[1], s[2] + a[2]});
+							if (!std::isfinite(density) || density <= 0.0)
+							{
+								throw std::invalid_argument("Measured volume coefficients produce zero extinction.");
+							}
+							for (int i = 0; i < 3; i++) color[i] = std::min(1.0, s[i] / density);
+						};
+						if (!preset.empty())
+						{
+							if (sigmaS || sigmaA)
+							{
+								throw std::runtime_error("Volume '" + name + "' defines both preset and explicit sigma coefficients.");
+							}
+							if (hasDensity || hasColor || hasAnisotropy)
+							{
+								throw std::runtime_error("Volume '" + name + "' preset cannot be combined with density, color, or anisotropy overrides. Use density_scale.");
+							}
+							const VolumePreset& vp = findPreset(kVolumes, preset, "volume");
+							g = vp.g;
+							measured(vp.sigmaS, vp.sigmaA);
+						}
+						else if (sigmaS || sigmaA)
+						{
+							if (hasDensity || hasColor)
+							{
+								throw std::runtime_error("Volume '" + name + "' sigma coefficients cannot be combined with density or color.");
+							}
+							measured(sigmaS.value_or(dvec3{0.0, 0.0, 0.0}), sigmaA.value_or(dvec3{0.0, 0.0, 0.0}));
+						}
 						auto positive = [&](double x, const std::string& what) {
 							if (!std::isfinite(x) || x <= 0.0)
 							{
@@ -1512,10 +1929,28 @@ namespace luzgpu
 					else if (key == "height") v.size[1] = parseDouble(value, key);
 					else if (key == "depth") v.size[2] = parseDouble(value, key);
 					else if (key == "radius") v.radius = parseDouble(value, key);
-					else if (key == "density" || key == "extinction" || key == "sigma_t") density = parseDouble(value, key);
+					else if (key == "density" || key == "extinction" || key == "sigma_t")
+					{
+						density = parseDouble(value, key);
+						hasDensity = true;
+					}
 					else if (key == "color" || key == "albedo" || key == "scatteringcolor" || key == "scattering_color")
+					{
 						color = parseColor(value, "volume color");
-					else if (key == "anisotropy" || key == "g") g = parseDouble(value, key);
+						hasColor = true;
+					}
+					else if (key == "anisotropy" || key == "g")
+					{
+						g = parseDouble(value, key);
+						hasAnisotropy = true;
+					}
+					else if (key == "preset" || key == "volume_preset" || key == "volumepreset" || key == "medium") preset = value;
+					else if (key == "density_scale" || key == "densityscale" || key == "coefficient_scale" || key == "coefficientscale")
+						densityScale = parseDouble(value, key);
+					else if (key == "sigma_s" || key == "sigmas" || key == "scattering" || key == "scattering_coefficient" || key == "scatteringcoefficient")
+						sigmaS = parseColor(value, key);
+					else if (key == "sigma_a" || key == "sigmaa" || key == "absorption" || key == "absorption_coefficient" || key == "absorptioncoefficient")
+						sigmaA = parseColor(value, key);
 					else if (key == "material")
 					{
 						mat = material(value);
@@ -1600,20 +2035,207 @@ namespace luzgpu
 				throw std::runtime_error("Directional light '" + name + "' is missing a closing }.");
 			}
 
-			bool	lightOrVolumeBlock(const std::string& t)
+			// sphere NAME { ... } (Objects.cpp:880-951)
+			void	sphereBlock(const std::string& name)
+			{
+				std::string line;
+				Sphere sp;
+				sp.radius = 1.0;
+				std::int64_t mat = -1;
+				bool visible = true;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "}")
+					{
+						if (!std::isfinite(sp.radius) || sp.radius <= 0.0)
+						{
+							throw std::runtime_error("Sphere object '" + name + "' radius must be finite and positive.");
+						}
+						if (!visible)
+						{
+							throw std::runtime_error("Unsupported for the device path: invisible sphere (visible=0)");
+						}
+						if (mat < 0)
+						{
+							scene.materials.push_back(Material{});	// SphereBlock default: Lambertian(0.6)
+							mat = static_cast<std::int64_t>(scene.materials.size() - 1);
+						}
+						scene.spheres.push_back(sp);
+						addHittable(HIT_SPHERE, scene.spheres.size() - 1, static_cast<std::uint32_t>(mat));
+						return;
+					}
+					std::string key, value;
+					if (!splitAssignment(t, key, value) || key.empty() || value.empty())
+					{
+						throw std::runtime_error("Invalid sphere property: " + t);
+					}
+					if (key == "position" || key == "center") sp.position = parseTriple(value, key);
+					else if (key == "radius") sp.radius = std::stod(value);
+					else if (key == "material") mat = material(value);
+					else if (key == "visible") visible = visibleFlag(value, "Sphere object '" + name + "'");
+					else if (key == "uv_projection" || key == "uvprojection" || key == "texture_projection" || key == "projection")
+					{
+						// parseSphereUVProjection (Objects.cpp:662-686): it places textures,
+						// which the device path does not take, so it is only checked
+						const std::string v = lower(value);
+						if (v != "latlong" && v != "lat_long" && v != "longitude_latitude" && v != "equirectangular"
+							&& v != "cube_cross" && v != "cubecross" && v != "cross" && v != "cube")
+						{
+							throw std::runtime_error("Unknown sphere uv projection: " + value);
+						}
+					}
+					else throw std::runtime_error("Unknown sphere property: " + t);
+				}
+				throw std::runtime_error("Sphere object '" + name + "' is missing a closing }.");
+			}
+
+			// area_light NAME { ... } (Objects.cpp:955-1048): an emissive Rectangle facing
+			// `normal`, its radiance from the block's physical quantity over w*h scene units^2.
+			void	areaLightBlock(const std::string& name)
+			{
+				std::string line;
+				Rectangle r;
+				r.orientation = {0.0, -1.0, 0.0};
+				dvec3 color = {1.0, 1.0, 1.0};
+				LightQuantity q;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "}")
+					{
+						if (r.width <= 0.0 || r.height <= 0.0)
+						{
+							throw std::runtime_error("Area light '" + name + "' size must be positive.");
+						}
+						if (dot(r.orientation, r.orientation) <= 0.0)
+						{
+							throw std::runtime_error("Area light '" + name + "' normal must be non-zero.");
+						}
+						const double mpu = scene.settings.metersPerUnit;
+						Material m;
+						m.kind = MAT_EMISSIVE;
+						m.color = lightEmission(color, q, r.width * r.height * mpu * mpu,
+							"Surface light must define radiance, luminance, power, or lumens.");
+						scene.materials.push_back(m);
+						scene.rectangles.push_back(r);
+						addHittable(HIT_RECTANGLE, scene.rectangles.size() - 1, static_cast<std::uint32_t>(scene.materials.size() - 1));
+						return;
+					}
+					std::string key, value;
+					if (!splitAssignment(t, key, value) || key.empty() || value.empty())
+					{
+						throw std::runtime_error("Invalid area light property: " + t);
+					}
+					if (key == "position") r.position = parseTriple(value, key);
+					else if (key == "normal" || key == "direction") r.orientation = parseTriple(value, key);
+					else if (key == "size")
+					{
+						// parseVector2Value (Objects.cpp:200-212)
+						if (std::sscanf(trim(value).c_str(), "(%lf,%lf)", &r.width, &r.height) != 2)
+						{
+							throw std::runtime_error("Invalid size value. Use size=(x,y).");
+						}
+					}
+					else if (key == "width") r.width = std::stod(value);
+					else if (key == "height") r.height = std::stod(value);
+					else if (key == "color") color = parseColor(value, key);
+					else if (!lightQuantity(q, key, value, false)) throw std::runtime_error("Unknown area light property: " + t);
+				}
+				throw std::runtime_error("Area light '" + name + "' is missing a closing }.");
+			}
+
+			// sphere_light / point_light NAME { ... } (Objects.cpp:1050-1160): an emissive
+			// Sphere, its radiance from the block's quantity over its 4 pi r^2 surface.
+			void	sphereLightBlock(const std::string& name)
+			{
+				std::string line;
+				Sphere sp;
+				sp.radius = 0.1;
+				dvec3 color = {1.0, 1.0, 1.0};
+				LightQuantity q;
+				bool visible = true;
+				bool ies = false;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty() || t[0] == '#')
+					{
+						continue;
+					}
+					if (t == "}")
+					{
+						if (sp.radius <= 0.0)
+						{
+							throw std::runtime_error("Sphere light '" + name + "' radius must be positive.");
+						}
+						if (ies)
+						{
+							throw std::runtime_error("Unsupported for the device path: IES light profile");
+						}
+						if (!visible)
+						{
+							throw std::runtime_error("Unsupported for the device path: invisible sphere light (visible=0)");
+						}
+						const double mpu = scene.settings.metersPerUnit;
+						Material m;
+						m.kind = MAT_EMISSIVE;
+						m.color = lightEmission(color, q, 4.0 * kPi * sp.radius * sp.radius * mpu * mpu,
+							"Sphere light must define radiance, luminance, power, lumens, candela, radiant_intensity, or ies.");
+						scene.materials.push_back(m);
+						scene.spheres.push_back(sp);
+						addHittable(HIT_SPHERE, scene.spheres.size() - 1, static_cast<std::uint32_t>(scene.materials.size() - 1));
+						return;
+					}
+					std::string key, value;
+					if (!splitAssignment(t, key, value) || key.empty() || value.empty())
+					{
+						throw std::runtime_error("Invalid sphere light property: " + t);
+					}
+					if (key == "position") sp.position = parseTriple(value, key);
+					else if (key == "radius") sp.radius = std::stod(value);
+					else if (key == "color") color = parseColor(value, key);
+					else if (lightQuantity(q, key, value, true)) continue;
+					else if (key == "ies" || key == "ies_profile" || key == "iesprofile" || key == "profile") ies = true;
+					else if (key == "ies_direction" || key == "iesdirection" || key == "profile_direction" || key == "profiledirection")
+						(void)parseTriple(value, key);
+					else if (key == "ies_rotation" || key == "iesrotation" || key == "profile_rotation" || key == "profilerotation")
+						(void)std::stod(value);
+					else if (key == "visible") visible = visibleFlag(value, "Sphere light '" + name + "'");
+					else throw std::runtime_error("Unknown sphere light property: " + t);
+				}
+				throw std::runtime_error("Sphere light '" + name + "' is missing a closing }.");
+			}
+
+			// addSceneObjectOrLightBlock (Objects.cpp:1338-1391): the named blocks that may
+			// stand in [scene] or inside objects{ }.
+			bool	namedSceneBlock(const std::string& t)
 			{
 				std::string name;
-				if (namedBlock(t, "volume", name))
+				if (namedBlock(t, "object", name)) objectBlock(name);
+				else if (namedBlock(t, "sphere", name)) sphereBlock(name);
+				else if (namedBlock(t, "area_light", name)) areaLightBlock(name);
+				else if (namedBlock(t, "directional_light", name)) directionalBlock(name);
+				else if (namedBlock(t, "volume", name)) volumeBlock(name);
+				else if (namedBlock(t, "sphere_light", name) || namedBlock(t, "point_light", name)) sphereLightBlock(name);
+				else
 				{
-					volumeBlock(name);
-					return (true);
+					const std::string lt = lower(t);
+					for (const char* k : {"object ", "sphere ", "area_light ", "directional_light ", "volume ", "sphere_light ", "point_light "})
+					{
+						if (lt.rfind(k, 0) == 0) throw std::runtime_error("Invalid scene block header: " + t);
+					}
+					return (false);
 				}
-				if (namedBlock(t, "directional_light", name))
-				{
-					directionalBlock(name);
-					return (true);
-				}
-				return (false);
+				return (true);
 			}
 
 			void	objects()
@@ -1631,10 +2253,8 @@ namespace luzgpu
 						return;
 					}
 					const std::string lt = lower(t);
-					std::string name;
-					if (namedBlock(t, "object", name))
+					if (namedSceneBlock(t))
 					{
-						objectBlock(name);
 						continue;
 					}
 					if (lt.rfind("triangle=", 0) == 0)
@@ -1666,10 +2286,6 @@ namespace luzgpu
 						r.height = parseDouble(p[3], "rectangle height");
 						scene.rectangles.push_back(r);
 						scene.hittables.push_back(Hittable{HIT_RECTANGLE, static_cast<std::uint32_t>(scene.rectangles.size() - 1), primitiveMaterial(p[4], line)});
-						continue;
-					}
-					if (lightOrVolumeBlock(t))
-					{
 						continue;
 					}
 					if (lt.rfind("sphere=", 0) == 0)
@@ -1763,15 +2379,11 @@ namespace luzgpu
 					{
 						cameraBlock();
 					}
-					else if (namedBlock(t, "object", name))
-					{
-						objectBlock(name);
-					}
 					else if (lower(t) == "objects{" || lower(t) == "objects {")
 					{
 						objects();
 					}
-					else if (lightOrVolumeBlock(t))
+					else if (namedSceneBlock(t))
 					{
 					}
 					else
@@ -1781,8 +2393,130 @@ namespace luzgpu
 				}
 			}
 
+			// [spectra] (Parsing.cpp:292-368): `reflectance NAME {` blocks of "nm value" lines,
+			// up to the section's first empty line; each block becomes the colour
+			// reflectance(NAME) stands for.
+			void	spectra()
+			{
+				std::string line;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty())
+					{
+						return;
+					}
+					if (t[0] == '#')
+					{
+						continue;
+					}
+					std::string name;
+					if (!namedBlock(t, "reflectance", name))
+					{
+						throw std::runtime_error("Invalid spectrum block header: " + line);
+					}
+					if (colors.spectra.count(name) != 0)
+					{
+						throw std::runtime_error("Duplicate spectrum name: " + name);
+					}
+					std::vector<std::pair<double, double>> samples;
+					bool closed = false;
+					while (!closed && next(line))
+					{
+						std::string clean = line.substr(0, line.find('#'));
+						std::replace(clean.begin(), clean.end(), ',', ' ');
+						std::replace(clean.begin(), clean.end(), ';', ' ');
+						if (trim(clean) == "}")
+						{
+							try
+							{
+								colors.spectra[name] = reflectanceColor(samples);
+							}
+							catch (const std::exception& e)
+							{
+								throw std::runtime_error("Invalid spectrum '" + name + "': " + e.what());
+							}
+							closed = true;
+							continue;
+						}
+						std::pair<double, double> sample;
+						if (spectrumSample(line, sample, "'" + name + "'")) samples.push_back(sample);
+					}
+					if (!closed)
+					{
+						throw std::runtime_error("Spectrum block '" + name + "' is missing a closing }.");
+					}
+				}
+			}
+
+			// [meshes] (Meshes.cpp:4-74): `mesh NAME { file=PATH }` blocks naming OBJ files
+			// that object blocks refer to with mesh=NAME.
+			void	meshes()
+			{
+				std::string line;
+				while (next(line))
+				{
+					const std::string t = trim(line);
+					if (t.empty())
+					{
+						return;
+					}
+					if (t[0] == '#')
+					{
+						continue;
+					}
+					std::string name;
+					if (!namedBlock(t, "mesh", name))
+					{
+						throw std::runtime_error("Invalid mesh block header: " + line);
+					}
+					if (meshPaths.count(name) != 0)
+					{
+						throw std::runtime_error("Duplicate mesh name: " + name);
+					}
+					std::string file;
+					bool closed = false;
+					while (!closed && next(line))
+					{
+						const std::string b = trim(line);
+						if (b.empty() || b[0] == '#')
+						{
+							continue;
+						}
+						if (b == "}")
+						{
+							if (file.empty())
+							{
+								throw std::runtime_error("Mesh block '" + name + "' ended before file was defined.");
+							}
+							meshPaths[name] = resolvePath(base, file);
+							closed = true;
+							continue;
+						}
+						std::string key, value;
+						if (!splitAssignment(b, key, value) || key.empty() || value.empty())
+						{
+							throw std::runtime_error("Invalid mesh property: " + b);
+						}
+						if (key == "file" || key == "path") file = value;
+						else throw std::runtime_error("Unknown mesh property: " + b);
+					}
+					if (!closed)
+					{
+						throw std::runtime_error("Mesh block '" + name + "' is missing a closing }.");
+					}
+				}
+			}
+
 			void	run()
 			{
+				colors.base = base;
+				struct Scope
+				{
+					const ColorContext* saved;
+					explicit Scope(const ColorContext* c) : saved(tColorContext) { tColorContext = c; }
+					~Scope() { tColorContext = saved; }
+				} scope(&colors);
 				std::string line;
 				while (next(line))
 				{
@@ -1791,9 +2525,12 @@ namespace luzgpu
 					{
 						continue;
 					}
-					if (t == "[settings]") settings();
-					else if (t == "[materials]") materials();
-					else if (t == "[scene]") sceneSection();
+					const std::string lt = lower(t);
+					if (lt.rfind("[settings]", 0) == 0) settings();
+					else if (lt.rfind("[spectra]", 0) == 0) spectra();
+					else if (lt.rfind("[materials]", 0) == 0) materials();
+					else if (lt.rfind("[meshes]", 0) == 0) meshes();
+					else if (lt.rfind("[scene]", 0) == 0) sceneSection();
 					else throw std::runtime_error("Unsupported section for the device path: " + t);
 				}
 			}

@@ -54,6 +54,7 @@ namespace luzgpu_glue
 		static double	height(const Rectangle& r) { return r._height; }
 		static bool	explicitAxes(const Rectangle& r) { return r._hasAxes; }
 		static const Transform&	transform(const Cube& c) { return c._transform; }
+		static bool	iesProfiled(const Sphere& s) { return static_cast<bool>(s._iesProfile); }
 		static void	size(const Cube& c, double out[3]) { out[0] = c._width; out[1] = c._height; out[2] = c._depth; }
 		static double	y(const Plane& p) { return p._y; }
 		static const Vector3&	orientation(const Plane& p) { return p._orientation; }
@@ -243,6 +244,14 @@ namespace luzgpu_glue
 				}
 				else if (const auto* sp = dynamic_cast<const Sphere*>(h))
 				{
+					if (!sp->isVisible())
+					{
+						throw std::runtime_error("Unsupported for the device path: invisible sphere (visible=0)");
+					}
+					if (Access::iesProfiled(*sp))
+					{
+						throw std::runtime_error("Unsupported for the device path: IES light profile");
+					}
 					d.kind = LUZGPU_HIT_SPHERE;
 					put(sp->getPosition(), d.a);
 					d.radius = sp->getRadius();

@@ -386,6 +386,74 @@ def batch_fixtures(tmp):
     print("batch fixtures", picks, [wls[i].triangles for i in picks])
 
 
+REF_EXAMPLES = "/root/reference/examples/scenes"
+EXAMPLE_DIR = os.path.join(HERE, "scenes")
+# the reference's example scenes rendered for the fixtures: name -> (width, height, spp)
+EXAMPLE_GOLDEN = {
+    "cornell": (128, 128, 64),
+    "suzanne": (150, 100, 32),
+    "rtiow": (160, 90, 32),
+    "volumetric_spotlight_fog": (160, 90, 64),
+    "volumetric_gray_sphere_white_bg": (96, 96, 64),
+}
+
+
+def example_scene_path(name):
+    width, height, spp = EXAMPLE_GOLDEN[name]
+    return os.path.join(EXAMPLE_DIR, f"{name}_{width}x{height}_{spp}spp.luz")
+
+
+def reduced_example(name):
+    """Writes the reference's example scene ``name`` (examples/scenes) as a fixture input: the
+    file as it stands, with its resolution and sample count cut to the fixture's, adaptive
+    sampling and denoising off (the reference harness renders with denoising off; on, both
+    it and the device draw each sample from one of two half-buffer streams) and mesh files
+    pointed at copies under scenes/assets."""
+    width, height, spp = EXAMPLE_GOLDEN[name]
+    src = os.path.join(REF_EXAMPLES, f"{name}.luz")
+    os.makedirs(os.path.join(EXAMPLE_DIR, "assets"), exist_ok=True)
+    lines = []
+    for line in open(src).read().splitlines():
+        key, _, value = line.partition("=")
+        key = key.strip().lower()
+        if key == "[settings]":
+            lines += [line, "adaptive=0", "denoise=0"]
+            continue
+        if key in ("adaptive", "denoise"):
+            continue
+        if key == "resolution":
+            line = f"resolution={width},{height}"
+        elif key == "samples":
+            line = f"samples={spp}"
+        elif key in ("file", "path") and value.strip().lower().endswith(".obj"):
+            asset = os.path.normpath(os.path.join(REF_EXAMPLES, value.strip()))
+            dst = os.path.join(EXAMPLE_DIR, "assets", os.path.basename(asset))
+            with open(asset, "rb") as fin, open(dst, "wb") as fout:
+                fout.write(fin.read())
+            line = f"{key}=assets/{os.path.basename(asset)}"
+        lines.append(line)
+    path = example_scene_path(name)
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return path
+
+
+def example_fixtures(tmp, names=None):
+    """The reference's example scenes at reduced sizes: the reduced scene files (inputs),
+    the reference's FP64 render of each at seed 1, and its primary hits."""
+    for name in names or EXAMPLE_GOLDEN:
+        width, height, spp = EXAMPLE_GOLDEN[name]
+        path = reduced_example(name)
+        out = os.path.join(tmp, f"example_{name}.bin")
+        run("render", path, 1, os.cpu_count() or 8, out)
+        img, ms = read_image(out)
+        assert img.shape == (height, width, 3)
+        np.savez_compressed(os.path.join(HERE, f"example_{name}_{width}x{height}_{spp}spp_seed1.npz"),
+                            image=img.astype(np.float32), render_ms=ms)
+        primary_fixture(f"example_{name}", path, 1, tmp)
+        print("example", name, float(img.mean()), f"{ms:.0f} ms")
+
+
 def main():
     if not os.path.exists(LUZREF):
         sys.exit("build the reference harness first: make -C oracle ref")
@@ -427,6 +495,11 @@ def main():
             adaptive_fixtures(tmp)
         if "post" in only:
             post_fixture(tmp)
+        if "examples" in only:
+            example_fixtures(tmp)
+        for name in EXAMPLE_GOLDEN:
+            if name in only:
+                example_fixtures(tmp, [name])
         return
     sampler_fixture(tmp)
     small_scene_fixtures("primitives", workloads.write_primitives(os.path.join(tmp, "primitives")), tmp)
@@ -445,6 +518,7 @@ def main():
     zstat_fixtures(tmp)
     nfor_fixtures(tmp)
     adaptive_fixtures(tmp)
+    example_fixtures(tmp)
 
     # Cornell box, 300x300, 50 spp, depth 8, seed 1 (the plain CPU-runnable config)
     c = workloads.write_cornell(os.path.join(tmp, "cornell"))

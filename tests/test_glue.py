@@ -52,6 +52,20 @@ def test_flattened_scene_matches_file_route_and_reference(request, name):
     print(f"{name}: {r[('info', 'device')]}, {len(r.get(('blas', 'device'), []))} mesh BVHs equal")
 
 
+@pytest.mark.parametrize("example", __import__("test_examples").EXAMPLES, ids=lambda e: e[0])
+def test_flattened_example_scene_matches_file_route_and_reference(example):
+    """The reference's example scenes (spectra, area and point lights, sphere blocks, inline
+    materials, [meshes]) read by the reference's loader and flattened, against our reader."""
+    r = _run("topology", example[1])
+    assert r["device"] == r["reference"]
+    assert r["device"] == r["file"]
+    # the file route also keeps named materials that no object uses (suzanne's floor_matte);
+    # the flattened route holds only those the Scene's hittables refer to
+    strip = lambda info: [re.sub(r" materials \d+", "", x) for x in info]
+    assert strip(r[("info", "device")]) == strip(r[("info", "file")])
+    assert r.get(("blas", "device")) == r.get(("blas", "file"))
+
+
 def test_scene_built_in_code_matches_reference_topology():
     r = _run("inmemory", 64, 48)
     assert r["device"] == r["reference"]

@@ -88,7 +88,7 @@ def test_mesh_light_scene_matches_reference(meshlight):
 
 
 def test_volume_and_phase_validation(tmp_path):
-    """HenyeyGreenstein.cpp:10-18 and Objects.cpp:491-497 reject these inputs."""
+    """HenyeyGreenstein.cpp:10-18 and Objects.cpp:491-497, 746-800 reject these inputs."""
     import pytest
     from luz_amd import _lib
     head = ("[settings]\nresolution=8,8\nsamples=1\n\n[materials]\nmaterial m {{\ntype=hg\ncolor=(1,1,1)\nanisotropy={g}\n}}\n\n"
@@ -103,11 +103,14 @@ def test_volume_and_phase_validation(tmp_path):
     with pytest.raises(_lib.LuzGpuError) as e:
         SceneFile.read(None, str(bad_v))
     assert "density" in str(e.value)
+    # a measured preset loads; overriding its density does not (Objects.cpp:768-775)
     preset = tmp_path / "p.luz"
     preset.write_text(head.format(g="0.5", obj="volume v {\nshape=box\npreset=fog\n}"))
+    SceneFile.read(None, str(preset))
+    preset.write_text(head.format(g="0.5", obj="volume v {\nshape=box\npreset=fog\ndensity=1\n}"))
     with pytest.raises(_lib.LuzGpuError) as e:
         SceneFile.read(None, str(preset))
-    assert e.value.code == _lib.E_UNSUPPORTED
+    assert "density_scale" in str(e.value)
 
 
 def test_material_validation(tmp_path):
