@@ -249,6 +249,14 @@ typedef struct luzgpu_material_desc
 	double		glossy_color[3];     /* diffuse-glossy                                           */
 	double		glossy_weight;
 	double		metallic, transmission, clearcoat, clearcoat_roughness, sheen;  /* principled   */
+	/* principled subsurface (Principled.cpp:39-88): weight in [0,1] (0 = none), the Burley
+	 * radius per channel times its scale in metres, the tint, and 1 for the thin method */
+	double		subsurface;
+	double		subsurface_radius[3];
+	double		subsurface_scale;
+	double		subsurface_color[3];
+	uint32_t	subsurface_thin;
+	uint32_t	pad;
 } luzgpu_material_desc;
 
 /* One mesh (Mesh::_triangles, after its degenerate-triangle removal): vertices[9*t..] =

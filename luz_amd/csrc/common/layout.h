@@ -161,17 +161,22 @@ namespace luzgpu
 		float	clearcoatAlpha;
 		float	sheen;
 		float	diffuseScale;	// principled diffuseScale(), Principled.cpp:141-156
-		float	pad3[2];
+		float	subsurface;		// principled subsurface weight (the diffuse lobe keeps 1 - subsurface)
+		float	subsurfaceRadius;	// Burley diffusion length: max channel of radius * scale, metres
 		float	lobe[8];		// principled component weights / total (diffuse, subsurface, sheen,
 								// dielectric specular, metal, transmission, clearcoat), Principled.cpp:158-202
+		float	subsurfaceAlbedo[3];	// clamp01(base colour) * clamp01(subsurface colour)
+		float	pad4;
 	};
-	static_assert(sizeof(GpuMaterial) == 160, "GpuMaterial must be 160 bytes");
+	static_assert(sizeof(GpuMaterial) == 176, "GpuMaterial must be 176 bytes");
 
 	enum : std::uint32_t
 	{
 		GPU_MATF_CONDUCTOR = 1u,
 		GPU_MATF_ABSORPTION = 2u,
 		GPU_MATF_SMOOTH_TRANSMISSION = 4u,	// principled isSmoothFullTransmission (Principled.cpp:107-124)
+		GPU_MATF_THIN_SUBSURFACE = 8u,		// principled usesThinSubsurface (Principled.cpp:550-556)
+		GPU_MATF_BURLEY_SUBSURFACE = 16u,	// subsurface > 0 with the Burley method: exits are relocated
 	};
 
 	struct alignas(16) GpuLight
@@ -272,6 +277,8 @@ namespace luzgpu
 	constexpr int	kFeatAnalytic = 1;	// spheres, cubes, volumes, planes, directional and mesh-instance lights
 	constexpr int	kFeatRich = 2;		// glossy, diffuse-glossy, principled, isotropic, Henyey-Greenstein; the atmosphere sky
 	constexpr int	kFeatAll = 3;
+	constexpr int	kFeatSubsurface = 4;	// Burley subsurface exits (a closest-hit probe ray while shading)
+	constexpr int	kFeatEvery = 7;
 
 	// Per-scene traversal entry of a multi-scene launch: TLAS root and the unbounded
 	// (plane) list as base << 16 | count.

@@ -297,6 +297,55 @@ namespace luzgpu
 		f[9] = normalizedColorChannel(albedo.z);
 	}
 
+	// relocateSubsurfaceExit (ColorHelper.cpp:376-504): after a Burley subsurface scatter, a
+	// point a sampled radius away on the tangent disk is probed along -normal from above
+	// (a closest-hit ray); where the probe meets the same material within reach, the vertex
+	// moves there and its cosine direction is drawn again about the exit's normal
+	// (refreshSubsurfaceScatterAtExit, :394-426, the same DIM_BSDF_DIRECTION sample). `h`,
+	// `sf` and `sc` become the exit's on success and are left as they were otherwise.
+	template <int Feat, typename Stack>
+	__device__ __forceinline__ void	relocateSubsurface(const SceneView& s, const GpuParams& p, const GpuMaterial& m, HitInfo& h,
+		Surface& sf, Scatter& sc, const PixelSampler& ps, Stack& stack, float logU, std::uint32_t& closestRays)
+	{
+		// subsurfaceSampleRadiusMeters (:376-392): one of two exponentials, 1 : 3 the
+		// diffusion length, picked 1 : 3
+		const double diffusion = m.subsurfaceRadius;
+		if (!(diffusion > 0.0) || !isfinite(diffusion)) return;
+		const double pick = ps.sample1D(DIM_SUBSURFACE_PROFILE_COMPONENT);
+		float sx, sy;
+		ps.sample2D(DIM_SUBSURFACE_SPATIAL, sx, sy);
+		const double u = fmax(1e-12, fmin(1.0 - 1e-12, static_cast<double>(sy)));
+		const double radiusM = -(pick < 0.25 ? diffusion : 3.0 * diffusion) * log(1.0 - u);
+		const double mpu = p.metersPerUnit;
+		if (!(radiusM > 0.0) || !isfinite(radiusM) || !(mpu > 0.0) || !isfinite(mpu)) return;
+		const float radius = static_cast<float>(radiusM / mpu);
+		if (radius <= kTMinF) return;
+		f3 n = sf.normal;
+		if (len2(n) <= 1e-12f) n = sf.geometricNormal;
+		if (len2(n) <= 1e-12f) return;
+		n = normalize(n);
+		float sa, ca;
+		sincosf((2.0f * kPiF) * sx, &sa, &ca);
+		const f3 candidate = sf.position + Onb(n).local(ca * radius, sa * radius, 0.0f);
+		const float reach = fmaxf(radius + static_cast<float>(4.0 * diffusion / mpu), 16.0f * kTMinF);
+		const Ray probe = makeRay(candidate + n * reach, -n);
+		HitInfo eh;
+		closestRays++;
+		if (!traverse<false, false, Feat>(s, p, probe, kTMinF, INFINITY, eh, stack, nullptr, logU, kNoOrigin)) return;
+		if (eh.t <= kTMinF || eh.t > 2.0f * reach + kTMinF) return;
+		const Surface ex = surfaceAt<Feat>(s, probe, eh);
+		if (ex.material != sf.material) return;
+		const f3 dir = cosineHemisphereDir(ex.normal, ps);
+		const f3 view = safeNormalize(-sc.incident, ex.normal);
+		const float pdf = principledPdf(m, ex, view, dir);
+		const f3 f = principledEval(m, ex, view, dir);
+		if (pdf <= 0.0f || !isfinite(pdf) || maxc(f) <= 0.0f) return;
+		h = eh;
+		sf = ex;
+		sc.direction = dir;
+		sc.attenuation = f * (1.0f / pdf);
+	}
+
 	// The sampler stream of a sample: 0, or with denoising (k % 2) + 1, the two half-buffers
 	// (Threads.cpp:594-598).
 	__device__ __forceinline__ std::uint32_t	sampleStream(const GpuParams& p, std::uint32_t sample)
@@ -342,7 +391,7 @@ namespace luzgpu
 				if constexpr (Adj::enabled) adj->arrival(c, w);
 				return acc + c;
 			}
-			const Surface sf = surfaceAt<Feat>(s, ray, h);
+			Surface sf = surfaceAt<Feat>(s, ray, h);
 			from = originOf(s, h);
 			const GpuMaterial& m = s.materials[sf.material];
 			if (b == 0 && features)
@@ -370,6 +419,14 @@ namespace luzgpu
 					adj->arrival(inc, w);
 				}
 				return acc + inc;
+			}
+			if constexpr ((Feat & kFeatSubsurface) != 0)
+			{
+				if (sc.subsurfaceExit)
+				{
+					relocateSubsurface<Feat>(s, p, m, h, sf, sc, ps, stack, logU, closestRays);
+					from = originOf(s, h);
+				}
 			}
 			f3 att = sc.attenuation;
 			if (!sf.frontFace && sc.absorbing)

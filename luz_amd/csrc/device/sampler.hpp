@@ -29,6 +29,11 @@ namespace luzgpu
 		DIM_ENVIRONMENT_SELECTION = 12,
 		DIM_ENVIRONMENT_POINT = 13,
 		DIM_ENVIRONMENT_STRATEGY = 14,
+		DIM_LIGHT_EMISSION_SIDE = 15,
+		DIM_LIGHT_EMISSION_DIRECTION = 16,
+		DIM_SUBSURFACE_SPATIAL = 17,
+		DIM_SUBSURFACE_PROFILE_COMPONENT = 18,
+		DIM_SUBSURFACE_THIN_SIDE = 19,
 	};
 
 	__host__ __device__ inline std::uint64_t	splitMix64(std::uint64_t v)

@@ -336,13 +336,14 @@ namespace luzgpu
 		float	g;			// Henyey-Greenstein anisotropy
 		bool	absorbing;	// dielectric medium absorption
 		bool	mirrorFallback;	// rough metal: the GGX sample reflected below the horizon
+		bool	subsurfaceExit;	// principled Burley subsurface lobe sampled: the exit is relocated
 	};
 
 	// materialCanTransmitThroughGeometry (ColorHelper.cpp:327-355)
 	__device__ __forceinline__ bool	canTransmit(const GpuMaterial& m)
 	{
 		return m.kind == GPU_MAT_DIELECTRIC || m.kind == GPU_MAT_ISOTROPIC || m.kind == GPU_MAT_HG
-			|| (m.kind == GPU_MAT_PRINCIPLED && m.transmission > 0.0f);
+			|| (m.kind == GPU_MAT_PRINCIPLED && (m.transmission > 0.0f || (m.flags & GPU_MATF_THIN_SUBSURFACE) != 0));
 	}
 
 	// Henyey-Greenstein pdf of scatterPDFValue (ColorHelper.cpp:56-77)
@@ -402,14 +403,19 @@ namespace luzgpu
 		return (!isfinite(f) || f <= 0.0f) ? mk(0, 0, 0) : c * f;
 	}
 
-	// Principled::evaluateBSDFCos without subsurface (Principled.cpp:763-853)
+	// Principled::evaluateBSDFCos (Principled.cpp:763-853)
 	__device__ __forceinline__ f3	principledEval(const GpuMaterial& m, const Surface& sf, f3 view, f3 l)
 	{
 		const f3 base = clampColor01(ld3(m.color));
 		const f3 n = sf.normal;
 		const float noL = dot(n, l);
 		const float diffuse = m.diffuseScale;
-		f3 r = (noL <= 0.0f || diffuse <= 0.0f) ? mk(0, 0, 0) : base * (diffuse * noL / kPiF);
+		const float surfaceDiffuse = diffuse * (1.0f - m.subsurface);
+		f3 r = (noL <= 0.0f || surfaceDiffuse <= 0.0f) ? mk(0, 0, 0) : base * (surfaceDiffuse * noL / kPiF);
+		// subsurfaceBSDFCos (:251-271): base * subsurface colour, two-sided when thin
+		const float ssScale = diffuse * m.subsurface;
+		const float ssCos = (m.flags & GPU_MATF_THIN_SUBSURFACE) ? fabsf(noL) : noL;
+		if (ssCos > 0.0f && ssScale > 0.0f) r = r + ld3(m.subsurfaceAlbedo) * (ssScale * ssCos / kPiF);
 		const float sheenScale = diffuse * m.sheen;
 		if (noL > 0.0f && sheenScale > 0.0f)
 		{
@@ -449,7 +455,9 @@ namespace luzgpu
 			trans = dielectricPdf(n, view, l, m.alpha, etaI, etaT);
 		}
 		const float coat = ggxReflectionPdf(n, view, l, m.clearcoatAlpha);
-		const float pdfs[7] = {cosP, 0.0f, cosP, spec, spec, trans, coat};
+		// subsurfaceCosinePDF (:236-249)
+		const float ssP = (m.flags & GPU_MATF_THIN_SUBSURFACE) ? fabsf(dot(n, l)) / (2.0f * kPiF) : cosP;
+		const float pdfs[7] = {cosP, ssP, cosP, spec, spec, trans, coat};
 		float pdf = 0.0f;
 		for (int i = 0; i < 7; i++)
 		{
@@ -518,8 +526,9 @@ namespace luzgpu
 		return d * (1.0f / l);
 	}
 
-	// Sampler::cosineHemisphere (Sampler.cpp:239-246) in the ONB of `n`
-	__device__ __forceinline__ f3	cosineHemisphereDir(f3 n, const PixelSampler& ps)
+	// Sampler::cosineHemisphere (Sampler.cpp:239-246) in the ONB of `n`; `flip` mirrors it
+	// below the surface (the thin subsurface lobe's far side, Principled.cpp:156-167)
+	__device__ __forceinline__ f3	cosineHemisphereDir(f3 n, const PixelSampler& ps, bool flip = false)
 	{
 		float sx, sy;
 		ps.sample2D(DIM_BSDF_DIRECTION, sx, sy);
@@ -528,7 +537,7 @@ namespace luzgpu
 		const float z = sqrtf(fmaxf(0.0f, 1.0f - sy));
 		float sp, cp;
 		sincosf(phi, &sp, &cp);
-		return Onb(n).local(r * cp, r * sp, z);
+		return Onb(n).local(r * cp, r * sp, flip ? -z : z);
 	}
 
 	// GGX half-vector reflection sample (Glossy.cpp:22-40, Principled.cpp:334-357)
@@ -565,6 +574,7 @@ namespace luzgpu
 		sc.absorbing = false;
 		sc.specular = false;
 		sc.mirrorFallback = false;
+		sc.subsurfaceExit = false;
 		sc.pdfType = PDF_NONE;
 		if (m.kind == GPU_MAT_LAMBERTIAN)
 		{
@@ -636,7 +646,9 @@ namespace luzgpu
 			f3 dir;
 			if (comp <= 2)
 			{
-				dir = cosineHemisphereDir(sf.normal, ps);
+				const bool thin = comp == 1 && (m.flags & GPU_MATF_THIN_SUBSURFACE) != 0;
+				dir = cosineHemisphereDir(sf.normal, ps, thin && ps.sample1D(DIM_SUBSURFACE_THIN_SIDE) < 0.5);
+				sc.subsurfaceExit = comp == 1 && (m.flags & GPU_MATF_BURLEY_SUBSURFACE) != 0;
 			}
 			else if (comp == 5)
 			{

@@ -554,7 +554,7 @@ namespace luzgpu
 					h.v = hv.z;
 					h.hittable = __float_as_uint(hv.w);
 					h.tri = w.qTri[i];
-					const Surface sf = surfaceAt<Feat>(s, ray, h);
+					Surface sf = surfaceAt<Feat>(s, ray, h);
 					shadePoint = sf.position;
 					from = originOf(s, h);
 					const GpuMaterial& m = s.materials[sf.material];
@@ -586,6 +586,18 @@ namespace luzgpu
 					}
 					else
 					{
+						if constexpr ((Feat & kFeatSubsurface) != 0)
+						{
+							if (sc.subsurfaceExit)
+							{
+								LocalStack probeStack;
+								probeStack.reset();
+								std::uint32_t probes = 0;
+								relocateSubsurface<Feat>(s, p, m, h, sf, sc, ps, probeStack, shadowLogU, probes);
+								from = originOf(s, h);
+								shadePoint = sf.position;
+							}
+						}
 						f3 att = sc.attenuation;
 						if (!sf.frontFace && sc.absorbing)
 						{
@@ -1038,23 +1050,24 @@ namespace luzgpu
 			checkHip(hipEventRecord(e, stream), "hipEventRecord");
 		};
 		const std::uint32_t vol = volumes ? 1u : 0u;
+		constexpr int T = Feat & kFeatAll;	// extend and shadow differ only by primitive / material class
 		wf_zero_counts<<<1, 4 * kMaxBounceSlots, 0, stream>>>(w.counts, 4 * kMaxBounceSlots, st.jobs, jobCount);
 		wf_raygen<<<kWaveGrid, kWaveBlock, 0, stream>>>(st.jobs, jobCount, w);
 		for (int b = 0; b <= maxBounces; b++)
 		{
 			mark();
 			const unsigned ch = tune.chunk;
-			if (count) wf_extend<true, Feat, kWaveLdsStack><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
-			else if (tune.ldsStack == 8) wf_extend<false, Feat, 8><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
-			else if (tune.ldsStack == 24) wf_extend<false, Feat, 24><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
-			else wf_extend<false, Feat, 16><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
+			if (count) wf_extend<true, T, kWaveLdsStack><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
+			else if (tune.ldsStack == 8) wf_extend<false, T, 8><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
+			else if (tune.ldsStack == 24) wf_extend<false, T, 24><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
+			else wf_extend<false, T, 16><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.extendRefill, ch);
 			mark();
 			wf_shade<Feat><<<kWaveGrid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, vol);
 			mark();
-			if (count) wf_shadow<true, Feat, kWaveLdsStack><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
-			else if (tune.ldsStack == 8) wf_shadow<false, Feat, 8><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
-			else if (tune.ldsStack == 24) wf_shadow<false, Feat, 24><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
-			else wf_shadow<false, Feat, 16><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
+			if (count) wf_shadow<true, T, kWaveLdsStack><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
+			else if (tune.ldsStack == 8) wf_shadow<false, T, 8><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
+			else if (tune.ldsStack == 24) wf_shadow<false, T, 24><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
+			else wf_shadow<false, T, 16><<<tune.grid, kWaveBlock, 0, stream>>>(s, st.jobs, w, b, counters, tune.shadowRefill, ch, vol);
 			mark();
 			wf_resolve<<<kWaveGrid, kWaveBlock, 0, stream>>>(w, b, st.colors);
 		}
@@ -1067,8 +1080,9 @@ namespace luzgpu
 	inline void	enqueuePass(int features, WavefrontState& st, const SceneView& s, std::uint32_t jobCount, int maxBounces, bool volumes,
 		unsigned long long* counters, hipStream_t stream, bool count, WaveTuning tune, std::vector<hipEvent_t>* marks)
 	{
-		switch (features & kFeatAll)
+		switch (features & kFeatEvery)
 		{
+			case kFeatEvery: enqueuePassF<kFeatEvery>(st, s, jobCount, maxBounces, volumes, counters, stream, count, tune, marks); break;
 			case 0: enqueuePassF<0>(st, s, jobCount, maxBounces, volumes, counters, stream, count, tune, marks); break;
 			case kFeatAnalytic: enqueuePassF<kFeatAnalytic>(st, s, jobCount, maxBounces, volumes, counters, stream, count, tune, marks); break;
 			case kFeatRich: enqueuePassF<kFeatRich>(st, s, jobCount, maxBounces, volumes, counters, stream, count, tune, marks); break;

@@ -988,6 +988,7 @@ namespace luzgpu
 			g.clearcoatAlpha = static_cast<float>(alphaOf(m.clearcoatRoughness));
 			g.sheen = static_cast<float>(m.sheen);
 			bool smoothTransmission = false;
+			std::uint32_t subsurfaceFlags = 0u;
 			if (m.kind == MAT_PRINCIPLED)
 			{
 				// componentWeights (Principled.cpp:158-202) for the material's constant base color
@@ -998,9 +999,11 @@ namespace luzgpu
 				const double f0 = c01(r0 * r0);
 				const double baseLum = std::max(0.05, lumW(base));
 				const double diffuse = (1.0 - m.metallic) * (1.0 - m.transmission) * (1.0 - f0) * (1.0 - 0.25 * m.clearcoat);
+				const dvec3 ssColor = {c01(m.subsurfaceColor[0]), c01(m.subsurfaceColor[1]), c01(m.subsurfaceColor[2])};
+				const dvec3 ssAlbedo = {base[0] * ssColor[0], base[1] * ssColor[1], base[2] * ssColor[2]};
 				const double w[7] = {
-					diffuse * 1.0 * baseLum,
-					0.0,
+					diffuse * (1.0 - m.subsurface) * baseLum,
+					diffuse * m.subsurface * std::max(0.05, lumW(ssAlbedo)),
 					diffuse * m.sheen * 0.35,
 					(1.0 - m.metallic) * std::max(0.03, f0) * (1.0 - 0.5 * m.transmission),
 					m.metallic * std::max(0.05, baseLum),
@@ -1012,10 +1015,18 @@ namespace luzgpu
 				for (int i = 0; i < 7; i++) g.lobe[i] = total > 0.0 ? static_cast<float>(w[i] / total) : (i == 0 ? 1.0f : 0.0f);
 				g.diffuseScale = static_cast<float>(diffuse);
 				smoothTransmission = m.metallic <= 1e-6 && m.roughness <= 1e-4 && m.transmission >= 1.0 - 1e-6
-					&& m.clearcoat <= 1e-6 && m.sheen <= 1e-6;
+					&& m.clearcoat <= 1e-6 && m.sheen <= 1e-6 && m.subsurface <= 1e-6;
+				g.subsurface = static_cast<float>(m.subsurface);
+				const dvec3 radius = m.subsurfaceRadius * m.subsurfaceScale;
+				g.subsurfaceRadius = static_cast<float>(std::max(radius[0], std::max(radius[1], radius[2])));
+				for (int i = 0; i < 3; i++) g.subsurfaceAlbedo[i] = static_cast<float>(ssAlbedo[i]);
+				if (m.subsurface > 0.0)
+				{
+					subsurfaceFlags = m.subsurfaceThin ? GPU_MATF_THIN_SUBSURFACE : GPU_MATF_BURLEY_SUBSURFACE;
+				}
 			}
 			p.materialColor.push_back(m.color);
-			g.flags = (smoothTransmission ? GPU_MATF_SMOOTH_TRANSMISSION : 0u) | (m.conductor ? GPU_MATF_CONDUCTOR : 0u)
+			g.flags = subsurfaceFlags | (smoothTransmission ? GPU_MATF_SMOOTH_TRANSMISSION : 0u) | (m.conductor ? GPU_MATF_CONDUCTOR : 0u)
 				| ((m.absorption[0] > 0.0 || m.absorption[1] > 0.0 || m.absorption[2] > 0.0) ? GPU_MATF_ABSORPTION : 0u);
 			p.materials.push_back(g);
 		}
@@ -1501,6 +1512,10 @@ namespace luzgpu
 			if (m.kind == MAT_GLOSSY || m.kind == MAT_DIFFUSE_GLOSSY || m.kind == MAT_PRINCIPLED || m.kind == MAT_ISOTROPIC || m.kind == MAT_HG)
 			{
 				p.features |= kFeatRich;
+			}
+			if (m.kind == MAT_PRINCIPLED && m.subsurface > 0.0 && !m.subsurfaceThin)
+			{
+				p.features |= kFeatEvery;	// the relocation probe is compiled only into these kernels
 			}
 		}
 		if (s.sky == SKY_ATMOSPHERE) p.features |= kFeatRich;	// the atmosphere march lives in the rich kernels

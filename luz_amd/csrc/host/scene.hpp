@@ -59,6 +59,13 @@ namespace luzgpu
 		double	clearcoat = 0.0;
 		double	clearcoatRoughness = 0.03;
 		double	sheen = 0.0;
+		// principled subsurface (Principled.cpp:39-88, 450-454): the lobe weight, the Burley
+		// diffusion radius per channel times its scale in metres, the tint, thin or Burley
+		double	subsurface = 0.0;
+		dvec3	subsurfaceRadius = {1.0, 1.0, 1.0};
+		double	subsurfaceScale = 0.001;
+		dvec3	subsurfaceColor = {1.0, 1.0, 1.0};
+		bool	subsurfaceThin = false;
 		std::string	name;
 	};
 

@@ -58,6 +58,34 @@ namespace luzgpu
 			m.clearcoat = d.clearcoat;
 			m.clearcoatRoughness = d.clearcoat_roughness;
 			m.sheen = d.sheen;
+			if (d.kind == LUZGPU_MAT_PRINCIPLED && d.subsurface != 0.0)
+			{
+				const std::string at = "material " + std::to_string(index) + ": subsurface ";
+				if (!std::isfinite(d.subsurface) || d.subsurface < 0.0 || d.subsurface > 1.0)
+				{
+					throw std::invalid_argument(at + "weight must be finite and in [0,1]");
+				}
+				bool positive = false;
+				for (int i = 0; i < 3; i++)
+				{
+					if (!std::isfinite(d.subsurface_radius[i]) || d.subsurface_radius[i] < 0.0
+						|| !std::isfinite(d.subsurface_color[i]) || d.subsurface_color[i] < 0.0)
+					{
+						throw std::invalid_argument(at + "radius and color must have finite non-negative channels");
+					}
+					positive = positive || d.subsurface_radius[i] > 0.0;
+				}
+				if (!positive) throw std::invalid_argument(at + "radius must have at least one positive channel");
+				if (!std::isfinite(d.subsurface_scale) || d.subsurface_scale <= 0.0)
+				{
+					throw std::invalid_argument(at + "scale must be finite and positive");
+				}
+				m.subsurface = d.subsurface;
+				m.subsurfaceRadius = d3(d.subsurface_radius);
+				m.subsurfaceScale = d.subsurface_scale;
+				m.subsurfaceColor = d3(d.subsurface_color);
+				m.subsurfaceThin = d.subsurface_thin != 0;
+			}
 			requireFinite(m.color, "material color");
 			if (!std::isfinite(m.roughness) || m.roughness < 0.0 || m.roughness > 1.0)
 			{

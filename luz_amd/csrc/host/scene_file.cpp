@@ -624,8 +624,17 @@ namespace luzgpu
 			bool	hasIor = false;
 			double	metallic = 0.0, transmission = 0.0, clearcoat = 0.0, clearcoatRoughness = 0.03, sheen = 0.0;
 			double	glossyWeight = 0.5, alphaKey = 1.0;
-			bool	hasSubsurface = false;
-			double	subsurface = 0.0;
+			bool	hasSubsurface = false, hasSubsurfaceRadius = false, hasSubsurfaceScale = false;
+			bool	hasSubsurfaceColor = false, hasSubsurfaceMethod = false;
+			double	subsurface = 0.0, subsurfaceScale = 0.001;
+			dvec3	subsurfaceRadius = {1.0, 1.0, 1.0}, subsurfaceColor = {1.0, 1.0, 1.0};
+			bool	subsurfaceThin = false;
+			std::string	subsurfaceProfile;
+			bool	anySubsurface() const
+			{
+				return hasSubsurface || hasSubsurfaceRadius || hasSubsurfaceScale || hasSubsurfaceColor || hasSubsurfaceMethod
+					|| !subsurfaceProfile.empty();
+			}
 			dvec3	glossyColor = {1.0, 1.0, 1.0};
 			bool	hasGlossyColor = false;
 			std::optional<double>	radiance, lum;
@@ -738,14 +747,13 @@ namespace luzgpu
 			{
 				throw std::runtime_error("Glass dispersion properties are only valid for dielectric or principled materials.");
 			}
-			if (k.hasSubsurface && type != "principled") throw std::runtime_error("Subsurface properties are only valid for principled materials.");
+			if (k.anySubsurface() && type != "principled") throw std::runtime_error("Subsurface properties are only valid for principled materials.");
 			Material m;
 			m.color = k.color;
 			if (type == "principled")
 			{
 				if (k.eta || k.extinction) throw std::runtime_error("Principled conductor metals use metallic color; use type=metal for eta/k.");
 				m.absorption = absorptionOf(k, "Principled");
-				if (k.hasSubsurface && k.subsurface != 0.0) throw std::runtime_error("Unsupported for the device path: principled subsurface");
 				m.kind = MAT_PRINCIPLED;
 				m.metallic = unitValue(k.metallic, "Principled metallic");
 				m.roughness = unitValue(k.roughness, "Principled roughness");
@@ -756,6 +764,47 @@ namespace luzgpu
 				m.clearcoat = unitValue(k.clearcoat, "Principled clearcoat");
 				m.clearcoatRoughness = unitValue(k.clearcoatRoughness, "Principled clearcoat roughness");
 				m.sheen = unitValue(k.sheen, "Principled sheen");
+				// the subsurface profile, then method, radius, scale, colour and weight
+				// (Materials.cpp:727-757 over Principled.cpp:39-88)
+				auto finiteNonNegative = [](const dvec3& c, const std::string& what) {
+					for (double v : c)
+					{
+						if (!std::isfinite(v) || v < 0.0) throw std::invalid_argument(what + " must have finite non-negative channels.");
+					}
+				};
+				if (!k.subsurfaceProfile.empty())
+				{
+					if (k.subsurfaceProfile != "skin") throw std::runtime_error("Unknown subsurface profile: " + k.subsurfaceProfile);
+					m.subsurfaceThin = false;
+					m.subsurfaceRadius = {1.0, 0.35, 0.18};
+					m.subsurfaceScale = 0.0012;
+					m.subsurfaceColor = {1.0, 0.42, 0.28};
+					if (m.subsurface <= 0.0) m.subsurface = 0.5;
+				}
+				if (k.hasSubsurfaceMethod) m.subsurfaceThin = k.subsurfaceThin;
+				if (k.hasSubsurfaceRadius)
+				{
+					finiteNonNegative(k.subsurfaceRadius, "Principled subsurface radius");
+					if (!(k.subsurfaceRadius[0] > 0.0 || k.subsurfaceRadius[1] > 0.0 || k.subsurfaceRadius[2] > 0.0))
+					{
+						throw std::invalid_argument("Principled subsurface radius must have at least one positive channel.");
+					}
+					m.subsurfaceRadius = k.subsurfaceRadius;
+				}
+				if (k.hasSubsurfaceScale)
+				{
+					if (!std::isfinite(k.subsurfaceScale) || k.subsurfaceScale <= 0.0)
+					{
+						throw std::invalid_argument("Principled subsurface scale must be finite and positive.");
+					}
+					m.subsurfaceScale = k.subsurfaceScale;
+				}
+				if (k.hasSubsurfaceColor)
+				{
+					finiteNonNegative(k.subsurfaceColor, "Principled subsurface color");
+					m.subsurfaceColor = k.subsurfaceColor;
+				}
+				if (k.hasSubsurface) m.subsurface = unitValue(k.subsurface, "Principled subsurface");
 				return (m);
 			}
 			if (type == "glossy")
@@ -1426,9 +1475,33 @@ namespace luzgpu
 						k.subsurface = std::stod(value);
 						k.hasSubsurface = true;
 					}
-					else if (key.rfind("subsurface_", 0) == 0 || key.rfind("subsurface", 0) == 0 || key.rfind("sss", 0) == 0)
+					else if (key == "subsurface_radius" || key == "subsurfaceradius" || key == "sss_radius" || key == "sssradius")
 					{
-						throw std::runtime_error("Unsupported for the device path: principled subsurface (" + key + ")");
+						k.subsurfaceRadius = parseColor(value, key);
+						k.hasSubsurfaceRadius = true;
+					}
+					else if (key == "subsurface_scale" || key == "subsurfacescale" || key == "sss_scale" || key == "sssscale")
+					{
+						k.subsurfaceScale = std::stod(value);
+						k.hasSubsurfaceScale = true;
+					}
+					else if (key == "subsurface_color" || key == "subsurfacecolor" || key == "sss_color" || key == "ssscolor")
+					{
+						k.subsurfaceColor = parseColor(value, key);
+						k.hasSubsurfaceColor = true;
+					}
+					else if (key == "subsurface_method" || key == "subsurfacemethod" || key == "sss_method" || key == "sssmethod")
+					{
+						// parseSubsurfaceMethod (Materials.cpp:349-367)
+						const std::string v = lower(trim(value));
+						if (v == "burley" || v == "diffusion" || v == "normalized_diffusion" || v == "normalizeddiffusion") k.subsurfaceThin = false;
+						else if (v == "thin" || v == "thin_translucent" || v == "thintranslucent") k.subsurfaceThin = true;
+						else throw std::runtime_error("Unknown subsurface method: " + value);
+						k.hasSubsurfaceMethod = true;
+					}
+					else if (key == "subsurface_profile" || key == "subsurfaceprofile" || key == "sss_profile" || key == "sssprofile")
+					{
+						k.subsurfaceProfile = lower(trim(value));
 					}
 					else if (key == "glossy_color" || key == "glossycolor" || key == "specular_color" || key == "specularcolor")
 					{

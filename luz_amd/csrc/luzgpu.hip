@@ -193,8 +193,9 @@ namespace luzgpu
 	void	launchMegaC(int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, const std::uint32_t* list,
 		std::uint32_t pixels, float* colors, unsigned long long* counters)
 	{
-		switch (features & kFeatAll)
+		switch (features & kFeatEvery)
 		{
+			case kFeatEvery: render_mega<Count, kFeatEvery><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, counters); break;
 			case 0: render_mega<Count, 0><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, counters); break;
 			case kFeatAnalytic: render_mega<Count, kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, counters); break;
 			case kFeatRich: render_mega<Count, kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, counters); break;
@@ -309,8 +310,9 @@ namespace luzgpu
 	void	launchMegaBatchC(int features, size_t grid, hipStream_t st, const SceneView& s, const MegaJob* jobs, std::uint32_t n,
 		float* colors, unsigned long long* counters)
 	{
-		switch (features & kFeatAll)
+		switch (features & kFeatEvery)
 		{
+			case kFeatEvery: render_mega_batch<Count, kFeatEvery><<<grid, kMegaBlock, 0, st>>>(s, jobs, n, colors, counters); break;
 			case 0: render_mega_batch<Count, 0><<<grid, kMegaBlock, 0, st>>>(s, jobs, n, colors, counters); break;
 			case kFeatAnalytic: render_mega_batch<Count, kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, jobs, n, colors, counters); break;
 			case kFeatRich: render_mega_batch<Count, kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, jobs, n, colors, counters); break;
@@ -358,8 +360,9 @@ namespace luzgpu
 	void	launchMegaDenoise(int features, size_t grid, hipStream_t st, const SceneView& s, const GpuParams& p, const std::uint32_t* list,
 		std::uint32_t pixels, float* colors, float* feats, unsigned long long* counters)
 	{
-		switch (features & kFeatAll)
+		switch (features & kFeatEvery)
 		{
+			case kFeatEvery: render_mega_denoise<kFeatEvery><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, feats, counters); break;
 			case 0: render_mega_denoise<0><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, feats, counters); break;
 			case kFeatAnalytic: render_mega_denoise<kFeatAnalytic><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, feats, counters); break;
 			case kFeatRich: render_mega_denoise<kFeatRich><<<grid, kMegaBlock, 0, st>>>(s, p, list, pixels, colors, feats, counters); break;
@@ -2288,6 +2291,10 @@ int	luzgpu_adjoint(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* 
 		if (shmem > 96 * 1024)
 		{
 			throw std::runtime_error("Unsupported for the adjoint: more than " + std::to_string(96 * 1024 / (kGradPerMaterial * 4)) + " materials");
+		}
+		if (scene->gpu.host.features & kFeatSubsurface)
+		{
+			throw std::runtime_error("Unsupported for the adjoint: Burley subsurface (its exits move the path vertex)");
 		}
 		hipStream_t st = ctx->stream;
 		const std::uint32_t pixels = p.width * p.height;

@@ -147,11 +147,12 @@ namespace luzgpu_glue
 				case PRINCIPLED:
 				{
 					const Principled& x = static_cast<const Principled&>(m);
-					if (x.getSubsurface() > 0.0)
-					{
-						throw std::runtime_error("Unsupported for the device path: principled subsurface");
-					}
 					d.kind = LUZGPU_MAT_PRINCIPLED;
+					d.subsurface = x.getSubsurface();
+					put(x.getSubsurfaceRadius(), d.subsurface_radius);
+					d.subsurface_scale = x.getSubsurfaceScale();
+					put(x.getSubsurfaceColor(), d.subsurface_color);
+					d.subsurface_thin = x.getSubsurfaceMethod() == SUBSURFACE_THIN ? 1u : 0u;
 					d.metallic = x.getMetallic();
 					d.roughness = x.getRoughness();
 					d.transmission = x.getTransmission();

@@ -573,6 +573,62 @@ def write_optics(directory: str, width=160, height=120, spp=128, depth=10, seed=
     return Workload("optics", directory, path, width, height, spp, depth, seed, 0)
 
 
+def subsurface_scene_text(width=160, height=120, spp=64, depth=8) -> str:
+    """Principled subsurface: Burley diffusion on analytic spheres (the skin profile with a
+    wider scale, and explicit radius / colour), on a displaced mesh sphere and on a cube,
+    whose exits are relocated by the probe ray; and a thin translucent panel lit from behind
+    (the two-sided lobe), under an overhead panel and a sphere light behind the leaf."""
+    mats = [
+        ("skin", ["type=principled", "color=(0.9,0.7,0.6)", "roughness=0.5", "subsurface_profile=skin",
+                  "subsurface=1", "subsurface_scale=0.06"]),
+        ("jade", ["type=principled", "color=(0.5,0.8,0.6)", "roughness=0.2", "subsurface=0.7",
+                  "subsurface_radius=(0.4,1.0,0.6)", "subsurface_scale=0.08", "subsurface_color=(0.8,1,0.9)",
+                  "sheen=0.3"]),
+        ("wax", ["type=principled", "color=(0.95,0.9,0.7)", "roughness=0.35", "subsurface=0.9",
+                 "subsurface_method=burley", "subsurface_radius=(1,0.8,0.5)", "subsurface_scale=0.1",
+                 "clearcoat=0.5"]),
+        ("marble", ["type=principled", "color=(0.9,0.9,0.88)", "roughness=0.25", "subsurface=0.6",
+                    "sss_scale=0.05"]),
+        ("leaf", ["type=principled", "color=(0.3,0.7,0.2)", "roughness=0.6", "subsurface=0.8",
+                  "subsurface_method=thin", "subsurface_color=(0.9,1,0.6)"]),
+    ]
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "sky=none", "background=(0.03,0.03,0.04)", "",
+        "[materials]",
+        "material floor {", "type=lambertian", "color=(0.55,0.55,0.52)", "}",
+        "material panel {", "type=emissive", "color=(1,0.95,0.85)", "radiance=7", "}",
+        "material back {", "type=emissive", "color=(1,0.9,0.7)", "radiance=40", "}",
+    ]
+    for name, props in mats:
+        lines += [f"material {name} {{", *props, "}"]
+    lines += [
+        "",
+        "[scene]",
+        "camera main {", "position=(0,3,-10)", "direction=(0,-0.2,1)", "up=(0,1,0)", "focal_length_mm=35",
+        "sensor_width_mm=36", "sensor_height_mm=27", "pinhole=1", "focus_distance=10", "}",
+        "object wax {", "file=wax.obj", "position=(0.2,1,1.5)", "rotation=(0,30,0)", "scale=(1,1,1)", "material=wax", "}",
+        "objects{",
+        "plane=0,(0,1,0),material=floor",
+        "rectangle=(0,7,2),(0,-1,0),6,3,material=panel",
+        "sphere=(-3,1,0.5),1,material=skin",
+        "sphere=(3,0.9,0.5),0.9,material=jade",
+        "cube=(-1.5,0.6,-1.5),(0,0,-1),1.2,1.2,1.2,material=marble",
+        "rectangle=(2,1.6,-2),(0.3,0,-1),1.6,2.4,material=leaf",
+        "sphere=(2.4,1.6,-0.6),0.25,material=back",
+        "}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def write_subsurface(directory: str, width=160, height=120, spp=64, depth=8, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    _write_if_changed(os.path.join(directory, "wax.obj"), _sphere_obj(random.Random(1008), 0.9).encode())
+    path = os.path.join(directory, "subsurface.luz")
+    _write_if_changed(path, subsurface_scene_text(width, height, spp, depth).encode())
+    return Workload("subsurface", directory, path, width, height, spp, depth, seed, 2 * SPHERE_SLICES * (SPHERE_STACKS - 1))
+
+
 def volume_scene_text(width=160, height=120, spp=32, depth=16) -> str:
     lines = [
         "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,

@@ -14,6 +14,9 @@
 //       The Scene built in code rendered twice with the same seed: through the binding's
 //       manageThreadsOnDevice and through the reference's Renderer::render on the CPU;
 //       prints "rel_mse <value>" of the device image against the reference's.
+//   glue_driver render-file <device> <scene.luz> <seed>
+//       The same for a Scene read by the reference's own loader (twice, one per route),
+//       with the raw view transform and no bloom on both.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -266,6 +269,28 @@ int	main(int argc, char** argv)
 			Scene scene;
 			buildScene(scene, std::stoul(argv[2]), std::stoul(argv[3]), 1);
 			printRoutes(scene, nullptr);
+			return 0;
+		}
+		if (cmd == "render-file" && argc >= 5)
+		{
+			const int device = std::stoi(argv[2]);
+			setRandomSeed(std::stoi(argv[4]));
+			Scene onDevice, onCpu;
+			for (Scene* sc : {&onDevice, &onCpu})
+			{
+				sc->setBenchmarkMode(true);
+				SceneFile::read(*sc, argv[3]);
+				sc->setRenderingThreads(std::max(1u, std::thread::hardware_concurrency()));
+				// compare the radiance images: the display transform (tested on its own)
+				// turns fp32 noise in dark pixels into large relative differences
+				sc->setViewTransform(ViewTransform::Raw);
+				sc->setBloom(false);
+			}
+			luzgpu_glue::manageThreadsOnDevice(onDevice, device, unwrap);
+			if (!Renderer::render(onCpu)) return 5;
+			std::cout << "device_ms " << onDevice.getRenderStats().renderMS << " cpu_ms " << onCpu.getRenderStats().renderMS
+				<< " samples " << onDevice.getRenderStats().renderedSamples << " " << onCpu.getRenderStats().renderedSamples << "\n";
+			std::cout << "rel_mse " << relMse(*onDevice.getImage(), *onCpu.getImage()) << "\n";
 			return 0;
 		}
 		if (cmd == "render-inmemory" && argc >= 7)

@@ -148,7 +148,8 @@ namespace oracle
 	{
 		D_CAMERA = 0, D_LENS = 1, D_LIGHT_STRATEGY = 2, D_LIGHT_SELECTION = 3, D_LIGHT_SURFACE_SELECTION = 4, D_LIGHT_SURFACE_POINT = 5,
 		D_BSDF_DIRECTION = 6, D_MATERIAL_DECISION = 7, D_RR = 9, D_VOLUME_DISTANCE = 10, D_ENV_SELECTION = 12,
-		D_ENV_POINT = 13, D_ENV_STRATEGY = 14
+		D_ENV_POINT = 13, D_ENV_STRATEGY = 14, D_SUBSURFACE_SPATIAL = 17, D_SUBSURFACE_PROFILE_COMPONENT = 18,
+		D_SUBSURFACE_THIN_SIDE = 19
 	};
 
 	// ---- rays (Ray.hpp) -----------------------------------------------------------------------
@@ -607,6 +608,12 @@ namespace oracle
 		bool	conductor = false;	// Metal::_useConductorFresnel (Metal.cpp:118-125)
 		C	eta = {1, 1, 1}, k = {0, 0, 0};
 		C	absorption = {0, 0, 0};	// dielectric / principled interior absorption, 1/m
+		// principled subsurface (Principled.cpp:39-88): weight, tint, Burley diffusion
+		// length (max channel of radius * scale, metres), thin or Burley
+		double	subsurface = 0.0;
+		C	subsurfaceColor = {1, 1, 1};
+		double	subsurfaceRadius = 0.0;
+		bool	subsurfaceThin = false;
 	};
 
 	struct	Light
@@ -711,6 +718,11 @@ namespace oracle
 			o.eta = fromC(m.eta);
 			o.k = fromC(m.k);
 			o.absorption = fromC(m.absorption);
+			o.subsurface = m.subsurface;
+			o.subsurfaceColor = fromC(m.subsurfaceColor);
+			o.subsurfaceRadius = std::max(m.subsurfaceRadius[0] * m.subsurfaceScale,
+				std::max(m.subsurfaceRadius[1] * m.subsurfaceScale, m.subsurfaceRadius[2] * m.subsurfaceScale));
+			o.subsurfaceThin = m.subsurfaceThin;
 			s.mats.push_back(o);
 		}
 		for (const auto& mesh : in.meshes)
@@ -1272,6 +1284,7 @@ namespace oracle
 		bool	mirrorFallback = false;	// rough metal: the GGX sample reflected below the horizon
 		bool	absorbing = false;	// ScatterRecord::hasMediumAbsorption
 		C	absorption;			// ScatterRecord::mediumAbsorptionCoefficient
+		bool	subsurface = false;	// ScatterRecord::hasSubsurface (the subsurface lobe was sampled)
 	};
 
 	// ---- glossy / diffuse-glossy / principled (Glossy.cpp, DiffuseGlossy.cpp, Principled.cpp) -----
@@ -1292,16 +1305,16 @@ namespace oracle
 		const double ce = 0.25 * m.clearcoat;
 		return (1.0 - m.metallic) * (1.0 - m.transmission) * (1.0 - dielF0(m.ior)) * (1.0 - ce);
 	}
-	Lobes	principledLobes(const Mat& m, C base)	// componentWeights, :158-202 (subsurface 0)
+	Lobes	principledLobes(const Mat& m, C base)	// componentWeights, :158-202
 	{
 		Lobes r;
 		const double baseLum = std::max(0.05, lumWeight(base));
 		const double diffuse = principledDiffuseScale(m);
 		const double spec = (1.0 - m.metallic) * std::max(0.03, dielF0(m.ior));
-		const double subsurfaceWeight = diffuse * 0.0;
-		const double surfaceDiffuse = diffuse * (1.0 - 0.0);
+		const double subsurfaceWeight = diffuse * m.subsurface;
+		const double surfaceDiffuse = diffuse * (1.0 - m.subsurface);
 		r.w[0] = surfaceDiffuse * baseLum;
-		r.w[1] = subsurfaceWeight * std::max(0.05, lumWeight(base * C{1, 1, 1}));
+		r.w[1] = subsurfaceWeight * std::max(0.05, lumWeight(base * clamp01(m.subsurfaceColor)));
 		r.w[2] = diffuse * m.sheen * 0.35;
 		r.w[3] = spec * (1.0 - 0.5 * m.transmission);
 		r.w[4] = m.metallic * std::max(0.05, baseLum);
@@ -1320,8 +1333,14 @@ namespace oracle
 		const V n = h.n;
 		C r;
 		{	// diffuseBSDFCos, :222-234
-			const double noL = dot(n, l), sc = diffuse * (1.0 - 0.0);
+			const double noL = dot(n, l), sc = diffuse * (1.0 - m.subsurface);
 			if (!(noL <= 0.0 || sc <= 0.0)) r = base * (sc * noL / PI);
+		}
+		{	// subsurfaceBSDFCos, :251-271 (two-sided when thin)
+			double noL = dot(n, l);
+			if (m.subsurfaceThin) noL = std::fabs(noL);
+			const double sc = diffuse * m.subsurface;
+			if (!(noL <= 0.0 || sc <= 0.0)) r = r + (base * clamp01(m.subsurfaceColor)) * (sc * noL / PI);
 		}
 		{	// sheenBSDFCos, :272-291
 			const double noL = dot(n, l), sc = diffuse * m.sheen;
@@ -1355,7 +1374,8 @@ namespace oracle
 		const double ei = h.front ? 1.0 : m.ior, et = h.front ? m.ior : 1.0;
 		double pdfs[7] = {};
 		pdfs[0] = cosPdf(h.n, l);
-		pdfs[1] = cosPdf(h.n, l);	// subsurfaceCosinePDF (Burley)
+		// subsurfaceCosinePDF (:236-249)
+		pdfs[1] = m.subsurfaceThin ? std::fabs(dot(h.n, l)) / (2.0 * PI) : cosPdf(h.n, l);
 		pdfs[2] = pdfs[0];
 		pdfs[3] = ggxReflPdf(h.n, view, l, a);
 		pdfs[4] = pdfs[3];
@@ -1444,13 +1464,14 @@ namespace oracle
 			if (dot(dir, h.n) <= 0.0) dir = reflect(r.d, h.n);
 			return dir;
 		};
-		auto cosineDir = [&]() {	// Sampler::cosineHemisphere in ONB(normal)
+		auto cosineDir = [&](bool flip = false) {	// Sampler::cosineHemisphere in ONB(normal)
 			float sx, sy;
 			ps.s2(D_BSDF_DIRECTION, sx, sy);
 			const float phi = static_cast<float>(2.0 * PI) * sx;
 			const float rr = std::sqrt(sy);
 			const float z = std::sqrt(std::max(0.0f, 1.0f - sy));
-			return Onb(h.n).local(static_cast<double>(rr * std::cos(phi)), static_cast<double>(rr * std::sin(phi)), static_cast<double>(z));
+			return Onb(h.n).local(static_cast<double>(rr * std::cos(phi)), static_cast<double>(rr * std::sin(phi)),
+				flip ? -static_cast<double>(z) : static_cast<double>(z));
 		};
 		auto accept = [&](V dir) {
 			sc.type = PDF_BSDF;
@@ -1475,7 +1496,8 @@ namespace oracle
 			sc.absorbing = m.transmission > 0.0 && (m.absorption.r > 0.0 || m.absorption.g > 0.0 || m.absorption.b > 0.0);
 			sc.absorption = m.absorption;
 			const C base = clamp01(m.color);
-			if (m.metallic <= 1e-6 && m.roughness <= 1e-4 && m.transmission >= 1.0 - 1e-6 && m.clearcoat <= 1e-6 && m.sheen <= 1e-6)
+			if (m.metallic <= 1e-6 && m.roughness <= 1e-4 && m.transmission >= 1.0 - 1e-6 && m.clearcoat <= 1e-6 && m.sheen <= 1e-6
+				&& m.subsurface <= 1e-6)
 			{
 				// scatterSmoothTransmission (:401-440)
 				const V view = safeNormalize(r.d * -1.0, h.n);
@@ -1509,7 +1531,8 @@ namespace oracle
 				if (target <= 0.0) { comp = i; break; }
 			}
 			V dir;
-			if (comp <= 2) dir = cosineDir();
+			// the thin subsurface lobe's far side (:156-167): the local z mirrored
+			if (comp <= 2) dir = cosineDir(comp == 1 && m.subsurfaceThin && ps.s1(D_SUBSURFACE_THIN_SIDE) < 0.5);
 			else if (comp == 5)	// sampleGGXTransmission (:359-399)
 			{
 				const V view = safeNormalize(r.d * -1.0, h.n);
@@ -1522,6 +1545,7 @@ namespace oracle
 			}
 			else if (comp == 6) dir = ggxReflect(roughnessToAlpha(m.clearcoatRoughness));
 			else dir = ggxReflect(roughnessToAlpha(m.roughness));
+			sc.subsurface = comp == 1;
 			return accept(dir);
 		}
 		if (m.kind == luzgpu::MAT_ISOTROPIC)	// Isotropic.cpp:15-24
@@ -1623,7 +1647,7 @@ namespace oracle
 	{
 		// materialCanTransmitThroughGeometry (:327-355)
 		if (m.kind == luzgpu::MAT_DIELECTRIC || m.kind == luzgpu::MAT_ISOTROPIC || m.kind == luzgpu::MAT_HG
-			|| (m.kind == luzgpu::MAT_PRINCIPLED && m.transmission > 0.0)) return true;
+			|| (m.kind == luzgpu::MAT_PRINCIPLED && (m.transmission > 0.0 || (m.subsurface > 0.0 && m.subsurfaceThin)))) return true;
 		V g = h.gn;
 		if (len2(g) <= 1e-12) g = h.n;
 		if (len2(g) <= 1e-12) return true;
@@ -2224,6 +2248,50 @@ namespace oracle
 		return Onb(pd).local(st * std::cos(phi), st * std::sin(phi), c);
 	}
 
+	// relocateSubsurfaceExit (ColorHelper.cpp:376-504): after a Burley subsurface scatter the
+	// vertex moves to where a probe ray, cast along -normal from above a point a sampled
+	// radius away on the tangent disk, meets the same material within reach; the cosine
+	// direction is drawn again about the exit's normal (refreshSubsurfaceScatterAtExit,
+	// :394-426). Unchanged when any step fails.
+	void	relocateSubsurface(const Scene& s, const Mat& m, Hit& h, Scatter& sc, const Sampler& ps)
+	{
+		if (!sc.subsurface || m.subsurfaceThin) return;
+		const double d = m.subsurfaceRadius;	// subsurfaceSampleRadiusMeters, :376-392
+		if (!std::isfinite(d) || d <= 0.0) return;
+		const double pick = ps.s1(D_SUBSURFACE_PROFILE_COMPONENT);
+		float rx, ry;
+		ps.s2(D_SUBSURFACE_SPATIAL, rx, ry);
+		const double u = std::max(1e-12, std::min(1.0 - 1e-12, static_cast<double>(ry)));
+		const double radiusM = -(pick < 0.25 ? d : 3.0 * d) * std::log(1.0 - u);
+		if (radiusM <= 0.0 || !std::isfinite(radiusM) || s.mpu <= 0.0 || !std::isfinite(s.mpu)) return;
+		const double radius = radiusM / s.mpu;
+		if (radius <= T_MIN) return;
+		V n = h.n;
+		if (len2(n) <= 1e-12) n = h.gn;
+		if (len2(n) <= 1e-12) return;
+		n = normalize(n);
+		const double angle = 2.0 * PI * static_cast<double>(rx);
+		const V candidate = h.p + Onb(n).local(std::cos(angle) * radius, std::sin(angle) * radius, 0.0);
+		const double reach = std::max(radius + (4.0 * d / s.mpu), 16.0 * T_MIN);
+		const Ray probe = Ray::make(candidate + n * reach, n * -1.0);
+		Hit e;
+		if (!closestHit(s, probe, e, &ps)) return;
+		if (e.material != h.material || e.t <= T_MIN || e.t > (2.0 * reach + T_MIN)) return;
+		float sx, sy;
+		ps.s2(D_BSDF_DIRECTION, sx, sy);
+		const float phi = static_cast<float>(2.0 * PI) * sx;
+		const float rr = std::sqrt(sy);
+		const float z = std::sqrt(std::max(0.0f, 1.0f - sy));
+		const V dir = Onb(e.n).local(static_cast<double>(rr * std::cos(phi)), static_cast<double>(rr * std::sin(phi)),
+			static_cast<double>(z));
+		const double pdf = bsdfPdf(m, e, sc, dir);
+		const C f = bsdfEval(m, e, sc, dir);
+		if (pdf <= 0.0 || !std::isfinite(pdf) || maxChannel(f) <= 0.0) return;
+		h = e;
+		sc.dir = dir;
+		sc.att = f / pdf;
+	}
+
 	C	trace(const Scene& s, Ray ray, Sampler& ps, Adjoint* adj = nullptr)	// calculateLightRaysColor, ColorHelper.cpp:1136-1326
 	{
 		C acc, thr = {1, 1, 1};
@@ -2272,6 +2340,7 @@ namespace oracle
 				}
 				return acc + inc;
 			}
+			relocateSubsurface(s, m, h, sc, ps);
 			// effectiveScatterAttenuation / mediumTransmittance (ColorHelper.cpp:695-723): a back-face
 			// hit attenuates by Beer-Lambert over the segment's length in metres
 			C att = sc.att;

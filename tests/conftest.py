@@ -100,6 +100,12 @@ def optics(workload_dir):
 
 
 @pytest.fixture(scope="session")
+def subsurface(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_subsurface(os.path.join(workload_dir, "subsurface"))
+
+
+@pytest.fixture(scope="session")
 def terrain(workload_dir):
     """The volume workload's scene: 1,002,528-triangle terrain, haze, sun (rendered at
     reduced sizes by the tests)."""

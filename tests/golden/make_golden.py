@@ -477,6 +477,8 @@ def main():
             small_scene_fixtures("atmosphere_env", workloads.write_atmosphere_env(os.path.join(tmp, "atmosphere_env")), tmp)
         if "optics" in only:
             small_scene_fixtures("optics", workloads.write_optics(os.path.join(tmp, "optics")), tmp)
+        if "subsurface" in only:
+            small_scene_fixtures("subsurface", workloads.write_subsurface(os.path.join(tmp, "subsurface")), tmp)
         if "meshlight" in only:
             small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
         if "fd" in only:
@@ -511,6 +513,7 @@ def main():
     small_scene_fixtures("spectral", workloads.write_spectral(os.path.join(tmp, "spectral")), tmp)
     small_scene_fixtures("atmosphere_env", workloads.write_atmosphere_env(os.path.join(tmp, "atmosphere_env")), tmp)
     small_scene_fixtures("optics", workloads.write_optics(os.path.join(tmp, "optics")), tmp)
+    small_scene_fixtures("subsurface", workloads.write_subsurface(os.path.join(tmp, "subsurface")), tmp)
     fd_fixture(tmp)
     fd_rough_fixture(tmp)
     terrain_fixtures(tmp)

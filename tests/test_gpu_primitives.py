@@ -6,7 +6,8 @@ sphere light), ``materials`` (glossy, diffuse-glossy and principled variants) an
 ``meshlight`` (emissive mesh instances under uniform and non-uniform scales, sampled by
 triangle area and re-intersected for their MIS pdf) and ``optics`` (the reference's default
 thin-lens camera, rough dielectrics, conductor-Fresnel metals, dispersive glass presets and
-interior absorption). Same bars as test_gpu_parity: primary-hit IDs against the reference's
+interior absorption) and ``subsurface`` (Burley exits relocated by a probe ray inside the
+shading kernels, the skin profile, the thin two-sided lobe). Same bars as test_gpu_parity: primary-hit IDs against the reference's
 FP64 brute force, images within relative MSE 1e-4 of the reference render at equal spp and
 seed (tests/golden/make_golden.py)."""
 
@@ -19,7 +20,8 @@ from test_gpu_parity import REL_MSE_BAR, _golden, _primary_check
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics",
+                                  "subsurface"])
 def test_primary_hits(ctx, request, name):
     from luz_amd import SceneFile
     wl = request.getfixturevalue(name)
@@ -33,7 +35,8 @@ def test_primary_hits(ctx, request, name):
 
 
 @pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
-@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics"])
+@pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics",
+                                  "subsurface"])
 def test_image_parity(ctx, request, name, integrator):
     from luz_amd import Renderer, SceneFile
     wl = request.getfixturevalue(name)

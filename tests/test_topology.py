@@ -114,14 +114,30 @@ def test_volume_and_phase_validation(tmp_path):
 
 
 def test_material_validation(tmp_path):
-    """Principled / glossy parameters outside [0,1] are rejected (Principled.cpp:35-42),
-    and subsurface scattering is outside the device path."""
+    """Principled / glossy parameters outside [0,1] are rejected (Principled.cpp:35-88), and
+    so are malformed subsurface properties (Materials.cpp:349-367, 805-818)."""
     import pytest
     from luz_amd import _lib
     text = ("[settings]\nresolution=8,8\nsamples=1\n\n[materials]\nmaterial m {{\n{props}\n}}\n\n"
             "[scene]\ncamera c {{\nposition=(0,0,0)\n}}\nobjects{{\nsphere=(0,0,5),1,material=m\n}}\n")
-    for props, code in (("type=principled\nmetallic=1.5", _lib.E_INVALID), ("type=glossy\nroughness=-0.1", _lib.E_INVALID),
-                        ("type=principled\nsubsurface=0.5", _lib.E_UNSUPPORTED)):
+    bad = ["type=principled\nmetallic=1.5", "type=glossy\nroughness=-0.1", "type=principled\nsubsurface=1.5",
+           "type=lambertian\nsubsurface=0.5", "type=lambertian\nsubsurface_scale=0.1",
+           "type=principled\nsubsurface=0.5\nsubsurface_method=volumetric", "type=principled\nsubsurface_radius=(0,0,0)",
+           "type=principled\nsubsurface_scale=0", "type=principled\nsubsurface_color=(-1,1,1)",
+           "type=principled\nsubsurface_profile=marble"]
+    for props in bad:
+        f = tmp_path / "m.luz"
+        f.write_text(text.format(props=props))
+        with pytest.raises(_lib.LuzGpuError) as e:
+            SceneFile.read(None, str(f))
+        # invalid values (std::invalid_argument) and malformed files (std::runtime_error)
+        assert e.value.code in (_lib.E_INVALID, _lib.E_IO), props
+    for props in ("type=principled\nsubsurface=0.5", "type=principled\nsss_profile=skin",
+                  "type=principled\nsubsurface=0.3\nsubsurface_method=thin_translucent"):
+        f = tmp_path / "m.luz"
+        f.write_text(text.format(props=props))
+        SceneFile.read(None, str(f))
+    for props, code in ():
         f = tmp_path / "m.luz"
         f.write_text(text.format(props=props))
         with pytest.raises(_lib.LuzGpuError) as e:
