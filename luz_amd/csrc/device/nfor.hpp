@@ -71,6 +71,11 @@ namespace luzgpu
 		return v < 0 ? 0u : (v >= static_cast<long long>(limit) ? limit - 1u : static_cast<std::uint32_t>(v));
 	}
 
+	__device__ __forceinline__ int	nforClampInt(int v, int limit)
+	{
+		return v < 0 ? 0 : (v >= limit ? limit - 1 : v);
+	}
+
 	// colorDistanceSquared, colorLuminance (NFOR.cpp:84-103)
 	__device__ __forceinline__ double	nforDist2(const double* a, const double* b)
 	{
@@ -268,88 +273,250 @@ namespace luzgpu
 		for (int j = 0; j < 3; j++) out[j] = total <= 2.220446049250313e-16 ? image[3 * c + j] : acc[j] / total;
 	}
 
-	constexpr int	kNforTile = 16;						// pixels per tile side (one lane each)
+	constexpr int	kNforTile = 16;						// pixels per tile side
+	constexpr int	kNforTilePixels = kNforTile * kNforTile;
 	constexpr int	kNforMaxPatchRadius = 8;
-	constexpr int	kNforApronMax = kNforTile + 2 * kNforMaxPatchRadius;
+	constexpr int	kNforChunk = 4;						// window displacements per pair of barriers
+	constexpr int	kNforHalfSlots = kNforSystem / 2;	// 57 system entries per half
 
-	// regressionPass (NFOR.cpp:679-737) with one lane per pixel, a 16 x 16 tile per block.
-	// The window is visited in the reference's raster order and every sum is formed in its
-	// order. The system lives in registers (78 upper-triangle entries + 36 right-hand-side
-	// entries while accumulating, the completed 12 x 15 system while solving, every index
-	// static after unrolling) and is solved by the reference's Gauss-Jordan elimination with
-	// partial pivoting (solveLinearSystemRGB, NFOR.cpp:466-516). A pair pass
-	// (regressionPairPass, :739-832) is two launches, one per bandwidth.
+	// The two halves of a pixel's system, by rows of the upper triangle (each row's entries
+	// and its three right-hand sides): 39 matrix + 18 right-hand-side entries each.
+	__host__ __device__ constexpr int	nforHalfRow(int half, int i)
+	{
+		constexpr int rows[2][6] = {{0, 3, 4, 7, 8, 11}, {1, 2, 5, 6, 9, 10}};
+		return rows[half][i];
+	}
+
+	// index of upper-triangle entry (row, col >= row) in the row-major packing
+	__host__ __device__ constexpr int	nforUpperIndex(int row, int col)
+	{
+		return row * kNforBasis - row * (row - 1) / 2 + (col - row);
+	}
+
+	// Dynamic LDS of nfor_accumulate, in doubles. Staged: the colours (three planes) and
+	// variances of the tile grown by the window and patch radii, and the 11 feature planes of
+	// the tile grown by the window radius, each slot holding the pixel of its clamped
+	// coordinate. Both forms: the patch terms and the weights of kNforChunk displacements.
+	struct	NforLdsLayout
+	{
+		int	colorWidth, featureWidth, apron;
+		std::uint32_t	colors, variances, features, pairs, weights, doubles;
+	};
+
+	__host__ __device__ inline NforLdsLayout	nforLdsLayout(std::uint32_t regressionRadius, std::uint32_t patchRadius, bool staged)
+	{
+		NforLdsLayout L;
+		L.colorWidth = staged ? kNforTile + 2 * static_cast<int>(regressionRadius + patchRadius) : 0;
+		L.featureWidth = staged ? kNforTile + 2 * static_cast<int>(regressionRadius) : 0;
+		L.apron = kNforTile + 2 * static_cast<int>(patchRadius);
+		const std::uint32_t cc = static_cast<std::uint32_t>(L.colorWidth * L.colorWidth);
+		const std::uint32_t ff = static_cast<std::uint32_t>(L.featureWidth * L.featureWidth);
+		L.colors = 0;
+		L.variances = 3 * cc;
+		L.features = L.variances + cc;
+		L.pairs = L.features + kNforFeatures * ff;
+		L.weights = L.pairs + kNforChunk * static_cast<std::uint32_t>(L.apron * L.apron);
+		L.doubles = L.weights + kNforChunk * kNforTilePixels;
+		return L;
+	}
+
+	// a half of a fresh system: the ridge 1e-4 on the diagonal, zero elsewhere
+	template <int Half>
+	__device__ __forceinline__ void	nforClearHalf(double (&acc)[kNforHalfSlots])
+	{
+		int e = 0;
+		#pragma unroll
+		for (int i = 0; i < 6; i++)
+		{
+			const int row = nforHalfRow(Half, i);
+			#pragma unroll
+			for (int col = row; col < kNforBasis; col++) acc[e++] = col == row ? 1e-4 : 0.0;
+			acc[e++] = 0.0;
+			acc[e++] = 0.0;
+			acc[e++] = 0.0;
+		}
+	}
+
+	// addRegressionSample (NFOR.cpp:582-600) for one half of the system. Each entry adds
+	// (w * basis[row]) * basis[col] with one fused multiply-add: one rounding per sample where
+	// the reference rounds the product and the sum separately, so the systems differ from
+	// the reference's by FP64 rounding only (the same entries, summed in the same order).
+	template <int Half>
+	__device__ __forceinline__ void	nforAddSample(double (&acc)[kNforHalfSlots], const double (&basis)[kNforBasis], double w, double cr,
+		double cg, double cb)
+	{
+		#pragma clang fp contract(off)
+		int e = 0;
+		#pragma unroll
+		for (int i = 0; i < 6; i++)
+		{
+			const int row = nforHalfRow(Half, i);
+			const double wb = w * basis[row];
+			#pragma unroll
+			for (int col = row; col < kNforBasis; col++)
+			{
+				acc[e] = fma(wb, basis[col], acc[e]);
+				e++;
+			}
+			acc[e] = fma(wb, cr, acc[e]);
+			acc[e + 1] = fma(wb, cg, acc[e + 1]);
+			acc[e + 2] = fma(wb, cb, acc[e + 2]);
+			e += 3;
+		}
+	}
+
+	// The half's entries into the pixel's system (slot-major: entry k of pixel p at k * n + p;
+	// entries 0..77 the upper triangle, 78 + 3 * row + channel the right-hand sides).
+	template <int Half>
+	__device__ __forceinline__ void	nforStoreHalf(const double (&acc)[kNforHalfSlots], double* __restrict__ system, size_t n, size_t pixel)
+	{
+		int e = 0;
+		#pragma unroll
+		for (int i = 0; i < 6; i++)
+		{
+			const int row = nforHalfRow(Half, i);
+			#pragma unroll
+			for (int col = row; col < kNforBasis; col++) system[static_cast<size_t>(nforUpperIndex(row, col)) * n + pixel] = acc[e++];
+			#pragma unroll
+			for (int j = 0; j < 3; j++) system[static_cast<size_t>(kNforUpper + 3 * row + j) * n + pixel] = acc[e++];
+		}
+	}
+
+	// regressionPass (NFOR.cpp:679-737), accumulation: a 16 x 16 tile per block of 512 lanes,
+	// two per pixel, each owning half of the pixel's 12 x 12 system and its right-hand sides
+	// (nforHalfRow). The halves are whole waves (lanes 0-255 and 256-511), so every branch on
+	// the half is wave-uniform and every register index static; both halves form the pixel's
+	// weights. The window is visited in the reference's raster order, so every entry is summed
+	// in the reference's order; nfor_solve then solves the systems.
 	//
 	// Patch distances: for a window displacement d, the term of patch offset o between pixel
 	// p and its sample p + d is D(p + o, p + o + d) (patchDistance, NFOR.cpp:240-311), which
-	// depends on p + o only. The block evaluates D(q, q + d) once for every q of its tile and
-	// apron into LDS, and each lane sums its (2r+1)^2 terms from there in the reference's
-	// order. Lanes whose patch or sample would be clamped at the image border take the
-	// direct, clamped form.
-	__global__ __launch_bounds__(kNforTile * kNforTile) void	nfor_regression_tile(const double* __restrict__ image,
-		const double* __restrict__ features, const double* __restrict__ variance, double* __restrict__ out, NforParams P,
+	// depends on p + o only. For kNforChunk displacements at a time the block evaluates
+	// D(q, q + d) once for every q of its tile and apron into LDS, the two halves form the
+	// chunk's weights between them (two displacements each) into LDS, and both then add the
+	// chunk's samples; three barriers fence the steps. Lanes whose patch or sample would be
+	// clamped at the image border take the direct, clamped form.
+	//
+	// Staged: the block first copies every colour, variance and feature its window reads into
+	// LDS, so the per-step reads are LDS reads (radius 9, patch radius 1: 158 KiB); the
+	// unstaged form reads them from global memory, for windows too wide for LDS.
+	template <bool Staged>
+	__global__ __launch_bounds__(2 * kNforTilePixels) void	nfor_accumulate(const double* __restrict__ image,
+		const double* __restrict__ features, const double* __restrict__ variance, double* __restrict__ system, NforParams P,
 		double bandwidth)
 	{
 		#pragma clang fp contract(off)
-		__shared__ double pairTerms[2][kNforApronMax * kNforApronMax];
-		const int tx = static_cast<int>(threadIdx.x) % kNforTile, ty = static_cast<int>(threadIdx.x) / kNforTile;
+		extern __shared__ double lds[];
+		const NforLdsLayout L = nforLdsLayout(P.regressionRadius, P.patchRadius, Staged);
+		double* const colors = lds + L.colors;
+		double* const vars = lds + L.variances;
+		double* const feats = lds + L.features;
+		double* const pairs = lds + L.pairs;
+		double* const weights = lds + L.weights;
+		const int cw = L.colorWidth, fw = L.featureWidth, apron = L.apron;
+		const int cc = cw * cw, ff = fw * fw, aa = apron * apron;
+		const int lane = static_cast<int>(threadIdx.x) % kNforTilePixels;
+		const int half = static_cast<int>(threadIdx.x) / kNforTilePixels;	// wave-uniform
+		// each 32-lane half of a wave takes tile rows h and h + 8: 8 rows of the staged feature
+		// (34 wide) and patch-term (18 wide) planes apart, their 16 doubles fall on disjoint
+		// LDS banks
+		const int tx = lane % kNforTile, ty = lane / 32 + ((lane % 32) >= kNforTile ? 8 : 0);
 		const std::uint32_t tilesX = (P.width + kNforTile - 1) / kNforTile;
 		const long long x0 = static_cast<long long>(blockIdx.x % tilesX) * kNforTile;
 		const long long y0 = static_cast<long long>(blockIdx.x / tilesX) * kNforTile;
 		const std::uint32_t x = static_cast<std::uint32_t>(x0) + tx, y = static_cast<std::uint32_t>(y0) + ty;
 		const bool active = x < P.width && y < P.height;
 		const std::uint32_t c = active ? y * P.width + x : 0u;
-		const long long pr = P.patchRadius;
-		const int apron = kNforTile + 2 * static_cast<int>(pr);
+		const long long r = P.regressionRadius, pr = P.patchRadius;
+		const long long cx0 = x0 - r - pr, cy0 = y0 - r - pr;	// staged colour region origin
+		const long long fx0 = x0 - r, fy0 = y0 - r;				// staged feature region origin
+		if constexpr (Staged)
+		{
+			for (int i = static_cast<int>(threadIdx.x); i < cc; i += 2 * kNforTilePixels)
+			{
+				const std::uint32_t g = nforClamp(cy0 + i / cw, P.height) * P.width + nforClamp(cx0 + i % cw, P.width);
+				colors[i] = image[3 * static_cast<size_t>(g)];
+				colors[cc + i] = image[3 * static_cast<size_t>(g) + 1];
+				colors[2 * cc + i] = image[3 * static_cast<size_t>(g) + 2];
+				vars[i] = variance[g];
+			}
+			for (int i = static_cast<int>(threadIdx.x); i < ff; i += 2 * kNforTilePixels)
+			{
+				const std::uint32_t g = nforClamp(fy0 + i / fw, P.height) * P.width + nforClamp(fx0 + i % fw, P.width);
+				#pragma unroll
+				for (int j = 0; j < kNforFeatures; j++) feats[j * ff + i] = features[kNforFeatures * static_cast<size_t>(g) + j];
+			}
+			__syncthreads();
+		}
 		const double patchTerms = static_cast<double>((2 * pr + 1) * (2 * pr + 1));
 		const bool patchInside = active && static_cast<long long>(x) >= pr && static_cast<long long>(y) >= pr
 			&& static_cast<long long>(x) + pr < static_cast<long long>(P.width) && static_cast<long long>(y) + pr < static_cast<long long>(P.height);
-		double upper[kNforUpper];
-		double rhs[kNforBasis][3];
-		{
-			int k = 0;
-			#pragma unroll
-			for (int row = 0; row < kNforBasis; row++)
-			{
-				#pragma unroll
-				for (int col = row; col < kNforBasis; col++) upper[k++] = (col == row) ? 1e-4 : 0.0;	// ridge
-				rhs[row][0] = rhs[row][1] = rhs[row][2] = 0.0;
-			}
-		}
-		double center[kNforFeatures];
-		#pragma unroll
-		for (int j = 0; j < kNforFeatures; j++) center[j] = features[kNforFeatures * static_cast<size_t>(c) + j];
+		double acc[kNforHalfSlots];
+		if (half == 0) nforClearHalf<0>(acc);
+		else nforClearHalf<1>(acc);
+		// 32-bit index arithmetic from here on (the image and window sizes are far below 2^31)
+		const int ix = static_cast<int>(x), iy = static_cast<int>(y);
+		const int iw = static_cast<int>(P.width), ih = static_cast<int>(P.height);
+		const int ir = static_cast<int>(r), ipr = static_cast<int>(pr);
+		const int fBase = (ty + ir) * fw + (tx + ir);				// the pixel's staged feature slot
+		const int cBase = (ty + ir + ipr) * cw + (tx + ir + ipr);	// the pixel's staged colour slot
 		const double bw2 = fmax(P.minVariance, bandwidth * bandwidth);
-		const long long r = P.regressionRadius;
-		int buf = 0;
-		for (long long oy = -r; oy <= r; oy++)
-		{
-			for (long long ox = -r; ox <= r; ox++)
+		const int side = 2 * ir + 1;
+		const int steps = side * side;
+		// the chunk's displacements, raster order: (stepY[k], stepX[k]) = (oy, ox)
+		int stepY[kNforChunk], stepX[kNforChunk];
+		// one window sample: its weight w, its basis (1, features - the pixel's) and colour
+		auto sampleAt = [&]<int Half>(int k, double w) {
+			const int oy = stepY[k], ox = stepX[k];
+			double basis[kNforBasis];
+			double cr, cg, cb;
+			basis[0] = 1.0;
+			if constexpr (Staged)
 			{
-				// D(q, q + d) over the tile and its apron (entries off the image are clamped
-				// reads whose values no lane uses); double-buffered, so one barrier per step
-				for (int i = static_cast<int>(threadIdx.x); i < apron * apron; i += kNforTile * kNforTile)
+				const int fi = fBase + oy * fw + ox;
+				const int ci = cBase + oy * cw + ox;
+				#pragma unroll
+				for (int j = 0; j < kNforFeatures; j++) basis[j + 1] = feats[j * ff + fi] - feats[j * ff + fBase];
+				cr = colors[ci];
+				cg = colors[cc + ci];
+				cb = colors[2 * cc + ci];
+			}
+			else
+			{
+				const size_t sp = static_cast<size_t>(nforClampInt(iy + oy, ih)) * P.width + nforClampInt(ix + ox, iw);
+				#pragma unroll
+				for (int j = 0; j < kNforFeatures; j++)
 				{
-					const long long qx = x0 - pr + i % apron, qy = y0 - pr + i / apron;
-					const std::uint32_t qa = nforClamp(qy, P.height) * P.width + nforClamp(qx, P.width);
-					const std::uint32_t qb = nforClamp(qy + oy, P.height) * P.width + nforClamp(qx + ox, P.width);
-					pairTerms[buf][i] = nforDist2(image + 3 * static_cast<size_t>(qa), image + 3 * static_cast<size_t>(qb))
-						/ (variance[qa] + variance[qb] + P.minVariance);
+					basis[j + 1] = features[kNforFeatures * sp + j] - features[kNforFeatures * static_cast<size_t>(c) + j];
 				}
-				__syncthreads();
-				const double* terms = pairTerms[buf];
-				buf ^= 1;
-				if (!active) continue;
-				const long long ux = static_cast<long long>(x) + ox, uy = static_cast<long long>(y) + oy;
-				const std::uint32_t sx = nforClamp(ux, P.width), sy = nforClamp(uy, P.height);
-				const size_t s = static_cast<size_t>(sy) * P.width + sx;
-				double distance = 0.0;
-				if (sx == x && sy == y)
+				cr = image[3 * sp];
+				cg = image[3 * sp + 1];
+				cb = image[3 * sp + 2];
+			}
+			nforAddSample<Half>(acc, basis, w, cr, cg, cb);
+		};
+		// colorWeightFromDistance (NFOR.cpp:313-318) of one window displacement
+		auto weightAt = [&](int k) {
+			const int ux = ix + stepX[k], uy = iy + stepY[k];
+			const int sx = nforClampInt(ux, iw), sy = nforClampInt(uy, ih);
+			double distance = 0.0;
+			if (sx == ix && sy == iy)
+			{
+				distance = 0.0;
+			}
+			else if (patchInside && ux == sx && uy == sy && ux >= ipr && uy >= ipr && ux + ipr < iw && uy + ipr < ih)
+			{
+				const double* terms = pairs + k * aa;
+				if (pr == 1)
 				{
-					distance = 0.0;
+					// the common 3 x 3 patch: the nine reads issued together, then summed in order
+					double t[9];
+					#pragma unroll
+					for (int i = 0; i < 9; i++) t[i] = terms[(ty + i / 3) * apron + (tx + i % 3)];
+					#pragma unroll
+					for (int i = 0; i < 9; i++) distance += t[i];
 				}
-				else if (patchInside && ux == sx && uy == sy && ux >= pr && uy >= pr && ux + pr < static_cast<long long>(P.width)
-					&& uy + pr < static_cast<long long>(P.height))
+				else
 				{
 					for (long long py = -pr; py <= pr; py++)
 					{
@@ -358,34 +525,112 @@ namespace luzgpu
 							distance += terms[(ty + pr + py) * apron + (tx + pr + px)];
 						}
 					}
-					distance = distance / patchTerms;
+				}
+				distance = distance / patchTerms;
+			}
+			else
+			{
+				distance = nforPatchDistance(image, variance, P, x, y, static_cast<std::uint32_t>(sx), static_cast<std::uint32_t>(sy),
+					P.patchRadius);
+			}
+			return exp(-distance / bw2);
+		};
+		int nextY = -ir, nextX = -ir;
+		for (int base = 0; base < steps; base += kNforChunk)
+		{
+			const int count = min(kNforChunk, steps - base);
+			#pragma unroll
+			for (int k = 0; k < kNforChunk; k++)
+			{
+				stepY[k] = nextY;
+				stepX[k] = nextX;
+				if (k < count && ++nextX > ir)
+				{
+					nextX = -ir;
+					nextY++;
+				}
+			}
+			// D(q, q + d) over the tile and its apron for the chunk's displacements (entries off
+			// the image are clamped reads whose values no lane uses)
+			for (int i = static_cast<int>(threadIdx.x); i < count * aa; i += 2 * kNforTilePixels)
+			{
+				const int k = i / aa, j = i % aa;
+				const int oy = stepY[k], ox = stepX[k];
+				const int jx = j % apron, jy = j / apron;
+				if constexpr (Staged)
+				{
+					const int a = (jy + ir) * cw + (jx + ir);	// q = tile origin - pr + (jx, jy)
+					const int b = a + oy * cw + ox;
+					const double dr = colors[a] - colors[b], dg = colors[cc + a] - colors[cc + b], db = colors[2 * cc + a] - colors[2 * cc + b];
+					pairs[i] = (((dr * dr) + (dg * dg) + (db * db)) / 3.0) / (vars[a] + vars[b] + P.minVariance);
 				}
 				else
 				{
-					distance = nforPatchDistance(image, variance, P, x, y, sx, sy, P.patchRadius);
-				}
-				const double w = exp(-distance / bw2);	// colorWeightFromDistance (NFOR.cpp:313-318)
-				double basis[kNforBasis];
-				basis[0] = 1.0;
-				#pragma unroll
-				for (int j = 0; j < kNforFeatures; j++) basis[j + 1] = features[kNforFeatures * s + j] - center[j];
-				const double cr = image[3 * s], cg = image[3 * s + 1], cb = image[3 * s + 2];
-				// addRegressionSample (NFOR.cpp:582-600)
-				int k = 0;
-				#pragma unroll
-				for (int row = 0; row < kNforBasis; row++)
-				{
-					const double wb = w * basis[row];
-					#pragma unroll
-					for (int col = row; col < kNforBasis; col++) upper[k++] += wb * basis[col];
-					rhs[row][0] += wb * cr;
-					rhs[row][1] += wb * cg;
-					rhs[row][2] += wb * cb;
+					const long long qx = x0 - pr + jx, qy = y0 - pr + jy;
+					const std::uint32_t qa = nforClamp(qy, P.height) * P.width + nforClamp(qx, P.width);
+					const std::uint32_t qb = nforClamp(qy + oy, P.height) * P.width + nforClamp(qx + ox, P.width);
+					pairs[i] = nforDist2(image + 3 * static_cast<size_t>(qa), image + 3 * static_cast<size_t>(qb))
+						/ (variance[qa] + variance[qb] + P.minVariance);
 				}
 			}
+			__syncthreads();
+			// the halves split the chunk's weights (steps k = half, half + 2), then both add
+			// every sample of the chunk
+			if (active)
+			{
+				if (count == kNforChunk)
+				{
+					// two independent exp chains in flight
+					const double w0 = weightAt(half), w1 = weightAt(half + 2);
+					weights[half * kNforTilePixels + lane] = w0;
+					weights[(half + 2) * kNforTilePixels + lane] = w1;
+				}
+				else
+				{
+					for (int k = half; k < count; k += 2) weights[k * kNforTilePixels + lane] = weightAt(k);
+				}
+			}
+			__syncthreads();
+			if (active)
+			{
+				if (half == 0)
+				{
+					for (int k = 0; k < count; k++) sampleAt.template operator()<0>(k, weights[k * kNforTilePixels + lane]);
+				}
+				else
+				{
+					for (int k = 0; k < count; k++) sampleAt.template operator()<1>(k, weights[k * kNforTilePixels + lane]);
+				}
+			}
+			__syncthreads();
 		}
 		if (!active) return;
-		// completeRegressionMatrix (NFOR.cpp:628-637), then solveLinearSystemRGB
+		const size_t n = static_cast<size_t>(P.width) * P.height;
+		if (half == 0) nforStoreHalf<0>(acc, system, n, c);
+		else nforStoreHalf<1>(acc, system, n, c);
+	}
+
+	// completeRegressionMatrix (NFOR.cpp:628-637), then solveLinearSystemRGB (:466-516): the
+	// reference's Gauss-Jordan elimination with partial pivoting on each pixel's system from
+	// nfor_accumulate; a singular system takes weightedAverageFallback.
+	__global__ __launch_bounds__(256) void	nfor_solve(const double* __restrict__ system, const double* __restrict__ image,
+		const double* __restrict__ variance, double* __restrict__ out, NforParams P)
+	{
+		#pragma clang fp contract(off)
+		const size_t n = static_cast<size_t>(P.width) * P.height;
+		const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+		if (c >= n) return;
+		const std::uint32_t x = static_cast<std::uint32_t>(c % P.width), y = static_cast<std::uint32_t>(c / P.width);
+		double upper[kNforUpper];
+		double rhs[kNforBasis][3];
+		#pragma unroll
+		for (int k = 0; k < kNforUpper; k++) upper[k] = system[static_cast<size_t>(k) * n + c];
+		#pragma unroll
+		for (int row = 0; row < kNforBasis; row++)
+		{
+			#pragma unroll
+			for (int j = 0; j < 3; j++) rhs[row][j] = system[static_cast<size_t>(kNforUpper + 3 * row + j) * n + c];
+		}
 		double m[kNforBasis][kNforBasis];
 		{
 			int k = 0;
@@ -460,7 +705,7 @@ namespace luzgpu
 				for (int j = 0; j < 3; j++) rhs[row][j] -= factor * rhs[pivot][j];
 			}
 		}
-		double* o = out + 3 * static_cast<size_t>(c);
+		double* o = out + 3 * c;
 		if (solved)
 		{
 			o[0] = rhs[0][0];

@@ -671,7 +671,7 @@ struct luzgpu_ctx
 	std::vector<std::pair<std::vector<std::uint64_t>, luzgpu_tuning>>	tuned;
 	luzgpu::DeviceBuffer	tuneSums;
 	// denoising: per-sample features, per-pixel statistics, NFOR inputs and temporaries
-	luzgpu::DeviceBuffer	sampleFeatures, denoiseStats, nforBuffers, nforTemp;
+	luzgpu::DeviceBuffer	sampleFeatures, denoiseStats, nforBuffers, nforTemp, nforSystems;
 	// adaptive rounds: per-pixel accumulators, sample counts, two active-pixel lists, counters
 	luzgpu::DeviceBuffer	adaptiveMem;
 	// display post-processing: the FP64 mean image and three working images
@@ -1022,16 +1022,40 @@ namespace luzgpu
 			nfor_prefilter<<<g, 256, 0, st>>>(featA, featB, featVarAvg, ffA, P);
 			nfor_prefilter<<<g, 256, 0, st>>>(featB, featA, featVarAvg, ffB, P);
 			const unsigned tiles = blocks(P.width, kNforTile) * blocks(P.height, kNforTile);
-			constexpr unsigned tb = kNforTile * kNforTile;
-			nfor_regression_tile<<<tiles, tb, 0, st>>>(supA, ffB, regVar, a0, P, 0.5);
-			nfor_regression_tile<<<tiles, tb, 0, st>>>(supA, ffB, regVar, a1, P, 1.0);
-			nfor_regression_tile<<<tiles, tb, 0, st>>>(supB, ffA, regVar, b0, P, 0.5);
-			nfor_regression_tile<<<tiles, tb, 0, st>>>(supB, ffA, regVar, b1, P, 1.0);
+			constexpr unsigned tb = 2 * kNforTilePixels;
+			// each regression's per-pixel systems (nfor_accumulate -> nfor_solve)
+			ctx->nforSystems.reserve(static_cast<size_t>(kNforSystem) * n * sizeof(double));
+			double* systems = ctx->nforSystems.as<double>();
+			// the LDS-staged accumulation when its window fits the device's LDS per block
+			// (radius 9, patch radius 1: 158 KiB of the MI355X's 160 KiB), else the unstaged form
+			const size_t stagedBytes = static_cast<size_t>(nforLdsLayout(P.regressionRadius, P.patchRadius, true).doubles) * sizeof(double);
+			const size_t plainBytes = static_cast<size_t>(nforLdsLayout(P.regressionRadius, P.patchRadius, false).doubles) * sizeof(double);
+			int device = 0, ldsLimit = 0;
+			check(hipGetDevice(&device), "hipGetDevice");
+			check(hipDeviceGetAttribute(&ldsLimit, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "LDS per block");
+			const bool staged = stagedBytes <= static_cast<size_t>(ldsLimit)
+				&& hipFuncSetAttribute(reinterpret_cast<const void*>(&nfor_accumulate<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+					static_cast<int>(stagedBytes)) == hipSuccess;
+			if (!staged && (plainBytes > static_cast<size_t>(ldsLimit)
+				|| hipFuncSetAttribute(reinterpret_cast<const void*>(&nfor_accumulate<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
+					static_cast<int>(plainBytes)) != hipSuccess))
+			{
+				throw Error{LUZGPU_E_UNSUPPORTED, "NFOR patch radius too large for the device's LDS"};
+			}
+			auto regress = [&](const double* img, const double* feat, const double* var, double* dst, double bandwidth) {
+				if (staged) nfor_accumulate<true><<<tiles, tb, stagedBytes, st>>>(img, feat, var, systems, P, bandwidth);
+				else nfor_accumulate<false><<<tiles, tb, plainBytes, st>>>(img, feat, var, systems, P, bandwidth);
+				nfor_solve<<<g, 256, 0, st>>>(systems, img, var, dst, P);
+			};
+			regress(supA, ffB, regVar, a0, 0.5);
+			regress(supA, ffB, regVar, a1, 1.0);
+			regress(supB, ffA, regVar, b0, 0.5);
+			regress(supB, ffA, regVar, b1, 1.0);
 			nfor_mse<<<g, 256, 0, st>>>(supA, supB, a0, b0, a1, b1, regVar, m0, m1, static_cast<std::uint32_t>(n));
 			nfor_filter_pair<<<g, 256, 0, st>>>(m0, m1, color, colorVariance, f0, f1, P, 0);
 			nfor_filter_pair<<<g, 256, 0, st>>>(f0, f1, color, colorVariance, s0, s1, P, 1);
 			nfor_combine<<<g, 256, 0, st>>>(s0, s1, f0, f1, a0, a1, b0, b1, ffA, ffB, combined, finalFeat, finalVar, P);
-			nfor_regression_tile<<<tiles, tb, 0, st>>>(combined, finalFeat, finalVar, out, P, 1.0);
+			regress(combined, finalFeat, finalVar, out, 1.0);
 			check(hipGetLastError(), "NFOR launch");
 		}
 

@@ -129,6 +129,8 @@ NFOR_CASES = [
     # (name, workload writer, spp, width, height, (prefilter, regression, patch) radii)
     ("nfor_cornell_48x48_8spp", "cornell", 8, 48, 48, (1, 2, 1)),
     ("nfor_primitives_64x48_8spp_r9", "primitives", 8, 64, 48, (1, 9, 1)),
+    # a window too wide for the device's LDS-staged regression (its unstaged form)
+    ("nfor_cornell_48x48_8spp_r12p2", "cornell", 8, 48, 48, (1, 12, 2)),
     # adaptive sampling on: (min samples, check interval, threshold)
     ("nfor_adaptive_cornell_48x48_48spp", "cornell", 48, 48, 48, (1, 2, 1), (8, 4, 0.2)),
 ]

@@ -4,7 +4,8 @@ Fixtures (tests/golden/make_golden.py, nfor_fixtures): the reference's own denoi
 pipeline at seed 1 (oracle/ref_harness.cpp, denoise command): its NFOR input buffers
 (even / odd half-buffer means on sampler streams 1 / 2, colour variance, 11 primary
 features and their variance), the raw mean image and Denoise::applyNFOR's output, for the
-Cornell box (default radii 1 / 2 / 1) and the primitives scene (regression radius 9).
+Cornell box (default radii 1 / 2 / 1, and regression radius 12 with patch radius 2, a window
+too wide for the LDS-staged regression) and the primitives scene (regression radius 9).
 
 * Device NFOR on the reference's buffers reproduces its output to FP64 rounding (the
   summation order differs).
@@ -21,7 +22,7 @@ from conftest import GOLDEN, rel_mse
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["nfor_cornell_48x48_8spp", "nfor_primitives_64x48_8spp_r9"]
+CASES = ["nfor_cornell_48x48_8spp", "nfor_primitives_64x48_8spp_r9", "nfor_cornell_48x48_8spp_r12p2"]
 
 
 def _fixture(name):
