@@ -12,9 +12,10 @@
 //   MSE estimation, its and the bandwidth selection's cross-bilateral filtering (:834-936);
 //   per-pixel blend, then a final regression of the blend (:1074-1101).
 //
-// The regression solves a 12 x 12 weighted least-squares system per pixel. Each lane owns
-// one pixel: it walks the window in the reference's raster order, so every sum is formed in
-// the reference's order, keeps the system in registers and solves it by the reference's
+// The regression solves a 12 x 12 weighted least-squares system per pixel. Two lanes own a
+// pixel, each half of its system in registers: they walk the window in the reference's
+// raster order, so every entry is summed in the reference's order (with a fused
+// multiply-add per term), and a second kernel solves each system by the reference's
 // Gauss-Jordan elimination with partial pivoting.
 #pragma once
 
