@@ -866,7 +866,7 @@ namespace luzgpu
 		// Picks the traversal launch shape (LDS stack depth, rays claimed per queue atomic,
 		// lane-refill thresholds, persistent grid) by timing a calibration render of the same
 		// jobs: each scene's samples cut so the whole calibration is one full pass (at most
-		// 16M paths), coordinate descent over the knobs, two sweeps, each candidate timed on
+		// kMaxPassPaths), coordinate descent over the knobs, two sweeps, each candidate timed on
 		// its second run (the first captures its graph). The image does not depend on any knob.
 		luzgpu_tuning	autotuneJobs(luzgpu_ctx* ctx, const SceneView& view, int features, const std::vector<BatchJob>& jobs)
 		{
