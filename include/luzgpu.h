@@ -274,7 +274,8 @@ typedef struct luzgpu_mesh_desc
  *   RECTANGLE    a = position, b = orientation (normal), size[0..1] = width, height
  *   INSTANCE     mesh; a = position, c = scale; cos_rotation / sin_rotation of the X, Y, Z
  *                Euler angles (MeshInstance::InstanceTransform)
- *   SPHERE       a = centre, radius
+ *   SPHERE       a = centre, radius; flags & 1: invisible (Sphere::isVisible false: never hit,
+ *                still a light when emissive)
  *   CUBE         a = position, size = width, height, depth
  *   VOLUME       flags & 1: sphere boundary (a = centre, radius), else box (a = position,
  *                size); density (per scene unit); material = the phase function

@@ -61,6 +61,8 @@ namespace luzgpu
 		GPU_HIT_VOLUME = 5,
 		GPU_HIT_PLANE = 6,		// unbounded: tested after the BVH
 		GPU_HIT_DIRECTIONAL = 7,	// unbounded light, never hit
+		GPU_HIT_HIDDEN_SPHERE = 8,	// BVH leaf entries only: an invisible sphere, bounded but
+									// never hit (its hittable and light entries say SPHERE)
 	};
 	constexpr std::uint32_t	kKindShift = 28;
 	constexpr std::uint32_t	kIndexMask = (1u << kKindShift) - 1u;

@@ -29,7 +29,7 @@ namespace luzgpu
 		const GpuTri*	topTris;
 		const GpuTri*	meshTris;
 		const float*	meshNormals;
-		const uint2*	tlasLeaf;	// top-level leaf items: (hittable index, kind << 30 | index)
+		const uint2*	tlasLeaf;	// top-level leaf items: (hittable index, kind << 28 | index)
 		const GpuHittable*	hittables;
 		const GpuRect*	rects;
 		const GpuInstance*	instances;
@@ -608,6 +608,10 @@ namespace luzgpu
 						st.hit.tri = -1;
 						if (AnyHit) return true;
 					}
+				}
+				else if ((Feat & kFeatAnalytic) && kind == GPU_HIT_HIDDEN_SPHERE)
+				{
+					// an invisible sphere (Sphere.cpp:227-233, 271-277): never hit
 				}
 				else
 				{

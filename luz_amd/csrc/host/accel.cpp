@@ -1275,7 +1275,12 @@ namespace luzgpu
 		for (const std::uint32_t h : p.tlasPrims)
 		{
 			p.tlasLeaf.push_back(h);
-			p.tlasLeaf.push_back(p.hittables[h].kindIndex);
+			std::uint32_t kindIndex = p.hittables[h].kindIndex;
+			if (scene.hittables[h].kind == HIT_SPHERE && !scene.spheres[scene.hittables[h].index].visible)
+			{
+				kindIndex = (GPU_HIT_HIDDEN_SPHERE << kKindShift) | (kindIndex & kIndexMask);
+			}
+			p.tlasLeaf.push_back(kindIndex);
 		}
 
 		// ---- lights (Scene.cpp:615-638) -----------------------------------------------------
@@ -1549,11 +1554,11 @@ namespace luzgpu
 			const std::uint32_t leafBase = static_cast<std::uint32_t>(m.tlasLeaf.size() / 2);
 			const std::uint32_t hittableBase = static_cast<std::uint32_t>(m.hittables.size());
 			const std::uint32_t materialBase = static_cast<std::uint32_t>(m.materials.size());
-			const std::uint32_t base[8] = {
+			const std::uint32_t base[9] = {
 				static_cast<std::uint32_t>(m.topTris.size()), static_cast<std::uint32_t>(m.rects.size()),
 				static_cast<std::uint32_t>(m.instances.size()), static_cast<std::uint32_t>(m.spheres.size()),
 				static_cast<std::uint32_t>(m.cubeFaces.size() / 6), static_cast<std::uint32_t>(m.volumes.size()),
-				static_cast<std::uint32_t>(m.planes.size()), 0u};
+				static_cast<std::uint32_t>(m.planes.size()), 0u, static_cast<std::uint32_t>(m.spheres.size())};
 			auto rebaseKind = [&](std::uint32_t kindIndex) {
 				const std::uint32_t kind = kindIndex >> kKindShift;
 				const std::uint32_t idx = (kindIndex & kIndexMask) + base[kind];

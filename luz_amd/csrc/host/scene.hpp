@@ -94,10 +94,13 @@ namespace luzgpu
 		HIT_DIRECTIONAL = 7,	// unbounded, never hit
 	};
 
+	// Sphere::hit / hitAny (Sphere.cpp:227-233, 271-277): an invisible sphere is never hit
+	// by any ray, but stays in the acceleration structure and, emissive, in the light list.
 	struct	Sphere
 	{
 		dvec3	position{};
 		double	radius = 1.0;
+		bool	visible = true;
 	};
 
 	// Cube.cpp:72-95: six axis-aligned rectangles around the position (the orientation

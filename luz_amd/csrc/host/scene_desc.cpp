@@ -292,6 +292,7 @@ namespace luzgpu
 					Sphere sp;
 					sp.position = d3(h.a);
 					sp.radius = h.radius;
+					sp.visible = (h.flags & 1u) == 0;
 					ht.kind = HIT_SPHERE;
 					ht.index = static_cast<std::uint32_t>(sc.spheres.size());
 					sc.spheres.push_back(sp);

@@ -2129,10 +2129,7 @@ namespace luzgpu
 						{
 							throw std::runtime_error("Sphere object '" + name + "' radius must be finite and positive.");
 						}
-						if (!visible)
-						{
-							throw std::runtime_error("Unsupported for the device path: invisible sphere (visible=0)");
-						}
+						sp.visible = visible;
 						if (mat < 0)
 						{
 							scene.materials.push_back(Material{});	// SphereBlock default: Lambertian(0.6)
@@ -2254,10 +2251,7 @@ namespace luzgpu
 						{
 							throw std::runtime_error("Unsupported for the device path: IES light profile");
 						}
-						if (!visible)
-						{
-							throw std::runtime_error("Unsupported for the device path: invisible sphere light (visible=0)");
-						}
+						sp.visible = visible;
 						const double mpu = scene.settings.metersPerUnit;
 						Material m;
 						m.kind = MAT_EMISSIVE;

@@ -245,10 +245,6 @@ namespace luzgpu_glue
 				}
 				else if (const auto* sp = dynamic_cast<const Sphere*>(h))
 				{
-					if (!sp->isVisible())
-					{
-						throw std::runtime_error("Unsupported for the device path: invisible sphere (visible=0)");
-					}
 					if (Access::iesProfiled(*sp))
 					{
 						throw std::runtime_error("Unsupported for the device path: IES light profile");
@@ -256,6 +252,7 @@ namespace luzgpu_glue
 					d.kind = LUZGPU_HIT_SPHERE;
 					put(sp->getPosition(), d.a);
 					d.radius = sp->getRadius();
+					d.flags = sp->isVisible() ? 0u : 1u;
 				}
 				else if (const auto* cu = dynamic_cast<const Cube*>(h))
 				{

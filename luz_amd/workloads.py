@@ -629,6 +629,46 @@ def write_subsurface(directory: str, width=160, height=120, spp=64, depth=8, see
     return Workload("subsurface", directory, path, width, height, spp, depth, seed, 2 * SPHERE_SLICES * (SPHERE_STACKS - 1))
 
 
+def invisible_scene_text(width=160, height=120, spp=32, depth=8, key_visible=0) -> str:
+    """Invisible spheres (visible=0): the key light is an invisible sphere light, lighting the
+    room, a rough chrome ball and a thin haze through light sampling alone (no ray ever hits
+    it, so neither the camera nor the chrome shows it); an invisible Lambertian ball stands
+    in front of the camera and must not appear or cast a shadow; a visible sphere light fills."""
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "sky=none", "background=(0.02,0.02,0.03)", "",
+        "[materials]",
+        "material floor {", "type=lambertian", "color=(0.6,0.58,0.55)", "}",
+        "material wall {", "type=lambertian", "color=(0.4,0.45,0.5)", "}",
+        "material chrome {", "type=metal", "color=(0.9,0.9,0.92)", "roughness=0.12", "}",
+        "material red {", "type=lambertian", "color=(0.8,0.15,0.1)", "}",
+        "",
+        "[scene]",
+        "camera main {", "position=(0,1.6,-6)", "direction=(0,-0.12,1)", "up=(0,1,0)", "focal_length_mm=30",
+        "sensor_width_mm=36", "sensor_height_mm=27", "pinhole=1", "focus_distance=6", "}",
+        "sphere_light key {", "position=(-1.4,2.2,-1.2)", "radius=0.35", "color=(1,0.9,0.75)", "radiance=25",
+        f"visible={key_visible}", "}",
+        "sphere_light fill {", "position=(2.2,1.2,1.8)", "radius=0.25", "color=(0.6,0.7,1)", "radiance=6", "visible=1", "}",
+        "sphere hidden_ball {", "position=(0.3,0.9,-3.2)", "radius=0.7", "material=red", "visible=0", "}",
+        "objects{",
+        "plane=0,(0,1,0),material=floor",
+        "rectangle=(0,2.5,3.5),(0,0,-1),10,5,material=wall",
+        "sphere=(0.6,0.8,0.8),0.8,material=chrome",
+        "cube=(-1.6,0.4,1.2),(0,0,-1),0.8,0.8,0.8,material=wall",
+        "volume haze {", "shape=box", "position=(0,2,0)", "size=(12,4,12)", "density=0.03",
+        "color=(0.9,0.9,0.9)", "anisotropy=0.3", "}",
+        "}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def write_invisible(directory: str, width=160, height=120, spp=32, depth=8, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "invisible.luz")
+    _write_if_changed(path, invisible_scene_text(width, height, spp, depth).encode())
+    return Workload("invisible", directory, path, width, height, spp, depth, seed, 0)
+
+
 def volume_scene_text(width=160, height=120, spp=32, depth=16) -> str:
     lines = [
         "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,

@@ -300,11 +300,12 @@ namespace oracle
 	{
 		V	c;
 		double	r = 1.0;
+		bool	visible = true;	// an invisible sphere is never hit, yet stays a light (:227-233, 271-277)
 
 		// Sphere::hit / hitAny (:226-301): unit direction, a = 1
 		bool	intersect(const Ray& ray, double tMin, double tMax, double& t) const
 		{
-			if (zeroDir(ray)) return false;
+			if (!visible || zeroDir(ray)) return false;
 			const V oc = ray.o - c;
 			const double b = dot(oc, ray.d);
 			const double cc = dot(oc, oc) - (r * r);
@@ -773,7 +774,7 @@ namespace oracle
 				const auto& sp = in.spheres[h.index];
 				t.kind = K_SPHERE;
 				t.index = static_cast<std::uint32_t>(s.spheres.size());
-				s.spheres.push_back(Sphere{fromD(sp.position), sp.radius});
+				s.spheres.push_back(Sphere{fromD(sp.position), sp.radius, sp.visible});
 			}
 			else if (h.kind == luzgpu::HIT_CUBE)
 			{

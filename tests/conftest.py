@@ -106,6 +106,12 @@ def subsurface(workload_dir):
 
 
 @pytest.fixture(scope="session")
+def invisible(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_invisible(os.path.join(workload_dir, "invisible"))
+
+
+@pytest.fixture(scope="session")
 def terrain(workload_dir):
     """The volume workload's scene: 1,002,528-triangle terrain, haze, sun (rendered at
     reduced sizes by the tests)."""

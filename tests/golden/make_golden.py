@@ -397,6 +397,7 @@ EXAMPLE_GOLDEN = {
     "rtiow": (160, 90, 32),
     "volumetric_spotlight_fog": (160, 90, 64),
     "volumetric_gray_sphere_white_bg": (96, 96, 64),
+    "volumetric_godrays": (160, 90, 32),
 }
 
 
@@ -481,6 +482,8 @@ def main():
             small_scene_fixtures("optics", workloads.write_optics(os.path.join(tmp, "optics")), tmp)
         if "subsurface" in only:
             small_scene_fixtures("subsurface", workloads.write_subsurface(os.path.join(tmp, "subsurface")), tmp)
+        if "invisible" in only:
+            small_scene_fixtures("invisible", workloads.write_invisible(os.path.join(tmp, "invisible")), tmp)
         if "meshlight" in only:
             small_scene_fixtures("meshlight", workloads.write_meshlight(os.path.join(tmp, "meshlight")), tmp)
         if "fd" in only:
@@ -516,6 +519,7 @@ def main():
     small_scene_fixtures("atmosphere_env", workloads.write_atmosphere_env(os.path.join(tmp, "atmosphere_env")), tmp)
     small_scene_fixtures("optics", workloads.write_optics(os.path.join(tmp, "optics")), tmp)
     small_scene_fixtures("subsurface", workloads.write_subsurface(os.path.join(tmp, "subsurface")), tmp)
+    small_scene_fixtures("invisible", workloads.write_invisible(os.path.join(tmp, "invisible")), tmp)
     fd_fixture(tmp)
     fd_rough_fixture(tmp)
     terrain_fixtures(tmp)

@@ -36,7 +36,8 @@ EXAMPLES = example_scenes()
 
 def test_example_fixtures_present():
     names = {e[0] for e in EXAMPLES}
-    assert {"cornell", "suzanne", "rtiow", "volumetric_spotlight_fog", "volumetric_gray_sphere_white_bg"} <= names
+    assert {"cornell", "suzanne", "rtiow", "volumetric_spotlight_fog", "volumetric_gray_sphere_white_bg",
+            "volumetric_godrays"} <= names
 
 
 @pytest.mark.parametrize("example", EXAMPLES, ids=[e[0] for e in EXAMPLES])
@@ -45,7 +46,9 @@ def test_example_scene_matches_reference(example):
     luminance units at meters_per_unit 0.01, sphere blocks; suzanne: [meshes], an area light
     with a blackbody colour, a point light in lumens, an inline material; rtiow: sphere
     blocks, a thin lens; volumetric_spotlight_fog: an area light in a Henyey-Greenstein box;
-    volumetric_gray_sphere_white_bg: an isotropic sphere volume on a white background."""
+    volumetric_gray_sphere_white_bg: an isotropic sphere volume on a white background;
+    volumetric_godrays: an invisible sphere light, a blackbody sun, a mist preset scaled by
+    density_scale, a metal preset, inline materials."""
     name, path, w, h, spp = example
     ref = np.load(os.path.join(GOLDEN, f"example_{name}_{w}x{h}_{spp}spp_seed1.npz"))["image"]
     img, _ = oracle_lib.render(path, 1, spp, w, h)
@@ -60,12 +63,12 @@ def test_example_scene_matches_reference(example):
 @pytest.mark.skipif(not os.path.isdir(REFERENCE_EXAMPLES), reason="the reference's example scenes are not here")
 def test_reference_example_scenes_load():
     """Every example scene either loads, or names the one feature the device path does not
-    take (textures, photon-map caustics, an invisible sphere light)."""
+    take (textures, photon-map caustics)."""
     from luz_amd import SceneFile
-    for name in ("cornell", "suzanne", "rtiow", "volumetric_spotlight_fog", "volumetric_gray_sphere_white_bg"):
+    for name in ("cornell", "suzanne", "rtiow", "volumetric_spotlight_fog", "volumetric_gray_sphere_white_bg",
+                 "volumetric_godrays"):
         SceneFile.read(None, os.path.join(REFERENCE_EXAMPLES, f"{name}.luz"))
-    refused = {"earth": "textures", "earth_atmosphere": "textures", "underwater_caustics": "caustics",
-               "volumetric_godrays": "visible=0"}
+    refused = {"earth": "textures", "earth_atmosphere": "textures", "underwater_caustics": "caustics"}
     for name, what in refused.items():
         with pytest.raises(Exception, match=re.escape(what)):
             SceneFile.read(None, os.path.join(REFERENCE_EXAMPLES, f"{name}.luz"))

@@ -6,8 +6,10 @@ sphere light), ``materials`` (glossy, diffuse-glossy and principled variants) an
 ``meshlight`` (emissive mesh instances under uniform and non-uniform scales, sampled by
 triangle area and re-intersected for their MIS pdf) and ``optics`` (the reference's default
 thin-lens camera, rough dielectrics, conductor-Fresnel metals, dispersive glass presets and
-interior absorption) and ``subsurface`` (Burley exits relocated by a probe ray inside the
-shading kernels, the skin profile, the thin two-sided lobe). Same bars as test_gpu_parity: primary-hit IDs against the reference's
+interior absorption), ``subsurface`` (Burley exits relocated by a probe ray inside the
+shading kernels, the skin profile, the thin two-sided lobe) and ``invisible`` (an invisible
+sphere light that lights the room through light sampling alone, an invisible ball before the
+camera). Same bars as test_gpu_parity: primary-hit IDs against the reference's
 FP64 brute force, images within relative MSE 1e-4 of the reference render at equal spp and
 seed (tests/golden/make_golden.py)."""
 
@@ -21,7 +23,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics",
-                                  "subsurface"])
+                                  "subsurface", "invisible"])
 def test_primary_hits(ctx, request, name):
     from luz_amd import SceneFile
     wl = request.getfixturevalue(name)
@@ -36,7 +38,7 @@ def test_primary_hits(ctx, request, name):
 
 @pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
 @pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics",
-                                  "subsurface"])
+                                  "subsurface", "invisible"])
 def test_image_parity(ctx, request, name, integrator):
     from luz_amd import Renderer, SceneFile
     wl = request.getfixturevalue(name)

@@ -5,6 +5,7 @@ forms pin what the goldens leave open."""
 
 import math
 import os
+import re
 
 import numpy as np
 
@@ -209,6 +210,47 @@ def test_subsurface_image_matches_reference(subsurface):
     g = np.load(os.path.join(GOLDEN, "subsurface_primary.npz"))
     ids, _ = oracle_lib.primary(subsurface.scene_path, 1, subsurface.width, subsurface.height)
     assert np.array_equal(ids[..., 0], g["top"]) and np.array_equal(ids[..., 1], g["tri"])
+
+
+def test_invisible_spheres_match_reference(invisible):
+    """Invisible spheres (Sphere.cpp:227-233, 271-277): the key light is an invisible sphere
+    light that still lights the scene through light sampling, and an invisible ball stands
+    in front of the camera; the restatement reproduces the reference's FP64 render and its
+    primary hits (which never name either sphere)."""
+    ref = np.load(os.path.join(GOLDEN, f"invisible_{invisible.width}x{invisible.height}_{invisible.spp}spp_seed1.npz"))["image"]
+    img, _ = oracle_lib.render(invisible.scene_path, 1, invisible.spp, invisible.width, invisible.height)
+    e = rel_mse(img, ref)
+    print(f"oracle vs reference (invisible): rel-MSE {e:.3e}")
+    assert e < 1e-12
+    g = np.load(os.path.join(GOLDEN, "invisible_primary.npz"))
+    ids, _ = oracle_lib.primary(invisible.scene_path, 1, invisible.width, invisible.height)
+    assert np.array_equal(ids[..., 0], g["top"]) and np.array_equal(ids[..., 1], g["tri"])
+    # hittables in file order: key light 0, fill light 1, hidden ball 2
+    assert not np.isin(g["top"], [0, 2]).any()
+
+
+def test_invisible_sphere_identities(tmp_path, invisible):
+    """A non-emissive invisible sphere is the same as no sphere at all: it is never hit,
+    casts no shadow and is not a light, so removing it leaves the image bit for bit. An
+    invisible sphere light still lights the scene (removing it changes the image), but a
+    ray never reaches it: making it visible changes the image too (its own glow, its
+    reflection in the chrome ball, the MIS share of scattered rays that hit it)."""
+    from luz_amd import workloads
+    text = open(invisible.scene_path).read()
+    ball = "sphere hidden_ball {\nposition=(0.3,0.9,-3.2)\nradius=0.7\nmaterial=red\nvisible=0\n}\n"
+    assert ball in text
+    variants = {"base": text, "no_ball": text.replace(ball, ""),
+                "no_key": re.sub(r"sphere_light key \{.*?\}\n", "", text, flags=re.S),
+                "visible_key": workloads.invisible_scene_text(invisible.width, invisible.height, invisible.spp,
+                                                              invisible.depth, key_visible=1)}
+    out = {}
+    for name, t in variants.items():
+        p = tmp_path / f"{name}.luz"
+        p.write_text(t)
+        out[name] = oracle_lib.render(str(p), 1, 8, 64, 48)[0]
+    assert np.array_equal(out["base"], out["no_ball"])
+    assert rel_mse(out["no_key"], out["base"]) > 1e-2
+    assert rel_mse(out["visible_key"], out["base"]) > 1e-3
 
 
 def test_subsurface_weight_zero_is_plain_principled(tmp_path, subsurface):

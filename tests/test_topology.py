@@ -87,6 +87,16 @@ def test_mesh_light_scene_matches_reference(meshlight):
     assert m.info["hittables"] == 6 and m.info["lights"] == 3 and m.info["instances"] == 4
 
 
+def test_invisible_spheres_stay_in_the_tlas_and_the_light_list(invisible):
+    """Invisible spheres keep their place in the reference's BVH (the builder sees every
+    bounded hittable) and an invisible sphere light stays a light; only ray hits skip them."""
+    s = SceneFile.read(None, invisible.scene_path)
+    assert s.tlas_topology() == open(os.path.join(GOLDEN, "invisible_tlas.txt")).read().strip()
+    # key light, fill light, hidden ball, the wall, the chrome ball, the cube and the haze
+    # are bounded; the plane is not. Lights: both sphere lights.
+    assert s.info["hittables"] == 8 and s.info["lights"] == 2
+
+
 def test_volume_and_phase_validation(tmp_path):
     """HenyeyGreenstein.cpp:10-18 and Objects.cpp:491-497, 746-800 reject these inputs."""
     import pytest
