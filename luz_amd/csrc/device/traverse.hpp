@@ -431,18 +431,15 @@ namespace luzgpu
 			if (Count) counter->nodes++;
 			// decode the four child boxes (plane = fma(q, 2^exp, origin)) and test them
 			const float org[3] = {__uint_as_float(n0.x), __uint_as_float(n0.y), __uint_as_float(n0.z)};
-			float sc[3];
-			for (int a = 0; a < 3; a++)
-			{
-				const int ex = static_cast<int>(static_cast<std::int8_t>((n0.w >> (8 * a)) & 0xffu));
-				sc[a] = __int_as_float((ex + 127) << 23);
-			}
+			int ex[3];
+			for (int a = 0; a < 3; a++) ex[a] = static_cast<int>(static_cast<std::int8_t>((n0.w >> (8 * a)) & 0xffu));
 			const std::uint32_t qlo[3] = {n1.x, n1.y, n1.z};
 			const std::uint32_t qhi[3] = {n1.w, n2.x, n2.y};
 			const std::uint32_t child[4] = {n2.z, n2.w, n3.x, n3.y};
-			std::uint32_t e[4];
-			float tn[4];
-			int hits = 0;
+			// per child: the entry distance as an ordering key (hit distances are >= T_MIN > 0,
+			// so their bit patterns order like the floats; a miss sorts last) and the stack entry
+			constexpr std::uint32_t kMissKey = 0xffffffffu;
+			std::uint32_t key[4], e[4];
 			const f3 inv = st.ray.inv;
 			if (inv.x != 0.0f && inv.y != 0.0f && inv.z != 0.0f)
 			{
@@ -450,36 +447,43 @@ namespace luzgpu
 				// b = (origin - o) / d per axis, a plane at q is at t = fma(q, s, b). Rounding
 				// differs from decoding the plane first by a few ulps, so the interval is
 				// widened by 2^-20 relative (the quantised boxes already enclose the FP64 ones).
-				const float s0 = sc[0] * inv.x, s1 = sc[1] * inv.y, s2 = sc[2] * inv.z;
+				// 2^exp is a normal power of two (exp in [-100, 127]), so ldexp(1 / d, exp) is
+				// exactly 2^exp * (1 / d).
+				const float s0 = __builtin_amdgcn_ldexpf(inv.x, ex[0]), s1 = __builtin_amdgcn_ldexpf(inv.y, ex[1]),
+					s2 = __builtin_amdgcn_ldexpf(inv.z, ex[2]);
 				const float b0 = (org[0] - st.ray.o.x) * inv.x, b1 = (org[1] - st.ray.o.y) * inv.y, b2 = (org[2] - st.ray.o.z) * inv.z;
+				// fma(q, s, b) is monotone in q (increasing for s >= 0), so on each axis the
+				// q_lo plane is the near one when the direction is positive and the q_hi plane
+				// otherwise: choosing the words once per node gives the min / max of each pair
+				const std::uint32_t near0 = inv.x > 0.0f ? qlo[0] : qhi[0], far0 = inv.x > 0.0f ? qhi[0] : qlo[0];
+				const std::uint32_t near1 = inv.y > 0.0f ? qlo[1] : qhi[1], far1 = inv.y > 0.0f ? qhi[1] : qlo[1];
+				const std::uint32_t near2 = inv.z > 0.0f ? qlo[2] : qhi[2], far2 = inv.z > 0.0f ? qhi[2] : qlo[2];
+				#pragma unroll
 				for (int k = 0; k < 4; k++)
 				{
 					const std::uint32_t cnt = (n3.z >> (8 * k)) & 0xffu;
-					if (cnt == 255u) continue;
-					const float l0 = fmaf(static_cast<float>((qlo[0] >> (8 * k)) & 0xffu), s0, b0);
-					const float h0 = fmaf(static_cast<float>((qhi[0] >> (8 * k)) & 0xffu), s0, b0);
-					const float l1 = fmaf(static_cast<float>((qlo[1] >> (8 * k)) & 0xffu), s1, b1);
-					const float h1 = fmaf(static_cast<float>((qhi[1] >> (8 * k)) & 0xffu), s1, b1);
-					const float l2 = fmaf(static_cast<float>((qlo[2] >> (8 * k)) & 0xffu), s2, b2);
-					const float h2 = fmaf(static_cast<float>((qhi[2] >> (8 * k)) & 0xffu), s2, b2);
-					const float near = fmaxf(fmaxf(fminf(l0, h0), fminf(l1, h1)), fminf(l2, h2)) * (1.0f - 0x1p-20f);
-					const float far = fminf(fminf(fmaxf(l0, h0), fmaxf(l1, h1)), fmaxf(l2, h2)) * (1.0f + 0x1p-20f);
+					const float l0 = fmaf(static_cast<float>((near0 >> (8 * k)) & 0xffu), s0, b0);
+					const float h0 = fmaf(static_cast<float>((far0 >> (8 * k)) & 0xffu), s0, b0);
+					const float l1 = fmaf(static_cast<float>((near1 >> (8 * k)) & 0xffu), s1, b1);
+					const float h1 = fmaf(static_cast<float>((far1 >> (8 * k)) & 0xffu), s1, b1);
+					const float l2 = fmaf(static_cast<float>((near2 >> (8 * k)) & 0xffu), s2, b2);
+					const float h2 = fmaf(static_cast<float>((far2 >> (8 * k)) & 0xffu), s2, b2);
+					const float near = fmaxf(fmaxf(l0, l1), l2) * (1.0f - 0x1p-20f);
+					const float far = fminf(fminf(h0, h1), h2) * (1.0f + 0x1p-20f);
 					const float t = fmaxf(near, kTMinF);
-					if (fminf(far, st.hit.t) > t)
-					{
-						tn[hits] = t;
-						e[hits] = cnt > 0 ? leafEntry(static_cast<std::int32_t>(child[k]), static_cast<std::int32_t>(cnt)) : child[k];
-						hits++;
-					}
+					key[k] = cnt != 255u && fminf(far, st.hit.t) > t ? __float_as_uint(t) : kMissKey;
+					e[k] = cnt > 0 ? leafEntry(static_cast<std::int32_t>(child[k]), static_cast<std::int32_t>(cnt)) : child[k];
 				}
 			}
 			else
 			{
 				// an axis-parallel ray: AABB::hit tests the origin against that slab (AABB.cpp:37-80)
+				float sc[3];
+				for (int a = 0; a < 3; a++) sc[a] = __int_as_float((ex[a] + 127) << 23);
+				#pragma unroll
 				for (int k = 0; k < 4; k++)
 				{
 					const std::uint32_t cnt = (n3.z >> (8 * k)) & 0xffu;
-					if (cnt == 255u) continue;
 					float lo[3], hi[3];
 					for (int a = 0; a < 3; a++)
 					{
@@ -487,36 +491,55 @@ namespace luzgpu
 						hi[a] = fmaf(static_cast<float>((qhi[a] >> (8 * k)) & 0xffu), sc[a], org[a]);
 					}
 					float t;
-					if (boxHit(st.ray, lo, hi, st.hit.t, t))
-					{
-						tn[hits] = t;
-						e[hits] = cnt > 0 ? leafEntry(static_cast<std::int32_t>(child[k]), static_cast<std::int32_t>(cnt)) : child[k];
-						hits++;
-					}
+					key[k] = cnt != 255u && boxHit(st.ray, lo, hi, st.hit.t, t) ? __float_as_uint(t) : kMissKey;
+					e[k] = cnt > 0 ? leafEntry(static_cast<std::int32_t>(child[k]), static_cast<std::int32_t>(cnt)) : child[k];
 				}
 			}
-			if (hits > 0)
+			if constexpr (!AnyHit)
 			{
-				// closest hit: order by entry distance (insertion sort of <= 4), push far to
-				// near; an any-hit query ends at the first hit, so its order does not matter
-				// (BVHNode::hitAny is unordered, BVHNode.cpp:326-380)
-				for (int i = 1; i < (AnyHit ? 0 : hits); i++)
+				// closest hit: the hit children by entry distance, in child order among equal
+				// distances (a compare-exchange network of adjacent pairs is a stable sort),
+				// pushed far to near, the nearest taken next
+				auto exchange = [&](int i) {
+					const bool swap = key[i] > key[i + 1];
+					const std::uint32_t ki = key[i], ei = e[i];
+					key[i] = swap ? key[i + 1] : ki;
+					e[i] = swap ? e[i + 1] : ei;
+					key[i + 1] = swap ? ki : key[i + 1];
+					e[i + 1] = swap ? ei : e[i + 1];
+				};
+				exchange(0); exchange(1); exchange(0); exchange(2); exchange(1); exchange(0);
+				if (key[0] != kMissKey)
 				{
-					const float kt = tn[i];
-					const std::uint32_t ke = e[i];
-					int j = i - 1;
-					while (j >= 0 && tn[j] > kt)
-					{
-						tn[j + 1] = tn[j];
-						e[j + 1] = e[j];
-						j--;
-					}
-					tn[j + 1] = kt;
-					e[j + 1] = ke;
+					if (key[3] != kMissKey) stack.push(e[3]);
+					if (key[2] != kMissKey) stack.push(e[2]);
+					if (key[1] != kMissKey) stack.push(e[1]);
+					st.entry = e[0];
+					return false;
 				}
-				for (int i = hits - 1; i > 0; i--) stack.push(e[i]);
-				st.entry = e[0];
-				return false;
+			}
+			else
+			{
+				// an any-hit query ends at the first hit, so its order does not matter
+				// (BVHNode::hitAny is unordered, BVHNode.cpp:326-380): the lowest hit child is
+				// taken next, the others pushed
+				bool have = false;
+				std::uint32_t next = 0;
+				#pragma unroll
+				for (int k = 3; k >= 0; k--)
+				{
+					if (key[k] != kMissKey)
+					{
+						if (have) stack.push(next);
+						next = e[k];
+						have = true;
+					}
+				}
+				if (have)
+				{
+					st.entry = next;
+					return false;
+				}
 			}
 		}
 		else
