@@ -138,6 +138,12 @@ namespace luzgpu
 	// faster than 1024 blocks on the sphere frame), whole-wave claims of 64 rays
 	constexpr unsigned	kDefaultGrid = 1280;
 	constexpr int	kDefaultRefill = 64;
+	// Lanes whose next entry is a leaf (primitive tests, an instance entry) wait while other
+	// lanes of the wave still have internal nodes to open, until this many are waiting or no
+	// lane has a node left; then every live lane takes one step. Each lane's entries are
+	// processed in the same order as without the wait, so results do not depend on it; it
+	// only lets the wave run the node code and the leaf code less often side by side.
+	constexpr int	kLeafBatch = 16;
 
 	__device__ __forceinline__ std::uint32_t	laneId() { return threadIdx.x & 63u; }
 
@@ -350,6 +356,7 @@ namespace luzgpu
 				{
 					break;
 				}
+				bool done = true;
 				if (k != 0xffffffffu)
 				{
 					Ray r;
@@ -363,12 +370,25 @@ namespace luzgpu
 						lastRoot = jp.tlasRoot;
 						lastUnbounded = unboundedOf(jp);
 					}
-					if (traceBegin(st, s, lastRoot, lastUnbounded, r, tMax, logU, from, stack))
+					done = !traceBegin(st, s, lastRoot, lastUnbounded, r, tMax, logU, from, stack);
+				}
+				for (;;)
+				{
+					const bool wantNode = !done && (st.entry & kLeafBit) == 0u;
+					const unsigned long long live = __ballot(!done);
+					if (live == 0ull)
 					{
-						while (!traceStep<AnyHit, Count, Feat>(s, st, kTMinF, stack, &visits))
-						{
-						}
+						break;
 					}
+					const unsigned long long nodes = __ballot(wantNode);
+					const bool leafTurn = nodes == 0ull || __popcll(live & ~nodes) >= kLeafBatch;
+					if (leafTurn ? !done : wantNode)
+					{
+						done = traceStep<AnyHit, Count, Feat>(s, st, kTMinF, stack, &visits);
+					}
+				}
+				if (k != 0xffffffffu)
+				{
 					src.store(k, st);
 					rays++;
 				}
@@ -408,7 +428,10 @@ namespace luzgpu
 			{
 				break;
 			}
-			if (active && traceStep<AnyHit, Count, Feat>(s, st, kTMinF, stack, &visits))
+			const bool wantNode = active && (st.entry & kLeafBit) == 0u;
+			const unsigned long long nodes = __ballot(wantNode);
+			const bool leafTurn = nodes == 0ull || __popcll(__ballot(active) & ~nodes) >= kLeafBatch;
+			if ((leafTurn ? active : wantNode) && traceStep<AnyHit, Count, Feat>(s, st, kTMinF, stack, &visits))
 			{
 				src.store(slot, st);
 				rays++;
@@ -476,8 +499,10 @@ namespace luzgpu
 		traceQueue<false, Count, Feat, LdsDepth>(s, jobs, src, w.counts + 2 * kMaxBounceSlots + b, refillAt, chunk, counters, 0);
 	}
 
+	// The plain specialisation is asked for 4 waves per SIMD (128 VGPRs, a few spills): its
+	// loads of per-path state are latency-bound, and 2 waves left them exposed.
 	template <int Feat>
-	__global__ __launch_bounds__(kWaveBlock) void	wf_shade(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
+	__global__ __launch_bounds__(kWaveBlock, (Feat == 0 ? 4 : 1)) void	wf_shade(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
 		std::uint32_t volumes)
 	{
 		const std::uint32_t n = w.counts[b];
