@@ -36,7 +36,7 @@ namespace luzgpu
 		DIM_SUBSURFACE_THIN_SIDE = 19,
 	};
 
-	__host__ __device__ inline std::uint64_t	splitMix64(std::uint64_t v)
+	__host__ __device__ constexpr std::uint64_t	splitMix64(std::uint64_t v)
 	{
 		v += 0x9e3779b97f4a7c15ull;
 		v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -44,7 +44,7 @@ namespace luzgpu
 		return (v ^ (v >> 31));
 	}
 
-	__host__ __device__ inline double	unitFromHash32(std::uint32_t h)
+	__host__ __device__ constexpr double	unitFromHash32(std::uint32_t h)
 	{
 		return (static_cast<double>(h >> 8) * (1.0 / 16777216.0));
 	}
@@ -54,7 +54,7 @@ namespace luzgpu
 		return (static_cast<double>(h >> 11) * (1.0 / 9007199254740992.0));
 	}
 
-	__host__ __device__ inline double	dimensionStep(std::uint32_t dim, std::uint32_t component)
+	__host__ __device__ constexpr double	dimensionStep(std::uint32_t dim, std::uint32_t component)
 	{
 #pragma clang fp contract(off)
 		const std::uint64_t key = (static_cast<std::uint64_t>(dim) + 1ull) * 0x9e3779b97f4a7c15ull
@@ -62,10 +62,36 @@ namespace luzgpu
 		double step = unitFromHash32(static_cast<std::uint32_t>(splitMix64(key) >> 32));
 		if (step < 0.05 || step > 0.95)
 		{
+			// x - floor(x) for x = step + 0.618...: x lies in [0.618, 0.668) or (1.568, 1.618),
+			// so its floor is 0 or 1 and x - 1 is exact
 			const double x = step + 0.6180339887498948;
-			step = x - floor(x);
+			step = x >= 1.0 ? x - 1.0 : x;
 		}
 		return (step);
+	}
+
+	// dimensionStep depends on the dimension key and the component alone: the steps of the
+	// first 64 bounces (keys below 2048) are evaluated at compile time, the rest when used.
+	constexpr std::uint32_t	kStepTableKeys = 32u * 64u;
+	struct	DimensionSteps
+	{
+		double	v[2 * kStepTableKeys];
+	};
+	constexpr DimensionSteps	makeDimensionSteps()
+	{
+		DimensionSteps t{};
+		for (std::uint32_t k = 0; k < kStepTableKeys; k++)
+		{
+			t.v[2 * k] = dimensionStep(k, 0);
+			t.v[2 * k + 1] = dimensionStep(k, 1);
+		}
+		return t;
+	}
+	static __constant__ DimensionSteps	kDimensionSteps = makeDimensionSteps();
+
+	__device__ __forceinline__ double	stepOf(std::uint32_t dimKey, std::uint32_t component)
+	{
+		return dimKey < kStepTableKeys ? kDimensionSteps.v[2 * dimKey + component] : dimensionStep(dimKey, component);
 	}
 
 	struct	PixelSampler
@@ -103,7 +129,7 @@ namespace luzgpu
 			return (unitFromHash64(splitMix64(key)));
 		}
 
-		__host__ __device__ double	sample1D(std::uint32_t dim) const
+		__device__ double	sample1D(std::uint32_t dim) const
 		{
 #pragma clang fp contract(off)
 			const std::uint32_t dk = dim + bounce * 32u;
@@ -112,14 +138,14 @@ namespace luzgpu
 				return (hashed(dk, 0));
 			}
 			const double shift = unitFromHash32(static_cast<std::uint32_t>(splitMix64(contextKey(dk)) >> 32));
-			const double step = dimensionStep(dk, 0);
+			const double step = stepOf(dk, 0);
 			const double s = static_cast<double>(sampleIndex) + 1.0;
 			const double prod = s * step;
 			const double x = (0.5 + prod) + shift;
 			return (x - floor(x));
 		}
 
-		__host__ __device__ void	sample2D(std::uint32_t dim, float& sx, float& sy) const
+		__device__ void	sample2D(std::uint32_t dim, float& sx, float& sy) const
 		{
 #pragma clang fp contract(off)
 			const std::uint32_t dk = dim + bounce * 32u;
@@ -132,8 +158,8 @@ namespace luzgpu
 			const std::uint64_t key = contextKey(dk);
 			const double shiftX = unitFromHash32(static_cast<std::uint32_t>(splitMix64(key) >> 32));
 			const double shiftY = unitFromHash32(static_cast<std::uint32_t>(splitMix64(key ^ 0xda942042e4dd58b5ull) >> 32));
-			const double stepX = dimensionStep(dk, 0);
-			const double stepY = dimensionStep(dk, 1);
+			const double stepX = stepOf(dk, 0);
+			const double stepY = stepOf(dk, 1);
 			const double s = static_cast<double>(sampleIndex) + 1.0;
 			const double px = s * stepX;
 			const double py = s * stepY;
