@@ -147,22 +147,6 @@ namespace luzgpu
 
 	__device__ __forceinline__ std::uint32_t	laneId() { return threadIdx.x & 63u; }
 
-	// Appends `want` (0/1) items per lane with a single atomic per wave; returns the slot.
-	__device__ __forceinline__ std::uint32_t	waveAppend(bool want, std::uint32_t* counter)
-	{
-		const unsigned long long mask = __ballot(want);
-		const std::uint32_t total = static_cast<std::uint32_t>(__popcll(mask));
-		std::uint32_t base = 0;
-		const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-		if (total > 0 && static_cast<int>(laneId()) == leader)
-		{
-			base = atomicAdd(counter, total);
-		}
-		base = __shfl(base, leader < 0 ? 0 : leader);
-		const unsigned long long below = mask & ((1ull << laneId()) - 1ull);
-		return base + static_cast<std::uint32_t>(__popcll(below));
-	}
-
 	// The pass's total path count: written into counts[0] (queue 0 holds every path).
 	__global__ void	wf_zero_counts(std::uint32_t* counts, std::uint32_t n, const PassJob* jobs, std::uint32_t jobCount)
 	{
@@ -509,12 +493,19 @@ namespace luzgpu
 		const int q = b & 1;
 		std::uint32_t* nextCount = w.counts + b + 1;
 		std::uint32_t* shadowCount = w.counts + kMaxBounceSlots + b;
-		// grid-stride loop with a wave-uniform trip count so every lane reaches the ballots
+		// Queue appends: each wave counts its continuing paths and its shadow rays (light
+		// connections, then environment ones) in LDS, and one thread claims the block's
+		// ranges of the next queue and the shadow queue with one atomic each. The counters
+		// are shared by every XCD, so per-wave atomics on them queue up behind each other.
+		constexpr int kWaves = kWaveBlock / 64;
+		__shared__ std::uint32_t waveNext[kWaves], waveShadow[kWaves], blockNext, blockShadow;
+		const int wave = static_cast<int>(threadIdx.x >> 6);
+		// grid-stride loop with a block-uniform trip count so every lane reaches the ballots
+		// and every wave the barriers
 		const std::uint32_t stride = gridDim.x * kWaveBlock;
-		const std::uint32_t first = blockIdx.x * kWaveBlock + (threadIdx.x & ~63u);
-		for (std::uint32_t waveBase = first; waveBase < n; waveBase += stride)
+		for (std::uint32_t blockBase = blockIdx.x * kWaveBlock; blockBase < n; blockBase += stride)
 		{
-			const std::uint32_t i = waveBase + laneId();
+			const std::uint32_t i = blockBase + threadIdx.x;
 			const bool valid = i < n;
 			bool cont = false, wantLight = false, wantEnv = false;
 			Ray next{};
@@ -718,7 +709,34 @@ namespace luzgpu
 				}
 				pending &= ~__ballot(mine);
 			}
-			const std::uint32_t pos = waveAppend(cont, nextCount);
+			const unsigned long long mNext = __ballot(cont), mLight = __ballot(wantLight), mEnv = __ballot(wantEnv);
+			const std::uint32_t nLight = static_cast<std::uint32_t>(__popcll(mLight));
+			if (laneId() == 0)
+			{
+				waveNext[wave] = static_cast<std::uint32_t>(__popcll(mNext));
+				waveShadow[wave] = nLight + static_cast<std::uint32_t>(__popcll(mEnv));
+			}
+			__syncthreads();
+			if (threadIdx.x == 0)
+			{
+				std::uint32_t tNext = 0, tShadow = 0;
+				for (int k = 0; k < kWaves; k++)
+				{
+					tNext += waveNext[k];
+					tShadow += waveShadow[k];
+				}
+				blockNext = tNext > 0 ? atomicAdd(nextCount, tNext) : 0u;
+				blockShadow = tShadow > 0 ? atomicAdd(shadowCount, tShadow) : 0u;
+			}
+			__syncthreads();
+			std::uint32_t nextBase = blockNext, shadowBase = blockShadow;
+			for (int k = 0; k < wave; k++)
+			{
+				nextBase += waveNext[k];
+				shadowBase += waveShadow[k];
+			}
+			const unsigned long long below = (1ull << laneId()) - 1ull;
+			const std::uint32_t pos = nextBase + static_cast<std::uint32_t>(__popcll(mNext & below));
 			if (cont)
 			{
 				const int nq = (b + 1) & 1;
@@ -728,7 +746,7 @@ namespace luzgpu
 				w.qPath[nq][pos] = qp;
 			}
 			const std::uint32_t tagScene = job;
-			const std::uint32_t sl = waveAppend(wantLight, shadowCount);
+			const std::uint32_t sl = shadowBase + static_cast<std::uint32_t>(__popcll(mLight & below));
 			if (wantLight)
 			{
 				w.sOrigin[sl] = make_float4(shadePoint.x, shadePoint.y, shadePoint.z, cl.tMax);
@@ -736,7 +754,7 @@ namespace luzgpu
 				w.sFrom[sl] = make_int2(static_cast<int>(from.hittable), from.tri);
 				if (volumes) w.sLogU[sl] = shadowLogU;
 			}
-			const std::uint32_t se = waveAppend(wantEnv, shadowCount);
+			const std::uint32_t se = shadowBase + nLight + static_cast<std::uint32_t>(__popcll(mEnv & below));
 			if (wantEnv)
 			{
 				w.sOrigin[se] = make_float4(shadePoint.x, shadePoint.y, shadePoint.z, INFINITY);
@@ -744,6 +762,8 @@ namespace luzgpu
 				w.sFrom[se] = make_int2(static_cast<int>(from.hittable), from.tri);
 				if (volumes) w.sLogU[se] = shadowLogU;
 			}
+			// the next iteration rewrites the per-wave counts
+			__syncthreads();
 		}
 	}
 
