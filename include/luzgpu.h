@@ -64,7 +64,9 @@ typedef struct luzgpu_settings
 	int32_t		extend_refill;   /* idle lanes (1..64) at which a wave takes new closest-hit rays;    *
 	                              * 64 (default): whole-wave claims, each lane traverses to the end   */
 	int32_t		shadow_refill;   /* the same for shadow rays (default 64)                             */
-	uint32_t	fetch_chunk;     /* rays a wave claims per queue atomic, 64..1024 (default 64)        */
+	uint32_t	fetch_chunk;     /* most rays a wave claims per queue atomic, 64..1024 (default 256;
+	                              * a claim is an eighth of a wave's even share of the queue,
+	                              * rounded up to 64, at most this)                             */
 	uint32_t	lds_stack;       /* LDS traversal-stack entries per lane: 8, 16 (default) or 24       */
 	uint32_t	autotune;        /* 1: use the context's tuned launch shape for these scenes and      *
 	                              * settings, tuning it first if there is none (luzgpu_autotune);     *

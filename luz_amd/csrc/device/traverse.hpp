@@ -495,51 +495,26 @@ namespace luzgpu
 					e[k] = cnt > 0 ? leafEntry(static_cast<std::int32_t>(child[k]), static_cast<std::int32_t>(cnt)) : child[k];
 				}
 			}
-			if constexpr (!AnyHit)
+			// the hit children by entry distance, in child order among equal distances (a
+			// compare-exchange network of adjacent pairs is a stable sort), pushed far to near,
+			// the nearest taken next. An any-hit query ends at its first hit whatever the order
+			// (BVHNode::hitAny is unordered, BVHNode.cpp:326-380); near first finds occluders sooner.
+			auto exchange = [&](int i) {
+				const bool swap = key[i] > key[i + 1];
+				const std::uint32_t ki = key[i], ei = e[i];
+				key[i] = swap ? key[i + 1] : ki;
+				e[i] = swap ? e[i + 1] : ei;
+				key[i + 1] = swap ? ki : key[i + 1];
+				e[i + 1] = swap ? ei : e[i + 1];
+			};
+			exchange(0); exchange(1); exchange(0); exchange(2); exchange(1); exchange(0);
+			if (key[0] != kMissKey)
 			{
-				// closest hit: the hit children by entry distance, in child order among equal
-				// distances (a compare-exchange network of adjacent pairs is a stable sort),
-				// pushed far to near, the nearest taken next
-				auto exchange = [&](int i) {
-					const bool swap = key[i] > key[i + 1];
-					const std::uint32_t ki = key[i], ei = e[i];
-					key[i] = swap ? key[i + 1] : ki;
-					e[i] = swap ? e[i + 1] : ei;
-					key[i + 1] = swap ? ki : key[i + 1];
-					e[i + 1] = swap ? ei : e[i + 1];
-				};
-				exchange(0); exchange(1); exchange(0); exchange(2); exchange(1); exchange(0);
-				if (key[0] != kMissKey)
-				{
-					if (key[3] != kMissKey) stack.push(e[3]);
-					if (key[2] != kMissKey) stack.push(e[2]);
-					if (key[1] != kMissKey) stack.push(e[1]);
-					st.entry = e[0];
-					return false;
-				}
-			}
-			else
-			{
-				// an any-hit query ends at the first hit, so its order does not matter
-				// (BVHNode::hitAny is unordered, BVHNode.cpp:326-380): the lowest hit child is
-				// taken next, the others pushed
-				bool have = false;
-				std::uint32_t next = 0;
-				#pragma unroll
-				for (int k = 3; k >= 0; k--)
-				{
-					if (key[k] != kMissKey)
-					{
-						if (have) stack.push(next);
-						next = e[k];
-						have = true;
-					}
-				}
-				if (have)
-				{
-					st.entry = next;
-					return false;
-				}
+				if (key[3] != kMissKey) stack.push(e[3]);
+				if (key[2] != kMissKey) stack.push(e[2]);
+				if (key[1] != kMissKey) stack.push(e[1]);
+				st.entry = e[0];
+				return false;
 			}
 		}
 		else

@@ -225,7 +225,7 @@ namespace luzgpu
 	// Queue slots handed to the lanes of one wave, claimed from the queue's fetch counter in
 	// chunks of `chunk` rays (one atomic per chunk; tunable 64..1024). All fields are
 	// wave-uniform.
-	constexpr std::uint32_t	kFetchChunk = 64;
+	constexpr std::uint32_t	kFetchChunk = 256;	// the most rays a wave claims per atomic
 
 	struct	WaveFeed
 	{
@@ -306,13 +306,19 @@ namespace luzgpu
 	// Source supplies the queue: count(), load(slot, ray, tMax) and store(slot, state).
 	template <bool AnyHit, bool Count, int Feat, int LdsDepth, typename Source>
 	__device__ __forceinline__ void	traceQueue(const SceneView& s, const PassJob* jobs, Source& src, std::uint32_t* fetch,
-		int refillAt, std::uint32_t chunk, unsigned long long* counters, int counterSlot)
+		int refillAt, std::uint32_t maxChunk, unsigned long long* counters, int counterSlot)
 	{
 		__shared__ std::uint32_t stackMem[LdsDepth * kWaveBlock];
 		const std::uint32_t gtid = blockIdx.x * kWaveBlock + threadIdx.x;
 		SpillStack<LdsDepth, kWaveBlock> stack{stackMem + threadIdx.x, src.spill + gtid, gridDim.x * kWaveBlock, 0,
 			kWaveStackCapacity};
 		const std::uint32_t n = src.count();
+		// Rays a wave claims per atomic on the queue's counter (shared by every XCD): about an
+		// eighth of an even share of the queue, a multiple of 64 between 64 and maxChunk, so a
+		// long queue is claimed with few atomics and a short one still spreads over the grid.
+		const std::uint32_t waves = gridDim.x * (kWaveBlock / 64);
+		const std::uint32_t share = ((n / (8u * waves)) + 63u) & ~63u;
+		const std::uint32_t chunk = min(max(share, 64u), maxChunk);
 		// Waves beyond what the queue can feed leave before touching the claim counter: a
 		// whole grid of claims on one counter costs ~50 us when the queue is short. The waves
 		// that stay claim until the queue is exhausted, so every ray is still traced.
@@ -890,7 +896,7 @@ namespace luzgpu
 		unsigned	grid = 0;			// persistent blocks per traversal launch
 		int	extendRefill = 0;		// idle lanes at which a wave takes new closest-hit rays
 		int	shadowRefill = 0;		// idle lanes at which a wave takes new shadow rays
-		unsigned	chunk = 0;			// rays a wave claims per queue atomic
+		unsigned	chunk = 0;			// most rays a wave claims per queue atomic
 		int	ldsStack = 0;			// LDS stack entries per lane (the rest spill to HBM)
 
 		WaveTuning	resolved() const
