@@ -182,7 +182,7 @@ def bench_batch(args, wdir, steps=None, warmup=None):
     outs = [torch.empty(w.width * w.height * 3, dtype=torch.float32, device="cuda") for w in wls]
     total = sum(w.width * w.height * w.spp for w in wls)
     log(f"batch: {len(wls)} scenes, {sum(w.triangles for w in wls)} triangles, {total / 1e6:.1f} M samples per step")
-    tuning = None
+    tuning = args.explicit_tuning
     if args.tuning == "auto":
         tuned = Renderer.autotune(scenes, seeds)
         tuning = "auto"
@@ -328,8 +328,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="spheres", choices=["spheres", "cornell", "batch", "terrain", "diff"])
     ap.add_argument("--integrator", default="wavefront", choices=["megakernel", "wavefront"])
-    ap.add_argument("--tuning", default="auto", choices=["auto", "default"],
-                    help="wavefront launch shape: autotuned (untimed calibration first) or built-in defaults")
+    ap.add_argument("--tuning", default="auto",
+                    help="wavefront launch shape: auto (untimed calibration first), default (built-in), or an "
+                         "explicit shape grid=G,extend_refill=E,shadow_refill=S,fetch_chunk=C,lds_stack=L "
+                         "(as the autotuner reports it, so a profile can run the timed shape alone)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=16)
@@ -337,6 +339,16 @@ def main():
                     help="skip the forward+adjoint and batch figures attached to the default line")
     ap.add_argument("--probe-copy", action="store_true", help="run one plain 1 GiB copy kernel (PMC scale probe)")
     args = ap.parse_args()
+    args.explicit_tuning = None
+    if args.tuning not in ("auto", "default"):
+        keys = {"grid", "extend_refill", "shadow_refill", "fetch_chunk", "lds_stack"}
+        try:
+            shape = {k: int(v) for k, v in (item.split("=") for item in args.tuning.split(","))}
+        except ValueError:
+            ap.error(f"--tuning {args.tuning!r}: expected auto, default or key=value pairs")
+        if not set(shape) <= keys:
+            ap.error(f"--tuning keys must be among {sorted(keys)}")
+        args.explicit_tuning = shape
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -388,7 +400,7 @@ def main():
         f"{cst.prims_tested / max(cst.rendered_samples, 1):.1f} primitives, "
         f"{(cst.closest_hit_rays + cst.shadow_rays) / max(cst.rendered_samples, 1):.2f} rays per sample")
 
-    tuning = None
+    tuning = args.explicit_tuning if wavefront else None
     tuned = None
     if wavefront and args.tuning == "auto":
         # the autotuner's calibration pass (untimed; its result is reused by every step)
@@ -455,7 +467,7 @@ def main():
                         f"{wl.width}x{wl.height}, {wl.spp} spp, depth {wl.depth}, seed {args.seed}",
             "width": wl.width, "height": wl.height, "spp": wl.spp, "max_depth": wl.depth,
             "triangles": wl.triangles, "integrator": args.integrator,
-            "launch_shape": tuned.as_dict() if tuned else "default",
+            "launch_shape": tuned.as_dict() if tuned else (args.explicit_tuning or "default"),
         },
         "mrays_per_s": round(rays_total / elapsed / 1e6, 3),
         "roofline": {

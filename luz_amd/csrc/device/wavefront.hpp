@@ -217,8 +217,8 @@ namespace luzgpu
 			w.qDir[0][g] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(kNoOrigin.hittable));
 			w.qFrom[0][g] = kNoOrigin.tri;
 			w.qPath[0][g] = w.queuedPath(job, path);
-			w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-			w.radiance[path] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(PF_PREV_SPECULAR));
+			// the path's throughput and radiance start as constants the first shade stage
+			// supplies itself
 		}
 	}
 
@@ -544,8 +544,10 @@ namespace luzgpu
 				const float4 o = w.qOrigin[q][i], d = w.qDir[q][i];
 				const Ray ray = makeRay(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
 				shadowLogU = o.w;
-				float4 t4 = w.throughput[path];
-				float4 r4 = w.radiance[path];
+				// a camera path starts with throughput 1, no radiance, and counts as leaving a
+				// specular vertex (the camera): those are not stored by ray generation
+				float4 t4 = b == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : w.throughput[path];
+				float4 r4 = b == 0 ? make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(PF_PREV_SPECULAR)) : w.radiance[path];
 				f3 thr = mk(t4.x, t4.y, t4.z), acc = mk(r4.x, r4.y, r4.z);
 				std::uint32_t flags = __float_as_uint(r4.w) & (PF_PREV_SPECULAR | PF_PREV_INFINITE);
 				const bool prevSpecular = flags & PF_PREV_SPECULAR;
@@ -601,8 +603,12 @@ namespace luzgpu
 					}
 					else if (!scatter<Feat>(m, sf, ray.d, ps, sc))
 					{
-						const float4 po = w.prevOrigin[path];
-						const float lp = (!prevSpecular && p.lightCount > 0) ? lightsPdf<Feat>(s, p, mk(po.x, po.y, po.z), ray.d) : 0.0f;
+						float lp = 0.0f;
+						if (!prevSpecular && p.lightCount > 0)
+						{
+							const float4 po = w.prevOrigin[path];
+							lp = lightsPdf<Feat>(s, p, mk(po.x, po.y, po.z), ray.d);
+						}
 						const float wgt = prevSpecular ? 1.0f : powerHeuristic(prevPdf, lp);
 						acc = acc + clampRay((thr * emitted) * wgt);
 					}
@@ -681,7 +687,11 @@ namespace luzgpu
 											flags = (flags & (PF_CONN_LIGHT | PF_CONN_ENV | PF_CONN_FREE_LIGHT))
 												| (samplesVisibleInfinite(p, sc) ? PF_PREV_INFINITE : 0u);
 											t4.w = pdf;
-											w.prevOrigin[path] = make_float4(sf.position.x, sf.position.y, sf.position.z, 0.0f);
+											// read back only for an emitter's MIS weight
+											if (p.lightCount > 0)
+											{
+												w.prevOrigin[path] = make_float4(sf.position.x, sf.position.y, sf.position.z, 0.0f);
+											}
 											done = false;
 										}
 									}
