@@ -482,7 +482,10 @@ namespace luzgpu
 
 	// closest hit of every ray of queue b
 	template <bool Count, int Feat, int LdsDepth>
-	__global__ __launch_bounds__(kWaveBlock) void	wf_extend(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
+	// The traversal kernels are asked for at least 4 waves per SIMD (at most 128 VGPRs): the
+	// analytic-primitive specialisation otherwise takes 136 and runs 3 (terrain extend 41.4 ->
+	// 36.8 ms); the triangle-only one needs 88 and is unaffected.
+	__global__ __launch_bounds__(kWaveBlock, 4) void	wf_extend(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
 		unsigned long long* counters, int refillAt, std::uint32_t chunk)
 	{
 		ExtendQueue src{w, b, w.spill};
@@ -785,7 +788,7 @@ namespace luzgpu
 
 	// any-hit test of the shadow rays appended in bounce b
 	template <bool Count, int Feat, int LdsDepth>
-	__global__ __launch_bounds__(kWaveBlock) void	wf_shadow(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
+	__global__ __launch_bounds__(kWaveBlock, 4) void	wf_shadow(SceneView s, const PassJob* jobs, WaveBuffers w, int b,
 		unsigned long long* counters, int refillAt, std::uint32_t chunk, std::uint32_t volumes)
 	{
 		ShadowQueue src{w, b, w.spill, volumes, 0u};
