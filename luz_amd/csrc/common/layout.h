@@ -67,6 +67,19 @@ namespace luzgpu
 	constexpr std::uint32_t	kKindShift = 28;
 	constexpr std::uint32_t	kIndexMask = (1u << kKindShift) - 1u;
 
+	// Guide table of an environment map's CDF: for bucket g of G, the first texel whose
+	// running weight reaches (g / G) x total (std::lower_bound), g = 0..G. A sample u falls
+	// in bucket floor(u G), and its texel lies between that bucket's entry and the next one's,
+	// so the search covers that range only and finds the texel the whole-table search finds.
+	// G is the largest power of two at most a quarter of the texels; maps of fewer than 8
+	// texels are searched whole. The table is texel-aligned with the CDF (its first G + 1
+	// entries are used).
+	constexpr std::uint32_t	kEnvGuideMinTexels = 8;
+	constexpr std::uint32_t	envGuideBuckets(std::uint32_t texels)
+	{
+		return 1u << (31 - __builtin_clz(texels / 4u));
+	}
+
 	struct GpuHittable
 	{
 		std::uint32_t	kindIndex;	// kind << 28 | index

@@ -1131,9 +1131,20 @@ namespace luzgpu
 			}
 			else
 			{
-				const double target = clampSample(sel) * p.envTotalWeight;
+				const double su = clampSample(sel);
+				const double target = su * p.envTotalWeight;
 				const std::uint32_t n = p.envWidth * p.envHeight;
 				std::uint32_t lo = 0, hi = n;
+				if (s.envGuide && n >= kEnvGuideMinTexels)
+				{
+					// the guide bucket of the sample bounds the search (its product with G is
+					// exact: G is a power of two)
+					const std::uint32_t buckets = envGuideBuckets(n);
+					const std::uint32_t g = min(static_cast<std::uint32_t>(su * static_cast<double>(buckets)), buckets - 1u);
+					const std::uint32_t* guide = s.envGuide + p.envBase;
+					lo = guide[g];
+					hi = min(guide[g + 1] + 1u, n);
+				}
 				while (lo < hi)	// std::lower_bound: first element >= target
 				{
 					const std::uint32_t mid = (lo + hi) >> 1;

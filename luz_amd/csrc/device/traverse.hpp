@@ -41,6 +41,7 @@ namespace luzgpu
 		const float4*	envPixels;
 		const float*	envPdf;
 		const double*	envCdf;
+		const std::uint32_t*	envGuide;	// texel-aligned with envCdf
 		const std::uint32_t*	unbounded;	// hittable ids of the planes, tested after the BVH
 		const GpuSphere*	spheres;
 		const GpuRect*	cubeFaces;

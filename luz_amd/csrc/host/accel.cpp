@@ -1475,6 +1475,19 @@ namespace luzgpu
 						p.envPdf[i] = static_cast<float>((weightsE[i] / tw) / solid[i]);
 					}
 				}
+				const std::uint32_t n = static_cast<std::uint32_t>(W * H);
+				if (n >= kEnvGuideMinTexels)
+				{
+					// the bucket bounds are formed as the sampler forms its target:
+					// (g / G exactly) x total, one rounding
+					const std::uint32_t G = envGuideBuckets(n);
+					p.envGuide.assign(n, 0u);
+					for (std::uint32_t g = 0; g <= G; g++)
+					{
+						const double target = (static_cast<double>(g) / static_cast<double>(G)) * tw;
+						p.envGuide[g] = static_cast<std::uint32_t>(std::lower_bound(p.envCdf.begin(), p.envCdf.end(), target) - p.envCdf.begin());
+					}
+				}
 			}
 			gp.envTotalWeight = tw;
 			gp.envWidth = static_cast<std::uint32_t>(W);
@@ -1639,6 +1652,8 @@ namespace luzgpu
 			// envCdf is empty when the map has no weight: keep it texel-aligned with envPdf
 			if (p.envCdf.empty()) m.envCdf.insert(m.envCdf.end(), p.envPdf.size(), 0.0);
 			else m.envCdf.insert(m.envCdf.end(), p.envCdf.begin(), p.envCdf.end());
+			if (p.envGuide.empty()) m.envGuide.insert(m.envGuide.end(), p.envPdf.size(), 0u);
+			else m.envGuide.insert(m.envGuide.end(), p.envGuide.begin(), p.envGuide.end());
 			gp.unboundedBase = static_cast<std::uint32_t>(m.unbounded.size());
 			for (std::uint32_t u : p.unbounded) m.unbounded.push_back(u + hittableBase);
 			if (m.unbounded.size() > 0xffffu) throw std::runtime_error("Unsupported: more than 65535 planes in a batch");

@@ -634,14 +634,14 @@ namespace luzgpu
 	{
 		PackedScene	host;
 		DeviceBuffer	tlas, blas, topTris, meshTris, meshNormals, tlasPrims, hittables, rects, instances, materials,
-			lights, lightCdf, lightTriCdf, lightTriSlot, envPixels, envPdf, envCdf, unbounded, spheres, cubeFaces, volumes, planes;
+			lights, lightCdf, lightTriCdf, lightTriSlot, envPixels, envPdf, envCdf, envGuide, unbounded, spheres, cubeFaces, volumes, planes;
 		SceneView	view{};
 		luzgpu_scene_info	info{};
 
 		void	releaseDevice()
 		{
 			for (DeviceBuffer* b : {&tlas, &blas, &topTris, &meshTris, &meshNormals, &tlasPrims, &hittables, &rects, &instances,
-				&materials, &lights, &lightCdf, &lightTriCdf, &lightTriSlot, &envPixels, &envPdf, &envCdf, &unbounded, &spheres,
+				&materials, &lights, &lightCdf, &lightTriCdf, &lightTriSlot, &envPixels, &envPdf, &envCdf, &envGuide, &unbounded, &spheres,
 				&cubeFaces, &volumes, &planes})
 			{
 				b->release();
@@ -749,6 +749,7 @@ namespace luzgpu
 			g.envPixels.upload(h.envPixels, st);
 			g.envPdf.upload(h.envPdf, st);
 			g.envCdf.upload(h.envCdf, st);
+			g.envGuide.upload(h.envGuide, st);
 			g.unbounded.upload(h.unbounded, st);
 			g.spheres.upload(h.spheres, st);
 			g.cubeFaces.upload(h.cubeFaces, st);
@@ -773,6 +774,7 @@ namespace luzgpu
 			v.envPixels = g.envPixels.as<float4>();
 			v.envPdf = g.envPdf.as<float>();
 			v.envCdf = g.envCdf.as<double>();
+			v.envGuide = h.envGuide.empty() ? nullptr : g.envGuide.as<std::uint32_t>();
 			v.unbounded = g.unbounded.as<std::uint32_t>();
 			v.spheres = g.spheres.as<GpuSphere>();
 			v.cubeFaces = g.cubeFaces.as<GpuRect>();
