@@ -208,7 +208,12 @@ namespace luzgpu
 		// matching triangle slots, and its local -> world map (rotate(p * scale) + position)
 		std::uint32_t	triBase;
 		std::uint32_t	triCount;
-		std::uint32_t	pad[2];
+		// IES-profiled sphere light: 1 + the offset of its profile in the IES table, else 0.
+		// A profile is [mean candela, unit vertical axis xyz, rotation (degrees), vertical
+		// count nV, horizontal count nH, nV vertical angles, nH horizontal angles, nH x nV
+		// candela values], all FP64.
+		std::uint32_t	ies;
+		std::uint32_t	pad;
 		float	localToWorld[12];
 	};
 	static_assert(sizeof(GpuLight) == 112, "GpuLight must be 112 bytes");

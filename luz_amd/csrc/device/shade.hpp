@@ -804,6 +804,99 @@ namespace luzgpu
 
 	// Sphere::sampleLight (Sphere.cpp:382-442): cone sampling toward the sphere, FP64 like
 	// the reference except the fp32 cone direction of Sphere::randomToSphere.
+	// IESProfile::candelaAt (IESProfile.cpp:245-300): bilinear in the vertical and horizontal
+	// angle (degrees) over the profile's grid, zero outside its vertical range; past the last
+	// horizontal angle it interpolates towards the first one a turn later.
+	__device__ inline double	iesCandelaAt(const double* V, int nv, const double* H, int nh, const double* C, double vdeg, double hdeg)
+	{
+		if (nv <= 0 || nh <= 0 || !isfinite(vdeg) || vdeg < V[0] || vdeg > V[nv - 1]) return 0.0;
+		int v0 = 0;
+		while (v0 + 1 < nv && V[v0 + 1] < vdeg) v0++;
+		const int v1 = min(v0 + 1, nv - 1);
+		const double vSpan = V[v1] - V[v0];
+		const double vt = vSpan > 0.0 ? (vdeg - V[v0]) / vSpan : 0.0;
+		int h0 = 0, h1 = 0;
+		double ht = 0.0;
+		if (nh > 1)
+		{
+			double h = 0.0;
+			if (isfinite(hdeg))
+			{
+				h = fmod(hdeg, 360.0);
+				if (h < 0.0) h += 360.0;
+			}
+			if (h < H[0]) h += 360.0;
+			if (h > H[nh - 1] && H[nh - 1] >= 360.0) h = fmin(h, H[nh - 1]);
+			while (h0 + 1 < nh && H[h0 + 1] < h) h0++;
+			if (h0 + 1 < nh)
+			{
+				h1 = h0 + 1;
+				const double span = H[h1] - H[h0];
+				ht = span > 0.0 ? (h - H[h0]) / span : 0.0;
+			}
+			else
+			{
+				h1 = 0;
+				const double span = (H[0] + 360.0) - H[h0];
+				ht = span > 0.0 ? (h - H[h0]) / span : 0.0;
+			}
+		}
+		const double c00 = C[h0 * nv + v0], c01 = C[h0 * nv + v1], c10 = C[h1 * nv + v0], c11 = C[h1 * nv + v1];
+		const double a = c00 * (1.0 - vt) + c01 * vt, b = c10 * (1.0 - vt) + c11 * vt;
+		return a * (1.0 - ht) + b * ht;
+	}
+
+	// IESProfile::relativeIntensity (IESProfile.cpp:302-323) in FP64 for an emission direction
+	// (from the light towards the shaded point): the vertical angle from the profile's axis,
+	// the horizontal angle in a frame built from the axis and turned by the rotation, and the
+	// candela there over the profile's mean. `t` is the profile's table (GpuLight::ies).
+	__device__ inline double	iesRelativeIntensity(const double* t, f3 emission)
+	{
+		const double mean = t[0];
+		if (!(mean > 0.0) || !isfinite(mean)) return 0.0;
+		const double ax = t[1], ay = t[2], az = t[3], rotation = t[4];
+		const int nv = static_cast<int>(t[5]), nh = static_cast<int>(t[6]);
+		const double* V = t + 7;
+		const double* H = V + nv;
+		const double* C = H + nh;
+		double dx = emission.x, dy = emission.y, dz = emission.z;
+		const double l2 = dx * dx + dy * dy + dz * dz;
+		if (!isfinite(l2) || l2 <= 0.0)
+		{
+			dx = ax; dy = ay; dz = az;
+		}
+		else
+		{
+			const double l = sqrt(l2);
+			dx = dx / l; dy = dy / l; dz = dz / l;
+		}
+		const double cosT = fmin(fmax(dx * ax + dy * ay + dz * az, -1.0), 1.0);
+		constexpr double kDeg = 180.0 / 3.14159265358979323846;
+		const double vdeg = acos(cosT) * kDeg;
+		const bool yHelper = fabs(ay) < 0.95;
+		const double hx = yHelper ? 0.0 : 1.0, hy = yHelper ? 1.0 : 0.0, hz = 0.0;
+		double tx = hy * az - hz * ay, ty = hz * ax - hx * az, tz = hx * ay - hy * ax;
+		const double tl2 = tx * tx + ty * ty + tz * tz;
+		if (!isfinite(tl2) || tl2 <= 0.0)
+		{
+			tx = 1.0; ty = 0.0; tz = 0.0;
+		}
+		else
+		{
+			const double tl = sqrt(tl2);
+			tx = tx / tl; ty = ty / tl; tz = tz / tl;
+		}
+		const double bx = ay * tz - az * ty, by = az * tx - ax * tz, bz = ax * ty - ay * tx;
+		double hdeg = atan2(dx * bx + dy * by + dz * bz, dx * tx + dy * ty + dz * tz) * kDeg + rotation;
+		if (!isfinite(hdeg)) hdeg = 0.0;
+		else
+		{
+			hdeg = fmod(hdeg, 360.0);
+			if (hdeg < 0.0) hdeg += 360.0;
+		}
+		return iesCandelaAt(V, nv, H, nh, C, vdeg, hdeg) / mean;
+	}
+
 	__device__ __forceinline__ LightSample	sampleSphereLight(const SceneView& s, const GpuLight& L, f3 origin, const PixelSampler& ps)
 	{
 		LightSample ls;
@@ -843,6 +936,12 @@ namespace luzgpu
 		ls.tMax = static_cast<float>(root);
 		if (ls.pdf <= 0.0f || ls.tMax <= kTMinF || !isfinite(ls.pdf) || !isfinite(ls.tMax)) return ls;
 		ls.emitted = ld3(s.materials[L.material].color);
+		if (L.ies)
+		{
+			// Sphere::sampleLight (Sphere.cpp:451-461): an IES profile scales the sample's
+			// radiance (the light's radiance seen by a ray that hits it is not scaled)
+			ls.emitted = ls.emitted * static_cast<float>(iesRelativeIntensity(s.iesData + (L.ies - 1u), -dir));
+		}
 		ls.valid = maxc(ls.emitted) > 0.0f;
 		return ls;
 	}

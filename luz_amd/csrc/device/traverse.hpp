@@ -42,6 +42,7 @@ namespace luzgpu
 		const float*	envPdf;
 		const double*	envCdf;
 		const std::uint32_t*	envGuide;	// texel-aligned with envCdf
+		const double*	iesData;	// IES profiles of sphere lights (GpuLight::ies)
 		const std::uint32_t*	unbounded;	// hittable ids of the planes, tested after the BVH
 		const GpuSphere*	spheres;
 		const GpuRect*	cubeFaces;

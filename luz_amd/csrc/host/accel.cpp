@@ -1323,6 +1323,26 @@ namespace luzgpu
 				area = 4.0 * kPi * sp.radius * sp.radius;
 				l.kind = GPU_HIT_SPHERE;
 				l.index = p.hittables[i].kindIndex & kIndexMask;
+				if (sp.ies >= 0)
+				{
+					// relativeIntensity's axis (IESProfile.cpp:302-305): normalised, or straight
+					// down when it has no length
+					const IesProfile& pr = scene.iesProfiles[static_cast<std::size_t>(sp.ies)];
+					dvec3 axis = sp.iesDirection;
+					const double l2 = dot(axis, axis);
+					if (!std::isfinite(l2) || l2 <= 0.0) axis = dvec3{0.0, -1.0, 0.0};
+					else
+					{
+						const double len = std::sqrt(l2);
+						axis = dvec3{axis[0] / len, axis[1] / len, axis[2] / len};
+					}
+					l.ies = static_cast<std::uint32_t>(p.iesData.size()) + 1u;
+					p.iesData.insert(p.iesData.end(), {pr.meanCandela, axis[0], axis[1], axis[2], sp.iesRotation,
+						static_cast<double>(pr.vertical.size()), static_cast<double>(pr.horizontal.size())});
+					p.iesData.insert(p.iesData.end(), pr.vertical.begin(), pr.vertical.end());
+					p.iesData.insert(p.iesData.end(), pr.horizontal.begin(), pr.horizontal.end());
+					p.iesData.insert(p.iesData.end(), pr.candela.begin(), pr.candela.end());
+				}
 			}
 			else if (h.kind == HIT_DIRECTIONAL)
 			{
@@ -1630,6 +1650,8 @@ namespace luzgpu
 			GpuParams gp = p.params;
 			gp.lightBase = static_cast<std::uint32_t>(m.lights.size());
 			const std::uint32_t lightTriBase = static_cast<std::uint32_t>(m.lightTriCdf.size());
+			const std::uint32_t iesBase = static_cast<std::uint32_t>(m.iesData.size());
+			m.iesData.insert(m.iesData.end(), p.iesData.begin(), p.iesData.end());
 			for (GpuLight l : p.lights)
 			{
 				if (l.kind == GPU_HIT_TRIANGLE || l.kind == GPU_HIT_RECTANGLE || l.kind == GPU_HIT_SPHERE || l.kind == GPU_HIT_INSTANCE)
@@ -1637,6 +1659,7 @@ namespace luzgpu
 					l.index += base[l.kind];
 				}
 				if (l.kind == GPU_HIT_INSTANCE) l.triBase += lightTriBase;
+				if (l.ies) l.ies += iesBase;
 				l.hittable += hittableBase;
 				l.material += materialBase;
 				m.lights.push_back(l);

@@ -78,6 +78,7 @@ namespace luzgpu
 		std::vector<float>	envPdf;		// (weight / total) / solid angle per texel
 		std::vector<double>	envCdf;
 		std::vector<std::uint32_t>	envGuide;	// texel-aligned with envCdf (layout.h, envGuideBuckets)
+		std::vector<double>	iesData;	// IES profiles of sphere lights (layout.h, GpuLight::ies)
 		std::vector<std::uint32_t>	unbounded;
 		GpuParams	params{};
 		std::string	tlasTopology;	// nested "(...)" form over top-level hittable indices

@@ -96,11 +96,27 @@ namespace luzgpu
 
 	// Sphere::hit / hitAny (Sphere.cpp:227-233, 271-277): an invisible sphere is never hit
 	// by any ray, but stays in the acceleration structure and, emissive, in the light list.
+	// An IES-profiled sphere light (Sphere::setIESProfile, Sphere.cpp:188-202) scales the
+	// radiance of its light samples by the profile's relative intensity towards the shaded
+	// point, about its vertical axis `iesDirection` and turned by `iesRotation` degrees.
 	struct	Sphere
 	{
 		dvec3	position{};
 		double	radius = 1.0;
 		bool	visible = true;
+		std::int32_t	ies = -1;	// index into Scene::iesProfiles
+		dvec3	iesDirection = {0.0, -1.0, 0.0};
+		double	iesRotation = 0.0;
+	};
+
+	// IESProfile (IESProfile.cpp:133-240): vertical and horizontal angles in degrees (strictly
+	// increasing), candela values (times the file's multiplier) for each horizontal angle over
+	// all vertical angles, and the luminous flux integrated over the sphere.
+	struct	IesProfile
+	{
+		std::vector<double>	vertical, horizontal, candela;
+		double	totalLumens = 0.0;
+		double	meanCandela = 0.0;	// totalLumens / 4 pi
 	};
 
 	// Cube.cpp:72-95: six axis-aligned rectangles around the position (the orientation
@@ -242,6 +258,7 @@ namespace luzgpu
 		std::vector<Plane>	planes;
 		std::vector<Volume>	volumes;
 		std::vector<DirectionalLight>	directionals;
+		std::vector<IesProfile>	iesProfiles;
 		std::vector<Hittable>	hittables;
 		Environment	environment;
 		bool	hasEnvironment = false;
@@ -256,6 +273,9 @@ namespace luzgpu
 	Scene	readSceneFile(const std::string& path);
 	Mesh	readObjMesh(const std::string& path);
 	Environment	readHDREnvironment(const std::string& path);
+	IesProfile	readIesProfile(const std::string& path);
+	// IESProfile::candelaAt (IESProfile.cpp:245-300)
+	double	iesCandelaAt(const IesProfile& profile, double verticalDegrees, double horizontalDegrees);
 	double	luminance(const dvec3& c);
 	// ColorScience::blackbody / ::wavelength (ColorScience.cpp:182-214): ACEScg colours of
 	// unit maximum channel; throw outside 1000-40000 K / 360-830 nm

@@ -2233,7 +2233,9 @@ namespace luzgpu
 				dvec3 color = {1.0, 1.0, 1.0};
 				LightQuantity q;
 				bool visible = true;
-				bool ies = false;
+				std::string iesPath;
+				dvec3 iesDirection = {0.0, -1.0, 0.0};
+				double iesRotation = 0.0;
 				while (next(line))
 				{
 					const std::string t = trim(line);
@@ -2247,12 +2249,28 @@ namespace luzgpu
 						{
 							throw std::runtime_error("Sphere light '" + name + "' radius must be positive.");
 						}
-						if (ies)
-						{
-							throw std::runtime_error("Unsupported for the device path: IES light profile");
-						}
 						sp.visible = visible;
 						const double mpu = scene.settings.metersPerUnit;
+						if (!iesPath.empty())
+						{
+							// the profile is read (and checked) before the emission, as the
+							// reference builds it (Objects.cpp:1066-1089)
+							const double rotation = iesRotation;
+							if (!std::isfinite(rotation)) throw std::invalid_argument("IES profile rotation must be finite.");
+							const double l2 = dot(iesDirection, iesDirection);
+							scene.iesProfiles.push_back(readIesProfile(resolvePath(base, iesPath)));
+							if (!std::isfinite(l2) || l2 <= 0.0) throw std::invalid_argument("IES profile direction must be non-zero.");
+							sp.ies = static_cast<std::int32_t>(scene.iesProfiles.size() - 1);
+							sp.iesDirection = iesDirection;
+							sp.iesRotation = rotation;
+							// sphericalLightEmission (Objects.cpp:632-643): without a unit of its
+							// own the light emits the profile's luminous flux
+							if (q.kind == LightQuantity::NONE)
+							{
+								q.kind = LightQuantity::LUMENS;
+								q.value = scene.iesProfiles.back().totalLumens;
+							}
+						}
 						Material m;
 						m.kind = MAT_EMISSIVE;
 						m.color = lightEmission(color, q, 4.0 * kPi * sp.radius * sp.radius * mpu * mpu,
@@ -2271,11 +2289,11 @@ namespace luzgpu
 					else if (key == "radius") sp.radius = std::stod(value);
 					else if (key == "color") color = parseColor(value, key);
 					else if (lightQuantity(q, key, value, true)) continue;
-					else if (key == "ies" || key == "ies_profile" || key == "iesprofile" || key == "profile") ies = true;
+					else if (key == "ies" || key == "ies_profile" || key == "iesprofile" || key == "profile") iesPath = value;
 					else if (key == "ies_direction" || key == "iesdirection" || key == "profile_direction" || key == "profiledirection")
-						(void)parseTriple(value, key);
+						iesDirection = parseTriple(value, key);
 					else if (key == "ies_rotation" || key == "iesrotation" || key == "profile_rotation" || key == "profilerotation")
-						(void)std::stod(value);
+						iesRotation = std::stod(value);
 					else if (key == "visible") visible = visibleFlag(value, "Sphere light '" + name + "'");
 					else throw std::runtime_error("Unknown sphere light property: " + t);
 				}
@@ -2729,6 +2747,136 @@ namespace luzgpu
 		}
 		std::fclose(f);
 		return (mesh);
+	}
+
+	// IESProfile::load (IESProfile.cpp:133-232): an LM-63 photometric file. Lines up to
+	// TILT=NONE (other tilt data is refused) are the header; then whitespace-separated
+	// numbers: lamp count, lumens per lamp, candela multiplier, vertical and horizontal angle
+	// counts, photometric type and seven more lamp fields, the vertical angles, the horizontal
+	// angles, and the candela values horizontal-major.
+	IesProfile	readIesProfile(const std::string& path)
+	{
+		std::ifstream in(path);
+		if (!in) throw std::runtime_error("IES profile could not be opened: " + path);
+		std::string line;
+		bool tilt = false;
+		while (std::getline(in, line))
+		{
+			const std::string t = lower(trim(line));
+			if (t.rfind("tilt=", 0) == 0)
+			{
+				if (t != "tilt=none") throw std::runtime_error("IES profile uses unsupported tilt data: " + path);
+				tilt = true;
+				break;
+			}
+		}
+		if (!tilt) throw std::runtime_error("IES profile is missing TILT=NONE: " + path);
+		std::vector<double> v;
+		while (std::getline(in, line))
+		{
+			std::istringstream ls(line);
+			double x;
+			while (ls >> x) v.push_back(x);
+		}
+		if (v.size() < 13) throw std::runtime_error("IES profile has incomplete lamp metadata: " + path);
+		auto count = [&](double x, const char* what) {
+			if (!std::isfinite(x) || x <= 0.0 || std::floor(x) != x) throw std::runtime_error(std::string(what) + " must be a positive integer.");
+			return static_cast<std::size_t>(x);
+		};
+		const double lamps = v[0], lumensPerLamp = v[1], multiplier = v[2];
+		const std::size_t nv = count(v[3], "IES vertical angle count");
+		const std::size_t nh = count(v[4], "IES horizontal angle count");
+		if (v.size() < 13 + nv + nh + nv * nh) throw std::runtime_error("IES profile has incomplete angle or candela data: " + path);
+		if (!std::isfinite(lamps) || lamps <= 0.0) throw std::runtime_error("IES profile lamp count must be finite and positive: " + path);
+		if (!std::isfinite(lumensPerLamp)) throw std::runtime_error("IES profile lumens per lamp must be finite: " + path);
+		if (!std::isfinite(multiplier) || multiplier <= 0.0)
+			throw std::runtime_error("IES profile candela multiplier must be finite and positive: " + path);
+		IesProfile pr;
+		std::size_t at = 13;
+		pr.vertical.assign(v.begin() + static_cast<std::ptrdiff_t>(at), v.begin() + static_cast<std::ptrdiff_t>(at + nv));
+		at += nv;
+		pr.horizontal.assign(v.begin() + static_cast<std::ptrdiff_t>(at), v.begin() + static_cast<std::ptrdiff_t>(at + nh));
+		at += nh;
+		for (std::size_t i = 0; i < nv * nh; i++)
+		{
+			const double c = v[at + i] * multiplier;
+			if (!std::isfinite(c) || c < 0.0) throw std::runtime_error("IES profile candela values must be finite and non-negative: " + path);
+			pr.candela.push_back(c);
+		}
+		auto increasing = [](const std::vector<double>& a, const char* what) {
+			for (std::size_t i = 0; i < a.size(); i++)
+			{
+				if (!std::isfinite(a[i])) throw std::runtime_error(std::string(what) + " contain a non-finite value.");
+				if (i > 0 && a[i] <= a[i - 1]) throw std::runtime_error(std::string(what) + " must be strictly increasing.");
+			}
+		};
+		increasing(pr.vertical, "IES vertical angles");
+		increasing(pr.horizontal, "IES horizontal angles");
+		// integrateTotalLumens (IESProfile.cpp:114-132): midpoint rule, 180 x 360 cells
+		const double dTheta = kPi / 180.0, dPhi = (2.0 * kPi) / 360.0;
+		double total = 0.0;
+		for (int a = 0; a < 180; a++)
+		{
+			const double theta = (static_cast<double>(a) + 0.5) * dTheta;
+			const double vdeg = theta * (180.0 / kPi);
+			const double st = std::sin(theta);
+			for (int b = 0; b < 360; b++)
+			{
+				const double hdeg = (static_cast<double>(b) + 0.5) * 360.0 / 360.0;
+				total += iesCandelaAt(pr, vdeg, hdeg) * st * dTheta * dPhi;
+			}
+		}
+		if (!std::isfinite(total) || total <= 0.0) throw std::runtime_error("IES profile has zero luminous flux: " + path);
+		pr.totalLumens = total;
+		pr.meanCandela = total / (4.0 * kPi);
+		return (pr);
+	}
+
+	double	iesCandelaAt(const IesProfile& pr, double vdeg, double hdeg)
+	{
+		const std::vector<double>& V = pr.vertical;
+		const std::vector<double>& H = pr.horizontal;
+		if (V.empty() || H.empty() || pr.candela.empty() || !std::isfinite(vdeg)) return (0.0);
+		if (vdeg < V.front() || vdeg > V.back()) return (0.0);
+		std::size_t v0 = 0;
+		while (v0 + 1 < V.size() && V[v0 + 1] < vdeg) v0++;
+		const std::size_t v1 = std::min(v0 + 1, V.size() - 1);
+		const double vSpan = V[v1] - V[v0];
+		const double vt = vSpan > 0.0 ? (vdeg - V[v0]) / vSpan : 0.0;
+		std::size_t h0 = 0, h1 = 0;
+		double ht = 0.0;
+		if (H.size() > 1)
+		{
+			// the angle in [0, 360), lifted by a turn below the first angle, held at a last
+			// angle of 360 or more
+			double h = 0.0;
+			if (std::isfinite(hdeg))
+			{
+				h = std::fmod(hdeg, 360.0);
+				if (h < 0.0) h += 360.0;
+			}
+			if (h < H.front()) h += 360.0;
+			if (h > H.back() && H.back() >= 360.0) h = std::min(h, H.back());
+			while (h0 + 1 < H.size() && H[h0 + 1] < h) h0++;
+			if (h0 + 1 < H.size())
+			{
+				h1 = h0 + 1;
+				const double span = H[h1] - H[h0];
+				ht = span > 0.0 ? (h - H[h0]) / span : 0.0;
+			}
+			else
+			{
+				// past the last angle: towards the first one, a turn later
+				h1 = 0;
+				const double span = (H.front() + 360.0) - H[h0];
+				ht = span > 0.0 ? (h - H[h0]) / span : 0.0;
+			}
+		}
+		const std::size_t nv = V.size();
+		auto lerp = [](double a, double b, double t) { return (a * (1.0 - t) + b * t); };
+		const double c00 = pr.candela[h0 * nv + v0], c01 = pr.candela[h0 * nv + v1];
+		const double c10 = pr.candela[h1 * nv + v0], c11 = pr.candela[h1 * nv + v1];
+		return (lerp(lerp(c00, c01, vt), lerp(c10, c11, vt), ht));
 	}
 
 	Environment	readHDREnvironment(const std::string& path)

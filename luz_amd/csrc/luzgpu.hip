@@ -634,14 +634,14 @@ namespace luzgpu
 	{
 		PackedScene	host;
 		DeviceBuffer	tlas, blas, topTris, meshTris, meshNormals, tlasPrims, hittables, rects, instances, materials,
-			lights, lightCdf, lightTriCdf, lightTriSlot, envPixels, envPdf, envCdf, envGuide, unbounded, spheres, cubeFaces, volumes, planes;
+			lights, lightCdf, lightTriCdf, lightTriSlot, envPixels, envPdf, envCdf, envGuide, iesData, unbounded, spheres, cubeFaces, volumes, planes;
 		SceneView	view{};
 		luzgpu_scene_info	info{};
 
 		void	releaseDevice()
 		{
 			for (DeviceBuffer* b : {&tlas, &blas, &topTris, &meshTris, &meshNormals, &tlasPrims, &hittables, &rects, &instances,
-				&materials, &lights, &lightCdf, &lightTriCdf, &lightTriSlot, &envPixels, &envPdf, &envCdf, &envGuide, &unbounded, &spheres,
+				&materials, &lights, &lightCdf, &lightTriCdf, &lightTriSlot, &envPixels, &envPdf, &envCdf, &envGuide, &iesData, &unbounded, &spheres,
 				&cubeFaces, &volumes, &planes})
 			{
 				b->release();
@@ -750,6 +750,7 @@ namespace luzgpu
 			g.envPdf.upload(h.envPdf, st);
 			g.envCdf.upload(h.envCdf, st);
 			g.envGuide.upload(h.envGuide, st);
+			g.iesData.upload(h.iesData, st);
 			g.unbounded.upload(h.unbounded, st);
 			g.spheres.upload(h.spheres, st);
 			g.cubeFaces.upload(h.cubeFaces, st);
@@ -775,6 +776,7 @@ namespace luzgpu
 			v.envPdf = g.envPdf.as<float>();
 			v.envCdf = g.envCdf.as<double>();
 			v.envGuide = h.envGuide.empty() ? nullptr : g.envGuide.as<std::uint32_t>();
+			v.iesData = h.iesData.empty() ? nullptr : g.iesData.as<double>();
 			v.unbounded = g.unbounded.as<std::uint32_t>();
 			v.spheres = g.spheres.as<GpuSphere>();
 			v.cubeFaces = g.cubeFaces.as<GpuRect>();
@@ -2321,6 +2323,10 @@ int	luzgpu_adjoint(luzgpu_ctx* ctx, luzgpu_scene* scene, const luzgpu_settings* 
 		if (scene->gpu.host.features & kFeatSubsurface)
 		{
 			throw std::runtime_error("Unsupported for the adjoint: Burley subsurface (its exits move the path vertex)");
+		}
+		if (!scene->gpu.host.iesData.empty())
+		{
+			throw std::runtime_error("Unsupported for the adjoint: IES-profiled sphere lights");
 		}
 		hipStream_t st = ctx->stream;
 		const std::uint32_t pixels = p.width * p.height;

@@ -669,6 +669,66 @@ def write_invisible(directory: str, width=160, height=120, spp=32, depth=8, seed
     return Workload("invisible", directory, path, width, height, spp, depth, seed, 0)
 
 
+def ies_text(vertical, horizontal, candela, name: str) -> str:
+    """An LM-63 photometric file: header, TILT=NONE, the 13 lamp numbers (one lamp, absolute
+    photometry, multiplier 1), the angles and the candela values for each horizontal angle."""
+    lines = ["IESNA:LM-63-2002", f"[TEST] {name}", "[MANUFAC] synthetic profile for the parity tests", "TILT=NONE",
+             f"1 -1 1 {len(vertical)} {len(horizontal)} 1 2 0.1 0.1 0.1", "1 1 40",
+             " ".join(f"{v:g}" for v in vertical), " ".join(f"{h:g}" for h in horizontal)]
+    for row in candela:
+        lines.append(" ".join(f"{c:.6g}" for c in row))
+    return "\n".join(lines) + "\n"
+
+
+def ies_profiles():
+    """(file name, text) of two profiles: a downlight over horizontal angles 0..360 (the last
+    angle held), and an asymmetric wash over 0..240 (interpolated back to 0 past 240)."""
+    vertical = list(range(0, 181, 10))
+    down = [[900.0 * max(0.0, math.cos(math.radians(v))) ** 1.5 + 40.0 + 150.0 * math.sin(math.radians(v)) ** 2
+             * (1.0 + 0.5 * math.cos(math.radians(h))) for v in vertical] for h in (0, 90, 180, 270, 360)]
+    wash = [[60.0 + 500.0 * math.sin(math.radians(v)) ** 2 * (1.0 + 0.8 * math.cos(math.radians(h - 60)))
+             for v in vertical] for h in (0, 60, 120, 180, 240)]
+    return [("downlight.ies", ies_text(vertical, (0, 90, 180, 270, 360), down, "downlight")),
+            ("wash.ies", ies_text(vertical, (0, 60, 120, 180, 240), wash, "wash"))]
+
+
+def ies_scene_text(width=160, height=120, spp=32, depth=8) -> str:
+    """IES-profiled sphere lights: a downlight that emits its profile's own flux, and a tilted,
+    turned wash light with an explicit flux; light samples carry the profiles, rays that hit
+    the lights see them unscaled (as the reference does)."""
+    lines = [
+        "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,
+        "sky=none", "background=(0.01,0.01,0.015)", "",
+        "[materials]",
+        "material floor {", "type=lambertian", "color=(0.6,0.58,0.55)", "}",
+        "material wall {", "type=lambertian", "color=(0.5,0.5,0.55)", "}",
+        "material steel {", "type=metal", "color=(0.85,0.85,0.9)", "roughness=0.2", "}",
+        "",
+        "[scene]",
+        "camera main {", "position=(0,1.7,-6)", "direction=(0,-0.15,1)", "up=(0,1,0)", "focal_length_mm=30",
+        "sensor_width_mm=36", "sensor_height_mm=27", "pinhole=1", "focus_distance=6", "}",
+        "sphere_light spot {", "position=(-1.2,2.6,0.6)", "radius=0.15", "color=(1,0.95,0.85)", "ies=downlight.ies", "}",
+        "sphere_light wash {", "position=(1.6,2.2,-0.4)", "radius=0.12", "color=(0.8,0.9,1)", "lumens=1500",
+        "ies=wash.ies", "ies_direction=(0.3,-1,0.2)", "ies_rotation=30", "}",
+        "objects{",
+        "plane=0,(0,1,0),material=floor",
+        "rectangle=(0,2.5,3.5),(0,0,-1),10,5,material=wall",
+        "sphere=(0.4,0.7,1.0),0.7,material=steel",
+        "cube=(-1.8,0.5,1.4),(0,0,-1),1,1,1,material=wall",
+        "}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def write_ies(directory: str, width=160, height=120, spp=32, depth=8, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    for name, text in ies_profiles():
+        _write_if_changed(os.path.join(directory, name), text.encode())
+    path = os.path.join(directory, "ies.luz")
+    _write_if_changed(path, ies_scene_text(width, height, spp, depth).encode())
+    return Workload("ies", directory, path, width, height, spp, depth, seed, 0)
+
+
 def volume_scene_text(width=160, height=120, spp=32, depth=16) -> str:
     lines = [
         "[settings]", f"resolution={width},{height}", f"samples={spp}", f"maxlightbounces={depth}", *_HEADER,

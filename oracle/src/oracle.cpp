@@ -301,6 +301,11 @@ namespace oracle
 		V	c;
 		double	r = 1.0;
 		bool	visible = true;	// an invisible sphere is never hit, yet stays a light (:227-233, 271-277)
+		// Sphere::setIESProfile (:188-202): the profile (index into the scene's), its vertical
+		// axis and horizontal rotation (degrees)
+		int	ies = -1;
+		V	iesDirection{0.0, -1.0, 0.0};
+		double	iesRotation = 0.0;
 
 		// Sphere::hit / hitAny (:226-301): unit direction, a = 1
 		bool	intersect(const Ray& ray, double tMin, double tMax, double& t) const
@@ -645,6 +650,7 @@ namespace oracle
 		std::vector<Tri>	topTris;	// indexed by Top.index for K_TRI
 		std::vector<Rect>	rects;
 		std::vector<Sphere>	spheres;
+		std::vector<luzgpu::IesProfile>	iesProfiles;
 		std::vector<Cube>	cubes;
 		std::vector<Plane>	planes;
 		std::vector<Volume>	volumes;
@@ -748,6 +754,7 @@ namespace oracle
 			}
 			s.meshes.push_back(std::move(m));
 		}
+		s.iesProfiles = in.iesProfiles;
 		for (const auto& h : in.hittables)
 		{
 			Top t{K_TRI, 0, h.material};
@@ -774,7 +781,7 @@ namespace oracle
 				const auto& sp = in.spheres[h.index];
 				t.kind = K_SPHERE;
 				t.index = static_cast<std::uint32_t>(s.spheres.size());
-				s.spheres.push_back(Sphere{fromD(sp.position), sp.radius, sp.visible});
+				s.spheres.push_back(Sphere{fromD(sp.position), sp.radius, sp.visible, sp.ies, fromD(sp.iesDirection), sp.iesRotation});
 			}
 			else if (h.kind == luzgpu::HIT_CUBE)
 			{
@@ -1674,6 +1681,36 @@ namespace oracle
 		std::int64_t	material = -1;
 	};
 
+	// IESProfile::relativeIntensity (IESProfile.cpp:302-323): the candela towards `emission`
+	// over the profile's mean, the vertical angle measured from the (normalised) axis, the
+	// horizontal one in the frame cross(helper, axis), cross(axis, tangent), turned by
+	// `rotation` degrees. The grid lookup is the reader's iesCandelaAt (IESProfile.cpp:245-300).
+	double	iesRelativeIntensity(const luzgpu::IesProfile& pr, V emission, V axisIn, double rotation)
+	{
+		if (pr.meanCandela <= 0.0 || !std::isfinite(pr.meanCandela)) return 0.0;
+		auto normalizedOr = [](V v, V fallback) {
+			const double l2 = len2(v);
+			if (!std::isfinite(l2) || l2 <= 0.0) return fallback;
+			return v / std::sqrt(l2);
+		};
+		const V axis = normalizedOr(axisIn, V{0.0, -1.0, 0.0});
+		const V d = normalizedOr(emission, axis);
+		const double cosT = std::clamp(dot(d, axis), -1.0, 1.0);
+		const double toDegrees = 180.0 / PI;
+		const double vdeg = std::acos(cosT) * toDegrees;
+		const V helper = std::fabs(axis.y) < 0.95 ? V{0.0, 1.0, 0.0} : V{1.0, 0.0, 0.0};
+		const V tangent = normalizedOr(cross(helper, axis), V{1.0, 0.0, 0.0});
+		const V bitangent = cross(axis, tangent);
+		double h = std::atan2(dot(d, bitangent), dot(d, tangent)) * toDegrees + rotation;
+		if (!std::isfinite(h)) h = 0.0;
+		else
+		{
+			h = std::fmod(h, 360.0);
+			if (h < 0.0) h += 360.0;
+		}
+		return luzgpu::iesCandelaAt(pr, vdeg, h) / pr.meanCandela;
+	}
+
 	LS	sampleLight(const Scene& s, V origin, const Sampler& ps)	// ColorHelper.cpp:914-971
 	{
 		LS out;
@@ -1739,6 +1776,11 @@ namespace oracle
 			out.tMax = root;
 			if (out.pdf <= 0.0 || out.tMax <= T_MIN || !std::isfinite(out.pdf) || !std::isfinite(out.tMax)) return out;
 			out.emitted = L.emitted;
+			if (sp.ies >= 0)	// :451-461: the profile scales the sample's radiance
+			{
+				out.emitted = L.emitted * iesRelativeIntensity(s.iesProfiles[static_cast<std::size_t>(sp.ies)], dir * -1.0,
+					sp.iesDirection, sp.iesRotation);
+			}
 			out.valid = maxChannel(out.emitted) > 0.0;
 			return out;
 		}

@@ -112,6 +112,12 @@ def invisible(workload_dir):
 
 
 @pytest.fixture(scope="session")
+def ies(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_ies(os.path.join(workload_dir, "ies"))
+
+
+@pytest.fixture(scope="session")
 def terrain(workload_dir):
     """The volume workload's scene: 1,002,528-triangle terrain, haze, sun (rendered at
     reduced sizes by the tests)."""

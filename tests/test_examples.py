@@ -228,7 +228,7 @@ def test_light_block_errors(tmp_path):
         "unknown key": PANEL.format(unit="radiance=5\nshape=disc"),
         "sphere light without a quantity": "sphere_light s {\nposition=(0,2,0)\n}",
         "sphere light radius": "point_light s {\nradius=0\nradiance=1\n}",
-        "sphere light ies": "point_light s {\nradiance=1\nies=lamp.ies\n}",
+        "sphere light ies file missing": "point_light s {\nradiance=1\nies=lamp.ies\n}",
         "not closed": "area_light a {\nradiance=1\n",
     }
     for what, light in bad.items():

@@ -9,7 +9,8 @@ thin-lens camera, rough dielectrics, conductor-Fresnel metals, dispersive glass 
 interior absorption), ``subsurface`` (Burley exits relocated by a probe ray inside the
 shading kernels, the skin profile, the thin two-sided lobe) and ``invisible`` (an invisible
 sphere light that lights the room through light sampling alone, an invisible ball before the
-camera). Same bars as test_gpu_parity: primary-hit IDs against the reference's
+camera) and ``ies`` (IES-profiled sphere lights: a downlight emitting its profile's flux,
+a tilted and turned wash light). Same bars as test_gpu_parity: primary-hit IDs against the reference's
 FP64 brute force, images within relative MSE 1e-4 of the reference render at equal spp and
 seed (tests/golden/make_golden.py)."""
 
@@ -23,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics",
-                                  "subsurface", "invisible"])
+                                  "subsurface", "invisible", "ies"])
 def test_primary_hits(ctx, request, name):
     from luz_amd import SceneFile
     wl = request.getfixturevalue(name)
@@ -38,7 +39,7 @@ def test_primary_hits(ctx, request, name):
 
 @pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
 @pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics",
-                                  "subsurface", "invisible"])
+                                  "subsurface", "invisible", "ies"])
 def test_image_parity(ctx, request, name, integrator):
     from luz_amd import Renderer, SceneFile
     wl = request.getfixturevalue(name)

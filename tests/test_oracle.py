@@ -229,6 +229,49 @@ def test_invisible_spheres_match_reference(invisible):
     assert not np.isin(g["top"], [0, 2]).any()
 
 
+def test_ies_lights_match_reference(ies):
+    """IES-profiled sphere lights (IESProfile.cpp, Sphere.cpp:188-202, 451-461): a downlight
+    emitting its profile's own flux over horizontal angles 0..360, and a tilted, turned wash
+    light with an explicit flux over 0..240 (interpolated back to 0 past its last angle).
+    The restatement reproduces the reference's FP64 render; the profiles shape it (the same
+    lights without them differ by rel-MSE ~0.2)."""
+    ref = np.load(os.path.join(GOLDEN, f"ies_{ies.width}x{ies.height}_{ies.spp}spp_seed1.npz"))["image"]
+    img, _ = oracle_lib.render(ies.scene_path, 1, ies.spp, ies.width, ies.height)
+    e = rel_mse(img, ref)
+    print(f"oracle vs reference (ies): rel-MSE {e:.3e}")
+    assert e < 1e-12
+    g = np.load(os.path.join(GOLDEN, "ies_primary.npz"))
+    ids, _ = oracle_lib.primary(ies.scene_path, 1, ies.width, ies.height)
+    assert np.array_equal(ids[..., 0], g["top"]) and np.array_equal(ids[..., 1], g["tri"])
+
+
+def test_uniform_ies_profile_is_a_candela_light(tmp_path, ies):
+    """A profile of constant candela c has relative intensity 4 pi c / flux in every
+    direction, and the light emits that flux (sphericalLightEmission, Objects.cpp:632-643):
+    its light samples then carry 4 c / (683 A), the radiance of a `candela=c` sphere light,
+    while a ray that hits it sees flux / (683 pi A). The flux is the midpoint rule over
+    180 x 360 cells, 4 pi c (pi / 360) / sin(pi / 360), so the two images agree to that
+    factor (1 + 1.3e-5) on the directly seen light only."""
+    from luz_amd import workloads
+    vertical = list(range(0, 181, 10))
+    (tmp_path / "flat.ies").write_text(workloads.ies_text(vertical, (0, 90, 180, 270), [[250.0] * len(vertical)] * 4, "flat"))
+    base = open(ies.scene_path).read()
+    wash = "sphere_light wash {\nposition=(1.6,2.2,-0.4)\nradius=0.12\ncolor=(0.8,0.9,1)\nlumens=1500\nies=wash.ies\nies_direction=(0.3,-1,0.2)\nies_rotation=30\n}\n"
+    assert wash in base
+    flat = base.replace("ies=downlight.ies", "ies=flat.ies").replace(wash, "")
+    cand = base.replace("ies=downlight.ies", "candela=250").replace(wash, "")
+    shaped = base.replace(wash, "")
+    for name in ("downlight.ies",):
+        (tmp_path / name).write_bytes(open(os.path.join(os.path.dirname(ies.scene_path), name), "rb").read())
+    out = {}
+    for name, text in (("flat", flat), ("cand", cand), ("shaped", shaped)):
+        p = tmp_path / f"{name}.luz"
+        p.write_text(text)
+        out[name] = oracle_lib.render(str(p), 1, 8, 64, 48)[0]
+    assert rel_mse(out["flat"], out["cand"]) < 1e-8
+    assert rel_mse(out["shaped"], out["flat"]) > 1e-2
+
+
 def test_invisible_sphere_identities(tmp_path, invisible):
     """A non-emissive invisible sphere is the same as no sphere at all: it is never hit,
     casts no shadow and is not a light, so removing it leaves the image bit for bit. An

@@ -97,6 +97,40 @@ def test_invisible_spheres_stay_in_the_tlas_and_the_light_list(invisible):
     assert s.info["hittables"] == 8 and s.info["lights"] == 2
 
 
+def test_ies_lights_load_and_bad_profiles_are_refused(tmp_path, ies):
+    """Both IES lights are lights; profiles the reference refuses are refused with its
+    messages (IESProfile.cpp:133-232), as are a zero axis and an unreadable file."""
+    from luz_amd import workloads
+    s = SceneFile.read(None, ies.scene_path)
+    assert s.info["lights"] == 2 and s.info["hittables"] == 6
+    base = open(ies.scene_path).read()
+    good = workloads.ies_profiles()[0][1]
+    bad = {
+        "tilt": (good.replace("TILT=NONE", "TILT=INCLUDE"), "unsupported tilt"),
+        "notilt": (good.replace("TILT=NONE\n", ""), "missing TILT=NONE"),
+        "short": ("TILT=NONE\n1 -1 1 19\n", "incomplete lamp metadata"),
+        "count": (good.replace("1 -1 1 19 5", "1 -1 1 0 5"), "vertical angle count"),
+        "order": (good.replace("0 10 20 30", "0 20 10 30"), "strictly increasing"),
+        "negative": (good.replace("\n" + good.split("\n")[8].split(" ")[0] + " ", "\n-5 ", 1), "non-negative"),
+    }
+    for name, (text, message) in bad.items():
+        (tmp_path / "downlight.ies").write_text(text)
+        (tmp_path / "wash.ies").write_text(workloads.ies_profiles()[1][1])
+        p = tmp_path / f"{name}.luz"
+        p.write_text(base)
+        with pytest.raises(Exception, match=message):
+            SceneFile.read(None, str(p))
+    (tmp_path / "downlight.ies").write_text(good)
+    p = tmp_path / "axis.luz"
+    p.write_text(base.replace("ies_direction=(0.3,-1,0.2)", "ies_direction=(0,0,0)"))
+    with pytest.raises(Exception, match="direction must be non-zero"):
+        SceneFile.read(None, str(p))
+    p = tmp_path / "missing.luz"
+    p.write_text(base.replace("ies=wash.ies", "ies=nowhere.ies"))
+    with pytest.raises(Exception, match="could not be opened"):
+        SceneFile.read(None, str(p))
+
+
 def test_volume_and_phase_validation(tmp_path):
     """HenyeyGreenstein.cpp:10-18 and Objects.cpp:491-497, 746-800 reject these inputs."""
     import pytest
