@@ -277,7 +277,9 @@ typedef struct luzgpu_mesh_desc
  *   INSTANCE     mesh; a = position, c = scale; cos_rotation / sin_rotation of the X, Y, Z
  *                Euler angles (MeshInstance::InstanceTransform)
  *   SPHERE       a = centre, radius; flags & 1: invisible (Sphere::isVisible false: never hit,
- *                still a light when emissive)
+ *                still a light when emissive); flags & 2: IES-profiled light (Sphere::
+ *                setIESProfile): mesh = index into ies_profiles, b = vertical axis,
+ *                size[0] = horizontal rotation (degrees)
  *   CUBE         a = position, size = width, height, depth
  *   VOLUME       flags & 1: sphere boundary (a = centre, radius), else box (a = position,
  *                size); density (per scene unit); material = the phase function
@@ -298,6 +300,18 @@ typedef struct luzgpu_hittable_desc
 	double		cos_rotation[3];
 	double		sin_rotation[3];
 } luzgpu_hittable_desc;
+
+/* IESProfile (IESProfile.hpp): the angle grids in degrees (strictly increasing), the candela
+ * values horizontal-major (vertical_count values per horizontal angle, the file's multiplier
+ * applied), and the luminous flux IESProfile::totalLumens() integrated from them. */
+typedef struct luzgpu_ies_desc
+{
+	uint32_t		vertical_count, horizontal_count;
+	const double*	vertical;
+	const double*	horizontal;
+	const double*	candela;
+	double			total_lumens;
+} luzgpu_ies_desc;
 
 /* Scene::getActiveCamera() (Camera.hpp), lengths in metres. */
 typedef struct luzgpu_camera_desc
@@ -340,6 +354,9 @@ typedef struct luzgpu_scene_desc
 	const luzgpu_material_desc*	materials;
 	const luzgpu_mesh_desc*		meshes;
 	const luzgpu_hittable_desc*	hittables;
+	uint32_t	ies_count;
+	uint32_t	pad3;
+	const luzgpu_ies_desc*		ies_profiles;   /* profiles of IES-profiled sphere lights */
 } luzgpu_scene_desc;
 
 /* The same as luzgpu_scene_load for a Scene given as records: the scene's pre-render sync

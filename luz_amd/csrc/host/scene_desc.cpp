@@ -198,6 +198,39 @@ namespace luzgpu
 			throw std::invalid_argument("NULL material, mesh or hittable array with a non-zero count");
 		}
 		for (std::uint32_t i = 0; i < d.material_count; i++) sc.materials.push_back(materialOf(d.materials[i], i));
+		if (d.ies_count && !d.ies_profiles) throw std::invalid_argument("NULL IES profile array with a non-zero count");
+		for (std::uint32_t i = 0; i < d.ies_count; i++)
+		{
+			// IESProfile::load's checks (IESProfile.cpp:180-229) on the profile as handed over
+			const luzgpu_ies_desc& id = d.ies_profiles[i];
+			const std::string who = "IES profile " + std::to_string(i);
+			if (id.vertical_count == 0 || id.horizontal_count == 0 || !id.vertical || !id.horizontal || !id.candela)
+			{
+				throw std::invalid_argument(who + ": empty angle or candela arrays");
+			}
+			IesProfile pr;
+			pr.vertical.assign(id.vertical, id.vertical + id.vertical_count);
+			pr.horizontal.assign(id.horizontal, id.horizontal + id.horizontal_count);
+			pr.candela.assign(id.candela, id.candela + static_cast<std::size_t>(id.vertical_count) * id.horizontal_count);
+			for (const std::vector<double>* a : {&pr.vertical, &pr.horizontal})
+			{
+				for (std::size_t k = 0; k < a->size(); k++)
+				{
+					if (!std::isfinite((*a)[k]) || (k > 0 && (*a)[k] <= (*a)[k - 1]))
+					{
+						throw std::invalid_argument(who + ": angles must be finite and strictly increasing");
+					}
+				}
+			}
+			for (const double c : pr.candela)
+			{
+				if (!std::isfinite(c) || c < 0.0) throw std::invalid_argument(who + ": candela values must be finite and non-negative");
+			}
+			if (!std::isfinite(id.total_lumens) || id.total_lumens <= 0.0) throw std::invalid_argument(who + ": zero luminous flux");
+			pr.totalLumens = id.total_lumens;
+			pr.meanCandela = id.total_lumens / (4.0 * 3.14159265358979323846);
+			sc.iesProfiles.push_back(std::move(pr));
+		}
 		for (std::uint32_t i = 0; i < d.mesh_count; i++)
 		{
 			const luzgpu_mesh_desc& md = d.meshes[i];
@@ -293,6 +326,16 @@ namespace luzgpu
 					sp.position = d3(h.a);
 					sp.radius = h.radius;
 					sp.visible = (h.flags & 1u) == 0;
+					if (h.flags & 2u)
+					{
+						if (h.mesh >= d.ies_count) throw std::invalid_argument(who + " IES profile index out of range");
+						const double dl2 = h.b[0] * h.b[0] + h.b[1] * h.b[1] + h.b[2] * h.b[2];
+						if (!std::isfinite(dl2) || dl2 <= 0.0) throw std::invalid_argument(who + " IES profile direction must be non-zero");
+						if (!std::isfinite(h.size[0])) throw std::invalid_argument(who + " IES profile rotation must be finite");
+						sp.ies = static_cast<std::int32_t>(h.mesh);
+						sp.iesDirection = d3(h.b);
+						sp.iesRotation = h.size[0];
+					}
 					ht.kind = HIT_SPHERE;
 					ht.index = static_cast<std::uint32_t>(sc.spheres.size());
 					sc.spheres.push_back(sp);

@@ -6,6 +6,7 @@
 #include "Camera.hpp"
 #include "ColorManagement.hpp"
 #include "EnvironmentMap.hpp"
+#include "IESProfile.hpp"
 #include "Image.hpp"
 #include "Random.hpp"
 #include "Scene/Scene.hpp"
@@ -54,7 +55,12 @@ namespace luzgpu_glue
 		static double	height(const Rectangle& r) { return r._height; }
 		static bool	explicitAxes(const Rectangle& r) { return r._hasAxes; }
 		static const Transform&	transform(const Cube& c) { return c._transform; }
-		static bool	iesProfiled(const Sphere& s) { return static_cast<bool>(s._iesProfile); }
+		static const IESProfile*	ies(const Sphere& s) { return s._iesProfile.get(); }
+		static const Vector3&	iesDirection(const Sphere& s) { return s._iesDirection; }
+		static double	iesRotation(const Sphere& s) { return s._iesRotationDegrees; }
+		static const std::vector<double>&	verticalAngles(const IESProfile& p) { return p._verticalAngles; }
+		static const std::vector<double>&	horizontalAngles(const IESProfile& p) { return p._horizontalAngles; }
+		static const std::vector<double>&	candela(const IESProfile& p) { return p._candelaValues; }
 		static void	size(const Cube& c, double out[3]) { out[0] = c._width; out[1] = c._height; out[2] = c._depth; }
 		static double	y(const Plane& p) { return p._y; }
 		static const Vector3&	orientation(const Plane& p) { return p._orientation; }
@@ -175,6 +181,25 @@ namespace luzgpu_glue
 			Resolver	resolve;
 			std::map<const Material*, std::uint32_t>	materialIndex;
 			std::map<const Mesh*, std::uint32_t>	meshIndex;
+			std::map<const IESProfile*, std::uint32_t>	iesIndex;
+
+			// each IESProfile once (lights may share one)
+			std::uint32_t	ies(const IESProfile* p)
+			{
+				const auto it = iesIndex.find(p);
+				if (it != iesIndex.end()) return it->second;
+				const std::uint32_t i = static_cast<std::uint32_t>(out.ies.size());
+				luzgpu_ies_desc d{};
+				d.vertical_count = static_cast<std::uint32_t>(Access::verticalAngles(*p).size());
+				d.horizontal_count = static_cast<std::uint32_t>(Access::horizontalAngles(*p).size());
+				d.vertical = Access::verticalAngles(*p).data();
+				d.horizontal = Access::horizontalAngles(*p).data();
+				d.candela = Access::candela(*p).data();
+				d.total_lumens = p->totalLumens();
+				out.ies.push_back(d);
+				iesIndex.emplace(p, i);
+				return i;
+			}
 
 			std::uint32_t	material(Material* m)
 			{
@@ -245,14 +270,17 @@ namespace luzgpu_glue
 				}
 				else if (const auto* sp = dynamic_cast<const Sphere*>(h))
 				{
-					if (Access::iesProfiled(*sp))
-					{
-						throw std::runtime_error("Unsupported for the device path: IES light profile");
-					}
 					d.kind = LUZGPU_HIT_SPHERE;
 					put(sp->getPosition(), d.a);
 					d.radius = sp->getRadius();
 					d.flags = sp->isVisible() ? 0u : 1u;
+					if (const IESProfile* p = Access::ies(*sp))
+					{
+						d.flags |= 2u;
+						d.mesh = ies(p);
+						put(Access::iesDirection(*sp), d.b);
+						d.size[0] = Access::iesRotation(*sp);
+					}
 				}
 				else if (const auto* cu = dynamic_cast<const Cube*>(h))
 				{
@@ -336,7 +364,8 @@ namespace luzgpu_glue
 		{
 			throw std::runtime_error("Unsupported for the device path: caustics (the photon map)");
 		}
-		Flattener f{out, resolve, {}, {}};
+		Flattener f{out, resolve, {}, {}, {}};
+		out.ies.clear();
 		out.hittables.clear();
 		for (const std::shared_ptr<Hittable>& h : scene.getHittables()) out.hittables.push_back(f.hittable(h));
 		luzgpu_scene_desc& d = out.desc;
@@ -412,6 +441,8 @@ namespace luzgpu_glue
 		d.materials = out.materials.data();
 		d.meshes = out.meshes.data();
 		d.hittables = out.hittables.data();
+		d.ies_count = static_cast<std::uint32_t>(out.ies.size());
+		d.ies_profiles = out.ies.empty() ? nullptr : out.ies.data();
 	}
 
 	void	manageThreadsOnDevice(Scene& scene, int device, Resolver resolve)

@@ -33,6 +33,7 @@ namespace luzgpu_glue
 		std::vector<std::vector<double>>	meshNormals;
 		std::vector<luzgpu_hittable_desc>	hittables;
 		std::vector<double>	environment;
+		std::vector<luzgpu_ies_desc>	ies;	// point into the Scene's IESProfile objects
 
 		FlatScene() = default;
 		FlatScene(const FlatScene&) = delete;
@@ -49,7 +50,7 @@ namespace luzgpu_glue
 	// Flattens `scene` into luzgpu_scene_desc records (Scene::getHittables() order, each
 	// distinct Material and Mesh once). Throws std::runtime_error naming what the device path
 	// does not take (textures, photon-map caustics, legacy meshes, a directional
-	// light as the atmosphere's sun, IES-profiled spheres).
+	// light as the atmosphere's sun).
 	void	flatten(const Scene& scene, FlatScene& out, Resolver resolve = nullptr);
 
 	// Renderer::internal::_manageThreads (Threads.cpp:375-548) on the GPU: the render seed,
