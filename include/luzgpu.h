@@ -284,7 +284,10 @@ typedef struct luzgpu_mesh_desc
  *   VOLUME       flags & 1: sphere boundary (a = centre, radius), else box (a = position,
  *                size); density (per scene unit); material = the phase function
  *   PLANE        plane_y, b = orientation
- *   DIRECTIONAL  a = the light's direction of travel (normalised) */
+ *   DIRECTIONAL  a = the light's direction of travel (normalised); flags & 1: b = its
+ *                atmosphere sun radiance (DirectionalLight::getAtmosphereSunRadiance, a
+ *                `solar=` light); with the atmosphere sky the first directional light is the
+ *                atmosphere's sun (Scene::syncAtmosphereSunDirection) */
 typedef struct luzgpu_hittable_desc
 {
 	uint32_t	kind;                /* LUZGPU_HIT_*                                             */

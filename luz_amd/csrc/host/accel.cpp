@@ -915,14 +915,26 @@ namespace luzgpu
 			return c;
 		}
 
-		GpuAtmosphere	packAtmosphere(const Settings& s)
+		GpuAtmosphere	packAtmosphere(const Scene& scene)
 		{
+			const Settings& s = scene.settings;
 			GpuAtmosphere a{};
 			// updateSunDirectionVector (Atmosphere.cpp:304-309), normalised (:32-44)
 			const double angle = M_PI * s.sunAngle;
-			const double d[3] = {0.0, std::cos(angle), -std::sin(angle)};
+			double d[3] = {0.0, std::cos(angle), -std::sin(angle)};
+			dvec3 sun = solarSunRadiance();
+			// Scene::syncAtmosphereSunDirection (Scene.cpp:457-475): the first directional light
+			// in hittable order is the sun, towards minus its direction, with its solar radiance
+			// or else its emission
+			for (const Hittable& h : scene.hittables)
+			{
+				if (h.kind != HIT_DIRECTIONAL) continue;
+				const DirectionalLight& dl = scene.directionals[h.index];
+				for (int k = 0; k < 3; k++) d[k] = dl.direction[k] * -1.0;
+				sun = dl.hasSunRadiance ? dl.sunRadiance : scene.materials[h.material].color;
+				break;
+			}
 			const double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-			const dvec3 sun = solarSunRadiance();
 			for (int k = 0; k < 3; k++)
 			{
 				a.sunDirection[k] = d[k] / len;
@@ -944,10 +956,6 @@ namespace luzgpu
 	{
 		PackedScene p;
 		const Settings& s = scene.settings;
-		if (s.sky == SKY_ATMOSPHERE && !scene.directionals.empty())
-		{
-			throw std::runtime_error("Unsupported for the device path: a directional light as the atmosphere's sun");
-		}
 		p.adaptive.enabled = s.adaptive;
 		p.adaptive.minSamples = s.adaptiveMinSamples;
 		p.adaptive.checkInterval = s.adaptiveCheckInterval;
@@ -1526,7 +1534,7 @@ namespace luzgpu
 		gp.totalSamples = s.samples;
 		gp.sky = s.sky == SKY_ENVIRONMENT ? GPU_SKY_ENVIRONMENT : s.sky == SKY_LINEAR ? GPU_SKY_LINEAR
 			: s.sky == SKY_ATMOSPHERE ? GPU_SKY_ATMOSPHERE : GPU_SKY_NONE;
-		if (s.sky == SKY_ATMOSPHERE) gp.atmosphere = packAtmosphere(s);
+		if (s.sky == SKY_ATMOSPHERE) gp.atmosphere = packAtmosphere(scene);
 		for (int k = 0; k < 3; k++) gp.background[k] = static_cast<float>(s.background[k]);
 		gp.backgroundLuminance = static_cast<float>(luminance(s.background));
 		gp.lightCount = static_cast<std::uint32_t>(p.lights.size());

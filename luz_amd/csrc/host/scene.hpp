@@ -145,9 +145,15 @@ namespace luzgpu
 		double	density = 0.1;
 	};
 
+	// With the atmosphere sky the scene's first directional light is also the atmosphere's sun
+	// (Scene::syncAtmosphereSunDirection, Scene.cpp:457-475): the sun shines along its
+	// direction with its solar radiance (a `solar=` light, DirectionalLight::
+	// setAtmosphereSunRadiance) or else its emission.
 	struct	DirectionalLight
 	{
 		dvec3	direction = {0.0, -1.0, 0.0};	// normalised (DirectionalLight.cpp:9-17)
+		bool	hasSunRadiance = false;
+		dvec3	sunRadiance{};
 	};
 
 	struct	Rectangle

@@ -381,6 +381,18 @@ namespace luzgpu
 				{
 					DirectionalLight dl;
 					dl.direction = d3(h.a);
+					if (h.flags & 1u)
+					{
+						dl.hasSunRadiance = true;
+						dl.sunRadiance = d3(h.b);
+						for (int k = 0; k < 3; k++)
+						{
+							if (!std::isfinite(dl.sunRadiance[k]) || dl.sunRadiance[k] < 0.0)
+							{
+								throw std::invalid_argument(who + ": atmosphere sun radiance must be finite and non-negative");
+							}
+						}
+					}
 					ht.kind = HIT_DIRECTIONAL;
 					ht.index = static_cast<std::uint32_t>(sc.directionals.size());
 					sc.directionals.push_back(dl);

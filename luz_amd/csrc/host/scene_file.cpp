@@ -2046,7 +2046,9 @@ namespace luzgpu
 				DirectionalLight d;
 				dvec3 direction = {0.0, -1.0, 0.0};
 				dvec3 color = {1.0, 1.0, 1.0};
+				bool hasColor = false;
 				double irradiance = -1.0;
+				double solar = -1.0;
 				while (next(line))
 				{
 					const std::string t = trim(line);
@@ -2059,6 +2061,16 @@ namespace luzgpu
 						if (dot(direction, direction) <= 0.0)
 						{
 							throw std::runtime_error("Directional light '" + name + "' direction must be non-zero.");
+						}
+						if (solar >= 0.0)
+						{
+							// solarDirectionalIrradiance (LightUnits.cpp:113-117): the solar
+							// constant times the scale, in ColorScience::solar()'s colour unless
+							// the light names one (Objects.cpp:1174-1189)
+							if (irradiance >= 0.0) throw std::runtime_error("Light block defines multiple physical light quantities.");
+							if (!hasColor) color = blackbodyColor(5778.0);
+							irradiance = 1361.0 * solar;
+							d.hasSunRadiance = true;
 						}
 						if (irradiance < 0.0)
 						{
@@ -2080,6 +2092,7 @@ namespace luzgpu
 							const double f = irradiance / cl;
 							m.color = {color[0] * f, color[1] * f, color[2] * f};
 						}
+						if (d.hasSunRadiance) d.sunRadiance = m.color;
 						scene.materials.push_back(m);
 						const double len = std::sqrt(dot(direction, direction));
 						d.direction = direction * (1.0 / len);
@@ -2093,7 +2106,11 @@ namespace luzgpu
 						throw std::runtime_error("Invalid directional light property: " + t);
 					}
 					if (key == "direction") direction = parseTriple(value, "directional light direction");
-					else if (key == "color") color = parseColor(value, "directional light color");
+					else if (key == "color")
+					{
+						color = parseColor(value, "directional light color");
+						hasColor = true;
+					}
 					else if (key == "irradiance" || key == "w_m2" || key == "wm2" || key == "illuminance" || key == "lux" || key == "lx")
 					{
 						const double v = parseDouble(value, key);
@@ -2101,7 +2118,15 @@ namespace luzgpu
 						{
 							throw std::invalid_argument("Directional irradiance must be finite and non-negative.");
 						}
+						if (irradiance >= 0.0 || solar >= 0.0) throw std::runtime_error("Light block defines multiple physical light quantities.");
 						irradiance = (key == "irradiance" || key == "w_m2" || key == "wm2") ? v : v / 683.0;
+					}
+					else if (key == "solar" || key == "sun" || key == "solar_scale" || key == "solarscale")
+					{
+						const double v = parseDouble(value, key);
+						if (!(v >= 0.0)) throw std::invalid_argument("Solar scale must be finite and non-negative.");
+						if (irradiance >= 0.0 || solar >= 0.0) throw std::runtime_error("Light block defines multiple physical light quantities.");
+						solar = v;
 					}
 					else throw std::runtime_error("Unsupported directional light property for the device path: " + t);
 				}

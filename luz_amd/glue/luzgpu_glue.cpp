@@ -76,7 +76,6 @@ namespace luzgpu_glue
 			c[0] = x.cosX; c[1] = x.cosY; c[2] = x.cosZ;
 			s[0] = x.sinX; s[1] = x.sinY; s[2] = x.sinZ;
 		}
-		static bool	atmosphereSun(const DirectionalLight& d) { return d._hasAtmosphereSunRadiance; }
 		static bool	textured(const Material& m) { return m._texture != nullptr; }
 		static const std::vector<Color>&	pixels(const EnvironmentMap& e) { return e._pixels; }
 		static double	hR(const Atmosphere& a) { return a._hR; }
@@ -319,12 +318,16 @@ namespace luzgpu_glue
 				}
 				else if (const auto* dl = dynamic_cast<const DirectionalLight*>(h))
 				{
-					if (Access::atmosphereSun(*dl))
-					{
-						throw std::runtime_error("Unsupported for the device path: a directional light as the atmosphere's sun");
-					}
 					d.kind = LUZGPU_HIT_DIRECTIONAL;
 					put(dl->getDirection(), d.a);
+					if (dl->hasAtmosphereSunRadiance())
+					{
+						d.flags = 1u;
+						const Color c = dl->getAtmosphereSunRadiance();
+						d.b[0] = c.getRed();
+						d.b[1] = c.getGreen();
+						d.b[2] = c.getBlue();
+					}
 				}
 				else
 				{
@@ -397,16 +400,6 @@ namespace luzgpu_glue
 		d.atmosphere_light_samples = Access::lightSamples(a);
 		d.stars_brightness = a.getStarsBrightness();
 		d.sun_radiance_scale = a.getSunRadianceScale();
-		if (d.sky == LUZGPU_SKY_ATMOSPHERE)
-		{
-			for (const luzgpu_hittable_desc& h : out.hittables)
-			{
-				if (h.kind == LUZGPU_HIT_DIRECTIONAL)
-				{
-					throw std::runtime_error("Unsupported for the device path: the atmosphere sky with a directional light as its sun");
-				}
-			}
-		}
 		if (s.hasEnvironmentMap())
 		{
 			const EnvironmentMap& e = *s.getEnvironmentMap();

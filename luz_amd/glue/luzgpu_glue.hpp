@@ -49,8 +49,7 @@ namespace luzgpu_glue
 
 	// Flattens `scene` into luzgpu_scene_desc records (Scene::getHittables() order, each
 	// distinct Material and Mesh once). Throws std::runtime_error naming what the device path
-	// does not take (textures, photon-map caustics, legacy meshes, a directional
-	// light as the atmosphere's sun).
+	// does not take (textures, photon-map caustics, legacy meshes).
 	void	flatten(const Scene& scene, FlatScene& out, Resolver resolve = nullptr);
 
 	// Renderer::internal::_manageThreads (Threads.cpp:375-548) on the GPU: the render seed,

@@ -380,6 +380,22 @@ def atmosphere_scene_text(width=96, height=64, spp=8, depth=6) -> str:
     return "\n".join(lines) + "\n"
 
 
+def sunlit_scene_text(width=96, height=64, spp=8, depth=6) -> str:
+    """The atmosphere scene lit by a `solar=` directional light, which also becomes the
+    atmosphere's sun: the sky's scattering follows the light's direction (low, from the
+    side, not the atmosphere= line's 45 degrees) and its solar radiance."""
+    text = atmosphere_scene_text(width, height, spp, depth)
+    light = "directional_light sun {\ndirection=(-0.5,-0.35,0.6)\nsolar=0.02\n}\n"
+    return text.replace("objects{\n", light + "objects{\n", 1)
+
+
+def write_sunlit(directory: str, width=96, height=64, spp=8, depth=6, seed=1) -> Workload:
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "sunlit.luz")
+    _write_if_changed(path, sunlit_scene_text(width, height, spp, depth).encode())
+    return Workload("sunlit", directory, path, width, height, spp, depth, seed, 0)
+
+
 def write_atmosphere(directory: str, width=96, height=64, spp=8, depth=6, seed=1) -> Workload:
     os.makedirs(directory, exist_ok=True)
     path = os.path.join(directory, "atmosphere.luz")

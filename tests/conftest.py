@@ -118,6 +118,12 @@ def ies(workload_dir):
 
 
 @pytest.fixture(scope="session")
+def sunlit(workload_dir):
+    from luz_amd import workloads
+    return workloads.write_sunlit(os.path.join(workload_dir, "sunlit"))
+
+
+@pytest.fixture(scope="session")
 def terrain(workload_dir):
     """The volume workload's scene: 1,002,528-triangle terrain, haze, sun (rendered at
     reduced sizes by the tests)."""

@@ -482,6 +482,8 @@ def main():
             small_scene_fixtures("optics", workloads.write_optics(os.path.join(tmp, "optics")), tmp)
         if "subsurface" in only:
             small_scene_fixtures("subsurface", workloads.write_subsurface(os.path.join(tmp, "subsurface")), tmp)
+        if "sunlit" in only:
+            small_scene_fixtures("sunlit", workloads.write_sunlit(os.path.join(tmp, "sunlit")), tmp)
         if "ies" in only:
             small_scene_fixtures("ies", workloads.write_ies(os.path.join(tmp, "ies")), tmp)
         if "invisible" in only:
@@ -523,6 +525,7 @@ def main():
     small_scene_fixtures("subsurface", workloads.write_subsurface(os.path.join(tmp, "subsurface")), tmp)
     small_scene_fixtures("invisible", workloads.write_invisible(os.path.join(tmp, "invisible")), tmp)
     small_scene_fixtures("ies", workloads.write_ies(os.path.join(tmp, "ies")), tmp)
+    small_scene_fixtures("sunlit", workloads.write_sunlit(os.path.join(tmp, "sunlit")), tmp)
     fd_fixture(tmp)
     fd_rough_fixture(tmp)
     terrain_fixtures(tmp)

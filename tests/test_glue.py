@@ -41,7 +41,7 @@ def _run(*args):
     return lines
 
 
-@pytest.mark.parametrize("name", ["cornell", "primitives", "volume", "materials", "meshlight", "optics", "invisible", "ies", "spheres"])
+@pytest.mark.parametrize("name", ["cornell", "primitives", "volume", "materials", "meshlight", "optics", "invisible", "ies", "sunlit", "spheres"])
 def test_flattened_scene_matches_file_route_and_reference(request, name):
     wl = request.getfixturevalue(name)
     r = _run("topology", wl.scene_path)

@@ -32,14 +32,14 @@ def test_scene_built_in_code_renders_on_device_like_the_reference():
     assert rel <= 1e-4
 
 
-@pytest.mark.parametrize("which", ["subsurface", "invisible", "ies", "cornell", "suzanne", "volumetric_godrays"])
+@pytest.mark.parametrize("which", ["subsurface", "invisible", "ies", "sunlit", "cornell", "suzanne", "volumetric_godrays"])
 def test_loaded_scene_renders_on_device_like_the_reference(request, which):
     """Principled subsurface (Burley and thin), invisible spheres, IES-profiled sphere lights
     (the profiles handed over as luzgpu_ies_desc records), and the reference's example
     scenes (spectral reflectances, area and point lights, sphere blocks, meshes, an invisible
     sphere light) read by the reference's loader, flattened and rendered on the device."""
     from test_examples import EXAMPLES
-    path = request.getfixturevalue(which).scene_path if which in ("subsurface", "invisible", "ies") else \
+    path = request.getfixturevalue(which).scene_path if which in ("subsurface", "invisible", "ies", "sunlit") else \
         next(e[1] for e in EXAMPLES if e[0] == which)
     print("glue_driver render-file 0", path, 1, flush=True)
     out = subprocess.run([DRIVER, "render-file", "0", path, "1"], stdout=subprocess.PIPE, stderr=subprocess.PIPE,

@@ -10,7 +10,8 @@ interior absorption), ``subsurface`` (Burley exits relocated by a probe ray insi
 shading kernels, the skin profile, the thin two-sided lobe) and ``invisible`` (an invisible
 sphere light that lights the room through light sampling alone, an invisible ball before the
 camera) and ``ies`` (IES-profiled sphere lights: a downlight emitting its profile's flux,
-a tilted and turned wash light). Same bars as test_gpu_parity: primary-hit IDs against the reference's
+a tilted and turned wash light) and ``sunlit`` (the atmosphere scene lit by a `solar=`
+directional light that is also the atmosphere's sun). Same bars as test_gpu_parity: primary-hit IDs against the reference's
 FP64 brute force, images within relative MSE 1e-4 of the reference render at equal spp and
 seed (tests/golden/make_golden.py)."""
 
@@ -24,7 +25,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics",
-                                  "subsurface", "invisible", "ies"])
+                                  "subsurface", "invisible", "ies", "sunlit"])
 def test_primary_hits(ctx, request, name):
     from luz_amd import SceneFile
     wl = request.getfixturevalue(name)
@@ -33,13 +34,13 @@ def test_primary_hits(ctx, request, name):
     # 4.8e-7 of rounding; over the ground plane's grazing rays (|d.y| down to ~0.005) that is
     # up to 1e-4 of t, so its distances are held to 2e-3 (its hit ids stay exact)
     # (the optics camera sees the floor plane out to the horizon: the same grazing rays)
-    t_rtol = 2e-3 if name in ("atmosphere", "optics") else 2e-5
+    t_rtol = 2e-3 if name in ("atmosphere", "optics", "sunlit") else 2e-5
     assert _primary_check(scene, f"{name}_primary.npz", t_rtol=t_rtol) <= 2
 
 
 @pytest.mark.parametrize("integrator", ["megakernel", "wavefront"])
 @pytest.mark.parametrize("name", ["primitives", "volume", "materials", "meshlight", "skylinear", "atmosphere", "spectral", "atmosphere_env", "optics",
-                                  "subsurface", "invisible", "ies"])
+                                  "subsurface", "invisible", "ies", "sunlit"])
 def test_image_parity(ctx, request, name, integrator):
     from luz_amd import Renderer, SceneFile
     wl = request.getfixturevalue(name)

@@ -51,9 +51,9 @@ def test_unsupported_feature_is_reported(tmp_path):
     import pytest
     from luz_amd import _lib
     p = tmp_path / "bad.luz"
-    # the atmosphere sky with a directional light as its sun (Scene.cpp:455-473) stays refused
-    p.write_text("[settings]\nsky=atmosphere\n\n[scene]\ncamera c {\nposition=(0,0,0)\n}\n"
-                 "directional_light sun {\ndirection=(0,-1,0)\ncolor=(1,1,1)\nirradiance=2\n}\n")
+    # material textures stay refused
+    p.write_text("[settings]\nsky=none\n\n[materials]\nmaterial m {\ntype=lambertian\ntexture=a.png\n}\n\n"
+                 "[scene]\ncamera c {\nposition=(0,0,0)\n}\n")
     with pytest.raises(_lib.LuzGpuError) as e:
         SceneFile.read(None, str(p))
     assert e.value.code == _lib.E_UNSUPPORTED
