@@ -143,10 +143,11 @@ namespace luzgpu
 	// lane has a node left; then every live lane takes one step. Each lane's entries are
 	// processed in the same order as without the wait, so results do not depend on it; it
 	// only lets the wave run the node code and the leaf code less often side by side. The
-	// triangle-only kernels wait for 16 lanes (the sphere frame is slower at 12, 24 and 32),
-	// the others for 32 (the hazy terrain: 103.7 ms at 32, 104.5 at 24, 107.6 at 16).
-	template <int Feat>
-	constexpr int	kLeafBatch = Feat == 0 ? 16 : 32;
+	// triangle-only kernels wait for 16 lanes (the sphere frame is slower at 8, 12, 24 and 32);
+	// the others for 32 on closest-hit rays and 48 on shadow rays (the hazy terrain: 102.8 ms,
+	// against 103.7 at 32 / 32, 104.5 at 24 / 24 and 107.6 at 16 / 16).
+	template <int Feat, bool AnyHit>
+	constexpr int	kLeafBatch = Feat == 0 ? 16 : (AnyHit ? 48 : 32);
 
 	__device__ __forceinline__ std::uint32_t	laneId() { return threadIdx.x & 63u; }
 
@@ -374,7 +375,7 @@ namespace luzgpu
 						break;
 					}
 					const unsigned long long nodes = __ballot(wantNode);
-					const bool leafTurn = nodes == 0ull || __popcll(live & ~nodes) >= kLeafBatch<Feat>;
+					const bool leafTurn = nodes == 0ull || __popcll(live & ~nodes) >= kLeafBatch<Feat, AnyHit>;
 					if (leafTurn ? !done : wantNode)
 					{
 						done = traceStep<AnyHit, Count, Feat>(s, st, kTMinF, stack, &visits);
@@ -423,7 +424,7 @@ namespace luzgpu
 			}
 			const bool wantNode = active && (st.entry & kLeafBit) == 0u;
 			const unsigned long long nodes = __ballot(wantNode);
-			const bool leafTurn = nodes == 0ull || __popcll(__ballot(active) & ~nodes) >= kLeafBatch<Feat>;
+			const bool leafTurn = nodes == 0ull || __popcll(__ballot(active) & ~nodes) >= kLeafBatch<Feat, AnyHit>;
 			if ((leafTurn ? active : wantNode) && traceStep<AnyHit, Count, Feat>(s, st, kTMinF, stack, &visits))
 			{
 				src.store(slot, st);
